@@ -1,0 +1,582 @@
+/*
+ * sfo.c -- CPU FP64 oracle for the Roberts (1983) periodic boundary-integral RK4 stepper.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker for the HIP implementation: only the
+ * test-suite, the smoke check and the CPU-baseline leg of the benchmark load it.  The product
+ * library never links or calls it.
+ *
+ * It restates, function by function, the FP64 algorithm of the reference CUDA solver
+ * (paths relative to CuSuperHelium/CuSuperHelium/):
+ *   sfo_cot               utilities.cuh:312-315,347-350      cot(w) = 1/tan(w)
+ *   sfo_fft               cufftExecZ2Z (Derivatives.cuh:197,243), unnormalised both ways
+ *   sfo_fft_derivative    Derivatives.cuh:189-257 with the multiplier tables
+ *                         utilities.cuh:106-148 (first) and 178-200 (second)
+ *   sfo_zphi_derivative   Derivatives.cuh:274-384
+ *   sfo_assemble          createM.cuh:43-63 (water) and 65-92 (finite depth / helium)
+ *   sfo_lu_factor/solve   cusolverDnDgetrf/getrs as called in MatrixSolver.cuh:113-126
+ *   sfo_velocity          WaterVelocities.cuh:38-115,205-242
+ *   sfo_rhs_phi           createM.cuh:96-117,171-211
+ *   sfo_rhs               BaseBoundaryIntegrator.cuh:137-306 (energies omitted: they do not
+ *                         feed the right-hand side)
+ *   sfo_rk4               AutonomousRungeKuttaStepper.cuh:31-36,123-361,417-437
+ *   sfo_adimensionalize   Export.cu:1222-1246
+ *
+ * Layouts follow the reference: a batch of B surfaces of N nodes is the complex128 state
+ * [Z_0..Z_{BN-1}, Phi_0..Phi_{BN-1}] (BaseBoundaryIntegrator.cuh:141-146), stored here as
+ * interleaved doubles.  Matrices are row-major per surface: M[b*N*N + k*N + j] is row k
+ * (collocation node) and column j (source node), i.e. element (k, j) of the system the
+ * reference hands to getrf (createM.cuh:52 writes it at k + j*n in column-major order).
+ *
+ * Independence from the HIP path: cot uses glibc's ctan, the FFT is a plain iterative radix-2
+ * (or recursive mixed-radix) transform, and the solve is classic LAPACK-style blocked GEPP
+ * with explicit row interchanges.
+ */
+#include <complex.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include <stdint.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+typedef double complex cplx;
+
+static const double PI_D = 3.14159265358979323846264338327950288;
+
+/* Problem description.  kind: 0 = water (WaterBoundaryProblem.cuh:7-40),
+ * 1 = helium film (HeliumBoundaryProblem.cuh:45-86).  g multiplies the restoring term of the
+ * Bernoulli equation; g = 1 is the reference (which has no such parameter). */
+typedef struct sfo_props {
+    int kind;
+    int infinite_depth;
+    int use_expansions;
+    int expansion_order;
+    int rho_quirk;       /* 1: water Bernoulli reads V1[1] as in createM.cuh:104 */
+    int pad_;
+    double rho;
+    double kappa;
+    double depth;
+    double U;
+    double g;
+} sfo_props;
+
+int sfo_abi_version(void) { return 3; }
+
+int sfo_num_threads(void)
+{
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
+void sfo_set_num_threads(int n)
+{
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+#else
+    (void)n;
+#endif
+}
+
+/* ------------------------------------------------------------------ cot */
+/* utilities.cuh:312-315: cot(z) = 1.0 / tan(z) (libcu++ complex tan). */
+static inline cplx cot_c(cplx z) { return 1.0 / ctan(z); }
+
+void sfo_cot(double re, double im, double *out)
+{
+    cplx c = cot_c(re + I * im);
+    out[0] = creal(c);
+    out[1] = cimag(c);
+}
+
+/* ------------------------------------------------------------------ FFT */
+/* Unnormalised DFT X_k = sum_j x_j exp(sign * 2 pi i j k / n), sign = -1 forward (cuFFT
+ * CUFFT_FORWARD), +1 inverse (CUFFT_INVERSE, no 1/n). */
+static cplx twiddle(long long num, long long den, int sign)
+{
+    long long r = num % den;
+    if (r < 0) r += den;
+    double ang = 2.0 * PI_D * (double)r / (double)den;
+    return cos(ang) + I * (sign * sin(ang));
+}
+
+static void fft_pow2(cplx *a, int n, int sign)
+{
+    /* bit reversal */
+    for (int i = 1, j = 0; i < n; ++i) {
+        int bit = n >> 1;
+        for (; j & bit; bit >>= 1) j ^= bit;
+        j ^= bit;
+        if (i < j) { cplx t = a[i]; a[i] = a[j]; a[j] = t; }
+    }
+    for (int len = 2; len <= n; len <<= 1) {
+        int half = len >> 1;
+        for (int k = 0; k < half; ++k) {
+            cplx w = twiddle(k, len, sign);
+            for (int i = k; i < n; i += len) {
+                cplx u = a[i], v = a[i + half] * w;
+                a[i] = u + v;
+                a[i + half] = u - v;
+            }
+        }
+    }
+}
+
+static void fft_generic(cplx *a, int n, int sign, cplx *scratch)
+{
+    if (n <= 1) return;
+    if ((n & (n - 1)) == 0) { fft_pow2(a, n, sign); return; }
+    if (n % 2 == 0) {
+        int h = n / 2;
+        cplx *ev = scratch, *od = scratch + h;
+        for (int i = 0; i < h; ++i) { ev[i] = a[2 * i]; od[i] = a[2 * i + 1]; }
+        cplx *sub = (cplx *)malloc(sizeof(cplx) * (size_t)n);
+        fft_generic(ev, h, sign, sub);
+        fft_generic(od, h, sign, sub);
+        free(sub);
+        for (int k = 0; k < h; ++k) {
+            cplx t = twiddle(k, n, sign) * od[k];
+            a[k] = ev[k] + t;
+            a[k + h] = ev[k] - t;
+        }
+        return;
+    }
+    /* odd length: direct DFT */
+    for (int k = 0; k < n; ++k) {
+        cplx s = 0;
+        for (int j = 0; j < n; ++j) s += a[j] * twiddle((long long)j * k, n, sign);
+        scratch[k] = s;
+    }
+    memcpy(a, scratch, sizeof(cplx) * (size_t)n);
+}
+
+void sfo_fft(double *data, int n, int sign)
+{
+    cplx *a = (cplx *)data;
+    cplx *scratch = (cplx *)malloc(sizeof(cplx) * (size_t)(n > 0 ? n : 1));
+    fft_generic(a, n, sign, scratch);
+    free(scratch);
+}
+
+/* ------------------------------------------------------ spectral derivatives */
+/* first_derivative_multiplication, utilities.cuh:106-148 (quirks: Nyquist times i*pi*(n/2)/n,
+ * mode n/2+1 zeroed). */
+static void first_mult(cplx *c, int n)
+{
+    for (int i = 0; i < n; ++i) {
+        double x = creal(c[i]), y = cimag(c[i]);
+        double rx, ry;
+        if (i < n / 2) {
+            rx = (double)(-i) * y / (double)n;
+            ry = (double)i * x / (double)n;
+        } else if (i == n / 2) {
+            rx = -PI_D * (double)i * y / (double)n;
+            ry = PI_D * (double)i * x / (double)n;
+        } else if (i == n / 2 + 1) {
+            rx = 0.0;
+            ry = 0.0;
+        } else {
+            rx = (double)(-(i - n)) * y / (double)n;
+            ry = (double)(i - n) * x / (double)n;
+        }
+        c[i] = rx + I * ry;
+    }
+}
+
+/* second_derivative_fft, utilities.cuh:178-200 */
+static void second_mult(cplx *c, int n)
+{
+    for (int i = 0; i < n; ++i) {
+        double x = creal(c[i]), y = cimag(c[i]);
+        double m = (i <= n / 2) ? (double)(-i * i) : (double)(-(i - n) * (i - n));
+        c[i] = (m * x / (double)n) + I * (m * y / (double)n);
+    }
+}
+
+/* FftDerivative::exec, Derivatives.cuh:189-257 (no filter). */
+void sfo_fft_derivative(const double *in, double *out, int n, int second, double scale)
+{
+    cplx *c = (cplx *)malloc(sizeof(cplx) * (size_t)n);
+    memcpy(c, in, sizeof(cplx) * (size_t)n);
+    sfo_fft((double *)c, n, -1);
+    if (second) second_mult(c, n); else first_mult(c, n);
+    sfo_fft((double *)c, n, +1);
+    if (scale != 1.0)
+        for (int i = 0; i < n; ++i) c[i] *= scale;
+    memcpy(out, c, sizeof(cplx) * (size_t)n);
+    free(c);
+}
+
+/* ZPhiDerivative::exec, Derivatives.cuh:274-384 for a batch of surfaces.
+ * Z, Phi: complex (interleaved) B*n.  Outputs Zp, Zpp, Phip complex B*n. */
+void sfo_zphi_derivative(const sfo_props *p, int n, int batch, const double *Z, const double *Phi,
+                         double *Zp, double *Zpp, double *Phip)
+{
+    const double U = p->U, rho = p->rho;
+    cplx *per = (cplx *)malloc(sizeof(cplx) * (size_t)n);
+    for (int b = 0; b < batch; ++b) {
+        const cplx *z = (const cplx *)Z + (size_t)b * n;
+        const cplx *ph = (const cplx *)Phi + (size_t)b * n;
+        for (int j = 0; j < n; ++j) per[j] = z[j] - 2 * PI_D * (double)j / n;      /* :285,317 */
+        sfo_fft_derivative((double *)per, Zpp + 2 * (size_t)b * n, n, 1, 4.0 * PI_D * PI_D / (n * n));
+        sfo_fft_derivative((double *)per, Zp + 2 * (size_t)b * n, n, 0, 2.0 * PI_D / n);
+        for (int j = 0; j < n; ++j) per[j] = ph[j] - (-(1 + rho) * PI_D * U / n * (double)j); /* :286 */
+        sfo_fft_derivative((double *)per, Phip + 2 * (size_t)b * n, n, 0, 2.0 * PI_D / n);
+        cplx *zp = (cplx *)Zp + (size_t)b * n;
+        cplx *pp = (cplx *)Phip + (size_t)b * n;
+        for (int j = 0; j < n; ++j) zp[j] += 2.0 * PI_D / n;                         /* :374 */
+        if (U != 0)
+            for (int j = 0; j < n; ++j) pp[j] += -(1 + rho) * PI_D * U / n;          /* :380-383 */
+    }
+    free(per);
+}
+
+/* --------------------------------------------------------------- assembly */
+/* createMKernel (createM.cuh:43-63) for kind 0, createFiniteDepthMKernel (:65-92) for kind 1. */
+void sfo_assemble(const sfo_props *p, int n, int batch, const double *Z, const double *Zp,
+                  const double *Zpp, double *M)
+{
+    const double rho = p->rho, h = p->depth;
+    for (int b = 0; b < batch; ++b) {
+        const cplx *z = (const cplx *)Z + (size_t)b * n;
+        const cplx *zp = (const cplx *)Zp + (size_t)b * n;
+        const cplx *zpp = (const cplx *)Zpp + (size_t)b * n;
+        double *m = M + (size_t)b * n * n;
+        #pragma omp parallel for schedule(static)
+        for (int k = 0; k < n; ++k) {
+            for (int j = 0; j < n; ++j) {
+                double v;
+                if (p->kind == 0) {
+                    if (k == j)
+                        v = 0.5 * (1 + rho) + 0.25 * (1 - rho) / PI_D * cimag(zpp[k] / zp[k]);
+                    else
+                        v = 0.25 * (1 - rho) / PI_D * cimag(zp[k] * cot_c(0.5 * (z[k] - z[j])));
+                } else {
+                    if (k == j) {
+                        v = 0.5 + 0.25 / PI_D * cimag(zpp[k] / zp[k]);
+                        if (!p->infinite_depth)
+                            v -= 0.25 / PI_D * cimag(cot_c(I * (cimag(z[k]) + h)));
+                    } else {
+                        v = 0.25 / PI_D * cimag(zp[k] * cot_c(0.5 * (z[k] - z[j])));
+                        if (!p->infinite_depth)
+                            v -= 0.25 / PI_D * cimag(cot_c(0.5 * (z[k] - conj(z[j])) + I * h));
+                    }
+                }
+                m[(size_t)k * n + j] = v;
+            }
+        }
+    }
+}
+
+/* ------------------------------------------------------------------ GEPP */
+/* LU with partial pivoting (row interchanges, first index of max |a|), as LAPACK dgetrf /
+ * cusolverDnDgetrf.  Row-major, in place, right-looking blocked (panel width 64).
+ * piv[j] = row swapped with row j at step j (0-based).  Returns 0, or j+1 for the first
+ * exactly-zero pivot (the factorisation continues, as getrf does). */
+int sfo_lu_factor(int n, double *A, int *piv)
+{
+    const int nb = 64;
+    int info = 0;
+    for (int j0 = 0; j0 < n; j0 += nb) {
+        const int jb = (n - j0 < nb) ? n - j0 : nb;
+        const int je = j0 + jb;
+        for (int j = j0; j < je; ++j) {
+            int pr = j;
+            double amax = fabs(A[(size_t)j * n + j]);
+            for (int i = j + 1; i < n; ++i) {
+                double v = fabs(A[(size_t)i * n + j]);
+                if (v > amax) { amax = v; pr = i; }
+            }
+            piv[j] = pr;
+            if (pr != j) {
+                double *r1 = A + (size_t)j * n, *r2 = A + (size_t)pr * n;
+                for (int c = 0; c < n; ++c) { double t = r1[c]; r1[c] = r2[c]; r2[c] = t; }
+            }
+            const double d = A[(size_t)j * n + j];
+            if (d == 0.0) { if (!info) info = j + 1; continue; }
+            const double *rj = A + (size_t)j * n;
+            #pragma omp parallel for schedule(static) if (n - j > 256)
+            for (int i = j + 1; i < n; ++i) {
+                double *ri = A + (size_t)i * n;
+                double l = ri[j] / d;
+                ri[j] = l;
+                for (int c = j + 1; c < je; ++c) ri[c] -= l * rj[c];
+            }
+        }
+        if (je >= n) break;
+        /* U12 = L11^{-1} A12 (unit lower triangular forward substitution) */
+        for (int r = j0 + 1; r < je; ++r) {
+            double *rr = A + (size_t)r * n;
+            for (int q = j0; q < r; ++q) {
+                const double l = rr[q];
+                const double *rq = A + (size_t)q * n;
+                for (int c = je; c < n; ++c) rr[c] -= l * rq[c];
+            }
+        }
+        /* A22 -= L21 U12 */
+        #pragma omp parallel for schedule(static)
+        for (int i = je; i < n; ++i) {
+            double *ri = A + (size_t)i * n;
+            for (int q = j0; q < je; ++q) {
+                const double l = ri[q];
+                const double *rq = A + (size_t)q * n;
+                for (int c = je; c < n; ++c) ri[c] -= l * rq[c];
+            }
+        }
+    }
+    return info;
+}
+
+/* getrs with CUBLAS_OP_N: apply the interchanges, then L y = Pb (unit), U x = y. */
+void sfo_lu_solve(int n, const double *LU, const int *piv, double *b)
+{
+    for (int j = 0; j < n; ++j)
+        if (piv[j] != j) { double t = b[j]; b[j] = b[piv[j]]; b[piv[j]] = t; }
+    for (int i = 0; i < n; ++i) {
+        double s = b[i];
+        const double *ri = LU + (size_t)i * n;
+        for (int q = 0; q < i; ++q) s -= ri[q] * b[q];
+        b[i] = s;
+    }
+    for (int i = n - 1; i >= 0; --i) {
+        double s = b[i];
+        const double *ri = LU + (size_t)i * n;
+        for (int q = i + 1; q < n; ++q) s -= ri[q] * b[q];
+        b[i] = s / ri[i];
+    }
+}
+
+/* --------------------------------------------------------------- velocity */
+/* createVelocityMatrices (WaterVelocities.cuh:38-70) / createHeliumVelocityMatrices (:72-107)
+ * followed by VelocityCalculator::calculateVelocities (:205-242):
+ *   w = V2 o mu' + V1 mu,  q = conj(w).
+ * mu is real (the reference stores it complex with zero imaginary part), mup complex. */
+void sfo_velocity(const sfo_props *p, int n, int batch, const double *Z, const double *Zp,
+                  const double *Zpp, const double *mu, const double *mup, int lower, double *q)
+{
+    const double h = p->depth;
+    const int image = (p->kind == 1) && !p->infinite_depth;
+    for (int b = 0; b < batch; ++b) {
+        const cplx *z = (const cplx *)Z + (size_t)b * n;
+        const cplx *zp = (const cplx *)Zp + (size_t)b * n;
+        const cplx *zpp = (const cplx *)Zpp + (size_t)b * n;
+        const double *a = mu + (size_t)b * n;
+        const cplx *ap = (const cplx *)mup + (size_t)b * n;
+        cplx *out = (cplx *)q + (size_t)b * n;
+        #pragma omp parallel for schedule(static)
+        for (int k = 0; k < n; ++k) {
+            cplx v2 = I * (1.0 / (2.0 * PI_D * zp[k]));
+            cplx w = v2 * ap[k];
+            for (int j = 0; j < n; ++j) {
+                cplx v1;
+                if (k == j) {
+                    v1 = I * (-1.0 / (4.0 * PI_D) * zpp[k] / (zp[k] * zp[k]));
+                    if (image) v1 += I * (1.0 / (4.0 * PI_D) * cot_c(I * (cimag(z[k]) + h)));
+                    if (lower) v1 += 1.0 / (2.0 * zp[k]);
+                    else v1 -= 0.5 / zp[k];
+                } else {
+                    v1 = I * (-1.0 / (4.0 * PI_D) * cot_c(0.5 * (z[k] - z[j])));
+                    if (image) v1 += I * (1.0 / (4.0 * PI_D) * cot_c(0.5 * (z[k] - conj(z[j])) + I * h));
+                }
+                w += v1 * a[j];
+            }
+            out[k] = conj(w);
+        }
+    }
+}
+
+/* ------------------------------------------------------------- Bernoulli */
+/* compute_rhs_phi_expression (createM.cuh:96-107), helium variants (:109-117, 171-193,
+ * 200-211).  total = B*n nodes; outputs real dPhi/dt. */
+void sfo_rhs_phi(const sfo_props *p, int total, const double *Z, const double *Zp, const double *Zpp,
+                 const double *qlow, const double *qup, double *out)
+{
+    const cplx *z = (const cplx *)Z;
+    const cplx *v1 = (const cplx *)qlow;
+    const cplx *v2 = (const cplx *)qup;
+    const double rho = p->rho, h = p->depth, g = p->g;
+    for (int i = 0; i < total; ++i) {
+        double y = cimag(z[i]);
+        double r;
+        if (p->kind == 0) {
+            double a1 = creal(v1[i]) * creal(v1[i]) + cimag(v1[i]) * cimag(v1[i]);
+            double a2 = creal(v2[i]) * creal(v2[i]) + cimag(v2[i]) * cimag(v2[i]);
+            int qi = (p->rho_quirk && total > 1) ? 1 : i;
+            double dot = creal(v1[qi]) * creal(v2[i]) + cimag(v1[i]) * cimag(v2[i]);
+            r = -(1 + rho) * g * y + 0.5 * a1 + 0.5 * rho * a2 - rho * dot;
+        } else if (p->use_expansions) {
+            double kin = 0.5 * creal(v1[i]) * creal(v1[i]) + 0.5 * cimag(v1[i]) * cimag(v1[i]);
+            double vdw = 0.0;
+            switch (p->expansion_order) {
+            case 3: vdw += -10.0 / 3.0 * pow(y, 3.0) / (h * h); /* fall through */
+            case 2: vdw += 2.0 * pow(y, 2.0) / h;               /* fall through */
+            case 1: vdw += -y;                                   /* fall through */
+            default: break;
+            }
+            r = g * vdw + kin;
+        } else if (p->kappa != 0.0) {
+            const cplx *zp = (const cplx *)Zp;
+            const cplx *zpp = (const cplx *)Zpp;
+            double a1 = creal(v1[i]) * creal(v1[i]) + cimag(v1[i]) * cimag(v1[i]);
+            double curv = (creal(zp[i]) * cimag(zpp[i]) - cimag(zp[i]) * creal(zpp[i])) /
+                          pow(creal(zp[i]) * creal(zp[i]) + cimag(zp[i]) * cimag(zp[i]), 1.5);
+            r = g * (20.447761896665416 * h / 3.0 * (1.0 / pow(1.0 + y / h, 3) - 1)) + 0.5 * a1 + p->kappa * curv;
+        } else {
+            double vdw = g * (h / 3.0);
+            r = vdw * pow(1.0 + y / h, -3.0) - vdw + 0.5 * creal(v1[i]) * creal(v1[i]) +
+                0.5 * cimag(v1[i]) * cimag(v1[i]);
+        }
+        out[i] = r;
+    }
+}
+
+/* -------------------------------------------------------------------- RHS */
+typedef struct sfo_diag {
+    double *mu;     /* B*n real */
+    double *Zp;     /* B*n complex */
+    double *Zpp;    /* B*n complex */
+    double *Phip;   /* B*n complex */
+    double *mup;    /* B*n complex */
+    double *qup;    /* B*n complex */
+} sfo_diag;
+
+/* BaseBoundaryIntegralCalculator::runTimeStep (BaseBoundaryIntegrator.cuh:137-273) and
+ * calculateVorticities (:275-306).  state/rhs: complex [Z (B n), Phi (B n)] interleaved. */
+int sfo_rhs(const sfo_props *p, int n, int batch, const double *state, double *rhs, sfo_diag *diag)
+{
+    const size_t tot = (size_t)n * batch;
+    const double *Z = state;
+    const double *Phi = state + 2 * tot;
+    double *Zp = (double *)malloc(sizeof(double) * 2 * tot);
+    double *Zpp = (double *)malloc(sizeof(double) * 2 * tot);
+    double *Phip = (double *)malloc(sizeof(double) * 2 * tot);
+    double *mu = (double *)malloc(sizeof(double) * tot);
+    double *mucx = (double *)malloc(sizeof(double) * 2 * tot);
+    double *mup = (double *)malloc(sizeof(double) * 2 * tot);
+    double *qup = (double *)malloc(sizeof(double) * 2 * tot);
+    double *phirhs = (double *)malloc(sizeof(double) * tot);
+    double *M = (double *)malloc(sizeof(double) * (size_t)n * n);
+    int *piv = (int *)malloc(sizeof(int) * (size_t)n);
+    int info = 0;
+
+    sfo_zphi_derivative(p, n, batch, Z, Phi, Zp, Zpp, Phip);
+    for (int b = 0; b < batch; ++b) {
+        /* CreateMMatrix + complex_to_real + MatrixSolver::solve, one surface at a time */
+        sfo_assemble(p, n, 1, Z + 2 * (size_t)b * n, Zp + 2 * (size_t)b * n, Zpp + 2 * (size_t)b * n, M);
+        double *a = mu + (size_t)b * n;
+        for (int j = 0; j < n; ++j) a[j] = Phip[2 * ((size_t)b * n + j)];
+        int inf = sfo_lu_factor(n, M, piv);
+        if (inf && !info) info = inf;
+        sfo_lu_solve(n, M, piv, a);
+    }
+    /* real_to_complex + FftDerivative::exec(a) (:201-203) */
+    for (size_t i = 0; i < tot; ++i) { mucx[2 * i] = mu[i]; mucx[2 * i + 1] = 0.0; }
+    for (int b = 0; b < batch; ++b)
+        sfo_fft_derivative(mucx + 2 * (size_t)b * n, mup + 2 * (size_t)b * n, n, 0, 2.0 * PI_D / n);
+    /* CalculateVelocities lower into rhs[0:tot], upper into scratch (:240-242) */
+    sfo_velocity(p, n, batch, Z, Zp, Zpp, mu, mup, 1, rhs);
+    int need_upper = (p->kind == 0 && p->rho != 0.0);
+    if (need_upper || (diag && diag->qup))
+        sfo_velocity(p, n, batch, Z, Zp, Zpp, mu, mup, 0, qup);
+    else
+        memset(qup, 0, sizeof(double) * 2 * tot);
+    sfo_rhs_phi(p, (int)tot, Z, Zp, Zpp, rhs, qup, phirhs);
+    for (size_t i = 0; i < tot; ++i) {
+        rhs[2 * tot + 2 * i] = phirhs[i];
+        rhs[2 * tot + 2 * i + 1] = 0.0;
+    }
+    if (diag) {
+        if (diag->mu) memcpy(diag->mu, mu, sizeof(double) * tot);
+        if (diag->Zp) memcpy(diag->Zp, Zp, sizeof(double) * 2 * tot);
+        if (diag->Zpp) memcpy(diag->Zpp, Zpp, sizeof(double) * 2 * tot);
+        if (diag->Phip) memcpy(diag->Phip, Phip, sizeof(double) * 2 * tot);
+        if (diag->mup) memcpy(diag->mup, mup, sizeof(double) * 2 * tot);
+        if (diag->qup) memcpy(diag->qup, qup, sizeof(double) * 2 * tot);
+    }
+    free(Zp); free(Zpp); free(Phip); free(mu); free(mucx); free(mup); free(qup); free(phirhs);
+    free(M); free(piv);
+    return info;
+}
+
+/* -------------------------------------------------------------------- RK4 */
+typedef int (*sfo_rhs_fn)(void *ctx, const double *y, double *k);
+
+/* AutonomousRungeKuttaStepperBase::runStep (AutonomousRungeKuttaStepper.cuh:123-307) with the
+ * complex Zaxpy stages (:344-374).  y holds m complex values (2m doubles).  One step:
+ *   k1 = f(y0); y1 = y0 + (dt/2) k1; k2 = f(y1); y2 = y0 + (dt/2) k2;
+ *   k3 = f(y2); y3 = y0 + dt k3;     k4 = f(y3);
+ *   K = ((k1 + 2 k2) + 2 k3) + k4  (add_k_vectors, utilities.cuh:78-83);  y0 += (dt/6) K. */
+int sfo_rk4_generic(sfo_rhs_fn f, void *ctx, size_t m, double dt, int steps, double *y0, double *traj)
+{
+    const size_t len = 2 * m;
+    const double half = dt * 0.5, sixth = dt / 6.0;   /* setTimeStep, :31-36 */
+    double *k1 = (double *)malloc(sizeof(double) * len);
+    double *k2 = (double *)malloc(sizeof(double) * len);
+    double *k3 = (double *)malloc(sizeof(double) * len);
+    double *k4 = (double *)malloc(sizeof(double) * len);
+    double *ys = (double *)malloc(sizeof(double) * len);
+    int info = 0, r;
+    for (int s = 0; s < steps; ++s) {
+        if ((r = f(ctx, y0, k1)) && !info) info = r;
+        for (size_t i = 0; i < len; ++i) ys[i] = y0[i] + half * k1[i];
+        if ((r = f(ctx, ys, k2)) && !info) info = r;
+        for (size_t i = 0; i < len; ++i) ys[i] = y0[i] + half * k2[i];
+        if ((r = f(ctx, ys, k3)) && !info) info = r;
+        for (size_t i = 0; i < len; ++i) ys[i] = y0[i] + dt * k3[i];
+        if ((r = f(ctx, ys, k4)) && !info) info = r;
+        for (size_t i = 0; i < len; ++i) {
+            double kk = k1[i] + 2.0 * k2[i] + 2.0 * k3[i] + k4[i];
+            y0[i] = y0[i] + sixth * kk;
+        }
+        if (traj) memcpy(traj + (size_t)s * len, y0, sizeof(double) * len);
+    }
+    free(k1); free(k2); free(k3); free(k4); free(ys);
+    return info;
+}
+
+typedef struct { const sfo_props *p; int n, batch; } rhs_ctx;
+
+static int rhs_adapter(void *ctx, const double *y, double *k)
+{
+    rhs_ctx *c = (rhs_ctx *)ctx;
+    return sfo_rhs(c->p, c->n, c->batch, y, k, NULL);
+}
+
+int sfo_rk4(const sfo_props *p, int n, int batch, double dt, int steps, double *state, double *traj)
+{
+    rhs_ctx c = {p, n, batch};
+    return sfo_rk4_generic(rhs_adapter, &c, (size_t)2 * n * batch, dt, steps, state, traj);
+}
+
+/* dx/dt = -y, dy/dt = x on each complex entry z = x + i y, i.e. dz/dt = i z (the flip_x_y
+ * oscillator of the reference's RK4 test, ODESolverTests.cuh:14-23). */
+static int rotation_rhs(void *ctx, const double *y, double *k)
+{
+    size_t m = *(size_t *)ctx;
+    for (size_t i = 0; i < m; ++i) { k[2 * i] = -y[2 * i + 1]; k[2 * i + 1] = y[2 * i]; }
+    return 0;
+}
+
+int sfo_rk4_rotation(size_t m, double dt, int steps, double *y)
+{
+    return sfo_rk4_generic(rotation_rhs, &m, m, dt, steps, y, NULL);
+}
+
+/* -------------------------------------------------- SI to base units */
+/* adimensionalizeProperties, Export.cu:1222-1246 (rhoHelium = 150, alpha = 3.5e-24). */
+void sfo_adimensionalize(double L, double rho, double kappa, double depth, double *out)
+{
+    const double alpha = 3.5e-24, rho_he = 150.0;
+    double base_length = L / (2.0 * PI_D);
+    double base_acc = 3 * alpha / pow(depth, 4);
+    double base_time = sqrt(base_length / base_acc);
+    double stf = rho_he * base_length * base_length * base_length / (base_time * base_time);
+    out[0] = rho / rho_he;
+    out[1] = kappa / stf;
+    out[2] = depth / base_length;
+    out[3] = base_length;
+    out[4] = base_time;
+}

@@ -1,0 +1,318 @@
+"""Pins the CPU FP64 oracle against golden vectors, closed forms and invariants.
+
+Each test names the reference test or formula it re-derives (paths relative to
+CuSuperHelium/).  None of them restates the oracle's own code: expected values come from
+high-precision evaluation (mpmath), LAPACK/numpy, exact linear-theory solutions or exact
+algebraic identities of the discrete scheme.
+"""
+import numpy as np
+import pytest
+import scipy.linalg
+
+mpmath = pytest.importorskip("mpmath")
+
+
+# ------------------------------------------------------------------ cot (A11)
+def test_cot_golden_vector(oracle):
+    # CuSuperHelium.Tests/ComplexFunctionsTests.cuh:184-199: cot(0.5*(zk - zj)) with
+    # zk = pi + i pi, zj = zk + 0.1(1+i), 50-digit golden value, tolerance 1e-13.
+    zk = complex(np.pi, np.pi)
+    zj = complex(np.pi + 0.1, np.pi + 0.1)
+    c = oracle.cot(0.5 * (zk - zj))
+    assert abs(c.real - (-9.9833388915330681153509188355277398344111330657474)) < 1e-13
+    assert abs(c.imag - 10.016672219575397493791065794281138271298250914604) < 1e-13
+
+
+def test_cot_against_mpmath_and_symmetries(oracle):
+    mpmath.mp.dps = 40
+    rng = np.random.default_rng(1)
+    pts = list(rng.uniform(-3, 3, 40) + 1j * rng.uniform(-2, 2, 40))
+    pts += [1e-6 + 1e-7j, -3e-9 + 2e-9j, 0.3 + 25j, 1.0 - 30j, 1e-12j + 1e-3]
+    for w in pts:
+        ref = complex(mpmath.cot(mpmath.mpc(w.real, w.imag)))
+        c = oracle.cot(w)
+        assert abs(c - ref) <= 4e-15 * max(1.0, abs(ref))
+        assert abs(oracle.cot(-w) + c) <= 1e-15 * max(1.0, abs(c))          # odd
+        if abs(w) > 1e-2:   # w + pi loses the low bits of a tiny w
+            assert abs(oracle.cot(w + np.pi) - c) <= 1e-12 * max(1.0, abs(c))   # pi-periodic
+    w = 1e-4 * (1 + 1j)                                                      # Laurent series
+    assert abs(oracle.cot(w) - (1 / w - w / 3 - w ** 3 / 45)) < 1e-15 * abs(1 / w)
+
+
+# ------------------------------------------------------------------ FFT
+@pytest.mark.parametrize("n", [1, 2, 3, 4, 8, 16, 256, 768, 1536, 2048, 4096])
+def test_fft_matches_numpy(oracle, n):
+    rng = np.random.default_rng(n)
+    a = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    scale = max(1.0, np.sqrt(n))
+    assert np.abs(oracle.fft(a, -1) - np.fft.fft(a)).max() < 1e-13 * scale * np.log2(n + 1) + 1e-15
+    assert np.abs(oracle.fft(a, +1) - n * np.fft.ifft(a)).max() < 1e-13 * scale * np.log2(n + 1) + 1e-15
+
+
+# ------------------------------------------------------------------ derivatives (A1)
+def _mode(n, m):
+    j = np.arange(n)
+    return np.exp(2j * np.pi * m * j / n)
+
+
+@pytest.mark.parametrize("n", [8, 16, 64, 96])
+def test_first_derivative_multiplier_table(oracle, n):
+    # Multiplier table of utilities.cuh:119-147 applied to a single Fourier mode m: FFT gives
+    # n at index m, the multiplier maps it to n * mult(m), the unnormalised inverse returns
+    # n * mult(m) * e_m.  Expected mult(m):  i m/n (m < n/2), i pi (n/2)/n (m = n/2, quirk),
+    # 0 (m = n/2 + 1, quirk), i (m - n)/n otherwise.
+    for m in range(n):
+        if m < n // 2:
+            mult = 1j * m / n
+        elif m == n // 2:
+            mult = 1j * np.pi * (n // 2) / n
+        elif m == n // 2 + 1:
+            mult = 0.0
+        else:
+            mult = 1j * (m - n) / n
+        out = oracle.fft_derivative(_mode(n, m), second=False)
+        assert np.abs(out - n * mult * _mode(n, m)).max() < 1e-12 * n
+
+
+@pytest.mark.parametrize("n", [8, 64, 96])
+def test_second_derivative_multiplier_table(oracle, n):
+    # utilities.cuh:185-199: -m^2/n for m <= n/2, -(m - n)^2/n otherwise.
+    for m in range(n):
+        k = m if m <= n // 2 else m - n
+        out = oracle.fft_derivative(_mode(n, m), second=True, scale=0.5)
+        assert np.abs(out - 0.5 * (-(k * k)) * _mode(n, m)).max() < 1e-11 * n * n
+
+
+def _linear_wave(n, h, omega, t, rho):
+    # Analytic linear-wave fields of CuSuperHelium.Tests/MatrixMTests.cuh:19-67.
+    th = 2 * np.pi * np.arange(n) / n
+    X = th - h * np.sin(th - omega * t)
+    Y = h * np.cos(th - omega * t)
+    Phi = h * (1 + rho) * omega * np.sin(th - omega * t)
+    s = 2 * np.pi / n
+    Zp = s * ((1 - h * np.cos(th - omega * t)) + 1j * (-h * np.sin(th - omega * t)))
+    Zpp = s * s * (h * np.sin(th - omega * t) + 1j * (-h * np.cos(th - omega * t)))
+    Php = s * h * (1 + rho) * omega * np.cos(th - omega * t)
+    return X + 1j * Y, Phi, Zp, Zpp, Php
+
+
+def test_zphi_derivative_linear_wave_n1024(oracle):
+    # Kernels.ZPhiDerivatives (MatrixMTests.cuh:477-607): N=1024, h=0.5, omega=10, t=0, 1e-14.
+    Z, Phi, Zp, Zpp, Php = _linear_wave(1024, 0.5, 10.0, 0.0, 0.0)
+    zp, zpp, php = oracle.zphi_derivative(oracle.Props(), Z, Phi)
+    assert np.abs(zp - Zp).max() < 1e-14
+    assert np.abs(zpp - Zpp).max() < 1e-14
+    assert np.abs(php.real - Php).max() < 1e-14
+
+
+# ------------------------------------------------------------------ assembly (A2)
+def test_assemble_two_by_two_closed_form(oracle):
+    # Kernels.TwoByTwoMMatrix (MatrixMTests.cuh:69-87,160-223), N=2, h=0.5, t=0, rho=0.
+    h = 0.5
+    Z, _, Zp, Zpp, _ = _linear_wave(2, h, 10.0, 0.0, 0.0)
+    M = oracle.assemble(oracle.Props(), Z, Zp, Zpp)
+    t = np.tanh(h)   # sinh(2h)/(cosh(2h)+1)
+    expect = np.array([[0.5 - h / (4 * (1 - h)), (h - 1) / 4 * t],
+                       [(h + 1) / 4 * t, 0.5 + h / (4 * (1 + h))]])
+    assert np.abs(M - expect).max() < 1e-14
+
+
+def test_assemble_brute_force_mpmath(oracle):
+    # Kernels.MMatrixKernel (MatrixMTests.cuh:225-282): N=4, h=0.5, omega=10, t=0.1, against
+    # a 40-digit evaluation of M_kk = 1/2 + Im(Z''/Z')/4pi, M_kj = Im(Z'_k cot((z_k-z_j)/2))/4pi.
+    mpmath.mp.dps = 40
+    for n, h, t in [(4, 0.5, 0.1), (7, 0.3, 0.37)]:
+        Z, _, Zp, Zpp, _ = _linear_wave(n, h, 10.0, t, 0.0)
+        M = oracle.assemble(oracle.Props(), Z, Zp, Zpp)
+        mz = [mpmath.mpc(z.real, z.imag) for z in Z]
+        mzp = [mpmath.mpc(z.real, z.imag) for z in Zp]
+        mzpp = [mpmath.mpc(z.real, z.imag) for z in Zpp]
+        for k in range(n):
+            for j in range(n):
+                if k == j:
+                    ref = 0.5 + mpmath.im(mzpp[k] / mzp[k]) / (4 * mpmath.pi)
+                else:
+                    ref = mpmath.im(mzp[k] * mpmath.cot((mz[k] - mz[j]) / 2)) / (4 * mpmath.pi)
+                assert abs(M[k, j] - float(ref)) < 1e-14
+
+
+def test_assemble_film_image_term_mpmath(oracle):
+    # createFiniteDepthMKernel (createM.cuh:65-92): image term -Im(cot((z_k - conj z_j)/2 + i h))/4pi
+    # without a Z'_k factor, diagonal image -Im(cot(i (y_k + h)))/4pi.
+    mpmath.mp.dps = 40
+    n, h = 6, 1.0
+    Z, _, Zp, Zpp, _ = _linear_wave(n, 0.2, 1.0, 0.3, 0.0)
+    M = oracle.assemble(oracle.Props(kind=oracle.HELIUM, depth=h), Z, Zp, Zpp)
+    mz = [mpmath.mpc(z.real, z.imag) for z in Z]
+    mzp = [mpmath.mpc(z.real, z.imag) for z in Zp]
+    mzpp = [mpmath.mpc(z.real, z.imag) for z in Zpp]
+    ih = mpmath.mpc(0, h)
+    for k in range(n):
+        for j in range(n):
+            if k == j:
+                ref = 0.5 + mpmath.im(mzpp[k] / mzp[k]) / (4 * mpmath.pi)
+                ref -= mpmath.im(mpmath.cot(mpmath.mpc(0, mz[k].imag + h))) / (4 * mpmath.pi)
+            else:
+                ref = mpmath.im(mzp[k] * mpmath.cot((mz[k] - mz[j]) / 2)) / (4 * mpmath.pi)
+                ref -= mpmath.im(mpmath.cot((mz[k] - mpmath.conj(mz[j])) / 2 + ih)) / (4 * mpmath.pi)
+            assert abs(M[k, j] - float(ref)) < 2e-14
+
+
+def test_flat_surface_matrix_is_half_identity(oracle):
+    # A flat surface has real Z' and real cot arguments, so K vanishes exactly: M = I/2, mu = 2 b.
+    n = 64
+    x = 2 * np.pi * np.arange(n) / n
+    Phi = 0.01 * np.sin(3 * x)
+    zp, zpp, php = oracle.zphi_derivative(oracle.Props(), x + 0j, Phi)
+    M = oracle.assemble(oracle.Props(), x + 0j, zp, zpp)
+    assert np.array_equal(M, 0.5 * np.eye(n))
+    _, _, d = oracle.rhs(oracle.Props(), x + 0j, Phi, diagnostics=True)
+    assert np.abs(d["mu"] - 2 * php.real).max() < 1e-17
+
+
+# ------------------------------------------------------------------ GEPP (A4)
+@pytest.mark.parametrize("n", [1, 5, 64, 65, 200])
+def test_lu_matches_lapack_getrf(oracle, n):
+    # cusolverDnDgetrf semantics: partial pivoting, first max.  LAPACK dgetrf (scipy) is the
+    # textbook routine; pivots must agree exactly and factors to rounding.
+    rng = np.random.default_rng(n)
+    A = rng.standard_normal((n, n))
+    LU, piv, info = oracle.lu_factor(A)
+    lu_ref, piv_ref = scipy.linalg.lu_factor(A)
+    assert info == 0
+    assert np.array_equal(piv, piv_ref)
+    assert np.abs(LU - lu_ref).max() < 1e-12 * n
+    b = rng.standard_normal(n)
+    x = oracle.lu_solve(LU, piv, b)
+    xr = np.linalg.solve(A, b)
+    assert np.abs(x - xr).max() < 1e-14 * np.linalg.cond(A) * np.abs(xr).max() * n
+    assert np.linalg.norm(A @ x - b) <= 1e-13 * n * np.linalg.norm(A) * np.linalg.norm(x) + 1e-300
+
+
+def test_lu_reports_zero_pivot(oracle):
+    A = np.ones((4, 4))
+    _, _, info = oracle.lu_factor(A)
+    assert info == 2
+
+
+# ------------------------------------------------------------------ velocity (A5-A7)
+@pytest.mark.parametrize("n,m", [(32, 1), (64, 3), (128, 5)])
+def test_flat_surface_velocity_is_exact_potential_flow(oracle, n, m):
+    # Flat surface with Phi = eps sin(m x): the deep-water potential eps e^{m y} sin(m x) gives
+    # q = u + i v = eps m (cos m x + i sin m x) at y = 0, and dPhi/dt = |q|^2 / 2.
+    x = 2 * np.pi * np.arange(n) / n
+    eps = 1e-3
+    q, dphi = oracle.rhs(oracle.Props(), x + 0j, eps * np.sin(m * x))
+    assert np.abs(q - eps * m * (np.cos(m * x) + 1j * np.sin(m * x))).max() < 1e-15
+    assert np.abs(dphi - 0.5 * eps * eps * m * m).max() < 1e-18
+
+
+def test_velocity_conserves_volume(oracle):
+    # Incompressible deep water: the normal flux through the surface, sum Im(conj(Z') q), is 0.
+    n = 128
+    x = 2 * np.pi * np.arange(n) / n
+    Z = x + 1j * (0.05 * np.cos(x) + 0.02 * np.sin(3 * x + 0.4))
+    Phi = 0.03 * np.sin(2 * x + 0.1)
+    q, _, d = oracle.rhs(oracle.Props(), Z, Phi, diagnostics=True)
+    assert abs(np.sum(np.imag(np.conj(d["Zp"]) * q))) < 1e-14
+
+
+# ------------------------------------------------------------------ Bernoulli (A8)
+def test_helium_bernoulli_matches_taylor_expansion(oracle):
+    # (h/3)((1 + y/h)^-3 - 1) = -y + 2 y^2/h - (10/3) y^3/h^2 + O(y^4): the plain kernel
+    # (createM.cuh:109-117) and the order-3 expansion kernel (:171-193) agree to O(y^4).
+    n = 16
+    rng = np.random.default_rng(3)
+    q = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    zp = np.full(n, 2 * np.pi / n + 0j)
+    for eps in (1e-2, 1e-3):
+        Z = np.arange(n) + 1j * eps * rng.uniform(-1, 1, n)
+        plain = oracle.rhs_phi(oracle.Props(kind=oracle.HELIUM, depth=1.0), Z, zp, zp, q, q)
+        exp3 = oracle.rhs_phi(oracle.Props(kind=oracle.HELIUM, depth=1.0, use_expansions=True,
+                                           expansion_order=3), Z, zp, zp, q, q)
+        assert np.abs(plain - exp3).max() < 5 * eps ** 4
+
+
+def test_water_bernoulli_deep_water_value(oracle):
+    # -(1+rho) y + |q1|^2/2 at rho = 0 (Kernels.RhsPhi, MatrixMTests.cuh:735-820).
+    y = np.array([0.1, -0.2, 0.3])
+    q = np.array([0.5 + 0.25j, -1.0 + 0j, 0.0 + 2.0j])
+    out = oracle.rhs_phi(oracle.Props(), 1j * y, q, q, q, np.zeros(3))
+    assert np.allclose(out, -y + 0.5 * np.abs(q) ** 2, rtol=0, atol=1e-16)
+
+
+# ------------------------------------------------------------------ RK4 (A10)
+def test_rk4_rotation_is_exact_discrete_map(oracle):
+    # ODE_Solvers.RK4 (ODESolverTests.cuh:14-23,73-246): dz/dt = i z.  Classical RK4 maps
+    # z -> R(i dt) z with R(w) = 1 + w + w^2/2 + w^3/6 + w^4/24, so after n steps z_n = R^n z_0.
+    rng = np.random.default_rng(5)
+    z0 = rng.standard_normal(256) + 1j * rng.standard_normal(256)
+    dt, steps = 0.001, 10000
+    zn = oracle.rk4_rotation(z0, dt, steps)
+    w = 1j * dt
+    R = 1 + w + w ** 2 / 2 + w ** 3 / 6 + w ** 4 / 24
+    assert np.abs(zn - R ** steps * z0).max() < 1e-11
+    assert np.abs(zn - np.exp(1j * dt * steps) * z0).max() < 1e-10
+
+
+def test_linear_standing_wave_dispersion(oracle):
+    # Deep water, g = 1, k = 1: eta = h cos x cos t, phi = -h sin t e^y cos x.  After half a
+    # period (t = pi) Y = -h cos x and X = x + 2 h sin x, up to O(h^2).
+    n = 64
+    x = 2 * np.pi * np.arange(n) / n
+    h = 1e-4
+    Z, P = oracle.rk4(oracle.Props(), x + 1j * h * np.cos(x), np.zeros(n), np.pi / 200, 200)
+    assert np.abs(Z.imag + h * np.cos(x)).max() < 5 * h * h
+    assert np.abs(Z.real - (x + 2 * h * np.sin(x))).max() < 5 * h * h
+
+
+def test_gravity_scaling_is_time_rescaling(oracle):
+    # dz/dt = q[phi], dphi/dt = |q|^2/2 + g P(y): with tau = t sqrt(g), psi = phi/sqrt(g) the
+    # RK4 maps for (g, dt) and (1, dt sqrt(g)) coincide in exact arithmetic.
+    n = 32
+    x = 2 * np.pi * np.arange(n) / n
+    Z = x + 1j * (0.05 * np.cos(x) + 0.02 * np.sin(3 * x))
+    g = 0.15
+    for kind in (oracle.WATER, oracle.HELIUM):
+        Za, Pa = oracle.rk4(oracle.Props(kind=kind, g=g), Z, np.zeros(n), 0.01, 30)
+        Zb, Pb = oracle.rk4(oracle.Props(kind=kind, g=1.0), Z, np.zeros(n), 0.01 * np.sqrt(g), 30)
+        assert np.abs(Za - Zb).max() < 1e-14
+        assert np.abs(Pa - np.sqrt(g) * Pb).max() < 1e-14
+
+
+def test_time_reversal(oracle):
+    n = 32
+    x = 2 * np.pi * np.arange(n) / n
+    Z0 = x + 1j * 0.05 * np.cos(2 * x)
+    P0 = 0.01 * np.sin(x)
+    Z1, P1 = oracle.rk4(oracle.Props(), Z0, P0, 0.01, 40)
+    Z2, P2 = oracle.rk4(oracle.Props(), Z1, P1, -0.01, 40)
+    assert np.abs(Z2 - Z0).max() < 1e-9 and np.abs(P2 - P0).max() < 1e-9
+
+
+# ------------------------------------------------------------------ batching
+def test_batched_rhs_equals_per_surface(oracle):
+    n, B = 32, 3
+    x = 2 * np.pi * np.arange(n) / n
+    Zs = [x + 1j * a * np.cos(x + s) for a, s in ((0.05, 0.0), (0.02, 1.0), (0.07, 2.0))]
+    Ps = [0.01 * np.sin(k * x) for k in (1, 2, 3)]
+    qb, db = oracle.rhs(oracle.Props(rho_quirk=False), np.concatenate(Zs), np.concatenate(Ps), batch=B)
+    for b in range(B):
+        q, d = oracle.rhs(oracle.Props(), Zs[b], Ps[b])
+        assert np.array_equal(qb[b * n:(b + 1) * n], q)
+        assert np.array_equal(db[b * n:(b + 1) * n], d)
+
+
+# ------------------------------------------------------------------ units (A12)
+def test_adimensionalize_closed_form(oracle):
+    # Export.cu:1222-1246 with alpha = 3.5e-24, rho_He = 150.
+    L, rho, kappa, depth = 1e-3, 3.0, 2e-4, 2e-8
+    d = oracle.adimensionalize(L, rho, kappa, depth)
+    L0 = L / (2 * np.pi)
+    g0 = 3 * 3.5e-24 / depth ** 4
+    t0 = np.sqrt(L0 / g0)
+    assert np.isclose(d["base_length"], L0, rtol=1e-15)
+    assert np.isclose(d["base_time"], t0, rtol=1e-14)
+    assert np.isclose(d["depth"], depth / L0, rtol=1e-15)
+    assert np.isclose(d["rho"], rho / 150.0, rtol=1e-15)
+    assert np.isclose(d["kappa"], kappa / (150.0 * L0 ** 3 / t0 ** 2), rtol=1e-14)
