@@ -1,0 +1,190 @@
+/*
+ * cusuperhelium.h -- C ABI of the gfx950 (MI355X) Roberts boundary-integral RK4 stepper.
+ *
+ * Two groups of entry points:
+ *
+ * 1. Drop-in replacements for the reference CuSuperHelium DLL exports (paths relative to
+ *    CuSuperHelium/CuSuperHelium/).  Same names, argument meaning, host-pointer semantics
+ *    (caller-owned SoA float64 arrays in, caller-allocated arrays out, trajectories allocated
+ *    with malloc and released by the *_freeMemory call) and SI-unit conversion
+ *    (adimensionalizeProperties, Export.cu:1222-1246).  Like the reference they build the
+ *    helium-film problem (HeliumBoundaryProblem).  Return codes: 0 success, -1 runtime/device
+ *    error, -2 unsupported size (the reference falls off the end of the function here),
+ *    -3 invalid argument.  No entry point ever calls exit().
+ *
+ * 2. The native handle API (sfh_*): run-time sizes, many surfaces of different N per handle,
+ *    nondimensional parameters, problem kind (water or helium film), state kept in HBM across
+ *    calls, RK4 steps replayed from a captured hipGraph.
+ *
+ * Every call is synchronous with respect to host memory it reads or writes.
+ */
+#ifndef CUSUPERHELIUM_H
+#define CUSUPERHELIUM_H
+
+#include <stdbool.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------------------------
+ * Reference C structs (ExportTypes.cuh:7-40), same field order and types.
+ * ---------------------------------------------------------------------------------------- */
+typedef struct __attribute__((aligned(16))) c_double {
+    double re;
+    double im;
+} c_double;
+
+typedef struct SimProperties {
+    double L;
+    double rho;
+    double kappa;
+    double depth;
+    bool use_expansions;
+    int expansion_order;
+    bool infinite_depth;
+} SimProperties;
+
+typedef struct RK4SolverOptions {
+    double timeStep;
+    double t0;
+    double t1;
+    bool returnTrajectory;
+} RK4SolverOptions;
+
+/* ------------------------------------------------------------------------------------------
+ * Drop-in entry points.
+ * ---------------------------------------------------------------------------------------- */
+
+/* Export.cuh:31 / Export.cu:560-599 -- RHS (vx + i vy = dZ/dt, rhsPhi = dPhi/dt) of one surface
+ * of N nodes, N in {1,2,4,...,4096} plus 768 and 1536. */
+int calculateRHSFromVectors(const double* x, const double* y, const double* phi, double* vx, double* vy,
+                            double* rhsPhi, double L, double rho, double kappa, double depth, size_t N);
+
+/* Export.cuh:30 / Export.cu:116-189 -- N = 256. */
+int calculateRHS256FromVectors(const double* x, const double* y, const double* phi, double* vx, double* vy,
+                               double* rhsPhi, double L, double rho, double kappa, double depth);
+
+/* Export.cuh:33 / Export.cu:267-277 -- N = 2048. */
+int calculateRHS2048FromVectors(const double* x, const double* y, const double* phi, double* vx, double* vy,
+                                double* rhsPhi, double L, double rho, double kappa, double depth);
+
+/* Export.cuh:51 / Export.cu:520-557 -- batchSize surfaces of N = 256 concatenated, batchSize in
+ * {1,2,4,...,2048} plus 768 and 1536. */
+int calculateRHS256FromVectorsBatched(const double* x, const double* y, const double* phi, double* vx,
+                                      double* vy, double* rhsPhi, double L, double rho, double kappa,
+                                      double depth, int batchSize);
+
+/* Export.cuh:48 / Export.cu:279-349 -- dipole strengths a (the reference's "vorticities") and,
+ * when the pointers are non-null, Z' and Z'' of a 256-node surface. */
+int calculateVorticities256FromVectors(const c_double* Z, const c_double* phi, double* a, c_double* Zp,
+                                       c_double* Zpp, double L, double rho, double kappa, double depth);
+
+/* Export.cuh:49 / Export.cu:351-384 -- spectral first derivative (reference multiplier table,
+ * unit scale) of a 256-sample complex field. */
+int calculateDerivativeFFT256(const c_double* input, c_double* output);
+
+/* Export.cuh:69 / Export.cu:748-770 (a stub returning 0 in the reference) -- classical RK4 from
+ * t0 to t1 (SI seconds) with floor((t1 - t0)/dt) steps.  initialState = [x(N), y(N), phi(N)].
+ * With returnTrajectory: *statesOut = malloc'd [steps][3N] states after every step and
+ * *timesOut = malloc'd [steps] nondimensional times; otherwise one final state and no times.
+ * N as for calculateRHSFromVectors. */
+int integrateSimulationRK4(double* initialState, double** statesOut, size_t* statesCount, double** timesOut,
+                           size_t* timesCount, SimProperties* simProperties, RK4SolverOptions* rkOptions,
+                           size_t N);
+
+/* Export.cuh:70 / Export.cu:772-777 -- releases the arrays returned by integrateSimulationRK4. */
+int integrateSimulationRK4_freeMemory(double* statesOut, double* timesOut);
+
+/* ------------------------------------------------------------------------------------------
+ * Native handle API.
+ * ---------------------------------------------------------------------------------------- */
+typedef struct sfh_solver* sfh_handle;
+
+enum { SFH_WATER = 0, SFH_HELIUM = 1 };
+enum { SFH_FP64 = 0, SFH_MIXED = 1 };
+enum { SFH_SOLVE_LU = 0, SFH_SOLVE_GMRES = 1 };
+
+/* Nondimensional problem (ProblemProperties.hpp:5-32 plus kind and g).  g scales the restoring
+ * term of the Bernoulli equation (1 reproduces the reference).  rho_quirk = 1 reproduces the
+ * V1[1] index of compute_rhs_phi_expression (createM.cuh:104), which matters only for rho != 0. */
+typedef struct sfh_props {
+    int kind;
+    int infinite_depth;
+    int use_expansions;
+    int expansion_order;
+    int rho_quirk;
+    int reserved;
+    double rho;
+    double kappa;
+    double depth;
+    double U;
+    double g;
+} sfh_props;
+
+typedef struct sfh_options {
+    int precision;    /* SFH_FP64 (the drop-in arithmetic) or SFH_MIXED */
+    int solver;       /* SFH_SOLVE_LU or SFH_SOLVE_GMRES */
+    int use_graph;    /* replay RK4 steps from a captured hipGraph */
+    int max_iter;     /* iterative solver / refinement iteration cap */
+    int restart;      /* GMRES restart length */
+    int reserved;
+    double tol;       /* iterative solver / refinement relative tolerance */
+} sfh_options;
+
+typedef struct sfh_stats {
+    long long rhs_evals;
+    long long steps;
+    int fallback_solves;   /* systems that needed row interchanges and went through GEPP */
+    int spin_timeouts;
+    int last_iterations;
+    int reserved;
+} sfh_stats;
+
+/* Defaults: FP64, LU, graph replay on, tol 1e-12. */
+void sfh_default_options(sfh_options* opt);
+void sfh_default_props(sfh_props* props);
+
+/* Creates a solver for n_surfaces independent periodic surfaces with n[s] nodes each
+ * (n = 2^k or 3 * 2^k, 1 <= n <= 8192).  Node arrays of all calls are the surfaces'
+ * node arrays concatenated in this order. */
+int sfh_create(const sfh_props* props, const int* n, int n_surfaces, const sfh_options* opt, sfh_handle* out);
+int sfh_destroy(sfh_handle h);
+int sfh_total_nodes(sfh_handle h);
+int sfh_set_state(sfh_handle h, const double* x, const double* y, const double* phi);
+int sfh_get_state(sfh_handle h, double* x, double* y, double* phi);
+/* RHS of the current state.  mu (dipole strengths), zp / zpp (interleaved re, im) and qup
+ * (upper-fluid velocity, interleaved) are optional outputs (may be NULL). */
+int sfh_rhs(sfh_handle h, double* vx, double* vy, double* dphi, double* mu, double* zp, double* zpp,
+            double* qup);
+/* Advances every surface by nsteps classical RK4 steps of size dt, in place. */
+int sfh_step(sfh_handle h, double dt, int nsteps);
+/* Dipole strengths of the last right-hand-side evaluation. */
+int sfh_get_mu(sfh_handle h, double* mu);
+int sfh_sync(sfh_handle h);
+int sfh_get_stats(sfh_handle h, sfh_stats* out);
+/* Device addresses of the state (complex128 Z, float64 phi) for zero-copy interop. */
+int sfh_device_state(sfh_handle h, void** Z, void** phi);
+/* Kernel timing.  While enabled, sfh_step / sfh_rhs launch without graph replay and bracket
+ * every kernel launch with hipEvents on the solver's stream.  sfh_kernel_report sums, for one
+ * kernel tag ("deriv", "assemble", "lu_panel", "lu_update", "lu_backsolve", "mu_deriv",
+ * "velocity"), the launches, milliseconds, algorithmic flops and algorithmic bytes recorded since
+ * timing was enabled.  No counterpart in the reference (it has no timers). */
+int sfh_kernel_timing(sfh_handle h, int enable);
+int sfh_kernel_report(sfh_handle h, const char* tag, int* launches, double* ms, double* flops, double* bytes);
+
+/* Message of the last failed call on this thread. */
+const char* sfh_last_error(void);
+
+/* Test hook: solves A x = b (n x n row-major host arrays) with the device LU path of the
+ * stepper (interchange-free blocked LU, GEPP kernel when a multiplier exceeds 1, reported in
+ * *interchanged).  No counterpart in the reference; it exposes MatrixSolver::solve semantics
+ * (MatrixSolver.cuh:113-126) on arbitrary matrices. */
+int sfh_debug_lu_solve(int n, const double* A, const double* b, double* x, int* interchanged);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CUSUPERHELIUM_H */

@@ -1,0 +1,51 @@
+// assemble.hip -- dipole matrix M = I/2 + K in FP64, one wave per row.
+//
+// Replaces createMKernel (CuSuperHelium/createM.cuh:43-63) and createFiniteDepthMKernel
+// (:65-92), which run one thread per entry on 16x16 blocks.  Here wave w of a workgroup owns
+// row k; its 64 lanes sweep the columns, so Z_k, Z'_k stay in registers, Z_j loads and M row
+// stores are fully coalesced, and cot comes from the cancellation-free form in sfh_common.hpp.
+//   M_kk = (1+rho)/2 + (1-rho)/(4 pi) Im(Z''_k / Z'_k)                       (water)
+//   M_kj = (1-rho)/(4 pi) Im(Z'_k cot((z_k - z_j)/2))
+//   helium: rho -> 0 and, with finite depth h, the image terms
+//   M_kk -= Im(cot(i (y_k + h)))/(4 pi),  M_kj -= Im(cot((z_k - conj z_j)/2 + i h))/(4 pi)
+#include "sfh_kernels.hpp"
+
+namespace sfh {
+
+__global__ __launch_bounds__(256) void assemble64_kernel(const SurfDesc* __restrict__ sd,
+                                                         const double2* __restrict__ Z,
+                                                         const double2* __restrict__ Zp,
+                                                         const double2* __restrict__ Zpp,
+                                                         double* __restrict__ M, Phys ph,
+                                                         const int* __restrict__ only_if) {
+    if (only_if && only_if[2 * blockIdx.y] == 0) return;   // per-surface status word
+    const SurfDesc d = sd[blockIdx.y];
+    const int n = d.n;
+    const int k = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (k >= n) return;
+    const double2* z = Z + d.voff;
+    const double2 zk = z[k];
+    const double2 zpk = Zp[d.voff + k];
+    const bool water = ph.kind == 0;
+    const double c = water ? 0.25 * (1 - ph.rho) / kPi : 0.25 / kPi;
+    const double ci = 0.25 / kPi;
+    const double h = ph.depth;
+    double* row = M + d.moff + (long long)k * n;
+    for (int j = lane; j < n; j += 64) {
+        double v;
+        if (j == k) {
+            const double2 r = cdiv(Zpp[d.voff + k], zpk);
+            v = (water ? 0.5 * (1 + ph.rho) : 0.5) + c * r.y;
+            if (ph.image) v -= ci * cot64(0.0, zk.y + h).y;
+        } else {
+            const double2 zj = z[j];
+            const double2 ct = cot64(0.5 * (zk.x - zj.x), 0.5 * (zk.y - zj.y));
+            v = c * (zpk.x * ct.y + zpk.y * ct.x);
+            if (ph.image) v -= ci * cot64(0.5 * (zk.x - zj.x), 0.5 * (zk.y + zj.y) + h).y;
+        }
+        row[j] = v;
+    }
+}
+
+}  // namespace sfh

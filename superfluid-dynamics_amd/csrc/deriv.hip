@@ -1,0 +1,206 @@
+// deriv.hip -- spectral tangential derivatives, one workgroup per (surface, output field).
+//
+// Replaces ZPhiDerivative::exec (CuSuperHelium/Derivatives.cuh:310-384) and
+// FftDerivative::exec (:189-257), which run 3 cuFFT forward/inverse pairs plus ~10 small
+// kernels per RHS.  Each workgroup loads one field into LDS in digit-reversed order,
+// transforms it forward in place (radix-2 decimation in time, plus a radix-3 combine when
+// n = 3 * 2^m), applies the reference multiplier table (utilities.cuh:106-148 first derivative,
+// :178-200 second), transforms back with the mirror-image decimation-in-frequency pass (so the
+// result lands in the same digit-reversed slots and no permutation pass is needed) and adds the
+// linear parts on the way out.
+#include "sfh_kernels.hpp"
+
+namespace sfh {
+
+__device__ inline int brev(int j, int bits) {
+    return bits == 0 ? 0 : (int)(__brev((unsigned)j) >> (32 - bits));
+}
+
+// LDS slot of time/frequency sample j: (j mod r) * 2^m + bitrev_m(j / r), r in {1, 3}.
+__device__ inline int lds_index(const SurfDesc& d, int j) {
+    if (!d.r3) return brev(j, d.log2m);
+    const int q = j % 3;
+    return q * (1 << d.log2m) + brev(j / 3, d.log2m);
+}
+
+__device__ inline double2 twid(const double2* __restrict__ tw, int t, int sign) {
+    double2 w = tw[t];
+    if (sign > 0) w.y = -w.y;
+    return w;
+}
+
+// Decimation in time: input in lds_index order, output in natural order.
+// tw[t] = exp(-2 pi i t / n), t < n; sign -1 forward, +1 inverse (unnormalised).
+__device__ void fft_dit(double2* s, const SurfDesc& d, int sign, const double2* __restrict__ tw) {
+    const int n = d.n, m = 1 << d.log2m, sub = d.r3 ? 3 : 1, halfm = m >> 1;
+    for (int len = 2; len <= m; len <<= 1) {
+        const int half = len >> 1, stride = n / len;
+        for (int t = threadIdx.x; t < sub * halfm; t += blockDim.x) {
+            const int q = t / halfm, b = t - q * halfm;
+            const int grp = b / half, pos = b - grp * half;
+            const int i = q * m + grp * len + pos;
+            const double2 u = s[i];
+            const double2 v = cmul(s[i + half], twid(tw, pos * stride, sign));
+            s[i] = cadd(u, v);
+            s[i + half] = csub(u, v);
+        }
+        __syncthreads();
+    }
+    if (d.r3) {   // X[k + q m] = sum_r W_n^{r k} W_3^{r q} F_r[k]
+        const double c = -0.5, s3 = 0.86602540378443864676372317075293618 * sign;
+        for (int k = threadIdx.x; k < m; k += blockDim.x) {
+            const double2 f0 = s[k];
+            const double2 f1 = cmul(s[m + k], twid(tw, k, sign));
+            const double2 f2 = cmul(s[2 * m + k], twid(tw, 2 * k, sign));
+            const double2 t = cadd(f1, f2), dd = csub(f1, f2);
+            const double2 base = make_double2(f0.x + c * t.x, f0.y + c * t.y);
+            const double2 rot = make_double2(-s3 * dd.y, s3 * dd.x);
+            s[k] = cadd(f0, t);
+            s[m + k] = cadd(base, rot);
+            s[2 * m + k] = csub(base, rot);
+        }
+        __syncthreads();
+    }
+}
+
+// Decimation in frequency: input in natural order, output in lds_index order.
+__device__ void fft_dif(double2* s, const SurfDesc& d, int sign, const double2* __restrict__ tw) {
+    const int n = d.n, m = 1 << d.log2m, sub = d.r3 ? 3 : 1, halfm = m >> 1;
+    if (d.r3) {   // y_q[j] = W_n^{j q} sum_r x[j + r m] W_3^{r q}
+        const double c = -0.5, s3 = 0.86602540378443864676372317075293618 * sign;
+        for (int j = threadIdx.x; j < m; j += blockDim.x) {
+            const double2 x0 = s[j], x1 = s[m + j], x2 = s[2 * m + j];
+            const double2 t = cadd(x1, x2), dd = csub(x1, x2);
+            const double2 base = make_double2(x0.x + c * t.x, x0.y + c * t.y);
+            const double2 rot = make_double2(-s3 * dd.y, s3 * dd.x);
+            s[j] = cadd(x0, t);
+            s[m + j] = cmul(cadd(base, rot), twid(tw, j, sign));
+            s[2 * m + j] = cmul(csub(base, rot), twid(tw, 2 * j, sign));
+        }
+        __syncthreads();
+    }
+    for (int len = m; len >= 2; len >>= 1) {
+        const int half = len >> 1, stride = n / len;
+        for (int t = threadIdx.x; t < sub * halfm; t += blockDim.x) {
+            const int q = t / halfm, b = t - q * halfm;
+            const int grp = b / half, pos = b - grp * half;
+            const int i = q * m + grp * len + pos;
+            const double2 u = s[i], v = s[i + half];
+            s[i] = cadd(u, v);
+            s[i + half] = cmul(csub(u, v), twid(tw, pos * stride, sign));
+        }
+        __syncthreads();
+    }
+}
+
+// First-derivative multiplier (utilities.cuh:119-147), applied to coefficient i.
+__device__ inline double2 first_mult(double2 c, int i, int n) {
+    const double dn = (double)n;
+    if (i < n / 2) return make_double2((double)(-i) * c.y / dn, (double)i * c.x / dn);
+    if (i == n / 2) return make_double2(-kPi * (double)i * c.y / dn, kPi * (double)i * c.x / dn);
+    if (i == n / 2 + 1) return make_double2(0.0, 0.0);
+    return make_double2((double)(-(i - n)) * c.y / dn, (double)(i - n) * c.x / dn);
+}
+
+// Second-derivative multiplier (utilities.cuh:185-199).
+__device__ inline double2 second_mult(double2 c, int i, int n) {
+    const double m = (i <= n / 2) ? (double)(-i * i) : (double)(-(i - n) * (i - n));
+    return make_double2(m * c.x / (double)n, m * c.y / (double)n);
+}
+
+// forward transform, multiplier, inverse transform; leaves sample j at s[lds_index(j)].
+__device__ inline void derivative_in_lds(double2* s, const SurfDesc& d, const double2* tw, bool second) {
+    fft_dit(s, d, -1, tw);
+    for (int i = threadIdx.x; i < d.n; i += blockDim.x)
+        s[i] = second ? second_mult(s[i], i, d.n) : first_mult(s[i], i, d.n);
+    __syncthreads();
+    fft_dif(s, d, +1, tw);
+}
+
+// blockIdx.x = surface, blockIdx.y = field: 0 -> Z', 1 -> Z'', 2 -> Re Phi' (into b and work).
+__global__ __launch_bounds__(256) void zphi_deriv_kernel(const SurfDesc* __restrict__ sd,
+                                                         const double2* __restrict__ tw_all,
+                                                         const double2* __restrict__ Z,
+                                                         const double* __restrict__ phi,
+                                                         double2* __restrict__ Zp,
+                                                         double2* __restrict__ Zpp,
+                                                         double* __restrict__ b,
+                                                         double* __restrict__ work, Phys ph,
+                                                         int* __restrict__ status) {
+    extern __shared__ double2 s[];
+    const SurfDesc d = sd[blockIdx.x];
+    const int field = blockIdx.y;
+    // re-arm the surface's interchange flag for the solve that follows in stream order
+    if (field == 0 && threadIdx.x == 0 && status) status[2 * blockIdx.x] = 0;
+    const int n = d.n, off = d.voff;
+    const double2* tw = tw_all + d.twoff;
+    const double plin = -(1 + ph.rho) * kPi * ph.U / n;   // Derivatives.cuh:286
+    for (int j = threadIdx.x; j < n; j += blockDim.x) {
+        double2 v;
+        if (field < 2) {
+            v = Z[off + j];
+            v.x -= 2 * kPi * (double)j / n;                 // Derivatives.cuh:285,317
+        } else {
+            v = make_double2(phi[off + j] - plin * (double)j, 0.0);
+        }
+        s[lds_index(d, j)] = v;
+    }
+    __syncthreads();
+    derivative_in_lds(s, d, tw, field == 1);
+    for (int j = threadIdx.x; j < n; j += blockDim.x) {
+        const double2 v = s[lds_index(d, j)];
+        if (field == 0) {
+            const double sc = 2.0 * kPi / n;
+            Zp[off + j] = make_double2(v.x * sc + 2.0 * kPi / n, v.y * sc);   // :323,374
+        } else if (field == 1) {
+            const double sc = 4.0 * kPi * kPi / (n * n);                      // :321
+            Zpp[off + j] = make_double2(v.x * sc, v.y * sc);
+        } else {
+            double r = v.x * (2.0 * kPi / n);                                  // :324, :299
+            if (ph.U != 0) r += -(1 + ph.rho) * kPi * ph.U / n;               // :380-383
+            b[off + j] = r;
+            work[off + j] = r;
+        }
+    }
+}
+
+// mu' = IFFT(first_mult(FFT(mu))) * 2 pi / n, mu real (BaseBoundaryIntegrator.cuh:201-203).
+__global__ __launch_bounds__(256) void mu_deriv_kernel(const SurfDesc* __restrict__ sd,
+                                                       const double2* __restrict__ tw_all,
+                                                       const double* __restrict__ mu,
+                                                       double2* __restrict__ mup) {
+    extern __shared__ double2 s[];
+    const SurfDesc d = sd[blockIdx.x];
+    const int n = d.n, off = d.voff;
+    const double2* tw = tw_all + d.twoff;
+    for (int j = threadIdx.x; j < n; j += blockDim.x) s[lds_index(d, j)] = make_double2(mu[off + j], 0.0);
+    __syncthreads();
+    derivative_in_lds(s, d, tw, false);
+    const double sc = 2.0 * kPi / n;
+    for (int j = threadIdx.x; j < n; j += blockDim.x) {
+        const double2 v = s[lds_index(d, j)];
+        mup[off + j] = make_double2(v.x * sc, v.y * sc);
+    }
+}
+
+// Standalone derivative of an arbitrary complex field (FftDerivative::exec with scale 1 and
+// no linear part), used by the calculateDerivativeFFT256 entry point.
+__global__ __launch_bounds__(256) void plain_deriv_kernel(const SurfDesc* __restrict__ sd,
+                                                          const double2* __restrict__ tw_all,
+                                                          const double2* __restrict__ in,
+                                                          double2* __restrict__ out, int second,
+                                                          double scale) {
+    extern __shared__ double2 s[];
+    const SurfDesc d = sd[blockIdx.x];
+    const int n = d.n, off = d.voff;
+    const double2* tw = tw_all + d.twoff;
+    for (int j = threadIdx.x; j < n; j += blockDim.x) s[lds_index(d, j)] = in[off + j];
+    __syncthreads();
+    derivative_in_lds(s, d, tw, second != 0);
+    for (int j = threadIdx.x; j < n; j += blockDim.x) {
+        const double2 v = s[lds_index(d, j)];
+        out[off + j] = (scale != 1.0) ? make_double2(v.x * scale, v.y * scale) : v;
+    }
+}
+
+}  // namespace sfh

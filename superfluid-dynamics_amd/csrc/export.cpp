@@ -1,0 +1,419 @@
+// export.cpp -- the C ABI (include/cusuperhelium.h): drop-in CuSuperHelium entry points and the
+// native sfh_* handle API, both on top of sfh::Solver.
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "../../include/cusuperhelium.h"
+#include "solver.hpp"
+
+using sfh::Solver;
+
+struct sfh_solver {
+    std::unique_ptr<Solver> s;
+};
+
+namespace {
+
+thread_local std::string g_last_error;
+
+template <typename F>
+int guarded(F&& f) {
+    try {
+        f();
+        return 0;
+    } catch (const sfh::ArgError& e) {
+        g_last_error = e.what();
+        return std::string(e.what()).rfind("unsupported node count", 0) == 0 ? -2 : -3;
+    } catch (const std::exception& e) {
+        g_last_error = e.what();
+        return -1;
+    } catch (...) {
+        g_last_error = "unknown error";
+        return -1;
+    }
+}
+
+// adimensionalizeProperties (Export.cu:1222-1246): alpha = 3.5e-24 (constants.cuh:165),
+// rho_He = 150.
+struct Units {
+    double rho, kappa, depth, base_length, base_time;
+};
+Units adimensionalize(double L, double rho, double kappa, double depth) {
+    const double alpha = 3.5e-24, rho_he = 150.0;
+    Units u;
+    u.base_length = L / (2.0 * sfh::kPi);
+    const double base_acc = 3 * alpha / std::pow(depth, 4);
+    u.base_time = std::sqrt(u.base_length / base_acc);
+    const double stf = rho_he * u.base_length * u.base_length * u.base_length / (u.base_time * u.base_time);
+    u.kappa = kappa / stf;
+    u.depth = depth / u.base_length;
+    u.rho = rho / rho_he;
+    return u;
+}
+
+// HeliumBoundaryProblem as the reference exports build it (ProblemProperties defaults:
+// infinite_depth = false, use_expansions = false, U = 0).
+sfh::Phys helium_phys(const Units& u, bool infinite_depth, bool use_exp, int exp_order) {
+    sfh::Phys ph{};
+    ph.kind = 1;
+    ph.image = infinite_depth ? 0 : 1;
+    ph.use_exp = use_exp ? 1 : 0;
+    ph.exp_order = exp_order;
+    ph.rho_quirk = 1;
+    ph.need_upper = 0;
+    ph.rho = u.rho;
+    ph.kappa = u.kappa;
+    ph.depth = u.depth;
+    ph.U = 0.0;
+    ph.g = 1.0;
+    return ph;
+}
+
+sfh::Phys phys_from(const sfh_props& p) {
+    sfh::Phys ph{};
+    ph.kind = p.kind;
+    ph.image = (p.kind == SFH_HELIUM && !p.infinite_depth) ? 1 : 0;
+    ph.use_exp = p.use_expansions;
+    ph.exp_order = p.expansion_order;
+    ph.rho_quirk = p.rho_quirk;
+    ph.need_upper = (p.kind == SFH_WATER && p.rho != 0.0) ? 1 : 0;
+    ph.rho = p.rho;
+    ph.kappa = p.kappa;
+    ph.depth = p.depth;
+    ph.U = p.U;
+    ph.g = p.g;
+    return ph;
+}
+
+// Workspaces of the drop-in entry points, kept across calls (the reference rebuilds plans,
+// handles and 11 buffers on every call, CuSuperHelium/Export.cu:194-265).
+std::mutex g_cache_mu;
+std::map<std::tuple<int, int, int, double, double, double, int, int, int>, std::unique_ptr<Solver>> g_cache;
+
+Solver& cached_solver(const sfh::Phys& ph, int n, int batch) {
+    auto key = std::make_tuple(n, batch, ph.kind, ph.rho, ph.kappa, ph.depth, ph.image, ph.use_exp, ph.exp_order);
+    auto it = g_cache.find(key);
+    if (it != g_cache.end()) return *it->second;
+    if (g_cache.size() > 16) g_cache.clear();
+    sfh::Options opt;
+    auto s = std::make_unique<Solver>(ph, std::vector<int>(batch, n), opt);
+    Solver& ref = *s;
+    g_cache.emplace(key, std::move(s));
+    return ref;
+}
+
+bool dropin_n(size_t N) {
+    static const size_t ok[] = {1, 2, 4, 8, 16, 32, 64, 128, 256, 512, 768, 1024, 1536, 2048, 4096};
+    for (size_t v : ok)
+        if (v == N) return true;
+    return false;
+}
+
+int rhs_dropin(const double* x, const double* y, const double* phi, double* vx, double* vy, double* rhsPhi,
+               double L, double rho, double kappa, double depth, size_t N, int batch) {
+    if (!x || !y || !phi || !vx || !vy || !rhsPhi) {
+        g_last_error = "null pointer argument";
+        return -3;
+    }
+    return guarded([&] {
+        const Units u = adimensionalize(L, rho, kappa, depth);
+        std::lock_guard<std::mutex> lk(g_cache_mu);
+        Solver& s = cached_solver(helium_phys(u, false, false, 1), (int)N, batch);
+        s.set_state(x, y, phi);
+        s.rhs(vx, vy, rhsPhi);
+    });
+}
+
+}  // namespace
+
+extern "C" {
+
+int calculateRHSFromVectors(const double* x, const double* y, const double* phi, double* vx, double* vy,
+                            double* rhsPhi, double L, double rho, double kappa, double depth, size_t N) {
+    if (!dropin_n(N)) {
+        g_last_error = "unsupported particle number " + std::to_string(N);
+        return -2;
+    }
+    return rhs_dropin(x, y, phi, vx, vy, rhsPhi, L, rho, kappa, depth, N, 1);
+}
+
+int calculateRHS256FromVectors(const double* x, const double* y, const double* phi, double* vx, double* vy,
+                               double* rhsPhi, double L, double rho, double kappa, double depth) {
+    return rhs_dropin(x, y, phi, vx, vy, rhsPhi, L, rho, kappa, depth, 256, 1);
+}
+
+int calculateRHS2048FromVectors(const double* x, const double* y, const double* phi, double* vx, double* vy,
+                                double* rhsPhi, double L, double rho, double kappa, double depth) {
+    return rhs_dropin(x, y, phi, vx, vy, rhsPhi, L, rho, kappa, depth, 2048, 1);
+}
+
+int calculateRHS256FromVectorsBatched(const double* x, const double* y, const double* phi, double* vx,
+                                      double* vy, double* rhsPhi, double L, double rho, double kappa,
+                                      double depth, int batchSize) {
+    static const int ok[] = {1, 2, 4, 8, 16, 32, 64, 128, 256, 512, 768, 1024, 1536, 2048};
+    bool good = false;
+    for (int v : ok) good = good || v == batchSize;
+    if (!good) {
+        g_last_error = "unsupported batch size " + std::to_string(batchSize);
+        return -2;
+    }
+    return rhs_dropin(x, y, phi, vx, vy, rhsPhi, L, rho, kappa, depth, 256, batchSize);
+}
+
+int calculateVorticities256FromVectors(const c_double* Z, const c_double* phi, double* a, c_double* Zp,
+                                       c_double* Zpp, double L, double rho, double kappa, double depth) {
+    if (!Z || !phi || !a) {
+        g_last_error = "null pointer argument";
+        return -3;
+    }
+    return guarded([&] {
+        const int N = 256;
+        std::vector<double> x(N), y(N), p(N), vx(N), vy(N), dp(N);
+        for (int i = 0; i < N; ++i) {
+            x[i] = Z[i].re;
+            y[i] = Z[i].im;
+            p[i] = phi[i].re;   // loadDataToDevice keeps the real part (SimulationRunner.cuh:244-260)
+        }
+        const Units u = adimensionalize(L, rho, kappa, depth);
+        std::lock_guard<std::mutex> lk(g_cache_mu);
+        Solver& s = cached_solver(helium_phys(u, false, false, 1), N, 1);
+        s.set_state(x.data(), y.data(), p.data());
+        std::vector<double2> zp(N), zpp(N);
+        s.rhs(vx.data(), vy.data(), dp.data(), a, zp.data(), zpp.data());
+        for (int i = 0; i < N; ++i) {
+            if (Zp) Zp[i] = c_double{zp[i].x, zp[i].y};
+            if (Zpp) Zpp[i] = c_double{zpp[i].x, zpp[i].y};
+        }
+    });
+}
+
+int calculateDerivativeFFT256(const c_double* input, c_double* output) {
+    if (!input || !output) {
+        g_last_error = "null pointer argument";
+        return -3;
+    }
+    guarded([&] {
+        const int N = 256;
+        sfh::Phys ph{};
+        ph.g = 1.0;
+        std::lock_guard<std::mutex> lk(g_cache_mu);
+        Solver& s = cached_solver(ph, N, 1);
+        std::vector<double2> in(N), out(N);
+        for (int i = 0; i < N; ++i) in[i] = make_double2(input[i].re, input[i].im);
+        s.derivative(in.data(), out.data(), false, 1.0);
+        for (int i = 0; i < N; ++i) output[i] = c_double{out[i].x, out[i].y};
+    });
+    return 0;   // the reference always returns 0 here (Export.cu:383)
+}
+
+int integrateSimulationRK4(double* initialState, double** statesOut, size_t* statesCount, double** timesOut,
+                           size_t* timesCount, SimProperties* simProperties, RK4SolverOptions* rkOptions,
+                           size_t N) {
+    if (!initialState || !statesOut || !statesCount || !timesOut || !timesCount || !simProperties ||
+        !rkOptions) {
+        g_last_error = "null pointer argument";
+        return -3;
+    }
+    if (!dropin_n(N)) {
+        g_last_error = "unsupported particle number " + std::to_string(N);
+        return -2;
+    }
+    return guarded([&] {
+        const SimProperties sp = *simProperties;
+        const Units u = adimensionalize(sp.L, sp.rho, sp.kappa, sp.depth);
+        // adimensionalizeRK4SolverOptions (Export.cu:1213-1220)
+        const double t0 = rkOptions->t0 / u.base_time;
+        const double t1 = rkOptions->t1 / u.base_time;
+        const double dt = rkOptions->timeStep / u.base_time;
+        if (!(dt > 0.0)) throw sfh::ArgError("time step must be positive");
+        const size_t steps = (size_t)((t1 - t0) / dt);   // runEvolution, AutonomousRungeKuttaStepper.cuh:421
+        sfh::Options opt;
+        Solver s(helium_phys(u, sp.infinite_depth, sp.use_expansions, sp.expansion_order),
+                 std::vector<int>(1, (int)N), opt);
+        const double* x = initialState;
+        s.set_state(x, x + N, x + 2 * N);
+        const bool traj = rkOptions->returnTrajectory;
+        const size_t nst = traj ? steps : 1;
+        double* states = (double*)std::malloc(sizeof(double) * 3 * N * (nst ? nst : 1));
+        double* times = traj ? (double*)std::malloc(sizeof(double) * (steps ? steps : 1)) : nullptr;
+        if (!states || (traj && !times)) {
+            std::free(states);
+            std::free(times);
+            throw std::runtime_error("host allocation failed");
+        }
+        double t = t0;
+        try {
+            if (traj) {
+                for (size_t k = 0; k < steps; ++k) {
+                    s.step(dt, 1);
+                    t += dt;
+                    s.get_state(states + 3 * N * k, states + 3 * N * k + N, states + 3 * N * k + 2 * N);
+                    times[k] = t;
+                }
+            } else {
+                s.step(dt, (int)steps);
+                s.get_state(states, states + N, states + 2 * N);
+            }
+        } catch (...) {
+            std::free(states);
+            std::free(times);
+            throw;
+        }
+        *statesOut = states;
+        *statesCount = nst;
+        *timesOut = times;
+        *timesCount = traj ? steps : 0;
+    });
+}
+
+int integrateSimulationRK4_freeMemory(double* statesOut, double* timesOut) {
+    std::free(statesOut);
+    std::free(timesOut);
+    return 0;
+}
+
+void sfh_default_options(sfh_options* opt) {
+    if (!opt) return;
+    std::memset(opt, 0, sizeof(*opt));
+    opt->precision = SFH_FP64;
+    opt->solver = SFH_SOLVE_LU;
+    opt->use_graph = 1;
+    opt->max_iter = 50;
+    opt->restart = 30;
+    opt->tol = 1e-12;
+}
+
+void sfh_default_props(sfh_props* p) {
+    if (!p) return;
+    std::memset(p, 0, sizeof(*p));
+    p->kind = SFH_WATER;
+    p->expansion_order = 1;
+    p->rho_quirk = 1;
+    p->depth = 1.0;
+    p->g = 1.0;
+}
+
+int sfh_create(const sfh_props* props, const int* n, int n_surfaces, const sfh_options* opt, sfh_handle* out) {
+    if (!props || !n || n_surfaces <= 0 || !out) {
+        g_last_error = "invalid argument";
+        return -3;
+    }
+    *out = nullptr;
+    return guarded([&] {
+        sfh::Options o;
+        if (opt) {
+            o.precision = opt->precision;
+            o.solver = opt->solver;
+            o.use_graph = opt->use_graph;
+            o.max_iter = opt->max_iter;
+            o.restart = opt->restart;
+            o.tol = opt->tol;
+        }
+        auto h = std::make_unique<sfh_solver>();
+        h->s = std::make_unique<Solver>(phys_from(*props), std::vector<int>(n, n + n_surfaces), o);
+        *out = h.release();
+    });
+}
+
+int sfh_destroy(sfh_handle h) {
+    return guarded([&] { delete h; });
+}
+
+int sfh_total_nodes(sfh_handle h) { return h ? h->s->total_nodes() : -3; }
+
+#define SFH_NEED(h)                              \
+    if (!(h)) {                                  \
+        g_last_error = "null handle";            \
+        return -3;                               \
+    }
+
+int sfh_set_state(sfh_handle h, const double* x, const double* y, const double* phi) {
+    SFH_NEED(h);
+    return guarded([&] { h->s->set_state(x, y, phi); });
+}
+
+int sfh_get_state(sfh_handle h, double* x, double* y, double* phi) {
+    SFH_NEED(h);
+    return guarded([&] { h->s->get_state(x, y, phi); });
+}
+
+int sfh_rhs(sfh_handle h, double* vx, double* vy, double* dphi, double* mu, double* zp, double* zpp,
+            double* qup) {
+    SFH_NEED(h);
+    return guarded([&] {
+        h->s->rhs(vx, vy, dphi, mu, reinterpret_cast<double2*>(zp), reinterpret_cast<double2*>(zpp),
+                  reinterpret_cast<double2*>(qup));
+    });
+}
+
+int sfh_step(sfh_handle h, double dt, int nsteps) {
+    SFH_NEED(h);
+    return guarded([&] { h->s->step(dt, nsteps); });
+}
+
+int sfh_get_mu(sfh_handle h, double* mu) {
+    SFH_NEED(h);
+    return guarded([&] { h->s->get_mu(mu); });
+}
+
+int sfh_sync(sfh_handle h) {
+    SFH_NEED(h);
+    return guarded([&] { h->s->sync(); });
+}
+
+int sfh_get_stats(sfh_handle h, sfh_stats* out) {
+    SFH_NEED(h);
+    if (!out) return -3;
+    return guarded([&] {
+        sfh::Stats st = h->s->stats();
+        std::memset(out, 0, sizeof(*out));
+        out->rhs_evals = st.rhs_evals;
+        out->steps = st.steps;
+        out->fallback_solves = st.fallback_solves;
+        out->spin_timeouts = st.spin_timeouts;
+        out->last_iterations = st.last_iterations;
+    });
+}
+
+int sfh_device_state(sfh_handle h, void** Z, void** phi) {
+    SFH_NEED(h);
+    if (Z) *Z = h->s->device_Z();
+    if (phi) *phi = h->s->device_phi();
+    return 0;
+}
+
+const char* sfh_last_error(void) { return g_last_error.c_str(); }
+
+int sfh_kernel_timing(sfh_handle h, int enable) {
+    SFH_NEED(h);
+    return guarded([&] { h->s->set_kernel_timing(enable != 0); });
+}
+
+int sfh_kernel_report(sfh_handle h, const char* tag, int* launches, double* ms, double* flops, double* bytes) {
+    SFH_NEED(h);
+    if (!tag) return -3;
+    return guarded([&] {
+        Solver::KernelReport r = h->s->kernel_report(tag);
+        if (launches) *launches = r.launches;
+        if (ms) *ms = r.ms;
+        if (flops) *flops = r.flops;
+        if (bytes) *bytes = r.bytes;
+    });
+}
+
+int sfh_debug_lu_solve(int n, const double* A, const double* b, double* x, int* interchanged) {
+    if (!A || !b || !x || n < 1) {
+        g_last_error = "invalid argument";
+        return -3;
+    }
+    return guarded([&] { sfh::debug_lu_solve(n, A, b, x, interchanged); });
+}
+
+}  // extern "C"

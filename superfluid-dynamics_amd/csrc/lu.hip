@@ -1,0 +1,371 @@
+// lu.hip -- dense FP64 solve M mu = b for the dipole strengths (one large surface per launch
+// grid, uniform batches on gridDim.z).
+//
+// Replaces MatrixSolver::solve (CuSuperHelium/MatrixSolver.cuh:113-172): cusolverDnDgetrf +
+// cusolverDnDgetrs (LU with partial pivoting), or getrfBatched/getrsBatched with a host sync.
+//
+// Algorithm.  Right-looking blocked LU with 64-wide panels.  For panel p (columns k0..k0+kb):
+//   lu_panel_kernel : every workgroup factors the kb x kb diagonal block A11 = L11 U11 in LDS
+//                     and inverts both factors; then workgroup 0 stores U11^{-1} and
+//                     y_p = L11^{-1} b_p, each "row" workgroup forms L21_i = A21_i U11^{-1} and
+//                     b_i -= L21_i y_p, each "column" workgroup forms U12_j = L11^{-1} A12_j.
+//   lu_update_kernel: A22 -= L21 U12 with v_mfma_f64_16x16x4 on 64 x 64 tiles.
+// lu_backsolve_kernel then runs x_p = U_pp^{-1} (y_p - sum_{q>p} U_pq x_q), one workgroup per
+// block row, chained by per-block ready flags.
+//
+// Row interchanges.  Partial pivoting (getrf: first row of maximal |a| in the current column)
+// leaves the rows in place exactly when every multiplier satisfies |l_ij| <= 1.  Both the
+// diagonal-block elimination and the L21 blocks check that condition; if it ever fails,
+// status[0] is set and gepp_fallback_kernel re-solves the freshly re-assembled system with
+// explicit interchanges.  The result is therefore the GEPP solution in every case, with the
+// interchange-free path taken for the diagonally dominated second-kind systems of this method.
+#include "sfh_kernels.hpp"
+
+namespace sfh {
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+constexpr int LD = kNB + 1;   // LDS leading dimension of 64 x 64 FP64 blocks
+
+__device__ inline v4d mfma_f64(double a, double b, v4d c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+// acc[i][j] (+)= A * B for the 32 x 32 quarter (wr, wc) of a 64 x 64 x 64 product, A and B in
+// LDS with leading dimension LD.  v_mfma_f64_16x16x4 lane layout: A[m = l%16][k = l/16],
+// B[k = l/16][n = l%16], D[m = 4 r + l/16][n = l%16] (register r of lane l).
+__device__ inline void mfma_block(const double* A, const double* B, v4d acc[2][2], int wr, int wc,
+                                  int lane, double sign) {
+    const int l16 = lane & 15, l4 = lane >> 4;
+#pragma unroll 4
+    for (int kk = 0; kk < kNB; kk += 4) {
+        double a0 = sign * A[(wr * 32 + l16) * LD + kk + l4];
+        double a1 = sign * A[(wr * 32 + 16 + l16) * LD + kk + l4];
+        double b0 = B[(kk + l4) * LD + wc * 32 + l16];
+        double b1 = B[(kk + l4) * LD + wc * 32 + 16 + l16];
+        acc[0][0] = mfma_f64(a0, b0, acc[0][0]);
+        acc[0][1] = mfma_f64(a0, b1, acc[0][1]);
+        acc[1][0] = mfma_f64(a1, b0, acc[1][0]);
+        acc[1][1] = mfma_f64(a1, b1, acc[1][1]);
+    }
+}
+
+// Stores the wave's accumulators into an LDS block.
+__device__ inline void acc_to_lds(double* C, const v4d acc[2][2], int wr, int wc, int lane) {
+    const int l16 = lane & 15, l4 = lane >> 4;
+    for (int i = 0; i < 2; ++i)
+        for (int j = 0; j < 2; ++j)
+            for (int r = 0; r < 4; ++r)
+                C[(wr * 32 + i * 16 + 4 * r + l4) * LD + wc * 32 + j * 16 + l16] = acc[i][j][r];
+}
+
+// Loads a rows x cols block of row-major global memory (leading dimension ld) into LDS,
+// zero-padding to 64 x 64.
+__device__ inline void load_block(double* dst, const double* src, long long ld, int rows, int cols) {
+    for (int e = threadIdx.x; e < kNB * kNB; e += blockDim.x) {
+        const int r = e >> 6, c = e & 63;
+        dst[r * LD + c] = (r < rows && c < cols) ? src[(long long)r * ld + c] : 0.0;
+    }
+}
+
+__device__ inline void store_block(double* dst, const double* src, long long ld, int rows, int cols) {
+    for (int e = threadIdx.x; e < kNB * kNB; e += blockDim.x) {
+        const int r = e >> 6, c = e & 63;
+        if (r < rows && c < cols) dst[(long long)r * ld + c] = src[r * LD + c];
+    }
+}
+
+// Factors D (kb x kb, no interchanges) into L\U and writes L^{-1} into Li and U^{-1} into Ui
+// (both zero outside the kb x kb corner).  Sets *bad if a multiplier exceeds 1 in magnitude.
+__device__ void factor_invert_diag(double* D, double* Li, double* Ui, int kb, int* bad) {
+    const int t = threadIdx.x;
+    for (int j = 0; j < kb; ++j) {
+        const double inv = 1.0 / D[j * LD + j];
+        const int i = j + 1 + (t >> 2), part = t & 3;
+        if (i < kb) {
+            const double l = D[i * LD + j] * inv;
+            for (int c = j + 1 + part; c < kb; c += 4) D[i * LD + c] -= l * D[j * LD + c];
+            if (part == 0) {
+                D[i * LD + j] = l;
+                if (!(fabs(l) <= 1.0)) *bad = 1;
+            }
+        }
+        __syncthreads();
+    }
+    for (int e = t; e < kNB * kNB; e += blockDim.x) {
+        const int r = e >> 6, c = e & 63;
+        Li[r * LD + c] = (r == c && r < kb) ? 1.0 : 0.0;
+        Ui[r * LD + c] = 0.0;
+    }
+    __syncthreads();
+    // row s of L^{-1} (top down) and row kb-1-s of U^{-1} (bottom up), 4 threads per column
+    for (int s = 0; s < kb; ++s) {
+        const int c = t >> 2, part = t & 3;
+        // L^{-1}[s][c] = -sum_{q=c}^{s-1} L[s][q] Li[q][c], c < s
+        double accl = 0.0;
+        if (c < s)
+            for (int q = c + part; q < s; q += 4) accl += D[s * LD + q] * Li[q * LD + c];
+        // U^{-1}[i][c] = -(sum_{q=i+1}^{c} U[i][q] Ui[q][c]) / U[i][i], c > i;  Ui[i][i] = 1/U[i][i]
+        const int i = kb - 1 - s;
+        double accu = 0.0;
+        if (c > i && c < kb)
+            for (int q = i + 1 + part; q <= c; q += 4) accu += D[i * LD + q] * Ui[q * LD + c];
+        accl += __shfl_xor(accl, 1, 64);
+        accl += __shfl_xor(accl, 2, 64);
+        accu += __shfl_xor(accu, 1, 64);
+        accu += __shfl_xor(accu, 2, 64);
+        if (part == 0) {
+            if (c < s) Li[s * LD + c] = -accl;
+            if (c == i) Ui[i * LD + c] = 1.0 / D[i * LD + i];
+            else if (c > i && c < kb) Ui[i * LD + c] = -accu / D[i * LD + i];
+        }
+        __syncthreads();
+    }
+}
+
+// Per-surface views for gridDim.z batches of equal size: surface s uses A + s n^2, vectors + s n,
+// uinv + s P 64^2, flags + s P, status + 2 s.
+struct LuView {
+    double *A, *work, *y, *x, *uinv;
+    int *flags, *status;
+};
+__device__ inline LuView view(const LuArgs& a) {
+    const int s = blockIdx.z, n = a.n, P = (n + kNB - 1) / kNB;
+    LuView v;
+    v.A = a.A + (long long)s * n * n;
+    v.work = a.work + (long long)s * n;
+    v.y = a.y + (long long)s * n;
+    v.x = a.x + (long long)s * n;
+    v.uinv = a.uinv + (long long)s * P * kNB * kNB;
+    v.flags = a.flags + (long long)s * P;
+    v.status = a.status + 2 * s;
+    return v;
+}
+
+__global__ __launch_bounds__(256) void lu_panel_kernel(LuArgs a, int k0) {
+    __shared__ double D[kNB * LD], Li[kNB * LD], Ui[kNB * LD], X[kNB * LD];
+    __shared__ double yv[kNB];
+    __shared__ int bad;
+    const LuView v = view(a);
+    const int n = a.n, t = threadIdx.x;
+    const int kb = min(kNB, n - k0);
+    const int rem = n - k0 - kb;
+    const int nrb = (rem + kNB - 1) / kNB;
+    const int role = blockIdx.x;
+    const int lane = t & 63, w = t >> 6, wr = w >> 1, wc = w & 1;
+    if (t == 0) bad = 0;
+    if (k0 == 0 && role == 0) {
+        const int P = (n + kNB - 1) / kNB;
+        for (int i = t; i < P; i += blockDim.x) v.flags[i] = 0;
+    }
+    load_block(D, v.A + (long long)k0 * n + k0, n, kb, kb);
+    __syncthreads();
+    factor_invert_diag(D, Li, Ui, kb, &bad);
+    // y_p = L11^{-1} b_p
+    if (t < kNB) {
+        double s = 0.0;
+        if (t < kb)
+            for (int c = 0; c <= t; ++c) s += Li[t * LD + c] * v.work[k0 + c];
+        yv[t] = s;
+    }
+    __syncthreads();
+    if (role == 0) {
+        double* U = v.uinv + (long long)(k0 / kNB) * kNB * kNB;
+        for (int e = t; e < kNB * kNB; e += blockDim.x) U[e] = Ui[(e >> 6) * LD + (e & 63)];
+        if (t < kb) v.y[k0 + t] = yv[t];
+        if (t == 0 && bad) atomicOr(&v.status[0], 1);
+        return;
+    }
+    v4d acc[2][2];
+    for (int i = 0; i < 2; ++i)
+        for (int j = 0; j < 2; ++j) acc[i][j] = v4d{0.0, 0.0, 0.0, 0.0};
+    if (role <= nrb) {
+        const int R0 = k0 + kb + (role - 1) * kNB;
+        const int rb = min(kNB, n - R0);
+        load_block(X, v.A + (long long)R0 * n + k0, n, rb, kb);
+        __syncthreads();
+        mfma_block(X, Ui, acc, wr, wc, lane, 1.0);        // L21 = A21 U11^{-1}
+        __syncthreads();
+        acc_to_lds(X, acc, wr, wc, lane);
+        __syncthreads();
+        int my_bad = 0;
+        for (int e = t; e < kNB * kNB; e += blockDim.x) {
+            const int r = e >> 6, c = e & 63;
+            if (r < rb && c < kb && !(fabs(X[r * LD + c]) <= 1.0)) my_bad = 1;
+        }
+        if (my_bad) atomicOr(&v.status[0], 1);
+        store_block(v.A + (long long)R0 * n + k0, X, n, rb, kb);
+        if (t < rb) {                                     // b_i -= L21_i y_p
+            double s = 0.0;
+            for (int c = 0; c < kb; ++c) s += X[t * LD + c] * yv[c];
+            v.work[R0 + t] -= s;
+        }
+    } else {
+        const int C0 = k0 + kb + (role - 1 - nrb) * kNB;
+        const int cb = min(kNB, n - C0);
+        load_block(X, v.A + (long long)k0 * n + C0, n, kb, cb);
+        __syncthreads();
+        mfma_block(Li, X, acc, wr, wc, lane, 1.0);        // U12 = L11^{-1} A12
+        __syncthreads();
+        acc_to_lds(X, acc, wr, wc, lane);
+        __syncthreads();
+        store_block(v.A + (long long)k0 * n + C0, X, n, kb, cb);
+    }
+    if (t == 0 && bad && role == 1) atomicOr(&v.status[0], 1);
+}
+
+// A22 -= L21 U12 on the 64 x 64 tile (blockIdx.y, blockIdx.x) of the trailing matrix.
+__global__ __launch_bounds__(256) void lu_update_kernel(LuArgs a, int k0) {
+    __shared__ double As[kNB * LD], Bs[kNB * LD];
+    const LuView v = view(a);
+    const int n = a.n, t = threadIdx.x;
+    const int kb = min(kNB, n - k0);
+    const int R0 = k0 + kb + blockIdx.y * kNB, C0 = k0 + kb + blockIdx.x * kNB;
+    const int rb = min(kNB, n - R0), cb = min(kNB, n - C0);
+    const int lane = t & 63, w = t >> 6, wr = w >> 1, wc = w & 1;
+    const int l16 = lane & 15, l4 = lane >> 4;
+    load_block(As, v.A + (long long)R0 * n + k0, n, rb, kb);
+    load_block(Bs, v.A + (long long)k0 * n + C0, n, kb, cb);
+    double* C = v.A + (long long)R0 * n + C0;
+    v4d acc[2][2];
+    for (int i = 0; i < 2; ++i)
+        for (int j = 0; j < 2; ++j)
+            for (int r = 0; r < 4; ++r) {
+                const int row = wr * 32 + i * 16 + 4 * r + l4, col = wc * 32 + j * 16 + l16;
+                acc[i][j][r] = (row < rb && col < cb) ? C[(long long)row * n + col] : 0.0;
+            }
+    __syncthreads();
+    mfma_block(As, Bs, acc, wr, wc, lane, -1.0);
+    for (int i = 0; i < 2; ++i)
+        for (int j = 0; j < 2; ++j)
+            for (int r = 0; r < 4; ++r) {
+                const int row = wr * 32 + i * 16 + 4 * r + l4, col = wc * 32 + j * 16 + l16;
+                if (row < rb && col < cb) C[(long long)row * n + col] = acc[i][j][r];
+            }
+}
+
+// x_p = U_pp^{-1} (y_p - sum_{q>p} U_pq x_q).  Workgroup b handles block row p = P-1-b, so a
+// workgroup only waits for workgroups dispatched before it; every wait is bounded.
+__global__ __launch_bounds__(256) void lu_backsolve_kernel(LuArgs a) {
+    __shared__ double part[4][kNB];
+    __shared__ double sv[kNB];
+    const LuView v = view(a);
+    const int n = a.n, t = threadIdx.x;
+    const int P = (n + kNB - 1) / kNB;
+    const int p = P - 1 - blockIdx.x;
+    const int R0 = p * kNB, rb = min(kNB, n - R0);
+    const int r = t & 63, chunk = t >> 6;
+    double acc = 0.0;
+    for (int q = P - 1; q > p; --q) {
+        if (t == 0) {
+            int spins = 0;
+            while (__hip_atomic_load(&v.flags[q], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == 0) {
+                __builtin_amdgcn_s_sleep(2);
+                if (++spins > (1 << 22)) {
+                    atomicOr(&v.status[1], 1);
+                    break;
+                }
+            }
+        }
+        __syncthreads();
+        (void)__hip_atomic_load(&v.flags[q], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+        const int C0 = q * kNB, cb = min(kNB, n - C0);
+        if (r < rb) {
+            const double* Ur = v.A + (long long)(R0 + r) * n + C0;
+            for (int c = chunk * 16; c < min(cb, chunk * 16 + 16); ++c) acc += Ur[c] * v.x[C0 + c];
+        }
+    }
+    part[chunk][r] = acc;
+    __syncthreads();
+    if (t < kNB) sv[t] = (t < rb) ? v.y[R0 + t] - (part[0][t] + part[1][t] + part[2][t] + part[3][t]) : 0.0;
+    __syncthreads();
+    const double* U = v.uinv + (long long)p * kNB * kNB;
+    if (t < rb) {
+        double s = 0.0;
+        for (int c = t; c < rb; ++c) s += U[t * kNB + c] * sv[c];
+        v.x[R0 + t] = s;
+    }
+    __syncthreads();
+    if (t == 0) __hip_atomic_store(&v.flags[p], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Unblocked GEPP (first row of maximal |a|) and solve by one workgroup per surface, run only
+// for surfaces whose interchange-free factorisation reported a multiplier above 1
+// (status[2 s] != 0).  M holds the freshly re-assembled matrix of that surface.
+__global__ __launch_bounds__(256) void gepp_fallback_kernel(const SurfDesc* sd, double* M, const double* bvec,
+                                                            double* xvec, int* pivv, int* status,
+                                                            int* counters) {
+    const int s_ = blockIdx.x;
+    if (status[2 * s_] == 0) return;
+    const SurfDesc dsc = sd[s_];
+    const int n = dsc.n;
+    double* A = M + dsc.moff;
+    const double* b = bvec + dsc.voff;
+    double* x = xvec + dsc.voff;
+    int* piv = pivv + dsc.voff;
+    if (threadIdx.x == 0) atomicAdd(&counters[0], 1);
+    __shared__ double rv[256];
+    __shared__ int ri[256];
+    const int t = threadIdx.x;
+    for (int i = t; i < n; i += blockDim.x) x[i] = b[i];
+    __syncthreads();
+    for (int j = 0; j < n; ++j) {
+        double best = -1.0;
+        int bi = n;
+        for (int i = j + t; i < n; i += blockDim.x) {
+            const double vv = fabs(A[(long long)i * n + j]);
+            if (vv > best) { best = vv; bi = i; }
+        }
+        rv[t] = best;
+        ri[t] = bi;
+        __syncthreads();
+        for (int s = 128; s > 0; s >>= 1) {
+            if (t < s) {
+                if (rv[t + s] > rv[t] || (rv[t + s] == rv[t] && ri[t + s] < ri[t])) {
+                    rv[t] = rv[t + s];
+                    ri[t] = ri[t + s];
+                }
+            }
+            __syncthreads();
+        }
+        const int pr = ri[0];
+        if (t == 0) piv[j] = pr;
+        if (pr != j && pr < n) {
+            for (int c = t; c < n; c += blockDim.x) {
+                const double tmp = A[(long long)j * n + c];
+                A[(long long)j * n + c] = A[(long long)pr * n + c];
+                A[(long long)pr * n + c] = tmp;
+            }
+            if (t == 0) {
+                const double tmp = x[j];
+                x[j] = x[pr];
+                x[pr] = tmp;
+            }
+        }
+        __syncthreads();
+        const double d = A[(long long)j * n + j];
+        if (d == 0.0) {   // exactly singular column: getrf records it and moves on
+            __syncthreads();
+            continue;
+        }
+        const double xj = x[j];
+        for (int i = j + 1 + t; i < n; i += blockDim.x) {
+            double* Ri = A + (long long)i * n;
+            const double l = Ri[j] / d;
+            Ri[j] = l;
+            const double* Rj = A + (long long)j * n;
+            for (int c = j + 1; c < n; ++c) Ri[c] -= l * Rj[c];
+            x[i] -= l * xj;
+        }
+        __syncthreads();
+    }
+    for (int i = n - 1; i >= 0; --i) {
+        if (t == 0) x[i] = x[i] / A[(long long)i * n + i];
+        __syncthreads();
+        const double xi = x[i];
+        for (int r = t; r < i; r += blockDim.x) x[r] -= A[(long long)r * n + i] * xi;
+        __syncthreads();
+    }
+}
+
+}  // namespace sfh

@@ -1,0 +1,103 @@
+// sfh_common.hpp -- shared device/host definitions of the gfx950 Roberts boundary-integral stepper.
+//
+// Layout in HBM (one "ensemble" = S independent periodic surfaces of N_s nodes each):
+//   vectors  : surfaces concatenated, node i of surface s at voff[s] + i
+//              Z, Z', Z'', mu', q  : double2 (re, im)      16 B/node
+//              phi, b (= Re Phi'), mu, dphi/dt : double    8 B/node
+//   matrices : surface s at moff[s], row-major N_s x N_s (row k = collocation node, column j =
+//              source node), i.e. element (k, j) of the reference's system (createM.cuh:52).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace sfh {
+
+constexpr double kPi = 3.14159265358979323846264338327950288;
+
+// Per-surface descriptor, uploaded once per ensemble.
+struct SurfDesc {
+    int n;          // nodes
+    int log2m;      // n = r * 2^log2m with r in {1, 3}
+    int r3;         // 1 if n = 3 * 2^log2m
+    int voff;       // vector offset (nodes)
+    long long moff; // matrix offset (elements)
+    int twoff;      // offset of this n's twiddle table (complex entries, n of them)
+    int pad;
+};
+
+// Physics selector passed by value to kernels.
+//   kind 0: water, createMKernel / createVelocityMatrices / compute_rhs_phi_expression
+//   kind 1: helium film, createFiniteDepthMKernel / createHeliumVelocityMatrices /
+//           compute_rhs_helium_phi_expression (+ kappa and expansion variants)
+struct Phys {
+    int kind;
+    int image;          // helium with finite depth: image terms on
+    int use_exp;
+    int exp_order;
+    int rho_quirk;      // water Bernoulli reads V1[1] (createM.cuh:104)
+    int need_upper;     // water with rho != 0 needs the upper-fluid velocity
+    double rho, kappa, depth, U, g;
+};
+
+// ------------------------------------------------------------------ complex helpers
+__host__ __device__ inline double2 cmul(double2 a, double2 b) {
+    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__host__ __device__ inline double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__host__ __device__ inline double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+__host__ __device__ inline double2 cscale(double2 a, double s) { return make_double2(a.x * s, a.y * s); }
+__host__ __device__ inline double2 cconj(double2 a) { return make_double2(a.x, -a.y); }
+__host__ __device__ inline double2 ctimes_i(double2 a) { return make_double2(-a.y, a.x); }
+__device__ inline double2 cdiv(double2 a, double2 b) {
+    double d = 1.0 / (b.x * b.x + b.y * b.y);
+    return make_double2((a.x * b.x + a.y * b.y) * d, (a.y * b.x - a.x * b.y) * d);
+}
+__device__ inline double2 crecip(double2 b) {
+    double d = 1.0 / (b.x * b.x + b.y * b.y);
+    return make_double2(b.x * d, -b.y * d);
+}
+
+// cot(a + i b) = (sin a cos a - i sinh b cosh b) / (sin^2 a + sinh^2 b)  (from DLMF 4.21.40 with
+// cosh 2b - cos 2a = 2 (sin^2 a + sinh^2 b), which removes the cancellation near the poles).
+// sinh/cosh come from one expm1 so that small |b| keeps full relative accuracy.
+__device__ inline double2 cot64(double a, double b) {
+    double sa, ca;
+    sincos(a, &sa, &ca);
+    double ab = fabs(b);
+    if (ab < 20.0) {
+        double E = expm1(ab);
+        double ep = E + 1.0;
+        double iep = 1.0 / ep;
+        double sh = 0.5 * E * (E + 2.0) * iep;   // sinh |b|
+        double ch = 0.5 * (ep + iep);            // cosh b
+        double id = 1.0 / (sa * sa + sh * sh);
+        return make_double2(sa * ca * id, -copysign(sh * ch, b) * id);
+    }
+    double t = exp(-2.0 * ab);
+    return make_double2(4.0 * sa * ca * t, -copysign(1.0 + 2.0 * t, b));
+}
+
+__device__ inline float2 cot32(float a, float b) {
+    float sa, ca;
+    sincosf(a, &sa, &ca);
+    float ab = fabsf(b);
+    if (ab < 10.0f) {
+        float E = expm1f(ab);
+        float ep = E + 1.0f;
+        float iep = 1.0f / ep;
+        float sh = 0.5f * E * (E + 2.0f) * iep;
+        float ch = 0.5f * (ep + iep);
+        float id = 1.0f / (sa * sa + sh * sh);
+        return make_float2(sa * ca * id, -copysignf(sh * ch, b) * id);
+    }
+    float t = expf(-2.0f * ab);
+    return make_float2(4.0f * sa * ca * t, -copysignf(1.0f + 2.0f * t, b));
+}
+
+// ------------------------------------------------------------------ wave reductions
+__device__ inline double wave_sum(double v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+}  // namespace sfh

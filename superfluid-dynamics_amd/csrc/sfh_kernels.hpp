@@ -1,0 +1,61 @@
+// sfh_kernels.hpp -- declarations of the gfx950 kernels of the Roberts stepper.
+#pragma once
+#include "sfh_common.hpp"
+
+namespace sfh {
+
+constexpr int kNB = 64;          // LU panel / tile width
+
+// Stage control for the fused velocity + Bernoulli + RK4 kernel.
+//   stage 0    : plain RHS evaluation, writes (q, dphi) into kZ[0], kP[0]
+//   stage 1..3 : writes k_s into kZ[s-1], kP[s-1] and y_s = y0 + alpha k_s into (Zout, Pout)
+//   stage 4    : y0 += alpha * (((k1 + 2 k2) + 2 k3) + k4) in place (alpha = dt / 6)
+struct StageCtl {
+    int stage;
+    int pad;
+    double alpha;
+    const double2* Z0;
+    const double* P0;
+    double2* Zout;
+    double* Pout;
+    double2* kZ[3];
+    double* kP[3];
+    double2* qup;      // optional upper-fluid velocity output (may be null)
+};
+
+// ---- deriv.hip
+__global__ void zphi_deriv_kernel(const SurfDesc* sd, const double2* tw_all, const double2* Z,
+                                  const double* phi, double2* Zp, double2* Zpp, double* b,
+                                  double* work, Phys ph, int* status);
+__global__ void mu_deriv_kernel(const SurfDesc* sd, const double2* tw_all, const double* mu,
+                                double2* mup);
+__global__ void plain_deriv_kernel(const SurfDesc* sd, const double2* tw_all, const double2* in,
+                                   double2* out, int second, double scale);
+
+// ---- assemble.hip
+__global__ void assemble64_kernel(const SurfDesc* sd, const double2* Z, const double2* Zp,
+                                  const double2* Zpp, double* M, Phys ph, const int* only_if);
+
+// ---- velocity.hip
+__global__ void velocity_stage_kernel(const SurfDesc* sd, const double2* Z, const double* phi,
+                                      const double2* Zp, const double2* Zpp, const double* mu,
+                                      const double2* mup, Phys ph, StageCtl ctl);
+
+// ---- lu.hip
+struct LuArgs {
+    double* A;        // n x n row-major, overwritten by L21 / U12 blocks (diagonal blocks kept)
+    double* work;     // right-hand side, updated in place (b_i -= L21_i y_1)
+    double* y;        // forward-substituted right-hand side L^{-1} P b
+    double* x;        // solution
+    double* uinv;     // per panel: U11^{-1} (64 x 64)
+    int* flags;       // per panel ready flags for the back substitution
+    int* status;      // [0] pivot check failed (needs interchanges), [1] spin timeout
+    int n;
+};
+__global__ void lu_panel_kernel(LuArgs a, int k0);
+__global__ void lu_update_kernel(LuArgs a, int k0);
+__global__ void lu_backsolve_kernel(LuArgs a);
+__global__ void gepp_fallback_kernel(const SurfDesc* sd, double* M, const double* b, double* x,
+                                     int* piv, int* status, int* counters);
+
+}  // namespace sfh

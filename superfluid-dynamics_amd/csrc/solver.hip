@@ -1,0 +1,461 @@
+// solver.hip -- host orchestration of the gfx950 Roberts RK4 stepper.
+//
+// One RHS evaluation (BaseBoundaryIntegralCalculator::runTimeStep,
+// CuSuperHelium/BaseBoundaryIntegrator.cuh:137-306) is enqueued as
+//   zphi_deriv_kernel     Z', Z'', Re Phi'            (S x 3 workgroups, LDS FFT)
+//   assemble64_kernel     M                           (one wave per row)
+//   LU solve              mu                          (panel / MFMA update / back substitution)
+//   GEPP fallback         no-op unless a system needed row interchanges
+//   mu_deriv_kernel       mu'
+//   velocity_stage_kernel q, dPhi/dt and the RK4 stage update of every node
+// and one RK4 step (AutonomousRungeKuttaStepper.cuh:123-361) is four of them.  The step is
+// captured once into a hipGraph and replayed, so a run of many steps costs one graph launch
+// per step and the state never leaves HBM.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "solver.hpp"
+
+namespace sfh {
+
+bool supported_n(int n) {
+    if (n < 1 || n > 8192) return false;
+    int m = n;
+    if (m % 3 == 0) m /= 3;
+    return (m & (m - 1)) == 0;
+}
+
+static SurfDesc make_desc(int n) {
+    SurfDesc d{};
+    d.n = n;
+    int m = n;
+    if (m % 3 == 0) {
+        d.r3 = 1;
+        m /= 3;
+    }
+    int l = 0;
+    while ((1 << l) < m) ++l;
+    d.log2m = l;
+    return d;
+}
+
+template <typename T>
+static T* dalloc(size_t count) {
+    T* p = nullptr;
+    if (count == 0) count = 1;
+    SFH_HIP(hipMalloc(&p, count * sizeof(T)));
+    return p;
+}
+
+Solver::Solver(const Phys& ph, const std::vector<int>& n, const Options& opt) : ph_(ph), opt_(opt), n_(n) {
+    if (n.empty()) throw ArgError("no surfaces");
+    S_ = (int)n.size();
+    for (int v : n)
+        if (!supported_n(v)) throw ArgError("unsupported node count " + std::to_string(v));
+    if (opt_.precision != kFP64 || opt_.solver != kSolveLU)
+        throw ArgError("requested precision / solver form is not available");
+    SFH_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+
+    // descriptors and twiddle tables (one table per distinct n, computed in long double)
+    std::vector<double2> tw;
+    std::vector<std::pair<int, int>> tw_of;
+    long long moff = 0;
+    int voff = 0;
+    for (int s = 0; s < S_; ++s) {
+        SurfDesc d = make_desc(n[s]);
+        d.voff = voff;
+        d.moff = moff;
+        int off = -1;
+        for (auto& p : tw_of)
+            if (p.first == n[s]) off = p.second;
+        if (off < 0) {
+            off = (int)tw.size();
+            tw_of.push_back({n[s], off});
+            for (int t = 0; t < n[s]; ++t) {
+                const long double ang =
+                    2.0L * 3.141592653589793238462643383279502884L * (long double)t / (long double)n[s];
+                tw.push_back(make_double2((double)cosl(ang), (double)(-sinl(ang))));
+            }
+        }
+        d.twoff = off;
+        desc_.push_back(d);
+        voff += n[s];
+        moff += (long long)n[s] * n[s];
+        maxN_ = std::max(maxN_, n[s]);
+        if (n[s] != n[0]) uniform_ = false;
+    }
+    T_ = voff;
+    MT_ = moff;
+
+    d_desc_ = dalloc<SurfDesc>(S_);
+    SFH_HIP(hipMemcpy(d_desc_, desc_.data(), S_ * sizeof(SurfDesc), hipMemcpyHostToDevice));
+    d_tw_ = dalloc<double2>(tw.size());
+    SFH_HIP(hipMemcpy(d_tw_, tw.data(), tw.size() * sizeof(double2), hipMemcpyHostToDevice));
+
+    Z0_ = dalloc<double2>(T_);
+    Za_ = dalloc<double2>(T_);
+    Zb_ = dalloc<double2>(T_);
+    P0_ = dalloc<double>(T_);
+    Pa_ = dalloc<double>(T_);
+    Pb_ = dalloc<double>(T_);
+    for (int i = 0; i < 3; ++i) {
+        kZ_[i] = dalloc<double2>(T_);
+        kP_[i] = dalloc<double>(T_);
+    }
+    Zp_ = dalloc<double2>(T_);
+    Zpp_ = dalloc<double2>(T_);
+    mup_ = dalloc<double2>(T_);
+    qup_ = dalloc<double2>(T_);
+    b_ = dalloc<double>(T_);
+    work_ = dalloc<double>(T_);
+    y_ = dalloc<double>(T_);
+    mu_ = dalloc<double>(T_);
+    M_ = dalloc<double>(MT_);
+    long long utot = 0, ftot = 0;
+    for (int s = 0; s < S_; ++s) {
+        const int P = (n[s] + kNB - 1) / kNB;
+        uoff_.push_back(utot);
+        foff_.push_back(ftot);
+        utot += (long long)P * kNB * kNB;
+        ftot += P;
+    }
+    uinv_ = dalloc<double>(utot);
+    flags_ = dalloc<int>(ftot);
+    status_ = dalloc<int>(2 * S_);
+    piv_ = dalloc<int>(T_);
+    counters_ = dalloc<int>(4);
+    SFH_HIP(hipMemset(status_, 0, 2 * S_ * sizeof(int)));
+    SFH_HIP(hipMemset(counters_, 0, 4 * sizeof(int)));
+    SFH_HIP(hipMemset(Z0_, 0, T_ * sizeof(double2)));
+    SFH_HIP(hipMemset(P0_, 0, T_ * sizeof(double)));
+    SFH_HIP(hipMemset(mu_, 0, T_ * sizeof(double)));
+    SFH_HIP(hipMemset(mup_, 0, T_ * sizeof(double2)));
+    SFH_HIP(hipDeviceSynchronize());
+}
+
+Solver::~Solver() {
+    if (exec_) (void)hipGraphExecDestroy(exec_);
+    if (graph_) (void)hipGraphDestroy(graph_);
+    if (stream_) (void)hipStreamSynchronize(stream_);
+    void* bufs[] = {d_desc_, d_tw_, Z0_, Za_, Zb_, P0_, Pa_, Pb_, kZ_[0], kZ_[1], kZ_[2], kP_[0],
+                    kP_[1], kP_[2], Zp_, Zpp_, mup_, qup_, b_, work_, y_, mu_, M_, uinv_, flags_,
+                    status_, piv_, counters_};
+    for (void* p : bufs)
+        if (p) (void)hipFree(p);
+    for (hipEvent_t e : event_pool_) (void)hipEventDestroy(e);
+    if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+template <class F>
+void Solver::launch(const char* tag, double flops, double bytes, F&& f) {
+    if (!ktime_) {
+        f();
+        return;
+    }
+    while (event_pool_.size() < event_next_ + 2) {
+        hipEvent_t e;
+        SFH_HIP(hipEventCreate(&e));
+        event_pool_.push_back(e);
+    }
+    hipEvent_t a = event_pool_[event_next_++], b = event_pool_[event_next_++];
+    SFH_HIP(hipEventRecord(a, stream_));
+    f();
+    SFH_HIP(hipEventRecord(b, stream_));
+    timed_.push_back({tag, a, b, flops, bytes});
+}
+
+void Solver::set_kernel_timing(bool on) {
+    SFH_HIP(hipStreamSynchronize(stream_));
+    ktime_ = on;
+    timed_.clear();
+    event_next_ = 0;
+}
+
+Solver::KernelReport Solver::kernel_report(const std::string& tag) {
+    SFH_HIP(hipStreamSynchronize(stream_));
+    KernelReport r;
+    for (const Timed& t : timed_) {
+        if (tag != t.tag) continue;
+        float ms = 0.f;
+        SFH_HIP(hipEventElapsedTime(&ms, t.a, t.b));
+        r.launches += 1;
+        r.ms += ms;
+        r.flops += t.flops;
+        r.bytes += t.bytes;
+    }
+    return r;
+}
+
+// M mu = b for every surface; M, b (and its working copy) are ready in stream order.
+void Solver::enqueue_solve(const double2* Z) {
+    auto run = [&](int s0, int count, int n) {
+        LuArgs a;
+        a.A = M_ + desc_[s0].moff;
+        a.work = work_ + desc_[s0].voff;
+        a.y = y_ + desc_[s0].voff;
+        a.x = mu_ + desc_[s0].voff;
+        a.uinv = uinv_ + uoff_[s0];
+        a.flags = flags_ + foff_[s0];
+        a.status = status_ + 2 * s0;
+        a.n = n;
+        for (int k0 = 0; k0 < n; k0 += kNB) {
+            const int kb = std::min(kNB, n - k0);
+            const int rem = n - k0 - kb;
+            const int nrb = (rem + kNB - 1) / kNB;
+            // panel: 2 kb^3 (two 64^3-class products per row/column block) + the diagonal block
+            const double pf = (double)count * (4.0 * kb * kb * rem + 2.0 * kb * kb * kb);
+            launch("lu_panel", pf, 8.0 * count * (2.0 * kb * rem + kb * kb), [&] {
+                hipLaunchKernelGGL(lu_panel_kernel, dim3(1 + 2 * nrb, 1, count), dim3(256), 0, stream_, a, k0);
+            });
+            if (rem > 0)
+                launch("lu_update", (double)count * 2.0 * rem * rem * kb,
+                       8.0 * count * (2.0 * rem * rem + 2.0 * rem * kb), [&] {
+                           hipLaunchKernelGGL(lu_update_kernel, dim3(nrb, nrb, count), dim3(256), 0, stream_, a,
+                                              k0);
+                       });
+        }
+        const int P = (n + kNB - 1) / kNB;
+        launch("lu_backsolve", (double)count * n * n, 8.0 * count * 0.5 * n * n, [&] {
+            hipLaunchKernelGGL(lu_backsolve_kernel, dim3(P, 1, count), dim3(256), 0, stream_, a);
+        });
+    };
+    if (uniform_) {
+        run(0, S_, n_[0]);
+    } else {
+        for (int s = 0; s < S_; ++s) run(s, 1, n_[s]);
+    }
+    // systems whose interchange-free check failed: re-assemble and solve with GEPP
+    hipLaunchKernelGGL(assemble64_kernel, dim3((maxN_ + 3) / 4, S_), dim3(256), 0, stream_, d_desc_, Z, Zp_,
+                       Zpp_, M_, ph_, (const int*)status_);
+    hipLaunchKernelGGL(gepp_fallback_kernel, dim3(S_), dim3(256), 0, stream_, d_desc_, M_, b_, mu_, piv_,
+                       status_, counters_);
+}
+
+void Solver::enqueue_rhs(const double2* Z, const double* P, const StageCtl& ctl) {
+    const size_t lds = (size_t)maxN_ * sizeof(double2);
+    double nsq = 0.0, nlog = 0.0;
+    for (int v : n_) {
+        nsq += (double)v * v;
+        nlog += 5.0 * v * std::log2((double)std::max(v, 2));
+    }
+    const double img = ph_.image ? 2.0 : 1.0;
+    launch("deriv", 3 * 2 * nlog, 64.0 * T_, [&] {
+        hipLaunchKernelGGL(zphi_deriv_kernel, dim3(S_, 3), dim3(256), lds, stream_, d_desc_, d_tw_, Z, P, Zp_,
+                           Zpp_, b_, work_, ph_, status_);
+    });
+    // assembly: flops counted as cot evaluations (the transcendental-bound unit), bytes = M written
+    launch("assemble", img * nsq, 8.0 * nsq, [&] {
+        hipLaunchKernelGGL(assemble64_kernel, dim3((maxN_ + 3) / 4, S_), dim3(256), 0, stream_, d_desc_, Z,
+                           Zp_, Zpp_, M_, ph_, (const int*)nullptr);
+    });
+    enqueue_solve(Z);
+    launch("mu_deriv", 2 * nlog, 24.0 * T_, [&] {
+        hipLaunchKernelGGL(mu_deriv_kernel, dim3(S_), dim3(256), lds, stream_, d_desc_, d_tw_, mu_, mup_);
+    });
+    launch("velocity", img * nsq, 96.0 * T_, [&] {
+        hipLaunchKernelGGL(velocity_stage_kernel, dim3((maxN_ + 3) / 4, S_), dim3(256), 0, stream_, d_desc_, Z,
+                           P, Zp_, Zpp_, mu_, mup_, ph_, ctl);
+    });
+    SFH_HIP(hipGetLastError());
+}
+
+void Solver::enqueue_step(double dt) {
+    const double half = dt * 0.5, sixth = dt / 6.0;   // setTimeStep, AutonomousRungeKuttaStepper.cuh:31-36
+    StageCtl c{};
+    c.Z0 = Z0_;
+    c.P0 = P0_;
+    for (int i = 0; i < 3; ++i) {
+        c.kZ[i] = kZ_[i];
+        c.kP[i] = kP_[i];
+    }
+    c.qup = nullptr;
+    c.stage = 1; c.alpha = half; c.Zout = Za_; c.Pout = Pa_;
+    enqueue_rhs(Z0_, P0_, c);
+    c.stage = 2; c.alpha = half; c.Zout = Zb_; c.Pout = Pb_;
+    enqueue_rhs(Za_, Pa_, c);
+    c.stage = 3; c.alpha = dt; c.Zout = Za_; c.Pout = Pa_;
+    enqueue_rhs(Zb_, Pb_, c);
+    c.stage = 4; c.alpha = sixth; c.Zout = nullptr; c.Pout = nullptr;
+    enqueue_rhs(Za_, Pa_, c);
+}
+
+void Solver::capture(double dt) {
+    if (exec_) {
+        SFH_HIP(hipGraphExecDestroy(exec_));
+        exec_ = nullptr;
+    }
+    if (graph_) {
+        SFH_HIP(hipGraphDestroy(graph_));
+        graph_ = nullptr;
+    }
+    SFH_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+    try {
+        enqueue_step(dt);
+    } catch (...) {
+        hipGraph_t g = nullptr;
+        (void)hipStreamEndCapture(stream_, &g);
+        if (g) (void)hipGraphDestroy(g);
+        throw;
+    }
+    SFH_HIP(hipStreamEndCapture(stream_, &graph_));
+    SFH_HIP(hipGraphInstantiate(&exec_, graph_, nullptr, nullptr, 0));
+    graph_dt_ = dt;
+}
+
+void Solver::step(double dt, int nsteps) {
+    if (nsteps < 0) throw ArgError("negative step count");
+    if (opt_.use_graph && !ktime_) {
+        if (!exec_ || graph_dt_ != dt) capture(dt);
+        for (int i = 0; i < nsteps; ++i) SFH_HIP(hipGraphLaunch(exec_, stream_));
+    } else {
+        for (int i = 0; i < nsteps; ++i) enqueue_step(dt);
+    }
+    stats_.steps += nsteps;
+    stats_.rhs_evals += 4LL * nsteps;
+}
+
+void Solver::set_state(const double* x, const double* y, const double* phi) {
+    std::vector<double2> z(T_);
+    for (int i = 0; i < T_; ++i) z[i] = make_double2(x[i], y[i]);
+    SFH_HIP(hipMemcpyAsync(Z0_, z.data(), T_ * sizeof(double2), hipMemcpyHostToDevice, stream_));
+    SFH_HIP(hipMemcpyAsync(P0_, phi, T_ * sizeof(double), hipMemcpyHostToDevice, stream_));
+    SFH_HIP(hipStreamSynchronize(stream_));
+}
+
+void Solver::get_state(double* x, double* y, double* phi) {
+    std::vector<double2> z(T_);
+    SFH_HIP(hipMemcpyAsync(z.data(), Z0_, T_ * sizeof(double2), hipMemcpyDeviceToHost, stream_));
+    SFH_HIP(hipMemcpyAsync(phi, P0_, T_ * sizeof(double), hipMemcpyDeviceToHost, stream_));
+    SFH_HIP(hipStreamSynchronize(stream_));
+    for (int i = 0; i < T_; ++i) {
+        x[i] = z[i].x;
+        y[i] = z[i].y;
+    }
+    check_status();
+}
+
+void Solver::rhs(double* vx, double* vy, double* dphi, double* mu, double2* zp, double2* zpp, double2* qup) {
+    StageCtl c{};
+    c.stage = 0;
+    c.kZ[0] = kZ_[0];
+    c.kP[0] = kP_[0];
+    c.qup = qup ? qup_ : nullptr;
+    enqueue_rhs(Z0_, P0_, c);
+    std::vector<double2> q(T_);
+    SFH_HIP(hipMemcpyAsync(q.data(), kZ_[0], T_ * sizeof(double2), hipMemcpyDeviceToHost, stream_));
+    SFH_HIP(hipMemcpyAsync(dphi, kP_[0], T_ * sizeof(double), hipMemcpyDeviceToHost, stream_));
+    if (mu) SFH_HIP(hipMemcpyAsync(mu, mu_, T_ * sizeof(double), hipMemcpyDeviceToHost, stream_));
+    if (zp) SFH_HIP(hipMemcpyAsync(zp, Zp_, T_ * sizeof(double2), hipMemcpyDeviceToHost, stream_));
+    if (zpp) SFH_HIP(hipMemcpyAsync(zpp, Zpp_, T_ * sizeof(double2), hipMemcpyDeviceToHost, stream_));
+    if (qup) SFH_HIP(hipMemcpyAsync(qup, qup_, T_ * sizeof(double2), hipMemcpyDeviceToHost, stream_));
+    SFH_HIP(hipStreamSynchronize(stream_));
+    for (int i = 0; i < T_; ++i) {
+        vx[i] = q[i].x;
+        vy[i] = q[i].y;
+    }
+    stats_.rhs_evals += 1;
+    check_status();
+}
+
+void Solver::get_mu(double* mu) {
+    SFH_HIP(hipMemcpyAsync(mu, mu_, T_ * sizeof(double), hipMemcpyDeviceToHost, stream_));
+    SFH_HIP(hipStreamSynchronize(stream_));
+}
+
+void Solver::sync() {
+    SFH_HIP(hipStreamSynchronize(stream_));
+    check_status();
+}
+
+void Solver::check_status() {
+    std::vector<int> st(2 * S_);
+    int cnt[4];
+    SFH_HIP(hipMemcpy(st.data(), status_, 2 * S_ * sizeof(int), hipMemcpyDeviceToHost));
+    SFH_HIP(hipMemcpy(cnt, counters_, sizeof(cnt), hipMemcpyDeviceToHost));
+    stats_.fallback_solves = cnt[0];
+    int timeouts = 0;
+    for (int s = 0; s < S_; ++s) timeouts += st[2 * s + 1] != 0;
+    stats_.spin_timeouts = timeouts;
+    if (timeouts) throw HipError("back-substitution wait timed out; results are invalid");
+}
+
+Stats Solver::stats() {
+    sync();
+    return stats_;
+}
+
+void Solver::derivative(const double2* in_host, double2* out_host, bool second, double scale) {
+    SFH_HIP(hipMemcpyAsync(Za_, in_host, T_ * sizeof(double2), hipMemcpyHostToDevice, stream_));
+    const size_t lds = (size_t)maxN_ * sizeof(double2);
+    hipLaunchKernelGGL(plain_deriv_kernel, dim3(S_), dim3(256), lds, stream_, d_desc_, d_tw_, Za_, Zb_,
+                       second ? 1 : 0, scale);
+    SFH_HIP(hipGetLastError());
+    SFH_HIP(hipMemcpyAsync(out_host, Zb_, T_ * sizeof(double2), hipMemcpyDeviceToHost, stream_));
+    SFH_HIP(hipStreamSynchronize(stream_));
+}
+
+}  // namespace sfh
+
+namespace sfh {
+
+// Solves A x = b (n x n, row-major, host arrays) with the device LU path used by the stepper:
+// interchange-free blocked LU, and the GEPP kernel when a multiplier exceeds 1.  Returns 1 in
+// *interchanged when the GEPP kernel ran.  Used to test the solver on arbitrary matrices.
+void debug_lu_solve(int n, const double* A, const double* b, double* x, int* interchanged) {
+    if (n < 1) throw ArgError("n must be positive");
+    hipStream_t st;
+    SFH_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    const int P = (n + kNB - 1) / kNB;
+    double *dA, *dA0, *dwork, *dy, *dx, *duinv, *db;
+    int *dflags, *dstatus, *dpiv, *dcnt;
+    SurfDesc desc{};
+    desc.n = n;
+    SurfDesc* ddesc;
+    SFH_HIP(hipMalloc(&dA, sizeof(double) * n * n));
+    SFH_HIP(hipMalloc(&dA0, sizeof(double) * n * n));
+    SFH_HIP(hipMalloc(&dwork, sizeof(double) * n));
+    SFH_HIP(hipMalloc(&db, sizeof(double) * n));
+    SFH_HIP(hipMalloc(&dy, sizeof(double) * n));
+    SFH_HIP(hipMalloc(&dx, sizeof(double) * n));
+    SFH_HIP(hipMalloc(&duinv, sizeof(double) * P * kNB * kNB));
+    SFH_HIP(hipMalloc(&dflags, sizeof(int) * P));
+    SFH_HIP(hipMalloc(&dstatus, sizeof(int) * 2));
+    SFH_HIP(hipMalloc(&dpiv, sizeof(int) * n));
+    SFH_HIP(hipMalloc(&dcnt, sizeof(int) * 4));
+    SFH_HIP(hipMalloc(&ddesc, sizeof(SurfDesc)));
+    SFH_HIP(hipMemcpy(ddesc, &desc, sizeof(desc), hipMemcpyHostToDevice));
+    SFH_HIP(hipMemcpy(dA, A, sizeof(double) * n * n, hipMemcpyHostToDevice));
+    SFH_HIP(hipMemcpy(dA0, A, sizeof(double) * n * n, hipMemcpyHostToDevice));
+    SFH_HIP(hipMemcpy(dwork, b, sizeof(double) * n, hipMemcpyHostToDevice));
+    SFH_HIP(hipMemcpy(db, b, sizeof(double) * n, hipMemcpyHostToDevice));
+    SFH_HIP(hipMemset(dstatus, 0, sizeof(int) * 2));
+    SFH_HIP(hipMemset(dcnt, 0, sizeof(int) * 4));
+    LuArgs a{dA, dwork, dy, dx, duinv, dflags, dstatus, n};
+    for (int k0 = 0; k0 < n; k0 += kNB) {
+        const int kb = std::min(kNB, n - k0);
+        const int rem = n - k0 - kb;
+        const int nrb = (rem + kNB - 1) / kNB;
+        hipLaunchKernelGGL(lu_panel_kernel, dim3(1 + 2 * nrb, 1, 1), dim3(256), 0, st, a, k0);
+        if (rem > 0) hipLaunchKernelGGL(lu_update_kernel, dim3(nrb, nrb, 1), dim3(256), 0, st, a, k0);
+    }
+    hipLaunchKernelGGL(lu_backsolve_kernel, dim3(P, 1, 1), dim3(256), 0, st, a);
+    SFH_HIP(hipGetLastError());
+    int status[2];
+    SFH_HIP(hipMemcpyAsync(status, dstatus, sizeof(status), hipMemcpyDeviceToHost, st));
+    SFH_HIP(hipStreamSynchronize(st));
+    if (status[0]) {
+        SFH_HIP(hipMemcpyAsync(dA, dA0, sizeof(double) * n * n, hipMemcpyDeviceToDevice, st));
+        hipLaunchKernelGGL(gepp_fallback_kernel, dim3(1), dim3(256), 0, st, ddesc, dA, db, dx, dpiv, dstatus, dcnt);
+        SFH_HIP(hipGetLastError());
+    }
+    SFH_HIP(hipMemcpyAsync(x, dx, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+    SFH_HIP(hipStreamSynchronize(st));
+    if (interchanged) *interchanged = status[0] ? 1 : 0;
+    void* bufs[] = {dA, dA0, dwork, db, dy, dx, duinv, dflags, dstatus, dpiv, dcnt, ddesc};
+    for (void* p : bufs) (void)hipFree(p);
+    (void)hipStreamDestroy(st);
+    if (status[1]) throw HipError("back-substitution wait timed out");
+}
+
+}  // namespace sfh

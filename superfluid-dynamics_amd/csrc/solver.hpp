@@ -1,0 +1,144 @@
+// solver.hpp -- host-side ensemble solver: device buffers, RHS pipeline and RK4 driver.
+//
+// Mirrors the reference object graph BaseBoundaryIntegralCalculator<N,B> + BoundaryProblem +
+// AutonomousRungeKuttaStepper (CuSuperHelium/BaseBoundaryIntegrator.cuh:10-85,
+// AutonomousRungeKuttaStepper.cuh:24-89) with run-time sizes instead of template parameters:
+// one Solver owns S surfaces of arbitrary sizes, allocates everything once, and advances the
+// state in HBM without host round trips.
+#pragma once
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "sfh_kernels.hpp"
+
+namespace sfh {
+
+#define SFH_HIP(expr)                                                                          \
+    do {                                                                                       \
+        hipError_t e_ = (expr);                                                                \
+        if (e_ != hipSuccess)                                                                  \
+            throw ::sfh::HipError(std::string(#expr) + ": " + hipGetErrorString(e_));          \
+    } while (0)
+
+struct HipError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+struct ArgError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+enum Precision { kFP64 = 0, kMixed = 1 };
+enum SolveKind { kSolveLU = 0, kSolveGMRES = 1 };
+
+struct Options {
+    int precision = kFP64;
+    int solver = kSolveLU;
+    int use_graph = 1;
+    int max_iter = 50;       // GMRES iterations / refinement sweeps
+    int restart = 30;        // GMRES restart length
+    double tol = 1e-12;      // GMRES relative residual / refinement tolerance
+};
+
+struct Stats {
+    long long rhs_evals = 0;
+    long long steps = 0;
+    int fallback_solves = 0;   // interchange-needing systems re-solved by GEPP
+    int spin_timeouts = 0;
+    int last_iterations = 0;   // iterative solvers: iterations of the last solve
+};
+
+bool supported_n(int n);
+void debug_lu_solve(int n, const double* A, const double* b, double* x, int* interchanged);
+
+class Solver {
+  public:
+    Solver(const Phys& ph, const std::vector<int>& n, const Options& opt);
+    ~Solver();
+    Solver(const Solver&) = delete;
+    Solver& operator=(const Solver&) = delete;
+
+    int surfaces() const { return (int)n_.size(); }
+    int total_nodes() const { return T_; }
+    const std::vector<int>& sizes() const { return n_; }
+    hipStream_t stream() const { return stream_; }
+
+    void set_state(const double* x, const double* y, const double* phi);
+    void get_state(double* x, double* y, double* phi);
+    // Right-hand side of the current state: vx + i vy = dZ/dt, dphi = dPhi/dt (host arrays).
+    void rhs(double* vx, double* vy, double* dphi, double* mu = nullptr, double2* zp = nullptr,
+             double2* zpp = nullptr, double2* qup = nullptr);
+    void step(double dt, int nsteps);
+    void get_mu(double* mu);
+    void sync();
+    Stats stats();
+
+    // Raw device buffers (state y0) for zero-copy interop.
+    double2* device_Z() { return Z0_; }
+    double* device_phi() { return P0_; }
+
+    // Standalone derivative of a complex field (FftDerivative::exec semantics).
+    void derivative(const double2* in_host, double2* out_host, bool second, double scale);
+
+    // Kernel timing: when enabled, steps run without graph replay and every launch is bracketed
+    // by hipEvents on the solver's stream; report() sums them per kernel tag.
+    void set_kernel_timing(bool on);
+    struct KernelReport {
+        int launches = 0;
+        double ms = 0.0;
+        double flops = 0.0;
+        double bytes = 0.0;
+    };
+    KernelReport kernel_report(const std::string& tag);
+
+  private:
+    void enqueue_rhs(const double2* Z, const double* P, const StageCtl& ctl);
+    void enqueue_solve(const double2* Z);
+    void enqueue_step(double dt);
+    void capture(double dt);
+    void check_status();
+    template <class F>
+    void launch(const char* tag, double flops, double bytes, F&& f);
+
+    struct Timed {
+        const char* tag;
+        hipEvent_t a, b;
+        double flops, bytes;
+    };
+    bool ktime_ = false;
+    std::vector<Timed> timed_;
+    std::vector<hipEvent_t> event_pool_;
+    size_t event_next_ = 0;
+
+    Phys ph_;
+    Options opt_;
+    std::vector<int> n_;
+    std::vector<SurfDesc> desc_;
+    int S_ = 0, T_ = 0, maxN_ = 0;
+    long long MT_ = 0;
+    bool uniform_ = true;
+    hipStream_t stream_ = nullptr;
+
+    SurfDesc* d_desc_ = nullptr;
+    double2* d_tw_ = nullptr;
+    double2 *Z0_ = nullptr, *Za_ = nullptr, *Zb_ = nullptr;
+    double *P0_ = nullptr, *Pa_ = nullptr, *Pb_ = nullptr;
+    double2* kZ_[3] = {nullptr, nullptr, nullptr};
+    double* kP_[3] = {nullptr, nullptr, nullptr};
+    double2 *Zp_ = nullptr, *Zpp_ = nullptr, *mup_ = nullptr, *qup_ = nullptr;
+    double *b_ = nullptr, *work_ = nullptr, *y_ = nullptr, *mu_ = nullptr;
+    double* M_ = nullptr;
+    double* uinv_ = nullptr;
+    int* flags_ = nullptr;
+    int* status_ = nullptr;
+    int* piv_ = nullptr;
+    int* counters_ = nullptr;   // [0] fallback solves
+    std::vector<long long> uoff_, foff_;
+
+    hipGraph_t graph_ = nullptr;
+    hipGraphExec_t exec_ = nullptr;
+    double graph_dt_ = 0.0;
+    Stats stats_;
+};
+
+}  // namespace sfh

@@ -1,0 +1,152 @@
+// velocity.hip -- surface velocity, Bernoulli right-hand side and RK4 stage update, fused.
+//
+// Replaces createVelocityMatrices / createHeliumVelocityMatrices (CuSuperHelium/
+// WaterVelocities.cuh:38-107), which materialise the N x N complex V1 (64 MB at N = 2048,
+// written twice per RHS), the Zgemv pair of VelocityCalculator::calculateVelocities (:205-242),
+// conjugate_vector, compute_rhs_phi_expression / compute_rhs_helium_phi_expression
+// (createM.cuh:96-117,171-211) and the Zaxpy / add_k_vectors stages of
+// AutonomousRungeKuttaStepper::runStep (AutonomousRungeKuttaStepper.cuh:123-361).
+//
+// One wave per node k.  The lanes sweep j != k accumulating
+//   S_k = sum_j mu_j cot((z_k - z_j)/2)      Si_k = sum_j mu_j cot((z_k - conj z_j)/2 + i h)
+// and lane 0 finishes
+//   w_k = V2_k mu'_k + V1_kk mu_k + i (-1/4pi) S_k [+ i (1/4pi) Si_k]
+//   V2_k = i / (2 pi Z'_k),  V1_kk = i (-1/4pi) Z''_k / Z'_k^2 [+ i (1/4pi) cot(i (y_k + h))] +- 1/(2 Z'_k)
+//   q_k = conj(w_k),  dphi_k = Bernoulli(q_k, y_k),  then the RK4 stage of node k.
+#include "sfh_kernels.hpp"
+
+namespace sfh {
+
+__device__ inline double bernoulli(const Phys& ph, double y, double2 q1, double2 q2, double2 q1_quirk,
+                                   double2 zp, double2 zpp) {
+    if (ph.kind == 0) {
+        const double a1 = q1.x * q1.x + q1.y * q1.y;
+        const double a2 = q2.x * q2.x + q2.y * q2.y;
+        const double dot = q1_quirk.x * q2.x + q1.y * q2.y;    // createM.cuh:104 reads V1[1].real()
+        return -(1 + ph.rho) * ph.g * y + 0.5 * a1 + 0.5 * ph.rho * a2 - ph.rho * dot;
+    }
+    const double h = ph.depth;
+    if (ph.use_exp) {
+        const double kin = 0.5 * q1.x * q1.x + 0.5 * q1.y * q1.y;
+        double v = 0.0;
+        switch (ph.exp_order) {
+        case 3: v += -10.0 / 3.0 * (y * y * y) / (h * h); [[fallthrough]];
+        case 2: v += 2.0 * (y * y) / h; [[fallthrough]];
+        case 1: v += -y; [[fallthrough]];
+        default: break;
+        }
+        return ph.g * v + kin;
+    }
+    if (ph.kappa != 0.0) {
+        const double a1 = q1.x * q1.x + q1.y * q1.y;
+        const double s = zp.x * zp.x + zp.y * zp.y;
+        const double curv = (zp.x * zpp.y - zp.y * zpp.x) / (s * sqrt(s));
+        const double u = 1.0 + y / h;
+        return ph.g * (20.447761896665416 * h / 3.0 * (1.0 / (u * u * u) - 1)) + 0.5 * a1 + ph.kappa * curv;
+    }
+    const double vdw = ph.g * (h / 3.0);
+    const double u = 1.0 + y / h;
+    return vdw / (u * u * u) - vdw + 0.5 * q1.x * q1.x + 0.5 * q1.y * q1.y;
+}
+
+// Lower (q1) and upper (q2) fluid velocity at node k of one surface; every lane of the wave
+// takes part and every lane returns the result.
+__device__ inline void node_velocity(const SurfDesc& d, const double2* __restrict__ z,
+                                     const double* __restrict__ m, const double2* __restrict__ Zp,
+                                     const double2* __restrict__ Zpp, const double2* __restrict__ mup,
+                                     const Phys& ph, int k, int lane, double2& q1, double2& q2) {
+    const int n = d.n;
+    const double2 zk = z[k];
+    const double h = ph.depth;
+    double sx = 0.0, sy = 0.0, ix = 0.0, iy = 0.0;
+    for (int j = lane; j < n; j += 64) {
+        if (j == k) continue;
+        const double2 zj = z[j];
+        const double mj = m[j];
+        const double a = 0.5 * (zk.x - zj.x);
+        const double2 ct = cot64(a, 0.5 * (zk.y - zj.y));
+        sx += mj * ct.x;
+        sy += mj * ct.y;
+        if (ph.image) {
+            const double2 ci = cot64(a, 0.5 * (zk.y + zj.y) + h);
+            ix += mj * ci.x;
+            iy += mj * ci.y;
+        }
+    }
+    sx = wave_sum(sx);
+    sy = wave_sum(sy);
+    if (ph.image) {
+        ix = wave_sum(ix);
+        iy = wave_sum(iy);
+    }
+    const int g = d.voff + k;
+    const double2 zp = Zp[g], zpp = Zpp[g];
+    const double mk = m[k];
+    const double q4 = 1.0 / (4.0 * kPi);
+    // off-diagonal sum: i (-1/4pi) S [+ i (1/4pi) Si]
+    double2 w = ctimes_i(make_double2(-q4 * sx, -q4 * sy));
+    if (ph.image) w = cadd(w, ctimes_i(make_double2(q4 * ix, q4 * iy)));
+    // diagonal V1_kk without the +-1/(2 Z') part
+    double2 v1 = ctimes_i(cscale(cdiv(zpp, cmul(zp, zp)), -q4));
+    if (ph.image) v1 = cadd(v1, ctimes_i(cscale(cot64(0.0, zk.y + h), q4)));
+    const double2 half_inv = cscale(crecip(zp), 0.5);
+    const double2 v2 = ctimes_i(crecip(cscale(zp, 2.0 * kPi)));
+    const double2 base = cadd(cadd(cmul(v2, mup[g]), w), cscale(v1, mk));
+    q1 = cconj(cadd(base, cscale(half_inv, mk)));
+    q2 = cconj(csub(base, cscale(half_inv, mk)));
+}
+
+__global__ __launch_bounds__(256) void velocity_stage_kernel(const SurfDesc* __restrict__ sd,
+                                                             const double2* __restrict__ Z,
+                                                             const double* __restrict__ phi,
+                                                             const double2* __restrict__ Zp,
+                                                             const double2* __restrict__ Zpp,
+                                                             const double* __restrict__ mu,
+                                                             const double2* __restrict__ mup,
+                                                             Phys ph, StageCtl ctl) {
+    const SurfDesc d = sd[blockIdx.y];
+    const int n = d.n;
+    const int k = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (k >= n) return;
+    const double2* z = Z + d.voff;
+    const double* m = mu + d.voff;
+    double2 q1, q2;
+    node_velocity(d, z, m, Zp, Zpp, mup, ph, k, lane, q1, q2);
+    // createM.cuh:104 reads V1[1].real() (lower velocity of node 1) in the rho * q1.q2 term;
+    // with rho != 0 node 1 is recomputed here (each surface uses its own node 1).
+    double2 q1q = q1;
+    if (ph.kind == 0 && ph.rho_quirk && ph.rho != 0.0 && n > 1 && k != 1) {
+        double2 u1, u2;
+        node_velocity(d, z, m, Zp, Zpp, mup, ph, 1, lane, u1, u2);
+        q1q = u1;
+    }
+    if (lane != 0) return;
+    const int g = d.voff + k;
+    if (ctl.qup) ctl.qup[g] = q2;
+    if (!ph.need_upper) q2 = make_double2(0.0, 0.0);
+    const double dphi = bernoulli(ph, z[k].y, q1, q2, q1q, Zp[g], Zpp[g]);
+
+    if (ctl.stage == 0) {
+        ctl.kZ[0][g] = q1;
+        ctl.kP[0][g] = dphi;
+    } else if (ctl.stage <= 3) {
+        ctl.kZ[ctl.stage - 1][g] = q1;
+        ctl.kP[ctl.stage - 1][g] = dphi;
+        const double2 z0 = ctl.Z0[g];
+        ctl.Zout[g] = make_double2(z0.x + ctl.alpha * q1.x, z0.y + ctl.alpha * q1.y);
+        ctl.Pout[g] = ctl.P0[g] + ctl.alpha * dphi;
+    } else {
+        const double2 k1 = ctl.kZ[0][g], k2 = ctl.kZ[1][g], k3 = ctl.kZ[2][g];
+        const double Kx = k1.x + 2.0 * k2.x + 2.0 * k3.x + q1.x;
+        const double Ky = k1.y + 2.0 * k2.y + 2.0 * k3.y + q1.y;
+        const double Kp = ctl.kP[0][g] + 2.0 * ctl.kP[1][g] + 2.0 * ctl.kP[2][g] + dphi;
+        double2* z0 = const_cast<double2*>(ctl.Z0) + g;
+        double* p0 = const_cast<double*>(ctl.P0) + g;
+        const double2 zz = *z0;
+        *z0 = make_double2(zz.x + ctl.alpha * Kx, zz.y + ctl.alpha * Ky);
+        *p0 = *p0 + ctl.alpha * Kp;
+    }
+}
+
+}  // namespace sfh

@@ -1,0 +1,148 @@
+"""Python front end of the native handle API (``sfh_*`` in include/cusuperhelium.h).
+
+``Ensemble`` holds S independent periodic surfaces (any mix of node counts) resident in HBM and
+advances them with the fused gfx950 RK4 stepper.  It plays the role of the reference's
+``BaseBoundaryIntegralCalculator`` + ``AutonomousRungeKuttaStepper`` pair
+(CuSuperHelium/BaseBoundaryIntegrator.cuh:10-85, AutonomousRungeKuttaStepper.cuh:24-89):
+``rhs()`` is ``run(state, rhs)`` and ``step(dt, n)`` is n calls of ``runStep``.
+"""
+from __future__ import annotations
+
+from ctypes import POINTER, byref, c_double, c_int, c_void_p
+from dataclasses import dataclass
+from typing import Sequence
+
+import numpy as np
+
+from ._lib import SfhOptions, SfhProps, SfhStats, check, lib
+
+WATER, HELIUM = 0, 1
+FP64, MIXED = 0, 1
+LU, GMRES = 0, 1
+
+
+def _dp(a):
+    return None if a is None else a.ctypes.data_as(POINTER(c_double))
+
+
+@dataclass
+class Props:
+    """Nondimensional problem (ProblemProperties.hpp:5-32) plus kind and gravity scale g."""
+
+    kind: int = WATER
+    rho: float = 0.0
+    kappa: float = 0.0
+    depth: float = 1.0
+    U: float = 0.0
+    g: float = 1.0
+    infinite_depth: bool = False
+    use_expansions: bool = False
+    expansion_order: int = 1
+    rho_quirk: bool = True
+
+    def c(self) -> SfhProps:
+        return SfhProps(int(self.kind), int(self.infinite_depth), int(self.use_expansions),
+                        int(self.expansion_order), int(self.rho_quirk), 0, float(self.rho),
+                        float(self.kappa), float(self.depth), float(self.U), float(self.g))
+
+
+class Ensemble:
+    def __init__(self, sizes: Sequence[int] | int, props: Props | None = None, precision: int = FP64,
+                 solver: int = LU, use_graph: bool = True, tol: float = 1e-12, max_iter: int = 50,
+                 restart: int = 30):
+        self.sizes = [int(sizes)] if np.isscalar(sizes) else [int(n) for n in sizes]
+        self.props = props or Props()
+        self._lib = lib()
+        opt = SfhOptions()
+        self._lib.sfh_default_options(byref(opt))
+        opt.precision, opt.solver, opt.use_graph = int(precision), int(solver), int(use_graph)
+        opt.tol, opt.max_iter, opt.restart = float(tol), int(max_iter), int(restart)
+        arr = (c_int * len(self.sizes))(*self.sizes)
+        h = c_void_p()
+        cp = self.props.c()
+        check(self._lib.sfh_create(byref(cp), arr, len(self.sizes), byref(opt), byref(h)), "sfh_create")
+        self._h = h
+        self.total = int(sum(self.sizes))
+        self.offsets = np.concatenate([[0], np.cumsum(self.sizes)]).astype(np.int64)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.sfh_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # ---- state
+    def set_state(self, Z, phi) -> None:
+        Z = np.asarray(Z, dtype=np.complex128).ravel()
+        x = np.ascontiguousarray(Z.real)
+        y = np.ascontiguousarray(Z.imag)
+        p = np.ascontiguousarray(phi, dtype=np.float64).ravel()
+        if x.size != self.total or p.size != self.total:
+            raise ValueError(f"state must have {self.total} nodes")
+        check(self._lib.sfh_set_state(self._h, _dp(x), _dp(y), _dp(p)), "sfh_set_state")
+
+    def get_state(self):
+        x, y, p = (np.empty(self.total) for _ in range(3))
+        check(self._lib.sfh_get_state(self._h, _dp(x), _dp(y), _dp(p)), "sfh_get_state")
+        return x + 1j * y, p
+
+    def split(self, v):
+        return [v[self.offsets[i]:self.offsets[i + 1]] for i in range(len(self.sizes))]
+
+    # ---- evaluation
+    def rhs(self, diagnostics: bool = False):
+        """(q = dZ/dt, dPhi/dt[, diag]) of the current state."""
+        vx, vy, dp = (np.empty(self.total) for _ in range(3))
+        mu = np.empty(self.total) if diagnostics else None
+        zp = np.empty(2 * self.total) if diagnostics else None
+        zpp = np.empty(2 * self.total) if diagnostics else None
+        qup = np.empty(2 * self.total) if diagnostics else None
+        check(self._lib.sfh_rhs(self._h, _dp(vx), _dp(vy), _dp(dp), _dp(mu), _dp(zp), _dp(zpp), _dp(qup)),
+              "sfh_rhs")
+        q = vx + 1j * vy
+        if diagnostics:
+            return q, dp, {"mu": mu, "Zp": zp.view(np.complex128), "Zpp": zpp.view(np.complex128),
+                           "qup": qup.view(np.complex128)}
+        return q, dp
+
+    def step(self, dt: float, nsteps: int = 1) -> None:
+        check(self._lib.sfh_step(self._h, float(dt), int(nsteps)), "sfh_step")
+
+    def mu(self) -> np.ndarray:
+        out = np.empty(self.total)
+        check(self._lib.sfh_get_mu(self._h, _dp(out)), "sfh_get_mu")
+        return out
+
+    def sync(self) -> None:
+        check(self._lib.sfh_sync(self._h), "sfh_sync")
+
+    def stats(self) -> dict:
+        st = SfhStats()
+        check(self._lib.sfh_get_stats(self._h, byref(st)), "sfh_get_stats")
+        return {k: getattr(st, k) for k, _ in SfhStats._fields_ if k != "reserved"}
+
+    def kernel_timing(self, enable: bool) -> None:
+        check(self._lib.sfh_kernel_timing(self._h, int(bool(enable))), "sfh_kernel_timing")
+
+    def kernel_report(self, tag: str) -> dict:
+        n = c_int()
+        ms, fl, by = c_double(), c_double(), c_double()
+        check(self._lib.sfh_kernel_report(self._h, tag.encode(), byref(n), byref(ms), byref(fl), byref(by)),
+              "sfh_kernel_report")
+        return {"launches": n.value, "ms": ms.value, "flops": fl.value, "bytes": by.value}
+
+    def device_state(self):
+        z, p = c_void_p(), c_void_p()
+        check(self._lib.sfh_device_state(self._h, byref(z), byref(p)), "sfh_device_state")
+        return z.value, p.value
