@@ -25,6 +25,7 @@ for p in (ROOT, os.path.join(ROOT, "superfluid-dynamics_amd")):
 
 METRIC = "RK4 steps/s (N=2048 FP64; batched mixed-prec) at ≤1e-9 err vs FP64 oracle"
 PEAK_FP64_TFLOPS = 78.6
+PEAK_HBM_GBS = 8000.0
 N = 2048
 DT = 0.002
 SEED = 20261202
@@ -59,6 +60,8 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--solver", choices=("lu", "gmres"), default="lu")
+    ap.add_argument("--max-iter", type=int, default=16, help="GMRES iteration cap per solve")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the stepper runs one "
@@ -69,7 +72,8 @@ def main():
 
     Z0 = surface(N, SEED)
     P0 = np.zeros(N)
-    e = c.Ensemble([N], c.Props(), precision=c.FP64, solver=c.LU)
+    solver = c.GMRES if args.solver == "gmres" else c.LU
+    e = c.Ensemble([N], c.Props(), precision=c.FP64, solver=solver, tol=1e-12, max_iter=args.max_iter)
     e.set_state(Z0, P0)
     e.step(DT, args.warmup)
     e.sync()
@@ -86,13 +90,22 @@ def main():
     e.step(DT, kt_steps)
     e.sync()
     reps = {t: e.kernel_report(t) for t in ("deriv", "assemble", "lu_panel", "lu_update", "lu_backsolve",
+                                            "gm_init", "gm_matvec", "gm_orth", "gm_extend", "gm_finish",
                                             "mu_deriv", "velocity")}
+    st = e.stats()
     e.kernel_timing(False)
-    up = reps["lu_update"]
-    avg_ms = up["ms"] / max(up["launches"], 1)
-    flops_per_launch = up["flops"] / max(up["launches"], 1)
-    achieved = flops_per_launch / (avg_ms * 1e-3) / 1e12
-    stages = {t: round(r["ms"] / kt_steps, 4) for t, r in reps.items()}
+    if args.solver == "gmres":
+        # dominant solver kernel: w = M v over the stored FP64 matrix (8 N^2 bytes per launch)
+        dom, bound, peak, unit, key, scale = "gm_matvec", "hbm", PEAK_HBM_GBS, "GB/s", "bytes", 1e9
+        kname = "gm_matvec_kernel (FP64 GMRES matrix-vector product)"
+    else:
+        dom, bound, peak, unit, key, scale = "lu_update", "mfma", PEAK_FP64_TFLOPS, "TFLOP/s", "flops", 1e12
+        kname = "lu_update_kernel (FP64 MFMA trailing update)"
+    r = reps[dom]
+    avg_ms = r["ms"] / max(r["launches"], 1)
+    work_per_launch = r[key] / max(r["launches"], 1)
+    achieved = work_per_launch / (avg_ms * 1e-3) / scale if avg_ms > 0 else 0.0
+    stages = {t: round(v["ms"] / kt_steps, 4) for t, v in reps.items() if v["launches"]}
     e.close()
 
     out = {
@@ -109,13 +122,15 @@ def main():
         "dtype": "f64",
         "data": f"synthetic: seeded 8-mode surface (numpy default_rng({SEED})), phi = 0",
         "config": {"workload": "single periodic water surface, N=2048 nodes, FP64, gravity 1, infinite depth, "
-                               "dt=0.002, blocked LU solve", "nodes": N, "dt": DT, "solver": "lu",
-                   "precision": "fp64"},
-        "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                     "frac": round(achieved / PEAK_FP64_TFLOPS, 4), "traffic": None,
-                     "kernel": "lu_update_kernel (FP64 MFMA trailing update)",
-                     "avg_launch_ms": round(avg_ms, 5), "flops_per_launch": flops_per_launch},
+                               f"dt=0.002, {'GMRES (tol 1e-12)' if args.solver == 'gmres' else 'blocked LU'} solve",
+                   "nodes": N, "dt": DT, "solver": args.solver, "precision": "fp64"},
+        "roofline": {"bound": bound, "achieved": round(achieved, 3), "peak": peak, "unit": unit,
+                     "frac": round(achieved / peak, 4), "traffic": None, "kernel": kname,
+                     "avg_launch_ms": round(avg_ms, 5), f"{key}_per_launch": work_per_launch,
+                     "launches_timed": r["launches"]},
         "stage_ms_per_step": stages,
+        "solver_stats": {"gmres_iterations_last": st["last_iterations"], "gmres_unconverged": st["unconverged"],
+                         "gepp_fallbacks": st["fallback_solves"]},
     }
     if not args.no_cpu:
         steps, cel, Zc, Pc, threads = cpu_baseline(Z0, P0)
@@ -123,7 +138,7 @@ def main():
                                "kind": "port",
                                "sample": f"{steps} RK4 steps of the same N=2048 workload on the CPU FP64 oracle "
                                          f"(OpenMP, {threads} threads), {cel:.1f} s"}
-        g = c.Ensemble([N], c.Props(), precision=c.FP64, solver=c.LU)
+        g = c.Ensemble([N], c.Props(), precision=c.FP64, solver=solver, tol=1e-12, max_iter=args.max_iter)
         g.set_state(Z0, P0)
         g.step(DT, steps)
         Zg, Pg = g.get_state()

@@ -138,7 +138,9 @@ typedef struct sfh_stats {
     long long steps;
     int fallback_solves;   /* systems that needed row interchanges and went through GEPP */
     int spin_timeouts;
-    int last_iterations;
+    int last_iterations;   /* GMRES: iterations of surface 0 in the last solve */
+    int max_iterations;    /* GMRES: most iterations any surface needed in the last solve */
+    int unconverged;       /* GMRES: solves that reached the iteration cap (accumulated) */
     int reserved;
 } sfh_stats;
 

@@ -3,7 +3,8 @@
 // Replaces createMKernel (CuSuperHelium/createM.cuh:43-63) and createFiniteDepthMKernel
 // (:65-92), which run one thread per entry on 16x16 blocks.  Here wave w of a workgroup owns
 // row k; its 64 lanes sweep the columns, so Z_k, Z'_k stay in registers, Z_j loads and M row
-// stores are fully coalesced, and cot comes from the cancellation-free form in sfh_common.hpp.
+// stores are fully coalesced, and cot comes from the per-node half-angle table (NodeTrig) with no
+// transcendental call per entry.
 //   M_kk = (1+rho)/2 + (1-rho)/(4 pi) Im(Z''_k / Z'_k)                       (water)
 //   M_kj = (1-rho)/(4 pi) Im(Z'_k cot((z_k - z_j)/2))
 //   helium: rho -> 0 and, with finite depth h, the image terms
@@ -13,6 +14,7 @@
 namespace sfh {
 
 __global__ __launch_bounds__(256) void assemble64_kernel(const SurfDesc* __restrict__ sd,
+                                                         const NodeTrig* __restrict__ trig,
                                                          const double2* __restrict__ Z,
                                                          const double2* __restrict__ Zp,
                                                          const double2* __restrict__ Zpp,
@@ -24,27 +26,37 @@ __global__ __launch_bounds__(256) void assemble64_kernel(const SurfDesc* __restr
     const int k = blockIdx.x * 4 + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     if (k >= n) return;
-    const double2* z = Z + d.voff;
-    const double2 zk = z[k];
+    const NodeTrig* tr = trig + d.voff;
+    const NodeTrig tk = tr[k];
     const double2 zpk = Zp[d.voff + k];
     const bool water = ph.kind == 0;
     const double c = water ? 0.25 * (1 - ph.rho) / kPi : 0.25 / kPi;
     const double ci = 0.25 / kPi;
-    const double h = ph.depth;
+    const double h = ph.depth, eh = exp(h), ehi = exp(-h);
     double* row = M + d.moff + (long long)k * n;
-    for (int j = lane; j < n; j += 64) {
-        double v;
-        if (j == k) {
-            const double2 r = cdiv(Zpp[d.voff + k], zpk);
-            v = (water ? 0.5 * (1 + ph.rho) : 0.5) + c * r.y;
-            if (ph.image) v -= ci * cot64(0.0, zk.y + h).y;
-        } else {
-            const double2 zj = z[j];
-            const double2 ct = cot64(0.5 * (zk.x - zj.x), 0.5 * (zk.y - zj.y));
-            v = c * (zpk.x * ct.y + zpk.y * ct.x);
-            if (ph.image) v -= ci * cot64(0.5 * (zk.x - zj.x), 0.5 * (zk.y + zj.y) + h).y;
+    for (int j0 = lane; j0 < n; j0 += 4 * 64) {
+        NodeTrig tj[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int j = j0 + u * 64;
+            if (j < n) tj[u] = tr[j];
         }
-        row[j] = v;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int j = j0 + u * 64;
+            if (j >= n) continue;
+            double v;
+            if (j == k) {
+                const double2 r = cdiv(Zpp[d.voff + k], zpk);
+                v = (water ? 0.5 * (1 + ph.rho) : 0.5) + c * r.y;
+                if (ph.image) v -= ci * cot64(0.0, Z[d.voff + k].y + h).y;
+            } else {
+                const double2 ct = cot_pair(tk, tj[u]);
+                v = c * (zpk.x * ct.y + zpk.y * ct.x);
+                if (ph.image) v -= ci * cot_image(tk, tj[u], eh, ehi).y;
+            }
+            row[j] = v;
+        }
     }
 }
 

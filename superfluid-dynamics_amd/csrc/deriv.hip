@@ -29,20 +29,47 @@ __device__ inline double2 twid(const double2* __restrict__ tw, int t, int sign) 
     return w;
 }
 
+constexpr int kFftThreads = 1024;
+
+// Radix-2 butterfly index i (the upper element is i + half) of butterfly t in a stage of
+// length len over `sub` independent sub-transforms of size m.
+__device__ inline int bfly_index(int t, int halfm, int half, int len, int m, int& pos) {
+    const int q = t / halfm, b = t - q * halfm;
+    const int grp = b / half;
+    pos = b - grp * half;
+    return q * m + grp * len + pos;
+}
+
 // Decimation in time: input in lds_index order, output in natural order.
-// tw[t] = exp(-2 pi i t / n), t < n; sign -1 forward, +1 inverse (unnormalised).
+// tw[t] = exp(-2 pi i t / n), t < n (LDS or global); sign -1 forward, +1 inverse (unnormalised).
+// Each thread loads all operands of up to 4 butterflies before computing any of them.
 __device__ void fft_dit(double2* s, const SurfDesc& d, int sign, const double2* __restrict__ tw) {
     const int n = d.n, m = 1 << d.log2m, sub = d.r3 ? 3 : 1, halfm = m >> 1;
+    const int nb = sub * halfm;
     for (int len = 2; len <= m; len <<= 1) {
         const int half = len >> 1, stride = n / len;
-        for (int t = threadIdx.x; t < sub * halfm; t += blockDim.x) {
-            const int q = t / halfm, b = t - q * halfm;
-            const int grp = b / half, pos = b - grp * half;
-            const int i = q * m + grp * len + pos;
-            const double2 u = s[i];
-            const double2 v = cmul(s[i + half], twid(tw, pos * stride, sign));
-            s[i] = cadd(u, v);
-            s[i + half] = csub(u, v);
+        for (int base = 0; base < nb; base += 4 * kFftThreads) {
+            double2 u[4], v[4], w[4];
+            int idx[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const int t = base + k * kFftThreads + (int)threadIdx.x;
+                int pos = 0;
+                idx[k] = (t < nb) ? bfly_index(t, halfm, half, len, m, pos) : -1;
+                if (idx[k] >= 0) {
+                    u[k] = s[idx[k]];
+                    v[k] = s[idx[k] + half];
+                    w[k] = twid(tw, pos * stride, sign);
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                if (idx[k] >= 0) {
+                    const double2 vw = cmul(v[k], w[k]);
+                    s[idx[k]] = cadd(u[k], vw);
+                    s[idx[k] + half] = csub(u[k], vw);
+                }
+            }
         }
         __syncthreads();
     }
@@ -66,6 +93,7 @@ __device__ void fft_dit(double2* s, const SurfDesc& d, int sign, const double2* 
 // Decimation in frequency: input in natural order, output in lds_index order.
 __device__ void fft_dif(double2* s, const SurfDesc& d, int sign, const double2* __restrict__ tw) {
     const int n = d.n, m = 1 << d.log2m, sub = d.r3 ? 3 : 1, halfm = m >> 1;
+    const int nb = sub * halfm;
     if (d.r3) {   // y_q[j] = W_n^{j q} sum_r x[j + r m] W_3^{r q}
         const double c = -0.5, s3 = 0.86602540378443864676372317075293618 * sign;
         for (int j = threadIdx.x; j < m; j += blockDim.x) {
@@ -81,16 +109,40 @@ __device__ void fft_dif(double2* s, const SurfDesc& d, int sign, const double2* 
     }
     for (int len = m; len >= 2; len >>= 1) {
         const int half = len >> 1, stride = n / len;
-        for (int t = threadIdx.x; t < sub * halfm; t += blockDim.x) {
-            const int q = t / halfm, b = t - q * halfm;
-            const int grp = b / half, pos = b - grp * half;
-            const int i = q * m + grp * len + pos;
-            const double2 u = s[i], v = s[i + half];
-            s[i] = cadd(u, v);
-            s[i + half] = cmul(csub(u, v), twid(tw, pos * stride, sign));
+        for (int base = 0; base < nb; base += 4 * kFftThreads) {
+            double2 u[4], v[4], w[4];
+            int idx[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const int t = base + k * kFftThreads + (int)threadIdx.x;
+                int pos = 0;
+                idx[k] = (t < nb) ? bfly_index(t, halfm, half, len, m, pos) : -1;
+                if (idx[k] >= 0) {
+                    u[k] = s[idx[k]];
+                    v[k] = s[idx[k] + half];
+                    w[k] = twid(tw, pos * stride, sign);
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                if (idx[k] >= 0) {
+                    s[idx[k]] = cadd(u[k], v[k]);
+                    s[idx[k] + half] = cmul(csub(u[k], v[k]), w[k]);
+                }
+            }
         }
         __syncthreads();
     }
+}
+
+// Twiddle source: the LDS copy (staged after the data, when it fits) or the global table.
+__device__ inline const double2* stage_twiddles(double2* smem, const SurfDesc& d, const double2* tw_all,
+                                                int lds_tw) {
+    const double2* tw = tw_all + d.twoff;
+    if (!lds_tw) return tw;
+    double2* t = smem + d.n;
+    for (int k = threadIdx.x; k < d.n; k += blockDim.x) t[k] = tw[k];
+    return t;   // visible after the caller's __syncthreads()
 }
 
 // First-derivative multiplier (utilities.cuh:119-147), applied to coefficient i.
@@ -118,27 +170,29 @@ __device__ inline void derivative_in_lds(double2* s, const SurfDesc& d, const do
 }
 
 // blockIdx.x = surface, blockIdx.y = field: 0 -> Z', 1 -> Z'', 2 -> Re Phi' (into b and work).
-__global__ __launch_bounds__(256) void zphi_deriv_kernel(const SurfDesc* __restrict__ sd,
-                                                         const double2* __restrict__ tw_all,
-                                                         const double2* __restrict__ Z,
-                                                         const double* __restrict__ phi,
-                                                         double2* __restrict__ Zp,
-                                                         double2* __restrict__ Zpp,
-                                                         double* __restrict__ b,
-                                                         double* __restrict__ work, Phys ph,
-                                                         int* __restrict__ status) {
+__global__ __launch_bounds__(1024) void zphi_deriv_kernel(const SurfDesc* __restrict__ sd,
+                                                          const double2* __restrict__ tw_all,
+                                                          const double2* __restrict__ Z,
+                                                          const double* __restrict__ phi,
+                                                          double2* __restrict__ Zp,
+                                                          double2* __restrict__ Zpp,
+                                                          double* __restrict__ b,
+                                                          double* __restrict__ work, Phys ph,
+                                                          int* __restrict__ status, int lds_tw,
+                                                          NodeTrig* __restrict__ trig) {
     extern __shared__ double2 s[];
     const SurfDesc d = sd[blockIdx.x];
     const int field = blockIdx.y;
     // re-arm the surface's interchange flag for the solve that follows in stream order
     if (field == 0 && threadIdx.x == 0 && status) status[2 * blockIdx.x] = 0;
     const int n = d.n, off = d.voff;
-    const double2* tw = tw_all + d.twoff;
+    const double2* tw = stage_twiddles(s, d, tw_all, lds_tw);
     const double plin = -(1 + ph.rho) * kPi * ph.U / n;   // Derivatives.cuh:286
     for (int j = threadIdx.x; j < n; j += blockDim.x) {
         double2 v;
         if (field < 2) {
             v = Z[off + j];
+            if (field == 0 && trig) trig[off + j] = node_trig(v);
             v.x -= 2 * kPi * (double)j / n;                 // Derivatives.cuh:285,317
         } else {
             v = make_double2(phi[off + j] - plin * (double)j, 0.0);
@@ -165,14 +219,14 @@ __global__ __launch_bounds__(256) void zphi_deriv_kernel(const SurfDesc* __restr
 }
 
 // mu' = IFFT(first_mult(FFT(mu))) * 2 pi / n, mu real (BaseBoundaryIntegrator.cuh:201-203).
-__global__ __launch_bounds__(256) void mu_deriv_kernel(const SurfDesc* __restrict__ sd,
-                                                       const double2* __restrict__ tw_all,
-                                                       const double* __restrict__ mu,
-                                                       double2* __restrict__ mup) {
+__global__ __launch_bounds__(1024) void mu_deriv_kernel(const SurfDesc* __restrict__ sd,
+                                                        const double2* __restrict__ tw_all,
+                                                        const double* __restrict__ mu,
+                                                        double2* __restrict__ mup, int lds_tw) {
     extern __shared__ double2 s[];
     const SurfDesc d = sd[blockIdx.x];
     const int n = d.n, off = d.voff;
-    const double2* tw = tw_all + d.twoff;
+    const double2* tw = stage_twiddles(s, d, tw_all, lds_tw);
     for (int j = threadIdx.x; j < n; j += blockDim.x) s[lds_index(d, j)] = make_double2(mu[off + j], 0.0);
     __syncthreads();
     derivative_in_lds(s, d, tw, false);
@@ -185,15 +239,15 @@ __global__ __launch_bounds__(256) void mu_deriv_kernel(const SurfDesc* __restric
 
 // Standalone derivative of an arbitrary complex field (FftDerivative::exec with scale 1 and
 // no linear part), used by the calculateDerivativeFFT256 entry point.
-__global__ __launch_bounds__(256) void plain_deriv_kernel(const SurfDesc* __restrict__ sd,
-                                                          const double2* __restrict__ tw_all,
-                                                          const double2* __restrict__ in,
-                                                          double2* __restrict__ out, int second,
-                                                          double scale) {
+__global__ __launch_bounds__(1024) void plain_deriv_kernel(const SurfDesc* __restrict__ sd,
+                                                           const double2* __restrict__ tw_all,
+                                                           const double2* __restrict__ in,
+                                                           double2* __restrict__ out, int second,
+                                                           double scale, int lds_tw) {
     extern __shared__ double2 s[];
     const SurfDesc d = sd[blockIdx.x];
     const int n = d.n, off = d.voff;
-    const double2* tw = tw_all + d.twoff;
+    const double2* tw = stage_twiddles(s, d, tw_all, lds_tw);
     for (int j = threadIdx.x; j < n; j += blockDim.x) s[lds_index(d, j)] = in[off + j];
     __syncthreads();
     derivative_in_lds(s, d, tw, second != 0);

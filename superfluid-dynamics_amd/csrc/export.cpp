@@ -379,6 +379,8 @@ int sfh_get_stats(sfh_handle h, sfh_stats* out) {
         out->fallback_solves = st.fallback_solves;
         out->spin_timeouts = st.spin_timeouts;
         out->last_iterations = st.last_iterations;
+        out->max_iterations = st.max_iterations;
+        out->unconverged = st.unconverged;
     });
 }
 

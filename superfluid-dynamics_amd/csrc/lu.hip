@@ -60,17 +60,26 @@ __device__ inline void acc_to_lds(double* C, const v4d acc[2][2], int wr, int wc
 }
 
 // Loads a rows x cols block of row-major global memory (leading dimension ld) into LDS,
-// zero-padding to 64 x 64.
+// zero-padding to 64 x 64.  Fixed trip count (256 threads x 16) so all 16 loads of a thread are
+// in flight together.
 __device__ inline void load_block(double* dst, const double* src, long long ld, int rows, int cols) {
-    for (int e = threadIdx.x; e < kNB * kNB; e += blockDim.x) {
-        const int r = e >> 6, c = e & 63;
-        dst[r * LD + c] = (r < rows && c < cols) ? src[(long long)r * ld + c] : 0.0;
+    double v[16];
+#pragma unroll
+    for (int it = 0; it < 16; ++it) {
+        const int e = threadIdx.x + it * 256, r = e >> 6, c = e & 63;
+        v[it] = (r < rows && c < cols) ? src[(long long)r * ld + c] : 0.0;
+    }
+#pragma unroll
+    for (int it = 0; it < 16; ++it) {
+        const int e = threadIdx.x + it * 256;
+        dst[(e >> 6) * LD + (e & 63)] = v[it];
     }
 }
 
 __device__ inline void store_block(double* dst, const double* src, long long ld, int rows, int cols) {
-    for (int e = threadIdx.x; e < kNB * kNB; e += blockDim.x) {
-        const int r = e >> 6, c = e & 63;
+#pragma unroll
+    for (int it = 0; it < 16; ++it) {
+        const int e = threadIdx.x + it * 256, r = e >> 6, c = e & 63;
         if (r < rows && c < cols) dst[(long long)r * ld + c] = src[r * LD + c];
     }
 }
@@ -83,8 +92,20 @@ __device__ void factor_invert_diag(double* D, double* Li, double* Ui, int kb, in
         const double inv = 1.0 / D[j * LD + j];
         const int i = j + 1 + (t >> 2), part = t & 3;
         if (i < kb) {
+            // all loads first: stores into row i must not serialise against the row-j loads
+            double rj[16], ri[16];
             const double l = D[i * LD + j] * inv;
-            for (int c = j + 1 + part; c < kb; c += 4) D[i * LD + c] -= l * D[j * LD + c];
+#pragma unroll
+            for (int m = 0; m < 16; ++m) {
+                const int c = j + 1 + part + 4 * m;
+                rj[m] = (c < kb) ? D[j * LD + c] : 0.0;
+                ri[m] = (c < kb) ? D[i * LD + c] : 0.0;
+            }
+#pragma unroll
+            for (int m = 0; m < 16; ++m) {
+                const int c = j + 1 + part + 4 * m;
+                if (c < kb) D[i * LD + c] = ri[m] - l * rj[m];
+            }
             if (part == 0) {
                 D[i * LD + j] = l;
                 if (!(fabs(l) <= 1.0)) *bad = 1;
@@ -98,18 +119,31 @@ __device__ void factor_invert_diag(double* D, double* Li, double* Ui, int kb, in
         Ui[r * LD + c] = 0.0;
     }
     __syncthreads();
-    // row s of L^{-1} (top down) and row kb-1-s of U^{-1} (bottom up), 4 threads per column
+    // row s of L^{-1} (top down) and row kb-1-s of U^{-1} (bottom up), 4 threads per column.
+    // Every operand is loaded before the first FMA so one LDS round trip covers the whole step.
+    const int c = t >> 2, part = t & 3;
     for (int s = 0; s < kb; ++s) {
-        const int c = t >> 2, part = t & 3;
-        // L^{-1}[s][c] = -sum_{q=c}^{s-1} L[s][q] Li[q][c], c < s
-        double accl = 0.0;
-        if (c < s)
-            for (int q = c + part; q < s; q += 4) accl += D[s * LD + q] * Li[q * LD + c];
-        // U^{-1}[i][c] = -(sum_{q=i+1}^{c} U[i][q] Ui[q][c]) / U[i][i], c > i;  Ui[i][i] = 1/U[i][i]
         const int i = kb - 1 - s;
-        double accu = 0.0;
-        if (c > i && c < kb)
-            for (int q = i + 1 + part; q <= c; q += 4) accu += D[i * LD + q] * Ui[q * LD + c];
+        double dl[16], ll[16], du[16], uu[16];
+#pragma unroll
+        for (int m = 0; m < 16; ++m) {
+            // L^{-1}[s][c] = -sum_{q=c}^{s-1} L[s][q] Li[q][c]           (c < s)
+            const int ql = c + part + 4 * m;
+            const bool okl = (c < s) && (ql < s);
+            dl[m] = okl ? D[s * LD + ql] : 0.0;
+            ll[m] = okl ? Li[ql * LD + c] : 0.0;
+            // U^{-1}[i][c] = -(sum_{q=i+1}^{c} U[i][q] Ui[q][c]) / U[i][i]  (i < c < kb)
+            const int qu = i + 1 + part + 4 * m;
+            const bool oku = (c > i) && (c < kb) && (qu <= c);
+            du[m] = oku ? D[i * LD + qu] : 0.0;
+            uu[m] = oku ? Ui[qu * LD + c] : 0.0;
+        }
+        double accl = 0.0, accu = 0.0;
+#pragma unroll
+        for (int m = 0; m < 16; ++m) {
+            accl += dl[m] * ll[m];
+            accu += du[m] * uu[m];
+        }
         accl += __shfl_xor(accl, 1, 64);
         accl += __shfl_xor(accl, 2, 64);
         accu += __shfl_xor(accu, 1, 64);
@@ -144,7 +178,7 @@ __device__ inline LuView view(const LuArgs& a) {
 
 __global__ __launch_bounds__(256) void lu_panel_kernel(LuArgs a, int k0) {
     __shared__ double D[kNB * LD], Li[kNB * LD], Ui[kNB * LD], X[kNB * LD];
-    __shared__ double yv[kNB];
+    __shared__ double yv[kNB], wv[kNB];
     __shared__ int bad;
     const LuView v = view(a);
     const int n = a.n, t = threadIdx.x;
@@ -159,19 +193,34 @@ __global__ __launch_bounds__(256) void lu_panel_kernel(LuArgs a, int k0) {
         for (int i = t; i < P; i += blockDim.x) v.flags[i] = 0;
     }
     load_block(D, v.A + (long long)k0 * n + k0, n, kb, kb);
+    if (t < kNB) wv[t] = (t < kb) ? v.work[k0 + t] : 0.0;
     __syncthreads();
     factor_invert_diag(D, Li, Ui, kb, &bad);
-    // y_p = L11^{-1} b_p
-    if (t < kNB) {
+    // y_p = L11^{-1} b_p (4 threads per row, operands staged in registers)
+    {
+        const int r = t >> 2, part = t & 3;
+        double lv[16], bv[16];
+#pragma unroll
+        for (int m = 0; m < 16; ++m) {
+            const int c = part + 4 * m;
+            lv[m] = Li[r * LD + c];
+            bv[m] = wv[c];
+        }
         double s = 0.0;
-        if (t < kb)
-            for (int c = 0; c <= t; ++c) s += Li[t * LD + c] * v.work[k0 + c];
-        yv[t] = s;
+#pragma unroll
+        for (int m = 0; m < 16; ++m) s += lv[m] * bv[m];
+        s += __shfl_xor(s, 1, 64);
+        s += __shfl_xor(s, 2, 64);
+        if (part == 0) yv[r] = (r < kb) ? s : 0.0;
     }
     __syncthreads();
     if (role == 0) {
         double* U = v.uinv + (long long)(k0 / kNB) * kNB * kNB;
-        for (int e = t; e < kNB * kNB; e += blockDim.x) U[e] = Ui[(e >> 6) * LD + (e & 63)];
+#pragma unroll
+        for (int it = 0; it < 16; ++it) {
+            const int e = t + it * 256;
+            U[e] = Ui[(e >> 6) * LD + (e & 63)];
+        }
         if (t < kb) v.y[k0 + t] = yv[t];
         if (t == 0 && bad) atomicOr(&v.status[0], 1);
         return;
@@ -195,10 +244,21 @@ __global__ __launch_bounds__(256) void lu_panel_kernel(LuArgs a, int k0) {
         }
         if (my_bad) atomicOr(&v.status[0], 1);
         store_block(v.A + (long long)R0 * n + k0, X, n, rb, kb);
-        if (t < rb) {                                     // b_i -= L21_i y_p
+        {                                                 // b_i -= L21_i y_p
+            const int r = t >> 2, part = t & 3;
+            double xv[16], yy[16];
+#pragma unroll
+            for (int m = 0; m < 16; ++m) {
+                const int c = part + 4 * m;
+                xv[m] = X[r * LD + c];
+                yy[m] = yv[c];
+            }
             double s = 0.0;
-            for (int c = 0; c < kb; ++c) s += X[t * LD + c] * yv[c];
-            v.work[R0 + t] -= s;
+#pragma unroll
+            for (int m = 0; m < 16; ++m) s += xv[m] * yy[m];
+            s += __shfl_xor(s, 1, 64);
+            s += __shfl_xor(s, 2, 64);
+            if (part == 0 && r < rb) v.work[R0 + r] -= s;
         }
     } else {
         const int C0 = k0 + kb + (role - 1 - nrb) * kNB;
@@ -245,22 +305,28 @@ __global__ __launch_bounds__(256) void lu_update_kernel(LuArgs a, int k0) {
 }
 
 // x_p = U_pp^{-1} (y_p - sum_{q>p} U_pq x_q).  Workgroup b handles block row p = P-1-b, so a
-// workgroup only waits for workgroups dispatched before it; every wait is bounded.
+// workgroup only waits for workgroups dispatched before it; every wait is bounded.  Each U_pq
+// block is staged into LDS before the wait for x_q, so only the x_q load and an LDS GEMV sit on
+// the dependency chain.
 __global__ __launch_bounds__(256) void lu_backsolve_kernel(LuArgs a) {
-    __shared__ double part[4][kNB];
-    __shared__ double sv[kNB];
+    __shared__ double Ub[kNB * LD], Ui[kNB * LD];
+    __shared__ double xq[kNB], part[4][kNB], sv[kNB];
     const LuView v = view(a);
     const int n = a.n, t = threadIdx.x;
     const int P = (n + kNB - 1) / kNB;
     const int p = P - 1 - blockIdx.x;
     const int R0 = p * kNB, rb = min(kNB, n - R0);
     const int r = t & 63, chunk = t >> 6;
+    const double* U = v.uinv + (long long)p * kNB * kNB;
+    load_block(Ui, U, kNB, kNB, kNB);
     double acc = 0.0;
     for (int q = P - 1; q > p; --q) {
+        const int C0 = q * kNB, cb = min(kNB, n - C0);
+        load_block(Ub, v.A + (long long)R0 * n + C0, n, rb, cb);
         if (t == 0) {
             int spins = 0;
             while (__hip_atomic_load(&v.flags[q], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == 0) {
-                __builtin_amdgcn_s_sleep(2);
+                __builtin_amdgcn_s_sleep(1);
                 if (++spins > (1 << 22)) {
                     atomicOr(&v.status[1], 1);
                     break;
@@ -268,23 +334,27 @@ __global__ __launch_bounds__(256) void lu_backsolve_kernel(LuArgs a) {
             }
         }
         __syncthreads();
-        (void)__hip_atomic_load(&v.flags[q], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-        const int C0 = q * kNB, cb = min(kNB, n - C0);
-        if (r < rb) {
-            const double* Ur = v.A + (long long)(R0 + r) * n + C0;
-            for (int c = chunk * 16; c < min(cb, chunk * 16 + 16); ++c) acc += Ur[c] * v.x[C0 + c];
+        if (t < kNB) {
+            (void)__hip_atomic_load(&v.flags[q], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+            xq[t] = (t < cb) ? v.x[C0 + t] : 0.0;
         }
+        __syncthreads();
+#pragma unroll
+        for (int c = 0; c < 16; ++c) acc += Ub[r * LD + chunk * 16 + c] * xq[chunk * 16 + c];
+        __syncthreads();
     }
     part[chunk][r] = acc;
     __syncthreads();
     if (t < kNB) sv[t] = (t < rb) ? v.y[R0 + t] - (part[0][t] + part[1][t] + part[2][t] + part[3][t]) : 0.0;
     __syncthreads();
-    const double* U = v.uinv + (long long)p * kNB * kNB;
-    if (t < rb) {
+    {   // x_p = U_pp^{-1} sv, 4 threads per row
         double s = 0.0;
-        for (int c = t; c < rb; ++c) s += U[t * kNB + c] * sv[c];
-        v.x[R0 + t] = s;
+#pragma unroll
+        for (int c = 0; c < 16; ++c) s += Ui[r * LD + chunk * 16 + c] * sv[chunk * 16 + c];
+        part[chunk][r] = s;
     }
+    __syncthreads();
+    if (t < rb) v.x[R0 + t] = part[0][t] + part[1][t] + part[2][t] + part[3][t];
     __syncthreads();
     if (t == 0) __hip_atomic_store(&v.flags[p], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }

@@ -94,6 +94,44 @@ __device__ inline float2 cot32(float a, float b) {
     return make_float2(4.0f * sa * ca * t, -copysignf(1.0f + 2.0f * t, b));
 }
 
+// Per-node table for pairwise cotangents: sin(x/2), cos(x/2), exp(y/2), exp(-y/2).
+// With it cot((z_k - z_j)/2) = (sa ca - i sh ch) / (sa^2 + sh^2) needs no transcendental call:
+//   sa = sin((x_k - x_j)/2) = s_k c_j - c_k s_j,  ca = c_k c_j + s_k s_j,
+//   e^{+-b} = e_k^{+-1} e_j^{-+1} with b = (y_k - y_j)/2, sh = (e^b - e^-b)/2, ch = (e^b + e^-b)/2.
+// The subtractions lose relative accuracy only in proportion to eps / |z_k - z_j|, the same order
+// as the rounding already present in forming x_k - x_j and y_k - y_j.
+struct NodeTrig {
+    double s, c, e, ei;
+};
+
+__device__ inline NodeTrig node_trig(double2 z) {
+    NodeTrig t;
+    sincos(0.5 * z.x, &t.s, &t.c);
+    t.e = exp(0.5 * z.y);
+    t.ei = 1.0 / t.e;
+    return t;
+}
+
+// cot((z_k - z_j) / 2)
+__device__ inline double2 cot_pair(const NodeTrig& k, const NodeTrig& j) {
+    const double sa = k.s * j.c - k.c * j.s;
+    const double ca = k.c * j.c + k.s * j.s;
+    const double ep = k.e * j.ei, em = k.ei * j.e;
+    const double sh = 0.5 * (ep - em), ch = 0.5 * (ep + em);
+    const double id = 1.0 / (sa * sa + sh * sh);
+    return make_double2(sa * ca * id, -(sh * ch) * id);
+}
+
+// cot((z_k - conj z_j) / 2 + i h), eh = exp(h), ehi = exp(-h): b = (y_k + y_j)/2 + h > 0.
+__device__ inline double2 cot_image(const NodeTrig& k, const NodeTrig& j, double eh, double ehi) {
+    const double sa = k.s * j.c - k.c * j.s;
+    const double ca = k.c * j.c + k.s * j.s;
+    const double ep = k.e * j.e * eh, em = k.ei * j.ei * ehi;
+    const double sh = 0.5 * (ep - em), ch = 0.5 * (ep + em);
+    const double id = 1.0 / (sa * sa + sh * sh);
+    return make_double2(sa * ca * id, -(sh * ch) * id);
+}
+
 // ------------------------------------------------------------------ wave reductions
 __device__ inline double wave_sum(double v) {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);

@@ -24,20 +24,22 @@ struct StageCtl {
 };
 
 // ---- deriv.hip
+constexpr int kDerivThreads = 1024;
 __global__ void zphi_deriv_kernel(const SurfDesc* sd, const double2* tw_all, const double2* Z,
                                   const double* phi, double2* Zp, double2* Zpp, double* b,
-                                  double* work, Phys ph, int* status);
+                                  double* work, Phys ph, int* status, int lds_tw, NodeTrig* trig);
 __global__ void mu_deriv_kernel(const SurfDesc* sd, const double2* tw_all, const double* mu,
-                                double2* mup);
+                                double2* mup, int lds_tw);
 __global__ void plain_deriv_kernel(const SurfDesc* sd, const double2* tw_all, const double2* in,
-                                   double2* out, int second, double scale);
+                                   double2* out, int second, double scale, int lds_tw);
 
 // ---- assemble.hip
-__global__ void assemble64_kernel(const SurfDesc* sd, const double2* Z, const double2* Zp,
-                                  const double2* Zpp, double* M, Phys ph, const int* only_if);
+__global__ void assemble64_kernel(const SurfDesc* sd, const NodeTrig* trig, const double2* Z,
+                                  const double2* Zp, const double2* Zpp, double* M, Phys ph,
+                                  const int* only_if);
 
 // ---- velocity.hip
-__global__ void velocity_stage_kernel(const SurfDesc* sd, const double2* Z, const double* phi,
+__global__ void velocity_stage_kernel(const SurfDesc* sd, const NodeTrig* trig, const double2* Z,
                                       const double2* Zp, const double2* Zpp, const double* mu,
                                       const double2* mup, Phys ph, StageCtl ctl);
 
@@ -57,5 +59,28 @@ __global__ void lu_update_kernel(LuArgs a, int k0);
 __global__ void lu_backsolve_kernel(LuArgs a);
 __global__ void gepp_fallback_kernel(const SurfDesc* sd, double* M, const double* b, double* x,
                                      int* piv, int* status, int* counters);
+
+// ---- gmres.hip
+struct GmArgs {
+    const double* M;     // matrices (FP64, row-major per surface at SurfDesc::moff)
+    const double* b;     // right-hand sides
+    double* x;           // initial guess in, solution out
+    double* V;           // Krylov basis, (m+1) * voff per surface
+    double* w;           // work vector
+    double* part;        // per-workgroup partial sums, part_stride per surface
+    double* small;       // per surface: scales, coefficients, rotations, g, H
+    int* ctl;            // per surface control block
+    long long part_stride;
+    long long small_stride;
+    int m;               // maximal number of iterations (Krylov dimension)
+    int pad;
+    double tol;
+};
+constexpr int kGmCtlInts = 16;   // ints per control block (holds the device struct)
+__global__ void gm_init_kernel(GmArgs a, const SurfDesc* sd);
+__global__ void gm_matvec_kernel(GmArgs a, const SurfDesc* sd, int j);
+__global__ void gm_orth_kernel(GmArgs a, const SurfDesc* sd, int j);
+__global__ void gm_extend_kernel(GmArgs a, const SurfDesc* sd, int j);
+__global__ void gm_finish_kernel(GmArgs a, const SurfDesc* sd);
 
 }  // namespace sfh

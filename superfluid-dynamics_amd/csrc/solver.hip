@@ -53,8 +53,8 @@ Solver::Solver(const Phys& ph, const std::vector<int>& n, const Options& opt) : 
     S_ = (int)n.size();
     for (int v : n)
         if (!supported_n(v)) throw ArgError("unsupported node count " + std::to_string(v));
-    if (opt_.precision != kFP64 || opt_.solver != kSolveLU)
-        throw ArgError("requested precision / solver form is not available");
+    if (opt_.precision != kFP64) throw ArgError("requested precision form is not available");
+    if (opt_.solver != kSolveLU && opt_.solver != kSolveGMRES) throw ArgError("unknown solver");
     SFH_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
 
     // descriptors and twiddle tables (one table per distinct n, computed in long double)
@@ -112,6 +112,7 @@ Solver::Solver(const Phys& ph, const std::vector<int>& n, const Options& opt) : 
     y_ = dalloc<double>(T_);
     mu_ = dalloc<double>(T_);
     M_ = dalloc<double>(MT_);
+    trig_ = dalloc<NodeTrig>(T_);
     long long utot = 0, ftot = 0;
     for (int s = 0; s < S_; ++s) {
         const int P = (n[s] + kNB - 1) / kNB;
@@ -131,6 +132,19 @@ Solver::Solver(const Phys& ph, const std::vector<int>& n, const Options& opt) : 
     SFH_HIP(hipMemset(P0_, 0, T_ * sizeof(double)));
     SFH_HIP(hipMemset(mu_, 0, T_ * sizeof(double)));
     SFH_HIP(hipMemset(mup_, 0, T_ * sizeof(double2)));
+    if (opt_.solver == kSolveGMRES) {
+        gm_m_ = std::max(1, std::min(opt_.max_iter, 32));   // kMaxKrylov in gmres.hip
+        gm_nwg_ = (maxN_ + 7) / 8;
+        gm_part_stride_ = (long long)gm_nwg_ * (gm_m_ + 2);
+        gm_small_stride_ = 3LL * (gm_m_ + 1) + 2LL * gm_m_ + (long long)gm_m_ * (gm_m_ + 1);
+        gmV_ = dalloc<double>((size_t)(gm_m_ + 1) * T_);
+        gmw_ = dalloc<double>(T_);
+        gmpart_ = dalloc<double>(gm_part_stride_ * S_);
+        gmsmall_ = dalloc<double>(gm_small_stride_ * S_);
+        gmctl_ = dalloc<int>((size_t)kGmCtlInts * S_);
+        SFH_HIP(hipMemset(gmctl_, 0, sizeof(int) * kGmCtlInts * S_));
+        SFH_HIP(hipMemset(gmsmall_, 0, sizeof(double) * gm_small_stride_ * S_));
+    }
     SFH_HIP(hipDeviceSynchronize());
 }
 
@@ -140,7 +154,7 @@ Solver::~Solver() {
     if (stream_) (void)hipStreamSynchronize(stream_);
     void* bufs[] = {d_desc_, d_tw_, Z0_, Za_, Zb_, P0_, Pa_, Pb_, kZ_[0], kZ_[1], kZ_[2], kP_[0],
                     kP_[1], kP_[2], Zp_, Zpp_, mup_, qup_, b_, work_, y_, mu_, M_, uinv_, flags_,
-                    status_, piv_, counters_};
+                    status_, piv_, counters_, gmV_, gmw_, gmpart_, gmsmall_, gmctl_, trig_};
     for (void* p : bufs)
         if (p) (void)hipFree(p);
     for (hipEvent_t e : event_pool_) (void)hipEventDestroy(e);
@@ -148,7 +162,7 @@ Solver::~Solver() {
 }
 
 template <class F>
-void Solver::launch(const char* tag, double flops, double bytes, F&& f) {
+void Solver::launch(const char* tag, double flops, double bytes, F&& f, int iter) {
     if (!ktime_) {
         f();
         return;
@@ -162,7 +176,7 @@ void Solver::launch(const char* tag, double flops, double bytes, F&& f) {
     SFH_HIP(hipEventRecord(a, stream_));
     f();
     SFH_HIP(hipEventRecord(b, stream_));
-    timed_.push_back({tag, a, b, flops, bytes});
+    timed_.push_back({tag, a, b, flops, bytes, iter, true});
 }
 
 void Solver::set_kernel_timing(bool on) {
@@ -176,7 +190,7 @@ Solver::KernelReport Solver::kernel_report(const std::string& tag) {
     SFH_HIP(hipStreamSynchronize(stream_));
     KernelReport r;
     for (const Timed& t : timed_) {
-        if (tag != t.tag) continue;
+        if (tag != t.tag || !t.active) continue;
         float ms = 0.f;
         SFH_HIP(hipEventElapsedTime(&ms, t.a, t.b));
         r.launches += 1;
@@ -187,8 +201,64 @@ Solver::KernelReport Solver::kernel_report(const std::string& tag) {
     return r;
 }
 
+// GMRES on the stored FP64 matrices of all surfaces at once (gridDim.z = surface), warm started
+// from the previous mu.
+void Solver::enqueue_gmres() {
+    GmArgs a{};
+    a.M = M_;
+    a.b = b_;
+    a.x = mu_;
+    a.V = gmV_;
+    a.w = gmw_;
+    a.part = gmpart_;
+    a.small = gmsmall_;
+    a.ctl = gmctl_;
+    a.part_stride = gm_part_stride_;
+    a.small_stride = gm_small_stride_;
+    a.m = gm_m_;
+    a.tol = opt_.tol;
+    double nsq = 0.0;
+    for (int v : n_) nsq += (double)v * v;
+    const dim3 grid(gm_nwg_, 1, S_);
+    const size_t first = timed_.size();
+    launch("gm_init", 2.0 * nsq, 8.0 * nsq, [&] {
+        hipLaunchKernelGGL(gm_init_kernel, grid, dim3(256), 0, stream_, a, (const SurfDesc*)d_desc_);
+    });
+    for (int j = 0; j < gm_m_; ++j) {
+        launch(
+            "gm_matvec", 2.0 * nsq, 8.0 * nsq,
+            [&] { hipLaunchKernelGGL(gm_matvec_kernel, grid, dim3(256), 0, stream_, a, (const SurfDesc*)d_desc_, j); },
+            j);
+        launch(
+            "gm_orth", 4.0 * (j + 1) * T_, 8.0 * (j + 3) * T_,
+            [&] { hipLaunchKernelGGL(gm_orth_kernel, grid, dim3(256), 0, stream_, a, (const SurfDesc*)d_desc_, j); },
+            j);
+        launch(
+            "gm_extend", 2.0 * (j + 1) * T_, 8.0 * (j + 3) * T_,
+            [&] { hipLaunchKernelGGL(gm_extend_kernel, grid, dim3(256), 0, stream_, a, (const SurfDesc*)d_desc_, j); },
+            j);
+    }
+    launch("gm_finish", 2.0 * gm_m_ * T_, 8.0 * gm_m_ * T_, [&] {
+        hipLaunchKernelGGL(gm_finish_kernel, dim3((maxN_ + 255) / 256, 1, S_), dim3(256), 0, stream_, a,
+                           (const SurfDesc*)d_desc_);
+    });
+    if (ktime_) {   // keep only iterations that did work (the rest returned after convergence)
+        std::vector<int> ctl((size_t)kGmCtlInts * S_);
+        SFH_HIP(hipStreamSynchronize(stream_));
+        SFH_HIP(hipMemcpy(ctl.data(), gmctl_, ctl.size() * sizeof(int), hipMemcpyDeviceToHost));
+        int iters = 0;
+        for (int s = 0; s < S_; ++s) iters = std::max(iters, ctl[(size_t)kGmCtlInts * s + 2]);
+        for (size_t k = first; k < timed_.size(); ++k)
+            if (timed_[k].iter >= iters) timed_[k].active = false;
+    }
+}
+
 // M mu = b for every surface; M, b (and its working copy) are ready in stream order.
 void Solver::enqueue_solve(const double2* Z) {
+    if (opt_.solver == kSolveGMRES) {
+        enqueue_gmres();
+        return;
+    }
     auto run = [&](int s0, int count, int n) {
         LuArgs a;
         a.A = M_ + desc_[s0].moff;
@@ -226,14 +296,16 @@ void Solver::enqueue_solve(const double2* Z) {
         for (int s = 0; s < S_; ++s) run(s, 1, n_[s]);
     }
     // systems whose interchange-free check failed: re-assemble and solve with GEPP
-    hipLaunchKernelGGL(assemble64_kernel, dim3((maxN_ + 3) / 4, S_), dim3(256), 0, stream_, d_desc_, Z, Zp_,
-                       Zpp_, M_, ph_, (const int*)status_);
+    hipLaunchKernelGGL(assemble64_kernel, dim3((maxN_ + 3) / 4, S_), dim3(256), 0, stream_, d_desc_,
+                       (const NodeTrig*)trig_, Z, Zp_, Zpp_, M_, ph_, (const int*)status_);
     hipLaunchKernelGGL(gepp_fallback_kernel, dim3(S_), dim3(256), 0, stream_, d_desc_, M_, b_, mu_, piv_,
                        status_, counters_);
 }
 
 void Solver::enqueue_rhs(const double2* Z, const double* P, const StageCtl& ctl) {
-    const size_t lds = (size_t)maxN_ * sizeof(double2);
+    // data + twiddles in LDS when both fit (n <= 4096), data only otherwise
+    const int lds_tw = (2 * (size_t)maxN_ * sizeof(double2) <= 144 * 1024) ? 1 : 0;
+    const size_t lds = (size_t)maxN_ * sizeof(double2) * (lds_tw ? 2 : 1);
     double nsq = 0.0, nlog = 0.0;
     for (int v : n_) {
         nsq += (double)v * v;
@@ -241,21 +313,22 @@ void Solver::enqueue_rhs(const double2* Z, const double* P, const StageCtl& ctl)
     }
     const double img = ph_.image ? 2.0 : 1.0;
     launch("deriv", 3 * 2 * nlog, 64.0 * T_, [&] {
-        hipLaunchKernelGGL(zphi_deriv_kernel, dim3(S_, 3), dim3(256), lds, stream_, d_desc_, d_tw_, Z, P, Zp_,
-                           Zpp_, b_, work_, ph_, status_);
+        hipLaunchKernelGGL(zphi_deriv_kernel, dim3(S_, 3), dim3(kDerivThreads), lds, stream_, d_desc_, d_tw_, Z,
+                           P, Zp_, Zpp_, b_, work_, ph_, status_, lds_tw, trig_);
     });
     // assembly: flops counted as cot evaluations (the transcendental-bound unit), bytes = M written
     launch("assemble", img * nsq, 8.0 * nsq, [&] {
-        hipLaunchKernelGGL(assemble64_kernel, dim3((maxN_ + 3) / 4, S_), dim3(256), 0, stream_, d_desc_, Z,
-                           Zp_, Zpp_, M_, ph_, (const int*)nullptr);
+        hipLaunchKernelGGL(assemble64_kernel, dim3((maxN_ + 3) / 4, S_), dim3(256), 0, stream_, d_desc_,
+                           (const NodeTrig*)trig_, Z, Zp_, Zpp_, M_, ph_, (const int*)nullptr);
     });
     enqueue_solve(Z);
     launch("mu_deriv", 2 * nlog, 24.0 * T_, [&] {
-        hipLaunchKernelGGL(mu_deriv_kernel, dim3(S_), dim3(256), lds, stream_, d_desc_, d_tw_, mu_, mup_);
+        hipLaunchKernelGGL(mu_deriv_kernel, dim3(S_), dim3(kDerivThreads), lds, stream_, d_desc_, d_tw_, mu_, mup_,
+                           lds_tw);
     });
     launch("velocity", img * nsq, 96.0 * T_, [&] {
-        hipLaunchKernelGGL(velocity_stage_kernel, dim3((maxN_ + 3) / 4, S_), dim3(256), 0, stream_, d_desc_, Z,
-                           P, Zp_, Zpp_, mu_, mup_, ph_, ctl);
+        hipLaunchKernelGGL(velocity_stage_kernel, dim3((maxN_ + 3) / 4, S_), dim3(256), 0, stream_, d_desc_,
+                           (const NodeTrig*)trig_, Z, Zp_, Zpp_, mu_, mup_, ph_, ctl);
     });
     SFH_HIP(hipGetLastError());
 }
@@ -377,6 +450,18 @@ void Solver::check_status() {
     int timeouts = 0;
     for (int s = 0; s < S_; ++s) timeouts += st[2 * s + 1] != 0;
     stats_.spin_timeouts = timeouts;
+    if (opt_.solver == kSolveGMRES) {
+        std::vector<int> ctl((size_t)kGmCtlInts * S_);
+        SFH_HIP(hipMemcpy(ctl.data(), gmctl_, ctl.size() * sizeof(int), hipMemcpyDeviceToHost));
+        int mx = 0, unc = 0;
+        for (int s = 0; s < S_; ++s) {
+            mx = std::max(mx, ctl[(size_t)kGmCtlInts * s + 2]);
+            unc += ctl[(size_t)kGmCtlInts * s + 3];
+        }
+        stats_.last_iterations = ctl[2];
+        stats_.max_iterations = mx;
+        stats_.unconverged = unc;
+    }
     if (timeouts) throw HipError("back-substitution wait timed out; results are invalid");
 }
 
@@ -387,9 +472,10 @@ Stats Solver::stats() {
 
 void Solver::derivative(const double2* in_host, double2* out_host, bool second, double scale) {
     SFH_HIP(hipMemcpyAsync(Za_, in_host, T_ * sizeof(double2), hipMemcpyHostToDevice, stream_));
-    const size_t lds = (size_t)maxN_ * sizeof(double2);
-    hipLaunchKernelGGL(plain_deriv_kernel, dim3(S_), dim3(256), lds, stream_, d_desc_, d_tw_, Za_, Zb_,
-                       second ? 1 : 0, scale);
+    const int lds_tw = (2 * (size_t)maxN_ * sizeof(double2) <= 144 * 1024) ? 1 : 0;
+    const size_t lds = (size_t)maxN_ * sizeof(double2) * (lds_tw ? 2 : 1);
+    hipLaunchKernelGGL(plain_deriv_kernel, dim3(S_), dim3(kDerivThreads), lds, stream_, d_desc_, d_tw_, Za_, Zb_,
+                       second ? 1 : 0, scale, lds_tw);
     SFH_HIP(hipGetLastError());
     SFH_HIP(hipMemcpyAsync(out_host, Zb_, T_ * sizeof(double2), hipMemcpyDeviceToHost, stream_));
     SFH_HIP(hipStreamSynchronize(stream_));

@@ -45,7 +45,9 @@ struct Stats {
     long long steps = 0;
     int fallback_solves = 0;   // interchange-needing systems re-solved by GEPP
     int spin_timeouts = 0;
-    int last_iterations = 0;   // iterative solvers: iterations of the last solve
+    int last_iterations = 0;   // iterative solvers: iterations of the last solve (surface 0)
+    int max_iterations = 0;    // iterative solvers: most iterations any surface used last solve
+    int unconverged = 0;       // iterative solves that hit the iteration cap
 };
 
 bool supported_n(int n);
@@ -98,12 +100,14 @@ class Solver {
     void capture(double dt);
     void check_status();
     template <class F>
-    void launch(const char* tag, double flops, double bytes, F&& f);
+    void launch(const char* tag, double flops, double bytes, F&& f, int iter = -1);
 
     struct Timed {
         const char* tag;
         hipEvent_t a, b;
         double flops, bytes;
+        int iter;      // GMRES iteration of the launch (-1 otherwise)
+        bool active;   // false for GMRES launches that returned at once after convergence
     };
     bool ktime_ = false;
     std::vector<Timed> timed_;
@@ -128,12 +132,19 @@ class Solver {
     double2 *Zp_ = nullptr, *Zpp_ = nullptr, *mup_ = nullptr, *qup_ = nullptr;
     double *b_ = nullptr, *work_ = nullptr, *y_ = nullptr, *mu_ = nullptr;
     double* M_ = nullptr;
+    NodeTrig* trig_ = nullptr;
     double* uinv_ = nullptr;
     int* flags_ = nullptr;
     int* status_ = nullptr;
     int* piv_ = nullptr;
     int* counters_ = nullptr;   // [0] fallback solves
     std::vector<long long> uoff_, foff_;
+    // GMRES workspace
+    int gm_m_ = 0, gm_nwg_ = 0;
+    double *gmV_ = nullptr, *gmw_ = nullptr, *gmpart_ = nullptr, *gmsmall_ = nullptr;
+    int* gmctl_ = nullptr;
+    long long gm_part_stride_ = 0, gm_small_stride_ = 0;
+    void enqueue_gmres();
 
     hipGraph_t graph_ = nullptr;
     hipGraphExec_t exec_ = nullptr;

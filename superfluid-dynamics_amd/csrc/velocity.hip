@@ -51,26 +51,38 @@ __device__ inline double bernoulli(const Phys& ph, double y, double2 q1, double2
 
 // Lower (q1) and upper (q2) fluid velocity at node k of one surface; every lane of the wave
 // takes part and every lane returns the result.
-__device__ inline void node_velocity(const SurfDesc& d, const double2* __restrict__ z,
-                                     const double* __restrict__ m, const double2* __restrict__ Zp,
-                                     const double2* __restrict__ Zpp, const double2* __restrict__ mup,
-                                     const Phys& ph, int k, int lane, double2& q1, double2& q2) {
+__device__ inline void node_velocity(const SurfDesc& d, const NodeTrig* __restrict__ tr,
+                                     const double2* __restrict__ z, const double* __restrict__ m,
+                                     const double2* __restrict__ Zp, const double2* __restrict__ Zpp,
+                                     const double2* __restrict__ mup, const Phys& ph, int k, int lane,
+                                     double2& q1, double2& q2) {
     const int n = d.n;
-    const double2 zk = z[k];
-    const double h = ph.depth;
+    const NodeTrig tk = tr[k];
+    const double h = ph.depth, eh = exp(h), ehi = exp(-h);
     double sx = 0.0, sy = 0.0, ix = 0.0, iy = 0.0;
-    for (int j = lane; j < n; j += 64) {
-        if (j == k) continue;
-        const double2 zj = z[j];
-        const double mj = m[j];
-        const double a = 0.5 * (zk.x - zj.x);
-        const double2 ct = cot64(a, 0.5 * (zk.y - zj.y));
-        sx += mj * ct.x;
-        sy += mj * ct.y;
-        if (ph.image) {
-            const double2 ci = cot64(a, 0.5 * (zk.y + zj.y) + h);
-            ix += mj * ci.x;
-            iy += mj * ci.y;
+    for (int j0 = lane; j0 < n; j0 += 4 * 64) {
+        NodeTrig tj[4];
+        double mj[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int j = j0 + u * 64;
+            if (j < n) {
+                tj[u] = tr[j];
+                mj[u] = m[j];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int j = j0 + u * 64;
+            if (j >= n || j == k) continue;
+            const double2 ct = cot_pair(tk, tj[u]);
+            sx += mj[u] * ct.x;
+            sy += mj[u] * ct.y;
+            if (ph.image) {
+                const double2 cim = cot_image(tk, tj[u], eh, ehi);
+                ix += mj[u] * cim.x;
+                iy += mj[u] * cim.y;
+            }
         }
     }
     sx = wave_sum(sx);
@@ -88,7 +100,7 @@ __device__ inline void node_velocity(const SurfDesc& d, const double2* __restric
     if (ph.image) w = cadd(w, ctimes_i(make_double2(q4 * ix, q4 * iy)));
     // diagonal V1_kk without the +-1/(2 Z') part
     double2 v1 = ctimes_i(cscale(cdiv(zpp, cmul(zp, zp)), -q4));
-    if (ph.image) v1 = cadd(v1, ctimes_i(cscale(cot64(0.0, zk.y + h), q4)));
+    if (ph.image) v1 = cadd(v1, ctimes_i(cscale(cot64(0.0, z[k].y + h), q4)));
     const double2 half_inv = cscale(crecip(zp), 0.5);
     const double2 v2 = ctimes_i(crecip(cscale(zp, 2.0 * kPi)));
     const double2 base = cadd(cadd(cmul(v2, mup[g]), w), cscale(v1, mk));
@@ -97,8 +109,8 @@ __device__ inline void node_velocity(const SurfDesc& d, const double2* __restric
 }
 
 __global__ __launch_bounds__(256) void velocity_stage_kernel(const SurfDesc* __restrict__ sd,
+                                                             const NodeTrig* __restrict__ trig,
                                                              const double2* __restrict__ Z,
-                                                             const double* __restrict__ phi,
                                                              const double2* __restrict__ Zp,
                                                              const double2* __restrict__ Zpp,
                                                              const double* __restrict__ mu,
@@ -111,14 +123,15 @@ __global__ __launch_bounds__(256) void velocity_stage_kernel(const SurfDesc* __r
     if (k >= n) return;
     const double2* z = Z + d.voff;
     const double* m = mu + d.voff;
+    const NodeTrig* tr = trig + d.voff;
     double2 q1, q2;
-    node_velocity(d, z, m, Zp, Zpp, mup, ph, k, lane, q1, q2);
+    node_velocity(d, tr, z, m, Zp, Zpp, mup, ph, k, lane, q1, q2);
     // createM.cuh:104 reads V1[1].real() (lower velocity of node 1) in the rho * q1.q2 term;
     // with rho != 0 node 1 is recomputed here (each surface uses its own node 1).
     double2 q1q = q1;
     if (ph.kind == 0 && ph.rho_quirk && ph.rho != 0.0 && n > 1 && k != 1) {
         double2 u1, u2;
-        node_velocity(d, z, m, Zp, Zpp, mup, ph, 1, lane, u1, u2);
+        node_velocity(d, tr, z, m, Zp, Zpp, mup, ph, 1, lane, u1, u2);
         q1q = u1;
     }
     if (lane != 0) return;

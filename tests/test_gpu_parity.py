@@ -252,3 +252,37 @@ def test_dropin_rejects_unsupported_sizes(csh):
     x = np.zeros(100)
     res, *_ = csh.SimulationManager().calculate_rhs_from_vectors(x, x, x, 1e-6, 0.0, 0.0, 15e-9)
     assert res == -2
+
+
+# ---------------------------------------------------------------- GMRES alternative
+@pytest.mark.parametrize("n,kw", [(256, {}), (2048, {}), (1024, dict(kind=1, depth=1.0)), (96, {})])
+def test_gmres_rhs_matches_oracle(oracle, csh, n, kw):
+    # GMRES (tol 1e-12 relative residual) against the oracle's GEPP solve
+    po, pc = oracle.Props(**kw), csh.Props(**kw)
+    Z = _surface(n, 40 + n, amp=0.05)
+    Phi = 0.01 * np.sin(2 * np.pi * np.arange(n) / n + 0.3)
+    q, dp, d = oracle.rhs(po, Z, Phi, diagnostics=True)
+    with csh.Ensemble([n], pc, solver=csh.GMRES, tol=1e-13, max_iter=40) as e:
+        e.set_state(Z, Phi)
+        qg, dpg, dg = e.rhs(diagnostics=True)
+        st = e.stats()
+    print(f"gmres n={n} {kw}: iterations {st['last_iterations']}", flush=True)
+    assert st["unconverged"] == 0 and st["last_iterations"] > 0
+    _close(dg["mu"], d["mu"], 1e-11)
+    _close(qg, q, 1e-11)
+    _close(dpg, dp, 1e-11)
+
+
+def test_gmres_rk4_matches_oracle_and_warm_starts(oracle, csh):
+    n = 512
+    Z0 = _surface(n, 77)
+    Zo, Po = oracle.rk4(oracle.Props(), Z0, np.zeros(n), 0.002, 10)
+    with csh.Ensemble([n], solver=csh.GMRES, tol=1e-12, max_iter=24) as e:
+        e.set_state(Z0, np.zeros(n))
+        e.step(0.002, 10)
+        Zg, Pg = e.get_state()
+        st = e.stats()
+    print("gmres rk4 warm-start iterations", st["last_iterations"], st["max_iterations"])
+    assert st["unconverged"] == 0 and st["max_iterations"] <= 12
+    assert np.abs(Zg - Zo).max() < 1e-12
+    assert np.abs(Pg - Po).max() < 1e-12
