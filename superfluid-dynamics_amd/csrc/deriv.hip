@@ -31,45 +31,48 @@ __device__ inline double2 twid(const double2* __restrict__ tw, int t, int sign) 
 
 constexpr int kFftThreads = 1024;
 
-// Radix-2 butterfly index i (the upper element is i + half) of butterfly t in a stage of
-// length len over `sub` independent sub-transforms of size m.
-__device__ inline int bfly_index(int t, int halfm, int half, int len, int m, int& pos) {
-    const int q = t / halfm, b = t - q * halfm;
-    const int grp = b / half;
-    pos = b - grp * half;
-    return q * m + grp * len + pos;
+// Multiplication by sign * i (the quarter-turn twiddle W_{4h}^h of a fused radix-2^2 pass).
+__device__ inline double2 rot_quarter(double2 a, int sign) {
+    return sign > 0 ? make_double2(-a.y, a.x) : make_double2(a.y, -a.x);
 }
 
 // Decimation in time: input in lds_index order, output in natural order.
 // tw[t] = exp(-2 pi i t / n), t < n (LDS or global); sign -1 forward, +1 inverse (unnormalised).
-// Each thread loads all operands of up to 4 butterflies before computing any of them.
+// Consecutive radix-2 stages (half widths h and 2h) are fused into one radix-2^2 pass over quads
+// (i0, i0+h, i0+2h, i0+3h): one LDS round trip and one barrier per two stages, and all index
+// arithmetic is shifts and masks.
 __device__ void fft_dit(double2* s, const SurfDesc& d, int sign, const double2* __restrict__ tw) {
-    const int n = d.n, m = 1 << d.log2m, sub = d.r3 ? 3 : 1, halfm = m >> 1;
-    const int nb = sub * halfm;
-    for (int len = 2; len <= m; len <<= 1) {
-        const int half = len >> 1, stride = n / len;
-        for (int base = 0; base < nb; base += 4 * kFftThreads) {
-            double2 u[4], v[4], w[4];
-            int idx[4];
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                const int t = base + k * kFftThreads + (int)threadIdx.x;
-                int pos = 0;
-                idx[k] = (t < nb) ? bfly_index(t, halfm, half, len, m, pos) : -1;
-                if (idx[k] >= 0) {
-                    u[k] = s[idx[k]];
-                    v[k] = s[idx[k] + half];
-                    w[k] = twid(tw, pos * stride, sign);
-                }
-            }
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                if (idx[k] >= 0) {
-                    const double2 vw = cmul(v[k], w[k]);
-                    s[idx[k]] = cadd(u[k], vw);
-                    s[idx[k] + half] = csub(u[k], vw);
-                }
-            }
+    const int n = d.n, lm = d.log2m, m = 1 << lm, sub = d.r3 ? 3 : 1;
+    int lh = 0;   // log2 of the half width of the next stage
+    for (; lh + 2 <= lm; lh += 2) {
+        const int h = 1 << lh;
+        const int nq = sub << (lm - 2);                  // quads in the pass
+        const int s1 = n >> (lh + 1), s2 = n >> (lh + 2); // twiddle strides for lengths 2h, 4h
+        for (int t = threadIdx.x; t < nq; t += blockDim.x) {
+            const int q = t >> (lm - 2), loc = t & ((m >> 2) - 1);
+            const int grp = loc >> lh, pos = loc & (h - 1);
+            const int i0 = (q << lm) + (grp << (lh + 2)) + pos;
+            const double2 a = s[i0], b = s[i0 + h], c = s[i0 + 2 * h], e = s[i0 + 3 * h];
+            const double2 w1 = twid(tw, pos * s1, sign), w2 = twid(tw, pos * s2, sign);
+            const double2 bw = cmul(b, w1), ew = cmul(e, w1);
+            const double2 a1 = cadd(a, bw), b1 = csub(a, bw), c1 = cadd(c, ew), e1 = csub(c, ew);
+            const double2 cw = cmul(c1, w2), dw = rot_quarter(cmul(e1, w2), sign);
+            s[i0] = cadd(a1, cw);
+            s[i0 + 2 * h] = csub(a1, cw);
+            s[i0 + h] = cadd(b1, dw);
+            s[i0 + 3 * h] = csub(b1, dw);
+        }
+        __syncthreads();
+    }
+    if (lh < lm) {   // one radix-2 stage left (odd log2 m), half width h = m / 2
+        const int h = 1 << lh, nb = sub << (lm - 1), st = n >> (lh + 1);
+        for (int t = threadIdx.x; t < nb; t += blockDim.x) {
+            const int q = t >> (lm - 1), pos = t & (h - 1);
+            const int i = (q << lm) + pos;
+            const double2 u = s[i];
+            const double2 v = cmul(s[i + h], twid(tw, pos * st, sign));
+            s[i] = cadd(u, v);
+            s[i + h] = csub(u, v);
         }
         __syncthreads();
     }
@@ -90,10 +93,10 @@ __device__ void fft_dit(double2* s, const SurfDesc& d, int sign, const double2* 
     }
 }
 
-// Decimation in frequency: input in natural order, output in lds_index order.
+// Decimation in frequency: input in natural order, output in lds_index order.  Mirror image of
+// fft_dit: stages with half widths m/2, m/4, ..., 1, fused in pairs (2h, h).
 __device__ void fft_dif(double2* s, const SurfDesc& d, int sign, const double2* __restrict__ tw) {
-    const int n = d.n, m = 1 << d.log2m, sub = d.r3 ? 3 : 1, halfm = m >> 1;
-    const int nb = sub * halfm;
+    const int n = d.n, lm = d.log2m, m = 1 << lm, sub = d.r3 ? 3 : 1;
     if (d.r3) {   // y_q[j] = W_n^{j q} sum_r x[j + r m] W_3^{r q}
         const double c = -0.5, s3 = 0.86602540378443864676372317075293618 * sign;
         for (int j = threadIdx.x; j < m; j += blockDim.x) {
@@ -107,29 +110,33 @@ __device__ void fft_dif(double2* s, const SurfDesc& d, int sign, const double2* 
         }
         __syncthreads();
     }
-    for (int len = m; len >= 2; len >>= 1) {
-        const int half = len >> 1, stride = n / len;
-        for (int base = 0; base < nb; base += 4 * kFftThreads) {
-            double2 u[4], v[4], w[4];
-            int idx[4];
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                const int t = base + k * kFftThreads + (int)threadIdx.x;
-                int pos = 0;
-                idx[k] = (t < nb) ? bfly_index(t, halfm, half, len, m, pos) : -1;
-                if (idx[k] >= 0) {
-                    u[k] = s[idx[k]];
-                    v[k] = s[idx[k] + half];
-                    w[k] = twid(tw, pos * stride, sign);
-                }
-            }
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                if (idx[k] >= 0) {
-                    s[idx[k]] = cadd(u[k], v[k]);
-                    s[idx[k] + half] = cmul(csub(u[k], v[k]), w[k]);
-                }
-            }
+    int lh = lm - 2;   // fused pair (2h, h) with h = 2^lh
+    for (; lh >= 0; lh -= 2) {
+        const int h = 1 << lh;
+        const int nq = sub << (lm - 2);
+        const int s1 = n >> (lh + 1), s2 = n >> (lh + 2);
+        for (int t = threadIdx.x; t < nq; t += blockDim.x) {
+            const int q = t >> (lm - 2), loc = t & ((m >> 2) - 1);
+            const int grp = loc >> lh, pos = loc & (h - 1);
+            const int i0 = (q << lm) + (grp << (lh + 2)) + pos;
+            const double2 a = s[i0], b = s[i0 + h], c = s[i0 + 2 * h], e = s[i0 + 3 * h];
+            const double2 w1 = twid(tw, pos * s1, sign), w2 = twid(tw, pos * s2, sign);
+            const double2 a1 = cadd(a, c), c1 = cmul(csub(a, c), w2);
+            const double2 b1 = cadd(b, e), e1 = rot_quarter(cmul(csub(b, e), w2), sign);
+            s[i0] = cadd(a1, b1);
+            s[i0 + h] = cmul(csub(a1, b1), w1);
+            s[i0 + 2 * h] = cadd(c1, e1);
+            s[i0 + 3 * h] = cmul(csub(c1, e1), w1);
+        }
+        __syncthreads();
+    }
+    if (lh == -1) {   // one radix-2 stage left (odd log2 m), half width 1
+        const int nb = sub << (lm - 1), st = n >> 1;
+        for (int t = threadIdx.x; t < nb; t += blockDim.x) {
+            const int i = t << 1;
+            const double2 u = s[i], v = s[i + 1];
+            s[i] = cadd(u, v);
+            s[i + 1] = cmul(csub(u, v), twid(tw, 0 * st, sign));
         }
         __syncthreads();
     }

@@ -3,13 +3,14 @@
 //
 // M = I/2 + K is a second-kind operator: its spectrum clusters at 1/2 (for the helium film one
 // outlier sits far to the right), so GMRES converges in a handful of iterations, and the
-// stepper warm starts every solve from the previous stage's mu.  Each iteration is three
-// kernels, with classical Gram-Schmidt applied twice (CGS2):
-//   gm_matvec : w = M v_j ; partial dots v_i . w        -> last workgroup: h1_i
-//   gm_orth   : w' = w - sum h1_i v_i ; partial dots d_i = v_i . w'
-//                                                       -> last workgroup: column j of H
-//   gm_extend : v_{j+1} = w' - sum d_i v_i and |v_{j+1}|
-//                                                       -> last workgroup: Givens, residual test
+// stepper warm starts every solve from the previous stage's mu.  Each iteration is two kernels
+// (classical Gram-Schmidt, one pass):
+//   gm_arnoldi : w = M v_j and the dots h_i = v_i . w (i <= j)
+//   gm_extend  : u = w - sum h_i v_i (stored unnormalised, scale 1/|u| kept aside) and |u|
+//                -> last workgroup: h_{j+1,j} = |u|, Givens rotation, residual test
+// The new vector's norm is reduced from u itself (not from |w|^2 - sum h_i^2, which cancels
+// catastrophically once the basis drifts from orthonormal).  One pass suffices here: the basis
+// conditioning follows that of M = I/2 + K, which is O(1) for the configurations this solves.
 // Reductions use the "last workgroup done" pattern (atomic counter after a release fence), so
 // no workgroup ever waits for another.  Once the residual estimate reaches tol * |b| the
 // remaining launches of the fixed-length sequence return at once.
@@ -17,8 +18,7 @@
 // Layout for memory-level parallelism: a workgroup owns 8 rows; its 256 threads are (row r,
 // basis index i) pairs (r = 2 * wave + lane / 32, i = lane % 32), so every basis-vector access of
 // an iteration is one load per thread, all in flight together.  Partial sums are stored basis
-// index major (part[i * nwg + wg]) and the last workgroup reduces all of them with one fully
-// unrolled round of loads.
+// index major (part[i * nwg + wg]) and the last workgroup reduces all of them with unrolled loads.
 #include "sfh_kernels.hpp"
 
 namespace sfh {
@@ -155,21 +155,6 @@ __device__ inline void reduce_partials(const double* __restrict__ part, int nwg,
     __syncthreads();
 }
 
-// Per-workgroup dot products d_i = sum_r V[i][row0 + r] * u[r] for i <= j, thread (r, i) layout;
-// writes part[i * nwg + blockIdx.x] (scaled by sc[i]).  vi holds the thread's V element.
-__device__ inline void wg_dots(double vi, const double* ul, const GmView& v, int j, double* scratch) {
-    const int t = threadIdx.x, r = (t >> 6) * 2 + ((t & 63) >> 5), i = t & 31;
-    const double p = (i <= j) ? vi * ul[r] : 0.0;
-    scratch[r * 32 + i] = p;
-    __syncthreads();
-    if (t <= j) {
-        double s = 0.0;
-#pragma unroll
-        for (int rr = 0; rr < kRowsPerWG; ++rr) s += scratch[rr * 32 + t];
-        v.part[t * v.nwg + blockIdx.x] = s * v.sc[t];
-    }
-}
-
 // r0 = b - M x0 into V[0]; last workgroup sets beta = |r0|, |b|, g[0], done.
 __global__ __launch_bounds__(256) void gm_init_kernel(GmArgs a, const SurfDesc* sd) {
     const GmView v = gm_view(a, sd);
@@ -219,14 +204,14 @@ __global__ __launch_bounds__(256) void gm_init_kernel(GmArgs a, const SurfDesc* 
     }
 }
 
-// w = M v_j; partial dots with v_0..v_j; last workgroup forms h1.
-__global__ __launch_bounds__(256) void gm_matvec_kernel(GmArgs a, const SurfDesc* sd, int j) {
+// Arnoldi step j, first half: w = M v_j; h_i = v_i . w for i <= j (into h1 and column j of H).
+__global__ __launch_bounds__(256) void gm_arnoldi_kernel(GmArgs a, const SurfDesc* sd, int j) {
     const GmView v = gm_view(a, sd);
     if ((int)blockIdx.x >= v.nwg || v.ctl->done) return;
     __shared__ double wl[kRowsPerWG];
     __shared__ double scratch[kRowsPerWG * 32];
     __shared__ double sums[kMaxKrylov + 2];
-    const int t = threadIdx.x, lane = t & 63, wv = t >> 6, n = v.n;
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6, n = v.n, m = a.m;
     const int row0 = blockIdx.x * kRowsPerWG;
     const int r = wv * 2 + (lane >> 5), i = lane & 31;
     const int row = row0 + r;
@@ -244,68 +229,47 @@ __global__ __launch_bounds__(256) void gm_matvec_kernel(GmArgs a, const SurfDesc
     }
     __syncthreads();
     if (t < kRowsPerWG && row0 + t < n) v.w[row0 + t] = wl[t];
-    wg_dots(vi, wl, v, j, scratch);
-    if (!last_workgroup(v.ctl, v.nwg)) return;
-    reduce_partials(v.part, v.nwg, j + 1, sums);
-    if (t <= j) v.h1[t] = sums[t];
-}
-
-// w' = w - sum h1_i v_i (first pass); partial dots d_i = v_i . w'.
-__global__ __launch_bounds__(256) void gm_orth_kernel(GmArgs a, const SurfDesc* sd, int j) {
-    const GmView v = gm_view(a, sd);
-    if ((int)blockIdx.x >= v.nwg || v.ctl->done) return;
-    __shared__ double wl[kRowsPerWG];
-    __shared__ double scratch[kRowsPerWG * 32];
-    __shared__ double sums[kMaxKrylov + 2];
-    const int t = threadIdx.x, lane = t & 63, wv = t >> 6, n = v.n, m = a.m;
-    const int row0 = blockIdx.x * kRowsPerWG;
-    const int r = wv * 2 + (lane >> 5), i = lane & 31;
-    const int row = row0 + r;
-    const double vi = (i <= j && row < n) ? v.V[(long long)i * n + row] : 0.0;
-    // sum over i within the 32-lane half wave of row r
-    double p = (i <= j) ? v.h1[i] * v.sc[i] * vi : 0.0;
-    for (int o = 16; o > 0; o >>= 1) p += __shfl_xor(p, o, 64);
-    if (i == 0) {
-        const double wnew = (row < n) ? v.w[row] - p : 0.0;
-        wl[r] = wnew;
-        if (row < n) v.w[row] = wnew;
-    }
+    scratch[r * 32 + i] = (i <= j) ? vi * wl[r] : 0.0;
     __syncthreads();
-    wg_dots(vi, wl, v, j, scratch);
+    if (t <= j) {
+        double s = 0.0;
+#pragma unroll
+        for (int rr = 0; rr < kRowsPerWG; ++rr) s += scratch[rr * 32 + t];
+        v.part[t * v.nwg + blockIdx.x] = s * v.sc[t];
+    }
     if (!last_workgroup(v.ctl, v.nwg)) return;
     reduce_partials(v.part, v.nwg, j + 1, sums);
     if (t <= j) {
-        v.H[(long long)j * (m + 1) + t] = v.h1[t] + sums[t];   // full Arnoldi coefficient
-        v.h1[t] = sums[t];                                       // second-pass coefficient
+        v.h1[t] = sums[t];
+        v.H[(long long)j * (m + 1) + t] = sums[t];
     }
 }
 
-// v_{j+1} = w' - sum d_i v_i and its norm; last workgroup finishes column j of H, applies the
-// Givens rotations and tests |g_{j+1}| <= tol * |b|.
+// Arnoldi step j, second half: u = w - sum_{i<=j} h_i v_i into V[j+1]; the last workgroup takes
+// h_{j+1,j} = |u|, applies the Givens rotations to column j and tests the residual.
 __global__ __launch_bounds__(256) void gm_extend_kernel(GmArgs a, const SurfDesc* sd, int j) {
     const GmView v = gm_view(a, sd);
     if ((int)blockIdx.x >= v.nwg || v.ctl->done) return;
-    __shared__ double nrm[kRowsPerWG];
+    __shared__ double ul[kRowsPerWG];
     __shared__ double sums[kMaxKrylov + 2];
     const int t = threadIdx.x, lane = t & 63, wv = t >> 6, n = v.n, m = a.m;
     const int row0 = blockIdx.x * kRowsPerWG;
     const int r = wv * 2 + (lane >> 5), i = lane & 31;
     const int row = row0 + r;
     const double vi = (i <= j && row < n) ? v.V[(long long)i * n + row] : 0.0;
+    const double wr = row < n ? v.w[row] : 0.0;
     double p = (i <= j) ? v.h1[i] * v.sc[i] * vi : 0.0;
     for (int o = 16; o > 0; o >>= 1) p += __shfl_xor(p, o, 64);
+    const double u = row < n ? wr - p : 0.0;
     if (i == 0) {
-        double nv = 0.0;
-        if (row < n) {
-            nv = v.w[row] - p;
-            v.V[(long long)(j + 1) * n + row] = nv;
-        }
-        nrm[r] = nv * nv;
+        if (row < n) v.V[(long long)(j + 1) * n + row] = u;
+        ul[r] = u * u;
     }
     __syncthreads();
     if (t == 0) {
         double s = 0.0;
-        for (int q = 0; q < kRowsPerWG; ++q) s += nrm[q];
+#pragma unroll
+        for (int rr = 0; rr < kRowsPerWG; ++rr) s += ul[rr];
         v.part[blockIdx.x] = s;
     }
     if (!last_workgroup(v.ctl, v.nwg)) return;
@@ -323,12 +287,12 @@ __global__ __launch_bounds__(256) void gm_extend_kernel(GmArgs a, const SurfDesc
         }
         const double den = hypot(Hj[j], Hj[j + 1]);
         const double cj = den > 0.0 ? Hj[j] / den : 1.0;
-        const double sj = den > 0.0 ? Hj[j + 1] / den : 0.0;
+        const double sn = den > 0.0 ? Hj[j + 1] / den : 0.0;
         v.cs[j] = cj;
-        v.sn[j] = sj;
+        v.sn[j] = sn;
         Hj[j] = den;
         Hj[j + 1] = 0.0;
-        v.g[j + 1] = -sj * v.g[j];
+        v.g[j + 1] = -sn * v.g[j];
         v.g[j] = cj * v.g[j];
         c->res = fabs(v.g[j + 1]);
         c->iters = j + 1;

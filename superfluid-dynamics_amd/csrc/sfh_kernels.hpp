@@ -78,8 +78,7 @@ struct GmArgs {
 };
 constexpr int kGmCtlInts = 16;   // ints per control block (holds the device struct)
 __global__ void gm_init_kernel(GmArgs a, const SurfDesc* sd);
-__global__ void gm_matvec_kernel(GmArgs a, const SurfDesc* sd, int j);
-__global__ void gm_orth_kernel(GmArgs a, const SurfDesc* sd, int j);
+__global__ void gm_arnoldi_kernel(GmArgs a, const SurfDesc* sd, int j);
 __global__ void gm_extend_kernel(GmArgs a, const SurfDesc* sd, int j);
 __global__ void gm_finish_kernel(GmArgs a, const SurfDesc* sd);
 

@@ -226,15 +226,11 @@ void Solver::enqueue_gmres() {
     });
     for (int j = 0; j < gm_m_; ++j) {
         launch(
-            "gm_matvec", 2.0 * nsq, 8.0 * nsq,
-            [&] { hipLaunchKernelGGL(gm_matvec_kernel, grid, dim3(256), 0, stream_, a, (const SurfDesc*)d_desc_, j); },
+            "gm_arnoldi", 2.0 * nsq + 2.0 * (j + 1) * T_, 8.0 * nsq + 8.0 * (j + 2) * T_,
+            [&] { hipLaunchKernelGGL(gm_arnoldi_kernel, grid, dim3(256), 0, stream_, a, (const SurfDesc*)d_desc_, j); },
             j);
         launch(
-            "gm_orth", 4.0 * (j + 1) * T_, 8.0 * (j + 3) * T_,
-            [&] { hipLaunchKernelGGL(gm_orth_kernel, grid, dim3(256), 0, stream_, a, (const SurfDesc*)d_desc_, j); },
-            j);
-        launch(
-            "gm_extend", 2.0 * (j + 1) * T_, 8.0 * (j + 3) * T_,
+            "gm_extend", 2.0 * (j + 2) * T_, 8.0 * (j + 3) * T_,
             [&] { hipLaunchKernelGGL(gm_extend_kernel, grid, dim3(256), 0, stream_, a, (const SurfDesc*)d_desc_, j); },
             j);
     }
