@@ -104,7 +104,12 @@ typedef struct sfh_solver* sfh_handle;
 
 enum { SFH_WATER = 0, SFH_HELIUM = 1 };
 enum { SFH_FP64 = 0, SFH_MIXED = 1 };
-enum { SFH_SOLVE_LU = 0, SFH_SOLVE_GMRES = 1 };
+/* LU: blocked FP64 LU (the reference's getrf/getrs semantics), one surface at a time or uniform
+ * batches.  GMRES: FP64 GMRES over the stored matrix, all surfaces of the handle at once.
+ * BATCHED: one persistent launch per RK stage, one workgroup per surface (every N <= 1024), GMRES
+ * inside the workgroup; with SFH_MIXED the matrix is FP32 and mu is refined in FP64 against a
+ * matrix-free FP64 residual. */
+enum { SFH_SOLVE_LU = 0, SFH_SOLVE_GMRES = 1, SFH_SOLVE_BATCHED = 2 };
 
 /* Nondimensional problem (ProblemProperties.hpp:5-32 plus kind and g).  g scales the restoring
  * term of the Bernoulli equation (1 reproduces the reference).  rho_quirk = 1 reproduces the
@@ -124,8 +129,8 @@ typedef struct sfh_props {
 } sfh_props;
 
 typedef struct sfh_options {
-    int precision;    /* SFH_FP64 (the drop-in arithmetic) or SFH_MIXED */
-    int solver;       /* SFH_SOLVE_LU or SFH_SOLVE_GMRES */
+    int precision;    /* SFH_FP64 (the drop-in arithmetic) or SFH_MIXED (SFH_SOLVE_BATCHED only) */
+    int solver;       /* SFH_SOLVE_LU, SFH_SOLVE_GMRES or SFH_SOLVE_BATCHED */
     int use_graph;    /* replay RK4 steps from a captured hipGraph */
     int max_iter;     /* iterative solver / refinement iteration cap */
     int restart;      /* GMRES restart length */
@@ -141,7 +146,7 @@ typedef struct sfh_stats {
     int last_iterations;   /* GMRES: iterations of surface 0 in the last solve */
     int max_iterations;    /* GMRES: most iterations any surface needed in the last solve */
     int unconverged;       /* GMRES: solves that reached the iteration cap (accumulated) */
-    int reserved;
+    int refinements;       /* mixed precision: FP64 refinement sweeps of surface 0, last solve */
 } sfh_stats;
 
 /* Defaults: FP64, LU, graph replay on, tol 1e-12. */

@@ -381,6 +381,7 @@ int sfh_get_stats(sfh_handle h, sfh_stats* out) {
         out->last_iterations = st.last_iterations;
         out->max_iterations = st.max_iterations;
         out->unconverged = st.unconverged;
+        out->refinements = st.refinements;
     });
 }
 

@@ -33,7 +33,9 @@ struct GmCtl {
     int unconverged;
     double bnorm, beta, res;
     double hnext;
+    int pad[4];
 };
+static_assert(sizeof(GmCtl) == kGmCtlInts * sizeof(int), "host reads control blocks with this stride");
 
 struct GmView {
     const double* M;
@@ -104,18 +106,23 @@ __device__ inline void rows_dot2(const double* __restrict__ R0, const double* __
     d1 = wave_sum(b0 + b1);
 }
 
-// Returns true on the workgroup that finished last (the counter is then re-armed).
+// Returns true on the workgroup that finished last (the counter is then re-armed).  Every
+// workgroup increments with release semantics (its partial sums become visible at agent scope);
+// only the last one pays for the acquire.  On gfx950 an agent-scope acquire invalidates the L2 of
+// the executing XCD, so a full fence in every workgroup would evict the data other workgroups are
+// streaming.
 __device__ inline bool last_workgroup(GmCtl* ctl, int nwg) {
     __shared__ int is_last;
     __syncthreads();
     if (threadIdx.x == 0) {
-        __threadfence();
-        const int prev = atomicAdd(&ctl->counter, 1);
+        const int prev = __hip_atomic_fetch_add(&ctl->counter, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
         is_last = (prev == nwg - 1);
-        if (is_last) ctl->counter = 0;
+        if (is_last) {
+            __hip_atomic_store(&ctl->counter, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        }
     }
     __syncthreads();
-    if (is_last) __threadfence();
     return is_last != 0;
 }
 

@@ -324,18 +324,20 @@ __global__ __launch_bounds__(256) void lu_backsolve_kernel(LuArgs a) {
         const int C0 = q * kNB, cb = min(kNB, n - C0);
         load_block(Ub, v.A + (long long)R0 * n + C0, n, rb, cb);
         if (t == 0) {
+            // relaxed polling, one acquire once the flag is seen: on gfx950 every agent-scope
+            // acquire invalidates this XCD's L2, which other workgroups are reading through
             int spins = 0;
-            while (__hip_atomic_load(&v.flags[q], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == 0) {
+            while (__hip_atomic_load(&v.flags[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) {
                 __builtin_amdgcn_s_sleep(1);
                 if (++spins > (1 << 22)) {
                     atomicOr(&v.status[1], 1);
                     break;
                 }
             }
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         }
         __syncthreads();
         if (t < kNB) {
-            (void)__hip_atomic_load(&v.flags[q], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
             xq[t] = (t < cb) ? v.x[C0 + t] : 0.0;
         }
         __syncthreads();

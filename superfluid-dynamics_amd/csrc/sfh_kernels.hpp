@@ -82,4 +82,28 @@ __global__ void gm_arnoldi_kernel(GmArgs a, const SurfDesc* sd, int j);
 __global__ void gm_extend_kernel(GmArgs a, const SurfDesc* sd, int j);
 __global__ void gm_finish_kernel(GmArgs a, const SurfDesc* sd);
 
+// ---- batched.hip (persistent per-surface stage kernel, ensembles with every N <= kBatchMaxN)
+constexpr int kBatchThreads = 1024;
+constexpr int kBatchMaxN = 1024;
+struct BatchArgs {
+    const SurfDesc* sd;
+    const int* order;          // workgroup -> surface, by decreasing N
+    const double2* tw_all;
+    const double2* Zin;        // stage input state
+    const double* Pin;
+    void* M;                   // per-surface column-major matrices (double, or float when mixed)
+    double2* Zp;
+    double2* Zpp;
+    double2* mup;
+    double* b;
+    double* mu;                // warm start in, solution out
+    int* stats;                // per surface: [iterations, unconverged solves, refinement sweeps, -]
+    double tol;
+    int mixed;
+    int cycles;                // GMRES restart cycles (FP64)
+    int max_refine;            // refinement sweeps (mixed)
+    int pad;
+};
+__global__ void batched_stage_kernel(BatchArgs a, Phys ph, StageCtl ctl);
+
 }  // namespace sfh

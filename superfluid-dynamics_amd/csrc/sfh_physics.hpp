@@ -1,0 +1,84 @@
+// sfh_physics.hpp -- per-node physics shared by the fused velocity kernel and the persistent
+// per-surface stage kernel: velocity from the cot sums, Bernoulli right-hand side, RK4 stage update.
+#pragma once
+#include "sfh_kernels.hpp"
+
+namespace sfh {
+
+__device__ inline double bernoulli(const Phys& ph, double y, double2 q1, double2 q2, double2 q1_quirk,
+                                   double2 zp, double2 zpp) {
+    if (ph.kind == 0) {
+        const double a1 = q1.x * q1.x + q1.y * q1.y;
+        const double a2 = q2.x * q2.x + q2.y * q2.y;
+        const double dot = q1_quirk.x * q2.x + q1.y * q2.y;    // createM.cuh:104 reads V1[1].real()
+        return -(1 + ph.rho) * ph.g * y + 0.5 * a1 + 0.5 * ph.rho * a2 - ph.rho * dot;
+    }
+    const double h = ph.depth;
+    if (ph.use_exp) {
+        const double kin = 0.5 * q1.x * q1.x + 0.5 * q1.y * q1.y;
+        double v = 0.0;
+        switch (ph.exp_order) {
+        case 3: v += -10.0 / 3.0 * (y * y * y) / (h * h); [[fallthrough]];
+        case 2: v += 2.0 * (y * y) / h; [[fallthrough]];
+        case 1: v += -y; [[fallthrough]];
+        default: break;
+        }
+        return ph.g * v + kin;
+    }
+    if (ph.kappa != 0.0) {
+        const double a1 = q1.x * q1.x + q1.y * q1.y;
+        const double s = zp.x * zp.x + zp.y * zp.y;
+        const double curv = (zp.x * zpp.y - zp.y * zpp.x) / (s * sqrt(s));
+        const double u = 1.0 + y / h;
+        return ph.g * (20.447761896665416 * h / 3.0 * (1.0 / (u * u * u) - 1)) + 0.5 * a1 + ph.kappa * curv;
+    }
+    const double vdw = ph.g * (h / 3.0);
+    const double u = 1.0 + y / h;
+    return vdw / (u * u * u) - vdw + 0.5 * q1.x * q1.x + 0.5 * q1.y * q1.y;
+}
+
+// Velocities at node k from the cot sums over j != k
+//   S = sum_j mu_j cot((z_k - z_j)/2) = sx + i sy,  Si = sum_j mu_j cot((z_k - conj z_j)/2 + i h) = ix + i iy
+//   w = V2_k mu'_k + V1_kk mu_k + i (-1/4pi) S [+ i (1/4pi) Si]
+//   V2_k = i / (2 pi Z'_k),  V1_kk = i (-1/4pi) Z''_k / Z'_k^2 [+ i (1/4pi) cot(i (y_k + h))] +- 1/(2 Z'_k)
+// q1 (lower fluid, + sign) and q2 (upper fluid, - sign) are conj(w) (WaterVelocities.cuh:38-107).
+__device__ inline void velocity_from_sums(const Phys& ph, double2 zp, double2 zpp, double2 mup, double mk,
+                                          double yk, double sx, double sy, double ix, double iy,
+                                          double2& q1, double2& q2) {
+    const double q4 = 1.0 / (4.0 * kPi);
+    double2 w = ctimes_i(make_double2(-q4 * sx, -q4 * sy));
+    if (ph.image) w = cadd(w, ctimes_i(make_double2(q4 * ix, q4 * iy)));
+    double2 v1 = ctimes_i(cscale(cdiv(zpp, cmul(zp, zp)), -q4));
+    if (ph.image) v1 = cadd(v1, ctimes_i(cscale(cot64(0.0, yk + ph.depth), q4)));
+    const double2 half_inv = cscale(crecip(zp), 0.5);
+    const double2 v2 = ctimes_i(crecip(cscale(zp, 2.0 * kPi)));
+    const double2 base = cadd(cadd(cmul(v2, mup), w), cscale(v1, mk));
+    q1 = cconj(cadd(base, cscale(half_inv, mk)));
+    q2 = cconj(csub(base, cscale(half_inv, mk)));
+}
+
+// RK4 stage bookkeeping of node g (AutonomousRungeKuttaStepper.cuh:338-375, utilities.cuh:78-83).
+__device__ inline void stage_update(const StageCtl& ctl, int g, double2 q1, double dphi) {
+    if (ctl.stage == 0) {
+        ctl.kZ[0][g] = q1;
+        ctl.kP[0][g] = dphi;
+    } else if (ctl.stage <= 3) {
+        ctl.kZ[ctl.stage - 1][g] = q1;
+        ctl.kP[ctl.stage - 1][g] = dphi;
+        const double2 z0 = ctl.Z0[g];
+        ctl.Zout[g] = make_double2(z0.x + ctl.alpha * q1.x, z0.y + ctl.alpha * q1.y);
+        ctl.Pout[g] = ctl.P0[g] + ctl.alpha * dphi;
+    } else {
+        const double2 k1 = ctl.kZ[0][g], k2 = ctl.kZ[1][g], k3 = ctl.kZ[2][g];
+        const double Kx = k1.x + 2.0 * k2.x + 2.0 * k3.x + q1.x;
+        const double Ky = k1.y + 2.0 * k2.y + 2.0 * k3.y + q1.y;
+        const double Kp = ctl.kP[0][g] + 2.0 * ctl.kP[1][g] + 2.0 * ctl.kP[2][g] + dphi;
+        double2* z0 = const_cast<double2*>(ctl.Z0) + g;
+        double* p0 = const_cast<double*>(ctl.P0) + g;
+        const double2 zz = *z0;
+        *z0 = make_double2(zz.x + ctl.alpha * Kx, zz.y + ctl.alpha * Ky);
+        *p0 = *p0 + ctl.alpha * Kp;
+    }
+}
+
+}  // namespace sfh

@@ -53,8 +53,14 @@ Solver::Solver(const Phys& ph, const std::vector<int>& n, const Options& opt) : 
     S_ = (int)n.size();
     for (int v : n)
         if (!supported_n(v)) throw ArgError("unsupported node count " + std::to_string(v));
-    if (opt_.precision != kFP64) throw ArgError("requested precision form is not available");
-    if (opt_.solver != kSolveLU && opt_.solver != kSolveGMRES) throw ArgError("unknown solver");
+    if (opt_.solver != kSolveLU && opt_.solver != kSolveGMRES && opt_.solver != kSolveBatched)
+        throw ArgError("unknown solver");
+    if (opt_.precision != kFP64 && opt_.precision != kMixed) throw ArgError("unknown precision");
+    if (opt_.precision == kMixed && opt_.solver != kSolveBatched)
+        throw ArgError("the mixed-precision form runs on the batched (per-surface workgroup) solver");
+    if (opt_.solver == kSolveBatched)
+        for (int v : n)
+            if (v > kBatchMaxN) throw ArgError("the batched solver takes surfaces of at most 1024 nodes");
     SFH_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
 
     // descriptors and twiddle tables (one table per distinct n, computed in long double)
@@ -145,6 +151,15 @@ Solver::Solver(const Phys& ph, const std::vector<int>& n, const Options& opt) : 
         SFH_HIP(hipMemset(gmctl_, 0, sizeof(int) * kGmCtlInts * S_));
         SFH_HIP(hipMemset(gmsmall_, 0, sizeof(double) * gm_small_stride_ * S_));
     }
+    if (opt_.solver == kSolveBatched) {
+        std::vector<int> ord(S_);
+        for (int s = 0; s < S_; ++s) ord[s] = s;
+        std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return n_[a] > n_[b]; });
+        order_ = dalloc<int>(S_);
+        SFH_HIP(hipMemcpy(order_, ord.data(), S_ * sizeof(int), hipMemcpyHostToDevice));
+        bstats_ = dalloc<int>(4 * (size_t)S_);
+        SFH_HIP(hipMemset(bstats_, 0, 4 * S_ * sizeof(int)));
+    }
     SFH_HIP(hipDeviceSynchronize());
 }
 
@@ -154,7 +169,7 @@ Solver::~Solver() {
     if (stream_) (void)hipStreamSynchronize(stream_);
     void* bufs[] = {d_desc_, d_tw_, Z0_, Za_, Zb_, P0_, Pa_, Pb_, kZ_[0], kZ_[1], kZ_[2], kP_[0],
                     kP_[1], kP_[2], Zp_, Zpp_, mup_, qup_, b_, work_, y_, mu_, M_, uinv_, flags_,
-                    status_, piv_, counters_, gmV_, gmw_, gmpart_, gmsmall_, gmctl_, trig_};
+                    status_, piv_, counters_, gmV_, gmw_, gmpart_, gmsmall_, gmctl_, trig_, order_, bstats_};
     for (void* p : bufs)
         if (p) (void)hipFree(p);
     for (hipEvent_t e : event_pool_) (void)hipEventDestroy(e);
@@ -298,7 +313,42 @@ void Solver::enqueue_solve(const double2* Z) {
                        status_, counters_);
 }
 
+// One launch for the whole right-hand side (and stage update) of every surface.
+void Solver::enqueue_batched(const double2* Z, const double* P, const StageCtl& ctl) {
+    BatchArgs a{};
+    a.sd = d_desc_;
+    a.order = order_;
+    a.tw_all = d_tw_;
+    a.Zin = Z;
+    a.Pin = P;
+    a.M = M_;
+    a.Zp = Zp_;
+    a.Zpp = Zpp_;
+    a.mup = mup_;
+    a.b = b_;
+    a.mu = mu_;
+    a.stats = bstats_;
+    a.tol = opt_.tol;
+    a.mixed = opt_.precision == kMixed ? 1 : 0;
+    a.cycles = std::max(1, (opt_.max_iter + 7) / 8);   // kBK = 8 vectors per cycle
+    a.max_refine = 6;
+    double nsq = 0.0;
+    for (int v : n_) nsq += (double)v * v;
+    const double img = ph_.image ? 2.0 : 1.0;
+    const double mbytes = (a.mixed ? 4.0 : 8.0) * nsq;
+    // algorithmic work per launch: cot pairs of assembly + velocity (and one more pass per
+    // refinement residual), matrix written once and read once per GMRES product
+    launch("stage", 2.0 * img * nsq, 2.0 * mbytes, [&] {
+        hipLaunchKernelGGL(batched_stage_kernel, dim3(S_), dim3(kBatchThreads), 0, stream_, a, ph_, ctl);
+    });
+    SFH_HIP(hipGetLastError());
+}
+
 void Solver::enqueue_rhs(const double2* Z, const double* P, const StageCtl& ctl) {
+    if (opt_.solver == kSolveBatched) {
+        enqueue_batched(Z, P, ctl);
+        return;
+    }
     // data + twiddles in LDS when both fit (n <= 4096), data only otherwise
     const int lds_tw = (2 * (size_t)maxN_ * sizeof(double2) <= 144 * 1024) ? 1 : 0;
     const size_t lds = (size_t)maxN_ * sizeof(double2) * (lds_tw ? 2 : 1);
@@ -457,6 +507,19 @@ void Solver::check_status() {
         stats_.last_iterations = ctl[2];
         stats_.max_iterations = mx;
         stats_.unconverged = unc;
+    }
+    if (opt_.solver == kSolveBatched) {
+        std::vector<int> bs(4 * (size_t)S_);
+        SFH_HIP(hipMemcpy(bs.data(), bstats_, bs.size() * sizeof(int), hipMemcpyDeviceToHost));
+        int mx = 0, unc = 0;
+        for (int s = 0; s < S_; ++s) {
+            mx = std::max(mx, bs[4 * s]);
+            unc += bs[4 * s + 1];
+        }
+        stats_.last_iterations = bs[0];
+        stats_.max_iterations = mx;
+        stats_.unconverged = unc;
+        stats_.refinements = bs[2];
     }
     if (timeouts) throw HipError("back-substitution wait timed out; results are invalid");
 }

@@ -29,7 +29,7 @@ struct ArgError : std::runtime_error {
 };
 
 enum Precision { kFP64 = 0, kMixed = 1 };
-enum SolveKind { kSolveLU = 0, kSolveGMRES = 1 };
+enum SolveKind { kSolveLU = 0, kSolveGMRES = 1, kSolveBatched = 2 };
 
 struct Options {
     int precision = kFP64;
@@ -48,6 +48,7 @@ struct Stats {
     int last_iterations = 0;   // iterative solvers: iterations of the last solve (surface 0)
     int max_iterations = 0;    // iterative solvers: most iterations any surface used last solve
     int unconverged = 0;       // iterative solves that hit the iteration cap
+    int refinements = 0;       // mixed precision: refinement sweeps of the last solve (surface 0)
 };
 
 bool supported_n(int n);
@@ -145,6 +146,10 @@ class Solver {
     int* gmctl_ = nullptr;
     long long gm_part_stride_ = 0, gm_small_stride_ = 0;
     void enqueue_gmres();
+    // persistent per-surface stage kernel (kSolveBatched)
+    int* order_ = nullptr;    // workgroup -> surface, by decreasing N
+    int* bstats_ = nullptr;   // 4 ints per surface
+    void enqueue_batched(const double2* Z, const double* P, const StageCtl& ctl);
 
     hipGraph_t graph_ = nullptr;
     hipGraphExec_t exec_ = nullptr;

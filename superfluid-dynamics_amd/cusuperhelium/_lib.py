@@ -44,7 +44,7 @@ class SfhOptions(Structure):
 class SfhStats(Structure):
     _fields_ = [("rhs_evals", c_longlong), ("steps", c_longlong), ("fallback_solves", c_int),
                 ("spin_timeouts", c_int), ("last_iterations", c_int), ("max_iterations", c_int),
-                ("unconverged", c_int), ("reserved", c_int)]
+                ("unconverged", c_int), ("refinements", c_int)]
 
 
 DP = POINTER(c_double)

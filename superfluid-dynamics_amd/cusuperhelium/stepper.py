@@ -18,7 +18,7 @@ from ._lib import SfhOptions, SfhProps, SfhStats, check, lib
 
 WATER, HELIUM = 0, 1
 FP64, MIXED = 0, 1
-LU, GMRES = 0, 1
+LU, GMRES, BATCHED = 0, 1, 2
 
 
 def _dp(a):
