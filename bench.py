@@ -1,14 +1,23 @@
-"""Benchmark: RK4 steps per second of one periodic water surface of N = 2048 nodes in FP64.
+"""Benchmark: RK4 steps per second of one periodic water surface of N = 2048 nodes in FP64, plus
+the batched mixed-precision ensemble leg.
 
-Workload: nodes x_j = 2 pi j / N, y = sum_{k=1..8} a_k cos(k x + theta_k) with a_k ~ U(0, 0.02/k),
-theta_k ~ U(0, 2 pi) (numpy default_rng(20261202)), phi = 0, gravity 1, infinite depth,
-dt = 0.002.  A step is one classical RK4 step (four right-hand sides: spectral derivatives,
-dipole-matrix assembly, dense solve, velocity + Bernoulli + stage update), state resident in HBM.
+Headline workload (`value`): nodes x_j = 2 pi j / N, y = sum_{k=1..8} a_k cos(k x + theta_k) with
+a_k ~ U(0, 0.02/k), theta_k ~ U(0, 2 pi) (numpy default_rng(20261202)), phi = 0, gravity 1, infinite
+depth, dt = 0.002.  A step is one classical RK4 step (four right-hand sides: spectral derivatives,
+dipole-matrix assembly, solve, velocity + Bernoulli + stage update), state resident in HBM, steps
+replayed from a captured hipGraph.  Default solver: FP64 GMRES (relative residual 1e-12, warm
+started); --solver lu selects the blocked FP64 LU.
 
-Prints one JSON line.  `roofline` is measured live on the dominant kernel (the FP64 MFMA
-trailing update of the LU) with hipEvents around each of its launches on the solver's stream;
-`cpu_baseline` times the CPU FP64 oracle (OpenMP, all host threads) on a bounded sample of the
-same workload and also serves as the accuracy reference for the same number of steps.
+Batched leg (`batched`): 512 independent surfaces with N drawn uniformly from {128, 256, 512, 1024}
+(default_rng(20261202)), surface i drawn with seed 20261202 + i as above, phi = 0, dt = 0.002; one
+persistent launch per RK stage, mixed precision (FP32 matrix, FP64-refined mu) and FP64.  It is
+compared with the plain FP64 stepper run over the same 512 surfaces one surface at a time:
+ensemble steps/s = 1 / sum_i t_step(N_i).
+
+Prints one JSON line.  `roofline` is measured live on the dominant kernel of the headline solver
+with hipEvents around each of its launches on the solver's stream; `cpu_baseline` times the CPU
+FP64 oracle (OpenMP, all host threads) on a bounded sample of the headline workload and also serves
+as its accuracy reference.
 """
 import argparse
 import json
@@ -29,6 +38,7 @@ PEAK_HBM_GBS = 8000.0
 N = 2048
 DT = 0.002
 SEED = 20261202
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic.json")
 
 
 def surface(n, seed, amp=0.02, modes=8):
@@ -37,6 +47,11 @@ def surface(n, seed, amp=0.02, modes=8):
     a = rng.uniform(0, amp / np.arange(1, modes + 1))
     th = rng.uniform(0, 2 * np.pi, modes)
     return x + 1j * sum(a[k] * np.cos((k + 1) * x + th[k]) for k in range(modes))
+
+
+def ensemble_sizes(count=512, seed=SEED):
+    rng = np.random.default_rng(seed)
+    return [int(v) for v in rng.choice([128, 256, 512, 1024], size=count)]
 
 
 def cpu_baseline(Z0, P0, budget_s=12.0, max_steps=12):
@@ -54,15 +69,102 @@ def cpu_baseline(Z0, P0, budget_s=12.0, max_steps=12):
     return steps, el, Z, P, O.num_threads()
 
 
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC summary, if present."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            return json.load(f).get(kernel)
+    except (OSError, ValueError):
+        return None
+
+
+def timed_steps(e, steps, warmup):
+    e.step(DT, warmup)
+    e.sync()
+    t0 = time.perf_counter()
+    e.step(DT, steps)
+    e.sync()
+    return time.perf_counter() - t0
+
+
+def batched_leg(c, steps, with_accuracy):
+    sizes = ensemble_sizes()
+    Z0 = np.concatenate([surface(n, SEED + i) for i, n in enumerate(sizes)])
+    P0 = np.zeros(len(Z0))
+    out = {"workload": "512 water surfaces, N uniform in {128,256,512,1024} (seeded), 8-mode draws, phi=0, "
+                       "g=1, deep water, dt=0.002; one persistent launch per RK stage",
+           "surfaces": len(sizes), "nodes_total": int(sum(sizes)), "steps": steps}
+    for prec, name in ((c.MIXED, "mixed"), (c.FP64, "fp64")):
+        with c.Ensemble(sizes, c.Props(), precision=prec, solver=c.BATCHED, tol=1e-12, max_iter=16) as e:
+            e.set_state(Z0, P0)
+            el = timed_steps(e, steps, 2)
+            st = e.stats()
+            e.kernel_timing(True)
+            e.step(DT, 1)
+            r = e.kernel_report("stage")
+            e.kernel_timing(False)
+        out[name] = {"ensemble_steps_per_s": round(steps / el, 3), "ms_per_step": round(1e3 * el / steps, 3),
+                     "stage_kernel_ms": round(r["ms"] / max(r["launches"], 1), 4),
+                     "gmres_iterations_last": st["max_iterations"], "refinement_sweeps_last": st["refinements"],
+                     "unconverged": st["unconverged"]}
+    # the plain FP64 stepper (the drop-in: blocked LU; and FP64 GMRES) one surface at a time
+    plain = {}
+    for solver, name, k in ((c.LU, "lu", 6), (c.GMRES, "gmres", 20)):
+        t = {}
+        for n in sorted(set(sizes)):
+            with c.Ensemble([n], c.Props(), solver=solver, tol=1e-12, max_iter=8) as e:
+                e.set_state(surface(n, SEED), np.zeros(n))
+                t[n] = timed_steps(e, k, 2) / k
+        ens = 1.0 / sum(t[n] for n in sizes)
+        plain[name] = {"ensemble_steps_per_s": round(ens, 4), "t_step_ms": {str(n): round(1e3 * v, 4) for n, v in t.items()}}
+    out["plain_fp64_one_at_a_time"] = plain
+    out["speedup_mixed_vs_plain_lu"] = round(out["mixed"]["ensemble_steps_per_s"] / plain["lu"]["ensemble_steps_per_s"], 1)
+    out["speedup_mixed_vs_plain_gmres"] = round(out["mixed"]["ensemble_steps_per_s"] / plain["gmres"]["ensemble_steps_per_s"], 1)
+    if with_accuracy:
+        # error vs the FP64 oracle on a sample: the first N=1024 and the first N=128 surface (each
+        # surface evolves independently, so a two-surface ensemble reproduces them exactly)
+        from oracle import oracle as O
+        O.build()
+        idx = [sizes.index(1024), sizes.index(128)]
+        acc_steps = 20
+        sub = [sizes[i] for i in idx]
+        Zs = [surface(sizes[i], SEED + i) for i in idx]
+        with c.Ensemble(sub, c.Props(), precision=c.MIXED, solver=c.BATCHED, tol=1e-12) as e:
+            e.set_state(np.concatenate(Zs), np.zeros(sum(sub)))
+            e.step(DT, acc_steps)
+            Zg, _ = e.get_state()
+        errs, o = [], 0
+        for i, n in enumerate(sub):
+            Zr, _ = O.rk4(O.Props(), Zs[i], np.zeros(n), DT, acc_steps)
+            errs.append(float(np.abs(Zg[o:o + n] - Zr).max()))
+            o += n
+        out["accuracy_mixed"] = {"surfaces": [f"#{idx[0]} N=1024", f"#{idx[1]} N=128"], "steps": acc_steps,
+                                 "max_abs_dz_vs_oracle": errs}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--solver", choices=("lu", "gmres"), default="lu")
-    ap.add_argument("--max-iter", type=int, default=16, help="GMRES iteration cap per solve")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and accuracy legs")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched ensemble leg")
+    ap.add_argument("--batched-steps", type=int, default=50)
+    ap.add_argument("--solver", choices=("lu", "gmres"), default="gmres")
+    ap.add_argument("--max-iter", type=int, default=8, help="GMRES iteration cap per solve")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run 5 device-to-device copies of MIB MiB (calibrates the HBM byte counters)")
     args = ap.parse_args()
+    if args.copy_probe:
+        import torch
+        a = torch.empty(args.copy_probe << 20, dtype=torch.uint8, device="cuda").fill_(1)
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes_read": a.numel(), "copy_probe_bytes_written": b.numel()}))
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the stepper runs one "
               f"surface ensemble per GPU and this build has no multi-GPU path)", file=sys.stderr)
@@ -75,14 +177,10 @@ def main():
     solver = c.GMRES if args.solver == "gmres" else c.LU
     e = c.Ensemble([N], c.Props(), precision=c.FP64, solver=solver, tol=1e-12, max_iter=args.max_iter)
     e.set_state(Z0, P0)
-    e.step(DT, args.warmup)
-    e.sync()
-    t0 = time.perf_counter()
-    e.step(DT, args.steps)
-    e.sync()
-    el = time.perf_counter() - t0
+    el = timed_steps(e, args.steps, args.warmup)
     value = args.steps / el
     ms_per_step = 1e3 * el / args.steps
+    st_main = e.stats()
 
     # live kernel timing: a few steps with hipEvents around every launch
     e.kernel_timing(True)
@@ -95,12 +193,12 @@ def main():
     st = e.stats()
     e.kernel_timing(False)
     if args.solver == "gmres":
-        # dominant solver kernel: w = M v over the stored FP64 matrix (8 N^2 bytes per launch)
+        # dominant kernel: w = M v over the stored FP64 matrix + Arnoldi dots, (8 N^2 + 8 N (j+2)) B
         dom, bound, peak, unit, key, scale = "gm_arnoldi", "hbm", PEAK_HBM_GBS, "GB/s", "bytes", 1e9
-        kname = "gm_arnoldi_kernel (FP64 GMRES matrix-vector product + Arnoldi dots)"
+        kname = "gm_arnoldi_kernel"
     else:
         dom, bound, peak, unit, key, scale = "lu_update", "mfma", PEAK_FP64_TFLOPS, "TFLOP/s", "flops", 1e12
-        kname = "lu_update_kernel (FP64 MFMA trailing update)"
+        kname = "lu_update_kernel"
     r = reps[dom]
     avg_ms = r["ms"] / max(r["launches"], 1)
     work_per_launch = r[key] / max(r["launches"], 1)
@@ -125,13 +223,15 @@ def main():
                                f"dt=0.002, {'GMRES (tol 1e-12)' if args.solver == 'gmres' else 'blocked LU'} solve",
                    "nodes": N, "dt": DT, "solver": args.solver, "precision": "fp64"},
         "roofline": {"bound": bound, "achieved": round(achieved, 3), "peak": peak, "unit": unit,
-                     "frac": round(achieved / peak, 4), "traffic": None, "kernel": kname,
+                     "frac": round(achieved / peak, 4), "traffic": measured_traffic(kname), "kernel": kname,
                      "avg_launch_ms": round(avg_ms, 5), f"{key}_per_launch": work_per_launch,
                      "launches_timed": r["launches"]},
         "stage_ms_per_step": stages,
-        "solver_stats": {"gmres_iterations_last": st["last_iterations"], "gmres_unconverged": st["unconverged"],
+        "solver_stats": {"gmres_iterations_last": st["last_iterations"], "gmres_unconverged": st_main["unconverged"],
                          "gepp_fallbacks": st["fallback_solves"]},
     }
+    if not args.no_batched:
+        out["batched"] = batched_leg(c, args.batched_steps, not args.no_cpu)
     if not args.no_cpu:
         steps, cel, Zc, Pc, threads = cpu_baseline(Z0, P0)
         out["cpu_baseline"] = {"value": round(steps / cel, 4), "unit": "RK4 steps/s", "cores": threads,
