@@ -167,6 +167,17 @@ int sfh_rhs(sfh_handle h, double* vx, double* vy, double* dphi, double* mu, doub
             double* qup);
 /* Advances every surface by nsteps classical RK4 steps of size dt, in place. */
 int sfh_step(sfh_handle h, double dt, int nsteps);
+
+/* Reverse mode of the water right-hand side (rho = 0, every N <= 1024); no counterpart in the
+ * reference, which has no adjoint.  sfh_rhs_vjp: cotangents (gvx, gvy, gdphi) of the right-hand
+ * side (vx, vy, dphi) at the current state -> cotangents (gx, gy, gphi) of the state (x, y, phi).
+ * sfh_step_vjp: cotangent of the state one RK4 step of size dt after the current state ->
+ * cotangent of the current state; the handle's state is not changed.  Together with saved
+ * checkpoints this gives the gradient of any function of a trajectory. */
+int sfh_rhs_vjp(sfh_handle h, const double* gvx, const double* gvy, const double* gdphi, double* gx, double* gy,
+                double* gphi);
+int sfh_step_vjp(sfh_handle h, double dt, const double* gx_in, const double* gy_in, const double* gphi_in,
+                 double* gx_out, double* gy_out, double* gphi_out);
 /* Dipole strengths of the last right-hand-side evaluation. */
 int sfh_get_mu(sfh_handle h, double* mu);
 int sfh_sync(sfh_handle h);

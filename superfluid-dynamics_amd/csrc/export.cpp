@@ -358,6 +358,20 @@ int sfh_step(sfh_handle h, double dt, int nsteps) {
     return guarded([&] { h->s->step(dt, nsteps); });
 }
 
+int sfh_rhs_vjp(sfh_handle h, const double* gvx, const double* gvy, const double* gdphi, double* gx, double* gy,
+                double* gphi) {
+    SFH_NEED(h);
+    if (!gvx || !gvy || !gdphi || !gx || !gy || !gphi) return -3;
+    return guarded([&] { h->s->rhs_vjp(gvx, gvy, gdphi, gx, gy, gphi); });
+}
+
+int sfh_step_vjp(sfh_handle h, double dt, const double* gx_in, const double* gy_in, const double* gphi_in,
+                 double* gx_out, double* gy_out, double* gphi_out) {
+    SFH_NEED(h);
+    if (!gx_in || !gy_in || !gphi_in || !gx_out || !gy_out || !gphi_out) return -3;
+    return guarded([&] { h->s->step_vjp(dt, gx_in, gy_in, gphi_in, gx_out, gy_out, gphi_out); });
+}
+
 int sfh_get_mu(sfh_handle h, double* mu) {
     SFH_NEED(h);
     return guarded([&] { h->s->get_mu(mu); });

@@ -106,4 +106,33 @@ struct BatchArgs {
 };
 __global__ void batched_stage_kernel(BatchArgs a, Phys ph, StageCtl ctl);
 
+// ---- adjoint.hip (reverse mode of the water right-hand side, every N <= kBatchMaxN)
+// Per-node intermediates kept in global scratch across the transpose solve.
+struct AdjNode {
+    double2 gzp, gzpp, gz, gw, p1, S;
+    double mu, pad;
+};
+struct VjpArgs {
+    const SurfDesc* sd;
+    const double2* tw_all;
+    const double2* Zin;        // state at which the Jacobian is taken
+    const double* Pin;
+    const double* mu_in;       // the forward mu at that state (null: solve for it)
+    const double2* gA_Z;       // input cotangent = ca * gA + cb * gB (gB may be null)
+    const double* gA_P;
+    const double2* gB_Z;
+    const double* gB_P;
+    double ca, cb;
+    double2* out_Z;            // J^T (input cotangent)
+    double* out_P;
+    double2* acc_Z;            // optional accumulator += out
+    double* acc_P;
+    AdjNode* scratch;
+    int* stats;                // per surface: [iterations, unconverged, -, -]
+    double tol;
+    int cycles;
+    int pad;
+};
+__global__ void rhs_vjp_kernel(VjpArgs a, Phys ph);
+
 }  // namespace sfh

@@ -172,6 +172,11 @@ Solver::~Solver() {
                     status_, piv_, counters_, gmV_, gmw_, gmpart_, gmsmall_, gmctl_, trig_, order_, bstats_};
     for (void* p : bufs)
         if (p) (void)hipFree(p);
+    void* vbufs[] = {vY_[0], vY_[1], vY_[2], vK_[0], vK_[1], vK_[2], vYP_[0], vYP_[1], vYP_[2], vKP_[0],
+                     vKP_[1], vKP_[2], vMu_[0], vMu_[1], vMu_[2], vMu_[3], gA_, gR_[0], gR_[1], gAcc_,
+                     gAP_, gRP_[0], gRP_[1], gAccP_, adj_, vstats_};
+    for (void* p : vbufs)
+        if (p) (void)hipFree(p);
     for (hipEvent_t e : event_pool_) (void)hipEventDestroy(e);
     if (stream_) (void)hipStreamDestroy(stream_);
 }
@@ -601,6 +606,144 @@ void debug_lu_solve(int n, const double* A, const double* b, double* x, int* int
     for (void* p : bufs) (void)hipFree(p);
     (void)hipStreamDestroy(st);
     if (status[1]) throw HipError("back-substitution wait timed out");
+}
+
+// ------------------------------------------------------------------ reverse mode
+
+void Solver::ensure_vjp() {
+    if (ph_.kind != 0 || ph_.rho != 0.0 || ph_.image)
+        throw ArgError("the adjoint is implemented for water with rho = 0");
+    if (maxN_ > kBatchMaxN) throw ArgError("the adjoint takes surfaces of at most 1024 nodes");
+    if (vjp_ready_) return;
+    for (int i = 0; i < 3; ++i) {
+        vY_[i] = dalloc<double2>(T_);
+        vK_[i] = dalloc<double2>(T_);
+        vYP_[i] = dalloc<double>(T_);
+        vKP_[i] = dalloc<double>(T_);
+    }
+    for (int i = 0; i < 4; ++i) vMu_[i] = dalloc<double>(T_);
+    gA_ = dalloc<double2>(T_);
+    gAcc_ = dalloc<double2>(T_);
+    gAP_ = dalloc<double>(T_);
+    gAccP_ = dalloc<double>(T_);
+    for (int i = 0; i < 2; ++i) {
+        gR_[i] = dalloc<double2>(T_);
+        gRP_[i] = dalloc<double>(T_);
+    }
+    adj_ = dalloc<AdjNode>(T_);
+    vstats_ = dalloc<int>(4 * (size_t)S_);
+    SFH_HIP(hipMemset(vstats_, 0, 4 * S_ * sizeof(int)));
+    vjp_ready_ = true;
+}
+
+void Solver::enqueue_vjp(const double2* Z, const double* P, const double* mu, const double2* gAZ, const double* gAP,
+                         double ca, const double2* gBZ, const double* gBP, double cb, double2* outZ, double* outP,
+                         bool accumulate) {
+    VjpArgs a{};
+    a.sd = d_desc_;
+    a.tw_all = d_tw_;
+    a.Zin = Z;
+    a.Pin = P;
+    a.mu_in = mu;
+    a.gA_Z = gAZ;
+    a.gA_P = gAP;
+    a.gB_Z = gBZ;
+    a.gB_P = gBP;
+    a.ca = ca;
+    a.cb = cb;
+    a.out_Z = outZ;
+    a.out_P = outP;
+    a.acc_Z = accumulate ? gAcc_ : nullptr;
+    a.acc_P = accumulate ? gAccP_ : nullptr;
+    a.scratch = adj_;
+    a.stats = vstats_;
+    a.tol = std::min(opt_.tol, 1e-13);
+    a.cycles = 6;
+    double nsq = 0.0;
+    for (int v : n_) nsq += (double)v * v;
+    launch("vjp", 3.0 * nsq, 0.0, [&] {
+        hipLaunchKernelGGL(rhs_vjp_kernel, dim3(S_), dim3(kBatchThreads), 0, stream_, a, ph_);
+    });
+    SFH_HIP(hipGetLastError());
+}
+
+void Solver::upload_cotangent(const double* gx, const double* gy, const double* gp, double2* Z, double* P) {
+    std::vector<double2> z(T_);
+    for (int i = 0; i < T_; ++i) z[i] = make_double2(gx[i], gy[i]);
+    SFH_HIP(hipMemcpy(Z, z.data(), T_ * sizeof(double2), hipMemcpyHostToDevice));
+    SFH_HIP(hipMemcpy(P, gp, T_ * sizeof(double), hipMemcpyHostToDevice));
+}
+
+void Solver::download_cotangent(const double2* Z, const double* P, double* gx, double* gy, double* gp) {
+    std::vector<double2> z(T_);
+    SFH_HIP(hipStreamSynchronize(stream_));
+    SFH_HIP(hipMemcpy(z.data(), Z, T_ * sizeof(double2), hipMemcpyDeviceToHost));
+    SFH_HIP(hipMemcpy(gp, P, T_ * sizeof(double), hipMemcpyDeviceToHost));
+    for (int i = 0; i < T_; ++i) {
+        gx[i] = z[i].x;
+        gy[i] = z[i].y;
+    }
+}
+
+void Solver::rhs_vjp(const double* gvx, const double* gvy, const double* gdphi, double* gx, double* gy,
+                     double* gphi) {
+    ensure_vjp();
+    SFH_HIP(hipStreamSynchronize(stream_));
+    upload_cotangent(gvx, gvy, gdphi, gA_, gAP_);
+    StageCtl c{};
+    c.stage = 0;
+    c.Z0 = Z0_;
+    c.P0 = P0_;
+    for (int i = 0; i < 3; ++i) {
+        c.kZ[i] = vK_[i];
+        c.kP[i] = vKP_[i];
+    }
+    enqueue_rhs(Z0_, P0_, c);   // the forward mu at the current state
+    SFH_HIP(hipMemcpyAsync(vMu_[0], mu_, T_ * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+    enqueue_vjp(Z0_, P0_, vMu_[0], gA_, gAP_, 1.0, nullptr, nullptr, 0.0, gR_[0], gRP_[0], false);
+    download_cotangent(gR_[0], gRP_[0], gx, gy, gphi);
+    check_status();
+}
+
+// Adjoint of one classical RK4 step from y_n (AutonomousRungeKuttaStepper.cuh:338-375):
+//   k1 = f(y_n), y2 = y_n + dt/2 k1, k2 = f(y2), y3 = y_n + dt/2 k2, k3 = f(y3), y4 = y_n + dt k3,
+//   k4 = f(y4), y_{n+1} = y_n + dt/6 (k1 + 2 k2 + 2 k3 + k4).
+// With a = cotangent of y_{n+1} and r_s = J(y_s)^T kbar_s:
+//   r4 from kbar4 = dt/6 a;  r3 from kbar3 = dt/3 a + dt r4;  r2 from kbar2 = dt/3 a + dt/2 r3;
+//   r1 from kbar1 = dt/6 a + dt/2 r2;  cotangent of y_n = a + r1 + r2 + r3 + r4.
+void Solver::step_vjp(double dt, const double* gx_in, const double* gy_in, const double* gp_in, double* gx_out,
+                      double* gy_out, double* gp_out) {
+    ensure_vjp();
+    SFH_HIP(hipStreamSynchronize(stream_));
+    upload_cotangent(gx_in, gy_in, gp_in, gA_, gAP_);
+    SFH_HIP(hipMemcpyAsync(gAcc_, gA_, T_ * sizeof(double2), hipMemcpyDeviceToDevice, stream_));
+    SFH_HIP(hipMemcpyAsync(gAccP_, gAP_, T_ * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+    // stage states and their mu
+    StageCtl c{};
+    c.Z0 = Z0_;
+    c.P0 = P0_;
+    for (int i = 0; i < 3; ++i) {
+        c.kZ[i] = vK_[i];
+        c.kP[i] = vKP_[i];
+    }
+    const double2* zin[4] = {Z0_, vY_[0], vY_[1], vY_[2]};
+    const double* pin[4] = {P0_, vYP_[0], vYP_[1], vYP_[2]};
+    const double alpha[3] = {0.5 * dt, 0.5 * dt, dt};
+    for (int st = 0; st < 4; ++st) {
+        c.stage = st < 3 ? st + 1 : 0;
+        c.alpha = st < 3 ? alpha[st] : 0.0;
+        c.Zout = st < 3 ? vY_[st] : nullptr;
+        c.Pout = st < 3 ? vYP_[st] : nullptr;
+        enqueue_rhs(zin[st], pin[st], c);
+        SFH_HIP(hipMemcpyAsync(vMu_[st], mu_, T_ * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+    }
+    // reverse sweep
+    enqueue_vjp(vY_[2], vYP_[2], vMu_[3], gA_, gAP_, dt / 6.0, nullptr, nullptr, 0.0, gR_[0], gRP_[0], true);
+    enqueue_vjp(vY_[1], vYP_[1], vMu_[2], gA_, gAP_, dt / 3.0, gR_[0], gRP_[0], dt, gR_[1], gRP_[1], true);
+    enqueue_vjp(vY_[0], vYP_[0], vMu_[1], gA_, gAP_, dt / 3.0, gR_[1], gRP_[1], 0.5 * dt, gR_[0], gRP_[0], true);
+    enqueue_vjp(Z0_, P0_, vMu_[0], gA_, gAP_, dt / 6.0, gR_[0], gRP_[0], 0.5 * dt, gR_[1], gRP_[1], true);
+    download_cotangent(gAcc_, gAccP_, gx_out, gy_out, gp_out);
+    check_status();
 }
 
 }  // namespace sfh

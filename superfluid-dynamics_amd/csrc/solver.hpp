@@ -80,6 +80,14 @@ class Solver {
     double2* device_Z() { return Z0_; }
     double* device_phi() { return P0_; }
 
+    // Reverse mode (water, rho = 0, every N <= 1024).  rhs_vjp: cotangents of (vx, vy, dphi) at the
+    // current state -> cotangents of (x, y, phi).  step_vjp: cotangent of the state after one RK4
+    // step of size dt taken from the current state -> cotangent of the current state (the state is
+    // left unchanged).  Host arrays of total_nodes() entries.
+    void rhs_vjp(const double* gvx, const double* gvy, const double* gdphi, double* gx, double* gy, double* gphi);
+    void step_vjp(double dt, const double* gx_in, const double* gy_in, const double* gp_in, double* gx_out,
+                  double* gy_out, double* gp_out);
+
     // Standalone derivative of a complex field (FftDerivative::exec semantics).
     void derivative(const double2* in_host, double2* out_host, bool second, double scale);
 
@@ -150,6 +158,21 @@ class Solver {
     int* order_ = nullptr;    // workgroup -> surface, by decreasing N
     int* bstats_ = nullptr;   // 4 ints per surface
     void enqueue_batched(const double2* Z, const double* P, const StageCtl& ctl);
+    // adjoint workspace (allocated on first use)
+    void ensure_vjp();
+    void enqueue_vjp(const double2* Z, const double* P, const double* mu, const double2* gAZ, const double* gAP,
+                     double ca, const double2* gBZ, const double* gBP, double cb, double2* outZ, double* outP,
+                     bool accumulate);
+    void upload_cotangent(const double* gx, const double* gy, const double* gp, double2* Z, double* P);
+    void download_cotangent(const double2* Z, const double* P, double* gx, double* gy, double* gp);
+    bool vjp_ready_ = false;
+    double2 *vY_[3] = {nullptr, nullptr, nullptr}, *vK_[3] = {nullptr, nullptr, nullptr};
+    double *vYP_[3] = {nullptr, nullptr, nullptr}, *vKP_[3] = {nullptr, nullptr, nullptr};
+    double* vMu_[4] = {nullptr, nullptr, nullptr, nullptr};
+    double2 *gA_ = nullptr, *gR_[2] = {nullptr, nullptr}, *gAcc_ = nullptr;
+    double *gAP_ = nullptr, *gRP_[2] = {nullptr, nullptr}, *gAccP_ = nullptr;
+    AdjNode* adj_ = nullptr;
+    int* vstats_ = nullptr;
 
     hipGraph_t graph_ = nullptr;
     hipGraphExec_t exec_ = nullptr;

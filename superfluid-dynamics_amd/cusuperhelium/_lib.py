@@ -71,6 +71,8 @@ SIGNATURES = {
     "sfh_get_state": (c_int, [c_void_p, DP, DP, DP]),
     "sfh_rhs": (c_int, [c_void_p, DP, DP, DP, DP, DP, DP, DP]),
     "sfh_step": (c_int, [c_void_p, c_double, c_int]),
+    "sfh_rhs_vjp": (c_int, [c_void_p, DP, DP, DP, DP, DP, DP]),
+    "sfh_step_vjp": (c_int, [c_void_p, c_double, DP, DP, DP, DP, DP, DP]),
     "sfh_get_mu": (c_int, [c_void_p, DP]),
     "sfh_sync": (c_int, [c_void_p]),
     "sfh_get_stats": (c_int, [c_void_p, POINTER(SfhStats)]),

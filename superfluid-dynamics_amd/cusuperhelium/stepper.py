@@ -119,6 +119,29 @@ class Ensemble:
     def step(self, dt: float, nsteps: int = 1) -> None:
         check(self._lib.sfh_step(self._h, float(dt), int(nsteps)), "sfh_step")
 
+    # ---- reverse mode (water, rho = 0, every N <= 1024)
+    def rhs_vjp(self, gq, gdphi):
+        """Cotangent of (Z, Phi) from cotangents of (q = dZ/dt, dPhi/dt) at the current state.
+
+        Complex cotangents follow dL = Re sum conj(g) dz, i.e. g = dL/dRe + i dL/dIm."""
+        gq = np.asarray(gq, dtype=np.complex128).ravel()
+        gvx, gvy = np.ascontiguousarray(gq.real), np.ascontiguousarray(gq.imag)
+        gp = np.ascontiguousarray(gdphi, dtype=np.float64).ravel()
+        gx, gy, gphi = (np.empty(self.total) for _ in range(3))
+        check(self._lib.sfh_rhs_vjp(self._h, _dp(gvx), _dp(gvy), _dp(gp), _dp(gx), _dp(gy), _dp(gphi)),
+              "sfh_rhs_vjp")
+        return gx + 1j * gy, gphi
+
+    def step_vjp(self, dt: float, gZ, gPhi):
+        """Cotangent of the current state from the cotangent of the state one RK4 step later."""
+        gZ = np.asarray(gZ, dtype=np.complex128).ravel()
+        gxi, gyi = np.ascontiguousarray(gZ.real), np.ascontiguousarray(gZ.imag)
+        gpi = np.ascontiguousarray(gPhi, dtype=np.float64).ravel()
+        gx, gy, gp = (np.empty(self.total) for _ in range(3))
+        check(self._lib.sfh_step_vjp(self._h, float(dt), _dp(gxi), _dp(gyi), _dp(gpi), _dp(gx), _dp(gy), _dp(gp)),
+              "sfh_step_vjp")
+        return gx + 1j * gy, gp
+
     def mu(self) -> np.ndarray:
         out = np.empty(self.total)
         check(self._lib.sfh_get_mu(self._h, _dp(out)), "sfh_get_mu")

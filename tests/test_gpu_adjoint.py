@@ -1,0 +1,170 @@
+"""GPU checks of the hand-written adjoint (water, rho = 0) through the C ABI.
+
+- right-hand side VJP: dot-product test <J^T l, d> = d/dh <l, f(y + h d)> against central
+  differences of the CPU FP64 oracle (h = 1e-6, relative agreement 1e-7);
+- one RK4 step VJP: the same test against the oracle's RK4 step;
+- the differentiable-stepper workload: N = 1024, 16 Fourier amplitudes c_k ~ U(0, 0.01),
+  y0 = sum c_k cos(k x), loss = 1/2 sum_j (y_j(T) - y*_j)^2 against a seeded target (seed + 1),
+  checkpoint every 20 steps; gradient from torch.autograd through the adjoint against central
+  finite differences (h = 1e-6) of the GPU stepper (all 32 perturbed surfaces advanced as one
+  batched ensemble), relative error <= 1e-6.  With dt = 0.005 the unfiltered method (the
+  reference's, reproduced by the oracle) is unstable on this surface: the Nyquist mode grows about
+  100x every 10 steps from rounding level and overflows near step 100, so T = 200 dt is checked at
+  dt = 0.001 (T = 0.2) and dt = 0.005 is checked over 60 steps;
+- the same pipeline at N = 64 over 20 steps against central differences of the CPU FP64 oracle.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def csh():
+    import cusuperhelium as c
+    c.lib()
+    return c
+
+
+def _surface(n, seed, amp=0.03, modes=8):
+    rng = np.random.default_rng(seed)
+    x = 2 * np.pi * np.arange(n) / n
+    a = rng.uniform(0, amp / np.arange(1, modes + 1))
+    th = rng.uniform(0, 2 * np.pi, modes)
+    return x + 1j * sum(a[k] * np.cos((k + 1) * x + th[k]) for k in range(modes))
+
+
+def _inner(gZ, gP, dZ, dP):
+    return float(np.real(np.sum(np.conj(gZ) * dZ)) + np.sum(gP * dP))
+
+
+def _smooth_direction(rng, n, modes=8):
+    # band-limited perturbation (modes 1..8 of x and phi): white noise would make the central
+    # difference's truncation error grow like N^2 through the curvature terms
+    x = 2 * np.pi * np.arange(n) / n
+    k = np.arange(1, modes + 1)[:, None]
+    cz = (rng.standard_normal(modes) + 1j * rng.standard_normal(modes)) / k[:, 0]
+    cp = rng.standard_normal(modes) / k[:, 0]
+    dZ = (cz[:, None] * np.exp(1j * k * x)).sum(0)
+    dP = (cp[:, None] * np.cos(k * x)).sum(0)
+    return dZ, dP
+
+
+def _richardson(f, h=1e-5):
+    # central differences at h and h/2 combined to cancel the O(h^2) term
+    d1 = (f(h) - f(-h)) / (2 * h)
+    d2 = (f(h / 2) - f(-h / 2)) / h
+    return (4 * d2 - d1) / 3
+
+
+@pytest.mark.parametrize("sizes", [[64], [256, 96, 1024]])
+def test_rhs_vjp_dot_product(oracle, csh, sizes):
+    print(f"rhs vjp sizes {sizes}", flush=True)
+    rng = np.random.default_rng(11)
+    Zs = [_surface(n, 100 + n) for n in sizes]
+    Ps = [0.02 * np.sin(2 * np.pi * np.arange(n) / n + 0.3) for n in sizes]
+    lq = [rng.standard_normal(n) + 1j * rng.standard_normal(n) for n in sizes]
+    lp = [rng.standard_normal(n) for n in sizes]
+    with csh.Ensemble(sizes, csh.Props(), solver=csh.BATCHED, tol=1e-13) as e:
+        e.set_state(np.concatenate(Zs), np.concatenate(Ps))
+        gZ, gP = e.rhs_vjp(np.concatenate(lq), np.concatenate(lp))
+    o = 0
+    for i, n in enumerate(sizes):
+        dZ, dP = _smooth_direction(rng, n)
+        fd = _richardson(lambda h: _inner(lq[i], lp[i], *oracle.rhs(oracle.Props(), Zs[i] + h * dZ, Ps[i] + h * dP)))
+        ad = _inner(gZ[o:o + n], gP[o:o + n], dZ, dP)
+        print(f"  n={n}: adjoint {ad:.12e}  fd {fd:.12e}", flush=True)
+        assert abs(ad - fd) <= 1e-7 * max(abs(fd), 1e-3)
+        o += n
+
+
+def test_step_vjp_dot_product(oracle, csh):
+    n, dt = 128, 0.005
+    rng = np.random.default_rng(5)
+    Z, P = _surface(n, 7), 0.01 * np.cos(2 * np.pi * np.arange(n) / n)
+    aZ = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    aP = rng.standard_normal(n)
+    with csh.Ensemble([n], csh.Props(), solver=csh.BATCHED, tol=1e-13) as e:
+        e.set_state(Z, P)
+        gZ, gP = e.step_vjp(dt, aZ, aP)
+        Zs, _ = e.get_state()
+    assert np.array_equal(Zs, Z)   # the handle's state is left unchanged
+    dZ, dP = _smooth_direction(rng, n)
+    fd = _richardson(lambda h: _inner(aZ, aP, *oracle.rk4(oracle.Props(), Z + h * dZ, P + h * dP, dt, 1)))
+    ad = _inner(gZ, gP, dZ, dP)
+    print(f"step vjp: adjoint {ad:.12e} fd {fd:.12e}", flush=True)
+    assert abs(ad - fd) <= 1e-7 * abs(fd)
+
+
+def _amplitude_problem(n, modes, seed):
+    x = 2 * np.pi * np.arange(n) / n
+    c = np.random.default_rng(seed).uniform(0, 0.01, modes)
+    ct = np.random.default_rng(seed + 1).uniform(0, 0.01, modes)
+    basis = np.array([np.cos((k + 1) * x) for k in range(modes)])
+    return x, c, basis, ct @ basis
+
+
+def _adjoint_gradient(csh, n, modes, seed, dt, steps, every):
+    import torch
+    from cusuperhelium.adjoint import Trajectory, evolve
+    x, c, basis, target = _amplitude_problem(n, modes, seed)
+    ct = torch.tensor(c, dtype=torch.float64, requires_grad=True)
+    B = torch.tensor(basis)
+    Z0 = torch.tensor(x, dtype=torch.complex128) + 1j * (ct @ B).to(torch.complex128)
+    P0 = torch.zeros(n, dtype=torch.float64)
+    with Trajectory([n], checkpoint_every=every) as tr:
+        ZT, _ = evolve(Z0, P0, tr, dt, steps)
+        loss = 0.5 * torch.sum((ZT.imag - torch.tensor(target)) ** 2)
+        loss.backward()
+    return x, c, basis, target, float(loss.detach()), ct.grad.numpy()
+
+
+@pytest.mark.parametrize("dt,steps", [(0.005, 60), (0.001, 200)])
+def test_differentiable_stepper_n1024_matches_fd(csh, dt, steps):
+    n, modes, seed = 1024, 16, 20261202
+    print(f"adjoint gradient n=1024, dt {dt}, {steps} steps", flush=True)
+    x, c, basis, target, loss, grad = _adjoint_gradient(csh, n, modes, seed, dt, steps, 20)
+    # central differences: the 32 perturbed initial surfaces advanced together
+    h = 1e-6
+    Z0s, P0s = [], []
+    for k in range(modes):
+        for sgn in (+1, -1):
+            cc = c.copy()
+            cc[k] += sgn * h
+            Z0s.append(x + 1j * (cc @ basis))
+            P0s.append(np.zeros(n))
+    with csh.Ensemble([n] * (2 * modes), csh.Props(), solver=csh.BATCHED, tol=1e-14, max_iter=48) as e:
+        e.set_state(np.concatenate(Z0s), np.concatenate(P0s))
+        e.step(dt, steps)
+        ZT, _ = e.get_state()
+    yT = ZT.imag.reshape(2 * modes, n)
+    L = 0.5 * np.sum((yT - target) ** 2, axis=1)
+    fd = (L[0::2] - L[1::2]) / (2 * h)
+    rel = np.abs(grad - fd).max() / np.abs(fd).max()
+    print(f"loss {loss:.6e}; max |grad - fd| / max |fd| = {rel:.2e}", flush=True)
+    assert np.isfinite(loss) and rel <= 1e-6
+
+
+def test_differentiable_stepper_small_matches_oracle_fd(oracle, csh):
+    n, modes, seed, dt, steps = 64, 4, 77, 0.005, 20
+    x, c, basis, target, loss, grad = _adjoint_gradient(csh, n, modes, seed, dt, steps, 7)
+    h = 1e-6
+    fd = np.empty(modes)
+    for k in range(modes):
+        Ls = []
+        for sgn in (+1, -1):
+            cc = c.copy()
+            cc[k] += sgn * h
+            ZT, _ = oracle.rk4(oracle.Props(), x + 1j * (cc @ basis), np.zeros(n), dt, steps)
+            Ls.append(0.5 * np.sum((ZT.imag - target) ** 2))
+        fd[k] = (Ls[0] - Ls[1]) / (2 * h)
+    rel = np.abs(grad - fd).max() / np.abs(fd).max()
+    print(f"oracle fd: grad {grad} fd {fd} rel {rel:.2e}", flush=True)
+    assert rel <= 1e-6
+
+
+def test_adjoint_rejects_unsupported_physics(csh):
+    with csh.Ensemble([64], csh.Props(kind=1, depth=1.0), solver=csh.BATCHED) as e:
+        e.set_state(_surface(64, 1), np.zeros(64))
+        with pytest.raises(Exception):
+            e.rhs_vjp(np.ones(64, complex), np.ones(64))
