@@ -143,6 +143,40 @@ def batched_leg(c, steps, with_accuracy):
     return out
 
 
+def film_leg(c, steps, with_accuracy):
+    """Helium film over a flat substrate: N = 4096, depth h0 = 1 with the image term, restoring
+    acceleration 3 A / (h0 + y)^4 with A = 0.05, i.e. g_eff = 3 A / h0^4 = 0.15 scaling the reference's
+    van der Waals term; 8-mode seeded surface with a_k ~ U(0, 0.05 / k); phi = 0; dt = 0.002."""
+    n = 4096
+    props = dict(kind=1, depth=1.0, g=0.15)
+    Z0 = surface(n, SEED + 5, amp=0.05)
+    P0 = np.zeros(n)
+    out = {"workload": "helium film N=4096, h0=1 (image term), g_eff=3A/h0^4=0.15 (A=0.05), 8-mode seeded "
+                       "surface a_k~U(0,0.05/k), phi=0, dt=0.002, FP64", "nodes": n}
+    for solver, name, mi, k in ((c.GMRES, "gmres", 24, steps), (c.LU, "lu", 8, max(2, steps // 25))):
+        with c.Ensemble([n], c.Props(**props), solver=solver, tol=1e-12, max_iter=mi) as e:
+            e.set_state(Z0, P0)
+            el = timed_steps(e, k, 1)
+            st = e.stats()
+        out[name] = {"steps_per_s": round(k / el, 3), "ms_per_step": round(1e3 * el / k, 3), "steps": k,
+                     "gmres_iterations_last": st["max_iterations"], "unconverged": st["unconverged"],
+                     "gepp_fallbacks": st["fallback_solves"]}
+    if with_accuracy:
+        from oracle import oracle as O
+        O.build()
+        acc_steps = 2
+        with c.Ensemble([n], c.Props(**props), solver=c.GMRES, tol=1e-12, max_iter=24) as e:
+            e.set_state(Z0, P0)
+            e.step(DT, acc_steps)
+            Zg, Pg = e.get_state()
+        t0 = time.perf_counter()
+        Zr, Pr = O.rk4(O.Props(**props), Z0, P0, DT, acc_steps)
+        out["accuracy_gmres"] = {"steps": acc_steps, "max_abs_dz_vs_oracle": float(np.abs(Zg - Zr).max()),
+                                 "max_abs_dphi_vs_oracle": float(np.abs(Pg - Pr).max()),
+                                 "oracle_s_per_step": round((time.perf_counter() - t0) / acc_steps, 2)}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -151,6 +185,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and accuracy legs")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched ensemble leg")
     ap.add_argument("--batched-steps", type=int, default=50)
+    ap.add_argument("--no-film", action="store_true", help="skip the N=4096 helium film leg")
+    ap.add_argument("--film-steps", type=int, default=50)
     ap.add_argument("--solver", choices=("lu", "gmres"), default="gmres")
     ap.add_argument("--max-iter", type=int, default=8, help="GMRES iteration cap per solve")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -232,6 +268,8 @@ def main():
     }
     if not args.no_batched:
         out["batched"] = batched_leg(c, args.batched_steps, not args.no_cpu)
+    if not args.no_film:
+        out["film"] = film_leg(c, args.film_steps, not args.no_cpu)
     if not args.no_cpu:
         steps, cel, Zc, Pc, threads = cpu_baseline(Z0, P0)
         out["cpu_baseline"] = {"value": round(steps / cel, 4), "unit": "RK4 steps/s", "cores": threads,
