@@ -1,30 +1,32 @@
-// gmres.hip -- FP64 GMRES for M mu = b over the stored dipole matrix (the north-star alternative
-// to the LU of MatrixSolver::solve, CuSuperHelium/MatrixSolver.cuh:113-172).
+// gmres.hip -- FP64 GMRES for M mu = b over the stored dipole matrix, all workgroups of the GPU on
+// one or a few large surfaces (the north-star alternative to the LU of MatrixSolver::solve,
+// CuSuperHelium/MatrixSolver.cuh:113-172).
 //
 // M = I/2 + K is a second-kind operator: its spectrum clusters at 1/2 (for the helium film one
-// outlier sits far to the right), so GMRES converges in a handful of iterations, and the
-// stepper warm starts every solve from the previous stage's mu.  Each iteration is two kernels
-// (classical Gram-Schmidt, one pass):
-//   gm_arnoldi : w = M v_j and the dots h_i = v_i . w (i <= j)
-//   gm_extend  : u = w - sum h_i v_i (stored unnormalised, scale 1/|u| kept aside) and |u|
-//                -> last workgroup: h_{j+1,j} = |u|, Givens rotation, residual test
-// The new vector's norm is reduced from u itself (not from |w|^2 - sum h_i^2, which cancels
-// catastrophically once the basis drifts from orthonormal).  One pass suffices here: the basis
-// conditioning follows that of M = I/2 + K, which is O(1) for the configurations this solves.
-// Reductions use the "last workgroup done" pattern (atomic counter after a release fence), so
-// no workgroup ever waits for another.  Once the residual estimate reaches tol * |b| the
-// remaining launches of the fixed-length sequence return at once.
+// outlier sits far to the right), so GMRES converges in a handful of iterations, and the stepper
+// warm starts every solve from the previous stage's mu.
 //
-// Layout for memory-level parallelism: a workgroup owns 8 rows; its 256 threads are (row r,
-// basis index i) pairs (r = 2 * wave + lane / 32, i = lane % 32), so every basis-vector access of
-// an iteration is one load per thread, all in flight together.  Partial sums are stored basis
-// index major (part[i * nwg + wg]) and the last workgroup reduces all of them with unrolled loads.
+// ONE kernel per Arnoldi step.  Kernel j:
+//   1. loads v_j = sc[j] V[j] into LDS and multiplies its rows of M by it (one row per wave, the
+//      whole row's loads in flight), writing w_j;
+//   2. forms partial dots v_i . w_j (i <= j) of its rows;
+//   3. the last workgroup (release increments, one acquire) reduces the dots, forms the next
+//      basis vector u = w_j - sum_i h_{i,j} v_i over all N entries (classical Gram-Schmidt, one
+//      pass), its norm h_{j+1,j} = |u|, stores V[j+1] = u and sc[j+1] = 1 / |u|, applies the
+//      Givens rotations to column j and tests the residual estimate.
+// Once the residual estimate reaches tol * |b| the remaining launches of the fixed-length sequence
+// return at once; gm_finish solves the small triangular system and updates mu.
+//
+// Initial guess of an RK stage: mu of the same stage extrapolated linearly over the two previous
+// steps, 2 mu_s(n-1) - mu_s(n-2) (its error is second order in dt, against first order for the
+// previous stage's mu), which saves Krylov steps on every solve of a time integration.
 #include "sfh_kernels.hpp"
 
 namespace sfh {
 
-constexpr int kRowsPerWG = 8;    // 4 waves x 2 rows
-constexpr int kMaxKrylov = 32;   // basis vectors per solve (iteration cap)
+constexpr int kRowsPerWG = 4;      // one matrix row per wave
+constexpr int kMaxKrylov = 32;     // basis vectors per solve (iteration cap)
+constexpr int kGmMaxN = 4096;      // LDS copy of the vector being multiplied
 
 struct GmCtl {
     int done;
@@ -41,15 +43,16 @@ struct GmView {
     const double* M;
     const double* b;
     double* x;
-    double* V;       // (m+1) x n, unnormalised
-    double* sc;      // m+1 scales: v_i = sc[i] * V[i]
-    double* w;       // n
-    double* part;    // (m+2) x nwg
-    double* H;       // column j at H[j*(m+1)]
-    double* h1;      // m+1
+    double* V;       // (m+1) x n, unnormalised: v_i = sc[i] * V[i]
+    double* w0;      // products of even steps
+    double* w1;      // products of odd steps
+    double* sc;      // m+1 scales
+    double* h1;      // unrotated column of the last step
     double* cs;
     double* sn;
     double* g;
+    double* H;       // column j at H[j*(m+1)]
+    double* part;    // (m+2) x nwg partial sums, basis index major
     GmCtl* ctl;
     int n, nwg;
 };
@@ -65,7 +68,8 @@ __device__ inline GmView gm_view(const GmArgs& a, const SurfDesc* sd) {
     v.b = a.b + d.voff;
     v.x = a.x + d.voff;
     v.V = a.V + (long long)(m + 1) * d.voff;
-    v.w = a.w + d.voff;
+    v.w0 = a.w + d.voff;
+    v.w1 = a.w + a.total + d.voff;
     v.sc = a.small + (long long)s * a.small_stride;
     v.h1 = v.sc + (m + 1);
     v.cs = v.h1 + (m + 1);
@@ -77,33 +81,43 @@ __device__ inline GmView gm_view(const GmArgs& a, const SurfDesc* sd) {
     return v;
 }
 
-// Dot products of two matrix rows with one vector, one wave, 8 loads per row in flight.
-__device__ inline void rows_dot2(const double* __restrict__ R0, const double* __restrict__ R1,
-                                 const double* __restrict__ vec, int n, int lane, double& d0, double& d1) {
-    double a0 = 0.0, a1 = 0.0, b0 = 0.0, b1 = 0.0;
-    int c = lane;
-    for (; c + 7 * 64 < n; c += 8 * 64) {
-        double m0[8], m1[8], vv[8];
+// Row R (global) times vs (LDS) by one wave, in two halves so the first 2048 columns' loads can
+// be issued before vs is ready: row_load issues 32 loads per lane (predicated at the row end),
+// row_dot_rest finishes the product (columns beyond 2048 are loaded and used chunk by chunk).
+struct RowChunk {
+    double mv[32];
+};
+__device__ inline void row_load(const double* __restrict__ R, int n, int lane, RowChunk& rc) {
 #pragma unroll
-        for (int u = 0; u < 8; ++u) {
-            m0[u] = R0[c + u * 64];
-            m1[u] = R1[c + u * 64];
-            vv[u] = vec[c + u * 64];
+    for (int u = 0; u < 32; ++u) {
+        const int c = lane + 64 * u;
+        rc.mv[u] = c < n ? R[c] : 0.0;
+    }
+}
+__device__ inline double row_dot_rest(const double* __restrict__ R, const RowChunk& rc, const double* vs, int n,
+                                      int lane) {
+    double a0 = 0.0, a1 = 0.0;
+#pragma unroll
+    for (int u = 0; u < 32; u += 2) {
+        const int c = lane + 64 * u;
+        a0 += rc.mv[u] * (c < n ? vs[c] : 0.0);
+        a1 += rc.mv[u + 1] * (c + 64 < n ? vs[c + 64] : 0.0);
+    }
+    for (int c0 = 32 * 64; c0 < n; c0 += 32 * 64) {
+        double mv[32];
+#pragma unroll
+        for (int u = 0; u < 32; ++u) {
+            const int c = c0 + lane + 64 * u;
+            mv[u] = c < n ? R[c] : 0.0;
         }
 #pragma unroll
-        for (int u = 0; u < 8; u += 2) {
-            a0 += m0[u] * vv[u];
-            a1 += m0[u + 1] * vv[u + 1];
-            b0 += m1[u] * vv[u];
-            b1 += m1[u + 1] * vv[u + 1];
+        for (int u = 0; u < 32; u += 2) {
+            const int c = c0 + lane + 64 * u;
+            a0 += mv[u] * (c < n ? vs[c] : 0.0);
+            a1 += mv[u + 1] * (c + 64 < n ? vs[c + 64] : 0.0);
         }
     }
-    for (; c < n; c += 64) {
-        a0 += R0[c] * vec[c];
-        b0 += R1[c] * vec[c];
-    }
-    d0 = wave_sum(a0 + a1);
-    d1 = wave_sum(b0 + b1);
+    return wave_sum(a0 + a1);
 }
 
 // Returns true on the workgroup that finished last (the counter is then re-armed).  Every
@@ -162,43 +176,150 @@ __device__ inline void reduce_partials(const double* __restrict__ part, int nwg,
     __syncthreads();
 }
 
-// r0 = b - M x0 into V[0]; last workgroup sets beta = |r0|, |b|, g[0], done.
+__device__ inline double block_sum4(double v, double* red4) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    v = wave_sum(v);
+    if (lane == 0) red4[wv] = v;
+    __syncthreads();
+    const double s = red4[0] + red4[1] + red4[2] + red4[3];
+    __syncthreads();
+    return s;
+}
+
+// u = w - sum_{i<j} co[i] V[i] over all n entries into vs (LDS); returns |u|^2 over the workgroup.
+// Each thread handles 8 entries per pass with the basis loads of 4 vectors in flight together.
+// The entries of rows [r0, r0 + kRowsPerWG) are also stored into Vj (when not null).
+__device__ inline void load_first_pass(const double* __restrict__ w, int n, double (&u)[8]) {
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+        const int c = threadIdx.x + 256 * e;
+        u[e] = c < n ? w[c] : 0.0;
+    }
+}
+__device__ inline double rebuild_vector(const double* __restrict__ w, const double* __restrict__ V,
+                                        const double* co, int j, int n, int r0, double* vs, double* Vj,
+                                        double* red4, const double* pre = nullptr) {
+    const int t = threadIdx.x;
+    double ss = 0.0;
+    for (int base = 0; base < n; base += 8 * 256) {
+        double u[8];
+        if (pre && base == 0) {
+#pragma unroll
+            for (int e = 0; e < 8; ++e) u[e] = pre[e];
+        } else {
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                const int c = base + t + 256 * e;
+                u[e] = c < n ? w[c] : 0.0;
+            }
+        }
+        int i = 0;
+        for (; i + 4 <= j; i += 4) {
+            double vv[4][8];
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+#pragma unroll
+                for (int e = 0; e < 8; ++e) {
+                    const int c = base + t + 256 * e;
+                    vv[q][e] = c < n ? V[(long long)(i + q) * n + c] : 0.0;
+                }
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const double cq = co[i + q];
+#pragma unroll
+                for (int e = 0; e < 8; ++e) u[e] -= cq * vv[q][e];
+            }
+        }
+        for (; i < j; ++i) {
+            double vv[8];
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                const int c = base + t + 256 * e;
+                vv[e] = c < n ? V[(long long)i * n + c] : 0.0;
+            }
+            const double ci = co[i];
+#pragma unroll
+            for (int e = 0; e < 8; ++e) u[e] -= ci * vv[e];
+        }
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const int c = base + t + 256 * e;
+            if (c < n) {
+                vs[c] = u[e];
+                ss += u[e] * u[e];
+                if (Vj && c >= r0 && c < r0 + kRowsPerWG) Vj[c] = u[e];
+            }
+        }
+    }
+    return block_sum4(ss, red4);
+}
+
+// Applies the stored rotations to column jc (entries 0..jc+1 in col), forms the new rotation and
+// updates g; returns |g[jc+1]|, the residual estimate.
+__device__ inline double givens_column(double* col, double* cs, double* sn, double* g, int jc) {
+    for (int q = 0; q < jc; ++q) {
+        const double t0 = cs[q] * col[q] + sn[q] * col[q + 1];
+        const double t1 = -sn[q] * col[q] + cs[q] * col[q + 1];
+        col[q] = t0;
+        col[q + 1] = t1;
+    }
+    const double den = hypot(col[jc], col[jc + 1]);
+    const double c = den > 0.0 ? col[jc] / den : 1.0;
+    const double s = den > 0.0 ? col[jc + 1] / den : 0.0;
+    cs[jc] = c;
+    sn[jc] = s;
+    col[jc] = den;
+    col[jc + 1] = 0.0;
+    g[jc + 1] = -s * g[jc];
+    g[jc] = c * g[jc];
+    return fabs(g[jc + 1]);
+}
+
+// r0 = b - M x0 into V[0]; last workgroup sets beta = |r0|, |b|, g[0], sc[0], done.
 __global__ __launch_bounds__(256) void gm_init_kernel(GmArgs a, const SurfDesc* sd) {
     const GmView v = gm_view(a, sd);
     if ((int)blockIdx.x >= v.nwg) return;
+    __shared__ double vs[kGmMaxN];
     __shared__ double rr[kRowsPerWG], bb[kRowsPerWG];
     __shared__ double sums[kMaxKrylov + 2];
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, n = v.n;
-    const int r0 = blockIdx.x * kRowsPerWG + wv * 2;
-    const int ra = min(r0, n - 1), rb = min(r0 + 1, n - 1);
-    double d0, d1;
-    rows_dot2(v.M + (long long)ra * n, v.M + (long long)rb * n, v.x, n, lane, d0, d1);
-    if (lane == 0) {
-        for (int q = 0; q < 2; ++q) {
-            const int row = r0 + q;
-            double r = 0.0, bv = 0.0;
-            if (row < n) {
-                bv = v.b[row];
-                r = bv - (q ? d1 : d0);
-                v.V[row] = r;
-            }
-            rr[wv * 2 + q] = r * r;
-            bb[wv * 2 + q] = bv * bv;
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6, n = v.n;
+    const int row = blockIdx.x * kRowsPerWG + wv;
+    const double* R = v.M + (long long)min(row, n - 1) * n;
+    RowChunk rc;
+    row_load(R, n, lane, rc);   // in flight while x0 is staged
+    const int hv = a.stage > 0 ? a.hvalid[a.stage - 1] : 0;
+    const int d_voff = (int)(v.x - a.x);
+    if (hv == 0) {
+        for (int c = t; c < n; c += 256) vs[c] = v.x[c];
+    } else {
+        const double* h1 = a.hist + (long long)(2 * a.stage - 2) * a.total + d_voff;
+        const double* h2 = h1 + a.total;
+        for (int c = t; c < n; c += 256) {
+            const double x0 = hv >= 2 ? 2.0 * h1[c] - h2[c] : h1[c];
+            vs[c] = x0;
+            if (c >= blockIdx.x * kRowsPerWG && c < (blockIdx.x + 1) * kRowsPerWG) v.x[c] = x0;
         }
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
-        double s1 = 0.0, s2 = 0.0;
-        for (int i = 0; i < kRowsPerWG; ++i) {
-            s1 += rr[i];
-            s2 += bb[i];
+    const double d = row < n ? row_dot_rest(R, rc, vs, n, lane) : 0.0;
+    if (lane == 0) {
+        double r = 0.0, bv = 0.0;
+        if (row < n) {
+            bv = v.b[row];
+            r = bv - d;
+            v.V[row] = r;
         }
-        v.part[blockIdx.x] = s1;
-        v.part[v.nwg + blockIdx.x] = s2;
+        rr[wv] = r * r;
+        bb[wv] = bv * bv;
+    }
+    __syncthreads();
+    if (t == 0) {
+        v.part[blockIdx.x] = rr[0] + rr[1] + rr[2] + rr[3];
+        v.part[v.nwg + blockIdx.x] = bb[0] + bb[1] + bb[2] + bb[3];
     }
     if (!last_workgroup(v.ctl, v.nwg)) return;
     reduce_partials(v.part, v.nwg, 2, sums);
-    if (threadIdx.x == 0) {
+    if (t == 0) {
         const double beta = sqrt(sums[0]), bnorm = sqrt(sums[1]);
         GmCtl* c = v.ctl;
         c->beta = beta;
@@ -211,100 +332,68 @@ __global__ __launch_bounds__(256) void gm_init_kernel(GmArgs a, const SurfDesc* 
     }
 }
 
-// Arnoldi step j, first half: w = M v_j; h_i = v_i . w for i <= j (into h1 and column j of H).
-__global__ __launch_bounds__(256) void gm_arnoldi_kernel(GmArgs a, const SurfDesc* sd, int j) {
+// Arnoldi step j (see the file comment).
+__global__ __launch_bounds__(256) void gm_iter_kernel(GmArgs a, const SurfDesc* sd, int j) {
     const GmView v = gm_view(a, sd);
     if ((int)blockIdx.x >= v.nwg || v.ctl->done) return;
+    __shared__ double vs[kGmMaxN];
+    __shared__ double co[kMaxKrylov + 1];
+    __shared__ double red4[4];
     __shared__ double wl[kRowsPerWG];
-    __shared__ double scratch[kRowsPerWG * 32];
     __shared__ double sums[kMaxKrylov + 2];
     const int t = threadIdx.x, lane = t & 63, wv = t >> 6, n = v.n, m = a.m;
-    const int row0 = blockIdx.x * kRowsPerWG;
-    const int r = wv * 2 + (lane >> 5), i = lane & 31;
-    const int row = row0 + r;
-    // this thread's basis element, fetched before the long matvec
-    const double vi = (i <= j && row < n) ? v.V[(long long)i * n + row] : 0.0;
+    const int r0 = blockIdx.x * kRowsPerWG;
+    double* wj = (j & 1) ? v.w1 : v.w0;
+    const int row = r0 + wv;
+    const double* R = v.M + (long long)min(row, n - 1) * n;
+    RowChunk rc;
+    row_load(R, n, lane, rc);   // the matrix row does not depend on v_j: in flight first
+    // operands of the partial dots (v_i of this workgroup's rows, i < j), also independent of v_j
+    double vrow[kRowsPerWG];
+#pragma unroll
+    for (int r = 0; r < kRowsPerWG; ++r)
+        vrow[r] = (t < j && r0 + r < n) ? v.V[(long long)t * n + r0 + r] : 0.0;
+    const double sct = (t < j) ? v.sc[t] : 0.0;
     const double sj = v.sc[j];
-    {
-        const int ra = min(row0 + wv * 2, n - 1), rb = min(row0 + wv * 2 + 1, n - 1);
-        double d0, d1;
-        rows_dot2(v.M + (long long)ra * n, v.M + (long long)rb * n, v.V + (long long)j * n, n, lane, d0, d1);
-        if (lane == 0) {
-            wl[wv * 2] = (row0 + wv * 2 < n) ? sj * d0 : 0.0;
-            wl[wv * 2 + 1] = (row0 + wv * 2 + 1 < n) ? sj * d1 : 0.0;
+    for (int c = t; c < n; c += 256) vs[c] = sj * v.V[(long long)j * n + c];
+    __syncthreads();
+    // w_j = M v_j for this workgroup's rows
+    const double d = row < n ? row_dot_rest(R, rc, vs, n, lane) : 0.0;    if (lane == 0) {
+        wl[wv] = row < n ? d : 0.0;
+        if (row < n) wj[row] = d;
+    }
+    __syncthreads();
+    // partial dots v_i . w_j, i <= j
+    if (t <= j) {
+        double s = 0.0;
+#pragma unroll
+        for (int r = 0; r < kRowsPerWG; ++r) {
+            const int rw = r0 + r;
+            if (rw < n) {
+                const double vi = (t == j) ? vs[rw] : sct * vrow[r];
+                s += vi * wl[r];
+            }
         }
-    }
-    __syncthreads();
-    if (t < kRowsPerWG && row0 + t < n) v.w[row0 + t] = wl[t];
-    scratch[r * 32 + i] = (i <= j) ? vi * wl[r] : 0.0;
-    __syncthreads();
-    if (t <= j) {
-        double s = 0.0;
-#pragma unroll
-        for (int rr = 0; rr < kRowsPerWG; ++rr) s += scratch[rr * 32 + t];
-        v.part[t * v.nwg + blockIdx.x] = s * v.sc[t];
+        v.part[t * v.nwg + blockIdx.x] = s;
     }
     if (!last_workgroup(v.ctl, v.nwg)) return;
+    double wpre[8];
+    load_first_pass(wj, n, wpre);   // in flight together with the partial sums
     reduce_partials(v.part, v.nwg, j + 1, sums);
-    if (t <= j) {
-        v.h1[t] = sums[t];
-        v.H[(long long)j * (m + 1) + t] = sums[t];
-    }
-}
-
-// Arnoldi step j, second half: u = w - sum_{i<=j} h_i v_i into V[j+1]; the last workgroup takes
-// h_{j+1,j} = |u|, applies the Givens rotations to column j and tests the residual.
-__global__ __launch_bounds__(256) void gm_extend_kernel(GmArgs a, const SurfDesc* sd, int j) {
-    const GmView v = gm_view(a, sd);
-    if ((int)blockIdx.x >= v.nwg || v.ctl->done) return;
-    __shared__ double ul[kRowsPerWG];
-    __shared__ double sums[kMaxKrylov + 2];
-    const int t = threadIdx.x, lane = t & 63, wv = t >> 6, n = v.n, m = a.m;
-    const int row0 = blockIdx.x * kRowsPerWG;
-    const int r = wv * 2 + (lane >> 5), i = lane & 31;
-    const int row = row0 + r;
-    const double vi = (i <= j && row < n) ? v.V[(long long)i * n + row] : 0.0;
-    const double wr = row < n ? v.w[row] : 0.0;
-    double p = (i <= j) ? v.h1[i] * v.sc[i] * vi : 0.0;
-    for (int o = 16; o > 0; o >>= 1) p += __shfl_xor(p, o, 64);
-    const double u = row < n ? wr - p : 0.0;
-    if (i == 0) {
-        if (row < n) v.V[(long long)(j + 1) * n + row] = u;
-        ul[r] = u * u;
-    }
+    if (t <= j) co[t] = sums[t] * v.sc[t];
     __syncthreads();
-    if (t == 0) {
-        double s = 0.0;
-#pragma unroll
-        for (int rr = 0; rr < kRowsPerWG; ++rr) s += ul[rr];
-        v.part[blockIdx.x] = s;
-    }
-    if (!last_workgroup(v.ctl, v.nwg)) return;
-    reduce_partials(v.part, v.nwg, 1, sums);
+    // u = w_j - sum_{i<=j} h_{i,j} v_i, |u|, V[j+1] = u (every entry of this one workgroup)
+    const double hn = sqrt(rebuild_vector(wj, v.V, co, j + 1, n, 0, vs, nullptr, red4, wpre));
+    for (int c = t; c < n; c += 256) v.V[(long long)(j + 1) * n + c] = vs[c];
     if (t == 0) {
         GmCtl* c = v.ctl;
-        double* Hj = v.H + (long long)j * (m + 1);
-        const double hn = sqrt(sums[0]);
-        Hj[j + 1] = hn;
-        for (int q = 0; q < j; ++q) {
-            const double t0 = v.cs[q] * Hj[q] + v.sn[q] * Hj[q + 1];
-            const double t1 = -v.sn[q] * Hj[q] + v.cs[q] * Hj[q + 1];
-            Hj[q] = t0;
-            Hj[q + 1] = t1;
-        }
-        const double den = hypot(Hj[j], Hj[j + 1]);
-        const double cj = den > 0.0 ? Hj[j] / den : 1.0;
-        const double sn = den > 0.0 ? Hj[j + 1] / den : 0.0;
-        v.cs[j] = cj;
-        v.sn[j] = sn;
-        Hj[j] = den;
-        Hj[j + 1] = 0.0;
-        v.g[j + 1] = -sn * v.g[j];
-        v.g[j] = cj * v.g[j];
-        c->res = fabs(v.g[j + 1]);
+        double* col = v.H + (long long)j * (m + 1);
+        for (int q = 0; q <= j; ++q) col[q] = sums[q];
+        col[j + 1] = hn;
+        v.sc[j + 1] = hn > 0.0 ? 1.0 / hn : 0.0;
+        c->res = givens_column(col, v.cs, v.sn, v.g, j);
         c->iters = j + 1;
         c->hnext = hn;
-        v.sc[j + 1] = hn > 0.0 ? 1.0 / hn : 0.0;
         if (c->res <= a.tol * c->bnorm || hn == 0.0) {
             c->done = 1;
         } else if (j + 1 >= m) {
@@ -314,15 +403,14 @@ __global__ __launch_bounds__(256) void gm_extend_kernel(GmArgs a, const SurfDesc
     }
 }
 
-// x += sum_i y_i v_i with R y = g (R = rotated H) solved by every workgroup redundantly.
+// x += sum_i y_i v_i with R y = g (R = the rotated Hessenberg matrix), y solved by every
+// workgroup redundantly.
 __global__ __launch_bounds__(256) void gm_finish_kernel(GmArgs a, const SurfDesc* sd) {
     const GmView v = gm_view(a, sd);
-    const int n = v.n, m = a.m;
+    const int n = v.n, m = a.m, t = threadIdx.x;
     const int k = v.ctl->iters;
-    if (k == 0) return;
-    __shared__ double y[kMaxKrylov];
-    __shared__ double ys[kMaxKrylov];
-    if (threadIdx.x == 0) {
+    __shared__ double y[kMaxKrylov], ys[kMaxKrylov];
+    if (k > 0 && t == 0) {
         for (int i = k - 1; i >= 0; --i) {
             double s = v.g[i];
             for (int l = i + 1; l < k; ++l) s -= v.H[(long long)l * (m + 1) + i] * y[l];
@@ -331,16 +419,26 @@ __global__ __launch_bounds__(256) void gm_finish_kernel(GmArgs a, const SurfDesc
         for (int i = 0; i < k; ++i) ys[i] = y[i] * v.sc[i];
     }
     __syncthreads();
-    for (int row = blockIdx.x * blockDim.x + threadIdx.x; row < n; row += gridDim.x * blockDim.x) {
-        double vv[kMaxKrylov];
-#pragma unroll
-        for (int i = 0; i < kMaxKrylov; ++i) vv[i] = (i < k) ? v.V[(long long)i * n + row] : 0.0;
+    double* h1 = nullptr;
+    if (a.stage > 0) h1 = a.hist + (long long)(2 * a.stage - 2) * a.total + (v.x - a.x);
+    for (int row = blockIdx.x * blockDim.x + t; row < n; row += gridDim.x * blockDim.x) {
         double s = v.x[row];
+        if (k > 0) {
+            double vv[kMaxKrylov];
 #pragma unroll
-        for (int i = 0; i < kMaxKrylov; ++i)
-            if (i < k) s += ys[i] * vv[i];
-        v.x[row] = s;
+            for (int i = 0; i < kMaxKrylov; ++i) vv[i] = (i < k) ? v.V[(long long)i * n + row] : 0.0;
+#pragma unroll
+            for (int i = 0; i < kMaxKrylov; ++i)
+                if (i < k) s += ys[i] * vv[i];
+            v.x[row] = s;
+        }
+        if (h1) {   // shift this stage's history
+            h1[a.total + row] = h1[row];
+            h1[row] = s;
+        }
     }
+    if (a.stage > 0 && blockIdx.x == 0 && blockIdx.z == 0 && t == 0)
+        a.hvalid[a.stage - 1] = min(a.hvalid[a.stage - 1] + 1, 2);
 }
 
 }  // namespace sfh

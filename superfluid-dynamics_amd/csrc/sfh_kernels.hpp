@@ -66,20 +66,26 @@ struct GmArgs {
     const double* b;     // right-hand sides
     double* x;           // initial guess in, solution out
     double* V;           // Krylov basis, (m+1) * voff per surface
-    double* w;           // work vector
+    double* w;           // two product vectors of total nodes each (even / odd steps)
     double* part;        // per-workgroup partial sums, part_stride per surface
     double* small;       // per surface: scales, coefficients, rotations, g, H
     int* ctl;            // per surface control block
     long long part_stride;
     long long small_stride;
     int m;               // maximal number of iterations (Krylov dimension)
-    int pad;
+    int total;           // nodes of all surfaces (stride between the two product vectors)
     double tol;
+    // warm start by extrapolation over steps: stage s (1..4) keeps its solutions of the last two
+    // steps at hist + (2 s - 2) total (newest) and hist + (2 s - 1) total; hvalid[s - 1] counts
+    // them (0..2).  stage 0: plain warm start from x.
+    double* hist;
+    int* hvalid;
+    int stage;
+    int pad;
 };
 constexpr int kGmCtlInts = 16;   // ints per control block (holds the device struct)
 __global__ void gm_init_kernel(GmArgs a, const SurfDesc* sd);
-__global__ void gm_arnoldi_kernel(GmArgs a, const SurfDesc* sd, int j);
-__global__ void gm_extend_kernel(GmArgs a, const SurfDesc* sd, int j);
+__global__ void gm_iter_kernel(GmArgs a, const SurfDesc* sd, int j);
 __global__ void gm_finish_kernel(GmArgs a, const SurfDesc* sd);
 
 // ---- batched.hip (persistent per-surface stage kernel, ensembles with every N <= kBatchMaxN)

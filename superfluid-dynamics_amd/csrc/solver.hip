@@ -140,16 +140,19 @@ Solver::Solver(const Phys& ph, const std::vector<int>& n, const Options& opt) : 
     SFH_HIP(hipMemset(mup_, 0, T_ * sizeof(double2)));
     if (opt_.solver == kSolveGMRES) {
         gm_m_ = std::max(1, std::min(opt_.max_iter, 32));   // kMaxKrylov in gmres.hip
-        gm_nwg_ = (maxN_ + 7) / 8;
+        gm_nwg_ = (maxN_ + 3) / 4;   // kRowsPerWG in gmres.hip
         gm_part_stride_ = (long long)gm_nwg_ * (gm_m_ + 2);
         gm_small_stride_ = 3LL * (gm_m_ + 1) + 2LL * gm_m_ + (long long)gm_m_ * (gm_m_ + 1);
         gmV_ = dalloc<double>((size_t)(gm_m_ + 1) * T_);
-        gmw_ = dalloc<double>(T_);
+        gmw_ = dalloc<double>(2 * (size_t)T_);
         gmpart_ = dalloc<double>(gm_part_stride_ * S_);
         gmsmall_ = dalloc<double>(gm_small_stride_ * S_);
         gmctl_ = dalloc<int>((size_t)kGmCtlInts * S_);
         SFH_HIP(hipMemset(gmctl_, 0, sizeof(int) * kGmCtlInts * S_));
         SFH_HIP(hipMemset(gmsmall_, 0, sizeof(double) * gm_small_stride_ * S_));
+        gmhist_ = dalloc<double>(8 * (size_t)T_);
+        gmhvalid_ = dalloc<int>(4);
+        SFH_HIP(hipMemset(gmhvalid_, 0, 4 * sizeof(int)));
     }
     if (opt_.solver == kSolveBatched) {
         std::vector<int> ord(S_);
@@ -169,7 +172,8 @@ Solver::~Solver() {
     if (stream_) (void)hipStreamSynchronize(stream_);
     void* bufs[] = {d_desc_, d_tw_, Z0_, Za_, Zb_, P0_, Pa_, Pb_, kZ_[0], kZ_[1], kZ_[2], kP_[0],
                     kP_[1], kP_[2], Zp_, Zpp_, mup_, qup_, b_, work_, y_, mu_, M_, uinv_, flags_,
-                    status_, piv_, counters_, gmV_, gmw_, gmpart_, gmsmall_, gmctl_, trig_, order_, bstats_};
+                    status_, piv_, counters_, gmV_, gmw_, gmpart_, gmsmall_, gmctl_, trig_, order_, bstats_,
+                    gmhist_, gmhvalid_};
     for (void* p : bufs)
         if (p) (void)hipFree(p);
     void* vbufs[] = {vY_[0], vY_[1], vY_[2], vK_[0], vK_[1], vK_[2], vYP_[0], vYP_[1], vYP_[2], vKP_[0],
@@ -223,7 +227,7 @@ Solver::KernelReport Solver::kernel_report(const std::string& tag) {
 
 // GMRES on the stored FP64 matrices of all surfaces at once (gridDim.z = surface), warm started
 // from the previous mu.
-void Solver::enqueue_gmres() {
+void Solver::enqueue_gmres(int stage) {
     GmArgs a{};
     a.M = M_;
     a.b = b_;
@@ -236,7 +240,11 @@ void Solver::enqueue_gmres() {
     a.part_stride = gm_part_stride_;
     a.small_stride = gm_small_stride_;
     a.m = gm_m_;
+    a.total = T_;
     a.tol = opt_.tol;
+    a.hist = gmhist_;
+    a.hvalid = gmhvalid_;
+    a.stage = stage;
     double nsq = 0.0;
     for (int v : n_) nsq += (double)v * v;
     const dim3 grid(gm_nwg_, 1, S_);
@@ -245,13 +253,10 @@ void Solver::enqueue_gmres() {
         hipLaunchKernelGGL(gm_init_kernel, grid, dim3(256), 0, stream_, a, (const SurfDesc*)d_desc_);
     });
     for (int j = 0; j < gm_m_; ++j) {
+        // product with the stored matrix + dots + the next basis vector (j + 1 vectors)
         launch(
-            "gm_arnoldi", 2.0 * nsq + 2.0 * (j + 1) * T_, 8.0 * nsq + 8.0 * (j + 2) * T_,
-            [&] { hipLaunchKernelGGL(gm_arnoldi_kernel, grid, dim3(256), 0, stream_, a, (const SurfDesc*)d_desc_, j); },
-            j);
-        launch(
-            "gm_extend", 2.0 * (j + 2) * T_, 8.0 * (j + 3) * T_,
-            [&] { hipLaunchKernelGGL(gm_extend_kernel, grid, dim3(256), 0, stream_, a, (const SurfDesc*)d_desc_, j); },
+            "gm_iter", 2.0 * nsq + 4.0 * (j + 1) * T_, 8.0 * nsq + 8.0 * (j + 3) * T_,
+            [&] { hipLaunchKernelGGL(gm_iter_kernel, grid, dim3(256), 0, stream_, a, (const SurfDesc*)d_desc_, j); },
             j);
     }
     launch("gm_finish", 2.0 * gm_m_ * T_, 8.0 * gm_m_ * T_, [&] {
@@ -270,9 +275,9 @@ void Solver::enqueue_gmres() {
 }
 
 // M mu = b for every surface; M, b (and its working copy) are ready in stream order.
-void Solver::enqueue_solve(const double2* Z) {
+void Solver::enqueue_solve(const double2* Z, int stage) {
     if (opt_.solver == kSolveGMRES) {
-        enqueue_gmres();
+        enqueue_gmres(stage);
         return;
     }
     auto run = [&](int s0, int count, int n) {
@@ -372,7 +377,7 @@ void Solver::enqueue_rhs(const double2* Z, const double* P, const StageCtl& ctl)
         hipLaunchKernelGGL(assemble64_kernel, dim3((maxN_ + 3) / 4, S_), dim3(256), 0, stream_, d_desc_,
                            (const NodeTrig*)trig_, Z, Zp_, Zpp_, M_, ph_, (const int*)nullptr);
     });
-    enqueue_solve(Z);
+    enqueue_solve(Z, ctl.stage);
     launch("mu_deriv", 2 * nlog, 24.0 * T_, [&] {
         hipLaunchKernelGGL(mu_deriv_kernel, dim3(S_), dim3(kDerivThreads), lds, stream_, d_desc_, d_tw_, mu_, mup_,
                            lds_tw);
@@ -444,6 +449,7 @@ void Solver::set_state(const double* x, const double* y, const double* phi) {
     for (int i = 0; i < T_; ++i) z[i] = make_double2(x[i], y[i]);
     SFH_HIP(hipMemcpyAsync(Z0_, z.data(), T_ * sizeof(double2), hipMemcpyHostToDevice, stream_));
     SFH_HIP(hipMemcpyAsync(P0_, phi, T_ * sizeof(double), hipMemcpyHostToDevice, stream_));
+    if (gmhvalid_) SFH_HIP(hipMemsetAsync(gmhvalid_, 0, 4 * sizeof(int), stream_));   // a new trajectory
     SFH_HIP(hipStreamSynchronize(stream_));
 }
 

@@ -104,7 +104,7 @@ class Solver {
 
   private:
     void enqueue_rhs(const double2* Z, const double* P, const StageCtl& ctl);
-    void enqueue_solve(const double2* Z);
+    void enqueue_solve(const double2* Z, int stage);
     void enqueue_step(double dt);
     void capture(double dt);
     void check_status();
@@ -153,7 +153,9 @@ class Solver {
     double *gmV_ = nullptr, *gmw_ = nullptr, *gmpart_ = nullptr, *gmsmall_ = nullptr;
     int* gmctl_ = nullptr;
     long long gm_part_stride_ = 0, gm_small_stride_ = 0;
-    void enqueue_gmres();
+    void enqueue_gmres(int stage);
+    double* gmhist_ = nullptr;   // 8 x total: per RK stage, mu of the last two steps
+    int* gmhvalid_ = nullptr;
     // persistent per-surface stage kernel (kSolveBatched)
     int* order_ = nullptr;    // workgroup -> surface, by decreasing N
     int* bstats_ = nullptr;   // 4 ints per surface
