@@ -108,7 +108,14 @@ struct BatchArgs {
     int mixed;
     int cycles;                // GMRES restart cycles (FP64)
     int max_refine;            // refinement sweeps (mixed)
+    int nsurf;
+    int total;                 // nodes of all surfaces
     int pad;
+    // warm start by extrapolation over steps: stage s (1..4) keeps each surface's mu of the last two
+    // steps at hist + (2 s - 2) total (newest) and hist + (2 s - 1) total; hvalid[(s - 1) nsurf + i]
+    // counts them for surface i (0..2)
+    double* hist;
+    int* hvalid;
 };
 __global__ void batched_stage_kernel(BatchArgs a, Phys ph, StageCtl ctl);
 

@@ -162,6 +162,9 @@ Solver::Solver(const Phys& ph, const std::vector<int>& n, const Options& opt) : 
         SFH_HIP(hipMemcpy(order_, ord.data(), S_ * sizeof(int), hipMemcpyHostToDevice));
         bstats_ = dalloc<int>(4 * (size_t)S_);
         SFH_HIP(hipMemset(bstats_, 0, 4 * S_ * sizeof(int)));
+        bhist_ = dalloc<double>(8 * (size_t)T_);
+        bhvalid_ = dalloc<int>(4 * (size_t)S_);
+        SFH_HIP(hipMemset(bhvalid_, 0, 4 * S_ * sizeof(int)));
     }
     SFH_HIP(hipDeviceSynchronize());
 }
@@ -173,7 +176,7 @@ Solver::~Solver() {
     void* bufs[] = {d_desc_, d_tw_, Z0_, Za_, Zb_, P0_, Pa_, Pb_, kZ_[0], kZ_[1], kZ_[2], kP_[0],
                     kP_[1], kP_[2], Zp_, Zpp_, mup_, qup_, b_, work_, y_, mu_, M_, uinv_, flags_,
                     status_, piv_, counters_, gmV_, gmw_, gmpart_, gmsmall_, gmctl_, trig_, order_, bstats_,
-                    gmhist_, gmhvalid_};
+                    gmhist_, gmhvalid_, bhist_, bhvalid_};
     for (void* p : bufs)
         if (p) (void)hipFree(p);
     void* vbufs[] = {vY_[0], vY_[1], vY_[2], vK_[0], vK_[1], vK_[2], vYP_[0], vYP_[1], vYP_[2], vKP_[0],
@@ -342,6 +345,10 @@ void Solver::enqueue_batched(const double2* Z, const double* P, const StageCtl& 
     a.mixed = opt_.precision == kMixed ? 1 : 0;
     a.cycles = std::max(1, (opt_.max_iter + 7) / 8);   // kBK = 8 vectors per cycle
     a.max_refine = 6;
+    a.nsurf = S_;
+    a.total = T_;
+    a.hist = bhist_;
+    a.hvalid = bhvalid_;
     double nsq = 0.0;
     for (int v : n_) nsq += (double)v * v;
     const double img = ph_.image ? 2.0 : 1.0;
@@ -450,6 +457,7 @@ void Solver::set_state(const double* x, const double* y, const double* phi) {
     SFH_HIP(hipMemcpyAsync(Z0_, z.data(), T_ * sizeof(double2), hipMemcpyHostToDevice, stream_));
     SFH_HIP(hipMemcpyAsync(P0_, phi, T_ * sizeof(double), hipMemcpyHostToDevice, stream_));
     if (gmhvalid_) SFH_HIP(hipMemsetAsync(gmhvalid_, 0, 4 * sizeof(int), stream_));   // a new trajectory
+    if (bhvalid_) SFH_HIP(hipMemsetAsync(bhvalid_, 0, 4 * S_ * sizeof(int), stream_));
     SFH_HIP(hipStreamSynchronize(stream_));
 }
 

@@ -159,6 +159,8 @@ class Solver {
     // persistent per-surface stage kernel (kSolveBatched)
     int* order_ = nullptr;    // workgroup -> surface, by decreasing N
     int* bstats_ = nullptr;   // 4 ints per surface
+    double* bhist_ = nullptr;  // per RK stage, each surface's mu of the last two steps
+    int* bhvalid_ = nullptr;
     void enqueue_batched(const double2* Z, const double* P, const StageCtl& ctl);
     // adjoint workspace (allocated on first use)
     void ensure_vjp();
