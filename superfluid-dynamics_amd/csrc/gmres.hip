@@ -30,11 +30,12 @@ constexpr int kGmMaxN = 4096;      // LDS copy of the vector being multiplied
 
 struct GmCtl {
     int done;
-    int counter;
+    int counter;       // top-level arrival counter
     int iters;
     int unconverged;
     double bnorm, beta, res;
     double hnext;
+    int group[16];     // group arrival counters
     int pad[4];
 };
 static_assert(sizeof(GmCtl) == kGmCtlInts * sizeof(int), "host reads control blocks with this stride");
@@ -120,20 +121,32 @@ __device__ inline double row_dot_rest(const double* __restrict__ R, const RowChu
     return wave_sum(a0 + a1);
 }
 
-// Returns true on the workgroup that finished last (the counter is then re-armed).  Every
-// workgroup increments with release semantics (its partial sums become visible at agent scope);
-// only the last one pays for the acquire.  On gfx950 an agent-scope acquire invalidates the L2 of
-// the executing XCD, so a full fence in every workgroup would evict the data other workgroups are
-// streaming.
+// Returns true on the workgroup that finished last (the counters are then re-armed).
+// Arrival is two-level: workgroup w increments the counter of group w % 16 (release), the last
+// member of each group increments the top counter, so no single address takes more than
+// ceil(nwg / 16) atomics (one shared counter serialises all nwg of them).  The last group member
+// acquires before passing its group on, and only the very last workgroup acquires at the top.  On
+// gfx950 an agent-scope acquire invalidates the L2 of the executing XCD, so acquiring in every
+// workgroup would evict the data other workgroups are streaming.
 __device__ inline bool last_workgroup(GmCtl* ctl, int nwg) {
+    constexpr int kGroups = 16;
     __shared__ int is_last;
     __syncthreads();
     if (threadIdx.x == 0) {
-        const int prev = __hip_atomic_fetch_add(&ctl->counter, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        is_last = (prev == nwg - 1);
-        if (is_last) {
-            __hip_atomic_store(&ctl->counter, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const int grp = blockIdx.x % kGroups;
+        const int ngroups = nwg < kGroups ? nwg : kGroups;
+        const int members = (nwg - grp + kGroups - 1) / kGroups;
+        is_last = 0;
+        const int prev = __hip_atomic_fetch_add(&ctl->group[grp], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        if (prev == members - 1) {
+            __hip_atomic_store(&ctl->group[grp], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            const int top = __hip_atomic_fetch_add(&ctl->counter, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            if (top == ngroups - 1) {
+                __hip_atomic_store(&ctl->counter, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+                is_last = 1;
+            }
         }
     }
     __syncthreads();
@@ -334,8 +347,10 @@ __global__ __launch_bounds__(256) void gm_init_kernel(GmArgs a, const SurfDesc* 
 
 // Arnoldi step j (see the file comment).
 __global__ __launch_bounds__(256) void gm_iter_kernel(GmArgs a, const SurfDesc* sd, int j) {
+    // converged solves: one load and out (most launches of the fixed sequence end here)
+    if (reinterpret_cast<const GmCtl*>(a.ctl)[blockIdx.z].done) return;
     const GmView v = gm_view(a, sd);
-    if ((int)blockIdx.x >= v.nwg || v.ctl->done) return;
+    if ((int)blockIdx.x >= v.nwg) return;
     __shared__ double vs[kGmMaxN];
     __shared__ double co[kMaxKrylov + 1];
     __shared__ double red4[4];

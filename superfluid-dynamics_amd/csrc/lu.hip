@@ -86,75 +86,97 @@ __device__ inline void store_block(double* dst, const double* src, long long ld,
 
 // Factors D (kb x kb, no interchanges) into L\U and writes L^{-1} into Li and U^{-1} into Ui
 // (both zero outside the kb x kb corner).  Sets *bad if a multiplier exceeds 1 in magnitude.
+//
+// Register-resident: thread t owns row r = t / 4 and the 16 columns c = (t % 4) + 4 m of the
+// 64 x 64 block (rows from kb on are padded with identity rows).  Every elimination or
+// substitution step publishes one row through a double-buffered LDS row (one barrier per step);
+// the entry a[r][j] a row needs from its own row comes from the neighbouring lane by a shuffle.
+// The step loops are fully unrolled so every register index is a compile-time constant.
+__device__ inline double lane_col(const double (&a)[16], int col, int lane) {
+    // a[r][col] is slot col / 4 of the lane with part col % 4 in this row's group of 4 lanes
+    return __shfl(a[col >> 2], (lane & ~3) | (col & 3), 64);
+}
+
 __device__ void factor_invert_diag(double* D, double* Li, double* Ui, int kb, int* bad) {
-    const int t = threadIdx.x;
-    for (int j = 0; j < kb; ++j) {
-        const double inv = 1.0 / D[j * LD + j];
-        const int i = j + 1 + (t >> 2), part = t & 3;
-        if (i < kb) {
-            // all loads first: stores into row i must not serialise against the row-j loads
-            double rj[16], ri[16];
-            const double l = D[i * LD + j] * inv;
+    __shared__ double prow[2][kNB];
+    __shared__ double qrow[2][kNB];
+    const int t = threadIdx.x, r = t >> 2, part = t & 3, lane = t & 63;
+    double a[16];
+#pragma unroll
+    for (int m = 0; m < 16; ++m) {
+        const int c = part + 4 * m;
+        a[m] = r < kb ? D[r * LD + c] : (c == r ? 1.0 : 0.0);
+    }
+    int my_bad = 0;
+    // ---- elimination: a becomes L (strictly below the diagonal) \ U
+#pragma unroll
+    for (int j = 0; j < kNB - 1; ++j) {
+        double* pr = prow[j & 1];
+        if (r == j) {
+#pragma unroll
+            for (int m = 0; m < 16; ++m) pr[part + 4 * m] = a[m];
+        }
+        __syncthreads();
+        const double arj = lane_col(a, j, lane);
+        if (r > j) {
+            const double l = arj / pr[j];
+            if (r < kb && !(fabs(l) <= 1.0)) my_bad = 1;
 #pragma unroll
             for (int m = 0; m < 16; ++m) {
-                const int c = j + 1 + part + 4 * m;
-                rj[m] = (c < kb) ? D[j * LD + c] : 0.0;
-                ri[m] = (c < kb) ? D[i * LD + c] : 0.0;
+                const int c = part + 4 * m;
+                a[m] = (c > j) ? a[m] - l * pr[c] : (c == j ? l : a[m]);
             }
+        }
+    }
+    if (my_bad) *bad = 1;
+    __syncthreads();   // prow is reused below
+    // ---- inverses: L^{-1} top down and U^{-1} bottom up, one step of each per barrier
+    double wl[16], wu[16];
+#pragma unroll
+    for (int m = 0; m < 16; ++m) {
+        const int c = part + 4 * m;
+        wl[m] = (c == r) ? 1.0 : 0.0;
+        wu[m] = (c == r) ? 1.0 : 0.0;
+    }
+#pragma unroll
+    for (int s = 0; s < kNB; ++s) {
+        const int iL = s, iU = kNB - 1 - s;
+        double* pl = prow[s & 1];
+        double* pu = qrow[s & 1];
+        const double uii = lane_col(a, iU, lane);   // U[iU][iU] on the lanes of row iU
+        if (r == iL) {
+#pragma unroll
+            for (int m = 0; m < 16; ++m) pl[part + 4 * m] = wl[m];
+        }
+        if (r == iU) {
+            const double inv = 1.0 / uii;
 #pragma unroll
             for (int m = 0; m < 16; ++m) {
-                const int c = j + 1 + part + 4 * m;
-                if (c < kb) D[i * LD + c] = ri[m] - l * rj[m];
-            }
-            if (part == 0) {
-                D[i * LD + j] = l;
-                if (!(fabs(l) <= 1.0)) *bad = 1;
+                wu[m] *= inv;
+                pu[part + 4 * m] = wu[m];
             }
         }
         __syncthreads();
+        const double lri = lane_col(a, iL, lane);   // L[r][iL] (used for r > iL)
+        const double uri = lane_col(a, iU, lane);   // U[r][iU] (used for r < iU)
+        if (r > iL) {
+#pragma unroll
+            for (int m = 0; m < 16; ++m) wl[m] -= lri * pl[part + 4 * m];
+        }
+        if (r < iU) {
+#pragma unroll
+            for (int m = 0; m < 16; ++m) wu[m] -= uri * pu[part + 4 * m];
+        }
     }
-    for (int e = t; e < kNB * kNB; e += blockDim.x) {
-        const int r = e >> 6, c = e & 63;
-        Li[r * LD + c] = (r == c && r < kb) ? 1.0 : 0.0;
-        Ui[r * LD + c] = 0.0;
+#pragma unroll
+    for (int m = 0; m < 16; ++m) {
+        const int c = part + 4 * m;
+        const bool in = r < kb && c < kb;
+        D[r * LD + c] = in ? a[m] : 0.0;
+        Li[r * LD + c] = in ? wl[m] : 0.0;
+        Ui[r * LD + c] = in ? wu[m] : 0.0;
     }
     __syncthreads();
-    // row s of L^{-1} (top down) and row kb-1-s of U^{-1} (bottom up), 4 threads per column.
-    // Every operand is loaded before the first FMA so one LDS round trip covers the whole step.
-    const int c = t >> 2, part = t & 3;
-    for (int s = 0; s < kb; ++s) {
-        const int i = kb - 1 - s;
-        double dl[16], ll[16], du[16], uu[16];
-#pragma unroll
-        for (int m = 0; m < 16; ++m) {
-            // L^{-1}[s][c] = -sum_{q=c}^{s-1} L[s][q] Li[q][c]           (c < s)
-            const int ql = c + part + 4 * m;
-            const bool okl = (c < s) && (ql < s);
-            dl[m] = okl ? D[s * LD + ql] : 0.0;
-            ll[m] = okl ? Li[ql * LD + c] : 0.0;
-            // U^{-1}[i][c] = -(sum_{q=i+1}^{c} U[i][q] Ui[q][c]) / U[i][i]  (i < c < kb)
-            const int qu = i + 1 + part + 4 * m;
-            const bool oku = (c > i) && (c < kb) && (qu <= c);
-            du[m] = oku ? D[i * LD + qu] : 0.0;
-            uu[m] = oku ? Ui[qu * LD + c] : 0.0;
-        }
-        double accl = 0.0, accu = 0.0;
-#pragma unroll
-        for (int m = 0; m < 16; ++m) {
-            accl += dl[m] * ll[m];
-            accu += du[m] * uu[m];
-        }
-        accl += __shfl_xor(accl, 1, 64);
-        accl += __shfl_xor(accl, 2, 64);
-        accu += __shfl_xor(accu, 1, 64);
-        accu += __shfl_xor(accu, 2, 64);
-        if (part == 0) {
-            if (c < s) Li[s * LD + c] = -accl;
-            if (c == i) Ui[i * LD + c] = 1.0 / D[i * LD + i];
-            else if (c > i && c < kb) Ui[i * LD + c] = -accu / D[i * LD + i];
-        }
-        __syncthreads();
-    }
 }
 
 // Per-surface views for gridDim.z batches of equal size: surface s uses A + s n^2, vectors + s n,

@@ -83,7 +83,7 @@ struct GmArgs {
     int stage;
     int pad;
 };
-constexpr int kGmCtlInts = 16;   // ints per control block (holds the device struct)
+constexpr int kGmCtlInts = 32;   // ints per control block (holds the device struct)
 __global__ void gm_init_kernel(GmArgs a, const SurfDesc* sd);
 __global__ void gm_iter_kernel(GmArgs a, const SurfDesc* sd, int j);
 __global__ void gm_finish_kernel(GmArgs a, const SurfDesc* sd);
