@@ -8,7 +8,7 @@
 // :178-200 second), transforms back with the mirror-image decimation-in-frequency pass (so the
 // result lands in the same digit-reversed slots and no permutation pass is needed) and adds the
 // linear parts on the way out.
-#include "sfh_kernels.hpp"
+#include "sfh_gmres.hpp"
 #include "sfh_fft.hpp"
 
 namespace sfh {
@@ -32,7 +32,7 @@ __global__ __launch_bounds__(1024) void zphi_deriv_kernel(const SurfDesc* __rest
                                                           double* __restrict__ b,
                                                           double* __restrict__ work, Phys ph,
                                                           int* __restrict__ status, int lds_tw,
-                                                          NodeTrig* __restrict__ trig) {
+                                                          NodeTrig* __restrict__ trig, GmArgs gm, int gm_on) {
     extern __shared__ double2 s[];
     const SurfDesc d = sd[blockIdx.x];
     const int field = blockIdx.y;
@@ -69,18 +69,70 @@ __global__ __launch_bounds__(1024) void zphi_deriv_kernel(const SurfDesc* __rest
             work[off + j] = r;
         }
     }
+    // GMRES initial guess of this RK stage: its mu over the two previous steps extrapolated,
+    // 2 mu(n-1) - mu(n-2) (error second order in dt), else the last solve's mu (kept in place)
+    if (gm_on && field == 2 && gm.stage > 0) {
+        const int hv = gm.hvalid[gm.stage - 1];
+        if (hv > 0) {
+            const double* h1 = gm.hist + (long long)(2 * gm.stage - 2) * gm.total + off;
+            for (int j = threadIdx.x; j < n; j += blockDim.x)
+                gm.x[off + j] = hv >= 2 ? 2.0 * h1[j] - h1[gm.total + j] : h1[j];
+        }
+    }
 }
 
 // mu' = IFFT(first_mult(FFT(mu))) * 2 pi / n, mu real (BaseBoundaryIntegrator.cuh:201-203).
+// With the GMRES solver it first finishes the solve of its surface: y from the rotated
+// Hessenberg system R y = g (one thread), mu += sum_i y_i v_i, and the stage's history shift.
 __global__ __launch_bounds__(1024) void mu_deriv_kernel(const SurfDesc* __restrict__ sd,
                                                         const double2* __restrict__ tw_all,
-                                                        const double* __restrict__ mu,
-                                                        double2* __restrict__ mup, int lds_tw) {
+                                                        double* __restrict__ mu,
+                                                        double2* __restrict__ mup, int lds_tw, GmArgs gm,
+                                                        int gm_on) {
     extern __shared__ double2 s[];
+    __shared__ double ys[kMaxKrylov];
+    __shared__ int kk;
     const SurfDesc d = sd[blockIdx.x];
     const int n = d.n, off = d.voff;
     const double2* tw = stage_twiddles(s, d, tw_all, lds_tw);
-    for (int j = threadIdx.x; j < n; j += blockDim.x) s[lds_index(d, j)] = make_double2(mu[off + j], 0.0);
+    if (gm_on) {
+        const GmView g = gm_view(gm, sd, blockIdx.x);
+        const int m = gm.m;
+        if (threadIdx.x == 0) {
+            const int k = g.ctl->iters;
+            for (int i = k - 1; i >= 0; --i) {   // ys holds y first, then y_i sc[i]
+                double acc = g.g[i];
+                for (int l = i + 1; l < k; ++l) acc -= g.H[(long long)l * (m + 1) + i] * ys[l];
+                ys[i] = acc / g.H[(long long)i * (m + 1) + i];
+            }
+            for (int i = 0; i < k; ++i) ys[i] *= g.sc[i];
+            kk = k;
+        }
+        __syncthreads();
+        const int k = kk;
+        double* h1 = gm.stage > 0 ? gm.hist + (long long)(2 * gm.stage - 2) * gm.total + off : nullptr;
+        for (int j = threadIdx.x; j < n; j += blockDim.x) {
+            double x = mu[off + j];
+            for (int i0 = 0; i0 < k; i0 += 8) {   // 8 basis loads in flight per pass
+                double vv[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) vv[u] = i0 + u < k ? g.V[(long long)(i0 + u) * n + j] : 0.0;
+#pragma unroll
+                for (int u = 0; u < 8; ++u)
+                    if (i0 + u < k) x += ys[i0 + u] * vv[u];
+            }
+            mu[off + j] = x;
+            if (h1) {
+                h1[gm.total + j] = h1[j];
+                h1[j] = x;
+            }
+            s[lds_index(d, j)] = make_double2(x, 0.0);
+        }
+        if (h1 && blockIdx.x == 0 && threadIdx.x == 0)
+            gm.hvalid[gm.stage - 1] = min(gm.hvalid[gm.stage - 1] + 1, 2);
+    } else {
+        for (int j = threadIdx.x; j < n; j += blockDim.x) s[lds_index(d, j)] = make_double2(mu[off + j], 0.0);
+    }
     __syncthreads();
     derivative_in_lds(s, d, tw, false);
     const double sc = 2.0 * kPi / n;

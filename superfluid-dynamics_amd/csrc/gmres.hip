@@ -14,73 +14,17 @@
 //      basis vector u = w_j - sum_i h_{i,j} v_i over all N entries (classical Gram-Schmidt, one
 //      pass), its norm h_{j+1,j} = |u|, stores V[j+1] = u and sc[j+1] = 1 / |u|, applies the
 //      Givens rotations to column j and tests the residual estimate.
-// Once the residual estimate reaches tol * |b| the remaining launches of the fixed-length sequence
-// return at once; gm_finish solves the small triangular system and updates mu.
+// The initial residual r0 = b - M x0 is formed by the assembly kernel while it writes M
+// (assemble.hip), and the correction x += V y by the mu' kernel before it differentiates mu
+// (deriv.hip), so a solve costs the Arnoldi steps only.  Once the residual estimate reaches
+// tol * |b| the remaining launches of the fixed-length sequence return at once.
 //
 // Initial guess of an RK stage: mu of the same stage extrapolated linearly over the two previous
 // steps, 2 mu_s(n-1) - mu_s(n-2) (its error is second order in dt, against first order for the
 // previous stage's mu), which saves Krylov steps on every solve of a time integration.
-#include "sfh_kernels.hpp"
+#include "sfh_gmres.hpp"
 
 namespace sfh {
-
-constexpr int kRowsPerWG = 4;      // one matrix row per wave
-constexpr int kMaxKrylov = 32;     // basis vectors per solve (iteration cap)
-constexpr int kGmMaxN = 4096;      // LDS copy of the vector being multiplied
-
-struct GmCtl {
-    int done;
-    int counter;       // top-level arrival counter
-    int iters;
-    int unconverged;
-    double bnorm, beta, res;
-    double hnext;
-    int group[16];     // group arrival counters
-    int pad[4];
-};
-static_assert(sizeof(GmCtl) == kGmCtlInts * sizeof(int), "host reads control blocks with this stride");
-
-struct GmView {
-    const double* M;
-    const double* b;
-    double* x;
-    double* V;       // (m+1) x n, unnormalised: v_i = sc[i] * V[i]
-    double* w0;      // products of even steps
-    double* w1;      // products of odd steps
-    double* sc;      // m+1 scales
-    double* h1;      // unrotated column of the last step
-    double* cs;
-    double* sn;
-    double* g;
-    double* H;       // column j at H[j*(m+1)]
-    double* part;    // (m+2) x nwg partial sums, basis index major
-    GmCtl* ctl;
-    int n, nwg;
-};
-
-__device__ inline GmView gm_view(const GmArgs& a, const SurfDesc* sd) {
-    const int s = blockIdx.z;
-    const SurfDesc d = sd[s];
-    GmView v;
-    const int m = a.m;
-    v.n = d.n;
-    v.nwg = (d.n + kRowsPerWG - 1) / kRowsPerWG;
-    v.M = a.M + d.moff;
-    v.b = a.b + d.voff;
-    v.x = a.x + d.voff;
-    v.V = a.V + (long long)(m + 1) * d.voff;
-    v.w0 = a.w + d.voff;
-    v.w1 = a.w + a.total + d.voff;
-    v.sc = a.small + (long long)s * a.small_stride;
-    v.h1 = v.sc + (m + 1);
-    v.cs = v.h1 + (m + 1);
-    v.sn = v.cs + m;
-    v.g = v.sn + m;
-    v.H = v.g + (m + 1);
-    v.part = a.part + (long long)s * a.part_stride;
-    v.ctl = reinterpret_cast<GmCtl*>(a.ctl) + s;
-    return v;
-}
 
 // Row R (global) times vs (LDS) by one wave, in two halves so the first 2048 columns' loads can
 // be issued before vs is ready: row_load issues 32 loads per lane (predicated at the row end),
@@ -119,84 +63,6 @@ __device__ inline double row_dot_rest(const double* __restrict__ R, const RowChu
         }
     }
     return wave_sum(a0 + a1);
-}
-
-// Returns true on the workgroup that finished last (the counters are then re-armed).
-// Arrival is two-level: workgroup w increments the counter of group w % 16 (release), the last
-// member of each group increments the top counter, so no single address takes more than
-// ceil(nwg / 16) atomics (one shared counter serialises all nwg of them).  The last group member
-// acquires before passing its group on, and only the very last workgroup acquires at the top.  On
-// gfx950 an agent-scope acquire invalidates the L2 of the executing XCD, so acquiring in every
-// workgroup would evict the data other workgroups are streaming.
-__device__ inline bool last_workgroup(GmCtl* ctl, int nwg) {
-    constexpr int kGroups = 16;
-    __shared__ int is_last;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        const int grp = blockIdx.x % kGroups;
-        const int ngroups = nwg < kGroups ? nwg : kGroups;
-        const int members = (nwg - grp + kGroups - 1) / kGroups;
-        is_last = 0;
-        const int prev = __hip_atomic_fetch_add(&ctl->group[grp], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        if (prev == members - 1) {
-            __hip_atomic_store(&ctl->group[grp], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-            const int top = __hip_atomic_fetch_add(&ctl->counter, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-            if (top == ngroups - 1) {
-                __hip_atomic_store(&ctl->counter, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-                is_last = 1;
-            }
-        }
-    }
-    __syncthreads();
-    return is_last != 0;
-}
-
-// Sums part[i * nwg + w] over w for i < count (count <= kMaxKrylov + 1) into out[i] (LDS).
-// Basis indices go in chunks of 12; within a chunk every load of a thread (12 indices x 2
-// workgroups per 512-workgroup pass) is issued before any addition.
-__device__ inline void reduce_partials(const double* __restrict__ part, int nwg, int count, double* out) {
-    constexpr int CH = 12;
-    __shared__ double red[4][kMaxKrylov + 2];
-    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
-    for (int i0 = 0; i0 < count; i0 += CH) {
-        double acc[CH];
-#pragma unroll
-        for (int u = 0; u < CH; ++u) acc[u] = 0.0;
-        for (int base = 0; base < nwg; base += 512) {
-            const int w0 = base + t, w1 = base + t + 256;
-            double v0[CH], v1[CH];
-#pragma unroll
-            for (int u = 0; u < CH; ++u) {
-                const bool ok = i0 + u < count;
-                v0[u] = (ok && w0 < nwg) ? part[(i0 + u) * nwg + w0] : 0.0;
-                v1[u] = (ok && w1 < nwg) ? part[(i0 + u) * nwg + w1] : 0.0;
-            }
-#pragma unroll
-            for (int u = 0; u < CH; ++u) acc[u] += v0[u] + v1[u];
-        }
-#pragma unroll
-        for (int u = 0; u < CH; ++u) {
-            if (i0 + u < count) {
-                const double s = wave_sum(acc[u]);
-                if (lane == 0) red[wv][i0 + u] = s;
-            }
-        }
-    }
-    __syncthreads();
-    if (t < count) out[t] = red[0][t] + red[1][t] + red[2][t] + red[3][t];
-    __syncthreads();
-}
-
-__device__ inline double block_sum4(double v, double* red4) {
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    v = wave_sum(v);
-    if (lane == 0) red4[wv] = v;
-    __syncthreads();
-    const double s = red4[0] + red4[1] + red4[2] + red4[3];
-    __syncthreads();
-    return s;
 }
 
 // u = w - sum_{i<j} co[i] V[i] over all n entries into vs (LDS); returns |u|^2 over the workgroup.
@@ -288,63 +154,6 @@ __device__ inline double givens_column(double* col, double* cs, double* sn, doub
     return fabs(g[jc + 1]);
 }
 
-// r0 = b - M x0 into V[0]; last workgroup sets beta = |r0|, |b|, g[0], sc[0], done.
-__global__ __launch_bounds__(256) void gm_init_kernel(GmArgs a, const SurfDesc* sd) {
-    const GmView v = gm_view(a, sd);
-    if ((int)blockIdx.x >= v.nwg) return;
-    __shared__ double vs[kGmMaxN];
-    __shared__ double rr[kRowsPerWG], bb[kRowsPerWG];
-    __shared__ double sums[kMaxKrylov + 2];
-    const int t = threadIdx.x, lane = t & 63, wv = t >> 6, n = v.n;
-    const int row = blockIdx.x * kRowsPerWG + wv;
-    const double* R = v.M + (long long)min(row, n - 1) * n;
-    RowChunk rc;
-    row_load(R, n, lane, rc);   // in flight while x0 is staged
-    const int hv = a.stage > 0 ? a.hvalid[a.stage - 1] : 0;
-    const int d_voff = (int)(v.x - a.x);
-    if (hv == 0) {
-        for (int c = t; c < n; c += 256) vs[c] = v.x[c];
-    } else {
-        const double* h1 = a.hist + (long long)(2 * a.stage - 2) * a.total + d_voff;
-        const double* h2 = h1 + a.total;
-        for (int c = t; c < n; c += 256) {
-            const double x0 = hv >= 2 ? 2.0 * h1[c] - h2[c] : h1[c];
-            vs[c] = x0;
-            if (c >= blockIdx.x * kRowsPerWG && c < (blockIdx.x + 1) * kRowsPerWG) v.x[c] = x0;
-        }
-    }
-    __syncthreads();
-    const double d = row < n ? row_dot_rest(R, rc, vs, n, lane) : 0.0;
-    if (lane == 0) {
-        double r = 0.0, bv = 0.0;
-        if (row < n) {
-            bv = v.b[row];
-            r = bv - d;
-            v.V[row] = r;
-        }
-        rr[wv] = r * r;
-        bb[wv] = bv * bv;
-    }
-    __syncthreads();
-    if (t == 0) {
-        v.part[blockIdx.x] = rr[0] + rr[1] + rr[2] + rr[3];
-        v.part[v.nwg + blockIdx.x] = bb[0] + bb[1] + bb[2] + bb[3];
-    }
-    if (!last_workgroup(v.ctl, v.nwg)) return;
-    reduce_partials(v.part, v.nwg, 2, sums);
-    if (t == 0) {
-        const double beta = sqrt(sums[0]), bnorm = sqrt(sums[1]);
-        GmCtl* c = v.ctl;
-        c->beta = beta;
-        c->bnorm = bnorm;
-        c->res = beta;
-        c->iters = 0;
-        c->done = (beta <= a.tol * bnorm) ? 1 : 0;
-        v.g[0] = beta;
-        v.sc[0] = beta > 0.0 ? 1.0 / beta : 0.0;
-    }
-}
-
 // Arnoldi step j (see the file comment).
 __global__ __launch_bounds__(256) void gm_iter_kernel(GmArgs a, const SurfDesc* sd, int j) {
     // converged solves: one load and out (most launches of the fixed sequence end here)
@@ -416,44 +225,6 @@ __global__ __launch_bounds__(256) void gm_iter_kernel(GmArgs a, const SurfDesc* 
             c->unconverged += 1;
         }
     }
-}
-
-// x += sum_i y_i v_i with R y = g (R = the rotated Hessenberg matrix), y solved by every
-// workgroup redundantly.
-__global__ __launch_bounds__(256) void gm_finish_kernel(GmArgs a, const SurfDesc* sd) {
-    const GmView v = gm_view(a, sd);
-    const int n = v.n, m = a.m, t = threadIdx.x;
-    const int k = v.ctl->iters;
-    __shared__ double y[kMaxKrylov], ys[kMaxKrylov];
-    if (k > 0 && t == 0) {
-        for (int i = k - 1; i >= 0; --i) {
-            double s = v.g[i];
-            for (int l = i + 1; l < k; ++l) s -= v.H[(long long)l * (m + 1) + i] * y[l];
-            y[i] = s / v.H[(long long)i * (m + 1) + i];
-        }
-        for (int i = 0; i < k; ++i) ys[i] = y[i] * v.sc[i];
-    }
-    __syncthreads();
-    double* h1 = nullptr;
-    if (a.stage > 0) h1 = a.hist + (long long)(2 * a.stage - 2) * a.total + (v.x - a.x);
-    for (int row = blockIdx.x * blockDim.x + t; row < n; row += gridDim.x * blockDim.x) {
-        double s = v.x[row];
-        if (k > 0) {
-            double vv[kMaxKrylov];
-#pragma unroll
-            for (int i = 0; i < kMaxKrylov; ++i) vv[i] = (i < k) ? v.V[(long long)i * n + row] : 0.0;
-#pragma unroll
-            for (int i = 0; i < kMaxKrylov; ++i)
-                if (i < k) s += ys[i] * vv[i];
-            v.x[row] = s;
-        }
-        if (h1) {   // shift this stage's history
-            h1[a.total + row] = h1[row];
-            h1[row] = s;
-        }
-    }
-    if (a.stage > 0 && blockIdx.x == 0 && blockIdx.z == 0 && t == 0)
-        a.hvalid[a.stage - 1] = min(a.hvalid[a.stage - 1] + 1, 2);
 }
 
 }  // namespace sfh

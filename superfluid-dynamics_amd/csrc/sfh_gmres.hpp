@@ -1,0 +1,148 @@
+// sfh_gmres.hpp -- GMRES state and the workgroup-arrival reduction, shared by the GMRES kernels
+// and the kernels that do GMRES work on the side (the assembly kernel forms the initial residual
+// while it writes the matrix; the mu' kernel adds the Krylov correction before differentiating).
+#pragma once
+#include <cstddef>
+
+#include "sfh_kernels.hpp"
+
+namespace sfh {
+
+constexpr int kRowsPerWG = 4;      // one matrix row per wave
+constexpr int kMaxKrylov = 32;     // basis vectors per solve (iteration cap)
+constexpr int kGmMaxN = 4096;      // LDS copy of the vector being multiplied
+
+struct GmCtl {
+    int done;
+    int counter;       // top-level arrival counter
+    int iters;
+    int unconverged;
+    double bnorm, beta, res;
+    double hnext;
+    int group[16];     // group arrival counters
+    int pad[4];
+};
+static_assert(sizeof(GmCtl) == kGmCtlInts * sizeof(int), "host reads control blocks with this stride");
+
+struct GmView {
+    const double* M;
+    const double* b;
+    double* x;
+    double* V;       // (m+1) x n, unnormalised: v_i = sc[i] * V[i]
+    double* w0;      // products of even steps
+    double* w1;      // products of odd steps
+    double* sc;      // m+1 scales
+    double* h1;      // unrotated column of the last step
+    double* cs;
+    double* sn;
+    double* g;
+    double* H;       // column j at H[j*(m+1)]
+    double* part;    // (m+2) x nwg partial sums, basis index major
+    GmCtl* ctl;
+    int n, nwg;
+};
+
+__device__ inline GmView gm_view(const GmArgs& a, const SurfDesc* sd, int s) {
+    const SurfDesc d = sd[s];
+    GmView v;
+    const int m = a.m;
+    v.n = d.n;
+    v.nwg = (d.n + kRowsPerWG - 1) / kRowsPerWG;
+    v.M = a.M + d.moff;
+    v.b = a.b + d.voff;
+    v.x = a.x + d.voff;
+    v.V = a.V + (long long)(m + 1) * d.voff;
+    v.w0 = a.w + d.voff;
+    v.w1 = a.w + a.total + d.voff;
+    v.sc = a.small + (long long)s * a.small_stride;
+    v.h1 = v.sc + (m + 1);
+    v.cs = v.h1 + (m + 1);
+    v.sn = v.cs + m;
+    v.g = v.sn + m;
+    v.H = v.g + (m + 1);
+    v.part = a.part + (long long)s * a.part_stride;
+    v.ctl = reinterpret_cast<GmCtl*>(a.ctl) + s;
+    return v;
+}
+
+__device__ inline GmView gm_view(const GmArgs& a, const SurfDesc* sd) { return gm_view(a, sd, blockIdx.z); }
+
+// Returns true on the workgroup that finished last (the counters are then re-armed).
+// Arrival is two-level: workgroup w increments the counter of group w % 16 (release), the last
+// member of each group increments the top counter, so no single address takes more than
+// ceil(nwg / 16) atomics (one shared counter serialises all nwg of them).  The last group member
+// acquires before passing its group on, and only the very last workgroup acquires at the top.  On
+// gfx950 an agent-scope acquire invalidates the L2 of the executing XCD, so acquiring in every
+// workgroup would evict the data other workgroups are streaming.
+__device__ inline bool last_workgroup(GmCtl* ctl, int nwg) {
+    constexpr int kGroups = 16;
+    __shared__ int is_last;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int grp = blockIdx.x % kGroups;
+        const int ngroups = nwg < kGroups ? nwg : kGroups;
+        const int members = (nwg - grp + kGroups - 1) / kGroups;
+        is_last = 0;
+        const int prev = __hip_atomic_fetch_add(&ctl->group[grp], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        if (prev == members - 1) {
+            __hip_atomic_store(&ctl->group[grp], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            const int top = __hip_atomic_fetch_add(&ctl->counter, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            if (top == ngroups - 1) {
+                __hip_atomic_store(&ctl->counter, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+                is_last = 1;
+            }
+        }
+    }
+    __syncthreads();
+    return is_last != 0;
+}
+
+// Sums part[i * nwg + w] over w for i < count (count <= kMaxKrylov + 1) into out[i] (LDS).
+// Basis indices go in chunks of 12; within a chunk every load of a thread (12 indices x 2
+// workgroups per 512-workgroup pass) is issued before any addition.
+__device__ inline void reduce_partials(const double* __restrict__ part, int nwg, int count, double* out) {
+    constexpr int CH = 12;
+    __shared__ double red[4][kMaxKrylov + 2];
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    for (int i0 = 0; i0 < count; i0 += CH) {
+        double acc[CH];
+#pragma unroll
+        for (int u = 0; u < CH; ++u) acc[u] = 0.0;
+        for (int base = 0; base < nwg; base += 512) {
+            const int w0 = base + t, w1 = base + t + 256;
+            double v0[CH], v1[CH];
+#pragma unroll
+            for (int u = 0; u < CH; ++u) {
+                const bool ok = i0 + u < count;
+                v0[u] = (ok && w0 < nwg) ? part[(i0 + u) * nwg + w0] : 0.0;
+                v1[u] = (ok && w1 < nwg) ? part[(i0 + u) * nwg + w1] : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < CH; ++u) acc[u] += v0[u] + v1[u];
+        }
+#pragma unroll
+        for (int u = 0; u < CH; ++u) {
+            if (i0 + u < count) {
+                const double s = wave_sum(acc[u]);
+                if (lane == 0) red[wv][i0 + u] = s;
+            }
+        }
+    }
+    __syncthreads();
+    if (t < count) out[t] = red[0][t] + red[1][t] + red[2][t] + red[3][t];
+    __syncthreads();
+}
+
+__device__ inline double block_sum4(double v, double* red4) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    v = wave_sum(v);
+    if (lane == 0) red4[wv] = v;
+    __syncthreads();
+    const double s = red4[0] + red4[1] + red4[2] + red4[3];
+    __syncthreads();
+    return s;
+}
+
+}  // namespace sfh

@@ -23,20 +23,43 @@ struct StageCtl {
     double2* qup;      // optional upper-fluid velocity output (may be null)
 };
 
+struct GmArgs {
+    const double* M;     // matrices (FP64, row-major per surface at SurfDesc::moff)
+    const double* b;     // right-hand sides
+    double* x;           // initial guess in, solution out
+    double* V;           // Krylov basis, (m+1) * voff per surface
+    double* w;           // two product vectors of total nodes each (even / odd steps)
+    double* part;        // per-workgroup partial sums, part_stride per surface
+    double* small;       // per surface: scales, coefficients, rotations, g, H
+    int* ctl;            // per surface control block
+    long long part_stride;
+    long long small_stride;
+    int m;               // maximal number of iterations (Krylov dimension)
+    int total;           // nodes of all surfaces (stride between the two product vectors)
+    double tol;
+    // warm start by extrapolation over steps: stage s (1..4) keeps its solutions of the last two
+    // steps at hist + (2 s - 2) total (newest) and hist + (2 s - 1) total; hvalid[s - 1] counts
+    // them (0..2).  stage 0: plain warm start from x.
+    double* hist;
+    int* hvalid;
+    int stage;
+    int pad;
+};
 // ---- deriv.hip
 constexpr int kDerivThreads = 1024;
 __global__ void zphi_deriv_kernel(const SurfDesc* sd, const double2* tw_all, const double2* Z,
                                   const double* phi, double2* Zp, double2* Zpp, double* b,
-                                  double* work, Phys ph, int* status, int lds_tw, NodeTrig* trig);
-__global__ void mu_deriv_kernel(const SurfDesc* sd, const double2* tw_all, const double* mu,
-                                double2* mup, int lds_tw);
+                                  double* work, Phys ph, int* status, int lds_tw, NodeTrig* trig, GmArgs gm,
+                                  int gm_on);
+__global__ void mu_deriv_kernel(const SurfDesc* sd, const double2* tw_all, double* mu, double2* mup, int lds_tw,
+                                GmArgs gm, int gm_on);
 __global__ void plain_deriv_kernel(const SurfDesc* sd, const double2* tw_all, const double2* in,
                                    double2* out, int second, double scale, int lds_tw);
 
 // ---- assemble.hip
 __global__ void assemble64_kernel(const SurfDesc* sd, const NodeTrig* trig, const double2* Z,
                                   const double2* Zp, const double2* Zpp, double* M, Phys ph,
-                                  const int* only_if);
+                                  const int* only_if, GmArgs gm, int gm_on);
 
 // ---- velocity.hip
 __global__ void velocity_stage_kernel(const SurfDesc* sd, const NodeTrig* trig, const double2* Z,
@@ -61,32 +84,8 @@ __global__ void gepp_fallback_kernel(const SurfDesc* sd, double* M, const double
                                      int* piv, int* status, int* counters);
 
 // ---- gmres.hip
-struct GmArgs {
-    const double* M;     // matrices (FP64, row-major per surface at SurfDesc::moff)
-    const double* b;     // right-hand sides
-    double* x;           // initial guess in, solution out
-    double* V;           // Krylov basis, (m+1) * voff per surface
-    double* w;           // two product vectors of total nodes each (even / odd steps)
-    double* part;        // per-workgroup partial sums, part_stride per surface
-    double* small;       // per surface: scales, coefficients, rotations, g, H
-    int* ctl;            // per surface control block
-    long long part_stride;
-    long long small_stride;
-    int m;               // maximal number of iterations (Krylov dimension)
-    int total;           // nodes of all surfaces (stride between the two product vectors)
-    double tol;
-    // warm start by extrapolation over steps: stage s (1..4) keeps its solutions of the last two
-    // steps at hist + (2 s - 2) total (newest) and hist + (2 s - 1) total; hvalid[s - 1] counts
-    // them (0..2).  stage 0: plain warm start from x.
-    double* hist;
-    int* hvalid;
-    int stage;
-    int pad;
-};
 constexpr int kGmCtlInts = 32;   // ints per control block (holds the device struct)
-__global__ void gm_init_kernel(GmArgs a, const SurfDesc* sd);
 __global__ void gm_iter_kernel(GmArgs a, const SurfDesc* sd, int j);
-__global__ void gm_finish_kernel(GmArgs a, const SurfDesc* sd);
 
 // ---- batched.hip (persistent per-surface stage kernel, ensembles with every N <= kBatchMaxN)
 constexpr int kBatchThreads = 1024;

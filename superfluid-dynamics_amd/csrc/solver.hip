@@ -228,9 +228,7 @@ Solver::KernelReport Solver::kernel_report(const std::string& tag) {
     return r;
 }
 
-// GMRES on the stored FP64 matrices of all surfaces at once (gridDim.z = surface), warm started
-// from the previous mu.
-void Solver::enqueue_gmres(int stage) {
+GmArgs Solver::gm_args(int stage) const {
     GmArgs a{};
     a.M = M_;
     a.b = b_;
@@ -248,13 +246,18 @@ void Solver::enqueue_gmres(int stage) {
     a.hist = gmhist_;
     a.hvalid = gmhvalid_;
     a.stage = stage;
+    return a;
+}
+
+// Arnoldi steps of the GMRES solve of all surfaces at once (gridDim.z = surface).  The initial
+// residual comes from the assembly kernel and the correction x += V y is applied by the mu'
+// kernel, so only the fixed-length sequence of steps is launched here.
+void Solver::enqueue_gmres(int stage) {
+    const GmArgs a = gm_args(stage);
     double nsq = 0.0;
     for (int v : n_) nsq += (double)v * v;
     const dim3 grid(gm_nwg_, 1, S_);
     const size_t first = timed_.size();
-    launch("gm_init", 2.0 * nsq, 8.0 * nsq, [&] {
-        hipLaunchKernelGGL(gm_init_kernel, grid, dim3(256), 0, stream_, a, (const SurfDesc*)d_desc_);
-    });
     for (int j = 0; j < gm_m_; ++j) {
         // product with the stored matrix + dots + the next basis vector (j + 1 vectors)
         launch(
@@ -262,10 +265,6 @@ void Solver::enqueue_gmres(int stage) {
             [&] { hipLaunchKernelGGL(gm_iter_kernel, grid, dim3(256), 0, stream_, a, (const SurfDesc*)d_desc_, j); },
             j);
     }
-    launch("gm_finish", 2.0 * gm_m_ * T_, 8.0 * gm_m_ * T_, [&] {
-        hipLaunchKernelGGL(gm_finish_kernel, dim3((maxN_ + 255) / 256, 1, S_), dim3(256), 0, stream_, a,
-                           (const SurfDesc*)d_desc_);
-    });
     if (ktime_) {   // keep only iterations that did work (the rest returned after convergence)
         std::vector<int> ctl((size_t)kGmCtlInts * S_);
         SFH_HIP(hipStreamSynchronize(stream_));
@@ -321,7 +320,7 @@ void Solver::enqueue_solve(const double2* Z, int stage) {
     }
     // systems whose interchange-free check failed: re-assemble and solve with GEPP
     hipLaunchKernelGGL(assemble64_kernel, dim3((maxN_ + 3) / 4, S_), dim3(256), 0, stream_, d_desc_,
-                       (const NodeTrig*)trig_, Z, Zp_, Zpp_, M_, ph_, (const int*)status_);
+                       (const NodeTrig*)trig_, Z, Zp_, Zpp_, M_, ph_, (const int*)status_, GmArgs{}, 0);
     hipLaunchKernelGGL(gepp_fallback_kernel, dim3(S_), dim3(256), 0, stream_, d_desc_, M_, b_, mu_, piv_,
                        status_, counters_);
 }
@@ -366,6 +365,8 @@ void Solver::enqueue_rhs(const double2* Z, const double* P, const StageCtl& ctl)
         enqueue_batched(Z, P, ctl);
         return;
     }
+    const int gm_on = opt_.solver == kSolveGMRES ? 1 : 0;
+    const GmArgs ga = gm_on ? gm_args(ctl.stage) : GmArgs{};
     // data + twiddles in LDS when both fit (n <= 4096), data only otherwise
     const int lds_tw = (2 * (size_t)maxN_ * sizeof(double2) <= 144 * 1024) ? 1 : 0;
     const size_t lds = (size_t)maxN_ * sizeof(double2) * (lds_tw ? 2 : 1);
@@ -377,17 +378,17 @@ void Solver::enqueue_rhs(const double2* Z, const double* P, const StageCtl& ctl)
     const double img = ph_.image ? 2.0 : 1.0;
     launch("deriv", 3 * 2 * nlog, 64.0 * T_, [&] {
         hipLaunchKernelGGL(zphi_deriv_kernel, dim3(S_, 3), dim3(kDerivThreads), lds, stream_, d_desc_, d_tw_, Z,
-                           P, Zp_, Zpp_, b_, work_, ph_, status_, lds_tw, trig_);
+                           P, Zp_, Zpp_, b_, work_, ph_, status_, lds_tw, trig_, ga, gm_on);
     });
     // assembly: flops counted as cot evaluations (the transcendental-bound unit), bytes = M written
     launch("assemble", img * nsq, 8.0 * nsq, [&] {
         hipLaunchKernelGGL(assemble64_kernel, dim3((maxN_ + 3) / 4, S_), dim3(256), 0, stream_, d_desc_,
-                           (const NodeTrig*)trig_, Z, Zp_, Zpp_, M_, ph_, (const int*)nullptr);
+                           (const NodeTrig*)trig_, Z, Zp_, Zpp_, M_, ph_, (const int*)nullptr, ga, gm_on);
     });
     enqueue_solve(Z, ctl.stage);
     launch("mu_deriv", 2 * nlog, 24.0 * T_, [&] {
         hipLaunchKernelGGL(mu_deriv_kernel, dim3(S_), dim3(kDerivThreads), lds, stream_, d_desc_, d_tw_, mu_, mup_,
-                           lds_tw);
+                           lds_tw, ga, gm_on);
     });
     launch("velocity", img * nsq, 96.0 * T_, [&] {
         hipLaunchKernelGGL(velocity_stage_kernel, dim3((maxN_ + 3) / 4, S_), dim3(256), 0, stream_, d_desc_,

@@ -154,6 +154,7 @@ class Solver {
     int* gmctl_ = nullptr;
     long long gm_part_stride_ = 0, gm_small_stride_ = 0;
     void enqueue_gmres(int stage);
+    GmArgs gm_args(int stage) const;
     double* gmhist_ = nullptr;   // 8 x total: per RK stage, mu of the last two steps
     int* gmhvalid_ = nullptr;
     // persistent per-surface stage kernel (kSolveBatched)
