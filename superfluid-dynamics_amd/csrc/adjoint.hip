@@ -34,15 +34,13 @@ __device__ inline double mv_free_t(int n, int G, const PairConst& pc, BLds& L) {
     if (t < G * n) {
         const int j = t % n, g = t / n;
         const NodeTrig tj = L.tr[j];
-        double ux = 0.0, uy = 0.0;
+        double uy = 0.0;   // Im sum v_k Z'_k c_jk
         for (int k = g; k < n; k += G) {
             if (k == j) continue;
             const double vk = L.vs[k];
             const double2 zk = L.zp[k];
             const double2 ct = cot_pair(tj, L.tr[k]);
-            const double wx = vk * zk.x, wy = vk * zk.y;
-            ux += wx * ct.x - wy * ct.y;
-            uy += wx * ct.y + wy * ct.x;
+            uy += vk * (zk.x * ct.y + zk.y * ct.x);
         }
         val = -pc.c * uy;
     }

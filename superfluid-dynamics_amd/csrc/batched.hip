@@ -90,15 +90,15 @@ __global__ __launch_bounds__(1024) void batched_stage_kernel(BatchArgs a, Phys p
     __syncthreads();   // the workgroup reads back only its own columns: a barrier suffices
 
     // ---------------------------------------------------------------- C: solve
-    // warm start: this stage's mu extrapolated over the last two steps, 2 mu(n-1) - mu(n-2) (error
-    // second order in dt), else the previous solve's mu
+    // warm start: this stage's mu extrapolated over the previous steps (error O(dt^3) once three
+    // are kept), else the previous solve's mu
     double x = own ? a.mu[off + t] : 0.0;
-    double* h1 = nullptr;
+    double* h0 = nullptr;
     int hv = 0;
     if (ctl.stage > 0 && a.hist) {
         hv = a.hvalid[(ctl.stage - 1) * a.nsurf + s];
-        h1 = a.hist + (long long)(2 * ctl.stage - 2) * a.total + off;
-        if (own && hv >= 1) x = hv >= 2 ? 2.0 * h1[t] - h1[a.total + t] : h1[t];
+        h0 = a.hist + (long long)(ctl.stage - 1) * kHistDepth * a.total + off;
+        if (own && hv >= 1) x = extrapolate_history(h0 + t, a.total, hv);
     }
     const double bn = sqrt(bsum1(own ? bk * bk : 0.0, L));
     const double tol_abs = a.tol * bn;
@@ -131,12 +131,9 @@ __global__ __launch_bounds__(1024) void batched_stage_kernel(BatchArgs a, Phys p
         st[0] = iters;
         st[1] += unconv;
         st[2] = sweeps;
-        if (h1) a.hvalid[(ctl.stage - 1) * a.nsurf + s] = min(hv + 1, 2);
+        if (h0) a.hvalid[(ctl.stage - 1) * a.nsurf + s] = min(hv + 1, kHistDepth);
     }
-    if (own && h1) {
-        h1[a.total + t] = h1[t];
-        h1[t] = x;
-    }
+    if (own && h0) push_history(h0 + t, a.total, x);
 
     // ---------------------------------------------------------------- D: mu'
     if (own) {

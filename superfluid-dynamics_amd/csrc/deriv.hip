@@ -69,14 +69,13 @@ __global__ __launch_bounds__(1024) void zphi_deriv_kernel(const SurfDesc* __rest
             work[off + j] = r;
         }
     }
-    // GMRES initial guess of this RK stage: its mu over the two previous steps extrapolated,
-    // 2 mu(n-1) - mu(n-2) (error second order in dt), else the last solve's mu (kept in place)
+    // GMRES initial guess of this RK stage: its mu over the previous steps extrapolated (error
+    // O(dt^3) once three are kept), else the last solve's mu (kept in place)
     if (gm_on && field == 2 && gm.stage > 0) {
         const int hv = gm.hvalid[gm.stage - 1];
         if (hv > 0) {
-            const double* h1 = gm.hist + (long long)(2 * gm.stage - 2) * gm.total + off;
-            for (int j = threadIdx.x; j < n; j += blockDim.x)
-                gm.x[off + j] = hv >= 2 ? 2.0 * h1[j] - h1[gm.total + j] : h1[j];
+            const double* h0 = gm.hist + (long long)(gm.stage - 1) * kHistDepth * gm.total + off;
+            for (int j = threadIdx.x; j < n; j += blockDim.x) gm.x[off + j] = extrapolate_history(h0 + j, gm.total, hv);
         }
     }
 }
@@ -110,7 +109,7 @@ __global__ __launch_bounds__(1024) void mu_deriv_kernel(const SurfDesc* __restri
         }
         __syncthreads();
         const int k = kk;
-        double* h1 = gm.stage > 0 ? gm.hist + (long long)(2 * gm.stage - 2) * gm.total + off : nullptr;
+        double* h0 = gm.stage > 0 ? gm.hist + (long long)(gm.stage - 1) * kHistDepth * gm.total + off : nullptr;
         for (int j = threadIdx.x; j < n; j += blockDim.x) {
             double x = mu[off + j];
             for (int i0 = 0; i0 < k; i0 += 8) {   // 8 basis loads in flight per pass
@@ -122,14 +121,11 @@ __global__ __launch_bounds__(1024) void mu_deriv_kernel(const SurfDesc* __restri
                     if (i0 + u < k) x += ys[i0 + u] * vv[u];
             }
             mu[off + j] = x;
-            if (h1) {
-                h1[gm.total + j] = h1[j];
-                h1[j] = x;
-            }
+            if (h0) push_history(h0 + j, gm.total, x);
             s[lds_index(d, j)] = make_double2(x, 0.0);
         }
-        if (h1 && blockIdx.x == 0 && threadIdx.x == 0)
-            gm.hvalid[gm.stage - 1] = min(gm.hvalid[gm.stage - 1] + 1, 2);
+        if (h0 && blockIdx.x == 0 && threadIdx.x == 0)
+            gm.hvalid[gm.stage - 1] = min(gm.hvalid[gm.stage - 1] + 1, kHistDepth);
     } else {
         for (int j = threadIdx.x; j < n; j += blockDim.x) s[lds_index(d, j)] = make_double2(mu[off + j], 0.0);
     }

@@ -19,9 +19,9 @@
 // (deriv.hip), so a solve costs the Arnoldi steps only.  Once the residual estimate reaches
 // tol * |b| the remaining launches of the fixed-length sequence return at once.
 //
-// Initial guess of an RK stage: mu of the same stage extrapolated linearly over the two previous
-// steps, 2 mu_s(n-1) - mu_s(n-2) (its error is second order in dt, against first order for the
-// previous stage's mu), which saves Krylov steps on every solve of a time integration.
+// Initial guess of an RK stage: mu of the same stage extrapolated over the three previous steps,
+// 3 (mu_s(n-1) - mu_s(n-2)) + mu_s(n-3) (its error is third order in dt, against first order for
+// the previous stage's mu), which saves Krylov steps on every solve of a time integration.
 #include "sfh_gmres.hpp"
 
 namespace sfh {

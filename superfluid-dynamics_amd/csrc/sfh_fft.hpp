@@ -28,50 +28,100 @@ __device__ inline double2 twid(const double2* __restrict__ tw, int t, int sign) 
 
 constexpr int kFftThreads = 1024;
 
-// Multiplication by sign * i (the quarter-turn twiddle W_{4h}^h of a fused radix-2^2 pass).
-__device__ inline double2 rot_quarter(double2 a, int sign) {
-    return sign > 0 ? make_double2(-a.y, a.x) : make_double2(a.y, -a.x);
+// Largest radix (log2) of one LDS pass: a thread keeps 2^R complex values in registers through R
+// radix-2 stages, so a transform of 2^m points makes ceil(m / R) LDS round trips and barriers.
+constexpr int kFftMaxLogRadix = 3;
+
+// Group t of a pass over half widths h, 2h, ..., 2^{R-1} h (h = 2^lh) in each of the sub
+// sub-transforms of length 2^lm: elements i0 + e h, e < 2^R; pos = i0 mod h.
+__device__ inline int pass_group(int t, int lm, int R, int lh, int& pos) {
+    const int q = t >> (lm - R), loc = t & ((1 << (lm - R)) - 1);
+    const int grp = loc >> lh;
+    pos = loc & ((1 << lh) - 1);
+    return (q << lm) + (grp << (lh + R)) + pos;
+}
+
+// Decimation-in-time pass: radix-2 stages of half widths h, ..., 2^{R-1} h applied to each
+// group in registers (stage st pairs e and e + 2^st, twiddle W_n^{(pos + (e mod 2^st) h) n / 2^{lh+st+1}}).
+template <int R>
+__device__ inline void dit_pass(double2* s, int n, int lm, int sub, int lh, int sign,
+                                const double2* __restrict__ tw) {
+    constexpr int G = 1 << R;
+    const int h = 1 << lh, ng = sub << (lm - R);
+    for (int t = threadIdx.x; t < ng; t += blockDim.x) {
+        int pos;
+        const int i0 = pass_group(t, lm, R, lh, pos);
+        double2 x[G];
+#pragma unroll
+        for (int e = 0; e < G; ++e) x[e] = s[i0 + e * h];
+#pragma unroll
+        for (int st = 0; st < R; ++st) {
+            const int half = 1 << st, stride = n >> (lh + st + 1);
+#pragma unroll
+            for (int j = 0; j < half; ++j) {
+                const double2 w = twid(tw, (pos + j * h) * stride, sign);
+#pragma unroll
+                for (int b = 0; b < G; b += 2 * half) {
+                    const double2 u = x[b + j], v = cmul(x[b + j + half], w);
+                    x[b + j] = cadd(u, v);
+                    x[b + j + half] = csub(u, v);
+                }
+            }
+        }
+#pragma unroll
+        for (int e = 0; e < G; ++e) s[i0 + e * h] = x[e];
+    }
+    __syncthreads();
+}
+
+// Decimation-in-frequency pass: the mirror image, stages of half widths 2^{R-1} h, ..., h with
+// the twiddle applied to the difference.
+template <int R>
+__device__ inline void dif_pass(double2* s, int n, int lm, int sub, int lh, int sign,
+                                const double2* __restrict__ tw) {
+    constexpr int G = 1 << R;
+    const int h = 1 << lh, ng = sub << (lm - R);
+    for (int t = threadIdx.x; t < ng; t += blockDim.x) {
+        int pos;
+        const int i0 = pass_group(t, lm, R, lh, pos);
+        double2 x[G];
+#pragma unroll
+        for (int e = 0; e < G; ++e) x[e] = s[i0 + e * h];
+#pragma unroll
+        for (int st = R - 1; st >= 0; --st) {
+            const int half = 1 << st, stride = n >> (lh + st + 1);
+#pragma unroll
+            for (int j = 0; j < half; ++j) {
+                const double2 w = twid(tw, (pos + j * h) * stride, sign);
+#pragma unroll
+                for (int b = 0; b < G; b += 2 * half) {
+                    const double2 u = x[b + j], v = x[b + j + half];
+                    x[b + j] = cadd(u, v);
+                    x[b + j + half] = cmul(csub(u, v), w);
+                }
+            }
+        }
+#pragma unroll
+        for (int e = 0; e < G; ++e) s[i0 + e * h] = x[e];
+    }
+    __syncthreads();
 }
 
 // Decimation in time: input in lds_index order, output in natural order.
 // tw[t] = exp(-2 pi i t / n), t < n (LDS or global); sign -1 forward, +1 inverse (unnormalised).
-// Consecutive radix-2 stages (half widths h and 2h) are fused into one radix-2^2 pass over quads
-// (i0, i0+h, i0+2h, i0+3h): one LDS round trip and one barrier per two stages, and all index
-// arithmetic is shifts and masks.
+// The power-of-two sub-transforms run as radix-2^R passes from half width 1 upwards, then the
+// radix-3 combination when n = 3 * 2^m.
 __device__ inline void fft_dit(double2* s, const SurfDesc& d, int sign, const double2* __restrict__ tw) {
     const int n = d.n, lm = d.log2m, m = 1 << lm, sub = d.r3 ? 3 : 1;
-    int lh = 0;   // log2 of the half width of the next stage
-    for (; lh + 2 <= lm; lh += 2) {
-        const int h = 1 << lh;
-        const int nq = sub << (lm - 2);                  // quads in the pass
-        const int s1 = n >> (lh + 1), s2 = n >> (lh + 2); // twiddle strides for lengths 2h, 4h
-        for (int t = threadIdx.x; t < nq; t += blockDim.x) {
-            const int q = t >> (lm - 2), loc = t & ((m >> 2) - 1);
-            const int grp = loc >> lh, pos = loc & (h - 1);
-            const int i0 = (q << lm) + (grp << (lh + 2)) + pos;
-            const double2 a = s[i0], b = s[i0 + h], c = s[i0 + 2 * h], e = s[i0 + 3 * h];
-            const double2 w1 = twid(tw, pos * s1, sign), w2 = twid(tw, pos * s2, sign);
-            const double2 bw = cmul(b, w1), ew = cmul(e, w1);
-            const double2 a1 = cadd(a, bw), b1 = csub(a, bw), c1 = cadd(c, ew), e1 = csub(c, ew);
-            const double2 cw = cmul(c1, w2), dw = rot_quarter(cmul(e1, w2), sign);
-            s[i0] = cadd(a1, cw);
-            s[i0 + 2 * h] = csub(a1, cw);
-            s[i0 + h] = cadd(b1, dw);
-            s[i0 + 3 * h] = csub(b1, dw);
+    for (int lh = 0; lh < lm;) {
+        const int R = min(kFftMaxLogRadix, lm - lh);
+        switch (R) {
+            case 4: dit_pass<4>(s, n, lm, sub, lh, sign, tw); break;
+            case 3: dit_pass<3>(s, n, lm, sub, lh, sign, tw); break;
+            case 2: dit_pass<2>(s, n, lm, sub, lh, sign, tw); break;
+            default: dit_pass<1>(s, n, lm, sub, lh, sign, tw); break;
         }
-        __syncthreads();
-    }
-    if (lh < lm) {   // one radix-2 stage left (odd log2 m), half width h = m / 2
-        const int h = 1 << lh, nb = sub << (lm - 1), st = n >> (lh + 1);
-        for (int t = threadIdx.x; t < nb; t += blockDim.x) {
-            const int q = t >> (lm - 1), pos = t & (h - 1);
-            const int i = (q << lm) + pos;
-            const double2 u = s[i];
-            const double2 v = cmul(s[i + h], twid(tw, pos * st, sign));
-            s[i] = cadd(u, v);
-            s[i + h] = csub(u, v);
-        }
-        __syncthreads();
+        lh += R;
     }
     if (d.r3) {   // X[k + q m] = sum_r W_n^{r k} W_3^{r q} F_r[k]
         const double c = -0.5, s3 = 0.86602540378443864676372317075293618 * sign;
@@ -91,7 +141,7 @@ __device__ inline void fft_dit(double2* s, const SurfDesc& d, int sign, const do
 }
 
 // Decimation in frequency: input in natural order, output in lds_index order.  Mirror image of
-// fft_dit: stages with half widths m/2, m/4, ..., 1, fused in pairs (2h, h).
+// fft_dit: the radix-3 split first, then radix-2^R passes from the widest half width down to 1.
 __device__ inline void fft_dif(double2* s, const SurfDesc& d, int sign, const double2* __restrict__ tw) {
     const int n = d.n, lm = d.log2m, m = 1 << lm, sub = d.r3 ? 3 : 1;
     if (d.r3) {   // y_q[j] = W_n^{j q} sum_r x[j + r m] W_3^{r q}
@@ -107,35 +157,15 @@ __device__ inline void fft_dif(double2* s, const SurfDesc& d, int sign, const do
         }
         __syncthreads();
     }
-    int lh = lm - 2;   // fused pair (2h, h) with h = 2^lh
-    for (; lh >= 0; lh -= 2) {
-        const int h = 1 << lh;
-        const int nq = sub << (lm - 2);
-        const int s1 = n >> (lh + 1), s2 = n >> (lh + 2);
-        for (int t = threadIdx.x; t < nq; t += blockDim.x) {
-            const int q = t >> (lm - 2), loc = t & ((m >> 2) - 1);
-            const int grp = loc >> lh, pos = loc & (h - 1);
-            const int i0 = (q << lm) + (grp << (lh + 2)) + pos;
-            const double2 a = s[i0], b = s[i0 + h], c = s[i0 + 2 * h], e = s[i0 + 3 * h];
-            const double2 w1 = twid(tw, pos * s1, sign), w2 = twid(tw, pos * s2, sign);
-            const double2 a1 = cadd(a, c), c1 = cmul(csub(a, c), w2);
-            const double2 b1 = cadd(b, e), e1 = rot_quarter(cmul(csub(b, e), w2), sign);
-            s[i0] = cadd(a1, b1);
-            s[i0 + h] = cmul(csub(a1, b1), w1);
-            s[i0 + 2 * h] = cadd(c1, e1);
-            s[i0 + 3 * h] = cmul(csub(c1, e1), w1);
+    for (int top = lm; top > 0;) {
+        const int R = min(kFftMaxLogRadix, top), lh = top - R;
+        switch (R) {
+            case 4: dif_pass<4>(s, n, lm, sub, lh, sign, tw); break;
+            case 3: dif_pass<3>(s, n, lm, sub, lh, sign, tw); break;
+            case 2: dif_pass<2>(s, n, lm, sub, lh, sign, tw); break;
+            default: dif_pass<1>(s, n, lm, sub, lh, sign, tw); break;
         }
-        __syncthreads();
-    }
-    if (lh == -1) {   // one radix-2 stage left (odd log2 m), half width 1
-        const int nb = sub << (lm - 1), st = n >> 1;
-        for (int t = threadIdx.x; t < nb; t += blockDim.x) {
-            const int i = t << 1;
-            const double2 u = s[i], v = s[i + 1];
-            s[i] = cadd(u, v);
-            s[i + 1] = cmul(csub(u, v), twid(tw, 0 * st, sign));
-        }
-        __syncthreads();
+        top = lh;
     }
 }
 

@@ -23,6 +23,22 @@ struct StageCtl {
     double2* qup;      // optional upper-fluid velocity output (may be null)
 };
 
+// Warm start of a solve by extrapolation over time steps: each RK stage keeps its solutions of the
+// last kHistDepth steps (newest first, levels stride apart) and the next initial guess is the
+// polynomial through the hv of them that exist, evaluated one step ahead:
+//   hv = 1: x0 = u0;  hv = 2: 2 u0 - u1;  hv = 3: 3 (u0 - u1) + u2   (error O(dt^hv)).
+constexpr int kHistDepth = 3;
+__device__ inline double extrapolate_history(const double* h, long long stride, int hv) {
+    if (hv >= 3) return 3.0 * (h[0] - h[stride]) + h[2 * stride];
+    if (hv == 2) return 2.0 * h[0] - h[stride];
+    return h[0];
+}
+__device__ inline void push_history(double* h, long long stride, double x) {
+    h[2 * stride] = h[stride];
+    h[stride] = h[0];
+    h[0] = x;
+}
+
 struct GmArgs {
     const double* M;     // matrices (FP64, row-major per surface at SurfDesc::moff)
     const double* b;     // right-hand sides
@@ -37,9 +53,9 @@ struct GmArgs {
     int m;               // maximal number of iterations (Krylov dimension)
     int total;           // nodes of all surfaces (stride between the two product vectors)
     double tol;
-    // warm start by extrapolation over steps: stage s (1..4) keeps its solutions of the last two
-    // steps at hist + (2 s - 2) total (newest) and hist + (2 s - 1) total; hvalid[s - 1] counts
-    // them (0..2).  stage 0: plain warm start from x.
+    // warm start by extrapolation over steps: stage s (1..4) keeps its solutions of the last
+    // kHistDepth steps at hist + ((s - 1) kHistDepth + l) total, l = 0 newest; hvalid[s - 1]
+    // counts them.  stage 0: plain warm start from x.
     double* hist;
     int* hvalid;
     int stage;
@@ -110,9 +126,9 @@ struct BatchArgs {
     int nsurf;
     int total;                 // nodes of all surfaces
     int pad;
-    // warm start by extrapolation over steps: stage s (1..4) keeps each surface's mu of the last two
-    // steps at hist + (2 s - 2) total (newest) and hist + (2 s - 1) total; hvalid[(s - 1) nsurf + i]
-    // counts them for surface i (0..2)
+    // warm start by extrapolation over steps: stage s (1..4) keeps each surface's mu of the last
+    // kHistDepth steps at hist + ((s - 1) kHistDepth + l) total, l = 0 newest;
+    // hvalid[(s - 1) nsurf + i] counts them for surface i
     double* hist;
     int* hvalid;
 };

@@ -150,7 +150,7 @@ Solver::Solver(const Phys& ph, const std::vector<int>& n, const Options& opt) : 
         gmctl_ = dalloc<int>((size_t)kGmCtlInts * S_);
         SFH_HIP(hipMemset(gmctl_, 0, sizeof(int) * kGmCtlInts * S_));
         SFH_HIP(hipMemset(gmsmall_, 0, sizeof(double) * gm_small_stride_ * S_));
-        gmhist_ = dalloc<double>(8 * (size_t)T_);
+        gmhist_ = dalloc<double>(4 * kHistDepth * (size_t)T_);
         gmhvalid_ = dalloc<int>(4);
         SFH_HIP(hipMemset(gmhvalid_, 0, 4 * sizeof(int)));
     }
@@ -162,7 +162,7 @@ Solver::Solver(const Phys& ph, const std::vector<int>& n, const Options& opt) : 
         SFH_HIP(hipMemcpy(order_, ord.data(), S_ * sizeof(int), hipMemcpyHostToDevice));
         bstats_ = dalloc<int>(4 * (size_t)S_);
         SFH_HIP(hipMemset(bstats_, 0, 4 * S_ * sizeof(int)));
-        bhist_ = dalloc<double>(8 * (size_t)T_);
+        bhist_ = dalloc<double>(4 * kHistDepth * (size_t)T_);
         bhvalid_ = dalloc<int>(4 * (size_t)S_);
         SFH_HIP(hipMemset(bhvalid_, 0, 4 * S_ * sizeof(int)));
     }

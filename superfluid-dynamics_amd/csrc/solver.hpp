@@ -155,12 +155,12 @@ class Solver {
     long long gm_part_stride_ = 0, gm_small_stride_ = 0;
     void enqueue_gmres(int stage);
     GmArgs gm_args(int stage) const;
-    double* gmhist_ = nullptr;   // 8 x total: per RK stage, mu of the last two steps
+    double* gmhist_ = nullptr;   // 4 kHistDepth x total: per RK stage, mu of the last steps
     int* gmhvalid_ = nullptr;
     // persistent per-surface stage kernel (kSolveBatched)
     int* order_ = nullptr;    // workgroup -> surface, by decreasing N
     int* bstats_ = nullptr;   // 4 ints per surface
-    double* bhist_ = nullptr;  // per RK stage, each surface's mu of the last two steps
+    double* bhist_ = nullptr;  // per RK stage, each surface's mu of the last kHistDepth steps
     int* bhvalid_ = nullptr;
     void enqueue_batched(const double2* Z, const double* P, const StageCtl& ctl);
     // adjoint workspace (allocated on first use)
