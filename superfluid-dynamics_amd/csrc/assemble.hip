@@ -11,10 +11,9 @@
 //   M_kk -= Im(cot(i (y_k + h)))/(4 pi),  M_kj -= Im(cot((z_k - conj z_j)/2 + i h))/(4 pi)
 //
 // With the GMRES solver the kernel also starts the solve: while it writes row k it accumulates
-// (M x0)_k, stores r0_k = b_k - (M x0)_k as the first (unnormalised) basis vector and the
-// workgroup's share of |r0|^2 and |b|^2; the last workgroup to finish reduces them (beta, |b|,
-// g[0], the convergence flag), so no separate pass over M is needed for the initial residual.
-#include "sfh_gmres.hpp"
+// (M x0)_k and stores r0_k = b_k - (M x0)_k as the first (unnormalised) basis vector; the first
+// Arnoldi step normalises it.
+#include "sfh_kernels.hpp"
 
 namespace sfh {
 
@@ -29,20 +28,17 @@ __global__ __launch_bounds__(256) void assemble64_kernel(const SurfDesc* __restr
     const SurfDesc d = sd[blockIdx.y];
     const int n = d.n;
     const int k = blockIdx.x * 4 + (threadIdx.x >> 6);
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    if (!gm_on && k >= n) return;
-    if (gm_on && blockIdx.x * 4 >= n) return;            // whole workgroup beyond this surface
-    const bool valid = k < n;
-    const int kk = valid ? k : n - 1;
+    const int lane = threadIdx.x & 63;
+    if (k >= n) return;
     const NodeTrig* tr = trig + d.voff;
-    const NodeTrig tk = tr[kk];
-    const double2 zpk = Zp[d.voff + kk];
+    const NodeTrig tk = tr[k];
+    const double2 zpk = Zp[d.voff + k];
     const double* x0 = gm_on ? gm.x + d.voff : nullptr;
     const bool water = ph.kind == 0;
     const double c = water ? 0.25 * (1 - ph.rho) / kPi : 0.25 / kPi;
     const double ci = 0.25 / kPi;
     const double h = ph.depth, eh = exp(h), ehi = exp(-h);
-    double* row = M + d.moff + (long long)kk * n;
+    double* row = M + d.moff + (long long)k * n;
     double acc = 0.0;
     for (int j0 = lane; j0 < n; j0 += 4 * 64) {
         NodeTrig tj[4];
@@ -60,53 +56,22 @@ __global__ __launch_bounds__(256) void assemble64_kernel(const SurfDesc* __restr
             const int j = j0 + u * 64;
             if (j >= n) continue;
             double v;
-            if (j == kk) {
-                const double2 r = cdiv(Zpp[d.voff + kk], zpk);
+            if (j == k) {
+                const double2 r = cdiv(Zpp[d.voff + k], zpk);
                 v = (water ? 0.5 * (1 + ph.rho) : 0.5) + c * r.y;
-                if (ph.image) v -= ci * cot64(0.0, Z[d.voff + kk].y + h).y;
+                if (ph.image) v -= ci * cot64(0.0, Z[d.voff + k].y + h).y;
             } else {
                 const double2 ct = cot_pair(tk, tj[u]);
                 v = c * (zpk.x * ct.y + zpk.y * ct.x);
                 if (ph.image) v -= ci * cot_image(tk, tj[u], eh, ehi).y;
             }
-            if (valid) row[j] = v;
+            row[j] = v;
             acc += v * xj[u];
         }
     }
     if (!gm_on) return;
-    // ---- initial GMRES residual of this workgroup's rows
-    const GmView g = gm_view(gm, sd, blockIdx.y);
-    __shared__ double rr[4], bb[4];
-    __shared__ double sums[kMaxKrylov + 2];
     acc = wave_sum(acc);
-    if (lane == 0) {
-        double r = 0.0, bv = 0.0;
-        if (valid) {
-            bv = g.b[k];
-            r = bv - acc;
-            g.V[k] = r;
-        }
-        rr[wv] = r * r;
-        bb[wv] = bv * bv;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        g.part[blockIdx.x] = rr[0] + rr[1] + rr[2] + rr[3];
-        g.part[g.nwg + blockIdx.x] = bb[0] + bb[1] + bb[2] + bb[3];
-    }
-    if (!last_workgroup(g.ctl, g.nwg)) return;
-    reduce_partials(g.part, g.nwg, 2, sums);
-    if (threadIdx.x == 0) {
-        const double beta = sqrt(sums[0]), bnorm = sqrt(sums[1]);
-        GmCtl* ctl = g.ctl;
-        ctl->beta = beta;
-        ctl->bnorm = bnorm;
-        ctl->res = beta;
-        ctl->iters = 0;
-        ctl->done = (beta <= gm.tol * bnorm) ? 1 : 0;
-        g.g[0] = beta;
-        g.sc[0] = beta > 0.0 ? 1.0 / beta : 0.0;
-    }
+    if (lane == 0) gm.V[(long long)(gm.m + 1) * d.voff + k] = gm.b[d.voff + k] - acc;
 }
 
 }  // namespace sfh

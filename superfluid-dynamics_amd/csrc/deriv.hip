@@ -54,6 +54,7 @@ __global__ __launch_bounds__(1024) void zphi_deriv_kernel(const SurfDesc* __rest
     }
     __syncthreads();
     derivative_in_lds(s, d, tw, field == 1);
+    double bb = 0.0;
     for (int j = threadIdx.x; j < n; j += blockDim.x) {
         const double2 v = s[lds_index(d, j)];
         if (field == 0) {
@@ -67,6 +68,23 @@ __global__ __launch_bounds__(1024) void zphi_deriv_kernel(const SurfDesc* __rest
             if (ph.U != 0) r += -(1 + ph.rho) * kPi * ph.U / n;               // :380-383
             b[off + j] = r;
             work[off + j] = r;
+            bb += r * r;
+        }
+    }
+    // GMRES: |b| for the stopping test, and the solve's control block re-armed (the Arnoldi steps
+    // that follow in stream order read it)
+    if (gm_on && field == 2) {
+        __shared__ double red[kDerivThreads / 64];
+        bb = wave_sum(bb);
+        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = bb;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double t = 0.0;
+            for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += red[w];
+            GmCtl* c = reinterpret_cast<GmCtl*>(gm.ctl) + blockIdx.x;
+            c->bnorm = sqrt(t);
+            c->done = 0;
+            c->iters = 0;
         }
     }
     // GMRES initial guess of this RK stage: its mu over the previous steps extrapolated (error

@@ -15,8 +15,10 @@
 //      pass), its norm h_{j+1,j} = |u|, stores V[j+1] = u and sc[j+1] = 1 / |u|, applies the
 //      Givens rotations to column j and tests the residual estimate.
 // The initial residual r0 = b - M x0 is formed by the assembly kernel while it writes M
-// (assemble.hip), and the correction x += V y by the mu' kernel before it differentiates mu
-// (deriv.hip), so a solve costs the Arnoldi steps only.  Once the residual estimate reaches
+// (assemble.hip) and normalised by step 0 (every workgroup reduces |r0| over its LDS copy, so
+// no grid-wide reduction precedes the first product), |b| by the kernel that forms b, and the
+// correction x += V y is applied by the mu' kernel before it differentiates mu (deriv.hip), so a
+// solve costs the Arnoldi steps only.  Once the residual estimate reaches
 // tol * |b| the remaining launches of the fixed-length sequence return at once.
 //
 // Initial guess of an RK stage: mu of the same stage extrapolated over the three previous steps,
@@ -178,11 +180,32 @@ __global__ __launch_bounds__(256) void gm_iter_kernel(GmArgs a, const SurfDesc* 
     for (int r = 0; r < kRowsPerWG; ++r)
         vrow[r] = (t < j && r0 + r < n) ? v.V[(long long)t * n + r0 + r] : 0.0;
     const double sct = (t < j) ? v.sc[t] : 0.0;
-    const double sj = v.sc[j];
-    for (int c = t; c < n; c += 256) vs[c] = sj * v.V[(long long)j * n + c];
-    __syncthreads();
-    // w_j = M v_j for this workgroup's rows
-    const double d = row < n ? row_dot_rest(R, rc, vs, n, lane) : 0.0;    if (lane == 0) {
+    // V[j] unscaled into LDS; step 0 also forms beta = |r0| (every workgroup, same order, same value)
+    double sq = 0.0;
+    for (int c = t; c < n; c += 256) {
+        const double x = v.V[(long long)j * n + c];
+        vs[c] = x;
+        sq += x * x;
+    }
+    double beta = 0.0, sj;
+    if (j == 0) {
+        beta = sqrt(block_sum4(sq, red4));
+        if (beta <= a.tol * v.ctl->bnorm) {   // the initial guess already solves the system
+            if (blockIdx.x == 0 && t == 0) {
+                v.ctl->beta = beta;
+                v.ctl->res = beta;
+                v.ctl->done = 1;
+            }
+            return;
+        }
+        sj = 1.0 / beta;
+    } else {
+        sj = v.sc[j];
+        __syncthreads();
+    }
+    // w_j = M v_j = sc_j (M V[j]) for this workgroup's rows
+    const double d = row < n ? sj * row_dot_rest(R, rc, vs, n, lane) : 0.0;
+    if (lane == 0) {
         wl[wv] = row < n ? d : 0.0;
         if (row < n) wj[row] = d;
     }
@@ -194,7 +217,7 @@ __global__ __launch_bounds__(256) void gm_iter_kernel(GmArgs a, const SurfDesc* 
         for (int r = 0; r < kRowsPerWG; ++r) {
             const int rw = r0 + r;
             if (rw < n) {
-                const double vi = (t == j) ? vs[rw] : sct * vrow[r];
+                const double vi = (t == j) ? sj * vs[rw] : sct * vrow[r];
                 s += vi * wl[r];
             }
         }
@@ -204,7 +227,12 @@ __global__ __launch_bounds__(256) void gm_iter_kernel(GmArgs a, const SurfDesc* 
     double wpre[8];
     load_first_pass(wj, n, wpre);   // in flight together with the partial sums
     reduce_partials(v.part, v.nwg, j + 1, sums);
-    if (t <= j) co[t] = sums[t] * v.sc[t];
+    if (t <= j) co[t] = sums[t] * (t == j ? sj : v.sc[t]);
+    if (j == 0 && t == 0) {   // the solve's start: scale of v_0 and the residual norm
+        v.sc[0] = sj;
+        v.g[0] = beta;
+        v.ctl->beta = beta;
+    }
     __syncthreads();
     // u = w_j - sum_{i<=j} h_{i,j} v_i, |u|, V[j+1] = u (every entry of this one workgroup)
     const double hn = sqrt(rebuild_vector(wj, v.V, co, j + 1, n, 0, vs, nullptr, red4, wpre));
