@@ -15,10 +15,9 @@
 //      pass), its norm h_{j+1,j} = |u|, stores V[j+1] = u and sc[j+1] = 1 / |u|, applies the
 //      Givens rotations to column j and tests the residual estimate.
 // The initial residual r0 = b - M x0 is formed by the assembly kernel while it writes M
-// (assemble.hip) and normalised by step 0 (every workgroup reduces |r0| over its LDS copy, so
-// no grid-wide reduction precedes the first product), |b| by the kernel that forms b, and the
-// correction x += V y is applied by the mu' kernel before it differentiates mu (deriv.hip), so a
-// solve costs the Arnoldi steps only.  Once the residual estimate reaches
+// (assemble.hip) and normalised by step 0 (every workgroup reduces |r0| and |b| over its own
+// full copies, so no grid-wide reduction precedes the first product), and the last workgroup of
+// the final step applies the correction x += V y, so a solve costs the Arnoldi steps only.  Once the residual estimate reaches
 // tol * |b| the remaining launches of the fixed-length sequence return at once.
 //
 // Initial guess of an RK stage: mu of the same stage extrapolated over the three previous steps,
@@ -135,6 +134,15 @@ __device__ inline double rebuild_vector(const double* __restrict__ w, const doub
     return block_sum4(ss, red4);
 }
 
+// This surface's history of solutions of the current RK stage (null for a plain evaluation), and
+// the stage's count of stored steps (one increment per solve, by surface 0).
+__device__ inline double* history(const GmArgs& a, const SurfDesc* sd) {
+    return a.stage > 0 ? a.hist + (long long)(a.stage - 1) * kHistDepth * a.total + sd[blockIdx.z].voff : nullptr;
+}
+__device__ inline void count_history(const GmArgs& a) {
+    a.hvalid[a.stage - 1] = min(a.hvalid[a.stage - 1] + 1, kHistDepth);
+}
+
 // Applies the stored rotations to column jc (entries 0..jc+1 in col), forms the new rotation and
 // updates g; returns |g[jc+1]|, the residual estimate.
 __device__ inline double givens_column(double* col, double* cs, double* sn, double* g, int jc) {
@@ -180,22 +188,40 @@ __global__ __launch_bounds__(256) void gm_iter_kernel(GmArgs a, const SurfDesc* 
     for (int r = 0; r < kRowsPerWG; ++r)
         vrow[r] = (t < j && r0 + r < n) ? v.V[(long long)t * n + r0 + r] : 0.0;
     const double sct = (t < j) ? v.sc[t] : 0.0;
-    // V[j] unscaled into LDS; step 0 also forms beta = |r0| (every workgroup, same order, same value)
-    double sq = 0.0;
+    // V[j] unscaled into LDS; step 0 also forms beta = |r0| and |b| (every workgroup, same order,
+    // same values)
+    double sq = 0.0, bq = 0.0;
     for (int c = t; c < n; c += 256) {
         const double x = v.V[(long long)j * n + c];
         vs[c] = x;
         sq += x * x;
+        if (j == 0) {
+            const double bc = v.b[c];
+            bq += bc * bc;
+        }
     }
-    double beta = 0.0, sj;
+    double beta = 0.0, bnorm = 0.0, sj;
     if (j == 0) {
-        beta = sqrt(block_sum4(sq, red4));
-        if (beta <= a.tol * v.ctl->bnorm) {   // the initial guess already solves the system
+        __shared__ double red8[8];
+        sq = wave_sum(sq);
+        bq = wave_sum(bq);
+        if (lane == 0) {
+            red8[wv] = sq;
+            red8[4 + wv] = bq;
+        }
+        __syncthreads();
+        beta = sqrt(red8[0] + red8[1] + red8[2] + red8[3]);
+        bnorm = sqrt(red8[4] + red8[5] + red8[6] + red8[7]);
+        if (beta <= a.tol * bnorm) {   // the initial guess already solves the system
             if (blockIdx.x == 0 && t == 0) {
+                v.ctl->bnorm = bnorm;
                 v.ctl->beta = beta;
                 v.ctl->res = beta;
                 v.ctl->done = 1;
             }
+            double* h0 = history(a, sd);   // x = x0 enters the history, this workgroup's rows
+            if (h0 && t < kRowsPerWG && r0 + t < n) push_history(h0 + r0 + t, a.total, v.x[r0 + t]);
+            if (h0 && blockIdx.x == 0 && blockIdx.z == 0 && t == 0) count_history(a);
             return;
         }
         sj = 1.0 / beta;
@@ -228,15 +254,16 @@ __global__ __launch_bounds__(256) void gm_iter_kernel(GmArgs a, const SurfDesc* 
     load_first_pass(wj, n, wpre);   // in flight together with the partial sums
     reduce_partials(v.part, v.nwg, j + 1, sums);
     if (t <= j) co[t] = sums[t] * (t == j ? sj : v.sc[t]);
-    if (j == 0 && t == 0) {   // the solve's start: scale of v_0 and the residual norm
+    if (j == 0 && t == 0) {   // the solve's start: scale of v_0, the residual norm, |b|
         v.sc[0] = sj;
         v.g[0] = beta;
         v.ctl->beta = beta;
+        v.ctl->bnorm = bnorm;
     }
     __syncthreads();
-    // u = w_j - sum_{i<=j} h_{i,j} v_i, |u|, V[j+1] = u (every entry of this one workgroup)
+    // u = w_j - sum_{i<=j} h_{i,j} v_i, |u| (every entry of this one workgroup, into LDS)
     const double hn = sqrt(rebuild_vector(wj, v.V, co, j + 1, n, 0, vs, nullptr, red4, wpre));
-    for (int c = t; c < n; c += 256) v.V[(long long)(j + 1) * n + c] = vs[c];
+    __shared__ int fin;
     if (t == 0) {
         GmCtl* c = v.ctl;
         double* col = v.H + (long long)j * (m + 1);
@@ -252,7 +279,65 @@ __global__ __launch_bounds__(256) void gm_iter_kernel(GmArgs a, const SurfDesc* 
             c->done = 1;
             c->unconverged += 1;
         }
+        fin = c->done;
     }
+    __syncthreads();
+    if (!fin) {   // V[j+1] = u for the next step
+        for (int c = t; c < n; c += 256) v.V[(long long)(j + 1) * n + c] = vs[c];
+        return;
+    }
+    // the solve ends here: y from the rotated Hessenberg system R y = g (one thread), then
+    // x += sum_i y_i v_i over all entries and the stage's history
+    const int k = j + 1;
+    if (t == 0) {
+        for (int i = k - 1; i >= 0; --i) {   // co holds y first, then y_i sc[i]
+            double acc = v.g[i];
+            for (int l = i + 1; l < k; ++l) acc -= v.H[(long long)l * (m + 1) + i] * co[l];
+            co[i] = acc / v.H[(long long)i * (m + 1) + i];
+        }
+        for (int i = 0; i < k; ++i) co[i] *= v.sc[i];
+    }
+    __syncthreads();
+    double* h0 = history(a, sd);
+    const long long T = a.total;
+    for (int base = 0; base < n; base += 8 * 256) {   // 8 entries per thread, every load in flight
+        double xs[8], p0[8], p1[8];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const int c = base + t + 256 * e;
+            xs[e] = c < n ? v.x[c] : 0.0;
+            p0[e] = (h0 && c < n) ? h0[c] : 0.0;
+            p1[e] = (h0 && c < n) ? h0[T + c] : 0.0;
+        }
+        for (int i0 = 0; i0 < k; i0 += 4) {
+            double vv[4][8];
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+#pragma unroll
+                for (int e = 0; e < 8; ++e) {
+                    const int c = base + t + 256 * e;
+                    vv[q][e] = (i0 + q < k && c < n) ? v.V[(long long)(i0 + q) * n + c] : 0.0;
+                }
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const double y = i0 + q < k ? co[i0 + q] : 0.0;
+#pragma unroll
+                for (int e = 0; e < 8; ++e) xs[e] += y * vv[q][e];
+            }
+        }
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const int c = base + t + 256 * e;
+            if (c >= n) continue;
+            v.x[c] = xs[e];
+            if (h0) {
+                h0[2 * T + c] = p1[e];
+                h0[T + c] = p0[e];
+                h0[c] = xs[e];
+            }
+        }
+    }
+    if (h0 && blockIdx.z == 0 && t == 0) count_history(a);
 }
 
 }  // namespace sfh

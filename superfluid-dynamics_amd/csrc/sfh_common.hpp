@@ -22,7 +22,7 @@ struct SurfDesc {
     int voff;       // vector offset (nodes)
     long long moff; // matrix offset (elements)
     int twoff;      // offset of this n's twiddle table (complex entries, n of them)
-    int pad;
+    int coff;       // offset of this n's circulant derivative kernels (3 n doubles: Re c1, Im c1, c2)
 };
 
 // Physics selector passed by value to kernels.

@@ -61,14 +61,14 @@ struct GmArgs {
     int stage;
     int pad;
 };
+// ---- spectral.hip
+constexpr int kCircRowsPerWG = 8;   // kCircRows in spectral.hip
+__global__ void zphi_circ_kernel(const SurfDesc* sd, const double* circ, const double2* Z, const double* phi,
+                                 double2* Zp, double2* Zpp, double* b, double* work, Phys ph, int* status,
+                                 NodeTrig* trig, GmArgs gm, int gm_on);
+__global__ void mu_circ_kernel(const SurfDesc* sd, const double* circ, const double* mu, double2* mup);
 // ---- deriv.hip
 constexpr int kDerivThreads = 1024;
-__global__ void zphi_deriv_kernel(const SurfDesc* sd, const double2* tw_all, const double2* Z,
-                                  const double* phi, double2* Zp, double2* Zpp, double* b,
-                                  double* work, Phys ph, int* status, int lds_tw, NodeTrig* trig, GmArgs gm,
-                                  int gm_on);
-__global__ void mu_deriv_kernel(const SurfDesc* sd, const double2* tw_all, double* mu, double2* mup, int lds_tw,
-                                GmArgs gm, int gm_on);
 __global__ void plain_deriv_kernel(const SurfDesc* sd, const double2* tw_all, const double2* in,
                                    double2* out, int second, double scale, int lds_tw);
 

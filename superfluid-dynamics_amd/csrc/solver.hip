@@ -2,11 +2,12 @@
 //
 // One RHS evaluation (BaseBoundaryIntegralCalculator::runTimeStep,
 // CuSuperHelium/BaseBoundaryIntegrator.cuh:137-306) is enqueued as
-//   zphi_deriv_kernel     Z', Z'', Re Phi'            (S x 3 workgroups, LDS FFT)
+//   zphi_circ_kernel      Z', Z'', Re Phi'            (circulant products, 8 rows per workgroup)
 //   assemble64_kernel     M                           (one wave per row)
 //   LU solve              mu                          (panel / MFMA update / back substitution)
 //   GEPP fallback         no-op unless a system needed row interchanges
-//   mu_deriv_kernel       mu'
+//     or GMRES            mu                          (one kernel per Arnoldi step)
+//   mu_circ_kernel        mu'
 //   velocity_stage_kernel q, dPhi/dt and the RK4 stage update of every node
 // and one RK4 step (AutonomousRungeKuttaStepper.cuh:123-361) is four of them.  The step is
 // captured once into a hipGraph and replayed, so a run of many steps costs one graph launch
@@ -24,6 +25,46 @@ bool supported_n(int n) {
     int m = n;
     if (m % 3 == 0) m /= 3;
     return (m & (m - 1)) == 0;
+}
+
+// Circulant kernels of the spectral derivatives for n nodes, appended to out as Re c1, Im c1, c2
+// (n doubles each): c_q = sum_i m_i exp(+2 pi i i q / n) over the multiplier table of
+// utilities.cuh:119-147 (first derivative: i m_i with m_i = i / n below n/2, pi i / n at n/2,
+// 0 at n/2 + 1, (i - n) / n above) and :185-199 (second: -i^2 / n up to n/2, -(i - n)^2 / n
+// above), summed in long double with exactly reduced angles.
+static void circulant_tables(int n, std::vector<double>& out) {
+    const long double pi = 3.141592653589793238462643383279502884L;
+    std::vector<long double> cs(n), sn(n), m1(n), m2(n);
+    for (int t = 0; t < n; ++t) {
+        const long double a = 2.0L * pi * (long double)t / (long double)n;
+        cs[t] = cosl(a);
+        sn[t] = sinl(a);
+    }
+    for (int i = 0; i < n; ++i) {
+        const long double ln = (long double)n;
+        if (i < n / 2) m1[i] = (long double)i / ln;
+        else if (i == n / 2) m1[i] = pi * (long double)i / ln;
+        else if (i == n / 2 + 1) m1[i] = 0.0L;
+        else m1[i] = (long double)(i - n) / ln;
+        const long double f = (i <= n / 2) ? (long double)i : (long double)(i - n);
+        m2[i] = -f * f / ln;
+    }
+    const size_t base = out.size();
+    out.resize(base + 3 * (size_t)n);
+    for (int q = 0; q < n; ++q) {
+        long double re = 0.0L, im = 0.0L, sec = 0.0L;
+        long long t = 0;   // i q mod n
+        for (int i = 0; i < n; ++i) {
+            re -= m1[i] * sn[t];   // i m (cos + i sin) = -m sin + i m cos
+            im += m1[i] * cs[t];
+            sec += m2[i] * cs[t];  // m2 is symmetric: the sine terms cancel
+            t += q;
+            if (t >= n) t -= n;
+        }
+        out[base + q] = (double)re;
+        out[base + n + q] = (double)im;
+        out[base + 2 * (size_t)n + q] = (double)sec;
+    }
 }
 
 static SurfDesc make_desc(int n) {
@@ -63,9 +104,11 @@ Solver::Solver(const Phys& ph, const std::vector<int>& n, const Options& opt) : 
             if (v > kBatchMaxN) throw ArgError("the batched solver takes surfaces of at most 1024 nodes");
     SFH_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
 
-    // descriptors and twiddle tables (one table per distinct n, computed in long double)
+    // descriptors, twiddle tables and circulant derivative kernels (one table each per distinct
+    // n, computed in long double)
     std::vector<double2> tw;
-    std::vector<std::pair<int, int>> tw_of;
+    std::vector<double> circ;
+    std::vector<std::pair<int, int>> tw_of, circ_of;
     long long moff = 0;
     int voff = 0;
     for (int s = 0; s < S_; ++s) {
@@ -85,6 +128,17 @@ Solver::Solver(const Phys& ph, const std::vector<int>& n, const Options& opt) : 
             }
         }
         d.twoff = off;
+        if (opt_.solver != kSolveBatched) {
+            int coff = -1;
+            for (auto& p : circ_of)
+                if (p.first == n[s]) coff = p.second;
+            if (coff < 0) {
+                coff = (int)circ.size();
+                circ_of.push_back({n[s], coff});
+                circulant_tables(n[s], circ);
+            }
+            d.coff = coff;
+        }
         desc_.push_back(d);
         voff += n[s];
         moff += (long long)n[s] * n[s];
@@ -98,6 +152,10 @@ Solver::Solver(const Phys& ph, const std::vector<int>& n, const Options& opt) : 
     SFH_HIP(hipMemcpy(d_desc_, desc_.data(), S_ * sizeof(SurfDesc), hipMemcpyHostToDevice));
     d_tw_ = dalloc<double2>(tw.size());
     SFH_HIP(hipMemcpy(d_tw_, tw.data(), tw.size() * sizeof(double2), hipMemcpyHostToDevice));
+    if (!circ.empty()) {
+        d_circ_ = dalloc<double>(circ.size());
+        SFH_HIP(hipMemcpy(d_circ_, circ.data(), circ.size() * sizeof(double), hipMemcpyHostToDevice));
+    }
 
     Z0_ = dalloc<double2>(T_);
     Za_ = dalloc<double2>(T_);
@@ -176,7 +234,7 @@ Solver::~Solver() {
     void* bufs[] = {d_desc_, d_tw_, Z0_, Za_, Zb_, P0_, Pa_, Pb_, kZ_[0], kZ_[1], kZ_[2], kP_[0],
                     kP_[1], kP_[2], Zp_, Zpp_, mup_, qup_, b_, work_, y_, mu_, M_, uinv_, flags_,
                     status_, piv_, counters_, gmV_, gmw_, gmpart_, gmsmall_, gmctl_, trig_, order_, bstats_,
-                    gmhist_, gmhvalid_, bhist_, bhvalid_};
+                    gmhist_, gmhvalid_, bhist_, bhvalid_, d_circ_};
     for (void* p : bufs)
         if (p) (void)hipFree(p);
     void* vbufs[] = {vY_[0], vY_[1], vY_[2], vK_[0], vK_[1], vK_[2], vYP_[0], vYP_[1], vYP_[2], vKP_[0],
@@ -250,8 +308,8 @@ GmArgs Solver::gm_args(int stage) const {
 }
 
 // Arnoldi steps of the GMRES solve of all surfaces at once (gridDim.z = surface).  The initial
-// residual comes from the assembly kernel and the correction x += V y is applied by the mu'
-// kernel, so only the fixed-length sequence of steps is launched here.
+// residual comes from the assembly kernel and the final step applies the correction x += V y,
+// so only the fixed-length sequence of steps is launched here.
 void Solver::enqueue_gmres(int stage) {
     const GmArgs a = gm_args(stage);
     double nsq = 0.0;
@@ -367,18 +425,15 @@ void Solver::enqueue_rhs(const double2* Z, const double* P, const StageCtl& ctl)
     }
     const int gm_on = opt_.solver == kSolveGMRES ? 1 : 0;
     const GmArgs ga = gm_on ? gm_args(ctl.stage) : GmArgs{};
-    // data + twiddles in LDS when both fit (n <= 4096), data only otherwise
-    const int lds_tw = (2 * (size_t)maxN_ * sizeof(double2) <= 144 * 1024) ? 1 : 0;
-    const size_t lds = (size_t)maxN_ * sizeof(double2) * (lds_tw ? 2 : 1);
-    double nsq = 0.0, nlog = 0.0;
-    for (int v : n_) {
-        nsq += (double)v * v;
-        nlog += 5.0 * v * std::log2((double)std::max(v, 2));
-    }
+    double nsq = 0.0;
+    for (int v : n_) nsq += (double)v * v;
     const double img = ph_.image ? 2.0 : 1.0;
-    launch("deriv", 3 * 2 * nlog, 64.0 * T_, [&] {
-        hipLaunchKernelGGL(zphi_deriv_kernel, dim3(S_, 3), dim3(kDerivThreads), lds, stream_, d_desc_, d_tw_, Z,
-                           P, Zp_, Zpp_, b_, work_, ph_, status_, lds_tw, trig_, ga, gm_on);
+    const dim3 cgrid((maxN_ + kCircRowsPerWG - 1) / kCircRowsPerWG, S_);
+    // circulant derivatives: 7 n^2 multiply-adds; bytes: the samples and kernels each workgroup
+    // stages (24 B per node and 24 B per kernel entry)
+    launch("deriv", 14.0 * nsq, 48.0 * nsq / kCircRowsPerWG, [&] {
+        hipLaunchKernelGGL(zphi_circ_kernel, cgrid, dim3(256), 0, stream_, d_desc_, (const double*)d_circ_, Z, P,
+                           Zp_, Zpp_, b_, work_, ph_, status_, trig_, ga, gm_on);
     });
     // assembly: flops counted as cot evaluations (the transcendental-bound unit), bytes = M written
     launch("assemble", img * nsq, 8.0 * nsq, [&] {
@@ -386,9 +441,9 @@ void Solver::enqueue_rhs(const double2* Z, const double* P, const StageCtl& ctl)
                            (const NodeTrig*)trig_, Z, Zp_, Zpp_, M_, ph_, (const int*)nullptr, ga, gm_on);
     });
     enqueue_solve(Z, ctl.stage);
-    launch("mu_deriv", 2 * nlog, 24.0 * T_, [&] {
-        hipLaunchKernelGGL(mu_deriv_kernel, dim3(S_), dim3(kDerivThreads), lds, stream_, d_desc_, d_tw_, mu_, mup_,
-                           lds_tw, ga, gm_on);
+    launch("mu_deriv", 4.0 * nsq, 24.0 * nsq / kCircRowsPerWG, [&] {
+        hipLaunchKernelGGL(mu_circ_kernel, cgrid, dim3(256), 0, stream_, d_desc_, (const double*)d_circ_,
+                           (const double*)mu_, mup_);
     });
     launch("velocity", img * nsq, 96.0 * T_, [&] {
         hipLaunchKernelGGL(velocity_stage_kernel, dim3((maxN_ + 3) / 4, S_), dim3(256), 0, stream_, d_desc_,

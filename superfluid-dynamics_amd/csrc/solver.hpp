@@ -134,6 +134,7 @@ class Solver {
 
     SurfDesc* d_desc_ = nullptr;
     double2* d_tw_ = nullptr;
+    double* d_circ_ = nullptr;   // circulant derivative kernels (multi-workgroup path)
     double2 *Z0_ = nullptr, *Za_ = nullptr, *Zb_ = nullptr;
     double *P0_ = nullptr, *Pa_ = nullptr, *Pb_ = nullptr;
     double2* kZ_[3] = {nullptr, nullptr, nullptr};
