@@ -1,10 +1,12 @@
 // assemble.hip -- dipole matrix M = I/2 + K in FP64, one wave per row.
 //
 // Replaces createMKernel (CuSuperHelium/createM.cuh:43-63) and createFiniteDepthMKernel
-// (:65-92), which run one thread per entry on 16x16 blocks.  Here wave w of a workgroup owns
-// row k; its 64 lanes sweep the columns, so Z_k, Z'_k stay in registers, Z_j loads and M row
-// stores are fully coalesced, and cot comes from the per-node half-angle table (NodeTrig) with no
-// transcendental call per entry.
+// (:65-92), which run one thread per entry on 16x16 blocks.  Here a workgroup owns kPairRows
+// rows: every thread takes the columns j = t (mod 256), loads the node's half-angle table entry
+// (NodeTrig) once and forms the entries of all the workgroup's rows with it, so each table entry
+// is read once per kPairRows rows, a thread has kPairRows independent entries in flight per
+// column, and a wave's stores to one row are 512 contiguous bytes.  cot comes from the table with
+// no transcendental call per entry.
 //   M_kk = (1+rho)/2 + (1-rho)/(4 pi) Im(Z''_k / Z'_k)                       (water)
 //   M_kj = (1-rho)/(4 pi) Im(Z'_k cot((z_k - z_j)/2))
 //   helium: rho -> 0 and, with finite depth h, the image terms
@@ -24,54 +26,79 @@ __global__ __launch_bounds__(256) void assemble64_kernel(const SurfDesc* __restr
                                                          const double2* __restrict__ Zpp,
                                                          double* __restrict__ M, Phys ph,
                                                          const int* __restrict__ only_if, GmArgs gm, int gm_on) {
+    constexpr int R = kPairRows;
     if (only_if && only_if[2 * blockIdx.y] == 0) return;   // per-surface status word
     const SurfDesc d = sd[blockIdx.y];
     const int n = d.n;
-    const int k = blockIdx.x * 4 + (threadIdx.x >> 6);
-    const int lane = threadIdx.x & 63;
-    if (k >= n) return;
+    const int k0 = blockIdx.x * R;
+    if (k0 >= n) return;
+    const int t = threadIdx.x;
     const NodeTrig* tr = trig + d.voff;
-    const NodeTrig tk = tr[k];
-    const double2 zpk = Zp[d.voff + k];
     const double* x0 = gm_on ? gm.x + d.voff : nullptr;
     const bool water = ph.kind == 0;
     const double c = water ? 0.25 * (1 - ph.rho) / kPi : 0.25 / kPi;
     const double ci = 0.25 / kPi;
     const double h = ph.depth, eh = exp(h), ehi = exp(-h);
-    double* row = M + d.moff + (long long)k * n;
-    double acc = 0.0;
-    for (int j0 = lane; j0 < n; j0 += 4 * 64) {
-        NodeTrig tj[4];
-        double xj[4];
+    // the rows' constants (the same in every thread)
+    NodeTrig tk[R];
+    double2 zpk[R];
+    double diag[R];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const int j = j0 + u * 64;
-            if (j < n) {
-                tj[u] = tr[j];
-                xj[u] = gm_on ? x0[j] : 0.0;
-            }
+    for (int r = 0; r < R; ++r) {
+        const int k = min(k0 + r, n - 1);
+        tk[r] = tr[k];
+        zpk[r] = Zp[d.voff + k];
+        const double2 q = cdiv(Zpp[d.voff + k], zpk[r]);
+        diag[r] = (water ? 0.5 * (1 + ph.rho) : 0.5) + c * q.y;
+        if (ph.image) diag[r] -= ci * cot64(0.0, Z[d.voff + k].y + h).y;
+    }
+    double* Mk = M + d.moff + (long long)k0 * n;
+    double acc[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) acc[r] = 0.0;
+    // columns in passes of 8 per thread: a pass loads the table entries of its 8 columns (all in
+    // flight) and then forms their entries of every row.  Each workgroup starts its sweep at its
+    // own column offset (a multiple of 64): rows are n * 8 bytes apart, so workgroups sweeping in
+    // lockstep from column 0 would all store to the same few memory channels at once.
+    const int shift = (int)(((long long)blockIdx.x * 64) % n);
+    for (int j0 = 0; j0 < n; j0 += 8 * 256) {
+        NodeTrig tj[8];
+        double xj[8];
+        int jc[8];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {   // unconditional (clamped) loads: no branch between them
+            int j = min(j0 + t + 256 * e, n - 1) + shift;
+            if (j >= n) j -= n;
+            jc[e] = j;
+            tj[e] = tr[j];
+            xj[e] = gm_on ? x0[j] : 0.0;
         }
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const int j = j0 + u * 64;
-            if (j >= n) continue;
-            double v;
-            if (j == k) {
-                const double2 r = cdiv(Zpp[d.voff + k], zpk);
-                v = (water ? 0.5 * (1 + ph.rho) : 0.5) + c * r.y;
-                if (ph.image) v -= ci * cot64(0.0, Z[d.voff + k].y + h).y;
-            } else {
-                const double2 ct = cot_pair(tk, tj[u]);
-                v = c * (zpk.x * ct.y + zpk.y * ct.x);
-                if (ph.image) v -= ci * cot_image(tk, tj[u], eh, ehi).y;
+        for (int e = 0; e < 8; ++e) {
+            if (j0 + t + 256 * e >= n) break;
+            const int j = jc[e];
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const int k = k0 + r;
+                if (k >= n) break;
+                double v;
+                if (j == k) {
+                    v = diag[r];
+                } else {
+                    const double2 ct = cot_pair(tk[r], tj[e]);
+                    v = c * (zpk[r].x * ct.y + zpk[r].y * ct.x);
+                    if (ph.image) v -= ci * cot_image(tk[r], tj[e], eh, ehi).y;
+                }
+                Mk[(long long)r * n + j] = v;
+                acc[r] += v * xj[e];
             }
-            row[j] = v;
-            acc += v * xj[u];
         }
     }
     if (!gm_on) return;
-    acc = wave_sum(acc);
-    if (lane == 0) gm.V[(long long)(gm.m + 1) * d.voff + k] = gm.b[d.voff + k] - acc;
+    __shared__ double red[(R < 4 ? 4 : R) * 256];
+    __shared__ double out[R];
+    rows_reduce<R>(acc, red, out);
+    if (t < R && k0 + t < n) gm.V[(long long)(gm.m + 1) * d.voff + k0 + t] = gm.b[d.voff + k0 + t] - out[t];
 }
 
 }  // namespace sfh

@@ -152,4 +152,38 @@ __device__ inline double wave_sum(double v) {
     return v;
 }
 
+// Sums of NV per-thread values over a 256-thread workgroup, into out[0..NV) (LDS; valid after the
+// call).  NV >= 4: the values go through LDS transposed, P = 256 / NV threads add NV entries each
+// and a P-lane butterfly finishes, so no value takes a full 64-lane shuffle chain; NV < 4: one
+// wave sum per value and the four waves' results added.  Fixed order: the result does not depend
+// on timing.  red holds max(NV, 4) * 256 doubles.
+template <int NV>
+__device__ inline void rows_reduce(const double (&v)[NV], double* red, double* out) {
+    static_assert(NV >= 1 && NV <= 256 && (NV & (NV - 1)) == 0, "NV: a power of two in [1, 256]");
+    const int t = threadIdx.x;
+    if constexpr (NV < 4) {
+        const int lane = t & 63, w = t >> 6;
+#pragma unroll
+        for (int q = 0; q < NV; ++q) {
+            const double s = wave_sum(v[q]);
+            if (lane == 0) red[q * 4 + w] = s;
+        }
+        __syncthreads();
+        if (t < NV) out[t] = red[t * 4] + red[t * 4 + 1] + red[t * 4 + 2] + red[t * 4 + 3];
+        __syncthreads();
+    } else {
+        constexpr int P = 256 / NV;
+#pragma unroll
+        for (int q = 0; q < NV; ++q) red[q * 256 + t] = v[q];
+        __syncthreads();
+        const int q = t / P, p = t % P;
+        double s = 0.0;
+#pragma unroll
+        for (int e = 0; e < NV; ++e) s += red[q * 256 + p + P * e];
+        for (int o = P / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+        if (p == 0) out[q] = s;
+        __syncthreads();
+    }
+}
+
 }  // namespace sfh

@@ -61,6 +61,7 @@ struct GmArgs {
     int stage;
     int pad;
 };
+constexpr int kPairRows = 4;        // rows per workgroup of the N^2 pair kernels (assembly, velocity)
 // ---- spectral.hip
 constexpr int kCircRowsPerWG = 8;   // kCircRows in spectral.hip
 __global__ void zphi_circ_kernel(const SurfDesc* sd, const double* circ, const double2* Z, const double* phi,

@@ -3,7 +3,7 @@
 // One RHS evaluation (BaseBoundaryIntegralCalculator::runTimeStep,
 // CuSuperHelium/BaseBoundaryIntegrator.cuh:137-306) is enqueued as
 //   zphi_circ_kernel      Z', Z'', Re Phi'            (circulant products, 8 rows per workgroup)
-//   assemble64_kernel     M                           (one wave per row)
+//   assemble64_kernel     M                           (8 rows per workgroup)
 //   LU solve              mu                          (panel / MFMA update / back substitution)
 //   GEPP fallback         no-op unless a system needed row interchanges
 //     or GMRES            mu                          (one kernel per Arnoldi step)
@@ -377,7 +377,7 @@ void Solver::enqueue_solve(const double2* Z, int stage) {
         for (int s = 0; s < S_; ++s) run(s, 1, n_[s]);
     }
     // systems whose interchange-free check failed: re-assemble and solve with GEPP
-    hipLaunchKernelGGL(assemble64_kernel, dim3((maxN_ + 3) / 4, S_), dim3(256), 0, stream_, d_desc_,
+    hipLaunchKernelGGL(assemble64_kernel, dim3((maxN_ + kPairRows - 1) / kPairRows, S_), dim3(256), 0, stream_, d_desc_,
                        (const NodeTrig*)trig_, Z, Zp_, Zpp_, M_, ph_, (const int*)status_, GmArgs{}, 0);
     hipLaunchKernelGGL(gepp_fallback_kernel, dim3(S_), dim3(256), 0, stream_, d_desc_, M_, b_, mu_, piv_,
                        status_, counters_);
@@ -437,7 +437,7 @@ void Solver::enqueue_rhs(const double2* Z, const double* P, const StageCtl& ctl)
     });
     // assembly: flops counted as cot evaluations (the transcendental-bound unit), bytes = M written
     launch("assemble", img * nsq, 8.0 * nsq, [&] {
-        hipLaunchKernelGGL(assemble64_kernel, dim3((maxN_ + 3) / 4, S_), dim3(256), 0, stream_, d_desc_,
+        hipLaunchKernelGGL(assemble64_kernel, dim3((maxN_ + kPairRows - 1) / kPairRows, S_), dim3(256), 0, stream_, d_desc_,
                            (const NodeTrig*)trig_, Z, Zp_, Zpp_, M_, ph_, (const int*)nullptr, ga, gm_on);
     });
     enqueue_solve(Z, ctl.stage);
@@ -446,7 +446,7 @@ void Solver::enqueue_rhs(const double2* Z, const double* P, const StageCtl& ctl)
                            (const double*)mu_, mup_);
     });
     launch("velocity", img * nsq, 96.0 * T_, [&] {
-        hipLaunchKernelGGL(velocity_stage_kernel, dim3((maxN_ + 3) / 4, S_), dim3(256), 0, stream_, d_desc_,
+        hipLaunchKernelGGL(velocity_stage_kernel, dim3((maxN_ + kPairRows - 1) / kPairRows, S_), dim3(256), 0, stream_, d_desc_,
                            (const NodeTrig*)trig_, Z, Zp_, Zpp_, mu_, mup_, ph_, ctl);
     });
     SFH_HIP(hipGetLastError());

@@ -12,16 +12,18 @@
 // An in-LDS FFT of one field is a chain of dependent LDS passes inside a single workgroup, so at
 // n = 2048 it takes ~4 us each way on one CU while the rest of the GPU idles.  As circulant
 // products the three derivatives of a right-hand side (Z', Z'', Re Phi') are 7 n^2 multiply-adds
-// spread over n / 8 workgroups of 8 rows; each workgroup streams the samples and the kernel
-// window it needs through LDS in chunks of kCircChunk columns, and every row is summed inside one
-// workgroup (no cross-workgroup reduction).
+// spread over n / 8 workgroups of 8 rows; each workgroup stages the samples and the kernel
+// window it needs in LDS, kCircChunk columns at a time with all of a chunk's loads in flight
+// together, and every row is summed inside one workgroup (no cross-workgroup reduction).
 #include "sfh_gmres.hpp"
 
 namespace sfh {
 
 constexpr int kCircRows = kCircRowsPerWG;   // rows per workgroup: wave w owns rows w and w + 4
-constexpr int kCircChunk = 1024;   // columns staged per LDS round
+constexpr int kCircChunk = 2048;   // columns staged per LDS round
 constexpr int kCircWin = kCircChunk + kCircRows - 1;
+constexpr int kCircPer = kCircChunk / 256;               // staged samples per thread
+constexpr int kCircWinPer = (kCircWin + 255) / 256;      // staged kernel entries per thread
 
 struct CircLds {
     double2 z[kCircChunk];   // samples of the chunk (Z minus its linear part; mu in .x)
@@ -74,17 +76,40 @@ __global__ __launch_bounds__(256) void zphi_circ_kernel(const SurfDesc* __restri
     for (int j0 = 0; j0 < n; j0 += kCircChunk) {
         const int jn = min(kCircChunk, n - j0);
         __syncthreads();   // the previous chunk has been consumed
-        for (int c = t; c < jn; c += 256) {
-            const int j = j0 + c;
-            double2 v = Z[off + j];
-            v.x -= 2 * kPi * (double)j / n;
-            L.z[c] = v;
-            L.p[c] = phi[off + j] - plin * (double)j;
-        }
-        for (int u = t; u < kCircWin; u += 256) {
-            const int q = circ_index(k0, j0, u, n);
-            L.c1[u] = make_double2(c1r[q], c1i[q]);
-            L.c2[u] = c2[q];
+        {   // every load of the chunk in flight before the first LDS store
+            double2 zv[kCircPer];
+            double pv[kCircPer], ar[kCircWinPer], ai[kCircWinPer], bv[kCircWinPer];
+#pragma unroll
+            for (int e = 0; e < kCircPer; ++e) {
+                const int c = t + 256 * e;
+                zv[e] = c < jn ? Z[off + j0 + c] : make_double2(0.0, 0.0);
+                pv[e] = c < jn ? phi[off + j0 + c] : 0.0;
+            }
+#pragma unroll
+            for (int e = 0; e < kCircWinPer; ++e) {
+                const int u = t + 256 * e;
+                const int q = circ_index(k0, j0, u < kCircWin ? u : 0, n);
+                ar[e] = c1r[q];
+                ai[e] = c1i[q];
+                bv[e] = c2[q];
+            }
+#pragma unroll
+            for (int e = 0; e < kCircPer; ++e) {
+                const int c = t + 256 * e;
+                if (c < jn) {
+                    const int j = j0 + c;
+                    L.z[c] = make_double2(zv[e].x - 2 * kPi * (double)j / n, zv[e].y);
+                    L.p[c] = pv[e] - plin * (double)j;
+                }
+            }
+#pragma unroll
+            for (int e = 0; e < kCircWinPer; ++e) {
+                const int u = t + 256 * e;
+                if (u < kCircWin) {
+                    L.c1[u] = make_double2(ar[e], ai[e]);
+                    L.c2[u] = bv[e];
+                }
+            }
         }
         __syncthreads();
 #pragma unroll 4
@@ -160,10 +185,30 @@ __global__ __launch_bounds__(256) void mu_circ_kernel(const SurfDesc* __restrict
     for (int j0 = 0; j0 < n; j0 += kCircChunk) {
         const int jn = min(kCircChunk, n - j0);
         __syncthreads();
-        for (int c = t; c < jn; c += 256) m[c] = mu[off + j0 + c];
-        for (int u = t; u < kCircWin; u += 256) {
-            const int q = circ_index(k0, j0, u, n);
-            c1[u] = make_double2(c1r[q], c1i[q]);
+        {   // every load of the chunk in flight before the first LDS store
+            double mv[kCircPer], ar[kCircWinPer], ai[kCircWinPer];
+#pragma unroll
+            for (int e = 0; e < kCircPer; ++e) {
+                const int c = t + 256 * e;
+                mv[e] = c < jn ? mu[off + j0 + c] : 0.0;
+            }
+#pragma unroll
+            for (int e = 0; e < kCircWinPer; ++e) {
+                const int u = t + 256 * e;
+                const int q = circ_index(k0, j0, u < kCircWin ? u : 0, n);
+                ar[e] = c1r[q];
+                ai[e] = c1i[q];
+            }
+#pragma unroll
+            for (int e = 0; e < kCircPer; ++e) {
+                const int c = t + 256 * e;
+                if (c < jn) m[c] = mv[e];
+            }
+#pragma unroll
+            for (int e = 0; e < kCircWinPer; ++e) {
+                const int u = t + 256 * e;
+                if (u < kCircWin) c1[u] = make_double2(ar[e], ai[e]);
+            }
         }
         __syncthreads();
 #pragma unroll 4

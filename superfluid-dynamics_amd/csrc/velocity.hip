@@ -7,9 +7,11 @@
 // (createM.cuh:96-117,171-211) and the Zaxpy / add_k_vectors stages of
 // AutonomousRungeKuttaStepper::runStep (AutonomousRungeKuttaStepper.cuh:123-361).
 //
-// One wave per node k.  The lanes sweep j != k accumulating
-//   S_k = sum_j mu_j cot((z_k - z_j)/2)      Si_k = sum_j mu_j cot((z_k - conj z_j)/2 + i h)
-// and lane 0 finishes
+// A workgroup owns kPairRows nodes k.  Every thread takes the columns j = t (mod 256) (each
+// workgroup starting at its own offset), loads the node's half-angle table entry and mu_j once per
+// column and adds its terms to the sums of all the workgroup's rows,
+//   S_k = sum_j mu_j cot((z_k - z_j)/2)      Si_k = sum_j mu_j cot((z_k - conj z_j)/2 + i h),
+// the sums are reduced over the workgroup, and one thread per row finishes
 //   w_k = V2_k mu'_k + V1_kk mu_k + i (-1/4pi) S_k [+ i (1/4pi) Si_k]
 //   V2_k = i / (2 pi Z'_k),  V1_kk = i (-1/4pi) Z''_k / Z'_k^2 [+ i (1/4pi) cot(i (y_k + h))] +- 1/(2 Z'_k)
 //   q_k = conj(w_k),  dphi_k = Bernoulli(q_k, y_k),  then the RK4 stage of node k.
@@ -17,52 +19,6 @@
 #include "sfh_physics.hpp"
 
 namespace sfh {
-
-// Lower (q1) and upper (q2) fluid velocity at node k of one surface; every lane of the wave
-// takes part and every lane returns the result.
-__device__ inline void node_velocity(const SurfDesc& d, const NodeTrig* __restrict__ tr,
-                                     const double2* __restrict__ z, const double* __restrict__ m,
-                                     const double2* __restrict__ Zp, const double2* __restrict__ Zpp,
-                                     const double2* __restrict__ mup, const Phys& ph, int k, int lane,
-                                     double2& q1, double2& q2) {
-    const int n = d.n;
-    const NodeTrig tk = tr[k];
-    const double h = ph.depth, eh = exp(h), ehi = exp(-h);
-    double sx = 0.0, sy = 0.0, ix = 0.0, iy = 0.0;
-    for (int j0 = lane; j0 < n; j0 += 4 * 64) {
-        NodeTrig tj[4];
-        double mj[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const int j = j0 + u * 64;
-            if (j < n) {
-                tj[u] = tr[j];
-                mj[u] = m[j];
-            }
-        }
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const int j = j0 + u * 64;
-            if (j >= n || j == k) continue;
-            const double2 ct = cot_pair(tk, tj[u]);
-            sx += mj[u] * ct.x;
-            sy += mj[u] * ct.y;
-            if (ph.image) {
-                const double2 cim = cot_image(tk, tj[u], eh, ehi);
-                ix += mj[u] * cim.x;
-                iy += mj[u] * cim.y;
-            }
-        }
-    }
-    sx = wave_sum(sx);
-    sy = wave_sum(sy);
-    if (ph.image) {
-        ix = wave_sum(ix);
-        iy = wave_sum(iy);
-    }
-    const int g = d.voff + k;
-    velocity_from_sums(ph, Zp[g], Zpp[g], mup[g], m[k], z[k].y, sx, sy, ix, iy, q1, q2);
-}
 
 __global__ __launch_bounds__(256) void velocity_stage_kernel(const SurfDesc* __restrict__ sd,
                                                              const NodeTrig* __restrict__ trig,
@@ -72,30 +28,103 @@ __global__ __launch_bounds__(256) void velocity_stage_kernel(const SurfDesc* __r
                                                              const double* __restrict__ mu,
                                                              const double2* __restrict__ mup,
                                                              Phys ph, StageCtl ctl) {
+    constexpr int R = kPairRows;
     const SurfDesc d = sd[blockIdx.y];
     const int n = d.n;
-    const int k = blockIdx.x * 4 + (threadIdx.x >> 6);
-    const int lane = threadIdx.x & 63;
-    if (k >= n) return;
+    const int k0 = blockIdx.x * R;
+    if (k0 >= n) return;
+    const int t = threadIdx.x;
     const double2* z = Z + d.voff;
     const double* m = mu + d.voff;
     const NodeTrig* tr = trig + d.voff;
-    double2 q1, q2;
-    node_velocity(d, tr, z, m, Zp, Zpp, mup, ph, k, lane, q1, q2);
-    // createM.cuh:104 reads V1[1].real() (lower velocity of node 1) in the rho * q1.q2 term;
-    // with rho != 0 node 1 is recomputed here (each surface uses its own node 1).
-    double2 q1q = q1;
-    if (ph.kind == 0 && ph.rho_quirk && ph.rho != 0.0 && n > 1 && k != 1) {
-        double2 u1, u2;
-        node_velocity(d, tr, z, m, Zp, Zpp, mup, ph, 1, lane, u1, u2);
-        q1q = u1;
+    // createM.cuh:104 reads V1[1].real() (lower velocity of node 1) in the rho * q1.q2 term; with
+    // rho != 0 every workgroup also sums node 1 (entry R of the row arrays; each surface its own)
+    const bool quirk = ph.kind == 0 && ph.rho_quirk && ph.rho != 0.0 && n > 1;
+    const double h = ph.depth, eh = exp(h), ehi = exp(-h);
+    NodeTrig tk[R + 1];
+#pragma unroll
+    for (int r = 0; r < R; ++r) tk[r] = tr[min(k0 + r, n - 1)];
+    tk[R] = tr[min(1, n - 1)];
+    double sx[R + 1], sy[R + 1], ix[R], iy[R];
+#pragma unroll
+    for (int r = 0; r <= R; ++r) sx[r] = sy[r] = 0.0;
+#pragma unroll
+    for (int r = 0; r < R; ++r) ix[r] = iy[r] = 0.0;
+    const int shift = (int)(((long long)blockIdx.x * 64) % n);
+    for (int j0 = 0; j0 < n; j0 += 8 * 256) {
+        NodeTrig tj[8];
+        double mj[8];
+        int jc[8];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {   // unconditional (clamped) loads: no branch between them
+            int j = min(j0 + t + 256 * e, n - 1) + shift;
+            if (j >= n) j -= n;
+            jc[e] = j;
+            tj[e] = tr[j];
+            mj[e] = m[j];
+        }
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            if (j0 + t + 256 * e >= n) break;
+            const int j = jc[e];
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                if (k0 + r >= n) break;
+                if (j == k0 + r) continue;
+                const double2 ct = cot_pair(tk[r], tj[e]);
+                sx[r] += mj[e] * ct.x;
+                sy[r] += mj[e] * ct.y;
+                if (ph.image) {
+                    const double2 cim = cot_image(tk[r], tj[e], eh, ehi);
+                    ix[r] += mj[e] * cim.x;
+                    iy[r] += mj[e] * cim.y;
+                }
+            }
+            if (quirk && j != 1) {
+                const double2 ct = cot_pair(tk[R], tj[e]);
+                sx[R] += mj[e] * ct.x;
+                sy[R] += mj[e] * ct.y;
+            }
+        }
     }
-    if (lane != 0) return;
-    const int g = d.voff + k;
+    __shared__ double red[2 * R * 256];
+    __shared__ double so[2 * R], io[2 * R], qo[2];
+    {
+        double v[2 * R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            v[r] = sx[r];
+            v[R + r] = sy[r];
+        }
+        rows_reduce<2 * R>(v, red, so);
+    }
+    if (ph.image) {
+        double v[2 * R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            v[r] = ix[r];
+            v[R + r] = iy[r];
+        }
+        rows_reduce<2 * R>(v, red, io);
+    }
+    if (quirk) {
+        const double v[2] = {sx[R], sy[R]};
+        rows_reduce<2>(v, red, qo);
+    }
+    if (t >= R || k0 + t >= n) return;
+    const int k = k0 + t, g = d.voff + k;
+    double2 q1, q2;
+    velocity_from_sums(ph, Zp[g], Zpp[g], mup[g], m[k], z[k].y, so[t], so[R + t], ph.image ? io[t] : 0.0,
+                       ph.image ? io[R + t] : 0.0, q1, q2);
+    double2 q1q = q1;
+    if (quirk && k != 1) {
+        const int g1 = d.voff + 1;
+        double2 u2;
+        velocity_from_sums(ph, Zp[g1], Zpp[g1], mup[g1], m[1], z[1].y, qo[0], qo[1], 0.0, 0.0, q1q, u2);
+    }
     if (ctl.qup) ctl.qup[g] = q2;
     if (!ph.need_upper) q2 = make_double2(0.0, 0.0);
     const double dphi = bernoulli(ph, z[k].y, q1, q2, q1q, Zp[g], Zpp[g]);
-
     stage_update(ctl, g, q1, dphi);
 }
 
