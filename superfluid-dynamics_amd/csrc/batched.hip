@@ -90,7 +90,7 @@ __global__ __launch_bounds__(1024) void batched_stage_kernel(BatchArgs a, Phys p
     __syncthreads();   // the workgroup reads back only its own columns: a barrier suffices
 
     // ---------------------------------------------------------------- C: solve
-    // warm start: this stage's mu extrapolated over the previous steps (error O(dt^3) once three
+    // warm start: this stage's mu extrapolated over the previous steps (error O(dt^4) once four
     // are kept), else the previous solve's mu
     double x = own ? a.mu[off + t] : 0.0;
     double* h0 = nullptr;

@@ -20,9 +20,10 @@
 // the final step applies the correction x += V y, so a solve costs the Arnoldi steps only.  Once the residual estimate reaches
 // tol * |b| the remaining launches of the fixed-length sequence return at once.
 //
-// Initial guess of an RK stage: mu of the same stage extrapolated over the three previous steps,
-// 3 (mu_s(n-1) - mu_s(n-2)) + mu_s(n-3) (its error is third order in dt, against first order for
-// the previous stage's mu), which saves Krylov steps on every solve of a time integration.
+// Initial guess of an RK stage: mu of the same stage extrapolated over the previous steps by the
+// polynomial through them (extrapolate_history; with four steps kept its error is fourth order in
+// dt, against first order for the previous stage's mu), which saves Krylov steps on every solve of
+// a time integration.
 #include "sfh_gmres.hpp"
 
 namespace sfh {
@@ -301,13 +302,13 @@ __global__ __launch_bounds__(256) void gm_iter_kernel(GmArgs a, const SurfDesc* 
     double* h0 = history(a, sd);
     const long long T = a.total;
     for (int base = 0; base < n; base += 8 * 256) {   // 8 entries per thread, every load in flight
-        double xs[8], p0[8], p1[8];
+        double xs[8], ph[kHistDepth - 1][8];   // history levels 0 .. depth-2 move down one level
 #pragma unroll
         for (int e = 0; e < 8; ++e) {
             const int c = base + t + 256 * e;
             xs[e] = c < n ? v.x[c] : 0.0;
-            p0[e] = (h0 && c < n) ? h0[c] : 0.0;
-            p1[e] = (h0 && c < n) ? h0[T + c] : 0.0;
+#pragma unroll
+            for (int l = 0; l < kHistDepth - 1; ++l) ph[l][e] = (h0 && c < n) ? h0[l * T + c] : 0.0;
         }
         for (int i0 = 0; i0 < k; i0 += 4) {
             double vv[4][8];
@@ -331,8 +332,8 @@ __global__ __launch_bounds__(256) void gm_iter_kernel(GmArgs a, const SurfDesc* 
             if (c >= n) continue;
             v.x[c] = xs[e];
             if (h0) {
-                h0[2 * T + c] = p1[e];
-                h0[T + c] = p0[e];
+#pragma unroll
+                for (int l = kHistDepth - 1; l > 0; --l) h0[l * T + c] = ph[l - 1][e];
                 h0[c] = xs[e];
             }
         }

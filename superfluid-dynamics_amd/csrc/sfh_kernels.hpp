@@ -26,16 +26,17 @@ struct StageCtl {
 // Warm start of a solve by extrapolation over time steps: each RK stage keeps its solutions of the
 // last kHistDepth steps (newest first, levels stride apart) and the next initial guess is the
 // polynomial through the hv of them that exist, evaluated one step ahead:
-//   hv = 1: x0 = u0;  hv = 2: 2 u0 - u1;  hv = 3: 3 (u0 - u1) + u2   (error O(dt^hv)).
-constexpr int kHistDepth = 3;
+//   hv = 1: u0;  2: 2 u0 - u1;  3: 3 (u0 - u1) + u2;  4: 4 (u0 + u2) - 6 u1 - u3   (error O(dt^hv)).
+constexpr int kHistDepth = 4;
 __device__ inline double extrapolate_history(const double* h, long long stride, int hv) {
-    if (hv >= 3) return 3.0 * (h[0] - h[stride]) + h[2 * stride];
+    if (hv >= 4) return 4.0 * (h[0] + h[2 * stride]) - 6.0 * h[stride] - h[3 * stride];
+    if (hv == 3) return 3.0 * (h[0] - h[stride]) + h[2 * stride];
     if (hv == 2) return 2.0 * h[0] - h[stride];
     return h[0];
 }
 __device__ inline void push_history(double* h, long long stride, double x) {
-    h[2 * stride] = h[stride];
-    h[stride] = h[0];
+#pragma unroll
+    for (int l = kHistDepth - 1; l > 0; --l) h[l * stride] = h[(l - 1) * stride];
     h[0] = x;
 }
 

@@ -155,7 +155,7 @@ __global__ __launch_bounds__(256) void zphi_circ_kernel(const SurfDesc* __restri
         const int k = k0 + t;
         if (trig) trig[off + k] = node_trig(Z[off + k]);
         // GMRES initial guess of this RK stage: its mu over the previous steps extrapolated
-        // (error O(dt^3) once three are kept), else the last solve's mu (kept in place)
+        // (error O(dt^4) once four are kept), else the last solve's mu (kept in place)
         if (gm_on && gm.stage > 0) {
             const int hv = gm.hvalid[gm.stage - 1];
             if (hv > 0) {
