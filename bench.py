@@ -188,7 +188,7 @@ def main():
     ap.add_argument("--no-film", action="store_true", help="skip the N=4096 helium film leg")
     ap.add_argument("--film-steps", type=int, default=50)
     ap.add_argument("--solver", choices=("lu", "gmres"), default="gmres")
-    ap.add_argument("--max-iter", type=int, default=6, help="GMRES iteration cap per solve")
+    ap.add_argument("--max-iter", type=int, default=32, help="GMRES iteration cap per solve")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run 5 device-to-device copies of MIB MiB (calibrates the HBM byte counters)")
     args = ap.parse_args()

@@ -298,7 +298,7 @@ GmArgs Solver::gm_args(int stage) const {
     a.ctl = gmctl_;
     a.part_stride = gm_part_stride_;
     a.small_stride = gm_small_stride_;
-    a.m = gm_m_;
+    a.m = seq_len_ > 0 ? seq_len_ : gm_m_;
     a.total = T_;
     a.tol = opt_.tol;
     a.hist = gmhist_;
@@ -316,7 +316,7 @@ void Solver::enqueue_gmres(int stage) {
     for (int v : n_) nsq += (double)v * v;
     const dim3 grid(gm_nwg_, 1, S_);
     const size_t first = timed_.size();
-    for (int j = 0; j < gm_m_; ++j) {
+    for (int j = 0; j < a.m; ++j) {
         // product with the stored matrix + dots + the next basis vector (j + 1 vectors)
         launch(
             "gm_iter", 2.0 * nsq + 4.0 * (j + 1) * T_, 8.0 * nsq + 8.0 * (j + 3) * T_,
@@ -472,7 +472,7 @@ void Solver::enqueue_step(double dt) {
     enqueue_rhs(Za_, Pa_, c);
 }
 
-void Solver::capture(double dt) {
+void Solver::capture(double dt, int len) {
     if (exec_) {
         SFH_HIP(hipGraphExecDestroy(exec_));
         exec_ = nullptr;
@@ -481,6 +481,7 @@ void Solver::capture(double dt) {
         SFH_HIP(hipGraphDestroy(graph_));
         graph_ = nullptr;
     }
+    seq_len_ = len;
     SFH_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
     try {
         enqueue_step(dt);
@@ -493,15 +494,65 @@ void Solver::capture(double dt) {
     SFH_HIP(hipStreamEndCapture(stream_, &graph_));
     SFH_HIP(hipGraphInstantiate(&exec_, graph_, nullptr, nullptr, 0));
     graph_dt_ = dt;
+    graph_len_ = len;
+}
+
+int Solver::step_len() {
+    if (opt_.solver != kSolveGMRES) return 0;
+    if (hist_steps_ < kHistDepth || trim_ == 0) return gm_m_;
+    return trim_;
+}
+
+void Solver::after_step(int len) {
+    if (opt_.solver != kSolveGMRES) return;
+    if (hist_steps_ < kHistDepth) {
+        if (++hist_steps_ < kHistDepth) return;
+        // the history is full: the last solve (stage 4, extrapolated from one step fewer than
+        // every later solve, so with a guess of one order less) bounds the launches the
+        // following solves need
+        sync();
+        trim_ = std::max(2, std::min(gm_m_, stats_.max_iterations));
+        unc_seen_ = stats_.unconverged;
+        return;
+    }
+    if (len < gm_m_) {   // a trimmed step: did any solve run out of launches?
+        sync();
+        if (stats_.unconverged > unc_seen_) trim_ = std::min(gm_m_, 2 * trim_);
+        unc_seen_ = stats_.unconverged;
+    }
 }
 
 void Solver::step(double dt, int nsteps) {
     if (nsteps < 0) throw ArgError("negative step count");
-    if (opt_.use_graph && !ktime_) {
-        if (!exec_ || graph_dt_ != dt) capture(dt);
-        for (int i = 0; i < nsteps; ++i) SFH_HIP(hipGraphLaunch(exec_, stream_));
-    } else {
-        for (int i = 0; i < nsteps; ++i) enqueue_step(dt);
+    int i = 0;
+    while (i < nsteps) {
+        const int len = step_len();
+        // steps that need no policy check between them go out as one batch: every step while the
+        // history fills is checked (it sets the trimmed length); a trimmed batch is checked after
+        // its first step only
+        const bool check = opt_.solver == kSolveGMRES && (hist_steps_ < kHistDepth || len < gm_m_);
+        const int batch = check ? 1 : nsteps - i;
+        if (opt_.use_graph && !ktime_) {
+            if (!exec_ || graph_dt_ != dt || graph_len_ != len) capture(dt, len);
+            for (int b = 0; b < batch; ++b) SFH_HIP(hipGraphLaunch(exec_, stream_));
+        } else {
+            seq_len_ = len;
+            for (int b = 0; b < batch; ++b) enqueue_step(dt);
+        }
+        i += batch;
+        if (check) {
+            after_step(len);
+            if (hist_steps_ >= kHistDepth && len < gm_m_ && i < nsteps) {   // rest of a trimmed batch
+                const int len2 = step_len();
+                if (opt_.use_graph && !ktime_) {
+                    if (!exec_ || graph_dt_ != dt || graph_len_ != len2) capture(dt, len2);
+                    for (; i < nsteps; ++i) SFH_HIP(hipGraphLaunch(exec_, stream_));
+                } else {
+                    seq_len_ = len2;
+                    for (; i < nsteps; ++i) enqueue_step(dt);
+                }
+            }
+        }
     }
     stats_.steps += nsteps;
     stats_.rhs_evals += 4LL * nsteps;
@@ -513,6 +564,7 @@ void Solver::set_state(const double* x, const double* y, const double* phi) {
     SFH_HIP(hipMemcpyAsync(Z0_, z.data(), T_ * sizeof(double2), hipMemcpyHostToDevice, stream_));
     SFH_HIP(hipMemcpyAsync(P0_, phi, T_ * sizeof(double), hipMemcpyHostToDevice, stream_));
     if (gmhvalid_) SFH_HIP(hipMemsetAsync(gmhvalid_, 0, 4 * sizeof(int), stream_));   // a new trajectory
+    hist_steps_ = 0;
     if (bhvalid_) SFH_HIP(hipMemsetAsync(bhvalid_, 0, 4 * S_ * sizeof(int), stream_));
     SFH_HIP(hipStreamSynchronize(stream_));
 }
@@ -530,6 +582,7 @@ void Solver::get_state(double* x, double* y, double* phi) {
 }
 
 void Solver::rhs(double* vx, double* vy, double* dphi, double* mu, double2* zp, double2* zpp, double2* qup) {
+    seq_len_ = gm_m_;   // a standalone solve: no extrapolated guess, the full launch sequence
     StageCtl c{};
     c.stage = 0;
     c.kZ[0] = kZ_[0];

@@ -106,7 +106,9 @@ class Solver {
     void enqueue_rhs(const double2* Z, const double* P, const StageCtl& ctl);
     void enqueue_solve(const double2* Z, int stage);
     void enqueue_step(double dt);
-    void capture(double dt);
+    void capture(double dt, int len);
+    int step_len();                 // GMRES launches per solve for the next step (trimming policy)
+    void after_step(int len);       // bookkeeping of the trimming policy after one step
     void check_status();
     template <class F>
     void launch(const char* tag, double flops, double bytes, F&& f, int iter = -1);
@@ -183,6 +185,18 @@ class Solver {
     hipGraph_t graph_ = nullptr;
     hipGraphExec_t exec_ = nullptr;
     double graph_dt_ = 0.0;
+    int graph_len_ = 0;
+    // GMRES launch sequence per solve.  The step graph holds a fixed number of Arnoldi launches per
+    // solve (launches after convergence return at once but still cost a dispatch each).  While the
+    // warm-start history fills (the first kHistDepth steps after set_state) every solve gets the
+    // full cap gm_m_; afterwards the sequence is trimmed to what the last of those solves needed
+    // (at least 2; later solves start from a higher-order extrapolation), and doubled (up to the
+    // cap) whenever a check finds a solve that ran out of launches.  seq_len_ is the length the
+    // kernels are being enqueued with.
+    int seq_len_ = 0;
+    int trim_ = 0;          // 0: not determined yet
+    int hist_steps_ = 0;    // steps since set_state (saturates at kHistDepth)
+    int unc_seen_ = 0;      // unconverged count at the last check
     Stats stats_;
 };
 

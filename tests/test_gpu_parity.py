@@ -286,3 +286,30 @@ def test_gmres_rk4_matches_oracle_and_warm_starts(oracle, csh):
     assert st["unconverged"] == 0 and st["max_iterations"] <= 12
     assert np.abs(Zg - Zo).max() < 1e-12
     assert np.abs(Pg - Po).max() < 1e-12
+
+
+@pytest.mark.parametrize("kw", [{}, dict(kind=1, depth=1.0)])
+def test_gmres_trimmed_sequence_matches_lu(csh, kw):
+    # After the warm-start history fills (4 steps) the step graph keeps only one launch more per
+    # solve than the last full-length solve needed.  Trajectories split over several step() calls
+    # (cold steps, trimmed batches, a re-captured graph) against the LU stepper on the same surface;
+    # both FP64, agreement 1e-11 after 40 steps, no solve runs out of launches.
+    n = 1024
+    Z0 = _surface(n, 91, amp=0.04)
+    P0 = 0.01 * np.cos(2 * np.pi * np.arange(n) / n)
+    pc = csh.Props(**kw)
+    with csh.Ensemble([n], pc, solver=csh.LU) as e:
+        e.set_state(Z0, P0)
+        e.step(0.002, 40)
+        Zl, Pl = e.get_state()
+    with csh.Ensemble([n], pc, solver=csh.GMRES, tol=1e-13, max_iter=32) as e:
+        e.set_state(Z0, P0)
+        for k in (3, 2, 7, 28):
+            e.step(0.002, k)
+        Zg, Pg = e.get_state()
+        st = e.stats()
+    print(f"trimmed gmres {kw}: last {st['last_iterations']} max {st['max_iterations']} "
+          f"unconverged {st['unconverged']} |dZ| {np.abs(Zg - Zl).max():.2e}", flush=True)
+    assert st["unconverged"] == 0
+    assert np.abs(Zg - Zl).max() < 1e-11
+    assert np.abs(Pg - Pl).max() < 1e-11
