@@ -54,7 +54,7 @@ def ensemble_sizes(count=512, seed=SEED):
     return [int(v) for v in rng.choice([128, 256, 512, 1024], size=count)]
 
 
-def cpu_baseline(Z0, P0, budget_s=12.0, max_steps=12):
+def cpu_baseline(Z0, P0, budget_s=25.0, max_steps=40):
     from oracle import oracle as O
     O.build()
     Z, P = Z0.copy(), P0.copy()
