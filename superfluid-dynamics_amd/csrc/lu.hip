@@ -87,20 +87,61 @@ __device__ inline void store_block(double* dst, const double* src, long long ld,
 // Factors D (kb x kb, no interchanges) into L\U and writes L^{-1} into Li and U^{-1} into Ui
 // (both zero outside the kb x kb corner).  Sets *bad if a multiplier exceeds 1 in magnitude.
 //
-// Register-resident: thread t owns row r = t / 4 and the 16 columns c = (t % 4) + 4 m of the
-// 64 x 64 block (rows from kb on are padded with identity rows).  Every elimination or
-// substitution step publishes one row through a double-buffered LDS row (one barrier per step);
-// the entry a[r][j] a row needs from its own row comes from the neighbouring lane by a shuffle.
-// The step loops are fully unrolled so every register index is a compile-time constant.
-__device__ inline double lane_col(const double (&a)[16], int col, int lane) {
-    // a[r][col] is slot col / 4 of the lane with part col % 4 in this row's group of 4 lanes
-    return __shfl(a[col >> 2], (lane & ~3) | (col & 3), 64);
+// Thread t owns row r = t / 4 of the 64 x 64 block (rows from kb on are padded with identity rows)
+// and, in its register slots, the columns part + 4 (m + q) of that row, part = t % 4, where q is
+// the group of four elimination steps in progress: after every group the finished column (slot
+// 0) goes to LDS and the slots shift down by one, so the pivot column of step j = 4 q + p is
+// always slot 0 of the row's lane with part p and reaches the row's other lanes by a quad
+// broadcast (DPP).  Each step publishes the pivot row through a double-buffered LDS row (one
+// barrier per step) and all 16 of a thread's pivot-row entries are read before any is used.  The
+// loops stay rolled (no dynamic register index anywhere), so the code is a few KB.
+//
+// The inverses then run one substitution step of each per barrier (L^{-1} top down, U^{-1}
+// bottom up), reading the factors from LDS.
+__device__ inline double quad_bcast(double v, int p) {
+    // value of lane 4 (lane / 4) + p, for a lane-uniform p in 0..3 (quad_perm DPP)
+    const int ctl = p | (p << 2) | (p << 4) | (p << 6);
+    int lo = __double2loint(v), hi = __double2hiint(v);
+    switch (ctl) {
+        case 0x00: lo = __builtin_amdgcn_mov_dpp(lo, 0x00, 0xf, 0xf, false); hi = __builtin_amdgcn_mov_dpp(hi, 0x00, 0xf, 0xf, false); break;
+        case 0x55: lo = __builtin_amdgcn_mov_dpp(lo, 0x55, 0xf, 0xf, false); hi = __builtin_amdgcn_mov_dpp(hi, 0x55, 0xf, 0xf, false); break;
+        case 0xaa: lo = __builtin_amdgcn_mov_dpp(lo, 0xaa, 0xf, 0xf, false); hi = __builtin_amdgcn_mov_dpp(hi, 0xaa, 0xf, 0xf, false); break;
+        default:   lo = __builtin_amdgcn_mov_dpp(lo, 0xff, 0xf, 0xf, false); hi = __builtin_amdgcn_mov_dpp(hi, 0xff, 0xf, 0xf, false); break;
+    }
+    return __hiloint2double(hi, lo);
+}
+
+// One elimination step j = 4 q + P (P a compile-time constant) on the shifted row slots.
+template <int P>
+__device__ inline void elim_step(double (&a)[16], double (*prow)[kNB], int q, int r, int part, int kb,
+                                 int& my_bad) {
+    const int j = 4 * q + P;
+    double* pr = prow[P & 1];
+    if (r == j) {
+#pragma unroll
+        for (int m = 0; m < 16; ++m)
+            if (m + q < 16) pr[part + 4 * (m + q)] = a[m];
+    }
+    __syncthreads();
+    const double arj = quad_bcast(a[0], P);   // a[r][j]
+    double pv[16];
+#pragma unroll
+    for (int m = 0; m < 16; ++m) pv[m] = (m + q < 16) ? pr[part + 4 * (m + q)] : 0.0;
+    const double piv = pr[j];
+    if (r > j) {
+        const double l = arj / piv;
+        if (r < kb && !(fabs(l) <= 1.0)) my_bad = 1;
+        // slot 0 holds column 4 q + part: updated when part > P, the multiplier when part == P
+        a[0] = part > P ? fma(-l, pv[0], a[0]) : (part == P ? l : a[0]);
+#pragma unroll
+        for (int m = 1; m < 16; ++m) a[m] = fma(-l, pv[m], a[m]);
+    }
 }
 
 __device__ void factor_invert_diag(double* D, double* Li, double* Ui, int kb, int* bad) {
     __shared__ double prow[2][kNB];
     __shared__ double qrow[2][kNB];
-    const int t = threadIdx.x, r = t >> 2, part = t & 3, lane = t & 63;
+    const int t = threadIdx.x, r = t >> 2, part = t & 3;
     double a[16];
 #pragma unroll
     for (int m = 0; m < 16; ++m) {
@@ -108,28 +149,20 @@ __device__ void factor_invert_diag(double* D, double* Li, double* Ui, int kb, in
         a[m] = r < kb ? D[r * LD + c] : (c == r ? 1.0 : 0.0);
     }
     int my_bad = 0;
-    // ---- elimination: a becomes L (strictly below the diagonal) \ U
+    __syncthreads();   // every thread has read D before the finished columns overwrite it
+    // ---- elimination: D becomes L (strictly below the diagonal) \ U, padded rows identity
+#pragma unroll 1
+    for (int q = 0; q < 16; ++q) {
+        elim_step<0>(a, prow, q, r, part, kb, my_bad);
+        elim_step<1>(a, prow, q, r, part, kb, my_bad);
+        elim_step<2>(a, prow, q, r, part, kb, my_bad);
+        if (q < 15) elim_step<3>(a, prow, q, r, part, kb, my_bad);   // step 63 has nothing below
+        D[r * LD + part + 4 * q] = a[0];   // column 4 q + part is final
 #pragma unroll
-    for (int j = 0; j < kNB - 1; ++j) {
-        double* pr = prow[j & 1];
-        if (r == j) {
-#pragma unroll
-            for (int m = 0; m < 16; ++m) pr[part + 4 * m] = a[m];
-        }
-        __syncthreads();
-        const double arj = lane_col(a, j, lane);
-        if (r > j) {
-            const double l = arj / pr[j];
-            if (r < kb && !(fabs(l) <= 1.0)) my_bad = 1;
-#pragma unroll
-            for (int m = 0; m < 16; ++m) {
-                const int c = part + 4 * m;
-                a[m] = (c > j) ? a[m] - l * pr[c] : (c == j ? l : a[m]);
-            }
-        }
+        for (int m = 0; m < 15; ++m) a[m] = a[m + 1];
     }
     if (my_bad) *bad = 1;
-    __syncthreads();   // prow is reused below
+    __syncthreads();
     // ---- inverses: L^{-1} top down and U^{-1} bottom up, one step of each per barrier
     double wl[16], wu[16];
 #pragma unroll
@@ -138,18 +171,17 @@ __device__ void factor_invert_diag(double* D, double* Li, double* Ui, int kb, in
         wl[m] = (c == r) ? 1.0 : 0.0;
         wu[m] = (c == r) ? 1.0 : 0.0;
     }
-#pragma unroll
+#pragma unroll 1
     for (int s = 0; s < kNB; ++s) {
         const int iL = s, iU = kNB - 1 - s;
         double* pl = prow[s & 1];
         double* pu = qrow[s & 1];
-        const double uii = lane_col(a, iU, lane);   // U[iU][iU] on the lanes of row iU
         if (r == iL) {
 #pragma unroll
             for (int m = 0; m < 16; ++m) pl[part + 4 * m] = wl[m];
         }
         if (r == iU) {
-            const double inv = 1.0 / uii;
+            const double inv = 1.0 / D[iU * LD + iU];
 #pragma unroll
             for (int m = 0; m < 16; ++m) {
                 wu[m] *= inv;
@@ -157,22 +189,26 @@ __device__ void factor_invert_diag(double* D, double* Li, double* Ui, int kb, in
             }
         }
         __syncthreads();
-        const double lri = lane_col(a, iL, lane);   // L[r][iL] (used for r > iL)
-        const double uri = lane_col(a, iU, lane);   // U[r][iU] (used for r < iU)
-        if (r > iL) {
+        const double lri = r > iL ? D[r * LD + iL] : 0.0;   // L[r][iL]
+        const double uri = r < iU ? D[r * LD + iU] : 0.0;   // U[r][iU]
+        double plv[16], puv[16];
 #pragma unroll
-            for (int m = 0; m < 16; ++m) wl[m] -= lri * pl[part + 4 * m];
+        for (int m = 0; m < 16; ++m) {
+            plv[m] = pl[part + 4 * m];
+            puv[m] = pu[part + 4 * m];
         }
-        if (r < iU) {
 #pragma unroll
-            for (int m = 0; m < 16; ++m) wu[m] -= uri * pu[part + 4 * m];
+        for (int m = 0; m < 16; ++m) {
+            wl[m] = fma(-lri, plv[m], wl[m]);
+            wu[m] = fma(-uri, puv[m], wu[m]);
         }
     }
+    __syncthreads();   // D is read by other rows until here
 #pragma unroll
     for (int m = 0; m < 16; ++m) {
         const int c = part + 4 * m;
         const bool in = r < kb && c < kb;
-        D[r * LD + c] = in ? a[m] : 0.0;
+        D[r * LD + c] = in ? D[r * LD + c] : 0.0;
         Li[r * LD + c] = in ? wl[m] : 0.0;
         Ui[r * LD + c] = in ? wu[m] : 0.0;
     }
