@@ -6,19 +6,25 @@
 // outlier sits far to the right), so GMRES converges in a handful of iterations, and the stepper
 // warm starts every solve from the previous stage's mu.
 //
-// ONE kernel per Arnoldi step.  Kernel j:
-//   1. loads v_j = sc[j] V[j] into LDS and multiplies its rows of M by it (one row per wave, the
-//      whole row's loads in flight), writing w_j;
-//   2. forms partial dots v_i . w_j (i <= j) of its rows;
-//   3. the last workgroup (release increments, one acquire) reduces the dots, forms the next
-//      basis vector u = w_j - sum_i h_{i,j} v_i over all N entries (classical Gram-Schmidt, one
-//      pass), its norm h_{j+1,j} = |u|, stores V[j+1] = u and sc[j+1] = 1 / |u|, applies the
-//      Givens rotations to column j and tests the residual estimate.
-// The initial residual r0 = b - M x0 is formed by the assembly kernel while it writes M
-// (assemble.hip) and normalised by step 0 (every workgroup reduces |r0| and |b| over its own
-// full copies, so no grid-wide reduction precedes the first product), and the last workgroup of
-// the final step applies the correction x += V y, so a solve costs the Arnoldi steps only.  Once the residual estimate reaches
-// tol * |b| the remaining launches of the fixed-length sequence return at once.
+// A solve is a fixed sequence of L launches of one kernel; launch i first finishes Arnoldi step
+// i - 1 and then, unless the solve is done, runs step i.  No workgroup waits for another: every
+// grid-wide result is read by the next launch.
+//   finish step i-1 (i >= 1), in every workgroup: sum the partial dots v_l . w_{i-1} of all
+//     workgroups (fixed order, so every workgroup gets the same h_{l,i-1}); rebuild
+//     u = w_{i-1} - sum_l h_{l,i-1} v_l over all N entries into LDS (classical Gram-Schmidt, one
+//     pass) and its norm h_{i,i-1} = |u|; rotate the Hessenberg column by the stored Givens
+//     rotations and form the new one.  Workgroup 0 stores the column, the rotation and the scale.
+//     Each workgroup stores its own rows of V[i] = u.
+//   converged (or the last launch): y from the rotated Hessenberg system; each workgroup applies
+//     x += V y to its own rows and pushes them into the warm-start history.
+//   step i: w_i = M v_i for the workgroup's rows (one row per wave, the row's loads issued at the
+//     kernel's start so they overlap the finishing work), and the partial dots v_l . w_i, l <= i.
+// Launch 0 takes v_0 = r0 / |r0|.  The assembly kernel forms r0 = b - M x0 while it writes M
+// (assemble.hip), and every workgroup of launch 0 reduces |r0| and |b| over its own full copies.
+// L launches therefore run up to L - 1 Arnoldi steps.  Once the residual estimate reaches
+// tol * |b|, the remaining launches return at once.  Partial dots and the rotated right-hand side
+// live in per-launch (even/odd or per-index) slots, because a slot must not be overwritten while
+// a slower workgroup of the same launch still reads it.
 //
 // Initial guess of an RK stage: mu of the same stage extrapolated over the previous steps by the
 // polynomial through them (extrapolate_history; with four steps kept its error is fourth order in
@@ -135,6 +141,23 @@ __device__ inline double rebuild_vector(const double* __restrict__ w, const doub
     return block_sum4(ss, red4);
 }
 
+// Rotates column jc of the Hessenberg matrix (entries 0..jc+1 in col, LDS) by the stored rotations
+// 0..jc-1 and forms rotation jc; returns it in (c, s).
+__device__ inline void givens_column(double* col, const double* cs, const double* sn, int jc, double& c,
+                                     double& s) {
+    for (int q = 0; q < jc; ++q) {
+        const double t0 = cs[q] * col[q] + sn[q] * col[q + 1];
+        const double t1 = -sn[q] * col[q] + cs[q] * col[q + 1];
+        col[q] = t0;
+        col[q + 1] = t1;
+    }
+    const double den = hypot(col[jc], col[jc + 1]);
+    c = den > 0.0 ? col[jc] / den : 1.0;
+    s = den > 0.0 ? col[jc + 1] / den : 0.0;
+    col[jc] = den;
+    col[jc + 1] = 0.0;
+}
+
 // This surface's history of solutions of the current RK stage (null for a plain evaluation), and
 // the stage's count of stored steps (one increment per solve, by surface 0).
 __device__ inline double* history(const GmArgs& a, const SurfDesc* sd) {
@@ -144,66 +167,39 @@ __device__ inline void count_history(const GmArgs& a) {
     a.hvalid[a.stage - 1] = min(a.hvalid[a.stage - 1] + 1, kHistDepth);
 }
 
-// Applies the stored rotations to column jc (entries 0..jc+1 in col), forms the new rotation and
-// updates g; returns |g[jc+1]|, the residual estimate.
-__device__ inline double givens_column(double* col, double* cs, double* sn, double* g, int jc) {
-    for (int q = 0; q < jc; ++q) {
-        const double t0 = cs[q] * col[q] + sn[q] * col[q + 1];
-        const double t1 = -sn[q] * col[q] + cs[q] * col[q + 1];
-        col[q] = t0;
-        col[q + 1] = t1;
-    }
-    const double den = hypot(col[jc], col[jc + 1]);
-    const double c = den > 0.0 ? col[jc] / den : 1.0;
-    const double s = den > 0.0 ? col[jc + 1] / den : 0.0;
-    cs[jc] = c;
-    sn[jc] = s;
-    col[jc] = den;
-    col[jc + 1] = 0.0;
-    g[jc + 1] = -s * g[jc];
-    g[jc] = c * g[jc];
-    return fabs(g[jc + 1]);
-}
-
-// Arnoldi step j (see the file comment).
-__global__ __launch_bounds__(256) void gm_iter_kernel(GmArgs a, const SurfDesc* sd, int j) {
-    // converged solves: one load and out (most launches of the fixed sequence end here)
-    if (reinterpret_cast<const GmCtl*>(a.ctl)[blockIdx.z].done) return;
+// Launch i of a solve's sequence (see the file comment).
+__global__ __launch_bounds__(256) void gm_iter_kernel(GmArgs a, const SurfDesc* sd, int i) {
+    GmCtl* ctl = reinterpret_cast<GmCtl*>(a.ctl) + blockIdx.z;
+    if (ctl->done) return;   // converged solves: one load and out
     const GmView v = gm_view(a, sd);
     if ((int)blockIdx.x >= v.nwg) return;
     __shared__ double vs[kGmMaxN];
     __shared__ double co[kMaxKrylov + 1];
-    __shared__ double red4[4];
+    __shared__ double col[kMaxKrylov + 2];
+    __shared__ double red8[8];
     __shared__ double wl[kRowsPerWG];
     __shared__ double sums[kMaxKrylov + 2];
+    __shared__ double ys[kMaxKrylov + 4];   // g[j], gc[j+1], c, s, then y
     const int t = threadIdx.x, lane = t & 63, wv = t >> 6, n = v.n, m = a.m;
     const int r0 = blockIdx.x * kRowsPerWG;
-    double* wj = (j & 1) ? v.w1 : v.w0;
     const int row = r0 + wv;
     const double* R = v.M + (long long)min(row, n - 1) * n;
+    const long long pslot = (long long)v.nwg * (m + 1);   // one partial-dot slot per launch parity
     RowChunk rc;
-    row_load(R, n, lane, rc);   // the matrix row does not depend on v_j: in flight first
-    // operands of the partial dots (v_i of this workgroup's rows, i < j), also independent of v_j
-    double vrow[kRowsPerWG];
-#pragma unroll
-    for (int r = 0; r < kRowsPerWG; ++r)
-        vrow[r] = (t < j && r0 + r < n) ? v.V[(long long)t * n + r0 + r] : 0.0;
-    const double sct = (t < j) ? v.sc[t] : 0.0;
-    // V[j] unscaled into LDS; step 0 also forms beta = |r0| and |b| (every workgroup, same order,
-    // same values)
-    double sq = 0.0, bq = 0.0;
-    for (int c = t; c < n; c += 256) {
-        const double x = v.V[(long long)j * n + c];
-        vs[c] = x;
-        sq += x * x;
-        if (j == 0) {
+    if (i < m - 1) row_load(R, n, lane, rc);   // independent of v_i: in flight first
+    int k = 0;        // Arnoldi steps completed
+    double sj = 0.0;  // scale of v_i = sj * vs
+    bool done = false;
+    if (i == 0) {
+        // v_0 = r0 / |r0|; beta = |r0| and |b| reduced by every workgroup over its own copies
+        double sq = 0.0, bq = 0.0;
+        for (int c = t; c < n; c += 256) {
+            const double x = v.V[c];
+            vs[c] = x;
+            sq += x * x;
             const double bc = v.b[c];
             bq += bc * bc;
         }
-    }
-    double beta = 0.0, bnorm = 0.0, sj;
-    if (j == 0) {
-        __shared__ double red8[8];
         sq = wave_sum(sq);
         bq = wave_sum(bq);
         if (lane == 0) {
@@ -211,134 +207,118 @@ __global__ __launch_bounds__(256) void gm_iter_kernel(GmArgs a, const SurfDesc* 
             red8[4 + wv] = bq;
         }
         __syncthreads();
-        beta = sqrt(red8[0] + red8[1] + red8[2] + red8[3]);
-        bnorm = sqrt(red8[4] + red8[5] + red8[6] + red8[7]);
-        if (beta <= a.tol * bnorm) {   // the initial guess already solves the system
-            if (blockIdx.x == 0 && t == 0) {
-                v.ctl->bnorm = bnorm;
-                v.ctl->beta = beta;
-                v.ctl->res = beta;
-                v.ctl->done = 1;
-            }
-            double* h0 = history(a, sd);   // x = x0 enters the history, this workgroup's rows
-            if (h0 && t < kRowsPerWG && r0 + t < n) push_history(h0 + r0 + t, a.total, v.x[r0 + t]);
-            if (h0 && blockIdx.x == 0 && blockIdx.z == 0 && t == 0) count_history(a);
-            return;
+        const double beta = sqrt(red8[0] + red8[1] + red8[2] + red8[3]);
+        const double bnorm = sqrt(red8[4] + red8[5] + red8[6] + red8[7]);
+        if (blockIdx.x == 0 && t == 0) {
+            ctl->bnorm = bnorm;
+            ctl->beta = beta;
+            ctl->res = beta;
+            v.gc[0] = beta;
+            v.sc[0] = beta > 0.0 ? 1.0 / beta : 0.0;
         }
-        sj = 1.0 / beta;
+        done = beta <= a.tol * bnorm || m < 2;
+        if (done && blockIdx.x == 0 && t == 0 && !(beta <= a.tol * bnorm)) ctl->unconverged += 1;
+        sj = beta > 0.0 ? 1.0 / beta : 0.0;
     } else {
-        sj = v.sc[j];
+        // ---- finish step i - 1
+        const int j = i - 1;
+        const double* part = v.part + (j & 1) * pslot;
+        double wpre[8];
+        const double* wprev = (j & 1) ? v.w1 : v.w0;
+        load_first_pass(wprev, n, wpre);          // in flight together with the partial dots
+        reduce_partials(part, v.nwg, j + 1, sums);
+        if (t <= j) co[t] = sums[t] * v.sc[t];
         __syncthreads();
+        const double hn = sqrt(rebuild_vector(wprev, v.V, co, j + 1, n, 0, vs, nullptr, red8, wpre));
+        if (t == 0) {
+            for (int q = 0; q <= j; ++q) col[q] = sums[q];
+            col[j + 1] = hn;
+            double c, s;
+            givens_column(col, v.cs, v.sn, j, c, s);
+            const double gold = v.gc[j];
+            ys[0] = c * gold;          // g[j]
+            ys[1] = -s * gold;         // gc[j + 1]
+            ys[2] = c;
+            ys[3] = s;
+        }
+        __syncthreads();
+        const double gj = ys[0], gnext = ys[1], res = fabs(gnext);
+        if (blockIdx.x == 0 && t == 0) {
+            double* Hc = v.H + (long long)j * (m + 1);
+            for (int q = 0; q <= j + 1; ++q) Hc[q] = col[q];
+            v.cs[j] = ys[2];
+            v.sn[j] = ys[3];
+            v.g[j] = gj;
+            v.gc[j + 1] = gnext;
+            v.sc[j + 1] = hn > 0.0 ? 1.0 / hn : 0.0;
+            ctl->res = res;
+            ctl->hnext = hn;
+        }
+        k = i;
+        const bool conv = res <= a.tol * ctl->bnorm || hn == 0.0;
+        done = conv || i >= m - 1;
+        if (done && !conv && blockIdx.x == 0 && t == 0) ctl->unconverged += 1;
+        if (!done) {
+            // V[i] = u: this workgroup's rows (every workgroup holds all of u in LDS)
+            if (t < kRowsPerWG && r0 + t < n) v.V[(long long)i * n + r0 + t] = vs[r0 + t];
+            sj = hn > 0.0 ? 1.0 / hn : 0.0;
+        } else if (t == 0) {
+            // y from R y = g over the k rotated columns (column k-1 is still in LDS), scaled by sc
+            for (int q = k - 1; q >= 0; --q) {
+                double acc = (q == k - 1) ? gj : v.g[q];
+                for (int l = q + 1; l < k; ++l) {
+                    const double hql = (l == k - 1) ? col[q] : v.H[(long long)l * (m + 1) + q];
+                    acc -= hql * ys[4 + l];
+                }
+                const double hqq = (q == k - 1) ? col[q] : v.H[(long long)q * (m + 1) + q];
+                ys[4 + q] = acc / hqq;
+            }
+            for (int q = 0; q < k; ++q) ys[4 + q] *= v.sc[q];
+        }
     }
-    // w_j = M v_j = sc_j (M V[j]) for this workgroup's rows
+    if (done) {
+        // x += V y on this workgroup's rows; the stage's history
+        __syncthreads();
+        double* h0 = history(a, sd);
+        if (t < kRowsPerWG && r0 + t < n) {
+            const int c = r0 + t;
+            double x = v.x[c];
+            for (int q = 0; q < k; ++q) x += ys[4 + q] * v.V[(long long)q * n + c];
+            v.x[c] = x;
+            if (h0) push_history(h0 + c, a.total, x);
+        }
+        if (blockIdx.x == 0 && t == 0) {
+            ctl->iters = k;
+            ctl->done = 1;
+            if (h0 && blockIdx.z == 0) count_history(a);
+        }
+        return;
+    }
+    // ---- step i: w_i = M v_i (v_i = sj * vs) for this workgroup's rows, partial dots v_l . w_i
+    double vrow[kRowsPerWG];
+#pragma unroll
+    for (int r = 0; r < kRowsPerWG; ++r)
+        vrow[r] = (t < i && r0 + r < n) ? v.V[(long long)t * n + r0 + r] : 0.0;
+    const double sct = (t < i) ? v.sc[t] : 0.0;
+    double* wi = (i & 1) ? v.w1 : v.w0;
     const double d = row < n ? sj * row_dot_rest(R, rc, vs, n, lane) : 0.0;
     if (lane == 0) {
         wl[wv] = row < n ? d : 0.0;
-        if (row < n) wj[row] = d;
+        if (row < n) wi[row] = d;
     }
     __syncthreads();
-    // partial dots v_i . w_j, i <= j
-    if (t <= j) {
+    if (t <= i) {
         double s = 0.0;
 #pragma unroll
         for (int r = 0; r < kRowsPerWG; ++r) {
             const int rw = r0 + r;
             if (rw < n) {
-                const double vi = (t == j) ? sj * vs[rw] : sct * vrow[r];
+                const double vi = (t == i) ? sj * vs[rw] : sct * vrow[r];
                 s += vi * wl[r];
             }
         }
-        v.part[t * v.nwg + blockIdx.x] = s;
+        v.part[(i & 1) * pslot + (long long)t * v.nwg + blockIdx.x] = s;
     }
-    if (!last_workgroup(v.ctl, v.nwg)) return;
-    double wpre[8];
-    load_first_pass(wj, n, wpre);   // in flight together with the partial sums
-    reduce_partials(v.part, v.nwg, j + 1, sums);
-    if (t <= j) co[t] = sums[t] * (t == j ? sj : v.sc[t]);
-    if (j == 0 && t == 0) {   // the solve's start: scale of v_0, the residual norm, |b|
-        v.sc[0] = sj;
-        v.g[0] = beta;
-        v.ctl->beta = beta;
-        v.ctl->bnorm = bnorm;
-    }
-    __syncthreads();
-    // u = w_j - sum_{i<=j} h_{i,j} v_i, |u| (every entry of this one workgroup, into LDS)
-    const double hn = sqrt(rebuild_vector(wj, v.V, co, j + 1, n, 0, vs, nullptr, red4, wpre));
-    __shared__ int fin;
-    if (t == 0) {
-        GmCtl* c = v.ctl;
-        double* col = v.H + (long long)j * (m + 1);
-        for (int q = 0; q <= j; ++q) col[q] = sums[q];
-        col[j + 1] = hn;
-        v.sc[j + 1] = hn > 0.0 ? 1.0 / hn : 0.0;
-        c->res = givens_column(col, v.cs, v.sn, v.g, j);
-        c->iters = j + 1;
-        c->hnext = hn;
-        if (c->res <= a.tol * c->bnorm || hn == 0.0) {
-            c->done = 1;
-        } else if (j + 1 >= m) {
-            c->done = 1;
-            c->unconverged += 1;
-        }
-        fin = c->done;
-    }
-    __syncthreads();
-    if (!fin) {   // V[j+1] = u for the next step
-        for (int c = t; c < n; c += 256) v.V[(long long)(j + 1) * n + c] = vs[c];
-        return;
-    }
-    // the solve ends here: y from the rotated Hessenberg system R y = g (one thread), then
-    // x += sum_i y_i v_i over all entries and the stage's history
-    const int k = j + 1;
-    if (t == 0) {
-        for (int i = k - 1; i >= 0; --i) {   // co holds y first, then y_i sc[i]
-            double acc = v.g[i];
-            for (int l = i + 1; l < k; ++l) acc -= v.H[(long long)l * (m + 1) + i] * co[l];
-            co[i] = acc / v.H[(long long)i * (m + 1) + i];
-        }
-        for (int i = 0; i < k; ++i) co[i] *= v.sc[i];
-    }
-    __syncthreads();
-    double* h0 = history(a, sd);
-    const long long T = a.total;
-    for (int base = 0; base < n; base += 8 * 256) {   // 8 entries per thread, every load in flight
-        double xs[8], ph[kHistDepth - 1][8];   // history levels 0 .. depth-2 move down one level
-#pragma unroll
-        for (int e = 0; e < 8; ++e) {
-            const int c = base + t + 256 * e;
-            xs[e] = c < n ? v.x[c] : 0.0;
-#pragma unroll
-            for (int l = 0; l < kHistDepth - 1; ++l) ph[l][e] = (h0 && c < n) ? h0[l * T + c] : 0.0;
-        }
-        for (int i0 = 0; i0 < k; i0 += 4) {
-            double vv[4][8];
-#pragma unroll
-            for (int q = 0; q < 4; ++q)
-#pragma unroll
-                for (int e = 0; e < 8; ++e) {
-                    const int c = base + t + 256 * e;
-                    vv[q][e] = (i0 + q < k && c < n) ? v.V[(long long)(i0 + q) * n + c] : 0.0;
-                }
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const double y = i0 + q < k ? co[i0 + q] : 0.0;
-#pragma unroll
-                for (int e = 0; e < 8; ++e) xs[e] += y * vv[q][e];
-            }
-        }
-#pragma unroll
-        for (int e = 0; e < 8; ++e) {
-            const int c = base + t + 256 * e;
-            if (c >= n) continue;
-            v.x[c] = xs[e];
-            if (h0) {
-#pragma unroll
-                for (int l = kHistDepth - 1; l > 0; --l) h0[l * T + c] = ph[l - 1][e];
-                h0[c] = xs[e];
-            }
-        }
-    }
-    if (h0 && blockIdx.z == 0 && t == 0) count_history(a);
 }
 
 }  // namespace sfh

@@ -1,6 +1,6 @@
-// sfh_gmres.hpp -- GMRES state and the workgroup-arrival reduction, shared by the GMRES kernels
-// and the kernels that do GMRES work on the side (the assembly kernel forms the initial residual
-// while it writes the matrix; the mu' kernel adds the Krylov correction before differentiating).
+// sfh_gmres.hpp -- GMRES state shared by the Arnoldi kernel and the kernels that do GMRES work on
+// the side (the assembly kernel forms the initial residual while it writes the matrix; the
+// derivative kernel re-arms the control block and writes the extrapolated initial guess).
 #pragma once
 #include <cstddef>
 
@@ -14,13 +14,12 @@ constexpr int kGmMaxN = 4096;      // LDS copy of the vector being multiplied
 
 struct GmCtl {
     int done;
-    int counter;       // top-level arrival counter
-    int iters;
-    int unconverged;
+    int spare0;
+    int iters;         // Arnoldi steps of the last solve
+    int unconverged;   // solves that ended at the launch limit (cumulative)
     double bnorm, beta, res;
     double hnext;
-    int group[16];     // group arrival counters
-    int pad[4];
+    int spare[20];
 };
 static_assert(sizeof(GmCtl) == kGmCtlInts * sizeof(int), "host reads control blocks with this stride");
 
@@ -32,12 +31,12 @@ struct GmView {
     double* w0;      // products of even steps
     double* w1;      // products of odd steps
     double* sc;      // m+1 scales
-    double* h1;      // unrotated column of the last step
+    double* gc;      // m+1: component i of the rotated right-hand side before rotation i
     double* cs;
     double* sn;
     double* g;
     double* H;       // column j at H[j*(m+1)]
-    double* part;    // (m+2) x nwg partial sums, basis index major
+    double* part;    // 2 slots (launch parity) of (m+1) x nwg partial sums, basis index major
     GmCtl* ctl;
     int n, nwg;
 };
@@ -55,8 +54,8 @@ __device__ inline GmView gm_view(const GmArgs& a, const SurfDesc* sd, int s) {
     v.w0 = a.w + d.voff;
     v.w1 = a.w + a.total + d.voff;
     v.sc = a.small + (long long)s * a.small_stride;
-    v.h1 = v.sc + (m + 1);
-    v.cs = v.h1 + (m + 1);
+    v.gc = v.sc + (m + 1);
+    v.cs = v.gc + (m + 1);
     v.sn = v.cs + m;
     v.g = v.sn + m;
     v.H = v.g + (m + 1);
@@ -66,38 +65,6 @@ __device__ inline GmView gm_view(const GmArgs& a, const SurfDesc* sd, int s) {
 }
 
 __device__ inline GmView gm_view(const GmArgs& a, const SurfDesc* sd) { return gm_view(a, sd, blockIdx.z); }
-
-// Returns true on the workgroup that finished last (the counters are then re-armed).
-// Arrival is two-level: workgroup w increments the counter of group w % 16 (release), the last
-// member of each group increments the top counter, so no single address takes more than
-// ceil(nwg / 16) atomics (one shared counter serialises all nwg of them).  The last group member
-// acquires before passing its group on, and only the very last workgroup acquires at the top.  On
-// gfx950 an agent-scope acquire invalidates the L2 of the executing XCD, so acquiring in every
-// workgroup would evict the data other workgroups are streaming.
-__device__ inline bool last_workgroup(GmCtl* ctl, int nwg) {
-    constexpr int kGroups = 16;
-    __shared__ int is_last;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        const int grp = blockIdx.x % kGroups;
-        const int ngroups = nwg < kGroups ? nwg : kGroups;
-        const int members = (nwg - grp + kGroups - 1) / kGroups;
-        is_last = 0;
-        const int prev = __hip_atomic_fetch_add(&ctl->group[grp], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        if (prev == members - 1) {
-            __hip_atomic_store(&ctl->group[grp], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-            const int top = __hip_atomic_fetch_add(&ctl->counter, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-            if (top == ngroups - 1) {
-                __hip_atomic_store(&ctl->counter, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-                is_last = 1;
-            }
-        }
-    }
-    __syncthreads();
-    return is_last != 0;
-}
 
 // Sums part[i * nwg + w] over w for i < count (count <= kMaxKrylov + 1) into out[i] (LDS).
 // Basis indices go in chunks of 12; within a chunk every load of a thread (12 indices x 2

@@ -199,7 +199,7 @@ Solver::Solver(const Phys& ph, const std::vector<int>& n, const Options& opt) : 
     if (opt_.solver == kSolveGMRES) {
         gm_m_ = std::max(1, std::min(opt_.max_iter, 32));   // kMaxKrylov in gmres.hip
         gm_nwg_ = (maxN_ + 3) / 4;   // kRowsPerWG in gmres.hip
-        gm_part_stride_ = (long long)gm_nwg_ * (gm_m_ + 2);
+        gm_part_stride_ = 2LL * gm_nwg_ * (gm_m_ + 1);
         gm_small_stride_ = 3LL * (gm_m_ + 1) + 2LL * gm_m_ + (long long)gm_m_ * (gm_m_ + 1);
         gmV_ = dalloc<double>((size_t)(gm_m_ + 1) * T_);
         gmw_ = dalloc<double>(2 * (size_t)T_);
@@ -317,20 +317,30 @@ void Solver::enqueue_gmres(int stage) {
     const dim3 grid(gm_nwg_, 1, S_);
     const size_t first = timed_.size();
     for (int j = 0; j < a.m; ++j) {
-        // product with the stored matrix + dots + the next basis vector (j + 1 vectors)
+        // finishing step j - 1 (the basis rebuilt from w_{j-1} and V[0..j-1]) + the product with
+        // the stored matrix + the partial dots
         launch(
             "gm_iter", 2.0 * nsq + 4.0 * (j + 1) * T_, 8.0 * nsq + 8.0 * (j + 3) * T_,
             [&] { hipLaunchKernelGGL(gm_iter_kernel, grid, dim3(256), 0, stream_, a, (const SurfDesc*)d_desc_, j); },
             j);
     }
-    if (ktime_) {   // keep only iterations that did work (the rest returned after convergence)
+    if (ktime_) {
+        // launches 0 .. iters-1 ran a product ("gm_iter"); launch iters only finished the last step
+        // and applied the correction ("gm_close": no product); later ones returned at once
         std::vector<int> ctl((size_t)kGmCtlInts * S_);
         SFH_HIP(hipStreamSynchronize(stream_));
         SFH_HIP(hipMemcpy(ctl.data(), gmctl_, ctl.size() * sizeof(int), hipMemcpyDeviceToHost));
         int iters = 0;
         for (int s = 0; s < S_; ++s) iters = std::max(iters, ctl[(size_t)kGmCtlInts * s + 2]);
-        for (size_t k = first; k < timed_.size(); ++k)
-            if (timed_[k].iter >= iters) timed_[k].active = false;
+        for (size_t k = first; k < timed_.size(); ++k) {
+            if (timed_[k].iter == iters) {
+                timed_[k].tag = "gm_close";
+                timed_[k].flops = 4.0 * (iters + 1) * T_;
+                timed_[k].bytes = 8.0 * (iters + 1) * T_;
+            } else if (timed_[k].iter > iters) {
+                timed_[k].active = false;
+            }
+        }
     }
 }
 
