@@ -85,7 +85,8 @@ __device__ inline void load_first_pass(const double* __restrict__ w, int n, doub
 }
 __device__ inline double rebuild_vector(const double* __restrict__ w, const double* __restrict__ V,
                                         const double* co, int j, int n, int r0, double* vs, double* Vj,
-                                        double* red4, const double* pre = nullptr) {
+                                        double* red4, const double* pre = nullptr,
+                                        const double* v0pre = nullptr) {
     const int t = threadIdx.x;
     double ss = 0.0;
     for (int base = 0; base < n; base += 8 * 256) {
@@ -108,7 +109,8 @@ __device__ inline double rebuild_vector(const double* __restrict__ w, const doub
 #pragma unroll
                 for (int e = 0; e < 8; ++e) {
                     const int c = base + t + 256 * e;
-                    vv[q][e] = c < n ? V[(long long)(i + q) * n + c] : 0.0;
+                    vv[q][e] = (v0pre && base == 0 && i + q == 0) ? v0pre[e]
+                                                                   : (c < n ? V[(long long)(i + q) * n + c] : 0.0);
                 }
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
@@ -122,7 +124,7 @@ __device__ inline double rebuild_vector(const double* __restrict__ w, const doub
 #pragma unroll
             for (int e = 0; e < 8; ++e) {
                 const int c = base + t + 256 * e;
-                vv[e] = c < n ? V[(long long)i * n + c] : 0.0;
+                vv[e] = (v0pre && base == 0 && i == 0) ? v0pre[e] : (c < n ? V[(long long)i * n + c] : 0.0);
             }
             const double ci = co[i];
 #pragma unroll
@@ -189,7 +191,23 @@ __global__ __launch_bounds__(256) void gm_iter_kernel(GmArgs a, const SurfDesc* 
     if (i < m - 1) row_load(R, n, lane, rc);   // independent of v_i: in flight first
     int k = 0;        // Arnoldi steps completed
     double sj = 0.0;  // scale of v_i = sj * vs
+    double sc_t = 0.0;
     bool done = false;
+    // this workgroup's own rows (threads t < kRowsPerWG): x, V[0] and the history levels that
+    // move down, for the correction at the end of a solve, loaded up front
+    double* h0 = history(a, sd);
+    const int oc = min(r0 + (t & (kRowsPerWG - 1)), n - 1);
+    double xo = 0.0, vo0 = 0.0, ho[kHistDepth - 1];
+#pragma unroll
+    for (int l = 0; l < kHistDepth - 1; ++l) ho[l] = 0.0;
+    if (t < kRowsPerWG) {
+        xo = v.x[oc];
+        vo0 = v.V[oc];
+        if (h0) {
+#pragma unroll
+            for (int l = 0; l < kHistDepth - 1; ++l) ho[l] = h0[(long long)l * a.total + oc];
+        }
+    }
     if (i == 0) {
         // v_0 = r0 / |r0|; beta = |r0| and |b| reduced by every workgroup over its own copies
         double sq = 0.0, bq = 0.0;
@@ -225,17 +243,22 @@ __global__ __launch_bounds__(256) void gm_iter_kernel(GmArgs a, const SurfDesc* 
         const double* part = v.part + (j & 1) * pslot;
         double wpre[8];
         const double* wprev = (j & 1) ? v.w1 : v.w0;
-        load_first_pass(wprev, n, wpre);          // in flight together with the partial dots
+        // loads that do not depend on the dot sums go out first, together with the partial dots
+        double v0pre[8];
+        load_first_pass(wprev, n, wpre);
+        load_first_pass(v.V, n, v0pre);
+        sc_t = t <= j ? v.sc[t] : 0.0;
+        const double gold = v.gc[j];
+        const double bnorm = ctl->bnorm;
         reduce_partials(part, v.nwg, j + 1, sums);
-        if (t <= j) co[t] = sums[t] * v.sc[t];
+        if (t <= j) co[t] = sums[t] * sc_t;
         __syncthreads();
-        const double hn = sqrt(rebuild_vector(wprev, v.V, co, j + 1, n, 0, vs, nullptr, red8, wpre));
+        const double hn = sqrt(rebuild_vector(wprev, v.V, co, j + 1, n, 0, vs, nullptr, red8, wpre, v0pre));
         if (t == 0) {
             for (int q = 0; q <= j; ++q) col[q] = sums[q];
             col[j + 1] = hn;
             double c, s;
             givens_column(col, v.cs, v.sn, j, c, s);
-            const double gold = v.gc[j];
             ys[0] = c * gold;          // g[j]
             ys[1] = -s * gold;         // gc[j + 1]
             ys[2] = c;
@@ -255,7 +278,7 @@ __global__ __launch_bounds__(256) void gm_iter_kernel(GmArgs a, const SurfDesc* 
             ctl->hnext = hn;
         }
         k = i;
-        const bool conv = res <= a.tol * ctl->bnorm || hn == 0.0;
+        const bool conv = res <= a.tol * bnorm || hn == 0.0;
         done = conv || i >= m - 1;
         if (done && !conv && blockIdx.x == 0 && t == 0) ctl->unconverged += 1;
         if (!done) {
@@ -273,19 +296,26 @@ __global__ __launch_bounds__(256) void gm_iter_kernel(GmArgs a, const SurfDesc* 
                 const double hqq = (q == k - 1) ? col[q] : v.H[(long long)q * (m + 1) + q];
                 ys[4 + q] = acc / hqq;
             }
-            for (int q = 0; q < k; ++q) ys[4 + q] *= v.sc[q];
+        }
+        if (done) {   // y_q sc_q
+            __syncthreads();
+            if (t <= j) ys[4 + t] *= sc_t;
         }
     }
     if (done) {
         // x += V y on this workgroup's rows; the stage's history
         __syncthreads();
-        double* h0 = history(a, sd);
         if (t < kRowsPerWG && r0 + t < n) {
             const int c = r0 + t;
-            double x = v.x[c];
-            for (int q = 0; q < k; ++q) x += ys[4 + q] * v.V[(long long)q * n + c];
+            double x = xo;
+            if (k > 0) x += ys[4] * vo0;
+            for (int q = 1; q < k; ++q) x += ys[4 + q] * v.V[(long long)q * n + c];
             v.x[c] = x;
-            if (h0) push_history(h0 + c, a.total, x);
+            if (h0) {
+#pragma unroll
+                for (int l = kHistDepth - 1; l > 0; --l) h0[(long long)l * a.total + c] = ho[l - 1];
+                h0[c] = x;
+            }
         }
         if (blockIdx.x == 0 && t == 0) {
             ctl->iters = k;
