@@ -313,3 +313,27 @@ def test_gmres_trimmed_sequence_matches_lu(csh, kw):
     assert st["unconverged"] == 0
     assert np.abs(Zg - Zl).max() < 1e-11
     assert np.abs(Pg - Pl).max() < 1e-11
+
+
+def test_gmres_uneven_ensemble_matches_oracle(oracle, csh):
+    # several surfaces of different N in one GMRES solve sequence (one grid slice per surface,
+    # each with its own control block, partial dots and history), 12 steps: the trimmed sequence
+    # starts after step 4; every surface against the oracle's GEPP stepper
+    sizes = [256, 96, 1024, 12, 384]
+    Zs = [_surface(n, 300 + i, amp=0.05) for i, n in enumerate(sizes)]
+    Ps = [0.01 * np.cos(2 * np.pi * np.arange(n) / n + i) for i, n in enumerate(sizes)]
+    with csh.Ensemble(sizes, solver=csh.GMRES, tol=1e-13, max_iter=24) as e:
+        e.set_state(np.concatenate(Zs), np.concatenate(Ps))
+        e.step(0.004, 5)
+        e.step(0.004, 7)
+        Zg, Pg = e.get_state()
+        st = e.stats()
+    print(f"gmres ensemble: last {st['last_iterations']} max {st['max_iterations']} "
+          f"unconverged {st['unconverged']}", flush=True)
+    assert st["unconverged"] == 0
+    o = 0
+    for i, n in enumerate(sizes):
+        Zo, Po = oracle.rk4(oracle.Props(), Zs[i], Ps[i], 0.004, 12)
+        assert np.abs(Zg[o:o + n] - Zo).max() < 1e-11, n
+        assert np.abs(Pg[o:o + n] - Po).max() < 1e-11, n
+        o += n
