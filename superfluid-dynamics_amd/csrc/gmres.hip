@@ -174,7 +174,11 @@ __global__ __launch_bounds__(256) void gm_iter_kernel(GmArgs a, const SurfDesc* 
     GmCtl* ctl = reinterpret_cast<GmCtl*>(a.ctl) + blockIdx.z;
     if (ctl->done) return;   // converged solves: one load and out
     const GmView v = gm_view(a, sd);
-    if ((int)blockIdx.x >= v.nwg) return;
+    const int m = a.m;
+    // the last launch of the sequence runs no product: it takes kCloseRows rows per workgroup
+    // (fewer workgroups repeating the finishing work); the others kRowsPerWG
+    const int rpw = (i == m - 1) ? kCloseRows : kRowsPerWG;
+    if ((int)blockIdx.x * rpw >= v.n) return;
     __shared__ double vs[kGmMaxN];
     __shared__ double co[kMaxKrylov + 1];
     __shared__ double col[kMaxKrylov + 2];
@@ -182,8 +186,8 @@ __global__ __launch_bounds__(256) void gm_iter_kernel(GmArgs a, const SurfDesc* 
     __shared__ double wl[kRowsPerWG];
     __shared__ double sums[kMaxKrylov + 2];
     __shared__ double ys[kMaxKrylov + 4];   // g[j], gc[j+1], c, s, then y
-    const int t = threadIdx.x, lane = t & 63, wv = t >> 6, n = v.n, m = a.m;
-    const int r0 = blockIdx.x * kRowsPerWG;
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6, n = v.n;
+    const int r0 = blockIdx.x * rpw;
     const int row = r0 + wv;
     const double* R = v.M + (long long)min(row, n - 1) * n;
     const long long pslot = (long long)v.nwg * (m + 1);   // one partial-dot slot per launch parity
@@ -193,14 +197,14 @@ __global__ __launch_bounds__(256) void gm_iter_kernel(GmArgs a, const SurfDesc* 
     double sj = 0.0;  // scale of v_i = sj * vs
     double sc_t = 0.0;
     bool done = false;
-    // this workgroup's own rows (threads t < kRowsPerWG): x, V[0] and the history levels that
-    // move down, for the correction at the end of a solve, loaded up front
+    // this workgroup's own rows (threads t < rpw): x, V[0] and the history levels that move
+    // down, for the correction at the end of a solve, loaded up front
     double* h0 = history(a, sd);
-    const int oc = min(r0 + (t & (kRowsPerWG - 1)), n - 1);
+    const int oc = min(r0 + t, n - 1);
     double xo = 0.0, vo0 = 0.0, ho[kHistDepth - 1];
 #pragma unroll
     for (int l = 0; l < kHistDepth - 1; ++l) ho[l] = 0.0;
-    if (t < kRowsPerWG) {
+    if (t < rpw) {
         xo = v.x[oc];
         vo0 = v.V[oc];
         if (h0) {
@@ -305,7 +309,7 @@ __global__ __launch_bounds__(256) void gm_iter_kernel(GmArgs a, const SurfDesc* 
     if (done) {
         // x += V y on this workgroup's rows; the stage's history
         __syncthreads();
-        if (t < kRowsPerWG && r0 + t < n) {
+        if (t < rpw && r0 + t < n) {
             const int c = r0 + t;
             double x = xo;
             if (k > 0) x += ys[4] * vo0;

@@ -9,6 +9,7 @@
 namespace sfh {
 
 constexpr int kRowsPerWG = 4;      // one matrix row per wave
+constexpr int kCloseRows = 32;     // own rows per workgroup in the last launch of a sequence
 constexpr int kMaxKrylov = 32;     // basis vectors per solve (iteration cap)
 constexpr int kGmMaxN = 4096;      // LDS copy of the vector being multiplied
 

@@ -321,7 +321,10 @@ void Solver::enqueue_gmres(int stage) {
         // the stored matrix + the partial dots
         launch(
             "gm_iter", 2.0 * nsq + 4.0 * (j + 1) * T_, 8.0 * nsq + 8.0 * (j + 3) * T_,
-            [&] { hipLaunchKernelGGL(gm_iter_kernel, grid, dim3(256), 0, stream_, a, (const SurfDesc*)d_desc_, j); },
+            [&] {
+                const dim3 g = (j == a.m - 1) ? dim3((maxN_ + 31) / 32, 1, S_) : grid;   // kCloseRows
+                hipLaunchKernelGGL(gm_iter_kernel, g, dim3(256), 0, stream_, a, (const SurfDesc*)d_desc_, j);
+            },
             j);
     }
     if (ktime_) {
