@@ -108,10 +108,11 @@ __device__ inline double fast_rcp(double d) {
 // mixed-precision form.
 __device__ inline float fast_rcpf(float d) { return __builtin_amdgcn_rcpf(d); }
 
-// Per-node table for pairwise cotangents: sin(x/2), cos(x/2), exp(y/2), exp(-y/2).
+// Per-node table for pairwise cotangents: sin(x/2), cos(x/2), exp(y/2) / 2, exp(-y/2).
 // With it cot((z_k - z_j)/2) = (sa ca - i sh ch) / (sa^2 + sh^2) needs no transcendental call:
 //   sa = sin((x_k - x_j)/2) = s_k c_j - c_k s_j,  ca = c_k c_j + s_k s_j,
-//   e^{+-b} = e_k^{+-1} e_j^{-+1} with b = (y_k - y_j)/2, sh = (e^b - e^-b)/2, ch = (e^b + e^-b)/2.
+//   b = (y_k - y_j)/2:  e^b / 2 = e_k ei_j,  e^-b / 2 = e_j ei_k  (the factor 1/2 is in e, exactly),
+//   sh = sinh b = e_k ei_j - e_j ei_k,  ch = cosh b = e_k ei_j + e_j ei_k.
 // The subtractions lose relative accuracy only in proportion to eps / |z_k - z_j|, the same order
 // as the rounding already present in forming x_k - x_j and y_k - y_j.
 struct NodeTrig {
@@ -121,8 +122,9 @@ struct NodeTrig {
 __device__ inline NodeTrig node_trig(double2 z) {
     NodeTrig t;
     sincos(0.5 * z.x, &t.s, &t.c);
-    t.e = exp(0.5 * z.y);
-    t.ei = 1.0 / t.e;
+    const double E = exp(0.5 * z.y);
+    t.e = 0.5 * E;
+    t.ei = 1.0 / E;
     return t;
 }
 
@@ -131,17 +133,18 @@ __device__ inline double2 cot_pair(const NodeTrig& k, const NodeTrig& j) {
     const double sa = k.s * j.c - k.c * j.s;
     const double ca = k.c * j.c + k.s * j.s;
     const double ep = k.e * j.ei, em = k.ei * j.e;
-    const double sh = 0.5 * (ep - em), ch = 0.5 * (ep + em);
+    const double sh = ep - em, ch = ep + em;
     const double id = fast_rcp(sa * sa + sh * sh);
     return make_double2(sa * ca * id, -(sh * ch) * id);
 }
 
-// cot((z_k - conj z_j) / 2 + i h), eh = exp(h), ehi = exp(-h): b = (y_k + y_j)/2 + h > 0.
+// cot((z_k - conj z_j) / 2 + i h), eh = exp(h), ehi = exp(-h): b = (y_k + y_j)/2 + h > 0,
+// e^b / 2 = e_k e_j (2 eh), e^-b / 2 = ei_k ei_j (ehi / 2).
 __device__ inline double2 cot_image(const NodeTrig& k, const NodeTrig& j, double eh, double ehi) {
     const double sa = k.s * j.c - k.c * j.s;
     const double ca = k.c * j.c + k.s * j.s;
-    const double ep = k.e * j.e * eh, em = k.ei * j.ei * ehi;
-    const double sh = 0.5 * (ep - em), ch = 0.5 * (ep + em);
+    const double ep = k.e * j.e * (2.0 * eh), em = k.ei * j.ei * (0.5 * ehi);
+    const double sh = ep - em, ch = ep + em;
     const double id = fast_rcp(sa * sa + sh * sh);
     return make_double2(sa * ca * id, -(sh * ch) * id);
 }

@@ -171,14 +171,14 @@ __device__ inline float2 cot_pair_mixed(const NodeTrig& k, const NodeTrig& j) {
     const float sa = (float)(k.s * j.c - k.c * j.s);
     const float ca = (float)(k.c * j.c + k.s * j.s);
     const double ep = k.e * j.ei, em = k.ei * j.e;
-    const float sh = (float)(0.5 * (ep - em)), ch = (float)(0.5 * (ep + em));
+    const float sh = (float)(ep - em), ch = (float)(ep + em);
     const float id = fast_rcpf(sa * sa + sh * sh);
     return make_float2(sa * ca * id, -(sh * ch) * id);
 }
 __device__ inline float cot_image_mixed_im(const NodeTrig& k, const NodeTrig& j, double eh, double ehi) {
     const float sa = (float)(k.s * j.c - k.c * j.s);
-    const double ep = k.e * j.e * eh, em = k.ei * j.ei * ehi;
-    const float sh = (float)(0.5 * (ep - em)), ch = (float)(0.5 * (ep + em));
+    const double ep = k.e * j.e * (2.0 * eh), em = k.ei * j.ei * (0.5 * ehi);
+    const float sh = (float)(ep - em), ch = (float)(ep + em);
     return -(sh * ch) * fast_rcpf(sa * sa + sh * sh);
 }
 
