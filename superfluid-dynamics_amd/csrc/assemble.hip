@@ -41,14 +41,16 @@ __global__ __launch_bounds__(256) void assemble64_kernel(const SurfDesc* __restr
     const double h = ph.depth, eh = exp(h), ehi = exp(-h);
     // the rows' constants (the same in every thread)
     NodeTrig tk[R];
-    double2 zpk[R];
+    double czx[R], czy[R];   // c Z'_k
     double diag[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         const int k = min(k0 + r, n - 1);
         tk[r] = tr[k];
-        zpk[r] = Zp[d.voff + k];
-        const double2 q = cdiv(Zpp[d.voff + k], zpk[r]);
+        const double2 zpk = Zp[d.voff + k];
+        czx[r] = c * zpk.x;
+        czy[r] = c * zpk.y;
+        const double2 q = cdiv(Zpp[d.voff + k], zpk);
         diag[r] = (water ? 0.5 * (1 + ph.rho) : 0.5) + c * q.y;
         if (ph.image) diag[r] -= ci * cot64(0.0, Z[d.voff + k].y + h).y;
     }
@@ -85,9 +87,13 @@ __global__ __launch_bounds__(256) void assemble64_kernel(const SurfDesc* __restr
                 if (j == k) {
                     v = diag[r];
                 } else {
-                    const double2 ct = cot_pair(tk[r], tj[e]);
-                    v = c * (zpk[r].x * ct.y + zpk[r].y * ct.x);
-                    if (ph.image) v -= ci * cot_image(tk[r], tj[e], eh, ehi).y;
+                    // c Im(Z'_k cot) = id (c y'_k p - c x'_k q)
+                    const CotParts cp = cot_parts(tk[r], tj[e]);
+                    v = cp.id * (czy[r] * cp.p - czx[r] * cp.q);
+                    if (ph.image) {
+                        const CotParts ci2 = cot_image_parts(tk[r], tj[e], eh, ehi);
+                        v += ci * (ci2.q * ci2.id);
+                    }
                 }
                 Mk[(long long)r * n + j] = v;
                 acc[r] += v * xj[e];

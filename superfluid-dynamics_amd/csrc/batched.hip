@@ -57,31 +57,35 @@ __global__ __launch_bounds__(1024) void batched_stage_kernel(BatchArgs a, Phys p
         const double2 zq = L.zp[k];
         if (a.mixed) {
             float* col = reinterpret_cast<float*>(a.M) + d.moff + k;
-            const float cf = (float)pc.c, cif = (float)pc.ci;
-            const float zx = (float)zq.x, zy = (float)zq.y;
+            const float cif = (float)pc.ci;
+            const float czx = (float)(pc.c * zq.x), czy = (float)(pc.c * zq.y);
             for (int j = g; j < n; j += G) {
                 float v;
                 if (j == k) {
                     v = (float)L.diag[k];
                 } else {
                     const NodeTrig tj = L.tr[j];
-                    const float2 ct = cot_pair_mixed(tk, tj);
-                    v = cf * (zx * ct.y + zy * ct.x);
+                    const float3 cp = cot_parts_mixed(tk, tj);   // c Im(Z'_k cot) = id (c y' p - c x' q)
+                    v = cp.z * (czy * cp.x - czx * cp.y);
                     if (pc.image) v -= cif * cot_image_mixed_im(tk, tj, pc.eh, pc.ehi);
                 }
                 col[(size_t)j * n] = v;
             }
         } else {
             double* col = reinterpret_cast<double*>(a.M) + d.moff + k;
+            const double czx = pc.c * zq.x, czy = pc.c * zq.y;
             for (int j = g; j < n; j += G) {
                 double v;
                 if (j == k) {
                     v = L.diag[k];
                 } else {
                     const NodeTrig tj = L.tr[j];
-                    const double2 ct = cot_pair(tk, tj);
-                    v = pc.c * (zq.x * ct.y + zq.y * ct.x);
-                    if (pc.image) v -= pc.ci * cot_image(tk, tj, pc.eh, pc.ehi).y;
+                    const CotParts cp = cot_parts(tk, tj);
+                    v = cp.id * (czy * cp.p - czx * cp.q);
+                    if (pc.image) {
+                        const CotParts cq = cot_image_parts(tk, tj, pc.eh, pc.ehi);
+                        v += pc.ci * (cq.q * cq.id);
+                    }
                 }
                 col[(size_t)j * n] = v;
             }
@@ -161,13 +165,15 @@ __global__ __launch_bounds__(1024) void batched_stage_kernel(BatchArgs a, Phys p
             if (j == k) continue;
             const NodeTrig tj = L.tr[j];
             const double mj = L.vs[j];
-            const double2 ct = cot_pair(tk, tj);
-            sx += mj * ct.x;
-            sy += mj * ct.y;
+            const CotParts cp = cot_parts(tk, tj);
+            const double w = mj * cp.id;
+            sx += w * cp.p;
+            sy -= w * cp.q;
             if (pc.image) {
-                const double2 cim = cot_image(tk, tj, pc.eh, pc.ehi);
-                ix += mj * cim.x;
-                iy += mj * cim.y;
+                const CotParts cq = cot_image_parts(tk, tj, pc.eh, pc.ehi);
+                const double wi = mj * cq.id;
+                ix += wi * cq.p;
+                iy -= wi * cq.q;
             }
         }
         L.part[t] = sx;

@@ -128,25 +128,37 @@ __device__ inline NodeTrig node_trig(double2 z) {
     return t;
 }
 
+// A pair cotangent in parts: cot = (p id, -q id) with p = sin a cos a, q = sinh b cosh b and
+// id = 1 / (sin^2 a + sinh^2 b), so a consumer folds its own factors into one product with id.
+struct CotParts {
+    double p, q, id;
+};
+
 // cot((z_k - z_j) / 2)
-__device__ inline double2 cot_pair(const NodeTrig& k, const NodeTrig& j) {
+__device__ inline CotParts cot_parts(const NodeTrig& k, const NodeTrig& j) {
     const double sa = k.s * j.c - k.c * j.s;
     const double ca = k.c * j.c + k.s * j.s;
     const double ep = k.e * j.ei, em = k.ei * j.e;
     const double sh = ep - em, ch = ep + em;
-    const double id = fast_rcp(sa * sa + sh * sh);
-    return make_double2(sa * ca * id, -(sh * ch) * id);
+    return {sa * ca, sh * ch, fast_rcp(sa * sa + sh * sh)};
+}
+__device__ inline double2 cot_pair(const NodeTrig& k, const NodeTrig& j) {
+    const CotParts c = cot_parts(k, j);
+    return make_double2(c.p * c.id, -c.q * c.id);
 }
 
 // cot((z_k - conj z_j) / 2 + i h), eh = exp(h), ehi = exp(-h): b = (y_k + y_j)/2 + h > 0,
 // e^b / 2 = e_k e_j (2 eh), e^-b / 2 = ei_k ei_j (ehi / 2).
-__device__ inline double2 cot_image(const NodeTrig& k, const NodeTrig& j, double eh, double ehi) {
+__device__ inline CotParts cot_image_parts(const NodeTrig& k, const NodeTrig& j, double eh, double ehi) {
     const double sa = k.s * j.c - k.c * j.s;
     const double ca = k.c * j.c + k.s * j.s;
     const double ep = k.e * j.e * (2.0 * eh), em = k.ei * j.ei * (0.5 * ehi);
     const double sh = ep - em, ch = ep + em;
-    const double id = fast_rcp(sa * sa + sh * sh);
-    return make_double2(sa * ca * id, -(sh * ch) * id);
+    return {sa * ca, sh * ch, fast_rcp(sa * sa + sh * sh)};
+}
+__device__ inline double2 cot_image(const NodeTrig& k, const NodeTrig& j, double eh, double ehi) {
+    const CotParts c = cot_image_parts(k, j, eh, ehi);
+    return make_double2(c.p * c.id, -c.q * c.id);
 }
 
 // ------------------------------------------------------------------ wave reductions

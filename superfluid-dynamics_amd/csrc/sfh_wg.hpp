@@ -146,10 +146,14 @@ __device__ inline double mv_free(int n, int G, const PairConst& pc, BLds& L) {
             if (j == k) continue;
             const NodeTrig tj = L.tr[j];
             const double vj = L.vs[j];
-            const double2 ct = cot_pair(tk, tj);
-            sx += ct.x * vj;
-            sy += ct.y * vj;
-            if (pc.image) si += cot_image(tk, tj, pc.eh, pc.ehi).y * vj;
+            const CotParts cp = cot_parts(tk, tj);
+            const double w = vj * cp.id;
+            sx += w * cp.p;
+            sy -= w * cp.q;
+            if (pc.image) {
+                const CotParts cq = cot_image_parts(tk, tj, pc.eh, pc.ehi);
+                si -= vj * (cq.q * cq.id);
+            }
         }
         const double2 zk = L.zp[k];
         val = pc.c * (zk.x * sy + zk.y * sx) - pc.ci * si;
@@ -167,13 +171,13 @@ __device__ inline double mv_free(int n, int G, const PairConst& pc, BLds& L) {
 
 // cot((z_k - z_j)/2) with the differences of the half-angle table taken in FP64 (they carry the
 // cancellation next to the diagonal) and the rest in FP32.
-__device__ inline float2 cot_pair_mixed(const NodeTrig& k, const NodeTrig& j) {
+// Parts as in cot_parts: cot = (p id, -q id).
+__device__ inline float3 cot_parts_mixed(const NodeTrig& k, const NodeTrig& j) {
     const float sa = (float)(k.s * j.c - k.c * j.s);
     const float ca = (float)(k.c * j.c + k.s * j.s);
     const double ep = k.e * j.ei, em = k.ei * j.e;
     const float sh = (float)(ep - em), ch = (float)(ep + em);
-    const float id = fast_rcpf(sa * sa + sh * sh);
-    return make_float2(sa * ca * id, -(sh * ch) * id);
+    return make_float3(sa * ca, sh * ch, fast_rcpf(sa * sa + sh * sh));
 }
 __device__ inline float cot_image_mixed_im(const NodeTrig& k, const NodeTrig& j, double eh, double ehi) {
     const float sa = (float)(k.s * j.c - k.c * j.s);

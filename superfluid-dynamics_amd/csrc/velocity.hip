@@ -71,19 +71,22 @@ __global__ __launch_bounds__(256) void velocity_stage_kernel(const SurfDesc* __r
             for (int r = 0; r < R; ++r) {
                 if (k0 + r >= n) break;
                 if (j == k0 + r) continue;
-                const double2 ct = cot_pair(tk[r], tj[e]);
-                sx[r] += mj[e] * ct.x;
-                sy[r] += mj[e] * ct.y;
+                const CotParts cp = cot_parts(tk[r], tj[e]);
+                const double w = mj[e] * cp.id;
+                sx[r] += w * cp.p;
+                sy[r] -= w * cp.q;
                 if (ph.image) {
-                    const double2 cim = cot_image(tk[r], tj[e], eh, ehi);
-                    ix[r] += mj[e] * cim.x;
-                    iy[r] += mj[e] * cim.y;
+                    const CotParts cq = cot_image_parts(tk[r], tj[e], eh, ehi);
+                    const double wi = mj[e] * cq.id;
+                    ix[r] += wi * cq.p;
+                    iy[r] -= wi * cq.q;
                 }
             }
             if (quirk && j != 1) {
-                const double2 ct = cot_pair(tk[R], tj[e]);
-                sx[R] += mj[e] * ct.x;
-                sy[R] += mj[e] * ct.y;
+                const CotParts cp = cot_parts(tk[R], tj[e]);
+                const double w = mj[e] * cp.id;
+                sx[R] += w * cp.p;
+                sy[R] -= w * cp.q;
             }
         }
     }
