@@ -96,8 +96,7 @@ __device__ inline void store_block(double* dst, const double* src, long long ld,
 // barrier per step) and all 16 of a thread's pivot-row entries are read before any is used.  The
 // loops stay rolled (no dynamic register index anywhere), so the code is a few KB.
 //
-// The inverses then run one substitution step of each per barrier (L^{-1} top down, U^{-1}
-// bottom up), reading the factors from LDS.
+// The inverses then follow in blocks (invert_factors).
 __device__ inline double quad_bcast(double v, int p) {
     // value of lane 4 (lane / 4) + p, for a lane-uniform p in 0..3 (quad_perm DPP)
     const int ctl = p | (p << 2) | (p << 4) | (p << 6);
@@ -138,9 +137,115 @@ __device__ inline void elim_step(double (&a)[16], double (*prow)[kNB], int q, in
     }
 }
 
-__device__ void factor_invert_diag(double* D, double* Li, double* Ui, int kb, int* bad) {
+// One level of the blocked inversion: for every pair of H x H diagonal blocks, T = C A^{-1}
+// (lower) and E B^{-1} (upper), then the off-diagonal block -B^{-1} T (lower) and -A^{-1} T (upper).
+// Thread t owns the entry (t / H, t % H) of each of the pairs x 2 products; the sums run over the
+// whole block (the inverses are zero outside their triangles), fully unrolled so every LDS read
+// is independent of the accumulation.
+template <int H>
+__device__ inline void combine_level(const double* D, double* Li, double* Ui, double* T) {
+    constexpr int per = H * H, pairs = kNB / (2 * H), E = pairs * 2 * per / 256;
+    static_assert(per == 256 || per == 1024, "H is 16 or 32");
+    const int t = threadIdx.x;
+#pragma unroll
+    for (int q = 0; q < E; ++q) {
+        const int e = t + 256 * q, p = e / (2 * per), lu = (e / per) & 1, idx = e % per;
+        const int r = idx / H, cc = idx % H, o0 = 2 * H * p, o1 = o0 + H;
+        double a0 = 0.0, a1 = 0.0;
+        if (lu == 0) {
+#pragma unroll
+            for (int k = 0; k < H; k += 2) {
+                a0 += D[(o1 + r) * LD + o0 + k] * Li[(o0 + k) * LD + o0 + cc];
+                a1 += D[(o1 + r) * LD + o0 + k + 1] * Li[(o0 + k + 1) * LD + o0 + cc];
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < H; k += 2) {
+                a0 += D[(o0 + r) * LD + o1 + k] * Ui[(o1 + k) * LD + o1 + cc];
+                a1 += D[(o0 + r) * LD + o1 + k + 1] * Ui[(o1 + k + 1) * LD + o1 + cc];
+            }
+        }
+        T[e] = a0 + a1;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < E; ++q) {
+        const int e = t + 256 * q, p = e / (2 * per), lu = (e / per) & 1, idx = e % per;
+        const int r = idx / H, cc = idx % H, o0 = 2 * H * p, o1 = o0 + H;
+        const double* Tb = T + (e - idx);
+        double a0 = 0.0, a1 = 0.0;
+        if (lu == 0) {
+#pragma unroll
+            for (int k = 0; k < H; k += 2) {
+                a0 += Li[(o1 + r) * LD + o1 + k] * Tb[k * H + cc];
+                a1 += Li[(o1 + r) * LD + o1 + k + 1] * Tb[(k + 1) * H + cc];
+            }
+            Li[(o1 + r) * LD + o0 + cc] = -(a0 + a1);
+        } else {
+#pragma unroll
+            for (int k = 0; k < H; k += 2) {
+                a0 += Ui[(o0 + r) * LD + o0 + k] * Tb[k * H + cc];
+                a1 += Ui[(o0 + r) * LD + o0 + k + 1] * Tb[(k + 1) * H + cc];
+            }
+            Ui[(o0 + r) * LD + o1 + cc] = -(a0 + a1);
+        }
+    }
+    __syncthreads();
+}
+
+// Inverts the unit lower factor L and the upper factor U held together in D (L\U, 64 x 64) into
+// Li = L^{-1} and Ui = U^{-1}, blocked:
+//   the eight 16 x 16 diagonal blocks (four of L, four of U) by substitution, one lane per column,
+//   all at once;
+//   then the off-diagonal blocks level by level (block sizes h = 16, 32), from
+//     L = [A 0; C B]  ->  L^{-1} = [A^-1 0; -B^-1 (C A^-1)  B^-1],
+//     U = [A E; 0 B]  ->  U^{-1} = [A^-1  -A^-1 (E B^-1); 0 B^-1],
+//   the products through LDS (T: 2048 doubles of scratch).
+// Five barriers in place of one per row: the 64-step substitution it replaces spent ~0.3 us per
+// barrier-separated step.
+__device__ void invert_factors(const double* D, double* Li, double* Ui, double* T) {
+    const int t = threadIdx.x;
+    for (int e = t; e < kNB * LD; e += 256) {
+        Li[e] = 0.0;
+        Ui[e] = 0.0;
+    }
+    __syncthreads();
+    {
+        const int blk = t >> 5, c = t & 31;   // 8 groups of 32 threads, 16 lanes of each work
+        if (c < 16) {
+            double x[16];
+            if (blk < 4) {   // L block: x[i] = [i == c] - sum_{k < i} L[i][k] x[k]
+                const int o = 16 * blk;
+#pragma unroll
+                for (int i = 0; i < 16; ++i) {
+                    double acc = (i == c) ? 1.0 : 0.0;
+#pragma unroll
+                    for (int k = 0; k < i; ++k) acc -= D[(o + i) * LD + o + k] * x[k];
+                    x[i] = (i < c) ? 0.0 : acc;
+                }
+#pragma unroll
+                for (int i = 0; i < 16; ++i) Li[(o + i) * LD + o + c] = x[i];
+            } else {         // U block: x[i] = ([i == c] - sum_{k > i} U[i][k] x[k]) / U[i][i]
+                const int o = 16 * (blk - 4);
+#pragma unroll
+                for (int i = 15; i >= 0; --i) {
+                    double acc = (i == c) ? 1.0 : 0.0;
+#pragma unroll
+                    for (int k = i + 1; k < 16; ++k) acc -= D[(o + i) * LD + o + k] * x[k];
+                    x[i] = (i > c) ? 0.0 : acc / D[(o + i) * LD + o + i];
+                }
+#pragma unroll
+                for (int i = 0; i < 16; ++i) Ui[(o + i) * LD + o + c] = x[i];
+            }
+        }
+    }
+    __syncthreads();
+    combine_level<16>(D, Li, Ui, T);
+    combine_level<32>(D, Li, Ui, T);
+}
+
+__device__ void factor_invert_diag(double* D, double* Li, double* Ui, double* T, int kb, int* bad) {
     __shared__ double prow[2][kNB];
-    __shared__ double qrow[2][kNB];
     const int t = threadIdx.x, r = t >> 2, part = t & 3;
     double a[16];
 #pragma unroll
@@ -163,54 +268,16 @@ __device__ void factor_invert_diag(double* D, double* Li, double* Ui, int kb, in
     }
     if (my_bad) *bad = 1;
     __syncthreads();
-    // ---- inverses: L^{-1} top down and U^{-1} bottom up, one step of each per barrier
-    double wl[16], wu[16];
+    invert_factors(D, Li, Ui, T);
+    // zero outside the kb x kb corner (padded rows were identity rows)
 #pragma unroll
     for (int m = 0; m < 16; ++m) {
         const int c = part + 4 * m;
-        wl[m] = (c == r) ? 1.0 : 0.0;
-        wu[m] = (c == r) ? 1.0 : 0.0;
-    }
-#pragma unroll 1
-    for (int s = 0; s < kNB; ++s) {
-        const int iL = s, iU = kNB - 1 - s;
-        double* pl = prow[s & 1];
-        double* pu = qrow[s & 1];
-        if (r == iL) {
-#pragma unroll
-            for (int m = 0; m < 16; ++m) pl[part + 4 * m] = wl[m];
+        if (!(r < kb && c < kb)) {
+            D[r * LD + c] = 0.0;
+            Li[r * LD + c] = 0.0;
+            Ui[r * LD + c] = 0.0;
         }
-        if (r == iU) {
-            const double inv = 1.0 / D[iU * LD + iU];
-#pragma unroll
-            for (int m = 0; m < 16; ++m) {
-                wu[m] *= inv;
-                pu[part + 4 * m] = wu[m];
-            }
-        }
-        __syncthreads();
-        const double lri = r > iL ? D[r * LD + iL] : 0.0;   // L[r][iL]
-        const double uri = r < iU ? D[r * LD + iU] : 0.0;   // U[r][iU]
-        double plv[16], puv[16];
-#pragma unroll
-        for (int m = 0; m < 16; ++m) {
-            plv[m] = pl[part + 4 * m];
-            puv[m] = pu[part + 4 * m];
-        }
-#pragma unroll
-        for (int m = 0; m < 16; ++m) {
-            wl[m] = fma(-lri, plv[m], wl[m]);
-            wu[m] = fma(-uri, puv[m], wu[m]);
-        }
-    }
-    __syncthreads();   // D is read by other rows until here
-#pragma unroll
-    for (int m = 0; m < 16; ++m) {
-        const int c = part + 4 * m;
-        const bool in = r < kb && c < kb;
-        D[r * LD + c] = in ? D[r * LD + c] : 0.0;
-        Li[r * LD + c] = in ? wl[m] : 0.0;
-        Ui[r * LD + c] = in ? wu[m] : 0.0;
     }
     __syncthreads();
 }
@@ -253,7 +320,7 @@ __global__ __launch_bounds__(256) void lu_panel_kernel(LuArgs a, int k0) {
     load_block(D, v.A + (long long)k0 * n + k0, n, kb, kb);
     if (t < kNB) wv[t] = (t < kb) ? v.work[k0 + t] : 0.0;
     __syncthreads();
-    factor_invert_diag(D, Li, Ui, kb, &bad);
+    factor_invert_diag(D, Li, Ui, X, kb, &bad);   // X is free until the panel products
     // y_p = L11^{-1} b_p (4 threads per row, operands staged in registers)
     {
         const int r = t >> 2, part = t & 3;
