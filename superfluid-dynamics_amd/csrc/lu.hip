@@ -145,7 +145,7 @@ __device__ inline void elim_step(double (&a)[16], double (*prow)[kNB], int q, in
 template <int H>
 __device__ inline void combine_level(const double* D, double* Li, double* Ui, double* T) {
     constexpr int per = H * H, pairs = kNB / (2 * H), E = pairs * 2 * per / 256;
-    static_assert(per == 256 || per == 1024, "H is 16 or 32");
+    static_assert(per == 64 || per == 256 || per == 1024, "H is 8, 16 or 32");
     const int t = threadIdx.x;
 #pragma unroll
     for (int q = 0; q < E; ++q) {
@@ -195,13 +195,13 @@ __device__ inline void combine_level(const double* D, double* Li, double* Ui, do
 
 // Inverts the unit lower factor L and the upper factor U held together in D (L\U, 64 x 64) into
 // Li = L^{-1} and Ui = U^{-1}, blocked:
-//   the eight 16 x 16 diagonal blocks (four of L, four of U) by substitution, one lane per column,
-//   all at once;
-//   then the off-diagonal blocks level by level (block sizes h = 16, 32), from
+//   the sixteen 8 x 8 diagonal blocks (eight of L, eight of U) by substitution, one lane per
+//   column, all at once;
+//   then the off-diagonal blocks level by level (block sizes h = 8, 16, 32), from
 //     L = [A 0; C B]  ->  L^{-1} = [A^-1 0; -B^-1 (C A^-1)  B^-1],
 //     U = [A E; 0 B]  ->  U^{-1} = [A^-1  -A^-1 (E B^-1); 0 B^-1],
 //   the products through LDS (T: 2048 doubles of scratch).
-// Five barriers in place of one per row: the 64-step substitution it replaces spent ~0.3 us per
+// Seven barriers in place of one per row: the 64-step substitution it replaces spent ~0.3 us per
 // barrier-separated step.
 __device__ void invert_factors(const double* D, double* Li, double* Ui, double* T) {
     const int t = threadIdx.x;
@@ -211,35 +211,36 @@ __device__ void invert_factors(const double* D, double* Li, double* Ui, double* 
     }
     __syncthreads();
     {
-        const int blk = t >> 5, c = t & 31;   // 8 groups of 32 threads, 16 lanes of each work
-        if (c < 16) {
-            double x[16];
-            if (blk < 4) {   // L block: x[i] = [i == c] - sum_{k < i} L[i][k] x[k]
-                const int o = 16 * blk;
+        const int blk = t >> 4, c = t & 15;   // 16 groups of 16 threads, 8 lanes of each work
+        if (c < 8) {
+            double x[8];
+            if (blk < 8) {   // L block: x[i] = [i == c] - sum_{k < i} L[i][k] x[k]
+                const int o = 8 * blk;
 #pragma unroll
-                for (int i = 0; i < 16; ++i) {
+                for (int i = 0; i < 8; ++i) {
                     double acc = (i == c) ? 1.0 : 0.0;
 #pragma unroll
                     for (int k = 0; k < i; ++k) acc -= D[(o + i) * LD + o + k] * x[k];
                     x[i] = (i < c) ? 0.0 : acc;
                 }
 #pragma unroll
-                for (int i = 0; i < 16; ++i) Li[(o + i) * LD + o + c] = x[i];
+                for (int i = 0; i < 8; ++i) Li[(o + i) * LD + o + c] = x[i];
             } else {         // U block: x[i] = ([i == c] - sum_{k > i} U[i][k] x[k]) / U[i][i]
-                const int o = 16 * (blk - 4);
+                const int o = 8 * (blk - 8);
 #pragma unroll
-                for (int i = 15; i >= 0; --i) {
+                for (int i = 7; i >= 0; --i) {
                     double acc = (i == c) ? 1.0 : 0.0;
 #pragma unroll
-                    for (int k = i + 1; k < 16; ++k) acc -= D[(o + i) * LD + o + k] * x[k];
+                    for (int k = i + 1; k < 8; ++k) acc -= D[(o + i) * LD + o + k] * x[k];
                     x[i] = (i > c) ? 0.0 : acc / D[(o + i) * LD + o + i];
                 }
 #pragma unroll
-                for (int i = 0; i < 16; ++i) Ui[(o + i) * LD + o + c] = x[i];
+                for (int i = 0; i < 8; ++i) Ui[(o + i) * LD + o + c] = x[i];
             }
         }
     }
     __syncthreads();
+    combine_level<8>(D, Li, Ui, T);
     combine_level<16>(D, Li, Ui, T);
     combine_level<32>(D, Li, Ui, T);
 }
