@@ -92,9 +92,10 @@ __device__ inline void store_block(double* dst, const double* src, long long ld,
 // the group of four elimination steps in progress: after every group the finished column (slot
 // 0) goes to LDS and the slots shift down by one, so the pivot column of step j = 4 q + p is
 // always slot 0 of the row's lane with part p and reaches the row's other lanes by a quad
-// broadcast (DPP).  Each step publishes the pivot row through a double-buffered LDS row (one
-// barrier per step) and all 16 of a thread's pivot-row entries are read before any is used.  The
-// loops stay rolled (no dynamic register index anywhere), so the code is a few KB.
+// broadcast (DPP).  A group publishes its four pivot rows through a double-buffered LDS block (one
+// barrier per four steps; elim_group) and all of a thread's pivot-row entries are read before any
+// is used.  The group loop stays rolled (no dynamic register index anywhere), so the code is a
+// few KB.
 //
 // The inverses then follow in blocks (invert_factors).
 __device__ inline double quad_bcast(double v, int p) {
@@ -111,29 +112,56 @@ __device__ inline double quad_bcast(double v, int p) {
 }
 
 // One elimination step j = 4 q + P (P a compile-time constant) on the shifted row slots.
-template <int P>
-__device__ inline void elim_step(double (&a)[16], double (*prow)[kNB], int q, int r, int part, int kb,
-                                 int& my_bad) {
-    const int j = 4 * q + P;
-    double* pr = prow[P & 1];
-    if (r == j) {
+// One group of four elimination steps j = 4 q .. 4 q + 3, with one barrier: the four pivot rows
+// are published as they stand before the group, and every thread carries them through the
+// group's earlier steps itself (at its own 16 columns and on the 4 x 4 block of the pivot
+// columns), with the same operations the rows' owners apply.  Own-row step j then uses the
+// updated pivot row; the row's entry in column j comes from slot 0 of the lane with part j % 4.
+template <int LAST>
+__device__ inline void elim_group(double (&a)[16], double (*P4)[4][kNB], int q, int r, int part, int kb,
+                                  int& my_bad) {
+    double (*pr)[kNB] = P4[q & 1];
+    const int j0 = 4 * q;
+    if (r >= j0 && r < j0 + 4) {
 #pragma unroll
         for (int m = 0; m < 16; ++m)
-            if (m + q < 16) pr[part + 4 * (m + q)] = a[m];
+            if (m + q < 16) pr[r - j0][part + 4 * (m + q)] = a[m];
     }
     __syncthreads();
-    const double arj = quad_bcast(a[0], P);   // a[r][j]
-    double pv[16];
+    double pv[4][16], B[4][4];
 #pragma unroll
-    for (int m = 0; m < 16; ++m) pv[m] = (m + q < 16) ? pr[part + 4 * (m + q)] : 0.0;
-    const double piv = pr[j];
-    if (r > j) {
-        const double l = arj / piv;
-        if (r < kb && !(fabs(l) <= 1.0)) my_bad = 1;
-        // slot 0 holds column 4 q + part: updated when part > P, the multiplier when part == P
-        a[0] = part > P ? fma(-l, pv[0], a[0]) : (part == P ? l : a[0]);
+    for (int P = 0; P < 4; ++P) {
 #pragma unroll
-        for (int m = 1; m < 16; ++m) a[m] = fma(-l, pv[m], a[m]);
+        for (int m = 0; m < 16; ++m) pv[P][m] = (m + q < 16) ? pr[P][part + 4 * (m + q)] : 0.0;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) B[P][c] = pr[P][j0 + c];
+    }
+    // pivot rows P through the group's steps P2 < P
+#pragma unroll
+    for (int P = 1; P < 4; ++P) {
+#pragma unroll
+        for (int P2 = 0; P2 < P; ++P2) {
+            const double mlt = B[P][P2] / B[P2][P2];
+#pragma unroll
+            for (int c = P2 + 1; c < 4; ++c) B[P][c] = fma(-mlt, B[P2][c], B[P][c]);
+            // slot 0 (column 4 q + part) changes only past the step's column, as in the owner's row
+            pv[P][0] = part > P2 ? fma(-mlt, pv[P2][0], pv[P][0]) : pv[P][0];
+#pragma unroll
+            for (int m = 1; m < 16; ++m) pv[P][m] = fma(-mlt, pv[P2][m], pv[P][m]);
+        }
+    }
+#pragma unroll
+    for (int P = 0; P < 4 - LAST; ++P) {
+        const int j = j0 + P;
+        const double arj = quad_bcast(a[0], P);   // a[r][j] after the group's earlier steps
+        if (r > j) {
+            const double l = arj / B[P][P];
+            if (r < kb && !(fabs(l) <= 1.0)) my_bad = 1;
+            // slot 0 holds column 4 q + part: updated when part > P, the multiplier when part == P
+            a[0] = part > P ? fma(-l, pv[P][0], a[0]) : (part == P ? l : a[0]);
+#pragma unroll
+            for (int m = 1; m < 16; ++m) a[m] = fma(-l, pv[P][m], a[m]);
+        }
     }
 }
 
@@ -246,7 +274,7 @@ __device__ void invert_factors(const double* D, double* Li, double* Ui, double* 
 }
 
 __device__ void factor_invert_diag(double* D, double* Li, double* Ui, double* T, int kb, int* bad) {
-    __shared__ double prow[2][kNB];
+    __shared__ double P4[2][4][kNB];   // the pivot rows of a group of four steps, by group parity
     const int t = threadIdx.x, r = t >> 2, part = t & 3;
     double a[16];
 #pragma unroll
@@ -259,10 +287,8 @@ __device__ void factor_invert_diag(double* D, double* Li, double* Ui, double* T,
     // ---- elimination: D becomes L (strictly below the diagonal) \ U, padded rows identity
 #pragma unroll 1
     for (int q = 0; q < 16; ++q) {
-        elim_step<0>(a, prow, q, r, part, kb, my_bad);
-        elim_step<1>(a, prow, q, r, part, kb, my_bad);
-        elim_step<2>(a, prow, q, r, part, kb, my_bad);
-        if (q < 15) elim_step<3>(a, prow, q, r, part, kb, my_bad);   // step 63 has nothing below
+        if (q < 15) elim_group<0>(a, P4, q, r, part, kb, my_bad);
+        else elim_group<1>(a, P4, q, r, part, kb, my_bad);   // step 63 has nothing below
         D[r * LD + part + 4 * q] = a[0];   // column 4 q + part is final
 #pragma unroll
         for (int m = 0; m < 15; ++m) a[m] = a[m + 1];
