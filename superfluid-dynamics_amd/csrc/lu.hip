@@ -136,12 +136,15 @@ __device__ inline void elim_group(double (&a)[16], double (*P4)[4][kNB], int q, 
 #pragma unroll
         for (int c = 0; c < 4; ++c) B[P][c] = pr[P][j0 + c];
     }
-    // pivot rows P through the group's steps P2 < P
+    // pivot rows P through the group's steps P2 < P; multipliers by the reciprocal of the pivot
+    // (as getf2 scales a column), one reciprocal per pivot
+    double inv[4];
+    inv[0] = fast_rcp(B[0][0]);
 #pragma unroll
     for (int P = 1; P < 4; ++P) {
 #pragma unroll
         for (int P2 = 0; P2 < P; ++P2) {
-            const double mlt = B[P][P2] / B[P2][P2];
+            const double mlt = B[P][P2] * inv[P2];
 #pragma unroll
             for (int c = P2 + 1; c < 4; ++c) B[P][c] = fma(-mlt, B[P2][c], B[P][c]);
             // slot 0 (column 4 q + part) changes only past the step's column, as in the owner's row
@@ -149,13 +152,14 @@ __device__ inline void elim_group(double (&a)[16], double (*P4)[4][kNB], int q, 
 #pragma unroll
             for (int m = 1; m < 16; ++m) pv[P][m] = fma(-mlt, pv[P2][m], pv[P][m]);
         }
+        inv[P] = fast_rcp(B[P][P]);
     }
 #pragma unroll
     for (int P = 0; P < 4 - LAST; ++P) {
         const int j = j0 + P;
         const double arj = quad_bcast(a[0], P);   // a[r][j] after the group's earlier steps
         if (r > j) {
-            const double l = arj / B[P][P];
+            const double l = arj * inv[P];
             if (r < kb && !(fabs(l) <= 1.0)) my_bad = 1;
             // slot 0 holds column 4 q + part: updated when part > P, the multiplier when part == P
             a[0] = part > P ? fma(-l, pv[P][0], a[0]) : (part == P ? l : a[0]);
