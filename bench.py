@@ -274,7 +274,7 @@ def main():
     e.step(DT, kt_steps)
     e.sync()
     reps = {t: e.kernel_report(t) for t in ("deriv", "assemble", "lu_panel", "lu_update", "lu_backsolve",
-                                            "gm_iter", "gm_close",
+                                            "gm_iter", "gm_close", "gm_close_mu",
                                             "mu_deriv", "velocity")}
     st = e.stats()
     e.kernel_timing(False)

@@ -54,6 +54,8 @@ __global__ __launch_bounds__(256) void assemble64_kernel(const SurfDesc* __restr
         diag[r] = (water ? 0.5 * (1 + ph.rho) : 0.5) + c * q.y;
         if (ph.image) diag[r] -= ci * cot64(0.0, Z[d.voff + k].y + h).y;
     }
+    // b of the rows whose residual the threads t < R store at the end
+    const double bt = (gm_on && t < R) ? gm.b[d.voff + min(k0 + t, n - 1)] : 0.0;
     double* Mk = M + d.moff + (long long)k0 * n;
     double acc[R];
 #pragma unroll
@@ -104,7 +106,7 @@ __global__ __launch_bounds__(256) void assemble64_kernel(const SurfDesc* __restr
     __shared__ double red[(R < 4 ? 4 : R) * 256];
     __shared__ double out[R];
     rows_reduce<R>(acc, red, out);
-    if (t < R && k0 + t < n) gm.V[(long long)(gm.m + 1) * d.voff + k0 + t] = gm.b[d.voff + k0 + t] - out[t];
+    if (t < R && k0 + t < n) gm.V[(long long)(gm.m + 1) * d.voff + k0 + t] = bt - out[t];
 }
 
 }  // namespace sfh

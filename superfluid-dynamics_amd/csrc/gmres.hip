@@ -6,17 +6,21 @@
 // outlier sits far to the right), so GMRES converges in a handful of iterations, and the stepper
 // warm starts every solve from the previous stage's mu.
 //
-// A solve is a fixed sequence of L launches of one kernel; launch i first finishes Arnoldi step
-// i - 1 and then, unless the solve is done, runs step i.  No workgroup waits for another: every
-// grid-wide result is read by the next launch.
+// A solve is a fixed sequence of L launches; launch i first finishes Arnoldi step i - 1 and then,
+// unless the solve is done, runs step i.  No workgroup waits for another: every grid-wide result
+// is read by the next launch.  Launches 0 .. L-2 are gm_iter_kernel; the last one, which only
+// finishes a step, is gm_close_mu_kernel, which goes on to form the derivative mu' of the
+// solution (with L = 1 the single launch is gm_iter_kernel).
 //   finish step i-1 (i >= 1), in every workgroup: sum the partial dots v_l . w_{i-1} of all
 //     workgroups (fixed order, so every workgroup gets the same h_{l,i-1}); rebuild
 //     u = w_{i-1} - sum_l h_{l,i-1} v_l over all N entries into LDS (classical Gram-Schmidt, one
 //     pass) and its norm h_{i,i-1} = |u|; rotate the Hessenberg column by the stored Givens
 //     rotations and form the new one.  Workgroup 0 stores the column, the rotation and the scale.
 //     Each workgroup stores its own rows of V[i] = u.
-//   converged (or the last launch): y from the rotated Hessenberg system; each workgroup applies
-//     x += V y to its own rows and pushes them into the warm-start history.
+//   converged (or the last launch): y from the rotated Hessenberg system; the last launch stores
+//     x0 + V y as the solution, each workgroup its own rows, and pushes them into the warm-start
+//     history (a launch that finds convergence earlier only keeps y for it).  The initial guess
+//     x0 and the solution are separate arrays.
 //   step i: w_i = M v_i for the workgroup's rows (one row per wave, the row's loads issued at the
 //     kernel's start so they overlap the finishing work), and the partial dots v_l . w_i, l <= i.
 // Launch 0 takes v_0 = r0 / |r0|.  The assembly kernel forms r0 = b - M x0 while it writes M
@@ -73,7 +77,8 @@ __device__ inline double row_dot_rest(const double* __restrict__ R, const RowChu
     return wave_sum(a0 + a1);
 }
 
-// u = w - sum_{i<j} co[i] V[i] over all n entries into vs (LDS); returns |u|^2 over the workgroup.
+// u = w - sum_{i<j} co[i] V[i] over all n entries into vs (LDS, unless null); returns |u|^2 over
+// the workgroup.
 // Each thread handles 8 entries per pass with the basis loads of 4 vectors in flight together.
 // The entries of rows [r0, r0 + kRowsPerWG) are also stored into Vj (when not null).
 __device__ inline void load_first_pass(const double* __restrict__ w, int n, double (&u)[8]) {
@@ -134,7 +139,7 @@ __device__ inline double rebuild_vector(const double* __restrict__ w, const doub
         for (int e = 0; e < 8; ++e) {
             const int c = base + t + 256 * e;
             if (c < n) {
-                vs[c] = u[e];
+                if (vs) vs[c] = u[e];
                 ss += u[e] * u[e];
                 if (Vj && c >= r0 && c < r0 + kRowsPerWG) Vj[c] = u[e];
             }
@@ -160,24 +165,45 @@ __device__ inline void givens_column(double* col, const double* cs, const double
     col[jc + 1] = 0.0;
 }
 
+// y from R y = g over the k rotated columns, into y[0..k): column k - 1 (col, LDS) is this
+// launch's, columns before it (H, column l at H[l (m+1)]) and g[0..k-2] come from earlier
+// launches.
+__device__ inline void back_substitute(const double* g, const double* H, int m, int k, double gk, const double* col,
+                                       double* y) {
+    for (int q = k - 1; q >= 0; --q) {
+        double acc = (q == k - 1) ? gk : g[q];
+        for (int l = q + 1; l < k; ++l) {
+            const double hql = (l == k - 1) ? col[q] : H[l * (m + 1) + q];
+            acc -= hql * y[l];
+        }
+        const double hqq = (q == k - 1) ? col[q] : H[q * (m + 1) + q];
+        y[q] = acc / hqq;
+    }
+}
+
 // This surface's history of solutions of the current RK stage (null for a plain evaluation), and
 // the stage's count of stored steps (one increment per solve, by surface 0).
-__device__ inline double* history(const GmArgs& a, const SurfDesc* sd) {
-    return a.stage > 0 ? a.hist + (long long)(a.stage - 1) * kHistDepth * a.total + sd[blockIdx.z].voff : nullptr;
+__device__ inline double* history(const GmArgs& a, const SurfDesc* sd, int s) {
+    return a.stage > 0 ? a.hist + (long long)(a.stage - 1) * kHistDepth * a.total + sd[s].voff : nullptr;
 }
 __device__ inline void count_history(const GmArgs& a) {
     a.hvalid[a.stage - 1] = min(a.hvalid[a.stage - 1] + 1, kHistDepth);
 }
 
 // Launch i of a solve's sequence (see the file comment).
-__global__ __launch_bounds__(256) void gm_iter_kernel(GmArgs a, const SurfDesc* sd, int i) {
+__global__ __launch_bounds__(256) void gm_iter_kernel(GmArgs a, const SurfDesc* __restrict__ sd, int i) {
     GmCtl* ctl = reinterpret_cast<GmCtl*>(a.ctl) + blockIdx.z;
-    if (ctl->done) return;   // converged solves: one load and out
+    // solves finished by an earlier launch return at once.  A finish stamped by this launch
+    // (workgroup 0 can get there before a late workgroup starts) does not count.  Launch 0 needs
+    // no check (the derivative kernel re-armed the block), and launch 1 makes it after its row
+    // loads have gone out (an earlier finish is the exception there); later launches first.
+    if (i >= 2) {
+        const int fin = ctl->done;
+        if (fin != 0 && fin <= i) return;
+    }
     const GmView v = gm_view(a, sd);
     const int m = a.m;
-    // the last launch of the sequence runs no product: it takes kCloseRows rows per workgroup
-    // (fewer workgroups repeating the finishing work); the others kRowsPerWG
-    const int rpw = (i == m - 1) ? kCloseRows : kRowsPerWG;
+    const int rpw = kRowsPerWG;
     if ((int)blockIdx.x * rpw >= v.n) return;
     __shared__ double vs[kGmMaxN];
     __shared__ double co[kMaxKrylov + 1];
@@ -191,15 +217,39 @@ __global__ __launch_bounds__(256) void gm_iter_kernel(GmArgs a, const SurfDesc* 
     const int row = r0 + wv;
     const double* R = v.M + (long long)min(row, n - 1) * n;
     const long long pslot = (long long)v.nwg * (m + 1);   // one partial-dot slot per launch parity
+    // Loads go out in the order their results are needed, because a wait for one load waits for
+    // all loads issued before it: first the vectors and sums of the step being finished (launch
+    // 0: r0 and b), then the matrix rows, which the finishing work then overlaps.
+    const int j = i - 1;
+    const double* wprev = (j & 1) ? v.w1 : v.w0;
+    const double* part = v.part + (j & 1) * pslot;
+    double pa[8], pb[8];   // first 2048 entries: launch 0 r0 and b, later w_{i-1} and V[0]
+    DotLoads dl;
+    const bool fastdots = i >= 1 && dots_fast(v.nwg, i);
+    double sc_t = 0.0, gold = 0.0, bnorm = 0.0;
+    if (i == 0) {
+        load_first_pass(v.V, n, pa);
+        load_first_pass(v.b, n, pb);
+    } else {
+        load_first_pass(wprev, n, pa);
+        load_first_pass(v.V, n, pb);
+        if (fastdots) dots_load(part, v.nwg, i, dl);
+        sc_t = t <= j ? v.sc[t] : 0.0;
+        gold = v.gc[j];
+        bnorm = ctl->bnorm;
+    }
     RowChunk rc;
-    if (i < m - 1) row_load(R, n, lane, rc);   // independent of v_i: in flight first
+    if (i < m - 1) row_load(R, n, lane, rc);   // independent of v_i
+    if (i == 1) {
+        const int fin = ctl->done;
+        if (fin != 0 && fin <= i) return;
+    }
     int k = 0;        // Arnoldi steps completed
     double sj = 0.0;  // scale of v_i = sj * vs
-    double sc_t = 0.0;
     bool done = false;
     // this workgroup's own rows (threads t < rpw): x, V[0] and the history levels that move
     // down, for the correction at the end of a solve, loaded up front
-    double* h0 = history(a, sd);
+    double* h0 = history(a, sd, blockIdx.z);
     const int oc = min(r0 + t, n - 1);
     double xo = 0.0, vo0 = 0.0, ho[kHistDepth - 1];
 #pragma unroll
@@ -215,12 +265,18 @@ __global__ __launch_bounds__(256) void gm_iter_kernel(GmArgs a, const SurfDesc* 
     if (i == 0) {
         // v_0 = r0 / |r0|; beta = |r0| and |b| reduced by every workgroup over its own copies
         double sq = 0.0, bq = 0.0;
-        for (int c = t; c < n; c += 256) {
-            const double x = v.V[c];
-            vs[c] = x;
-            sq += x * x;
-            const double bc = v.b[c];
-            bq += bc * bc;
+        for (int base = 0; base < n; base += 2048) {
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                const int c = base + t + 256 * e;
+                const double x = base == 0 ? pa[e] : (c < n ? v.V[c] : 0.0);
+                const double bc = base == 0 ? pb[e] : (c < n ? v.b[c] : 0.0);
+                if (c < n) {
+                    vs[c] = x;
+                    sq += x * x;
+                    bq += bc * bc;
+                }
+            }
         }
         sq = wave_sum(sq);
         bq = wave_sum(bq);
@@ -243,21 +299,13 @@ __global__ __launch_bounds__(256) void gm_iter_kernel(GmArgs a, const SurfDesc* 
         sj = beta > 0.0 ? 1.0 / beta : 0.0;
     } else {
         // ---- finish step i - 1
-        const int j = i - 1;
-        const double* part = v.part + (j & 1) * pslot;
-        double wpre[8];
-        const double* wprev = (j & 1) ? v.w1 : v.w0;
-        // loads that do not depend on the dot sums go out first, together with the partial dots
-        double v0pre[8];
-        load_first_pass(wprev, n, wpre);
-        load_first_pass(v.V, n, v0pre);
-        sc_t = t <= j ? v.sc[t] : 0.0;
-        const double gold = v.gc[j];
-        const double bnorm = ctl->bnorm;
-        reduce_partials(part, v.nwg, j + 1, sums);
+        if (fastdots)
+            dots_finish(dl, j + 1, sums);
+        else
+            reduce_partials(part, v.nwg, j + 1, sums);
         if (t <= j) co[t] = sums[t] * sc_t;
         __syncthreads();
-        const double hn = sqrt(rebuild_vector(wprev, v.V, co, j + 1, n, 0, vs, nullptr, red8, wpre, v0pre));
+        const double hn = sqrt(rebuild_vector(wprev, v.V, co, j + 1, n, 0, vs, nullptr, red8, pa, pb));
         if (t == 0) {
             for (int q = 0; q <= j; ++q) col[q] = sums[q];
             col[j + 1] = hn;
@@ -290,31 +338,34 @@ __global__ __launch_bounds__(256) void gm_iter_kernel(GmArgs a, const SurfDesc* 
             if (t < kRowsPerWG && r0 + t < n) v.V[(long long)i * n + r0 + t] = vs[r0 + t];
             sj = hn > 0.0 ? 1.0 / hn : 0.0;
         } else if (t == 0) {
-            // y from R y = g over the k rotated columns (column k-1 is still in LDS), scaled by sc
-            for (int q = k - 1; q >= 0; --q) {
-                double acc = (q == k - 1) ? gj : v.g[q];
-                for (int l = q + 1; l < k; ++l) {
-                    const double hql = (l == k - 1) ? col[q] : v.H[(long long)l * (m + 1) + q];
-                    acc -= hql * ys[4 + l];
-                }
-                const double hqq = (q == k - 1) ? col[q] : v.H[(long long)q * (m + 1) + q];
-                ys[4 + q] = acc / hqq;
-            }
+            back_substitute(v.g, v.H, m, k, gj, col, ys + 4);   // then scaled by sc
         }
         if (done) {   // y_q sc_q
             __syncthreads();
             if (t <= j) ys[4 + t] *= sc_t;
         }
     }
+    if (done && i < m - 1) {
+        // converged before the last launch, which stores the solution (gm_close_mu_kernel): keep y
+        if (blockIdx.x == 0) {
+            if (t < k) v.y[t] = ys[4 + t];   // scaled by this thread above
+            if (t == 0) {
+                ctl->iters = k;
+                ctl->done = i + 1;
+            }
+        }
+        return;
+    }
     if (done) {
-        // x += V y on this workgroup's rows; the stage's history
+        // the single launch of a one-launch sequence: x0 + V y on this workgroup's rows; the
+        // stage's history
         __syncthreads();
         if (t < rpw && r0 + t < n) {
             const int c = r0 + t;
             double x = xo;
             if (k > 0) x += ys[4] * vo0;
             for (int q = 1; q < k; ++q) x += ys[4 + q] * v.V[(long long)q * n + c];
-            v.x[c] = x;
+            v.sol[c] = x;
             if (h0) {
 #pragma unroll
                 for (int l = kHistDepth - 1; l > 0; --l) h0[(long long)l * a.total + c] = ho[l - 1];
@@ -323,7 +374,7 @@ __global__ __launch_bounds__(256) void gm_iter_kernel(GmArgs a, const SurfDesc* 
         }
         if (blockIdx.x == 0 && t == 0) {
             ctl->iters = k;
-            ctl->done = 1;
+            ctl->done = i + 1;
             if (h0 && blockIdx.z == 0) count_history(a);
         }
         return;
@@ -352,6 +403,269 @@ __global__ __launch_bounds__(256) void gm_iter_kernel(GmArgs a, const SurfDesc* 
             }
         }
         v.part[(i & 1) * pslot + (long long)t * v.nwg + blockIdx.x] = s;
+    }
+}
+
+// The last launch of a sequence (i = m - 1 >= 1) fused with the derivative of the solution,
+// mu' = D1 mu 2 pi / n (mu_circ_kernel's circulant product; BaseBoundaryIntegrator.cuh:201-203).
+// Every workgroup finishes Arnoldi step i - 1 as gm_iter_kernel does, forms y and stages
+// mu = x0 + V y of all nodes in LDS, kCircChunk at a time.  It stores its own kCircRowsPerWG rows
+// of the solution (and pushes them into the stage's history) from the staged values, and
+// multiplies the staged mu by the circulant kernel for mu' of the same rows.  A solve that
+// converged in an earlier launch left its (scaled) y behind, and this launch stores its solution
+// all the same.  Everything the launch reads that does not depend on another of its loads is
+// requested at the start: the kernel window of the first chunk, the first 2048 entries of x0,
+// V[0], V[1] and w_{i-1}, the partial dots, and the small state (scales, rotations, g, H, y) into
+// LDS, so that the finishing work runs on data already at hand.  A workgroup reads the initial
+// guess, the basis and earlier launches' results, never the solution entries that other
+// workgroups of this launch are storing.  grid = (ceil(max n / kCircRowsPerWG), surfaces).
+__global__ __launch_bounds__(256) void gm_close_mu_kernel(GmArgs a, const SurfDesc* __restrict__ sd,
+                                                          const double* __restrict__ circ,
+                                                          double2* __restrict__ mup, int i) {
+    constexpr int CR = kCircRowsPerWG;
+    constexpr int KM = kMaxKrylov;
+    static_assert(kCircPer == 8, "the first staged chunk reuses the 8-entry first passes");
+    const int s = blockIdx.y;
+    const SurfDesc d = sd[s];
+    const int n = d.n;
+    const int k0 = blockIdx.x * CR;
+    if (k0 >= n) return;
+    const GmView v = gm_view(a, sd, s);
+    GmCtl* ctl = reinterpret_cast<GmCtl*>(a.ctl) + s;
+    const int m = a.m;
+    const int j = i - 1;
+    __shared__ double ms[kCircChunk];
+    __shared__ double2 c1[kCircWin];
+    __shared__ double Hs[KM * (KM + 1)];
+    __shared__ double cs_s[KM], sn_s[KM], g_s[KM + 1];
+    __shared__ double co[KM + 1];
+    __shared__ double col[KM + 2];
+    __shared__ double red8[8];
+    __shared__ double sums[KM + 2];
+    __shared__ double ys[KM + 4];   // g[j], gc[j+1], c, s, then y
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    const double* c1r = circ + d.coff;
+    const double* c1i = c1r + n;
+    double ar[kCircWinPer], ai[kCircWinPer];
+    {
+        int q[kCircWinPer];
+        circ_window(k0, 0, n, q);
+#pragma unroll
+        for (int e = 0; e < kCircWinPer; ++e) {
+            ar[e] = c1r[q[e]];
+            ai[e] = c1i[q[e]];
+        }
+    }
+    const double* wprev = (j & 1) ? v.w1 : v.w0;
+    double xpre[8], v0pre[8], v1pre[8], wpre[8];
+    load_first_pass(v.x, n, xpre);
+    load_first_pass(v.V, n, v0pre);
+    load_first_pass(j >= 1 ? v.V + n : v.V, n, v1pre);
+    load_first_pass(wprev, n, wpre);
+    const long long pslot = (long long)v.nwg * (m + 1);
+    const double* part = v.part + (j & 1) * pslot;
+    DotLoads dl;
+    const bool fastdots = dots_fast(v.nwg, j + 1);
+    if (fastdots) dots_load(part, v.nwg, j + 1, dl);
+    const double sc_t = t <= j ? v.sc[t] : 0.0;
+    const double y_t = t < m ? v.y[t] : 0.0;
+    const double gold = v.gc[j];
+    double cs_t = 0.0, sn_t = 0.0, g_t = 0.0, hreg[4];
+    if (t < j) {
+        cs_t = v.cs[t];
+        sn_t = v.sn[t];
+        g_t = v.g[t];
+    }
+    static_assert(4 * 256 >= (KM - 1) * (KM + 1), "H columns 0 .. j-1 in four registers per thread");
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int e = t + 256 * r;
+        hreg[r] = e < j * (m + 1) ? v.H[e] : 0.0;
+    }
+    double* h0 = history(a, sd, s);
+    double ho[kHistDepth - 1];
+#pragma unroll
+    for (int l = 0; l < kHistDepth - 1; ++l) ho[l] = 0.0;
+    if (h0 && t < CR) {
+        const int oc = min(k0 + t, n - 1);
+#pragma unroll
+        for (int l = 0; l < kHistDepth - 1; ++l) ho[l] = h0[(long long)l * a.total + oc];
+    }
+    // the control block last: its fields decide branches, so the first use waits for them
+    const int fin = ctl->done;
+    const int iters_before = ctl->iters;
+    const double bnorm = ctl->bnorm;
+    // all loads are out; the small state into LDS and the partial dots summed (used when the
+    // solve is open)
+    if (t < j) {
+        cs_s[t] = cs_t;
+        sn_s[t] = sn_t;
+        g_s[t] = g_t;
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int e = t + 256 * r;
+        if (e < j * (m + 1)) Hs[e] = hreg[r];
+    }
+    if (fastdots)
+        dots_finish(dl, j + 1, sums);
+    else
+        reduce_partials(part, v.nwg, j + 1, sums);
+    // a finish stamped by this launch (workgroup 0 got there first) leaves the solve open here
+    const bool open = !(fin != 0 && fin <= i);
+    int k;
+    if (!open) {
+        // converged in an earlier launch, which kept y (scaled) for this one
+        k = iters_before;
+        if (t < k) ys[4 + t] = y_t;
+        __syncthreads();
+    } else {
+        // ---- finish step j: h_{l,j} = sums[l] sc_l, |u| with u = w_j - sum_l h_{l,j} v_l
+        if (t <= j) co[t] = sums[t] * sc_t;
+        __syncthreads();
+        double ss = 0.0;
+        for (int base = 0; base < n; base += 2048) {
+            double u[8];
+            if (base == 0) {
+#pragma unroll
+                for (int e = 0; e < 8; ++e) u[e] = wpre[e] - co[0] * v0pre[e];
+                if (j >= 1) {
+#pragma unroll
+                    for (int e = 0; e < 8; ++e) u[e] -= co[1] * v1pre[e];
+                }
+                for (int l = 2; l <= j; ++l) {
+                    double vv[8];
+                    load_first_pass(v.V + (long long)l * n, n, vv);
+#pragma unroll
+                    for (int e = 0; e < 8; ++e) u[e] -= co[l] * vv[e];
+                }
+            } else {
+#pragma unroll
+                for (int e = 0; e < 8; ++e) {
+                    const int c = base + t + 256 * e;
+                    u[e] = c < n ? wprev[c] : 0.0;
+                }
+                for (int l = 0; l <= j; ++l) {
+                    double vv[8];
+#pragma unroll
+                    for (int e = 0; e < 8; ++e) {
+                        const int c = base + t + 256 * e;
+                        vv[e] = c < n ? v.V[(long long)l * n + c] : 0.0;
+                    }
+#pragma unroll
+                    for (int e = 0; e < 8; ++e) u[e] -= co[l] * vv[e];
+                }
+            }
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                const int c = base + t + 256 * e;
+                if (c < n) ss += u[e] * u[e];
+            }
+        }
+        const double hn = sqrt(block_sum4(ss, red8));
+        if (t == 0) {
+            for (int q = 0; q <= j; ++q) col[q] = sums[q];
+            col[j + 1] = hn;
+            double c, sn;
+            givens_column(col, cs_s, sn_s, j, c, sn);
+            ys[0] = c * gold;    // g[j]
+            ys[1] = -sn * gold;  // gc[j + 1]
+            ys[2] = c;
+            ys[3] = sn;
+            back_substitute(g_s, Hs, m, j + 1, ys[0], col, ys + 4);
+        }
+        __syncthreads();
+        const double gj = ys[0], gnext = ys[1], res = fabs(gnext);
+        k = i;
+        if (blockIdx.x == 0 && t == 0) {
+            double* Hc = v.H + (long long)j * (m + 1);
+            for (int q = 0; q <= j + 1; ++q) Hc[q] = col[q];
+            v.cs[j] = ys[2];
+            v.sn[j] = ys[3];
+            v.g[j] = gj;
+            v.gc[j + 1] = gnext;
+            v.sc[j + 1] = hn > 0.0 ? 1.0 / hn : 0.0;
+            ctl->res = res;
+            ctl->hnext = hn;
+            if (!(res <= a.tol * bnorm || hn == 0.0)) ctl->unconverged += 1;
+            ctl->iters = k;
+            ctl->done = i + 1;
+        }
+        if (t <= j) ys[4 + t] *= sc_t;   // y_q sc_q
+        __syncthreads();
+    }
+    if (h0 && s == 0 && blockIdx.x == 0 && t == 0) count_history(a);
+    // mu = x0 + y_0 V[0] + y_1 V[1] (+ the rest when k > 2), without branches in the common case
+    const double y0 = k > 0 ? ys[4] : 0.0, y1 = k > 1 ? ys[5] : 0.0;
+    double2 a0s = {0.0, 0.0}, a1s = {0.0, 0.0};
+    for (int j0 = 0; j0 < n; j0 += kCircChunk) {
+        const int jn = min(kCircChunk, n - j0);
+        if (j0 > 0) {
+            __syncthreads();   // the previous chunk has been consumed
+            int q[kCircWinPer];
+            circ_window(k0, j0, n, q);
+#pragma unroll
+            for (int e = 0; e < kCircWinPer; ++e) {
+                ar[e] = c1r[q[e]];
+                ai[e] = c1i[q[e]];
+            }
+            load_first_pass(v.x + j0, n - j0, xpre);
+            load_first_pass(v.V + j0, n - j0, v0pre);
+            load_first_pass(v.V + n + j0, n - j0, v1pre);
+        }
+        double mv[kCircPer];
+#pragma unroll
+        for (int e = 0; e < kCircPer; ++e) mv[e] = xpre[e] + y0 * v0pre[e] + y1 * v1pre[e];
+        for (int q = 2; q < k; ++q) {
+            double vq[8];
+            load_first_pass(v.V + (long long)q * n + j0, n - j0, vq);
+            const double yq = ys[4 + q];
+#pragma unroll
+            for (int e = 0; e < kCircPer; ++e) mv[e] += yq * vq[e];
+        }
+#pragma unroll
+        for (int e = 0; e < kCircPer; ++e) {
+            const int c = t + 256 * e;
+            if (c < jn) ms[c] = mv[e];
+        }
+#pragma unroll
+        for (int e = 0; e < kCircWinPer; ++e) {
+            const int u = t + 256 * e;
+            if (u < kCircWin) c1[u] = make_double2(ar[e], ai[e]);
+        }
+        __syncthreads();
+        if (t < CR) {   // the own rows of the solution, as staged
+            const int r = k0 + t;
+            if (r < n && r >= j0 && r < j0 + jn) {
+                const double x = ms[r - j0];
+                v.sol[r] = x;
+                if (h0) {
+#pragma unroll
+                    for (int l = kHistDepth - 1; l > 0; --l) h0[(long long)l * a.total + r] = ho[l - 1];
+                    h0[r] = x;
+                }
+            }
+        }
+#pragma unroll 4
+        for (int c = lane; c < jn; c += 64) {
+            const double x = ms[c];
+            const int u0 = wv - c + kCircChunk - 1;
+            const double2 p0 = c1[u0], p1 = c1[u0 + 4];
+            a0s.x += p0.x * x;
+            a0s.y += p0.y * x;
+            a1s.x += p1.x * x;
+            a1s.y += p1.y * x;
+        }
+    }
+    a0s.x = wave_sum(a0s.x); a0s.y = wave_sum(a0s.y);
+    a1s.x = wave_sum(a1s.x); a1s.y = wave_sum(a1s.y);
+    if (lane < 2) {   // lane 0: row wv, lane 1: row wv + 4
+        const int r = k0 + wv + 4 * lane;
+        if (r < n) {
+            const double2 acc = lane ? a1s : a0s;
+            const double xs = 2.0 * kPi / n;
+            mup[d.voff + r] = make_double2(acc.x * xs, acc.y * xs);
+        }
     }
 }
 

@@ -9,12 +9,11 @@
 namespace sfh {
 
 constexpr int kRowsPerWG = 4;      // one matrix row per wave
-constexpr int kCloseRows = 32;     // own rows per workgroup in the last launch of a sequence
 constexpr int kMaxKrylov = 32;     // basis vectors per solve (iteration cap)
 constexpr int kGmMaxN = 4096;      // LDS copy of the vector being multiplied
 
 struct GmCtl {
-    int done;
+    int done;          // 0 while the solve runs, else 1 + the launch that finished it
     int spare0;
     int iters;         // Arnoldi steps of the last solve
     int unconverged;   // solves that ended at the launch limit (cumulative)
@@ -27,7 +26,8 @@ static_assert(sizeof(GmCtl) == kGmCtlInts * sizeof(int), "host reads control blo
 struct GmView {
     const double* M;
     const double* b;
-    double* x;
+    const double* x;   // initial guess
+    double* sol;       // solution
     double* V;       // (m+1) x n, unnormalised: v_i = sc[i] * V[i]
     double* w0;      // products of even steps
     double* w1;      // products of odd steps
@@ -37,6 +37,7 @@ struct GmView {
     double* sn;
     double* g;
     double* H;       // column j at H[j*(m+1)]
+    double* y;       // m+1: y_q sc_q of a solve that converged before the last launch of its sequence
     double* part;    // 2 slots (launch parity) of (m+1) x nwg partial sums, basis index major
     GmCtl* ctl;
     int n, nwg;
@@ -51,6 +52,7 @@ __device__ inline GmView gm_view(const GmArgs& a, const SurfDesc* sd, int s) {
     v.M = a.M + d.moff;
     v.b = a.b + d.voff;
     v.x = a.x + d.voff;
+    v.sol = a.sol + d.voff;
     v.V = a.V + (long long)(m + 1) * d.voff;
     v.w0 = a.w + d.voff;
     v.w1 = a.w + a.total + d.voff;
@@ -60,6 +62,7 @@ __device__ inline GmView gm_view(const GmArgs& a, const SurfDesc* sd, int s) {
     v.sn = v.cs + m;
     v.g = v.sn + m;
     v.H = v.g + (m + 1);
+    v.y = v.H + (long long)m * (m + 1);
     v.part = a.part + (long long)s * a.part_stride;
     v.ctl = reinterpret_cast<GmCtl*>(a.ctl) + s;
     return v;
@@ -96,6 +99,38 @@ __device__ inline void reduce_partials(const double* __restrict__ part, int nwg,
                 const double s = wave_sum(acc[u]);
                 if (lane == 0) red[wv][i0 + u] = s;
             }
+        }
+    }
+    __syncthreads();
+    if (t < count) out[t] = red[0][t] + red[1][t] + red[2][t] + red[3][t];
+    __syncthreads();
+}
+
+// The same sums for count <= kFastDots and nwg <= 512, split in two so that a kernel can issue
+// the loads together with its other early loads and add them up when it needs the sums (a wait
+// for one load waits for every load issued before it).
+constexpr int kFastDots = 4;
+struct DotLoads {
+    double a[kFastDots], b[kFastDots];
+};
+__device__ inline bool dots_fast(int nwg, int count) { return count <= kFastDots && nwg <= 512; }
+__device__ inline void dots_load(const double* __restrict__ part, int nwg, int count, DotLoads& L) {
+    const int w0 = threadIdx.x, w1 = threadIdx.x + 256;
+#pragma unroll
+    for (int u = 0; u < kFastDots; ++u) {
+        const bool ok = u < count;
+        L.a[u] = (ok && w0 < nwg) ? part[u * nwg + w0] : 0.0;
+        L.b[u] = (ok && w1 < nwg) ? part[u * nwg + w1] : 0.0;
+    }
+}
+__device__ inline void dots_finish(const DotLoads& L, int count, double* out) {
+    __shared__ double red[4][kFastDots];
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+#pragma unroll
+    for (int u = 0; u < kFastDots; ++u) {
+        if (u < count) {
+            const double s = wave_sum(L.a[u] + L.b[u]);
+            if (lane == 0) red[wv][u] = s;
         }
     }
     __syncthreads();

@@ -28,11 +28,17 @@ struct StageCtl {
 // polynomial through the hv of them that exist, evaluated one step ahead:
 //   hv = 1: u0;  2: 2 u0 - u1;  3: 3 (u0 - u1) + u2;  4: 4 (u0 + u2) - 6 u1 - u3   (error O(dt^hv)).
 constexpr int kHistDepth = 4;
-__device__ inline double extrapolate_history(const double* h, long long stride, int hv) {
-    if (hv >= 4) return 4.0 * (h[0] + h[2 * stride]) - 6.0 * h[stride] - h[3 * stride];
-    if (hv == 3) return 3.0 * (h[0] - h[stride]) + h[2 * stride];
-    if (hv == 2) return 2.0 * h[0] - h[stride];
+__device__ inline double extrapolate_values(const double (&h)[kHistDepth], int hv) {
+    if (hv >= 4) return 4.0 * (h[0] + h[2]) - 6.0 * h[1] - h[3];
+    if (hv == 3) return 3.0 * (h[0] - h[1]) + h[2];
+    if (hv == 2) return 2.0 * h[0] - h[1];
     return h[0];
+}
+__device__ inline double extrapolate_history(const double* h, long long stride, int hv) {
+    double v[kHistDepth];
+#pragma unroll
+    for (int l = 0; l < kHistDepth; ++l) v[l] = l < hv ? h[l * stride] : 0.0;
+    return extrapolate_values(v, hv);
 }
 __device__ inline void push_history(double* h, long long stride, double x) {
 #pragma unroll
@@ -43,7 +49,8 @@ __device__ inline void push_history(double* h, long long stride, double x) {
 struct GmArgs {
     const double* M;     // matrices (FP64, row-major per surface at SurfDesc::moff)
     const double* b;     // right-hand sides
-    double* x;           // initial guess in, solution out
+    double* x;           // initial guess (written by zphi_circ_kernel, read only during the solve)
+    double* sol;         // solution, written by the launch that finishes the solve
     double* V;           // Krylov basis, (m+1) * voff per surface
     double* w;           // two product vectors of total nodes each (even / odd steps)
     double* part;        // per-workgroup partial sums, part_stride per surface
@@ -64,7 +71,28 @@ struct GmArgs {
 };
 constexpr int kPairRows = 4;        // rows per workgroup of the N^2 pair kernels (assembly, velocity)
 // ---- spectral.hip
-constexpr int kCircRowsPerWG = 8;   // kCircRows in spectral.hip
+// circulant derivative products: rows per workgroup (wave w owns rows w and w + 4), columns
+// staged in LDS per round, and the kernel window one round needs
+constexpr int kCircRowsPerWG = 8;
+constexpr int kCircChunk = 2048;
+constexpr int kCircWin = kCircChunk + kCircRowsPerWG - 1;
+constexpr int kCircPer = kCircChunk / 256;               // staged samples per thread
+constexpr int kCircWinPer = (kCircWin + 255) / 256;      // staged kernel entries per thread
+// Window entry u holds c[(k0 - j0 - (kCircChunk - 1) + u) mod n]; row r, column c of the chunk
+// read entry r - c + kCircChunk - 1.  circ_window gives the kernel indices of the entries
+// u = threadIdx.x + 256 e a thread stages (entries beyond the window get valid indices too): one
+// division, then steps of 256 mod n, each reduced by one conditional subtraction.
+__device__ inline void circ_window(int k0, int j0, int n, int (&q)[kCircWinPer]) {
+    int r = (k0 - j0 - (kCircChunk - 1) + (int)threadIdx.x) % n;
+    if (r < 0) r += n;
+    const int step = 256 % n;
+#pragma unroll
+    for (int e = 0; e < kCircWinPer; ++e) {
+        q[e] = r;
+        r += step;
+        if (r >= n) r -= n;
+    }
+}
 __global__ void zphi_circ_kernel(const SurfDesc* sd, const double* circ, const double2* Z, const double* phi,
                                  double2* Zp, double2* Zpp, double* b, double* work, Phys ph, int* status,
                                  NodeTrig* trig, GmArgs gm, int gm_on);
@@ -104,6 +132,7 @@ __global__ void gepp_fallback_kernel(const SurfDesc* sd, double* M, const double
 // ---- gmres.hip
 constexpr int kGmCtlInts = 32;   // ints per control block (holds the device struct)
 __global__ void gm_iter_kernel(GmArgs a, const SurfDesc* sd, int j);
+__global__ void gm_close_mu_kernel(GmArgs a, const SurfDesc* sd, const double* circ, double2* mup, int i);
 
 // ---- batched.hip (persistent per-surface stage kernel, ensembles with every N <= kBatchMaxN)
 constexpr int kBatchThreads = 1024;

@@ -57,28 +57,48 @@ __device__ inline void velocity_from_sums(const Phys& ph, double2 zp, double2 zp
     q2 = cconj(csub(base, cscale(half_inv, mk)));
 }
 
-// RK4 stage bookkeeping of node g (AutonomousRungeKuttaStepper.cuh:338-375, utilities.cuh:78-83).
-__device__ inline void stage_update(const StageCtl& ctl, int g, double2 q1, double dphi) {
+// RK4 stage bookkeeping of node g (AutonomousRungeKuttaStepper.cuh:338-375, utilities.cuh:78-83),
+// in two halves so that a kernel can load a node's stage inputs long before it has the node's
+// velocity: stage_load reads what the stage combines, stage_store writes the stage's results.
+struct StageIn {
+    double2 z0, k1, k2, k3;
+    double p0, p1, p2, p3;
+};
+__device__ inline StageIn stage_load(const StageCtl& ctl, int g) {
+    StageIn s{};
+    if (ctl.stage >= 1) {
+        s.z0 = ctl.Z0[g];
+        s.p0 = ctl.P0[g];
+    }
+    if (ctl.stage >= 4) {
+        s.k1 = ctl.kZ[0][g];
+        s.k2 = ctl.kZ[1][g];
+        s.k3 = ctl.kZ[2][g];
+        s.p1 = ctl.kP[0][g];
+        s.p2 = ctl.kP[1][g];
+        s.p3 = ctl.kP[2][g];
+    }
+    return s;
+}
+__device__ inline void stage_store(const StageCtl& ctl, int g, double2 q1, double dphi, const StageIn& s) {
     if (ctl.stage == 0) {
         ctl.kZ[0][g] = q1;
         ctl.kP[0][g] = dphi;
     } else if (ctl.stage <= 3) {
         ctl.kZ[ctl.stage - 1][g] = q1;
         ctl.kP[ctl.stage - 1][g] = dphi;
-        const double2 z0 = ctl.Z0[g];
-        ctl.Zout[g] = make_double2(z0.x + ctl.alpha * q1.x, z0.y + ctl.alpha * q1.y);
-        ctl.Pout[g] = ctl.P0[g] + ctl.alpha * dphi;
+        ctl.Zout[g] = make_double2(s.z0.x + ctl.alpha * q1.x, s.z0.y + ctl.alpha * q1.y);
+        ctl.Pout[g] = s.p0 + ctl.alpha * dphi;
     } else {
-        const double2 k1 = ctl.kZ[0][g], k2 = ctl.kZ[1][g], k3 = ctl.kZ[2][g];
-        const double Kx = k1.x + 2.0 * k2.x + 2.0 * k3.x + q1.x;
-        const double Ky = k1.y + 2.0 * k2.y + 2.0 * k3.y + q1.y;
-        const double Kp = ctl.kP[0][g] + 2.0 * ctl.kP[1][g] + 2.0 * ctl.kP[2][g] + dphi;
-        double2* z0 = const_cast<double2*>(ctl.Z0) + g;
-        double* p0 = const_cast<double*>(ctl.P0) + g;
-        const double2 zz = *z0;
-        *z0 = make_double2(zz.x + ctl.alpha * Kx, zz.y + ctl.alpha * Ky);
-        *p0 = *p0 + ctl.alpha * Kp;
+        const double Kx = s.k1.x + 2.0 * s.k2.x + 2.0 * s.k3.x + q1.x;
+        const double Ky = s.k1.y + 2.0 * s.k2.y + 2.0 * s.k3.y + q1.y;
+        const double Kp = s.p1 + 2.0 * s.p2 + 2.0 * s.p3 + dphi;
+        const_cast<double2*>(ctl.Z0)[g] = make_double2(s.z0.x + ctl.alpha * Kx, s.z0.y + ctl.alpha * Ky);
+        const_cast<double*>(ctl.P0)[g] = s.p0 + ctl.alpha * Kp;
     }
+}
+__device__ inline void stage_update(const StageCtl& ctl, int g, double2 q1, double dphi) {
+    stage_store(ctl, g, q1, dphi, stage_load(ctl, g));
 }
 
 }  // namespace sfh

@@ -6,8 +6,9 @@
 //   assemble64_kernel     M                           (8 rows per workgroup)
 //   LU solve              mu                          (panel / MFMA update / back substitution)
 //   GEPP fallback         no-op unless a system needed row interchanges
-//     or GMRES            mu                          (one kernel per Arnoldi step)
-//   mu_circ_kernel        mu'
+//     or GMRES            mu                          (one kernel per Arnoldi step; the last
+//                                                      launch also forms mu', gm_close_mu_kernel)
+//   mu_circ_kernel        mu'                         (LU, and GMRES with a one-launch sequence)
 //   velocity_stage_kernel q, dPhi/dt and the RK4 stage update of every node
 // and one RK4 step (AutonomousRungeKuttaStepper.cuh:123-361) is four of them.  The step is
 // captured once into a hipGraph and replayed, so a run of many steps costs one graph launch
@@ -200,8 +201,10 @@ Solver::Solver(const Phys& ph, const std::vector<int>& n, const Options& opt) : 
         gm_m_ = std::max(1, std::min(opt_.max_iter, 32));   // kMaxKrylov in gmres.hip
         gm_nwg_ = (maxN_ + 3) / 4;   // kRowsPerWG in gmres.hip
         gm_part_stride_ = 2LL * gm_nwg_ * (gm_m_ + 1);
-        gm_small_stride_ = 3LL * (gm_m_ + 1) + 2LL * gm_m_ + (long long)gm_m_ * (gm_m_ + 1);
+        gm_small_stride_ = 4LL * (gm_m_ + 1) + 2LL * gm_m_ + (long long)gm_m_ * (gm_m_ + 1);
         gmV_ = dalloc<double>((size_t)(gm_m_ + 1) * T_);
+        gmx0_ = dalloc<double>(T_);
+        SFH_HIP(hipMemset(gmx0_, 0, T_ * sizeof(double)));
         gmw_ = dalloc<double>(2 * (size_t)T_);
         gmpart_ = dalloc<double>(gm_part_stride_ * S_);
         gmsmall_ = dalloc<double>(gm_small_stride_ * S_);
@@ -234,7 +237,7 @@ Solver::~Solver() {
     void* bufs[] = {d_desc_, d_tw_, Z0_, Za_, Zb_, P0_, Pa_, Pb_, kZ_[0], kZ_[1], kZ_[2], kP_[0],
                     kP_[1], kP_[2], Zp_, Zpp_, mup_, qup_, b_, work_, y_, mu_, M_, uinv_, flags_,
                     status_, piv_, counters_, gmV_, gmw_, gmpart_, gmsmall_, gmctl_, trig_, order_, bstats_,
-                    gmhist_, gmhvalid_, bhist_, bhvalid_, d_circ_};
+                    gmhist_, gmhvalid_, bhist_, bhvalid_, d_circ_, gmx0_};
     for (void* p : bufs)
         if (p) (void)hipFree(p);
     void* vbufs[] = {vY_[0], vY_[1], vY_[2], vK_[0], vK_[1], vK_[2], vYP_[0], vYP_[1], vYP_[2], vKP_[0],
@@ -290,7 +293,8 @@ GmArgs Solver::gm_args(int stage) const {
     GmArgs a{};
     a.M = M_;
     a.b = b_;
-    a.x = mu_;
+    a.x = gmx0_;
+    a.sol = mu_;
     a.V = gmV_;
     a.w = gmw_;
     a.part = gmpart_;
@@ -309,33 +313,44 @@ GmArgs Solver::gm_args(int stage) const {
 
 // Arnoldi steps of the GMRES solve of all surfaces at once (gridDim.z = surface).  The initial
 // residual comes from the assembly kernel and the final step applies the correction x += V y,
-// so only the fixed-length sequence of steps is launched here.
-void Solver::enqueue_gmres(int stage) {
+// so only the fixed-length sequence of steps is launched here.  With two launches or more the
+// last one is gm_close_mu_kernel, which also forms mu' (returns true: no mu_circ_kernel).
+bool Solver::enqueue_gmres(int stage) {
     const GmArgs a = gm_args(stage);
     double nsq = 0.0;
     for (int v : n_) nsq += (double)v * v;
     const dim3 grid(gm_nwg_, 1, S_);
     const size_t first = timed_.size();
-    for (int j = 0; j < a.m; ++j) {
+    const bool fused = a.m >= 2;
+    const int nprod = fused ? a.m - 1 : a.m;
+    for (int j = 0; j < nprod; ++j) {
         // finishing step j - 1 (the basis rebuilt from w_{j-1} and V[0..j-1]) + the product with
         // the stored matrix + the partial dots
         launch(
             "gm_iter", 2.0 * nsq + 4.0 * (j + 1) * T_, 8.0 * nsq + 8.0 * (j + 3) * T_,
             [&] {
-                const dim3 g = (j == a.m - 1) ? dim3((maxN_ + 31) / 32, 1, S_) : grid;   // kCloseRows
-                hipLaunchKernelGGL(gm_iter_kernel, g, dim3(256), 0, stream_, a, (const SurfDesc*)d_desc_, j);
+                hipLaunchKernelGGL(gm_iter_kernel, grid, dim3(256), 0, stream_, a, (const SurfDesc*)d_desc_, j);
             },
             j);
     }
+    if (fused) {
+        // finishing the last step + x += V y + mu' = D1 mu: the rebuild and the circulant product
+        launch("gm_close_mu", 4.0 * a.m * T_ + 4.0 * nsq, 8.0 * (a.m + 2) * T_ + 24.0 * nsq / kCircRowsPerWG, [&] {
+            hipLaunchKernelGGL(gm_close_mu_kernel, dim3((maxN_ + kCircRowsPerWG - 1) / kCircRowsPerWG, S_), dim3(256),
+                               0, stream_, a, (const SurfDesc*)d_desc_, (const double*)d_circ_, mup_, a.m - 1);
+        });
+    }
     if (ktime_) {
-        // launches 0 .. iters-1 ran a product ("gm_iter"); launch iters only finished the last step
-        // and applied the correction ("gm_close": no product); later ones returned at once
+        // launches 0 .. iters-1 ran a product ("gm_iter"); a gm_iter_kernel launch iters only
+        // finished the last step and stored the solution ("gm_close": no product); later ones
+        // returned at once.  The fused last launch ("gm_close_mu") always forms mu'.
         std::vector<int> ctl((size_t)kGmCtlInts * S_);
         SFH_HIP(hipStreamSynchronize(stream_));
         SFH_HIP(hipMemcpy(ctl.data(), gmctl_, ctl.size() * sizeof(int), hipMemcpyDeviceToHost));
         int iters = 0;
         for (int s = 0; s < S_; ++s) iters = std::max(iters, ctl[(size_t)kGmCtlInts * s + 2]);
         for (size_t k = first; k < timed_.size(); ++k) {
+            if (timed_[k].tag != std::string("gm_iter")) continue;
             if (timed_[k].iter == iters) {
                 timed_[k].tag = "gm_close";
                 timed_[k].flops = 4.0 * (iters + 1) * T_;
@@ -345,14 +360,13 @@ void Solver::enqueue_gmres(int stage) {
             }
         }
     }
+    return fused;
 }
 
-// M mu = b for every surface; M, b (and its working copy) are ready in stream order.
-void Solver::enqueue_solve(const double2* Z, int stage) {
-    if (opt_.solver == kSolveGMRES) {
-        enqueue_gmres(stage);
-        return;
-    }
+// M mu = b for every surface; M, b (and its working copy) are ready in stream order.  Returns
+// true when mu' has been formed too.
+bool Solver::enqueue_solve(const double2* Z, int stage) {
+    if (opt_.solver == kSolveGMRES) return enqueue_gmres(stage);
     auto run = [&](int s0, int count, int n) {
         LuArgs a;
         a.A = M_ + desc_[s0].moff;
@@ -394,6 +408,7 @@ void Solver::enqueue_solve(const double2* Z, int stage) {
                        (const NodeTrig*)trig_, Z, Zp_, Zpp_, M_, ph_, (const int*)status_, GmArgs{}, 0);
     hipLaunchKernelGGL(gepp_fallback_kernel, dim3(S_), dim3(256), 0, stream_, d_desc_, M_, b_, mu_, piv_,
                        status_, counters_);
+    return false;
 }
 
 // One launch for the whole right-hand side (and stage update) of every surface.
@@ -453,11 +468,12 @@ void Solver::enqueue_rhs(const double2* Z, const double* P, const StageCtl& ctl)
         hipLaunchKernelGGL(assemble64_kernel, dim3((maxN_ + kPairRows - 1) / kPairRows, S_), dim3(256), 0, stream_, d_desc_,
                            (const NodeTrig*)trig_, Z, Zp_, Zpp_, M_, ph_, (const int*)nullptr, ga, gm_on);
     });
-    enqueue_solve(Z, ctl.stage);
-    launch("mu_deriv", 4.0 * nsq, 24.0 * nsq / kCircRowsPerWG, [&] {
-        hipLaunchKernelGGL(mu_circ_kernel, cgrid, dim3(256), 0, stream_, d_desc_, (const double*)d_circ_,
-                           (const double*)mu_, mup_);
-    });
+    if (!enqueue_solve(Z, ctl.stage)) {
+        launch("mu_deriv", 4.0 * nsq, 24.0 * nsq / kCircRowsPerWG, [&] {
+            hipLaunchKernelGGL(mu_circ_kernel, cgrid, dim3(256), 0, stream_, d_desc_, (const double*)d_circ_,
+                               (const double*)mu_, mup_);
+        });
+    }
     launch("velocity", img * nsq, 96.0 * T_, [&] {
         hipLaunchKernelGGL(velocity_stage_kernel, dim3((maxN_ + kPairRows - 1) / kPairRows, S_), dim3(256), 0, stream_, d_desc_,
                            (const NodeTrig*)trig_, Z, Zp_, Zpp_, mu_, mup_, ph_, ctl);

@@ -104,7 +104,7 @@ class Solver {
 
   private:
     void enqueue_rhs(const double2* Z, const double* P, const StageCtl& ctl);
-    void enqueue_solve(const double2* Z, int stage);
+    bool enqueue_solve(const double2* Z, int stage);
     void enqueue_step(double dt);
     void capture(double dt, int len);
     int step_len();                 // GMRES launches per solve for the next step (trimming policy)
@@ -154,9 +154,10 @@ class Solver {
     // GMRES workspace
     int gm_m_ = 0, gm_nwg_ = 0;
     double *gmV_ = nullptr, *gmw_ = nullptr, *gmpart_ = nullptr, *gmsmall_ = nullptr;
+    double* gmx0_ = nullptr;     // initial guess of the solve in flight (the solution goes to mu_)
     int* gmctl_ = nullptr;
     long long gm_part_stride_ = 0, gm_small_stride_ = 0;
-    void enqueue_gmres(int stage);
+    bool enqueue_gmres(int stage);
     GmArgs gm_args(int stage) const;
     double* gmhist_ = nullptr;   // 4 kHistDepth x total: per RK stage, mu of the last steps
     int* gmhvalid_ = nullptr;

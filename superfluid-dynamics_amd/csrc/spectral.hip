@@ -20,10 +20,6 @@
 namespace sfh {
 
 constexpr int kCircRows = kCircRowsPerWG;   // rows per workgroup: wave w owns rows w and w + 4
-constexpr int kCircChunk = 2048;   // columns staged per LDS round
-constexpr int kCircWin = kCircChunk + kCircRows - 1;
-constexpr int kCircPer = kCircChunk / 256;               // staged samples per thread
-constexpr int kCircWinPer = (kCircWin + 255) / 256;      // staged kernel entries per thread
 
 struct CircLds {
     double2 z[kCircChunk];   // samples of the chunk (Z minus its linear part; mu in .x)
@@ -31,13 +27,6 @@ struct CircLds {
     double2 c1[kCircWin];    // first-derivative kernel over the chunk's window of k - j
     double c2[kCircWin];     // second-derivative kernel over the same window
 };
-
-// Window entry u holds c[(k0 - j0 - (kCircChunk - 1) + u) mod n]; row r, column c of the chunk
-// read entry r - c + kCircChunk - 1.
-__device__ inline int circ_index(int k0, int j0, int u, int n) {
-    int q = (k0 - j0 - (kCircChunk - 1) + u) % n;
-    return q < 0 ? q + n : q;
-}
 
 // Z' (field scale 2 pi / n, plus the slope of the linear part), Z'' (4 pi^2 / n^2) and
 // b = Re Phi' (2 pi / n, plus the uniform-flow term) for kCircRows rows of one surface;
@@ -71,6 +60,25 @@ __global__ __launch_bounds__(256) void zphi_circ_kernel(const SurfDesc* __restri
     const double* c2 = c1i + n;
     const double xs = 2.0 * kPi / n;                       // Derivatives.cuh:285,317
     const double plin = -(1 + ph.rho) * kPi * ph.U / n;    // Derivatives.cuh:286
+    // the own rows' inputs of the half-angle table and of the initial guess (threads t < kCircRows),
+    // loaded before the products so that their latency overlaps them
+    const int kt = min(k0 + t, n - 1);
+    double2 zk = {0.0, 0.0};
+    int hv = 0;
+    double hk[kHistDepth] = {}, solk = 0.0;
+    if (t < kCircRows) {
+        zk = Z[off + kt];
+        if (gm_on) {
+            hv = gm.stage > 0 ? gm.hvalid[gm.stage - 1] : 0;
+            if (hv > 0) {
+                const double* h0 = gm.hist + (long long)(gm.stage - 1) * kHistDepth * gm.total + off + kt;
+#pragma unroll
+                for (int l = 0; l < kHistDepth; ++l) hk[l] = l < hv ? h0[(long long)l * gm.total] : 0.0;
+            } else {
+                solk = gm.sol[off + kt];
+            }
+        }
+    }
     double2 zp0 = {0.0, 0.0}, zp1 = {0.0, 0.0}, zq0 = {0.0, 0.0}, zq1 = {0.0, 0.0};
     double pp0 = 0.0, pp1 = 0.0;
     for (int j0 = 0; j0 < n; j0 += kCircChunk) {
@@ -85,13 +93,13 @@ __global__ __launch_bounds__(256) void zphi_circ_kernel(const SurfDesc* __restri
                 zv[e] = c < jn ? Z[off + j0 + c] : make_double2(0.0, 0.0);
                 pv[e] = c < jn ? phi[off + j0 + c] : 0.0;
             }
+            int q[kCircWinPer];
+            circ_window(k0, j0, n, q);
 #pragma unroll
             for (int e = 0; e < kCircWinPer; ++e) {
-                const int u = t + 256 * e;
-                const int q = circ_index(k0, j0, u < kCircWin ? u : 0, n);
-                ar[e] = c1r[q];
-                ai[e] = c1i[q];
-                bv[e] = c2[q];
+                ar[e] = c1r[q[e]];
+                ai[e] = c1i[q[e]];
+                bv[e] = c2[q[e]];
             }
 #pragma unroll
             for (int e = 0; e < kCircPer; ++e) {
@@ -153,16 +161,10 @@ __global__ __launch_bounds__(256) void zphi_circ_kernel(const SurfDesc* __restri
     }
     if (t < kCircRows && k0 + t < n) {
         const int k = k0 + t;
-        if (trig) trig[off + k] = node_trig(Z[off + k]);
+        if (trig) trig[off + k] = node_trig(zk);
         // GMRES initial guess of this RK stage: its mu over the previous steps extrapolated
-        // (error O(dt^4) once four are kept), else the last solve's mu (kept in place)
-        if (gm_on && gm.stage > 0) {
-            const int hv = gm.hvalid[gm.stage - 1];
-            if (hv > 0) {
-                const double* h0 = gm.hist + (long long)(gm.stage - 1) * kHistDepth * gm.total + off;
-                gm.x[off + k] = extrapolate_history(h0 + k, gm.total, hv);
-            }
-        }
+        // (error O(dt^4) once four are kept), else the last solve's mu
+        if (gm_on) gm.x[off + k] = hv > 0 ? extrapolate_values(hk, hv) : solk;
     }
 }
 
@@ -192,12 +194,12 @@ __global__ __launch_bounds__(256) void mu_circ_kernel(const SurfDesc* __restrict
                 const int c = t + 256 * e;
                 mv[e] = c < jn ? mu[off + j0 + c] : 0.0;
             }
+            int q[kCircWinPer];
+            circ_window(k0, j0, n, q);
 #pragma unroll
             for (int e = 0; e < kCircWinPer; ++e) {
-                const int u = t + 256 * e;
-                const int q = circ_index(k0, j0, u < kCircWin ? u : 0, n);
-                ar[e] = c1r[q];
-                ai[e] = c1i[q];
+                ar[e] = c1r[q[e]];
+                ai[e] = c1i[q[e]];
             }
 #pragma unroll
             for (int e = 0; e < kCircPer; ++e) {

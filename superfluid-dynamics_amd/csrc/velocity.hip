@@ -50,6 +50,30 @@ __global__ __launch_bounds__(256) void velocity_stage_kernel(const SurfDesc* __r
     for (int r = 0; r <= R; ++r) sx[r] = sy[r] = 0.0;
 #pragma unroll
     for (int r = 0; r < R; ++r) ix[r] = iy[r] = 0.0;
+    // the finishing threads' own inputs (and those of node 1 for the quirk), loaded before the
+    // sweep so that their latency overlaps it
+    const int kt = min(k0 + min(t, R - 1), n - 1), gt = d.voff + kt;
+    double2 zp_t = {0.0, 0.0}, zpp_t = {0.0, 0.0}, mup_t = {0.0, 0.0};
+    double m_t = 0.0, y_t = 0.0;
+    StageIn st{};
+    double2 zp_1 = {0.0, 0.0}, zpp_1 = {0.0, 0.0}, mup_1 = {0.0, 0.0};
+    double m_1 = 0.0, y_1 = 0.0;
+    if (t < R) {
+        zp_t = Zp[gt];
+        zpp_t = Zpp[gt];
+        mup_t = mup[gt];
+        m_t = m[kt];
+        y_t = z[kt].y;
+        st = stage_load(ctl, gt);
+        if (quirk) {
+            const int g1 = d.voff + 1;
+            zp_1 = Zp[g1];
+            zpp_1 = Zpp[g1];
+            mup_1 = mup[g1];
+            m_1 = m[1];
+            y_1 = z[1].y;
+        }
+    }
     const int shift = (int)(((long long)blockIdx.x * 64) % n);
     for (int j0 = 0; j0 < n; j0 += 8 * 256) {
         NodeTrig tj[8];
@@ -117,18 +141,17 @@ __global__ __launch_bounds__(256) void velocity_stage_kernel(const SurfDesc* __r
     if (t >= R || k0 + t >= n) return;
     const int k = k0 + t, g = d.voff + k;
     double2 q1, q2;
-    velocity_from_sums(ph, Zp[g], Zpp[g], mup[g], m[k], z[k].y, so[t], so[R + t], ph.image ? io[t] : 0.0,
+    velocity_from_sums(ph, zp_t, zpp_t, mup_t, m_t, y_t, so[t], so[R + t], ph.image ? io[t] : 0.0,
                        ph.image ? io[R + t] : 0.0, q1, q2);
     double2 q1q = q1;
     if (quirk && k != 1) {
-        const int g1 = d.voff + 1;
         double2 u2;
-        velocity_from_sums(ph, Zp[g1], Zpp[g1], mup[g1], m[1], z[1].y, qo[0], qo[1], 0.0, 0.0, q1q, u2);
+        velocity_from_sums(ph, zp_1, zpp_1, mup_1, m_1, y_1, qo[0], qo[1], 0.0, 0.0, q1q, u2);
     }
     if (ctl.qup) ctl.qup[g] = q2;
     if (!ph.need_upper) q2 = make_double2(0.0, 0.0);
-    const double dphi = bernoulli(ph, z[k].y, q1, q2, q1q, Zp[g], Zpp[g]);
-    stage_update(ctl, g, q1, dphi);
+    const double dphi = bernoulli(ph, y_t, q1, q2, q1q, zp_t, zpp_t);
+    stage_store(ctl, g, q1, dphi, st);
 }
 
 }  // namespace sfh
