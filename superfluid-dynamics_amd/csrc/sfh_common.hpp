@@ -162,9 +162,27 @@ __device__ inline double2 cot_image(const NodeTrig& k, const NodeTrig& j, double
 }
 
 // ------------------------------------------------------------------ wave reductions
+// v taken from another lane of the same row of 16 by a DPP control (both halves of the double).
+template <int CTL>
+__device__ inline double dpp_double(double v) {
+    const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), CTL, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), CTL, 0xf, 0xf, false);
+    return __hiloint2double(hi, lo);
+}
+__device__ inline double read_lane(double v, int l) {
+    return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), l),
+                            __builtin_amdgcn_readlane(__double2loint(v), l));
+}
+// Sum over the 64 lanes of a wave (every lane active), the same value in every lane: inside each
+// row of 16 by DPP (quad_perm xor 1, xor 2, half-row mirror, row mirror; a lane adds the
+// partner's partial sum to its own, and both orders of the two operands give the same double),
+// then the four row sums read as scalars and added in a fixed order.
 __device__ inline double wave_sum(double v) {
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
+    v += dpp_double<0xB1>(v);    // quad_perm [1,0,3,2]
+    v += dpp_double<0x4E>(v);    // quad_perm [2,3,0,1]
+    v += dpp_double<0x141>(v);   // row_half_mirror
+    v += dpp_double<0x140>(v);   // row_mirror
+    return (read_lane(v, 0) + read_lane(v, 16)) + (read_lane(v, 32) + read_lane(v, 48));
 }
 
 // Sums of NV per-thread values over a 256-thread workgroup, into out[0..NV) (LDS; valid after the
