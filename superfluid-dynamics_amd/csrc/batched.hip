@@ -9,7 +9,8 @@
 //   A  spectral derivatives in LDS: Z' and Z'' from ONE forward transform of Z - 2 pi j / N,
 //      Phi' from a second (Derivatives.cuh:310-384), plus the per-node half-angle table;
 //   B  the dipole matrix M = I/2 + K (createM.cuh:43-92) into HBM, column-major per surface
-//      (element (k, j) at j N + k, the reference's own layout), FP64 or FP32;
+//      (element (k, j) at j N + k, the reference's own layout), FP64 or FP32 (FP64 entries
+//      rounded), and in the same pair loop M x for the warm start x (the first residual);
 //   C  the solve M mu = b, warm-started from the surface's previous mu:
 //        FP64  : GMRES on the stored FP64 matrix (classical Gram-Schmidt with one
 //                re-orthogonalisation, restarted cycles of kBK vectors);
@@ -48,33 +49,61 @@ __global__ __launch_bounds__(1024) void batched_stage_kernel(BatchArgs a, Phys p
         a.b[off + t] = bk;
     }
 
+    // warm start: this stage's mu extrapolated over the previous steps (error O(dt^4) once four
+    // are kept), else the previous solve's mu
+    double x = own ? a.mu[off + t] : 0.0;
+    double* h0 = nullptr;
+    int hv = 0;
+    if (ctl.stage > 0 && a.hist) {
+        hv = a.hvalid[(ctl.stage - 1) * a.nsurf + s];
+        h0 = a.hist + (long long)(ctl.stage - 1) * kHistDepth * a.total + off;
+        if (own && hv >= 1) x = extrapolate_history(h0 + t, a.total, hv);
+    }
+
     // ---------------------------------------------------------------- B: assembly
+    // The pair loop also forms (M x)_k for the warm start x in FP64, the first residual of the
+    // solve: FP64 from the stored entries, in the order of mv_stored; mixed from the FP64 entries
+    // that are rounded to FP32 for storage, in the order of mv_free.
     const bool water = ph.kind == 0;
     const PairConst pc = wg_diagonal(ph, nv, n, L);
+    if (own) L.vs[t] = x;
+    __syncthreads();
+    double mval = 0.0;
     if (t < G * n) {
         const int k = t % n, g = t / n;
         const NodeTrig tk = L.tr[k];
         const double2 zq = L.zp[k];
+        const double czx = pc.c * zq.x, czy = pc.c * zq.y;
         if (a.mixed) {
             float* col = reinterpret_cast<float*>(a.M) + d.moff + k;
-            const float cif = (float)pc.ci;
-            const float czx = (float)(pc.c * zq.x), czy = (float)(pc.c * zq.y);
+            double sx = 0.0, sy = 0.0, si = 0.0;
             for (int j = g; j < n; j += G) {
                 float v;
                 if (j == k) {
                     v = (float)L.diag[k];
                 } else {
                     const NodeTrig tj = L.tr[j];
-                    const float3 cp = cot_parts_mixed(tk, tj);   // c Im(Z'_k cot) = id (c y' p - c x' q)
-                    v = cp.z * (czy * cp.x - czx * cp.y);
-                    if (pc.image) v -= cif * cot_image_mixed_im(tk, tj, pc.eh, pc.ehi);
+                    const double vj = L.vs[j];
+                    const CotParts cp = cot_parts(tk, tj);   // c Im(Z'_k cot) = id (c y' p - c x' q)
+                    const double w = vj * cp.id;
+                    sx += w * cp.p;
+                    sy -= w * cp.q;
+                    double e = cp.id * (czy * cp.p - czx * cp.q);
+                    if (pc.image) {
+                        const CotParts cq = cot_image_parts(tk, tj, pc.eh, pc.ehi);
+                        si -= vj * (cq.q * cq.id);
+                        e += pc.ci * (cq.q * cq.id);
+                    }
+                    v = (float)e;
                 }
                 col[(size_t)j * n] = v;
             }
+            mval = pc.c * (zq.x * sy + zq.y * sx) - pc.ci * si;
         } else {
             double* col = reinterpret_cast<double*>(a.M) + d.moff + k;
-            const double czx = pc.c * zq.x, czy = pc.c * zq.y;
-            for (int j = g; j < n; j += G) {
+            double acc0 = 0.0, acc1 = 0.0;
+            int u = 0;
+            for (int j = g; j < n; j += G, ++u) {
                 double v;
                 if (j == k) {
                     v = L.diag[k];
@@ -88,35 +117,41 @@ __global__ __launch_bounds__(1024) void batched_stage_kernel(BatchArgs a, Phys p
                     }
                 }
                 col[(size_t)j * n] = v;
+                if (u & 1)
+                    acc1 += v * L.vs[j];
+                else
+                    acc0 += v * L.vs[j];
             }
+            mval = acc0 + acc1;
         }
     }
+    L.part[t] = mval;
     __syncthreads();   // the workgroup reads back only its own columns: a barrier suffices
+    double mx0 = 0.0;   // (M x)_t
+    if (own) {
+        for (int q = 0; q < G; ++q) mx0 += L.part[q * n + t];
+        if (a.mixed) mx0 += L.diag[t] * L.vs[t];
+    }
+    __syncthreads();
 
     // ---------------------------------------------------------------- C: solve
-    // warm start: this stage's mu extrapolated over the previous steps (error O(dt^4) once four
-    // are kept), else the previous solve's mu
-    double x = own ? a.mu[off + t] : 0.0;
-    double* h0 = nullptr;
-    int hv = 0;
-    if (ctl.stage > 0 && a.hist) {
-        hv = a.hvalid[(ctl.stage - 1) * a.nsurf + s];
-        h0 = a.hist + (long long)(ctl.stage - 1) * kHistDepth * a.total + off;
-        if (own && hv >= 1) x = extrapolate_history(h0 + t, a.total, hv);
-    }
     const double bn = sqrt(bsum1(own ? bk * bk : 0.0, L));
     const double tol_abs = a.tol * bn;
     int unconv = 0, iters = 0, sweeps = 0;
     if (!a.mixed) {
         const double* Md = reinterpret_cast<const double*>(a.M) + d.moff;
-        iters = wg_gmres([&]() { return mv_stored(Md, n, G, L); }, n, bk, x, false, tol_abs, a.cycles, L, &unconv);
+        iters = wg_gmres([&]() { return mv_stored(Md, n, G, L); }, n, bk, x, false, tol_abs, a.cycles, L, &unconv,
+                         true, own ? bk - mx0 : 0.0);
     } else {
         const float* Mf = reinterpret_cast<const float*>(a.M) + d.moff;
         bool done = false;
         for (; sweeps < a.max_refine; ++sweeps) {
-            if (own) L.vs[t] = x;
-            __syncthreads();
-            const double mx = mv_free(n, G, pc, L);   // every thread: it synchronises
+            double mx = mx0;   // sweep 0: formed by the assembly
+            if (sweeps > 0) {
+                if (own) L.vs[t] = x;
+                __syncthreads();
+                mx = mv_free(n, G, pc, L);   // every thread: it synchronises
+            }
             const double r = own ? bk - mx : 0.0;
             const double rn = sqrt(bsum1(r * r, L));
             if (!(rn > tol_abs)) {

@@ -104,9 +104,6 @@ __device__ inline double fast_rcp(double d) {
     r = fma(fma(-d, r, 1.0), r, r);
     return r;
 }
-// FP32 counterpart (v_rcp_f32 is accurate to about 1 ulp of float): for the FP32 matrix of the
-// mixed-precision form.
-__device__ inline float fast_rcpf(float d) { return __builtin_amdgcn_rcpf(d); }
 
 // Per-node table for pairwise cotangents: sin(x/2), cos(x/2), exp(y/2) / 2, exp(-y/2).
 // With it cot((z_k - z_j)/2) = (sa ca - i sh ch) / (sa^2 + sh^2) needs no transcendental call:

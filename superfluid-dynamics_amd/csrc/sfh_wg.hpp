@@ -169,35 +169,21 @@ __device__ inline double mv_free(int n, int G, const PairConst& pc, BLds& L) {
     return w;
 }
 
-// cot((z_k - z_j)/2) with the differences of the half-angle table taken in FP64 (they carry the
-// cancellation next to the diagonal) and the rest in FP32.
-// Parts as in cot_parts: cot = (p id, -q id).
-__device__ inline float3 cot_parts_mixed(const NodeTrig& k, const NodeTrig& j) {
-    const float sa = (float)(k.s * j.c - k.c * j.s);
-    const float ca = (float)(k.c * j.c + k.s * j.s);
-    const double ep = k.e * j.ei, em = k.ei * j.e;
-    const float sh = (float)(ep - em), ch = (float)(ep + em);
-    return make_float3(sa * ca, sh * ch, fast_rcpf(sa * sa + sh * sh));
-}
-__device__ inline float cot_image_mixed_im(const NodeTrig& k, const NodeTrig& j, double eh, double ehi) {
-    const float sa = (float)(k.s * j.c - k.c * j.s);
-    const double ep = k.e * j.e * (2.0 * eh), em = k.ei * j.ei * (0.5 * ehi);
-    const float sh = (float)(ep - em), ch = (float)(ep + em);
-    return -(sh * ch) * fast_rcpf(sa * sa + sh * sh);
-}
-
 // Restarted GMRES on A x = rhs over the workgroup; x (thread t < n holds x_t) is updated in
-// place.  x_zero: x is known to be 0 (the first residual is rhs, no product needed).  Returns
-// the number of Arnoldi steps; *unconv is set when tol_abs was not reached.
+// place.  x_zero: x is known to be 0 (the first residual is rhs, no product needed); r_given:
+// the first residual rhs - A x is r_first (no product needed either).  Returns the number of
+// Arnoldi steps; *unconv is set when tol_abs was not reached.
 template <class MV>
 __device__ int wg_gmres(MV&& mv, int n, double rhs, double& x, bool x_zero, double tol_abs, int cycles,
-                        BLds& L, int* unconv) {
+                        BLds& L, int* unconv, bool r_given = false, double r_first = 0.0) {
     const int t = threadIdx.x;
     const bool own = t < n;
     int total = 0;
     for (int cyc = 0; cyc < cycles; ++cyc) {
         double r = own ? rhs : 0.0;
-        if (!(x_zero && cyc == 0)) {
+        if (cyc == 0 && r_given) {
+            r = own ? r_first : 0.0;   // the caller formed rhs - M x already
+        } else if (!(x_zero && cyc == 0)) {
             if (own) L.vs[t] = x;
             __syncthreads();
             const double w = mv();
