@@ -87,86 +87,117 @@ __device__ inline void store_block(double* dst, const double* src, long long ld,
 // Factors D (kb x kb, no interchanges) into L\U and writes L^{-1} into Li and U^{-1} into Ui
 // (both zero outside the kb x kb corner).  Sets *bad if a multiplier exceeds 1 in magnitude.
 //
-// Thread t owns row r = t / 4 of the 64 x 64 block (rows from kb on are padded with identity rows)
-// and, in its register slots, the columns part + 4 (m + q) of that row, part = t % 4, where q is
-// the group of four elimination steps in progress: after every group the finished column (slot
-// 0) goes to LDS and the slots shift down by one, so the pivot column of step j = 4 q + p is
-// always slot 0 of the row's lane with part p and reaches the row's other lanes by a quad
-// broadcast (DPP).  A group publishes its four pivot rows through a double-buffered LDS block (one
-// barrier per four steps; elim_group) and all of a thread's pivot-row entries are read before any
-// is used.  The group loop stays rolled (no dynamic register index anywhere), so the code is a
-// few KB.
-//
+// The elimination runs on 4 x 4 tiles: thread t owns tile (t / 16, t % 16) of the 64 x 64 block
+// (rows from kb on are padded with identity rows) in registers.  Block step kk (16 of them, two
+// barriers each):
+//   the diagonal tile's thread factors it (getf2 order: multipliers by the reciprocal pivot) and
+//   publishes L_kk \ U_kk and the reciprocals;
+//   the tiles below it form L_ik = A_ik U_kk^{-1}, the tiles right of it U_kj = L_kk^{-1} A_kj,
+//   and publish them;
+//   every trailing tile applies A_ij -= L_ik U_kj.
+// Every entry sees the same multiply-adds in the same order as in the unblocked right-looking
+// elimination (step p's update of entry (r, c), for p = 0 .. min(r, c) - 1), so the factors are
+// those of getf2 without interchanges.  Per step a thread reads 32 published values, against 80
+// for the previous row-slot layout, whose LDS traffic bounded it.
 // The inverses then follow in blocks (invert_factors).
-__device__ inline double quad_bcast(double v, int p) {
-    // value of lane 4 (lane / 4) + p, for a lane-uniform p in 0..3 (quad_perm DPP)
-    const int ctl = p | (p << 2) | (p << 4) | (p << 6);
-    int lo = __double2loint(v), hi = __double2hiint(v);
-    switch (ctl) {
-        case 0x00: lo = __builtin_amdgcn_mov_dpp(lo, 0x00, 0xf, 0xf, false); hi = __builtin_amdgcn_mov_dpp(hi, 0x00, 0xf, 0xf, false); break;
-        case 0x55: lo = __builtin_amdgcn_mov_dpp(lo, 0x55, 0xf, 0xf, false); hi = __builtin_amdgcn_mov_dpp(hi, 0x55, 0xf, 0xf, false); break;
-        case 0xaa: lo = __builtin_amdgcn_mov_dpp(lo, 0xaa, 0xf, 0xf, false); hi = __builtin_amdgcn_mov_dpp(hi, 0xaa, 0xf, 0xf, false); break;
-        default:   lo = __builtin_amdgcn_mov_dpp(lo, 0xff, 0xf, 0xf, false); hi = __builtin_amdgcn_mov_dpp(hi, 0xff, 0xf, 0xf, false); break;
-    }
-    return __hiloint2double(hi, lo);
-}
-
-// One elimination step j = 4 q + P (P a compile-time constant) on the shifted row slots.
-// One group of four elimination steps j = 4 q .. 4 q + 3, with one barrier: the four pivot rows
-// are published as they stand before the group, and every thread carries them through the
-// group's earlier steps itself (at its own 16 columns and on the 4 x 4 block of the pivot
-// columns), with the same operations the rows' owners apply.  Own-row step j then uses the
-// updated pivot row; the row's entry in column j comes from slot 0 of the lane with part j % 4.
-template <int LAST>
-__device__ inline void elim_group(double (&a)[16], double (*P4)[4][kNB], int q, int r, int part, int kb,
-                                  int& my_bad) {
-    double (*pr)[kNB] = P4[q & 1];
-    const int j0 = 4 * q;
-    if (r >= j0 && r < j0 + 4) {
+__device__ inline void eliminate_tiles(double* D, int kb, int& my_bad) {
+    __shared__ double Dk[16], Dinv[4];             // the diagonal tile of the step, its reciprocals
+    __shared__ double Lp[16][16], Up[16][16];      // the step's L_ik (by i) and U_kj (by j) tiles
+    const int t = threadIdx.x, ti = t >> 4, tj = t & 15;
+    double A[4][4];
 #pragma unroll
-        for (int m = 0; m < 16; ++m)
-            if (m + q < 16) pr[r - j0][part + 4 * (m + q)] = a[m];
-    }
-    __syncthreads();
-    double pv[4][16], B[4][4];
+    for (int r = 0; r < 4; ++r)
 #pragma unroll
-    for (int P = 0; P < 4; ++P) {
-#pragma unroll
-        for (int m = 0; m < 16; ++m) pv[P][m] = (m + q < 16) ? pr[P][part + 4 * (m + q)] : 0.0;
-#pragma unroll
-        for (int c = 0; c < 4; ++c) B[P][c] = pr[P][j0 + c];
-    }
-    // pivot rows P through the group's steps P2 < P; multipliers by the reciprocal of the pivot
-    // (as getf2 scales a column), one reciprocal per pivot
-    double inv[4];
-    inv[0] = fast_rcp(B[0][0]);
-#pragma unroll
-    for (int P = 1; P < 4; ++P) {
-#pragma unroll
-        for (int P2 = 0; P2 < P; ++P2) {
-            const double mlt = B[P][P2] * inv[P2];
-#pragma unroll
-            for (int c = P2 + 1; c < 4; ++c) B[P][c] = fma(-mlt, B[P2][c], B[P][c]);
-            // slot 0 (column 4 q + part) changes only past the step's column, as in the owner's row
-            pv[P][0] = part > P2 ? fma(-mlt, pv[P2][0], pv[P][0]) : pv[P][0];
-#pragma unroll
-            for (int m = 1; m < 16; ++m) pv[P][m] = fma(-mlt, pv[P2][m], pv[P][m]);
+        for (int c = 0; c < 4; ++c) {
+            const int gr = 4 * ti + r, gc = 4 * tj + c;
+            A[r][c] = gr < kb ? D[gr * LD + gc] : (gc == gr ? 1.0 : 0.0);
         }
-        inv[P] = fast_rcp(B[P][P]);
-    }
+#pragma unroll 1
+    for (int kk = 0; kk < 16; ++kk) {
+        if (ti == kk && tj == kk) {
+            double inv[4];
 #pragma unroll
-    for (int P = 0; P < 4 - LAST; ++P) {
-        const int j = j0 + P;
-        const double arj = quad_bcast(a[0], P);   // a[r][j] after the group's earlier steps
-        if (r > j) {
-            const double l = arj * inv[P];
-            if (r < kb && !(fabs(l) <= 1.0)) my_bad = 1;
-            // slot 0 holds column 4 q + part: updated when part > P, the multiplier when part == P
-            a[0] = part > P ? fma(-l, pv[P][0], a[0]) : (part == P ? l : a[0]);
+            for (int p = 0; p < 4; ++p) {
+                inv[p] = fast_rcp(A[p][p]);
 #pragma unroll
-            for (int m = 1; m < 16; ++m) a[m] = fma(-l, pv[P][m], a[m]);
+                for (int r = p + 1; r < 4; ++r) {
+                    const double l = A[r][p] * inv[p];
+                    if (4 * kk + r < kb && !(fabs(l) <= 1.0)) my_bad = 1;
+                    A[r][p] = l;
+#pragma unroll
+                    for (int c = p + 1; c < 4; ++c) A[r][c] = fma(-l, A[p][c], A[r][c]);
+                }
+            }
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+#pragma unroll
+                for (int c = 0; c < 4; ++c) Dk[4 * r + c] = A[r][c];
+#pragma unroll
+            for (int p = 0; p < 4; ++p) Dinv[p] = inv[p];
+        }
+        __syncthreads();
+        if (tj == kk && ti > kk) {   // L_ik = A_ik U_kk^{-1}, column by column
+            double U[4][4], inv[4];
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+#pragma unroll
+                for (int c = 0; c < 4; ++c) U[r][c] = Dk[4 * r + c];
+#pragma unroll
+            for (int p = 0; p < 4; ++p) inv[p] = Dinv[p];
+#pragma unroll
+            for (int c = 0; c < 4; ++c)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    double x = A[r][c];
+#pragma unroll
+                    for (int p = 0; p < c; ++p) x = fma(-A[r][p], U[p][c], x);
+                    const double l = x * inv[c];
+                    if (4 * ti + r < kb && !(fabs(l) <= 1.0)) my_bad = 1;
+                    A[r][c] = l;
+                }
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+#pragma unroll
+                for (int c = 0; c < 4; ++c) Lp[ti][4 * r + c] = A[r][c];
+        } else if (ti == kk && tj > kk) {   // U_kj = L_kk^{-1} A_kj, step by step
+            double L[4][4];
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+#pragma unroll
+                for (int c = 0; c < 4; ++c) L[r][c] = Dk[4 * r + c];
+#pragma unroll
+            for (int p = 0; p < 3; ++p)
+#pragma unroll
+                for (int r = p + 1; r < 4; ++r)
+#pragma unroll
+                    for (int c = 0; c < 4; ++c) A[r][c] = fma(-L[r][p], A[p][c], A[r][c]);
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+#pragma unroll
+                for (int c = 0; c < 4; ++c) Up[tj][4 * r + c] = A[r][c];
+        }
+        __syncthreads();
+        if (ti > kk && tj > kk) {   // A_ij -= L_ik U_kj, the four steps in order
+            double L[4][4], U[4][4];
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    L[r][c] = Lp[ti][4 * r + c];
+                    U[r][c] = Up[tj][4 * r + c];
+                }
+#pragma unroll
+            for (int p = 0; p < 4; ++p)
+#pragma unroll
+                for (int r = 0; r < 4; ++r)
+#pragma unroll
+                    for (int c = 0; c < 4; ++c) A[r][c] = fma(-L[r][p], U[p][c], A[r][c]);
         }
     }
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) D[(4 * ti + r) * LD + 4 * tj + c] = A[r][c];
 }
 
 // One level of the blocked inversion: for every pair of H x H diagonal blocks, T = C A^{-1}
@@ -278,25 +309,10 @@ __device__ void invert_factors(const double* D, double* Li, double* Ui, double* 
 }
 
 __device__ void factor_invert_diag(double* D, double* Li, double* Ui, double* T, int kb, int* bad) {
-    __shared__ double P4[2][4][kNB];   // the pivot rows of a group of four steps, by group parity
     const int t = threadIdx.x, r = t >> 2, part = t & 3;
-    double a[16];
-#pragma unroll
-    for (int m = 0; m < 16; ++m) {
-        const int c = part + 4 * m;
-        a[m] = r < kb ? D[r * LD + c] : (c == r ? 1.0 : 0.0);
-    }
     int my_bad = 0;
-    __syncthreads();   // every thread has read D before the finished columns overwrite it
     // ---- elimination: D becomes L (strictly below the diagonal) \ U, padded rows identity
-#pragma unroll 1
-    for (int q = 0; q < 16; ++q) {
-        if (q < 15) elim_group<0>(a, P4, q, r, part, kb, my_bad);
-        else elim_group<1>(a, P4, q, r, part, kb, my_bad);   // step 63 has nothing below
-        D[r * LD + part + 4 * q] = a[0];   // column 4 q + part is final
-#pragma unroll
-        for (int m = 0; m < 15; ++m) a[m] = a[m + 1];
-    }
+    eliminate_tiles(D, kb, my_bad);
     if (my_bad) *bad = 1;
     __syncthreads();
     invert_factors(D, Li, Ui, T);
