@@ -256,6 +256,51 @@ __device__ inline void combine_level(const double* D, double* Li, double* Ui, do
     __syncthreads();
 }
 
+// C[m0.., n0..] (16 x 16) = sign * sum_{k < K} A[m0 + m][a0 + k] B[b0 + k][n0 + n] on one wave with
+// v_mfma_f64_16x16x4 (lane layout as in mfma_block); A, B, C in LDS with the given leading dimensions.
+template <int K>
+__device__ inline void mfma_tile16(const double* A, int lda, const double* B, int ldb, double* C, int ldc,
+                                   double sign, int lane) {
+    const int l16 = lane & 15, l4 = lane >> 4;
+    v4d acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int kk = 0; kk < K; kk += 4) acc = mfma_f64(A[l16 * lda + kk + l4], B[(kk + l4) * ldb + l16], acc);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) C[(4 * r + l4) * ldc + l16] = sign * acc[r];
+}
+
+// combine_level for H = 16 and 32 on the matrix cores: the same two products per pair and factor,
+// as 16 x 16 output tiles, one or two per wave.
+template <int H>
+__device__ inline void combine_level_mfma(const double* D, double* Li, double* Ui, double* T) {
+    static_assert(H == 16 || H == 32, "16 x 16 tiles");
+    constexpr int pairs = kNB / (2 * H), per = H * H, tpp = (H / 16) * (H / 16);   // tiles per product
+    constexpr int tiles = pairs * 2 * tpp;                                           // 8 (H = 32) or 4 (H = 16)
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    // T = C A^{-1} (lower) and E B^{-1} (upper), T of pair p and factor lu at T + (2 p + lu) per
+    for (int q = wv; q < tiles; q += 4) {
+        const int p = q / (2 * tpp), lu = (q / tpp) & 1, tt = q % tpp;
+        const int tr = 16 * (tt / (H / 16)), tc = 16 * (tt % (H / 16)), o0 = 2 * H * p, o1 = o0 + H;
+        double* Tb = T + (2 * p + lu) * per;
+        if (lu == 0)
+            mfma_tile16<H>(D + (o1 + tr) * LD + o0, LD, Li + o0 * LD + o0 + tc, LD, Tb + tr * H + tc, H, 1.0, lane);
+        else
+            mfma_tile16<H>(D + (o0 + tr) * LD + o1, LD, Ui + o1 * LD + o1 + tc, LD, Tb + tr * H + tc, H, 1.0, lane);
+    }
+    __syncthreads();
+    // the off-diagonal blocks: -B^{-1} T (lower) and -A^{-1} T (upper)
+    for (int q = wv; q < tiles; q += 4) {
+        const int p = q / (2 * tpp), lu = (q / tpp) & 1, tt = q % tpp;
+        const int tr = 16 * (tt / (H / 16)), tc = 16 * (tt % (H / 16)), o0 = 2 * H * p, o1 = o0 + H;
+        const double* Tb = T + (2 * p + lu) * per;
+        if (lu == 0)
+            mfma_tile16<H>(Li + (o1 + tr) * LD + o1, LD, Tb + tc, H, Li + (o1 + tr) * LD + o0 + tc, LD, -1.0, lane);
+        else
+            mfma_tile16<H>(Ui + (o0 + tr) * LD + o0, LD, Tb + tc, H, Ui + (o0 + tr) * LD + o1 + tc, LD, -1.0, lane);
+    }
+    __syncthreads();
+}
+
 // Inverts the unit lower factor L and the upper factor U held together in D (L\U, 64 x 64) into
 // Li = L^{-1} and Ui = U^{-1}, blocked:
 //   the sixteen 8 x 8 diagonal blocks (eight of L, eight of U) by substitution, one lane per
@@ -304,8 +349,8 @@ __device__ void invert_factors(const double* D, double* Li, double* Ui, double* 
     }
     __syncthreads();
     combine_level<8>(D, Li, Ui, T);
-    combine_level<16>(D, Li, Ui, T);
-    combine_level<32>(D, Li, Ui, T);
+    combine_level_mfma<16>(D, Li, Ui, T);
+    combine_level_mfma<32>(D, Li, Ui, T);
 }
 
 __device__ void factor_invert_diag(double* D, double* Li, double* Ui, double* T, int kb, int* bad) {
