@@ -14,6 +14,9 @@ persistent launch per RK stage, mixed precision (FP32 matrix, FP64-refined mu) a
 compared with the plain FP64 stepper run over the same 512 surfaces one surface at a time:
 ensemble steps/s = 1 / sum_i t_step(N_i).
 
+Other legs: `c1` (N = 256, y = 0.1 cos x, dt = 0.01, 100 steps, LU and GMRES, error vs the oracle),
+`film` (N = 4096 helium film with the image term) and `adjoint` (gradient through the adjoint).
+
 Prints one JSON line.  `roofline` is measured live on the dominant kernel of the headline solver
 with hipEvents around each of its launches on the solver's stream; `cpu_baseline` times the CPU
 FP64 oracle (OpenMP, all host threads) on a bounded sample of the headline workload and also serves
@@ -143,6 +146,40 @@ def batched_leg(c, steps, with_accuracy):
     return out
 
 
+def c1_leg(c, with_accuracy):
+    """The reference's CPU-runnable case: N = 256, y = 0.1 cos x, phi = 0, water with g = 1 in deep
+    water, 100 RK4 steps of dt = 0.01 from a fresh state (for GMRES this includes the four steps that
+    fill the warm-start history and the graph captures), with the drop-in LU solver and with GMRES;
+    the error of the final surface against the FP64 oracle run over the same 100 steps."""
+    n, dt, steps = 256, 0.01, 100
+    x = 2 * np.pi * np.arange(n) / n
+    Z0 = x + 1j * 0.1 * np.cos(x)
+    P0 = np.zeros(n)
+    out = {"workload": "N=256 water surface y=0.1 cos x, phi=0, g=1, deep water, dt=0.01, 100 RK4 steps, FP64",
+           "nodes": n, "steps": steps}
+    finals = {}
+    for solver, name in ((c.LU, "lu"), (c.GMRES, "gmres")):
+        with c.Ensemble([n], c.Props(), solver=solver, tol=1e-12) as e:
+            e.set_state(Z0, P0)
+            e.step(dt, 2)   # warm-up: graph capture, first launches
+            e.set_state(Z0, P0)
+            e.sync()
+            t0 = time.perf_counter()
+            e.step(dt, steps)
+            e.sync()
+            el = time.perf_counter() - t0
+            finals[name] = e.get_state()
+        out[name] = {"steps_per_s": round(steps / el, 2), "ms_per_step": round(1e3 * el / steps, 4)}
+    if with_accuracy:
+        from oracle import oracle as O
+        O.build()
+        Zr, Pr = O.rk4(O.Props(), Z0, P0, dt, steps)
+        for name, (Zg, Pg) in finals.items():
+            out[name]["max_abs_dz_vs_oracle"] = float(np.abs(Zg - Zr).max())
+            out[name]["max_abs_dphi_vs_oracle"] = float(np.abs(Pg - Pr).max())
+    return out
+
+
 def film_leg(c, steps, with_accuracy):
     """Helium film over a flat substrate: N = 4096, depth h0 = 1 with the image term, restoring
     acceleration 3 A / (h0 + y)^4 with A = 0.05, i.e. g_eff = 3 A / h0^4 = 0.15 scaling the reference's
@@ -234,6 +271,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and accuracy legs")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched ensemble leg")
     ap.add_argument("--batched-steps", type=int, default=50)
+    ap.add_argument("--no-c1", action="store_true", help="skip the N=256 100-step leg")
     ap.add_argument("--no-film", action="store_true", help="skip the N=4096 helium film leg")
     ap.add_argument("--film-steps", type=int, default=50)
     ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiable-stepper leg")
@@ -319,6 +357,8 @@ def main():
     }
     if not args.no_batched:
         out["batched"] = batched_leg(c, args.batched_steps, not args.no_cpu)
+    if not args.no_c1:
+        out["c1"] = c1_leg(c, not args.no_cpu)
     if not args.no_film:
         out["film"] = film_leg(c, args.film_steps, not args.no_cpu)
     if not args.no_adjoint:
