@@ -105,7 +105,8 @@ typedef struct sfh_solver* sfh_handle;
 enum { SFH_WATER = 0, SFH_HELIUM = 1 };
 enum { SFH_FP64 = 0, SFH_MIXED = 1 };
 /* LU: blocked FP64 LU (the reference's getrf/getrs semantics), one surface at a time or uniform
- * batches.  GMRES: FP64 GMRES over the stored matrix, all surfaces of the handle at once.
+ * batches.  GMRES: FP64 GMRES over the stored matrix, all surfaces of the handle at once (every
+ * N <= 4096).
  * BATCHED: one persistent launch per RK stage, one workgroup per surface (every N <= 1024), GMRES
  * inside the workgroup; with SFH_MIXED the matrix is FP32 and mu is refined in FP64 against a
  * matrix-free FP64 residual. */

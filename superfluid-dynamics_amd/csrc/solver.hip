@@ -17,6 +17,7 @@
 #include <cmath>
 #include <cstring>
 
+#include "sfh_gmres.hpp"
 #include "solver.hpp"
 
 namespace sfh {
@@ -100,6 +101,9 @@ Solver::Solver(const Phys& ph, const std::vector<int>& n, const Options& opt) : 
     if (opt_.precision != kFP64 && opt_.precision != kMixed) throw ArgError("unknown precision");
     if (opt_.precision == kMixed && opt_.solver != kSolveBatched)
         throw ArgError("the mixed-precision form runs on the batched (per-surface workgroup) solver");
+    if (opt_.solver == kSolveGMRES)
+        for (int v : n)
+            if (v > kGmMaxN) throw ArgError("the GMRES solver takes surfaces of at most 4096 nodes");
     if (opt_.solver == kSolveBatched)
         for (int v : n)
             if (v > kBatchMaxN) throw ArgError("the batched solver takes surfaces of at most 1024 nodes");

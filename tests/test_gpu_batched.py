@@ -147,3 +147,6 @@ def test_batched_rejects_large_surfaces(csh):
         csh.Ensemble([2048], csh.Props(), solver=csh.BATCHED)
     with pytest.raises(Exception):
         csh.Ensemble([256], csh.Props(), precision=csh.MIXED, solver=csh.LU)
+    # the multi-workgroup GMRES keeps the vector being multiplied in LDS: at most 4096 nodes
+    with pytest.raises(Exception):
+        csh.Ensemble([8192], csh.Props(), solver=csh.GMRES)
