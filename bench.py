@@ -189,9 +189,12 @@ def film_leg(c, steps, with_accuracy):
     Z0 = surface(n, SEED + 5, amp=0.05)
     P0 = np.zeros(n)
     out = {"workload": "helium film N=4096, h0=1 (image term), g_eff=3A/h0^4=0.15 (A=0.05), 8-mode seeded "
-                       "surface a_k~U(0,0.05/k), phi=0, dt=0.002, FP64", "nodes": n}
-    for solver, name, mi, k in ((c.GMRES, "gmres", 24, steps), (c.LU, "lu", 8, max(2, steps // 25))):
-        with c.Ensemble([n], c.Props(**props), solver=solver, tol=1e-12, max_iter=mi) as e:
+                       "surface a_k~U(0,0.05/k), phi=0, dt=0.002; FP64 GMRES, mixed GMRES (FP32 matrix, "
+                       "FP64 r0 and refinement residual), FP64 LU", "nodes": n}
+    for solver, prec, name, mi, k in ((c.GMRES, c.FP64, "gmres", 24, steps),
+                                      (c.GMRES, c.MIXED, "gmres_mixed", 24, steps),
+                                      (c.LU, c.FP64, "lu", 8, max(2, steps // 25))):
+        with c.Ensemble([n], c.Props(**props), precision=prec, solver=solver, tol=1e-12, max_iter=mi) as e:
             e.set_state(Z0, P0)
             el = timed_steps(e, k, 1)
             st = e.stats()
@@ -202,15 +205,18 @@ def film_leg(c, steps, with_accuracy):
         from oracle import oracle as O
         O.build()
         acc_steps = 2
-        with c.Ensemble([n], c.Props(**props), solver=c.GMRES, tol=1e-12, max_iter=24) as e:
-            e.set_state(Z0, P0)
-            e.step(DT, acc_steps)
-            Zg, Pg = e.get_state()
+        finals = {}
+        for prec, name in ((c.FP64, "accuracy_gmres"), (c.MIXED, "accuracy_gmres_mixed")):
+            with c.Ensemble([n], c.Props(**props), precision=prec, solver=c.GMRES, tol=1e-12, max_iter=24) as e:
+                e.set_state(Z0, P0)
+                e.step(DT, acc_steps)
+                finals[name] = e.get_state()
         t0 = time.perf_counter()
         Zr, Pr = O.rk4(O.Props(**props), Z0, P0, DT, acc_steps)
-        out["accuracy_gmres"] = {"steps": acc_steps, "max_abs_dz_vs_oracle": float(np.abs(Zg - Zr).max()),
-                                 "max_abs_dphi_vs_oracle": float(np.abs(Pg - Pr).max()),
-                                 "oracle_s_per_step": round((time.perf_counter() - t0) / acc_steps, 2)}
+        t_or = round((time.perf_counter() - t0) / acc_steps, 2)
+        for name, (Zg, Pg) in finals.items():
+            out[name] = {"steps": acc_steps, "max_abs_dz_vs_oracle": float(np.abs(Zg - Zr).max()),
+                         "max_abs_dphi_vs_oracle": float(np.abs(Pg - Pr).max()), "oracle_s_per_step": t_or}
     return out
 
 

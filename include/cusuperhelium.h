@@ -106,7 +106,7 @@ enum { SFH_WATER = 0, SFH_HELIUM = 1 };
 enum { SFH_FP64 = 0, SFH_MIXED = 1 };
 /* LU: blocked FP64 LU (the reference's getrf/getrs semantics), one surface at a time or uniform
  * batches.  GMRES: FP64 GMRES over the stored matrix, all surfaces of the handle at once (every
- * N <= 4096).
+ * N <= 4096); with SFH_MIXED the matrix is stored in FP32, r0 and a refinement residual are FP64.
  * BATCHED: one persistent launch per RK stage, one workgroup per surface (every N <= 1024), GMRES
  * inside the workgroup; with SFH_MIXED the matrix is FP32 and mu is refined in FP64 against a
  * matrix-free FP64 residual. */
@@ -130,7 +130,7 @@ typedef struct sfh_props {
 } sfh_props;
 
 typedef struct sfh_options {
-    int precision;    /* SFH_FP64 (the drop-in arithmetic) or SFH_MIXED (SFH_SOLVE_BATCHED only) */
+    int precision;    /* SFH_FP64 (the drop-in arithmetic) or SFH_MIXED (SFH_SOLVE_BATCHED or _GMRES) */
     int solver;       /* SFH_SOLVE_LU, SFH_SOLVE_GMRES or SFH_SOLVE_BATCHED */
     int use_graph;    /* replay RK4 steps from a captured hipGraph */
     int max_iter;     /* iterative solver / refinement iteration cap */
@@ -147,7 +147,8 @@ typedef struct sfh_stats {
     int last_iterations;   /* GMRES: iterations of surface 0 in the last solve */
     int max_iterations;    /* GMRES: most iterations any surface needed in the last solve */
     int unconverged;       /* GMRES: solves that reached the iteration cap (accumulated) */
-    int refinements;       /* mixed precision: FP64 refinement sweeps of surface 0, last solve */
+    int refinements;       /* mixed precision, surface 0, last solve: FP64 refinement sweeps (batched);
+                              1 if the refinement solve ran Arnoldi steps, else 0 (GMRES) */
 } sfh_stats;
 
 /* Defaults: FP64, LU, graph replay on, tol 1e-12. */

@@ -19,13 +19,14 @@
 
 namespace sfh {
 
-__global__ __launch_bounds__(256) void assemble64_kernel(const SurfDesc* __restrict__ sd,
-                                                         const NodeTrig* __restrict__ trig,
-                                                         const double2* __restrict__ Z,
-                                                         const double2* __restrict__ Zp,
-                                                         const double2* __restrict__ Zpp,
-                                                         double* __restrict__ M, Phys ph,
-                                                         const int* __restrict__ only_if, GmArgs gm, int gm_on) {
+// The rows of M (stored as T: FP64, or FP32 rounded from the FP64 entries; not stored when M is
+// null) and, with gm_on, r = b - M x in FP64 for x = the initial guess, or (residual pass) x = the
+// solution, whose own rows the pass then copies into the initial guess of the next solve.
+template <typename T>
+__device__ inline void assemble_rows(const SurfDesc* __restrict__ sd, const NodeTrig* __restrict__ trig,
+                                     const double2* __restrict__ Z, const double2* __restrict__ Zp,
+                                     const double2* __restrict__ Zpp, T* __restrict__ M, const Phys& ph,
+                                     const int* __restrict__ only_if, const GmArgs& gm, int gm_on, bool residual) {
     constexpr int R = kPairRows;
     if (only_if && only_if[2 * blockIdx.y] == 0) return;   // per-surface status word
     const SurfDesc d = sd[blockIdx.y];
@@ -33,8 +34,9 @@ __global__ __launch_bounds__(256) void assemble64_kernel(const SurfDesc* __restr
     const int k0 = blockIdx.x * R;
     if (k0 >= n) return;
     const int t = threadIdx.x;
+    if (residual && blockIdx.x == 0 && t == 0) reinterpret_cast<int*>(gm.ctl)[kGmCtlInts * blockIdx.y] = 0;   // done
     const NodeTrig* tr = trig + d.voff;
-    const double* x0 = gm_on ? gm.x + d.voff : nullptr;
+    const double* x0 = gm_on ? (residual ? gm.sol : gm.x) + d.voff : nullptr;
     const bool water = ph.kind == 0;
     const double c = water ? 0.25 * (1 - ph.rho) / kPi : 0.25 / kPi;
     const double ci = 0.25 / kPi;
@@ -56,7 +58,7 @@ __global__ __launch_bounds__(256) void assemble64_kernel(const SurfDesc* __restr
     }
     // b of the rows whose residual the threads t < R store at the end
     const double bt = (gm_on && t < R) ? gm.b[d.voff + min(k0 + t, n - 1)] : 0.0;
-    double* Mk = M + d.moff + (long long)k0 * n;
+    T* Mk = M ? M + d.moff + (long long)k0 * n : nullptr;
     double acc[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) acc[r] = 0.0;
@@ -97,7 +99,7 @@ __global__ __launch_bounds__(256) void assemble64_kernel(const SurfDesc* __restr
                         v += ci * (ci2.q * ci2.id);
                     }
                 }
-                Mk[(long long)r * n + j] = v;
+                if (Mk) Mk[(long long)r * n + j] = static_cast<T>(v);
                 acc[r] += v * xj[e];
             }
         }
@@ -106,7 +108,41 @@ __global__ __launch_bounds__(256) void assemble64_kernel(const SurfDesc* __restr
     __shared__ double red[(R < 4 ? 4 : R) * 256];
     __shared__ double out[R];
     rows_reduce<R>(acc, red, out);
-    if (t < R && k0 + t < n) gm.V[(long long)(gm.m + 1) * d.voff + k0 + t] = bt - out[t];
+    if (t < R && k0 + t < n) {
+        gm.V[(long long)(gm.m + 1) * d.voff + k0 + t] = bt - out[t];
+        if (residual) gm.x[d.voff + k0 + t] = x0[k0 + t];
+    }
+}
+
+__global__ __launch_bounds__(256) void assemble64_kernel(const SurfDesc* __restrict__ sd,
+                                                         const NodeTrig* __restrict__ trig,
+                                                         const double2* __restrict__ Z,
+                                                         const double2* __restrict__ Zp,
+                                                         const double2* __restrict__ Zpp,
+                                                         double* __restrict__ M, Phys ph,
+                                                         const int* __restrict__ only_if, GmArgs gm, int gm_on) {
+    assemble_rows<double>(sd, trig, Z, Zp, Zpp, M, ph, only_if, gm, gm_on, false);
+}
+
+// Mixed precision: M stored in FP32, r0 = b - M x0 from the FP64 entries.
+__global__ __launch_bounds__(256) void assemble32_kernel(const SurfDesc* __restrict__ sd,
+                                                         const NodeTrig* __restrict__ trig,
+                                                         const double2* __restrict__ Z,
+                                                         const double2* __restrict__ Zp,
+                                                         const double2* __restrict__ Zpp,
+                                                         float* __restrict__ M, Phys ph, GmArgs gm) {
+    assemble_rows<float>(sd, trig, Z, Zp, Zpp, M, ph, nullptr, gm, 1, false);
+}
+
+// Mixed precision, refinement: r = b - M mu in FP64, matrix-free (the assembly's entries, not
+// stored), into the first basis vector; mu becomes the next solve's initial guess, and the
+// control block is re-armed (done = 0) for that solve.
+__global__ __launch_bounds__(256) void residual64_kernel(const SurfDesc* __restrict__ sd,
+                                                         const NodeTrig* __restrict__ trig,
+                                                         const double2* __restrict__ Z,
+                                                         const double2* __restrict__ Zp,
+                                                         const double2* __restrict__ Zpp, Phys ph, GmArgs gm) {
+    assemble_rows<double>(sd, trig, Z, Zp, Zpp, static_cast<double*>(nullptr), ph, nullptr, gm, 1, true);
 }
 
 }  // namespace sfh

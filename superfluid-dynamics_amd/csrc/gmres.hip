@@ -41,37 +41,41 @@ namespace sfh {
 // Row R (global) times vs (LDS) by one wave, in two halves so the first 2048 columns' loads can
 // be issued before vs is ready: row_load issues 32 loads per lane (predicated at the row end),
 // row_dot_rest finishes the product (columns beyond 2048 are loaded and used chunk by chunk).
+// The matrix is FP64, or FP32 in the mixed-precision form (entries widened to FP64 as they are used).
+template <typename T>
 struct RowChunk {
-    double mv[32];
+    T mv[32];
 };
-__device__ inline void row_load(const double* __restrict__ R, int n, int lane, RowChunk& rc) {
+template <typename T>
+__device__ inline void row_load(const T* __restrict__ R, int n, int lane, RowChunk<T>& rc) {
 #pragma unroll
     for (int u = 0; u < 32; ++u) {
         const int c = lane + 64 * u;
-        rc.mv[u] = c < n ? R[c] : 0.0;
+        rc.mv[u] = c < n ? R[c] : T(0);
     }
 }
-__device__ inline double row_dot_rest(const double* __restrict__ R, const RowChunk& rc, const double* vs, int n,
+template <typename T>
+__device__ inline double row_dot_rest(const T* __restrict__ R, const RowChunk<T>& rc, const double* vs, int n,
                                       int lane) {
     double a0 = 0.0, a1 = 0.0;
 #pragma unroll
     for (int u = 0; u < 32; u += 2) {
         const int c = lane + 64 * u;
-        a0 += rc.mv[u] * (c < n ? vs[c] : 0.0);
-        a1 += rc.mv[u + 1] * (c + 64 < n ? vs[c + 64] : 0.0);
+        a0 += (double)rc.mv[u] * (c < n ? vs[c] : 0.0);
+        a1 += (double)rc.mv[u + 1] * (c + 64 < n ? vs[c + 64] : 0.0);
     }
     for (int c0 = 32 * 64; c0 < n; c0 += 32 * 64) {
-        double mv[32];
+        T mv[32];
 #pragma unroll
         for (int u = 0; u < 32; ++u) {
             const int c = c0 + lane + 64 * u;
-            mv[u] = c < n ? R[c] : 0.0;
+            mv[u] = c < n ? R[c] : T(0);
         }
 #pragma unroll
         for (int u = 0; u < 32; u += 2) {
             const int c = c0 + lane + 64 * u;
-            a0 += mv[u] * (c < n ? vs[c] : 0.0);
-            a1 += mv[u + 1] * (c + 64 < n ? vs[c + 64] : 0.0);
+            a0 += (double)mv[u] * (c < n ? vs[c] : 0.0);
+            a1 += (double)mv[u + 1] * (c + 64 < n ? vs[c + 64] : 0.0);
         }
     }
     return wave_sum(a0 + a1);
@@ -181,6 +185,14 @@ __device__ inline void back_substitute(const double* g, const double* H, int m, 
     }
 }
 
+// Arnoldi steps of the solve, in the field of its kind (a refinement solve keeps the first solve's)
+__device__ inline void set_iters(GmCtl* ctl, const GmArgs& a, int k) {
+    if (a.check_first)
+        ctl->refine_iters = k;
+    else
+        ctl->iters = k;
+}
+
 // This surface's history of solutions of the current RK stage (null for a plain evaluation), and
 // the stage's count of stored steps (one increment per solve, by surface 0).
 __device__ inline double* history(const GmArgs& a, const SurfDesc* sd, int s) {
@@ -191,13 +203,14 @@ __device__ inline void count_history(const GmArgs& a) {
 }
 
 // Launch i of a solve's sequence (see the file comment).
+template <typename T>
 __global__ __launch_bounds__(256) void gm_iter_kernel(GmArgs a, const SurfDesc* __restrict__ sd, int i) {
     GmCtl* ctl = reinterpret_cast<GmCtl*>(a.ctl) + blockIdx.z;
     // solves finished by an earlier launch return at once.  A finish stamped by this launch
     // (workgroup 0 can get there before a late workgroup starts) does not count.  Launch 0 needs
     // no check (the derivative kernel re-armed the block), and launch 1 makes it after its row
     // loads have gone out (an earlier finish is the exception there); later launches first.
-    if (i >= 2) {
+    if (i >= 2 || (i == 1 && a.check_first)) {
         const int fin = ctl->done;
         if (fin != 0 && fin <= i) return;
     }
@@ -215,7 +228,8 @@ __global__ __launch_bounds__(256) void gm_iter_kernel(GmArgs a, const SurfDesc* 
     const int t = threadIdx.x, lane = t & 63, wv = t >> 6, n = v.n;
     const int r0 = blockIdx.x * rpw;
     const int row = r0 + wv;
-    const double* R = v.M + (long long)min(row, n - 1) * n;
+    const T* R = (sizeof(T) == 8 ? reinterpret_cast<const T*>(v.M) : reinterpret_cast<const T*>(v.M32)) +
+                 (long long)min(row, n - 1) * n;
     const long long pslot = (long long)v.nwg * (m + 1);   // one partial-dot slot per launch parity
     // Loads go out in the order their results are needed, because a wait for one load waits for
     // all loads issued before it: first the vectors and sums of the step being finished (launch
@@ -238,9 +252,12 @@ __global__ __launch_bounds__(256) void gm_iter_kernel(GmArgs a, const SurfDesc* 
         gold = v.gc[j];
         bnorm = ctl->bnorm;
     }
-    RowChunk rc;
-    if (i < m - 1) row_load(R, n, lane, rc);   // independent of v_i
-    if (i == 1) {
+    // the rows up front, except in a refinement solve, which most often ends at once: there they
+    // are fetched when the step is known to run
+    RowChunk<T> rc;
+    const bool early_rows = !a.check_first;
+    if (i < m - 1 && early_rows) row_load(R, n, lane, rc);
+    if (i == 1 && !a.check_first) {
         const int fin = ctl->done;
         if (fin != 0 && fin <= i) return;
     }
@@ -350,7 +367,7 @@ __global__ __launch_bounds__(256) void gm_iter_kernel(GmArgs a, const SurfDesc* 
         if (blockIdx.x == 0) {
             if (t < k) v.y[t] = ys[4 + t];   // scaled by this thread above
             if (t == 0) {
-                ctl->iters = k;
+                set_iters(ctl, a, k);
                 ctl->done = i + 1;
             }
         }
@@ -373,7 +390,7 @@ __global__ __launch_bounds__(256) void gm_iter_kernel(GmArgs a, const SurfDesc* 
             }
         }
         if (blockIdx.x == 0 && t == 0) {
-            ctl->iters = k;
+            set_iters(ctl, a, k);
             ctl->done = i + 1;
             if (h0 && blockIdx.z == 0) count_history(a);
         }
@@ -386,6 +403,7 @@ __global__ __launch_bounds__(256) void gm_iter_kernel(GmArgs a, const SurfDesc* 
         vrow[r] = (t < i && r0 + r < n) ? v.V[(long long)t * n + r0 + r] : 0.0;
     const double sct = (t < i) ? v.sc[t] : 0.0;
     double* wi = (i & 1) ? v.w1 : v.w0;
+    if (!early_rows) row_load(R, n, lane, rc);
     const double d = row < n ? sj * row_dot_rest(R, rc, vs, n, lane) : 0.0;
     if (lane == 0) {
         wl[wv] = row < n ? d : 0.0;
@@ -493,7 +511,7 @@ __global__ __launch_bounds__(256) void gm_close_mu_kernel(GmArgs a, const SurfDe
     }
     // the control block last: its fields decide branches, so the first use waits for them
     const int fin = ctl->done;
-    const int iters_before = ctl->iters;
+    const int iters_before = a.check_first ? ctl->refine_iters : ctl->iters;
     const double bnorm = ctl->bnorm;
     // all loads are out; the small state into LDS and the partial dots summed (used when the
     // solve is open)
@@ -588,7 +606,7 @@ __global__ __launch_bounds__(256) void gm_close_mu_kernel(GmArgs a, const SurfDe
             ctl->res = res;
             ctl->hnext = hn;
             if (!(res <= a.tol * bnorm || hn == 0.0)) ctl->unconverged += 1;
-            ctl->iters = k;
+            set_iters(ctl, a, k);
             ctl->done = i + 1;
         }
         if (t <= j) ys[4 + t] *= sc_t;   // y_q sc_q
@@ -668,5 +686,8 @@ __global__ __launch_bounds__(256) void gm_close_mu_kernel(GmArgs a, const SurfDe
         }
     }
 }
+
+template __global__ void gm_iter_kernel<double>(GmArgs, const SurfDesc*, int);
+template __global__ void gm_iter_kernel<float>(GmArgs, const SurfDesc*, int);
 
 }  // namespace sfh

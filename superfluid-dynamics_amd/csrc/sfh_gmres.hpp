@@ -14,7 +14,7 @@ constexpr int kGmMaxN = 4096;      // LDS copy of the vector being multiplied
 
 struct GmCtl {
     int done;          // 0 while the solve runs, else 1 + the launch that finished it
-    int spare0;
+    int refine_iters;  // Arnoldi steps of the refinement solve (mixed precision; 0 when none ran)
     int iters;         // Arnoldi steps of the last solve
     int unconverged;   // solves that ended at the launch limit (cumulative)
     double bnorm, beta, res;
@@ -25,6 +25,7 @@ static_assert(sizeof(GmCtl) == kGmCtlInts * sizeof(int), "host reads control blo
 
 struct GmView {
     const double* M;
+    const float* M32;
     const double* b;
     const double* x;   // initial guess
     double* sol;       // solution
@@ -50,6 +51,7 @@ __device__ inline GmView gm_view(const GmArgs& a, const SurfDesc* sd, int s) {
     v.n = d.n;
     v.nwg = (d.n + kRowsPerWG - 1) / kRowsPerWG;
     v.M = a.M + d.moff;
+    v.M32 = a.M32 ? a.M32 + d.moff : nullptr;
     v.b = a.b + d.voff;
     v.x = a.x + d.voff;
     v.sol = a.sol + d.voff;

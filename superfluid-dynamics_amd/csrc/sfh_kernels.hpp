@@ -48,6 +48,7 @@ __device__ inline void push_history(double* h, long long stride, double x) {
 
 struct GmArgs {
     const double* M;     // matrices (FP64, row-major per surface at SurfDesc::moff)
+    const float* M32;    // the same in FP32 (mixed precision)
     const double* b;     // right-hand sides
     double* x;           // initial guess (written by zphi_circ_kernel, read only during the solve)
     double* sol;         // solution, written by the launch that finishes the solve
@@ -67,7 +68,7 @@ struct GmArgs {
     double* hist;
     int* hvalid;
     int stage;
-    int pad;
+    int check_first;     // refinement solve: every launch checks for a finished solve before its matrix loads
 };
 constexpr int kPairRows = 4;        // rows per workgroup of the N^2 pair kernels (assembly, velocity)
 // ---- spectral.hip
@@ -106,6 +107,10 @@ __global__ void plain_deriv_kernel(const SurfDesc* sd, const double2* tw_all, co
 __global__ void assemble64_kernel(const SurfDesc* sd, const NodeTrig* trig, const double2* Z,
                                   const double2* Zp, const double2* Zpp, double* M, Phys ph,
                                   const int* only_if, GmArgs gm, int gm_on);
+__global__ void assemble32_kernel(const SurfDesc* sd, const NodeTrig* trig, const double2* Z,
+                                  const double2* Zp, const double2* Zpp, float* M, Phys ph, GmArgs gm);
+__global__ void residual64_kernel(const SurfDesc* sd, const NodeTrig* trig, const double2* Z,
+                                  const double2* Zp, const double2* Zpp, Phys ph, GmArgs gm);
 
 // ---- velocity.hip
 __global__ void velocity_stage_kernel(const SurfDesc* sd, const NodeTrig* trig, const double2* Z,
@@ -131,6 +136,7 @@ __global__ void gepp_fallback_kernel(const SurfDesc* sd, double* M, const double
 
 // ---- gmres.hip
 constexpr int kGmCtlInts = 32;   // ints per control block (holds the device struct)
+template <typename T>
 __global__ void gm_iter_kernel(GmArgs a, const SurfDesc* sd, int j);
 __global__ void gm_close_mu_kernel(GmArgs a, const SurfDesc* sd, const double* circ, double2* mup, int i);
 

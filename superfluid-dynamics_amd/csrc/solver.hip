@@ -99,8 +99,8 @@ Solver::Solver(const Phys& ph, const std::vector<int>& n, const Options& opt) : 
     if (opt_.solver != kSolveLU && opt_.solver != kSolveGMRES && opt_.solver != kSolveBatched)
         throw ArgError("unknown solver");
     if (opt_.precision != kFP64 && opt_.precision != kMixed) throw ArgError("unknown precision");
-    if (opt_.precision == kMixed && opt_.solver != kSolveBatched)
-        throw ArgError("the mixed-precision form runs on the batched (per-surface workgroup) solver");
+    if (opt_.precision == kMixed && opt_.solver == kSolveLU)
+        throw ArgError("the mixed-precision form runs with the GMRES or the batched solver");
     if (opt_.solver == kSolveGMRES)
         for (int v : n)
             if (v > kGmMaxN) throw ArgError("the GMRES solver takes surfaces of at most 4096 nodes");
@@ -208,6 +208,7 @@ Solver::Solver(const Phys& ph, const std::vector<int>& n, const Options& opt) : 
         gm_small_stride_ = 4LL * (gm_m_ + 1) + 2LL * gm_m_ + (long long)gm_m_ * (gm_m_ + 1);
         gmV_ = dalloc<double>((size_t)(gm_m_ + 1) * T_);
         gmx0_ = dalloc<double>(T_);
+        if (opt_.precision == kMixed) M32_ = dalloc<float>(MT_);
         SFH_HIP(hipMemset(gmx0_, 0, T_ * sizeof(double)));
         gmw_ = dalloc<double>(2 * (size_t)T_);
         gmpart_ = dalloc<double>(gm_part_stride_ * S_);
@@ -241,7 +242,7 @@ Solver::~Solver() {
     void* bufs[] = {d_desc_, d_tw_, Z0_, Za_, Zb_, P0_, Pa_, Pb_, kZ_[0], kZ_[1], kZ_[2], kP_[0],
                     kP_[1], kP_[2], Zp_, Zpp_, mup_, qup_, b_, work_, y_, mu_, M_, uinv_, flags_,
                     status_, piv_, counters_, gmV_, gmw_, gmpart_, gmsmall_, gmctl_, trig_, order_, bstats_,
-                    gmhist_, gmhvalid_, bhist_, bhvalid_, d_circ_, gmx0_};
+                    gmhist_, gmhvalid_, bhist_, bhvalid_, d_circ_, gmx0_, M32_};
     for (void* p : bufs)
         if (p) (void)hipFree(p);
     void* vbufs[] = {vY_[0], vY_[1], vY_[2], vK_[0], vK_[1], vK_[2], vYP_[0], vYP_[1], vYP_[2], vKP_[0],
@@ -296,6 +297,7 @@ Solver::KernelReport Solver::kernel_report(const std::string& tag) {
 GmArgs Solver::gm_args(int stage) const {
     GmArgs a{};
     a.M = M_;
+    a.M32 = M32_;
     a.b = b_;
     a.x = gmx0_;
     a.sol = mu_;
@@ -319,8 +321,9 @@ GmArgs Solver::gm_args(int stage) const {
 // residual comes from the assembly kernel and the final step applies the correction x += V y,
 // so only the fixed-length sequence of steps is launched here.  With two launches or more the
 // last one is gm_close_mu_kernel, which also forms mu' (returns true: no mu_circ_kernel).
-bool Solver::enqueue_gmres(int stage) {
-    const GmArgs a = gm_args(stage);
+// Mixed precision runs the products over the FP32 matrix (4 bytes per entry).
+bool Solver::enqueue_gmres(const GmArgs& a) {
+    const bool f32 = opt_.precision == kMixed;
     double nsq = 0.0;
     for (int v : n_) nsq += (double)v * v;
     const dim3 grid(gm_nwg_, 1, S_);
@@ -331,9 +334,15 @@ bool Solver::enqueue_gmres(int stage) {
         // finishing step j - 1 (the basis rebuilt from w_{j-1} and V[0..j-1]) + the product with
         // the stored matrix + the partial dots
         launch(
-            "gm_iter", 2.0 * nsq + 4.0 * (j + 1) * T_, 8.0 * nsq + 8.0 * (j + 3) * T_,
+            a.check_first ? "gm_refine" : "gm_iter", 2.0 * nsq + 4.0 * (j + 1) * T_,
+            (f32 ? 4.0 : 8.0) * nsq + 8.0 * (j + 3) * T_,
             [&] {
-                hipLaunchKernelGGL(gm_iter_kernel, grid, dim3(256), 0, stream_, a, (const SurfDesc*)d_desc_, j);
+                if (f32)
+                    hipLaunchKernelGGL(gm_iter_kernel<float>, grid, dim3(256), 0, stream_, a,
+                                       (const SurfDesc*)d_desc_, j);
+                else
+                    hipLaunchKernelGGL(gm_iter_kernel<double>, grid, dim3(256), 0, stream_, a,
+                                       (const SurfDesc*)d_desc_, j);
             },
             j);
     }
@@ -344,7 +353,7 @@ bool Solver::enqueue_gmres(int stage) {
                                0, stream_, a, (const SurfDesc*)d_desc_, (const double*)d_circ_, mup_, a.m - 1);
         });
     }
-    if (ktime_) {
+    if (ktime_ && !a.check_first) {
         // launches 0 .. iters-1 ran a product ("gm_iter"); a gm_iter_kernel launch iters only
         // finished the last step and stored the solution ("gm_close": no product); later ones
         // returned at once.  The fused last launch ("gm_close_mu") always forms mu'.
@@ -369,8 +378,27 @@ bool Solver::enqueue_gmres(int stage) {
 
 // M mu = b for every surface; M, b (and its working copy) are ready in stream order.  Returns
 // true when mu' has been formed too.
+//   Mixed precision (GMRES): the first solve runs over the FP32 matrix from r0 = b - M x0 formed in
+//   FP64 by the assembly; then the FP64 residual of its solution is formed matrix-free and a
+//   refinement solve over the FP32 matrix starts from it (it ends at launch 0 when the residual is
+//   already within the tolerance, the usual case).  Only the refinement solve stores the stage's
+//   warm-start history.
 bool Solver::enqueue_solve(const double2* Z, int stage) {
-    if (opt_.solver == kSolveGMRES) return enqueue_gmres(stage);
+    if (opt_.solver == kSolveGMRES) {
+        if (opt_.precision != kMixed) return enqueue_gmres(gm_args(stage));
+        GmArgs first = gm_args(stage);
+        first.stage = 0;   // no history: the refinement solve stores the solution
+        enqueue_gmres(first);
+        GmArgs refine = gm_args(stage);
+        double nsq = 0.0;
+        for (int v : n_) nsq += (double)v * v;
+        launch("residual", (ph_.image ? 2.0 : 1.0) * nsq, 16.0 * T_, [&] {
+            hipLaunchKernelGGL(residual64_kernel, dim3((maxN_ + kPairRows - 1) / kPairRows, S_), dim3(256), 0,
+                               stream_, d_desc_, (const NodeTrig*)trig_, Z, Zp_, Zpp_, ph_, refine);
+        });
+        refine.check_first = 1;
+        return enqueue_gmres(refine);
+    }
     auto run = [&](int s0, int count, int n) {
         LuArgs a;
         a.A = M_ + desc_[s0].moff;
@@ -468,10 +496,17 @@ void Solver::enqueue_rhs(const double2* Z, const double* P, const StageCtl& ctl)
                            Zp_, Zpp_, b_, work_, ph_, status_, trig_, ga, gm_on);
     });
     // assembly: flops counted as cot evaluations (the transcendental-bound unit), bytes = M written
-    launch("assemble", img * nsq, 8.0 * nsq, [&] {
-        hipLaunchKernelGGL(assemble64_kernel, dim3((maxN_ + kPairRows - 1) / kPairRows, S_), dim3(256), 0, stream_, d_desc_,
-                           (const NodeTrig*)trig_, Z, Zp_, Zpp_, M_, ph_, (const int*)nullptr, ga, gm_on);
-    });
+    if (gm_on && opt_.precision == kMixed) {
+        launch("assemble", img * nsq, 4.0 * nsq, [&] {
+            hipLaunchKernelGGL(assemble32_kernel, dim3((maxN_ + kPairRows - 1) / kPairRows, S_), dim3(256), 0, stream_,
+                               d_desc_, (const NodeTrig*)trig_, Z, Zp_, Zpp_, M32_, ph_, ga);
+        });
+    } else {
+        launch("assemble", img * nsq, 8.0 * nsq, [&] {
+            hipLaunchKernelGGL(assemble64_kernel, dim3((maxN_ + kPairRows - 1) / kPairRows, S_), dim3(256), 0, stream_,
+                               d_desc_, (const NodeTrig*)trig_, Z, Zp_, Zpp_, M_, ph_, (const int*)nullptr, ga, gm_on);
+        });
+    }
     if (!enqueue_solve(Z, ctl.stage)) {
         launch("mu_deriv", 4.0 * nsq, 24.0 * nsq / kCircRowsPerWG, [&] {
             hipLaunchKernelGGL(mu_circ_kernel, cgrid, dim3(256), 0, stream_, d_desc_, (const double*)d_circ_,
@@ -666,6 +701,8 @@ void Solver::check_status() {
             unc += ctl[(size_t)kGmCtlInts * s + 3];
         }
         stats_.last_iterations = ctl[2];
+        // mixed precision: 1 when surface 0's last refinement solve ran Arnoldi steps
+        if (opt_.precision == kMixed) stats_.refinements = ctl[1] > 0 ? 1 : 0;
         stats_.max_iterations = mx;
         stats_.unconverged = unc;
     }

@@ -155,9 +155,10 @@ class Solver {
     int gm_m_ = 0, gm_nwg_ = 0;
     double *gmV_ = nullptr, *gmw_ = nullptr, *gmpart_ = nullptr, *gmsmall_ = nullptr;
     double* gmx0_ = nullptr;     // initial guess of the solve in flight (the solution goes to mu_)
+    float* M32_ = nullptr;       // FP32 matrices of the mixed-precision GMRES form
     int* gmctl_ = nullptr;
     long long gm_part_stride_ = 0, gm_small_stride_ = 0;
-    bool enqueue_gmres(int stage);
+    bool enqueue_gmres(const GmArgs& a);
     GmArgs gm_args(int stage) const;
     double* gmhist_ = nullptr;   // 4 kHistDepth x total: per RK stage, mu of the last steps
     int* gmhvalid_ = nullptr;

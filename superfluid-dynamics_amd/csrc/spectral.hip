@@ -53,6 +53,7 @@ __global__ __launch_bounds__(256) void zphi_circ_kernel(const SurfDesc* __restri
             GmCtl* c = reinterpret_cast<GmCtl*>(gm.ctl) + blockIdx.y;
             c->done = 0;
             c->iters = 0;
+            c->refine_iters = 0;
         }
     }
     const double* c1r = circ + d.coff;

@@ -337,3 +337,44 @@ def test_gmres_uneven_ensemble_matches_oracle(oracle, csh):
         assert np.abs(Zg[o:o + n] - Zo).max() < 1e-11, n
         assert np.abs(Pg[o:o + n] - Po).max() < 1e-11, n
         o += n
+
+
+# ---------------------------------------------------------------- mixed precision, multi-workgroup GMRES
+def test_mixed_gmres_uneven_ensemble_matches_oracle(oracle, csh):
+    # FP32 matrix, GMRES over it from an FP64 r0, then the FP64 residual of the solution formed
+    # matrix-free and a refinement solve from it: the trajectory must stay within 1e-9 of the FP64
+    # oracle (GEPP), over the cold steps and the trimmed sequence after them
+    sizes = [256, 96, 1024, 384]
+    Zs = [_surface(n, 400 + i, amp=0.05) for i, n in enumerate(sizes)]
+    Ps = [0.01 * np.cos(2 * np.pi * np.arange(n) / n + i) for i, n in enumerate(sizes)]
+    with csh.Ensemble(sizes, precision=csh.MIXED, solver=csh.GMRES, tol=1e-12, max_iter=24) as e:
+        e.set_state(np.concatenate(Zs), np.concatenate(Ps))
+        e.step(0.004, 5)
+        e.step(0.004, 7)
+        Zg, Pg = e.get_state()
+        st = e.stats()
+    print(f"mixed gmres ensemble: last {st['last_iterations']} max {st['max_iterations']} "
+          f"unconverged {st['unconverged']} refinements {st['refinements']}", flush=True)
+    assert st["unconverged"] == 0
+    o = 0
+    for i, n in enumerate(sizes):
+        Zo, Po = oracle.rk4(oracle.Props(), Zs[i], Ps[i], 0.004, 12)
+        assert np.abs(Zg[o:o + n] - Zo).max() < 1e-9, n
+        assert np.abs(Pg[o:o + n] - Po).max() < 1e-9, n
+        o += n
+
+
+def test_mixed_gmres_film_matches_oracle(oracle, csh):
+    # the helium film with the image term (an outlier eigenvalue, several Arnoldi steps per solve)
+    n = 1024
+    kw = dict(kind=1, depth=1.0)
+    Z0 = _surface(n, 7, amp=0.05)
+    Zo, Po = oracle.rk4(oracle.Props(**kw), Z0, np.zeros(n), 0.002, 6)
+    with csh.Ensemble([n], csh.Props(**kw), precision=csh.MIXED, solver=csh.GMRES, tol=1e-12, max_iter=24) as e:
+        e.set_state(Z0, np.zeros(n))
+        e.step(0.002, 6)
+        Zg, Pg = e.get_state()
+        st = e.stats()
+    assert st["unconverged"] == 0
+    assert np.abs(Zg - Zo).max() < 1e-9
+    assert np.abs(Pg - Po).max() < 1e-9
