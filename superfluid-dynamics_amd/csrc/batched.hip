@@ -16,7 +16,9 @@
 //                re-orthogonalisation, restarted cycles of kBK vectors);
 //        mixed : FP64 iterative refinement  r = b - M mu  with the residual evaluated
 //                matrix-free in FP64 (same pair formula as the assembly), correction solves by
-//                GMRES on the stored FP32 matrix;
+//                GMRES on the stored FP32 matrix; a correction whose GMRES residual plus
+//                ||M - M32||_F |d| is within the tolerance ends the refinement without another
+//                FP64 residual pass;
 //   D  mu' (one more transform pair);
 //   E  the cot-sum velocity, Bernoulli and the RK4 stage update (WaterVelocities.cuh:38-107,
 //      createM.cuh:96-117, AutonomousRungeKuttaStepper.cuh:338-375).
@@ -69,6 +71,7 @@ __global__ __launch_bounds__(1024) void batched_stage_kernel(BatchArgs a, Phys p
     if (own) L.vs[t] = x;
     __syncthreads();
     double mval = 0.0;
+    double ds2 = 0.0;   // mixed: squared rounding errors of the FP32 entries (this thread's)
     if (t < G * n) {
         const int k = t % n, g = t / n;
         const NodeTrig tk = L.tr[k];
@@ -81,6 +84,8 @@ __global__ __launch_bounds__(1024) void batched_stage_kernel(BatchArgs a, Phys p
                 float v;
                 if (j == k) {
                     v = (float)L.diag[k];
+                    const double rd = L.diag[k] - (double)v;
+                    ds2 += rd * rd;
                 } else {
                     const NodeTrig tj = L.tr[j];
                     const double vj = L.vs[j];
@@ -95,6 +100,8 @@ __global__ __launch_bounds__(1024) void batched_stage_kernel(BatchArgs a, Phys p
                         e += pc.ci * (cq.q * cq.id);
                     }
                     v = (float)e;
+                    const double rd = e - (double)v;
+                    ds2 += rd * rd;
                 }
                 col[(size_t)j * n] = v;
             }
@@ -144,6 +151,7 @@ __global__ __launch_bounds__(1024) void batched_stage_kernel(BatchArgs a, Phys p
                          true, own ? bk - mx0 : 0.0);
     } else {
         const float* Mf = reinterpret_cast<const float*>(a.M) + d.moff;
+        const double dM = sqrt(bsum1(ds2, L));   // ||M - M32||_F
         bool done = false;
         for (; sweeps < a.max_refine; ++sweeps) {
             double mx = mx0;   // sweep 0: formed by the assembly
@@ -162,6 +170,16 @@ __global__ __launch_bounds__(1024) void batched_stage_kernel(BatchArgs a, Phys p
             iters += wg_gmres([&]() { return mv_stored(Mf, n, G, L); }, n, r, dx, true,
                               fmax(1e-7 * rn, 0.25 * tol_abs), 1, L, nullptr);
             x += dx;
+            // the FP64 residual of the corrected x, bounded without forming it:
+            //   |b - M x| <= |r - M32 dx| + |(M - M32) dx| <= (GMRES residual) + ||M - M32||_F |dx|
+            // which at steady state is far below the tolerance (|dx| is the warm start's error)
+            const double est = L.scal[0];   // |r - M32 dx|, the last rotated right-hand side entry
+            const double dn = sqrt(bsum1(own ? dx * dx : 0.0, L));
+            if (est + dM * dn <= tol_abs) {
+                ++sweeps;
+                done = true;
+                break;
+            }
         }
         if (!done) unconv = 1;
     }
