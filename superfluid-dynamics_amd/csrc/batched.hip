@@ -51,7 +51,7 @@ __global__ __launch_bounds__(1024) void batched_stage_kernel(BatchArgs a, Phys p
         a.b[off + t] = bk;
     }
 
-    // warm start: this stage's mu extrapolated over the previous steps (error O(dt^4) once four
+    // warm start: this stage's mu extrapolated over the previous steps (error O(dt^5) once five
     // are kept), else the previous solve's mu
     double x = own ? a.mu[off + t] : 0.0;
     double* h0 = nullptr;

@@ -31,7 +31,7 @@
 // a slower workgroup of the same launch still reads it.
 //
 // Initial guess of an RK stage: mu of the same stage extrapolated over the previous steps by the
-// polynomial through them (extrapolate_history; with four steps kept its error is fourth order in
+// polynomial through them (extrapolate_history; with five steps kept its error is fifth order in
 // dt, against first order for the previous stage's mu), which saves Krylov steps on every solve of
 // a time integration.
 #include "sfh_gmres.hpp"
@@ -187,6 +187,7 @@ __device__ inline void back_substitute(const double* g, const double* H, int m, 
 
 // Arnoldi steps of the solve, in the field of its kind (a refinement solve keeps the first solve's)
 __device__ inline void set_iters(GmCtl* ctl, const GmArgs& a, int k) {
+    ctl->fill_iters = max(ctl->fill_iters, k);
     if (a.check_first)
         ctl->refine_iters = k;
     else

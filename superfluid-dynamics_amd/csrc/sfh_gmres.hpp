@@ -19,7 +19,9 @@ struct GmCtl {
     int unconverged;   // solves that ended at the launch limit (cumulative)
     double bnorm, beta, res;
     double hnext;
-    int spare[20];
+    int fill_iters;    // most Arnoldi steps of a solve since the stage history last held fewer
+                       // than kHistDepth steps
+    int spare[19];
 };
 static_assert(sizeof(GmCtl) == kGmCtlInts * sizeof(int), "host reads control blocks with this stride");
 

@@ -26,10 +26,14 @@ struct StageCtl {
 // Warm start of a solve by extrapolation over time steps: each RK stage keeps its solutions of the
 // last kHistDepth steps (newest first, levels stride apart) and the next initial guess is the
 // polynomial through the hv of them that exist, evaluated one step ahead:
-//   hv = 1: u0;  2: 2 u0 - u1;  3: 3 (u0 - u1) + u2;  4: 4 (u0 + u2) - 6 u1 - u3   (error O(dt^hv)).
-constexpr int kHistDepth = 4;
+//   hv = 1: u0;  2: 2 u0 - u1;  3: 3 (u0 - u1) + u2;  4: 4 (u0 + u2) - 6 u1 - u3;
+//   5: 5 (u0 - u3) - 10 (u1 - u2) + u4   (error O(dt^hv); the coefficients are the binomial
+//   coefficients of the hv-th difference, whose absolute sum 2^hv - 1 bounds how much the solve
+//   tolerance left in the history is amplified: 31 at hv = 5).
+constexpr int kHistDepth = 5;
 __device__ inline double extrapolate_values(const double (&h)[kHistDepth], int hv) {
-    if (hv >= 4) return 4.0 * (h[0] + h[2]) - 6.0 * h[1] - h[3];
+    if (hv >= 5) return 5.0 * (h[0] - h[3]) - 10.0 * (h[1] - h[2]) + h[4];
+    if (hv == 4) return 4.0 * (h[0] + h[2]) - 6.0 * h[1] - h[3];
     if (hv == 3) return 3.0 * (h[0] - h[1]) + h[2];
     if (hv == 2) return 2.0 * h[0] - h[1];
     return h[0];

@@ -565,21 +565,28 @@ void Solver::capture(double dt, int len) {
     graph_len_ = len;
 }
 
+// Steps that run every solve with the full launch cap: kHistDepth while the warm-start history
+// fills, then one with the full history that sets the trimmed length.
+constexpr int kCalibSteps = kHistDepth + 1;
+
 int Solver::step_len() {
     if (opt_.solver != kSolveGMRES) return 0;
-    if (hist_steps_ < kHistDepth || trim_ == 0) return gm_m_;
+    if (hist_steps_ < kCalibSteps || trim_ == 0) return gm_m_;
     return trim_;
 }
 
 void Solver::after_step(int len) {
     if (opt_.solver != kSolveGMRES) return;
-    if (hist_steps_ < kHistDepth) {
-        if (++hist_steps_ < kHistDepth) return;
-        // the history is full: the last solve (stage 4, extrapolated from one step fewer than
-        // every later solve, so with a guess of one order less) bounds the launches the
-        // following solves need
+    if (hist_steps_ < kCalibSteps) {
+        if (++hist_steps_ < kCalibSteps) return;
+        // the first step with the full history: its solves (all four stages) bound the Arnoldi
+        // steps of the following ones.  L launches run up to L - 1 steps.  Solves that need 1 or
+        // 2 steps start from guesses whose error is at the level of the solve tolerance (the
+        // extrapolation's truncation error is smaller), and get no spare launch; slower ones,
+        // whose step count varies more from solve to solve, get one spare step.
         sync();
-        trim_ = std::max(2, std::min(gm_m_, stats_.max_iterations));
+        const int need = fill_iters_ + 1;
+        trim_ = std::max(2, std::min(gm_m_, fill_iters_ >= 3 ? need + 1 : need));
         unc_seen_ = stats_.unconverged;
         return;
     }
@@ -598,7 +605,7 @@ void Solver::step(double dt, int nsteps) {
         // steps that need no policy check between them go out as one batch: every step while the
         // history fills is checked (it sets the trimmed length); a trimmed batch is checked after
         // its first step only
-        const bool check = opt_.solver == kSolveGMRES && (hist_steps_ < kHistDepth || len < gm_m_);
+        const bool check = opt_.solver == kSolveGMRES && (hist_steps_ < kCalibSteps || len < gm_m_);
         const int batch = check ? 1 : nsteps - i;
         if (opt_.use_graph && !ktime_) {
             if (!exec_ || graph_dt_ != dt || graph_len_ != len) capture(dt, len);
@@ -610,7 +617,7 @@ void Solver::step(double dt, int nsteps) {
         i += batch;
         if (check) {
             after_step(len);
-            if (hist_steps_ >= kHistDepth && len < gm_m_ && i < nsteps) {   // rest of a trimmed batch
+            if (hist_steps_ >= kCalibSteps && len < gm_m_ && i < nsteps) {   // rest of a trimmed batch
                 const int len2 = step_len();
                 if (opt_.use_graph && !ktime_) {
                     if (!exec_ || graph_dt_ != dt || graph_len_ != len2) capture(dt, len2);
@@ -695,11 +702,15 @@ void Solver::check_status() {
     if (opt_.solver == kSolveGMRES) {
         std::vector<int> ctl((size_t)kGmCtlInts * S_);
         SFH_HIP(hipMemcpy(ctl.data(), gmctl_, ctl.size() * sizeof(int), hipMemcpyDeviceToHost));
-        int mx = 0, unc = 0;
+        int mx = 0, unc = 0, fill = 0;
         for (int s = 0; s < S_; ++s) {
-            mx = std::max(mx, ctl[(size_t)kGmCtlInts * s + 2]);
-            unc += ctl[(size_t)kGmCtlInts * s + 3];
+            GmCtl c;
+            std::memcpy(&c, ctl.data() + (size_t)kGmCtlInts * s, sizeof(GmCtl));
+            mx = std::max(mx, c.iters);
+            unc += c.unconverged;
+            fill = std::max(fill, c.fill_iters);
         }
+        fill_iters_ = fill;
         stats_.last_iterations = ctl[2];
         // mixed precision: 1 when surface 0's last refinement solve ran Arnoldi steps
         if (opt_.precision == kMixed) stats_.refinements = ctl[1] > 0 ? 1 : 0;

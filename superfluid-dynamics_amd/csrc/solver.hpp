@@ -160,7 +160,7 @@ class Solver {
     long long gm_part_stride_ = 0, gm_small_stride_ = 0;
     bool enqueue_gmres(const GmArgs& a);
     GmArgs gm_args(int stage) const;
-    double* gmhist_ = nullptr;   // 4 kHistDepth x total: per RK stage, mu of the last steps
+    double* gmhist_ = nullptr;   // 4 x kHistDepth x total: per RK stage, mu of the last steps
     int* gmhvalid_ = nullptr;
     // persistent per-surface stage kernel (kSolveBatched)
     int* order_ = nullptr;    // workgroup -> surface, by decreasing N
@@ -190,15 +190,16 @@ class Solver {
     int graph_len_ = 0;
     // GMRES launch sequence per solve.  The step graph holds a fixed number of Arnoldi launches per
     // solve (launches after convergence return at once but still cost a dispatch each).  While the
-    // warm-start history fills (the first kHistDepth steps after set_state) every solve gets the
-    // full cap gm_m_; afterwards the sequence is trimmed to what the last of those solves needed
-    // (at least 2; later solves start from a higher-order extrapolation), and doubled (up to the
-    // cap) whenever a check finds a solve that ran out of launches.  seq_len_ is the length the
-    // kernels are being enqueued with.
+    // warm-start history fills (the first kHistDepth steps after set_state) and for one step with
+    // the full history, every solve gets the full cap gm_m_; afterwards the sequence is trimmed to
+    // what the solves of that last step needed (one launch per Arnoldi step, one to finish, a spare
+    // step when they needed 3 or more), and doubled (up to the cap) whenever a check finds a solve
+    // that ran out of launches.  seq_len_ is the length the kernels are being enqueued with.
     int seq_len_ = 0;
     int trim_ = 0;          // 0: not determined yet
-    int hist_steps_ = 0;    // steps since set_state (saturates at kHistDepth)
+    int hist_steps_ = 0;    // steps since set_state (saturates at kCalibSteps)
     int unc_seen_ = 0;      // unconverged count at the last check
+    int fill_iters_ = 0;    // most Arnoldi steps of a solve with the full history, any surface
     Stats stats_;
 };
 

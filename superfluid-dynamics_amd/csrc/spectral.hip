@@ -54,6 +54,7 @@ __global__ __launch_bounds__(256) void zphi_circ_kernel(const SurfDesc* __restri
             c->done = 0;
             c->iters = 0;
             c->refine_iters = 0;
+            if (gm.stage > 0 && gm.hvalid[gm.stage - 1] < kHistDepth) c->fill_iters = 0;
         }
     }
     const double* c1r = circ + d.coff;
@@ -164,7 +165,7 @@ __global__ __launch_bounds__(256) void zphi_circ_kernel(const SurfDesc* __restri
         const int k = k0 + t;
         if (trig) trig[off + k] = node_trig(zk);
         // GMRES initial guess of this RK stage: its mu over the previous steps extrapolated
-        // (error O(dt^4) once four are kept), else the last solve's mu
+        // (error O(dt^5) once five are kept), else the last solve's mu
         if (gm_on) gm.x[off + k] = hv > 0 ? extrapolate_values(hk, hv) : solk;
     }
 }

@@ -290,8 +290,8 @@ def test_gmres_rk4_matches_oracle_and_warm_starts(oracle, csh):
 
 @pytest.mark.parametrize("kw", [{}, dict(kind=1, depth=1.0)])
 def test_gmres_trimmed_sequence_matches_lu(csh, kw):
-    # After the warm-start history fills (4 steps) the step graph keeps only one launch more per
-    # solve than the last full-length solve needed.  Trajectories split over several step() calls
+    # After the warm-start history fills (5 steps) and one more full-length step, the step graph
+    # keeps only the launches the solves of that step needed.  Trajectories split over several step() calls
     # (cold steps, trimmed batches, a re-captured graph) against the LU stepper on the same surface;
     # both FP64, agreement 1e-11 after 40 steps, no solve runs out of launches.
     n = 1024
@@ -318,7 +318,7 @@ def test_gmres_trimmed_sequence_matches_lu(csh, kw):
 def test_gmres_uneven_ensemble_matches_oracle(oracle, csh):
     # several surfaces of different N in one GMRES solve sequence (one grid slice per surface,
     # each with its own control block, partial dots and history), 12 steps: the trimmed sequence
-    # starts after step 4; every surface against the oracle's GEPP stepper
+    # starts after step 6; every surface against the oracle's GEPP stepper
     sizes = [256, 96, 1024, 12, 384]
     Zs = [_surface(n, 300 + i, amp=0.05) for i, n in enumerate(sizes)]
     Ps = [0.01 * np.cos(2 * np.pi * np.arange(n) / n + i) for i, n in enumerate(sizes)]
