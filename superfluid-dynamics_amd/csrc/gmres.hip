@@ -185,24 +185,6 @@ __device__ inline void back_substitute(const double* g, const double* H, int m, 
     }
 }
 
-// Arnoldi steps of the solve, in the field of its kind (a refinement solve keeps the first solve's)
-__device__ inline void set_iters(GmCtl* ctl, const GmArgs& a, int k) {
-    ctl->fill_iters = max(ctl->fill_iters, k);
-    if (a.check_first)
-        ctl->refine_iters = k;
-    else
-        ctl->iters = k;
-}
-
-// This surface's history of solutions of the current RK stage (null for a plain evaluation), and
-// the stage's count of stored steps (one increment per solve, by surface 0).
-__device__ inline double* history(const GmArgs& a, const SurfDesc* sd, int s) {
-    return a.stage > 0 ? a.hist + (long long)(a.stage - 1) * kHistDepth * a.total + sd[s].voff : nullptr;
-}
-__device__ inline void count_history(const GmArgs& a) {
-    a.hvalid[a.stage - 1] = min(a.hvalid[a.stage - 1] + 1, kHistDepth);
-}
-
 // Launch i of a solve's sequence (see the file comment).
 template <typename T>
 __global__ __launch_bounds__(256) void gm_iter_kernel(GmArgs a, const SurfDesc* __restrict__ sd, int i) {
@@ -422,6 +404,14 @@ __global__ __launch_bounds__(256) void gm_iter_kernel(GmArgs a, const SurfDesc* 
             }
         }
         v.part[(i & 1) * pslot + (long long)t * v.nwg + blockIdx.x] = s;
+    }
+    if (i == 0 && t == 1) {
+        // launch 0 also leaves |w_0|^2 (slot 1): a sequence of one Arnoldi step is finished from
+        // the two sums alone by velocity_gm1_kernel
+        double s = 0.0;
+#pragma unroll
+        for (int r = 0; r < kRowsPerWG; ++r) s += wl[r] * wl[r];
+        v.part[(long long)v.nwg + blockIdx.x] = s;
     }
 }
 

@@ -74,6 +74,24 @@ __device__ inline GmView gm_view(const GmArgs& a, const SurfDesc* sd, int s) {
 
 __device__ inline GmView gm_view(const GmArgs& a, const SurfDesc* sd) { return gm_view(a, sd, blockIdx.z); }
 
+// Arnoldi steps of the solve, in the field of its kind (a refinement solve keeps the first solve's)
+__device__ inline void set_iters(GmCtl* ctl, const GmArgs& a, int k) {
+    ctl->fill_iters = max(ctl->fill_iters, k);
+    if (a.check_first)
+        ctl->refine_iters = k;
+    else
+        ctl->iters = k;
+}
+
+// This surface's history of solutions of the current RK stage (null for a plain evaluation), and
+// the stage's count of stored steps (one increment per solve, by surface 0).
+__device__ inline double* history(const GmArgs& a, const SurfDesc* sd, int s) {
+    return a.stage > 0 ? a.hist + (long long)(a.stage - 1) * kHistDepth * a.total + sd[s].voff : nullptr;
+}
+__device__ inline void count_history(const GmArgs& a) {
+    a.hvalid[a.stage - 1] = min(a.hvalid[a.stage - 1] + 1, kHistDepth);
+}
+
 // Sums part[i * nwg + w] over w for i < count (count <= kMaxKrylov + 1) into out[i] (LDS).
 // Basis indices go in chunks of 12; within a chunk every load of a thread (12 indices x 2
 // workgroups per 512-workgroup pass) is issued before any addition.

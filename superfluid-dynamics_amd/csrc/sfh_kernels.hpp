@@ -120,6 +120,9 @@ __global__ void residual64_kernel(const SurfDesc* sd, const NodeTrig* trig, cons
 __global__ void velocity_stage_kernel(const SurfDesc* sd, const NodeTrig* trig, const double2* Z,
                                       const double2* Zp, const double2* Zpp, const double* mu,
                                       const double2* mup, Phys ph, StageCtl ctl);
+__global__ void velocity_gm1_kernel(const SurfDesc* sd, const NodeTrig* trig, const double2* Z,
+                                    const double2* Zp, const double2* Zpp, double2* mup, Phys ph,
+                                    StageCtl ctl, GmArgs gm, const double* circ);
 
 // ---- lu.hip
 struct LuArgs {

@@ -10,6 +10,9 @@
 //                                                      launch also forms mu', gm_close_mu_kernel)
 //   mu_circ_kernel        mu'                         (LU, and GMRES with a one-launch sequence)
 //   velocity_stage_kernel q, dPhi/dt and the RK4 stage update of every node
+//     or velocity_gm1_kernel                          (FP64 GMRES sequence of two launches: the
+//                                                      product, then this kernel, which finishes
+//                                                      the step and forms mu and mu' as well)
 // and one RK4 step (AutonomousRungeKuttaStepper.cuh:123-361) is four of them.  The step is
 // captured once into a hipGraph and replayed, so a run of many steps costs one graph launch
 // per step and the state never leaves HBM.
@@ -321,8 +324,9 @@ GmArgs Solver::gm_args(int stage) const {
 // residual comes from the assembly kernel and the final step applies the correction x += V y,
 // so only the fixed-length sequence of steps is launched here.  With two launches or more the
 // last one is gm_close_mu_kernel, which also forms mu' (returns true: no mu_circ_kernel).
+// g1 (a sequence of two launches): only the product launch; velocity_gm1_kernel finishes the solve.
 // Mixed precision runs the products over the FP32 matrix (4 bytes per entry).
-bool Solver::enqueue_gmres(const GmArgs& a) {
+bool Solver::enqueue_gmres(const GmArgs& a, bool g1) {
     const bool f32 = opt_.precision == kMixed;
     double nsq = 0.0;
     for (int v : n_) nsq += (double)v * v;
@@ -346,7 +350,7 @@ bool Solver::enqueue_gmres(const GmArgs& a) {
             },
             j);
     }
-    if (fused) {
+    if (fused && !g1) {
         // finishing the last step + x += V y + mu' = D1 mu: the rebuild and the circulant product
         launch("gm_close_mu", 4.0 * a.m * T_ + 4.0 * nsq, 8.0 * (a.m + 2) * T_ + 24.0 * nsq / kCircRowsPerWG, [&] {
             hipLaunchKernelGGL(gm_close_mu_kernel, dim3((maxN_ + kCircRowsPerWG - 1) / kCircRowsPerWG, S_), dim3(256),
@@ -362,6 +366,11 @@ bool Solver::enqueue_gmres(const GmArgs& a) {
         SFH_HIP(hipMemcpy(ctl.data(), gmctl_, ctl.size() * sizeof(int), hipMemcpyDeviceToHost));
         int iters = 0;
         for (int s = 0; s < S_; ++s) iters = std::max(iters, ctl[(size_t)kGmCtlInts * s + 2]);
+        if (g1) {   // the velocity kernel finishes the step: launch 0 ran a product unless it ended the solve
+            iters = 0;
+            for (int s = 0; s < S_; ++s)
+                if (ctl[(size_t)kGmCtlInts * s] == 0) iters = 1;
+        }
         for (size_t k = first; k < timed_.size(); ++k) {
             if (timed_[k].tag != std::string("gm_iter")) continue;
             if (timed_[k].iter == iters) {
@@ -383,9 +392,9 @@ bool Solver::enqueue_gmres(const GmArgs& a) {
 //   refinement solve over the FP32 matrix starts from it (it ends at launch 0 when the residual is
 //   already within the tolerance, the usual case).  Only the refinement solve stores the stage's
 //   warm-start history.
-bool Solver::enqueue_solve(const double2* Z, int stage) {
+bool Solver::enqueue_solve(const double2* Z, int stage, bool g1) {
     if (opt_.solver == kSolveGMRES) {
-        if (opt_.precision != kMixed) return enqueue_gmres(gm_args(stage));
+        if (opt_.precision != kMixed) return enqueue_gmres(gm_args(stage), g1);
         GmArgs first = gm_args(stage);
         first.stage = 0;   // no history: the refinement solve stores the solution
         enqueue_gmres(first);
@@ -507,16 +516,28 @@ void Solver::enqueue_rhs(const double2* Z, const double* P, const StageCtl& ctl)
                                d_desc_, (const NodeTrig*)trig_, Z, Zp_, Zpp_, M_, ph_, (const int*)nullptr, ga, gm_on);
         });
     }
-    if (!enqueue_solve(Z, ctl.stage)) {
+    // an FP64 GMRES sequence of two launches (one Arnoldi step) ends in the velocity kernel; not
+    // with the rho quirk, whose node-1 term needs mu' outside a workgroup's rows
+    const bool quirk = ph_.kind == 0 && ph_.rho_quirk && ph_.rho != 0.0;
+    const bool g1 = gm_on && opt_.precision != kMixed && ga.m == 2 && !quirk;
+    if (!enqueue_solve(Z, ctl.stage, g1)) {
         launch("mu_deriv", 4.0 * nsq, 24.0 * nsq / kCircRowsPerWG, [&] {
             hipLaunchKernelGGL(mu_circ_kernel, cgrid, dim3(256), 0, stream_, d_desc_, (const double*)d_circ_,
                                (const double*)mu_, mup_);
         });
     }
-    launch("velocity", img * nsq, 96.0 * T_, [&] {
-        hipLaunchKernelGGL(velocity_stage_kernel, dim3((maxN_ + kPairRows - 1) / kPairRows, S_), dim3(256), 0, stream_, d_desc_,
-                           (const NodeTrig*)trig_, Z, Zp_, Zpp_, mu_, mup_, ph_, ctl);
-    });
+    const dim3 vgrid((maxN_ + kPairRows - 1) / kPairRows, S_);
+    if (g1) {
+        launch("velocity", img * nsq, 96.0 * T_, [&] {
+            hipLaunchKernelGGL(velocity_gm1_kernel, vgrid, dim3(256), 0, stream_, d_desc_, (const NodeTrig*)trig_, Z,
+                               Zp_, Zpp_, mup_, ph_, ctl, ga, (const double*)d_circ_);
+        });
+    } else {
+        launch("velocity", img * nsq, 96.0 * T_, [&] {
+            hipLaunchKernelGGL(velocity_stage_kernel, vgrid, dim3(256), 0, stream_, d_desc_, (const NodeTrig*)trig_, Z,
+                               Zp_, Zpp_, mu_, mup_, ph_, ctl);
+        });
+    }
     SFH_HIP(hipGetLastError());
 }
 

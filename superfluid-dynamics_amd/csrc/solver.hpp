@@ -104,7 +104,7 @@ class Solver {
 
   private:
     void enqueue_rhs(const double2* Z, const double* P, const StageCtl& ctl);
-    bool enqueue_solve(const double2* Z, int stage);
+    bool enqueue_solve(const double2* Z, int stage, bool g1);
     void enqueue_step(double dt);
     void capture(double dt, int len);
     int step_len();                 // GMRES launches per solve for the next step (trimming policy)
@@ -158,7 +158,7 @@ class Solver {
     float* M32_ = nullptr;       // FP32 matrices of the mixed-precision GMRES form
     int* gmctl_ = nullptr;
     long long gm_part_stride_ = 0, gm_small_stride_ = 0;
-    bool enqueue_gmres(const GmArgs& a);
+    bool enqueue_gmres(const GmArgs& a, bool g1 = false);
     GmArgs gm_args(int stage) const;
     double* gmhist_ = nullptr;   // 4 x kHistDepth x total: per RK stage, mu of the last steps
     int* gmhvalid_ = nullptr;

@@ -315,6 +315,43 @@ def test_gmres_trimmed_sequence_matches_lu(csh, kw):
     assert np.abs(Pg - Pl).max() < 1e-11
 
 
+def test_gmres_one_step_sequence_ends_in_velocity_kernel(csh):
+    # Once every solve takes one Arnoldi step, the two-launch sequence (the product, then
+    # velocity_gm1_kernel, which finishes the step, stores mu and forms mu' from the partial sums
+    # of the product launch) has no gm_close_mu_kernel launch.  Kernel-timed steps run it eagerly
+    # between graph-replayed ones; the trajectory against the LU stepper after 30 steps (FP64,
+    # 1e-11).  Water: helium solves take 4-6 Arnoldi steps from the warm start (deep or finite
+    # depth, dt = 0.002 or 1e-4), so they never reach this path.
+    kw, dt, tol, bound = {}, 0.002, 1e-12, 1e-11
+    n = 2048
+    Z0 = _surface(n, 17)
+    P0 = 0.005 * np.sin(2 * np.pi * np.arange(n) / n)
+    pc = csh.Props(**kw)
+    with csh.Ensemble([n], pc, solver=csh.LU) as e:
+        e.set_state(Z0, P0)
+        e.step(dt, 30)
+        Zl, Pl = e.get_state()
+    with csh.Ensemble([n], pc, solver=csh.GMRES, tol=tol, max_iter=32) as e:
+        e.set_state(Z0, P0)
+        e.step(dt, 12)
+        e.kernel_timing(True)
+        e.step(dt, 3)
+        e.sync()
+        close = e.kernel_report("gm_close_mu")
+        vel = e.kernel_report("velocity")
+        prod = e.kernel_report("gm_iter")
+        e.kernel_timing(False)
+        e.step(dt, 15)
+        Zg, Pg = e.get_state()
+        st = e.stats()
+    print(f"one-step sequence {kw}: close {close['launches']} velocity {vel['launches']} products "
+          f"{prod['launches']} unconverged {st['unconverged']} |dZ| {np.abs(Zg - Zl).max():.2e}", flush=True)
+    assert close["launches"] == 0 and vel["launches"] == 12 and prod["launches"] == 12
+    assert st["unconverged"] == 0
+    assert np.abs(Zg - Zl).max() < bound
+    assert np.abs(Pg - Pl).max() < bound
+
+
 def test_gmres_uneven_ensemble_matches_oracle(oracle, csh):
     # several surfaces of different N in one GMRES solve sequence (one grid slice per surface,
     # each with its own control block, partial dots and history), 12 steps: the trimmed sequence
