@@ -18,7 +18,8 @@ Other legs: `c1` (N = 256, y = 0.1 cos x, dt = 0.01, 100 steps, LU and GMRES, er
 `film` (N = 4096 helium film with the image term) and `adjoint` (gradient through the adjoint).
 
 Prints one JSON line.  `roofline` is measured live on the dominant kernel of the headline solver
-with hipEvents around each of its launches on the solver's stream; `cpu_baseline` times the CPU
+with a pair of hipEvents per launch on the solver's stream, recorded by the launch itself
+(hipExtLaunchKernelGGL: the kernel's own start and end, without the dispatch); `cpu_baseline` times the CPU
 FP64 oracle (OpenMP, all host threads) on a bounded sample of the headline workload and also serves
 as its accuracy reference.
 """
@@ -312,7 +313,7 @@ def main():
     ms_per_step = 1e3 * el / args.steps
     st_main = e.stats()
 
-    # live kernel timing: a few steps with hipEvents around every launch
+    # live kernel timing: a few steps with a hipEvent pair on every launch (its start and end)
     e.kernel_timing(True)
     kt_steps = 3
     e.step(DT, kt_steps)

@@ -20,6 +20,8 @@
 #include <cmath>
 #include <cstring>
 
+#include <hip/hip_ext.h>
+
 #include "sfh_gmres.hpp"
 #include "solver.hpp"
 
@@ -257,6 +259,15 @@ Solver::~Solver() {
     if (stream_) (void)hipStreamDestroy(stream_);
 }
 
+template <typename K, typename... Args>
+void Solver::go(K kernel, dim3 grid, dim3 block, unsigned shm, Args... args) {
+    if (ev_a_)
+        hipExtLaunchKernelGGL(kernel, grid, block, shm, stream_, ev_a_, ev_b_, 0, args...);
+    else
+        hipLaunchKernelGGL(kernel, grid, block, shm, stream_, args...);
+}
+
+// f makes exactly one kernel launch through go()
 template <class F>
 void Solver::launch(const char* tag, double flops, double bytes, F&& f, int iter) {
     if (!ktime_) {
@@ -269,9 +280,10 @@ void Solver::launch(const char* tag, double flops, double bytes, F&& f, int iter
         event_pool_.push_back(e);
     }
     hipEvent_t a = event_pool_[event_next_++], b = event_pool_[event_next_++];
-    SFH_HIP(hipEventRecord(a, stream_));
+    ev_a_ = a;
+    ev_b_ = b;
     f();
-    SFH_HIP(hipEventRecord(b, stream_));
+    ev_a_ = ev_b_ = nullptr;
     timed_.push_back({tag, a, b, flops, bytes, iter, true});
 }
 
@@ -342,10 +354,10 @@ bool Solver::enqueue_gmres(const GmArgs& a, bool g1) {
             (f32 ? 4.0 : 8.0) * nsq + 8.0 * (j + 3) * T_,
             [&] {
                 if (f32)
-                    hipLaunchKernelGGL(gm_iter_kernel<float>, grid, dim3(256), 0, stream_, a,
+                    go(gm_iter_kernel<float>, grid, dim3(256), 0, a,
                                        (const SurfDesc*)d_desc_, j);
                 else
-                    hipLaunchKernelGGL(gm_iter_kernel<double>, grid, dim3(256), 0, stream_, a,
+                    go(gm_iter_kernel<double>, grid, dim3(256), 0, a,
                                        (const SurfDesc*)d_desc_, j);
             },
             j);
@@ -353,8 +365,8 @@ bool Solver::enqueue_gmres(const GmArgs& a, bool g1) {
     if (fused && !g1) {
         // finishing the last step + x += V y + mu' = D1 mu: the rebuild and the circulant product
         launch("gm_close_mu", 4.0 * a.m * T_ + 4.0 * nsq, 8.0 * (a.m + 2) * T_ + 24.0 * nsq / kCircRowsPerWG, [&] {
-            hipLaunchKernelGGL(gm_close_mu_kernel, dim3((maxN_ + kCircRowsPerWG - 1) / kCircRowsPerWG, S_), dim3(256),
-                               0, stream_, a, (const SurfDesc*)d_desc_, (const double*)d_circ_, mup_, a.m - 1);
+            go(gm_close_mu_kernel, dim3((maxN_ + kCircRowsPerWG - 1) / kCircRowsPerWG, S_), dim3(256),
+                               0, a, (const SurfDesc*)d_desc_, (const double*)d_circ_, mup_, a.m - 1);
         });
     }
     if (ktime_ && !a.check_first) {
@@ -402,8 +414,8 @@ bool Solver::enqueue_solve(const double2* Z, int stage, bool g1) {
         double nsq = 0.0;
         for (int v : n_) nsq += (double)v * v;
         launch("residual", (ph_.image ? 2.0 : 1.0) * nsq, 16.0 * T_, [&] {
-            hipLaunchKernelGGL(residual64_kernel, dim3((maxN_ + kPairRows - 1) / kPairRows, S_), dim3(256), 0,
-                               stream_, d_desc_, (const NodeTrig*)trig_, Z, Zp_, Zpp_, ph_, refine);
+            go(residual64_kernel, dim3((maxN_ + kPairRows - 1) / kPairRows, S_), dim3(256), 0,
+                               d_desc_, (const NodeTrig*)trig_, Z, Zp_, Zpp_, ph_, refine);
         });
         refine.check_first = 1;
         return enqueue_gmres(refine);
@@ -425,18 +437,18 @@ bool Solver::enqueue_solve(const double2* Z, int stage, bool g1) {
             // panel: 2 kb^3 (two 64^3-class products per row/column block) + the diagonal block
             const double pf = (double)count * (4.0 * kb * kb * rem + 2.0 * kb * kb * kb);
             launch("lu_panel", pf, 8.0 * count * (2.0 * kb * rem + kb * kb), [&] {
-                hipLaunchKernelGGL(lu_panel_kernel, dim3(1 + 2 * nrb, 1, count), dim3(256), 0, stream_, a, k0);
+                go(lu_panel_kernel, dim3(1 + 2 * nrb, 1, count), dim3(256), 0, a, k0);
             });
             if (rem > 0)
                 launch("lu_update", (double)count * 2.0 * rem * rem * kb,
                        8.0 * count * (2.0 * rem * rem + 2.0 * rem * kb), [&] {
-                           hipLaunchKernelGGL(lu_update_kernel, dim3(nrb, nrb, count), dim3(256), 0, stream_, a,
+                           go(lu_update_kernel, dim3(nrb, nrb, count), dim3(256), 0, a,
                                               k0);
                        });
         }
         const int P = (n + kNB - 1) / kNB;
         launch("lu_backsolve", (double)count * n * n, 8.0 * count * 0.5 * n * n, [&] {
-            hipLaunchKernelGGL(lu_backsolve_kernel, dim3(P, 1, count), dim3(256), 0, stream_, a);
+            go(lu_backsolve_kernel, dim3(P, 1, count), dim3(256), 0, a);
         });
     };
     if (uniform_) {
@@ -445,9 +457,9 @@ bool Solver::enqueue_solve(const double2* Z, int stage, bool g1) {
         for (int s = 0; s < S_; ++s) run(s, 1, n_[s]);
     }
     // systems whose interchange-free check failed: re-assemble and solve with GEPP
-    hipLaunchKernelGGL(assemble64_kernel, dim3((maxN_ + kPairRows - 1) / kPairRows, S_), dim3(256), 0, stream_, d_desc_,
+    go(assemble64_kernel, dim3((maxN_ + kPairRows - 1) / kPairRows, S_), dim3(256), 0, d_desc_,
                        (const NodeTrig*)trig_, Z, Zp_, Zpp_, M_, ph_, (const int*)status_, GmArgs{}, 0);
-    hipLaunchKernelGGL(gepp_fallback_kernel, dim3(S_), dim3(256), 0, stream_, d_desc_, M_, b_, mu_, piv_,
+    go(gepp_fallback_kernel, dim3(S_), dim3(256), 0, d_desc_, M_, b_, mu_, piv_,
                        status_, counters_);
     return false;
 }
@@ -482,7 +494,7 @@ void Solver::enqueue_batched(const double2* Z, const double* P, const StageCtl& 
     // algorithmic work per launch: cot pairs of assembly + velocity (and one more pass per
     // refinement residual), matrix written once and read once per GMRES product
     launch("stage", 2.0 * img * nsq, 2.0 * mbytes, [&] {
-        hipLaunchKernelGGL(batched_stage_kernel, dim3(S_), dim3(kBatchThreads), 0, stream_, a, ph_, ctl);
+        go(batched_stage_kernel, dim3(S_), dim3(kBatchThreads), 0, a, ph_, ctl);
     });
     SFH_HIP(hipGetLastError());
 }
@@ -501,19 +513,17 @@ void Solver::enqueue_rhs(const double2* Z, const double* P, const StageCtl& ctl)
     // circulant derivatives: 7 n^2 multiply-adds; bytes: the samples and kernels each workgroup
     // stages (24 B per node and 24 B per kernel entry)
     launch("deriv", 14.0 * nsq, 48.0 * nsq / kCircRowsPerWG, [&] {
-        hipLaunchKernelGGL(zphi_circ_kernel, cgrid, dim3(256), 0, stream_, d_desc_, (const double*)d_circ_, Z, P,
+        go(zphi_circ_kernel, cgrid, dim3(256), 0, d_desc_, (const double*)d_circ_, Z, P,
                            Zp_, Zpp_, b_, work_, ph_, status_, trig_, ga, gm_on);
     });
     // assembly: flops counted as cot evaluations (the transcendental-bound unit), bytes = M written
     if (gm_on && opt_.precision == kMixed) {
         launch("assemble", img * nsq, 4.0 * nsq, [&] {
-            hipLaunchKernelGGL(assemble32_kernel, dim3((maxN_ + kPairRows - 1) / kPairRows, S_), dim3(256), 0, stream_,
-                               d_desc_, (const NodeTrig*)trig_, Z, Zp_, Zpp_, M32_, ph_, ga);
+            go(assemble32_kernel, dim3((maxN_ + kPairRows - 1) / kPairRows, S_), dim3(256), 0, d_desc_, (const NodeTrig*)trig_, Z, Zp_, Zpp_, M32_, ph_, ga);
         });
     } else {
         launch("assemble", img * nsq, 8.0 * nsq, [&] {
-            hipLaunchKernelGGL(assemble64_kernel, dim3((maxN_ + kPairRows - 1) / kPairRows, S_), dim3(256), 0, stream_,
-                               d_desc_, (const NodeTrig*)trig_, Z, Zp_, Zpp_, M_, ph_, (const int*)nullptr, ga, gm_on);
+            go(assemble64_kernel, dim3((maxN_ + kPairRows - 1) / kPairRows, S_), dim3(256), 0, d_desc_, (const NodeTrig*)trig_, Z, Zp_, Zpp_, M_, ph_, (const int*)nullptr, ga, gm_on);
         });
     }
     // an FP64 GMRES sequence of two launches (one Arnoldi step) ends in the velocity kernel; not
@@ -522,19 +532,19 @@ void Solver::enqueue_rhs(const double2* Z, const double* P, const StageCtl& ctl)
     const bool g1 = gm_on && opt_.precision != kMixed && ga.m == 2 && !quirk;
     if (!enqueue_solve(Z, ctl.stage, g1)) {
         launch("mu_deriv", 4.0 * nsq, 24.0 * nsq / kCircRowsPerWG, [&] {
-            hipLaunchKernelGGL(mu_circ_kernel, cgrid, dim3(256), 0, stream_, d_desc_, (const double*)d_circ_,
+            go(mu_circ_kernel, cgrid, dim3(256), 0, d_desc_, (const double*)d_circ_,
                                (const double*)mu_, mup_);
         });
     }
     const dim3 vgrid((maxN_ + kPairRows - 1) / kPairRows, S_);
     if (g1) {
         launch("velocity", img * nsq, 96.0 * T_, [&] {
-            hipLaunchKernelGGL(velocity_gm1_kernel, vgrid, dim3(256), 0, stream_, d_desc_, (const NodeTrig*)trig_, Z,
+            go(velocity_gm1_kernel, vgrid, dim3(256), 0, d_desc_, (const NodeTrig*)trig_, Z,
                                Zp_, Zpp_, mup_, ph_, ctl, ga, (const double*)d_circ_);
         });
     } else {
         launch("velocity", img * nsq, 96.0 * T_, [&] {
-            hipLaunchKernelGGL(velocity_stage_kernel, vgrid, dim3(256), 0, stream_, d_desc_, (const NodeTrig*)trig_, Z,
+            go(velocity_stage_kernel, vgrid, dim3(256), 0, d_desc_, (const NodeTrig*)trig_, Z,
                                Zp_, Zpp_, mu_, mup_, ph_, ctl);
         });
     }
@@ -763,7 +773,7 @@ void Solver::derivative(const double2* in_host, double2* out_host, bool second, 
     SFH_HIP(hipMemcpyAsync(Za_, in_host, T_ * sizeof(double2), hipMemcpyHostToDevice, stream_));
     const int lds_tw = (2 * (size_t)maxN_ * sizeof(double2) <= 144 * 1024) ? 1 : 0;
     const size_t lds = (size_t)maxN_ * sizeof(double2) * (lds_tw ? 2 : 1);
-    hipLaunchKernelGGL(plain_deriv_kernel, dim3(S_), dim3(kDerivThreads), lds, stream_, d_desc_, d_tw_, Za_, Zb_,
+    go(plain_deriv_kernel, dim3(S_), dim3(kDerivThreads), lds, d_desc_, d_tw_, Za_, Zb_,
                        second ? 1 : 0, scale, lds_tw);
     SFH_HIP(hipGetLastError());
     SFH_HIP(hipMemcpyAsync(out_host, Zb_, T_ * sizeof(double2), hipMemcpyDeviceToHost, stream_));
@@ -887,7 +897,7 @@ void Solver::enqueue_vjp(const double2* Z, const double* P, const double* mu, co
     double nsq = 0.0;
     for (int v : n_) nsq += (double)v * v;
     launch("vjp", 3.0 * nsq, 0.0, [&] {
-        hipLaunchKernelGGL(rhs_vjp_kernel, dim3(S_), dim3(kBatchThreads), 0, stream_, a, ph_);
+        go(rhs_vjp_kernel, dim3(S_), dim3(kBatchThreads), 0, a, ph_);
     });
     SFH_HIP(hipGetLastError());
 }

@@ -112,6 +112,11 @@ class Solver {
     void check_status();
     template <class F>
     void launch(const char* tag, double flops, double bytes, F&& f, int iter = -1);
+    // A kernel launch on stream_.  Inside a timed launch() it records the kernel's own start and
+    // end into that launch's event pair (hipExtLaunchKernelGGL), as a profiler times a kernel.
+    template <typename K, typename... Args>
+    void go(K kernel, dim3 grid, dim3 block, unsigned shm, Args... args);
+    hipEvent_t ev_a_ = nullptr, ev_b_ = nullptr;
 
     struct Timed {
         const char* tag;
