@@ -599,6 +599,8 @@ void Solver::capture(double dt, int len) {
 // Steps that run every solve with the full launch cap: kHistDepth while the warm-start history
 // fills, then one with the full history that sets the trimmed length.
 constexpr int kCalibSteps = kHistDepth + 1;
+// Steps between checks of a trimmed sequence within one step() call.
+constexpr int kCheckSteps = 32;
 
 int Solver::step_len() {
     if (opt_.solver != kSolveGMRES) return 0;
@@ -614,7 +616,9 @@ void Solver::after_step(int len) {
         // steps of the following ones.  L launches run up to L - 1 steps.  Solves that need 1 or
         // 2 steps start from guesses whose error is at the level of the solve tolerance (the
         // extrapolation's truncation error is smaller), and get no spare launch; slower ones,
-        // whose step count varies more from solve to solve, get one spare step.
+        // whose step count varies more from solve to solve, get one spare step.  (A third of
+        // their steps as the spare cost the film 2% and still let helium solves whose count
+        // drifts up by 2 run out of launches; the periodic checks in step() catch those.)
         sync();
         const int need = fill_iters_ + 1;
         trim_ = std::max(2, std::min(gm_m_, fill_iters_ >= 3 ? need + 1 : need));
@@ -648,15 +652,20 @@ void Solver::step(double dt, int nsteps) {
         i += batch;
         if (check) {
             after_step(len);
-            if (hist_steps_ >= kCalibSteps && len < gm_m_ && i < nsteps) {   // rest of a trimmed batch
+            // the rest of a trimmed batch, checked every kCheckSteps steps, so that solves running
+            // out of launches lengthen the sequence within a bounded number of steps
+            while (hist_steps_ >= kCalibSteps && step_len() < gm_m_ && i < nsteps) {
                 const int len2 = step_len();
+                const int chunk = std::min(kCheckSteps, nsteps - i);
                 if (opt_.use_graph && !ktime_) {
                     if (!exec_ || graph_dt_ != dt || graph_len_ != len2) capture(dt, len2);
-                    for (; i < nsteps; ++i) SFH_HIP(hipGraphLaunch(exec_, stream_));
+                    for (int b = 0; b < chunk; ++b) SFH_HIP(hipGraphLaunch(exec_, stream_));
                 } else {
                     seq_len_ = len2;
-                    for (; i < nsteps; ++i) enqueue_step(dt);
+                    for (int b = 0; b < chunk; ++b) enqueue_step(dt);
                 }
+                i += chunk;
+                if (i < nsteps) after_step(len2);
             }
         }
     }
