@@ -324,13 +324,12 @@ __global__ __launch_bounds__(256) void velocity_stage_kernel(const SurfDesc* __r
 // Finishes a GMRES solve of one Arnoldi step (launch 0 of gm_iter_kernel ran its product), forms
 // mu and mu' and runs the velocity stage.  Not for the rho quirk, whose node-1 term needs mu' of a
 // row outside the workgroup's.  mup receives mu' and gm.sol mu.
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void velocity_gm1_kernel(const SurfDesc* __restrict__ sd,
-                                                           const NodeTrig* __restrict__ trig,
-                                                           const double2* __restrict__ Z,
-                                                           const double2* __restrict__ Zp,
-                                                           const double2* __restrict__ Zpp,
-                                                           double2* __restrict__ mup, Phys ph, StageCtl ctl,
-                                                           GmArgs gm, const double* __restrict__ circ) {
+// Two waves per SIMD (all 512 workgroups of an N = 2048 surface resident at once): the compiler
+// keeps the kernel within 256 VGPRs, spilling 2.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void velocity_gm1_kernel(
+    const SurfDesc* __restrict__ sd, const NodeTrig* __restrict__ trig, const double2* __restrict__ Z,
+    const double2* __restrict__ Zp, const double2* __restrict__ Zpp, double2* __restrict__ mup, Phys ph,
+    StageCtl ctl, GmArgs gm, const double* __restrict__ circ) {
     velocity_rows<true>(sd, trig, Z, Zp, Zpp, nullptr, nullptr, mup, ph, ctl, gm, circ);
 }
 
