@@ -352,6 +352,38 @@ def test_gmres_one_step_sequence_ends_in_velocity_kernel(csh):
     assert np.abs(Pg - Pl).max() < bound
 
 
+def test_gmres_one_step_sequence_uneven_ensemble(csh):
+    # the fused one-step path on an ensemble of uneven surfaces: each grid slice reduces its own
+    # surface's partial sums, stores its own mu and history, and closes its own control block;
+    # every surface against the LU stepper after 24 steps
+    sizes = [2048, 1024, 512]
+    Zs = [_surface(n, 40 + i) for i, n in enumerate(sizes)]
+    Ps = [0.004 * np.cos(2 * np.pi * np.arange(n) / n + i) for i, n in enumerate(sizes)]
+    Z0, P0 = np.concatenate(Zs), np.concatenate(Ps)
+    with csh.Ensemble(sizes, solver=csh.LU) as e:
+        e.set_state(Z0, P0)
+        e.step(0.002, 24)
+        Zl, Pl = e.get_state()
+    with csh.Ensemble(sizes, solver=csh.GMRES, tol=1e-12, max_iter=32) as e:
+        e.set_state(Z0, P0)
+        e.step(0.002, 10)
+        e.kernel_timing(True)
+        e.step(0.002, 2)
+        e.sync()
+        close = e.kernel_report("gm_close_mu")
+        vel = e.kernel_report("velocity")
+        e.kernel_timing(False)
+        e.step(0.002, 12)
+        Zg, Pg = e.get_state()
+        st = e.stats()
+    print(f"one-step ensemble: close {close['launches']} velocity {vel['launches']} unconverged "
+          f"{st['unconverged']} |dZ| {np.abs(Zg - Zl).max():.2e}", flush=True)
+    assert close["launches"] == 0 and vel["launches"] == 8
+    assert st["unconverged"] == 0
+    assert np.abs(Zg - Zl).max() < 1e-11
+    assert np.abs(Pg - Pl).max() < 1e-11
+
+
 def test_gmres_uneven_ensemble_matches_oracle(oracle, csh):
     # several surfaces of different N in one GMRES solve sequence (one grid slice per surface,
     # each with its own control block, partial dots and history), 12 steps: the trimmed sequence
