@@ -149,8 +149,8 @@ def batched_leg(c, steps, with_accuracy):
 
 def c1_leg(c, with_accuracy):
     """The reference's CPU-runnable case: N = 256, y = 0.1 cos x, phi = 0, water with g = 1 in deep
-    water, 100 RK4 steps of dt = 0.01 from a fresh state (for GMRES this includes the four steps that
-    fill the warm-start history and the graph captures), with the drop-in LU solver and with GMRES;
+    water, 100 RK4 steps of dt = 0.01 from a fresh state (for GMRES this includes the six full-length steps
+    that fill the warm-start history and set the trimmed sequence, and the graph captures), with the drop-in LU solver and with GMRES;
     the error of the final surface against the FP64 oracle run over the same 100 steps."""
     n, dt, steps = 256, 0.01, 100
     x = 2 * np.pi * np.arange(n) / n
