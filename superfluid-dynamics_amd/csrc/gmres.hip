@@ -1,5 +1,5 @@
 // gmres.hip -- FP64 GMRES for M mu = b over the stored dipole matrix, all workgroups of the GPU on
-// one or a few large surfaces (the north-star alternative to the LU of MatrixSolver::solve,
+// one or a few large surfaces (the iterative alternative to the LU of MatrixSolver::solve,
 // CuSuperHelium/MatrixSolver.cuh:113-172).
 //
 // M = I/2 + K is a second-kind operator: its spectrum clusters at 1/2 (for the helium film one
