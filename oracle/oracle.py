@@ -28,7 +28,7 @@ class _Props(ctypes.Structure):
         ("use_expansions", ctypes.c_int),
         ("expansion_order", ctypes.c_int),
         ("rho_quirk", ctypes.c_int),
-        ("pad_", ctypes.c_int),
+        ("image_zp", ctypes.c_int),
         ("rho", ctypes.c_double),
         ("kappa", ctypes.c_double),
         ("depth", ctypes.c_double),
@@ -59,10 +59,11 @@ class Props:
     use_expansions: bool = False
     expansion_order: int = 1
     rho_quirk: bool = True
+    image_zp: bool = False   # test-only: film image term of M with the Z'_k factor (not the reference)
 
     def c(self) -> _Props:
         return _Props(int(self.kind), int(self.infinite_depth), int(self.use_expansions),
-                      int(self.expansion_order), int(self.rho_quirk), 0, float(self.rho),
+                      int(self.expansion_order), int(self.rho_quirk), int(self.image_zp), float(self.rho),
                       float(self.kappa), float(self.depth), float(self.U), float(self.g))
 
 

@@ -53,7 +53,8 @@ typedef struct sfo_props {
     int use_expansions;
     int expansion_order;
     int rho_quirk;       /* 1: water Bernoulli reads V1[1] as in createM.cuh:104 */
-    int pad_;
+    int image_zp;        /* test-only, 0 = the reference: 1 multiplies the film image term of M by
+                            Z'_k like the direct term (createM.cuh:80,87-88 omit that factor) */
     double rho;
     double kappa;
     double depth;
@@ -235,7 +236,10 @@ void sfo_zphi_derivative(const sfo_props *p, int n, int batch, const double *Z, 
 }
 
 /* --------------------------------------------------------------- assembly */
-/* createMKernel (createM.cuh:43-63) for kind 0, createFiniteDepthMKernel (:65-92) for kind 1. */
+/* createMKernel (createM.cuh:43-63) for kind 0, createFiniteDepthMKernel (:65-92) for kind 1.
+ * With image_zp set (a test-only option, never the reference) the image term carries the factor
+ * Z'_k of the direct term; with it the tangential condition Re(conj(q_k) Z'_k) = Re(Phi'_k) holds
+ * for the film as it does for deep water, which pins the image terms of M and of the velocity. */
 void sfo_assemble(const sfo_props *p, int n, int batch, const double *Z, const double *Zp,
                   const double *Zpp, double *M)
 {
@@ -255,14 +259,15 @@ void sfo_assemble(const sfo_props *p, int n, int batch, const double *Z, const d
                     else
                         v = 0.25 * (1 - rho) / PI_D * cimag(zp[k] * cot_c(0.5 * (z[k] - z[j])));
                 } else {
+                    const cplx f = p->image_zp ? zp[k] : 1.0;
                     if (k == j) {
                         v = 0.5 + 0.25 / PI_D * cimag(zpp[k] / zp[k]);
                         if (!p->infinite_depth)
-                            v -= 0.25 / PI_D * cimag(cot_c(I * (cimag(z[k]) + h)));
+                            v -= 0.25 / PI_D * cimag(f * cot_c(I * (cimag(z[k]) + h)));
                     } else {
                         v = 0.25 / PI_D * cimag(zp[k] * cot_c(0.5 * (z[k] - z[j])));
                         if (!p->infinite_depth)
-                            v -= 0.25 / PI_D * cimag(cot_c(0.5 * (z[k] - conj(z[j])) + I * h));
+                            v -= 0.25 / PI_D * cimag(f * cot_c(0.5 * (z[k] - conj(z[j])) + I * h));
                     }
                 }
                 m[(size_t)k * n + j] = v;

@@ -217,6 +217,82 @@ def test_velocity_conserves_volume(oracle):
     assert abs(np.sum(np.imag(np.conj(d["Zp"]) * q))) < 1e-14
 
 
+@pytest.mark.parametrize("n,m,h", [(64, 1, 1.0), (128, 3, 0.5), (256, 2, 2.0)])
+def test_flat_film_closed_form(oracle, n, m, h):
+    # Flat film over a substrate at y = -h, Phi = eps sin(m x): the potential
+    # eps sin(m x) cosh(m (y + h)) / cosh(m h) gives q = u + i v = eps m (cos m x + i tanh(m h) sin m x)
+    # at y = 0.  With the image term of M carrying Z'_k (image_zp) the scheme reproduces it to
+    # rounding.  The reference's image term (createM.cuh:80,87-88, no Z'_k) is off by O(eps):
+    # at (h, m) = (0.5, 3) about 1.4e-3 on |q| = 3e-3; the test records that the defect is there.
+    x = 2 * np.pi * np.arange(n) / n
+    eps = 1e-3
+    exact = eps * m * (np.cos(m * x) + 1j * np.tanh(m * h) * np.sin(m * x))
+    q, dphi = oracle.rhs(oracle.Props(kind=oracle.HELIUM, depth=h, image_zp=True), x + 0j, eps * np.sin(m * x))
+    assert np.abs(q - exact).max() < 1e-15
+    q_ref, _ = oracle.rhs(oracle.Props(kind=oracle.HELIUM, depth=h), x + 0j, eps * np.sin(m * x))
+    assert np.abs(q_ref - exact).max() > 1e-3 * eps * m
+
+
+def _wavy(n, amp=0.05):
+    x = 2 * np.pi * np.arange(n) / n
+    Z = x + 1j * (amp * np.cos(x) + 0.3 * amp * np.sin(2 * x + 0.4) + 0.1 * amp * np.cos(5 * x + 1.0))
+    Phi = 0.02 * np.sin(x + 0.3) + 0.01 * np.cos(3 * x)
+    return Z, Phi
+
+
+@pytest.mark.parametrize("kind,h,n", [("water", 0.0, 128), ("water", 0.0, 256), ("film", 1.0, 256),
+                                      ("film", 0.3, 256), ("film", 0.5, 512)])
+def test_tangential_velocity_condition(oracle, kind, h, n):
+    # The dipole system is the tangential condition: Re(conj(q_k) Z'_k) = dphi/dj = Re(Phi'_k) on the
+    # surface (u x' + v y' = phi').  M (createM.cuh) and the velocity (WaterVelocities.cuh) are
+    # written independently, so a wrong sign, factor or index in either image term breaks the
+    # identity on a wavy surface.  For the film it needs the Z'_k factor in M's image term
+    # (image_zp); the velocity's image term is the reference's in both cases.
+    Z, Phi = _wavy(n, 0.08 if h == 0.3 else 0.05)
+    if kind == "water":
+        props = oracle.Props()
+    else:
+        props = oracle.Props(kind=oracle.HELIUM, depth=h, image_zp=True)
+    q, _, d = oracle.rhs(props, Z, Phi, diagnostics=True)
+    tang = np.real(np.conj(q) * d["Zp"])
+    assert np.abs(tang - d["Phip"].real).max() < 2e-16 * max(1.0, np.abs(d["Phip"]).max() * n)
+    # volume: no net normal flux through the surface (incompressible, closed bottom)
+    assert abs(np.sum(np.imag(np.conj(d["Zp"]) * q))) < 1e-15 * n
+    if kind == "film":
+        # the reference's own M leaves the condition violated by O(|Phi'| * image)
+        _, _, d0 = oracle.rhs(oracle.Props(kind=oracle.HELIUM, depth=h), Z, Phi, diagnostics=True)
+        q0, _ = oracle.rhs(oracle.Props(kind=oracle.HELIUM, depth=h), Z, Phi)
+        assert np.abs(np.real(np.conj(q0) * d0["Zp"]) - d0["Phip"].real).max() > 1e-6
+
+
+@pytest.mark.parametrize("lower", [True, False])
+def test_film_velocity_brute_force_mpmath(oracle, lower):
+    # createHeliumVelocityMatrices (WaterVelocities.cuh:72-107) + calculateVelocities (:205-242),
+    # evaluated entry by entry in 40-digit arithmetic: V1_kk = -i Z''_k / (4 pi Z'_k^2)
+    # + i cot(i (y_k + h)) / 4 pi +- 1/(2 Z'_k), V1_kj = -i cot((z_k - z_j)/2) / 4 pi
+    # + i cot((z_k - conj z_j)/2 + i h) / 4 pi, V2_k = i / (2 pi Z'_k), q = conj(V1 mu + V2 mu').
+    mpmath.mp.dps = 40
+    n, h = 7, 0.7
+    Z, _, Zp, Zpp, _ = _linear_wave(n, 0.15, 1.0, 0.2, 0.0)
+    rng = np.random.default_rng(11)
+    mu = rng.standard_normal(n)
+    mup = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    q = oracle.velocity(oracle.Props(kind=oracle.HELIUM, depth=h), Z, Zp, Zpp, mu, mup, lower=lower)
+    c = lambda v: mpmath.mpc(v.real, v.imag)  # noqa: E731
+    I4 = mpmath.mpc(0, 1) / (4 * mpmath.pi)
+    for k in range(n):
+        w = mpmath.mpc(0, 1) / (2 * mpmath.pi * c(Zp[k])) * c(mup[k])
+        for j in range(n):
+            if k == j:
+                v1 = -I4 * c(Zpp[k]) / c(Zp[k]) ** 2 + I4 * mpmath.cot(mpmath.mpc(0, Z[k].imag + h))
+                v1 += (1 if lower else -1) / (2 * c(Zp[k]))
+            else:
+                v1 = -I4 * mpmath.cot((c(Z[k]) - c(Z[j])) / 2)
+                v1 += I4 * mpmath.cot((c(Z[k]) - mpmath.conj(c(Z[j]))) / 2 + mpmath.mpc(0, h))
+            w += v1 * mu[j]
+        assert abs(q[k] - complex(mpmath.conj(w))) < 1e-14
+
+
 # ------------------------------------------------------------------ Bernoulli (A8)
 def test_helium_bernoulli_matches_taylor_expansion(oracle):
     # (h/3)((1 + y/h)^-3 - 1) = -y + 2 y^2/h - (10/3) y^3/h^2 + O(y^4): the plain kernel
@@ -239,6 +315,27 @@ def test_water_bernoulli_deep_water_value(oracle):
     q = np.array([0.5 + 0.25j, -1.0 + 0j, 0.0 + 2.0j])
     out = oracle.rhs_phi(oracle.Props(), 1j * y, q, q, q, np.zeros(3))
     assert np.allclose(out, -y + 0.5 * np.abs(q) ** 2, rtol=0, atol=1e-16)
+
+
+def test_helium_surface_tension_branch(oracle):
+    # compute_rhs_helium_phi_expression_with_surface_tension (createM.cuh:200-211): kappa times the
+    # curvature (x'y'' - y'x'')/(x'^2 + y'^2)^{3/2}, plus the van der Waals term with the literal
+    # prefactor 20.447761896665416 of createM.cuh:209.  The curvature of y = a cos x is
+    # y_xx / (1 + y_x^2)^{3/2} whatever the parametrisation scale s (Z' = s(1 + i y_x),
+    # Z'' = i s^2 y_xx), and with q = 0 the branch is 20.447761896665416 times the plain one.
+    n, a, h, s = 32, 0.3, 1.0, 2 * np.pi / 32
+    x = 2 * np.pi * np.arange(n) / n
+    y = 0.05 * np.cos(x)
+    Z = x + 1j * y
+    zp = s * (1 - 1j * a * np.sin(x))
+    zpp = s * s * (-1j * a * np.cos(x))
+    curv = -a * np.cos(x) / (1 + a * a * np.sin(x) ** 2) ** 1.5
+    q0 = np.zeros(n, complex)
+    r1 = oracle.rhs_phi(oracle.Props(kind=oracle.HELIUM, depth=h, kappa=0.2), Z, zp, zpp, q0, q0)
+    r2 = oracle.rhs_phi(oracle.Props(kind=oracle.HELIUM, depth=h, kappa=0.7), Z, zp, zpp, q0, q0)
+    assert np.abs((r2 - r1) / 0.5 - curv).max() < 1e-14
+    plain = oracle.rhs_phi(oracle.Props(kind=oracle.HELIUM, depth=h), Z, zp, zpp, q0, q0)
+    assert np.abs((r1 - 0.2 * curv) - 20.447761896665416 * plain).max() < 4e-15
 
 
 # ------------------------------------------------------------------ RK4 (A10)
