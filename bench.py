@@ -1,12 +1,15 @@
 """Benchmark: RK4 steps per second of one periodic water surface of N = 2048 nodes in FP64, plus
-the batched mixed-precision ensemble leg.
+the batched mixed-precision ensemble leg and the other workloads.
 
 Headline workload (`value`): nodes x_j = 2 pi j / N, y = sum_{k=1..8} a_k cos(k x + theta_k) with
 a_k ~ U(0, 0.02/k), theta_k ~ U(0, 2 pi) (numpy default_rng(20261202)), phi = 0, gravity 1, infinite
 depth, dt = 0.002.  A step is one classical RK4 step (four right-hand sides: spectral derivatives,
 dipole-matrix assembly, solve, velocity + Bernoulli + stage update), state resident in HBM, steps
 replayed from a captured hipGraph.  Default solver: FP64 GMRES (relative residual 1e-12, warm
-started); --solver lu selects the blocked FP64 LU.
+started); --solver lu selects the blocked FP64 LU.  The warm-up runs the requested W steps and,
+for GMRES, continues (untimed) through the solver's calibration: 5 steps fill the warm-start
+history, 1 sets the trimmed launch sequence and 2 verify it, so the timed K steps run the
+steady-state graph.
 
 Batched leg (`batched`): 512 independent surfaces with N drawn uniformly from {128, 256, 512, 1024}
 (default_rng(20261202)), surface i drawn with seed 20261202 + i as above, phi = 0, dt = 0.002; one
@@ -14,14 +17,18 @@ persistent launch per RK stage, mixed precision (FP32 matrix, FP64-refined mu) a
 compared with the plain FP64 stepper run over the same 512 surfaces one surface at a time:
 ensemble steps/s = 1 / sum_i t_step(N_i).
 
-Other legs: `c1` (N = 256, y = 0.1 cos x, dt = 0.01, 100 steps, LU and GMRES, error vs the oracle),
-`film` (N = 4096 helium film with the image term) and `adjoint` (gradient through the adjoint).
+Other legs: `c2_lu` (the headline surface with the drop-in LU solver), `c1` (N = 256,
+y = 0.1 cos x, dt = 0.01, 100 steps, LU and GMRES), `film` (N = 4096 helium film with the image
+term) and `adjoint` (gradient through the adjoint).
+
+Accuracy: every workload is compared with golden states of the CPU FP64 oracle stored in
+tests/golden/ (the same seeded inputs): max |z - z_oracle| and max |mu - mu_oracle| (mu of the
+right-hand side at the final state), per instance for the ensemble.
 
 Prints one JSON line.  `roofline` is measured live on the dominant kernel of the headline solver
 with a pair of hipEvents per launch on the solver's stream, recorded by the launch itself
 (hipExtLaunchKernelGGL: the kernel's own start and end, without the dispatch); `cpu_baseline` times the CPU
-FP64 oracle (OpenMP, all host threads) on a bounded sample of the headline workload and also serves
-as its accuracy reference.
+FP64 oracle (OpenMP, all host threads) on a bounded sample of the headline workload.
 """
 import argparse
 import json
@@ -40,22 +47,54 @@ METRIC = "RK4 steps/s (N=2048 FP64; batched mixed-prec) at ≤1e-9 err vs FP64 o
 PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
 N = 2048
-DT = 0.002
-SEED = 20261202
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic.json")
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+CALIB_STEPS = 8   # GMRES: history fill (5) + calibration (1) + first verified chunk (2)
+
+from cusuperhelium import workloads as W  # noqa: E402  (pure numpy: no device, no library)
+
+DT, SEED = W.DT, W.SEED
+surface = W.surface
 
 
-def surface(n, seed, amp=0.02, modes=8):
-    rng = np.random.default_rng(seed)
-    x = 2 * np.pi * np.arange(n) / n
-    a = rng.uniform(0, amp / np.arange(1, modes + 1))
-    th = rng.uniform(0, 2 * np.pi, modes)
-    return x + 1j * sum(a[k] * np.cos((k + 1) * x + th[k]) for k in range(modes))
+def golden(name):
+    """Golden oracle states (tests/golden/*.npz): steps, Z, phi, mu[, sizes, index]."""
+    with np.load(os.path.join(GOLDEN, name)) as f:
+        return {k: f[k] for k in f.files}
 
 
-def ensemble_sizes(count=512, seed=SEED):
-    rng = np.random.default_rng(seed)
-    return [int(v) for v in rng.choice([128, 256, 512, 1024], size=count)]
+def errors(Z, P, mu, Zr, Pr, mur):
+    return {"max_abs_dz": float(np.abs(Z - Zr).max()), "max_abs_dphi": float(np.abs(P - Pr).max()),
+            "max_abs_dmu": float(np.abs(mu - mur).max())}
+
+
+def trajectory_errors(e, g, dt):
+    """Runs the handle from its current state through the fixture's step marks; errors at each mark,
+    mu of the right-hand side at the last one."""
+    out, done, dz, dp = {}, 0, [], []
+    for k, m in enumerate(g["steps"]):
+        e.step(dt, int(m) - done)
+        done = int(m)
+        Z, P = e.get_state()
+        dz.append(float(np.abs(Z - g["Z"][k]).max()))
+        dp.append(float(np.abs(P - g["phi"][k]).max()))
+    _, _, d = e.rhs(diagnostics=True)
+    out["steps"] = [int(m) for m in g["steps"]]
+    out["max_abs_dz"] = dz
+    out["max_abs_dphi"] = dp
+    out["max_abs_dmu_final"] = float(np.abs(d["mu"] - g["mu"][-1]).max())
+    return out
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def cpu_baseline(Z0, P0, budget_s=25.0, max_steps=40):
@@ -82,8 +121,9 @@ def measured_traffic(kernel):
         return None
 
 
-def timed_steps(e, steps, warmup):
-    e.step(DT, warmup)
+def timed_steps(e, steps, warmup, calib=0):
+    """W warm-up steps (at least `calib`), then the wall time of exactly `steps` steps."""
+    e.step(DT, max(warmup, calib))
     e.sync()
     t0 = time.perf_counter()
     e.step(DT, steps)
@@ -92,8 +132,8 @@ def timed_steps(e, steps, warmup):
 
 
 def batched_leg(c, steps, with_accuracy):
-    sizes = ensemble_sizes()
-    Z0 = np.concatenate([surface(n, SEED + i) for i, n in enumerate(sizes)])
+    sizes, Zs = W.c3()
+    Z0 = np.concatenate(Zs)
     P0 = np.zeros(len(Z0))
     out = {"workload": "512 water surfaces, N uniform in {128,256,512,1024} (seeded), 8-mode draws, phi=0, "
                        "g=1, deep water, dt=0.002; one persistent launch per RK stage",
@@ -113,52 +153,55 @@ def batched_leg(c, steps, with_accuracy):
                      "unconverged": st["unconverged"]}
     # the plain FP64 stepper (the drop-in: blocked LU; and FP64 GMRES) one surface at a time
     plain = {}
-    for solver, name, k in ((c.LU, "lu", 6), (c.GMRES, "gmres", 20)):
+    for solver, name, k, mi, cal in ((c.LU, "lu", 6, 8, 0), (c.GMRES, "gmres", 20, 8, CALIB_STEPS)):
         t = {}
         for n in sorted(set(sizes)):
-            with c.Ensemble([n], c.Props(), solver=solver, tol=1e-12, max_iter=8) as e:
+            with c.Ensemble([n], c.Props(), solver=solver, tol=1e-12, max_iter=mi) as e:
                 e.set_state(surface(n, SEED), np.zeros(n))
-                t[n] = timed_steps(e, k, 2) / k
+                t[n] = timed_steps(e, k, 2, cal) / k
         ens = 1.0 / sum(t[n] for n in sizes)
         plain[name] = {"ensemble_steps_per_s": round(ens, 4), "t_step_ms": {str(n): round(1e3 * v, 4) for n, v in t.items()}}
     out["plain_fp64_one_at_a_time"] = plain
     out["speedup_mixed_vs_plain_lu"] = round(out["mixed"]["ensemble_steps_per_s"] / plain["lu"]["ensemble_steps_per_s"], 1)
     out["speedup_mixed_vs_plain_gmres"] = round(out["mixed"]["ensemble_steps_per_s"] / plain["gmres"]["ensemble_steps_per_s"], 1)
     if with_accuracy:
-        # error vs the FP64 oracle on a sample: the first N=1024 and the first N=128 surface (each
-        # surface evolves independently, so a two-surface ensemble reproduces them exactly)
-        from oracle import oracle as O
-        O.build()
-        idx = [sizes.index(1024), sizes.index(128)]
-        acc_steps = 20
+        # every one of the 512 instances after 20 steps against the oracle's golden states, and two
+        # surfaces of each N over 500 steps
+        g = golden("c3_ensemble_512_20steps.npz")
+        off = np.concatenate([[0], np.cumsum(sizes)])
+        for prec, name in ((c.MIXED, "accuracy_mixed"), (c.FP64, "accuracy_fp64")):
+            with c.Ensemble(sizes, c.Props(), precision=prec, solver=c.BATCHED, tol=1e-12, max_iter=16) as e:
+                e.set_state(Z0, P0)
+                e.step(DT, int(g["steps"]))
+                Z, _ = e.get_state()
+                _, _, d = e.rhs(diagnostics=True)
+            dz = np.array([np.abs(Z[a:b] - g["Z"][a:b]).max() for a, b in zip(off[:-1], off[1:])])
+            dm = np.array([np.abs(d["mu"][a:b] - g["mu"][a:b]).max() for a, b in zip(off[:-1], off[1:])])
+            out[name] = {"instances": len(sizes), "steps": int(g["steps"]),
+                         "max_abs_dz": {"max": float(dz.max()), "median": float(np.median(dz)),
+                                        "worst_instance": int(np.argmax(dz))},
+                         "max_abs_dmu": {"max": float(dm.max()), "median": float(np.median(dm))},
+                         "instances_within_1e-9": int((dz <= 1e-9).sum())}
+        g = golden("c3_sample_500steps.npz")
+        idx = [int(i) for i in g["index"]]
         sub = [sizes[i] for i in idx]
-        Zs = [surface(sizes[i], SEED + i) for i in idx]
-        with c.Ensemble(sub, c.Props(), precision=c.MIXED, solver=c.BATCHED, tol=1e-12) as e:
-            e.set_state(np.concatenate(Zs), np.zeros(sum(sub)))
-            e.step(DT, acc_steps)
-            Zg, _ = e.get_state()
-        errs, o = [], 0
-        for i, n in enumerate(sub):
-            Zr, _ = O.rk4(O.Props(), Zs[i], np.zeros(n), DT, acc_steps)
-            errs.append(float(np.abs(Zg[o:o + n] - Zr).max()))
-            o += n
-        out["accuracy_mixed"] = {"surfaces": [f"#{idx[0]} N=1024", f"#{idx[1]} N=128"], "steps": acc_steps,
-                                 "max_abs_dz_vs_oracle": errs}
+        with c.Ensemble(sub, c.Props(), precision=c.MIXED, solver=c.BATCHED, tol=1e-12, max_iter=16) as e:
+            e.set_state(np.concatenate([Zs[i] for i in idx]), np.zeros(sum(sub)))
+            out["accuracy_mixed_500_steps"] = dict(surfaces=[f"#{i} N={sizes[i]}" for i in idx],
+                                                   **trajectory_errors(e, g, DT))
     return out
 
 
 def c1_leg(c, with_accuracy):
     """The reference's CPU-runnable case: N = 256, y = 0.1 cos x, phi = 0, water with g = 1 in deep
-    water, 100 RK4 steps of dt = 0.01 from a fresh state (for GMRES this includes the six full-length steps
-    that fill the warm-start history and set the trimmed sequence, and the graph captures), with the drop-in LU solver and with GMRES;
-    the error of the final surface against the FP64 oracle run over the same 100 steps."""
-    n, dt, steps = 256, 0.01, 100
-    x = 2 * np.pi * np.arange(n) / n
-    Z0 = x + 1j * 0.1 * np.cos(x)
-    P0 = np.zeros(n)
+    water, 100 RK4 steps of dt = 0.01 from a fresh state (for GMRES this includes the steps that
+    fill the warm-start history and set the trimmed sequence, and the graph captures), with the
+    drop-in LU solver and with GMRES; the error of the final surface against the oracle's."""
+    Z0, P0, dt, steps = W.c1()
+    n = len(Z0)
     out = {"workload": "N=256 water surface y=0.1 cos x, phi=0, g=1, deep water, dt=0.01, 100 RK4 steps, FP64",
            "nodes": n, "steps": steps}
-    finals = {}
+    g = golden("c1_water_n256.npz") if with_accuracy else None
     for solver, name in ((c.LU, "lu"), (c.GMRES, "gmres")):
         with c.Ensemble([n], c.Props(), solver=solver, tol=1e-12) as e:
             e.set_state(Z0, P0)
@@ -169,15 +212,43 @@ def c1_leg(c, with_accuracy):
             e.step(dt, steps)
             e.sync()
             el = time.perf_counter() - t0
-            finals[name] = e.get_state()
-        out[name] = {"steps_per_s": round(steps / el, 2), "ms_per_step": round(1e3 * el / steps, 4)}
-    if with_accuracy:
-        from oracle import oracle as O
-        O.build()
-        Zr, Pr = O.rk4(O.Props(), Z0, P0, dt, steps)
-        for name, (Zg, Pg) in finals.items():
-            out[name]["max_abs_dz_vs_oracle"] = float(np.abs(Zg - Zr).max())
-            out[name]["max_abs_dphi_vs_oracle"] = float(np.abs(Pg - Pr).max())
+            out[name] = {"steps_per_s": round(steps / el, 2), "ms_per_step": round(1e3 * el / steps, 4)}
+            if g is not None:
+                Z, P = e.get_state()
+                _, _, d = e.rhs(diagnostics=True)
+                out[name].update(errors(Z, P, d["mu"], g["Z"][0], g["phi"][0], g["mu"][0]))
+    return out
+
+
+def c2_lu_leg(c, steps):
+    """The headline surface with the drop-in solver (blocked FP64 LU on MFMA): steps/s, the MFMA
+    roofline of its trailing-update kernel and the time of each LU kernel per step, and the
+    error after 100 steps against the oracle's golden state."""
+    Z0, P0 = W.c2()
+    out = {"workload": "N=2048 water surface (the headline input), FP64, blocked LU (getrf/getrs semantics)"}
+    with c.Ensemble([N], c.Props(), solver=c.LU) as e:
+        e.set_state(Z0, P0)
+        el = timed_steps(e, steps, 2)
+        e.kernel_timing(True)
+        e.step(DT, 1)
+        e.sync()
+        reps = {t: e.kernel_report(t) for t in ("deriv", "assemble", "lu_panel", "lu_update", "lu_backsolve",
+                                                "mu_deriv", "velocity")}
+        e.kernel_timing(False)
+        st = e.stats()
+    r = reps["lu_update"]
+    upd_ms = r["ms"] / max(r["launches"], 1)
+    lu_flops = 4 * (2 / 3 * N ** 3 + 2 * N ** 2)
+    lu_ms = sum(reps[t]["ms"] for t in ("lu_panel", "lu_update", "lu_backsolve"))
+    out.update({"steps_per_s": round(steps / el, 3), "ms_per_step": round(1e3 * el / steps, 4), "steps": steps,
+                "kernel_ms_per_step": {t: round(v["ms"], 4) for t, v in reps.items() if v["launches"]},
+                "lu_update_roofline": {"bound": "mfma", "achieved": round(r["flops"] / (r["ms"] * 1e-3) / 1e12, 3),
+                                       "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                                       "frac": round(r["flops"] / (r["ms"] * 1e-3) / 1e12 / PEAK_FP64_TFLOPS, 4),
+                                       "avg_launch_ms": round(upd_ms, 5), "launches": r["launches"]},
+                "lu_solve_tflops": round(lu_flops / (lu_ms * 1e-3) / 1e12, 3),
+                "lu_solve_frac_of_fp64_mfma_peak": round(lu_flops / (lu_ms * 1e-3) / 1e12 / PEAK_FP64_TFLOPS, 4),
+                "gepp_fallbacks": st["fallback_solves"]})
     return out
 
 
@@ -185,39 +256,36 @@ def film_leg(c, steps, with_accuracy):
     """Helium film over a flat substrate: N = 4096, depth h0 = 1 with the image term, restoring
     acceleration 3 A / (h0 + y)^4 with A = 0.05, i.e. g_eff = 3 A / h0^4 = 0.15 scaling the reference's
     van der Waals term; 8-mode seeded surface with a_k ~ U(0, 0.05 / k); phi = 0; dt = 0.002."""
-    n = 4096
-    props = dict(kind=1, depth=1.0, g=0.15)
-    Z0 = surface(n, SEED + 5, amp=0.05)
-    P0 = np.zeros(n)
+    props = W.C5_PROPS
+    Z0, P0 = W.c5()
+    n = len(Z0)
     out = {"workload": "helium film N=4096, h0=1 (image term), g_eff=3A/h0^4=0.15 (A=0.05), 8-mode seeded "
                        "surface a_k~U(0,0.05/k), phi=0, dt=0.002; FP64 GMRES, mixed GMRES (FP32 matrix, "
                        "FP64 r0 and refinement residual), FP64 LU", "nodes": n}
-    for solver, prec, name, mi, k in ((c.GMRES, c.FP64, "gmres", 24, steps),
-                                      (c.GMRES, c.MIXED, "gmres_mixed", 24, steps),
-                                      (c.LU, c.FP64, "lu", 8, max(2, steps // 25))):
+    for solver, prec, name, mi, k, cal in ((c.GMRES, c.FP64, "gmres", 24, steps, CALIB_STEPS),
+                                           (c.GMRES, c.MIXED, "gmres_mixed", 24, steps, CALIB_STEPS),
+                                           (c.LU, c.FP64, "lu", 8, max(2, steps // 25), 0)):
         with c.Ensemble([n], c.Props(**props), precision=prec, solver=solver, tol=1e-12, max_iter=mi) as e:
             e.set_state(Z0, P0)
-            el = timed_steps(e, k, 1)
+            el = timed_steps(e, k, 1, cal)
             st = e.stats()
         out[name] = {"steps_per_s": round(k / el, 3), "ms_per_step": round(1e3 * el / k, 3), "steps": k,
                      "gmres_iterations_last": st["max_iterations"], "unconverged": st["unconverged"],
+                     "redone_steps": st["redone_steps"], "launches_per_solve": st["sequence_length"],
                      "gepp_fallbacks": st["fallback_solves"]}
     if with_accuracy:
-        from oracle import oracle as O
-        O.build()
-        acc_steps = 2
-        finals = {}
+        g = golden("c5_film_n4096.npz")
         for prec, name in ((c.FP64, "accuracy_gmres"), (c.MIXED, "accuracy_gmres_mixed")):
             with c.Ensemble([n], c.Props(**props), precision=prec, solver=c.GMRES, tol=1e-12, max_iter=24) as e:
                 e.set_state(Z0, P0)
-                e.step(DT, acc_steps)
-                finals[name] = e.get_state()
-        t0 = time.perf_counter()
-        Zr, Pr = O.rk4(O.Props(**props), Z0, P0, DT, acc_steps)
-        t_or = round((time.perf_counter() - t0) / acc_steps, 2)
-        for name, (Zg, Pg) in finals.items():
-            out[name] = {"steps": acc_steps, "max_abs_dz_vs_oracle": float(np.abs(Zg - Zr).max()),
-                         "max_abs_dphi_vs_oracle": float(np.abs(Pg - Pr).max()), "oracle_s_per_step": t_or}
+                out[name] = trajectory_errors(e, g, DT)
+        with c.Ensemble([n], c.Props(**props), solver=c.LU) as e:
+            e.set_state(Z0, P0)
+            e.step(DT, int(g["steps"][0]))
+            Z, P = e.get_state()
+            _, _, d = e.rhs(diagnostics=True)
+            out["accuracy_lu"] = dict(steps=int(g["steps"][0]),
+                                      **errors(Z, P, d["mu"], g["Z"][0], g["phi"][0], g["mu"][0]))
     return out
 
 
@@ -229,11 +297,9 @@ def adjoint_leg(c):
     of the GPU stepper, all 32 perturbed surfaces advanced as one batched ensemble."""
     import torch
     from cusuperhelium.adjoint import Trajectory, evolve
-    n, modes, dt, steps, every, h = 1024, 16, 0.005, 60, 20, 1e-6
-    x = 2 * np.pi * np.arange(n) / n
-    amp = np.random.default_rng(SEED).uniform(0, 0.01, modes)
-    tgt = np.random.default_rng(SEED + 1).uniform(0, 0.01, modes)
-    basis = np.array([np.cos((k + 1) * x) for k in range(modes)])
+    dt, steps, every, h = 0.005, 60, 20, 1e-6
+    x, amp, tgt, basis = W.c4()
+    n, modes = len(x), len(amp)
     target = torch.tensor(tgt @ basis)
     B = torch.tensor(basis)
 
@@ -275,10 +341,13 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and accuracy legs")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-accuracy", action="store_true", help="skip the comparisons with the golden states")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched ensemble leg")
     ap.add_argument("--batched-steps", type=int, default=50)
     ap.add_argument("--no-c1", action="store_true", help="skip the N=256 100-step leg")
+    ap.add_argument("--no-lu", action="store_true", help="skip the N=2048 LU leg")
+    ap.add_argument("--lu-steps", type=int, default=20)
     ap.add_argument("--no-film", action="store_true", help="skip the N=4096 helium film leg")
     ap.add_argument("--film-steps", type=int, default=50)
     ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiable-stepper leg")
@@ -303,12 +372,13 @@ def main():
 
     import cusuperhelium as c
 
-    Z0 = surface(N, SEED)
-    P0 = np.zeros(N)
+    acc = not args.no_accuracy
+    Z0, P0 = W.c2()
     solver = c.GMRES if args.solver == "gmres" else c.LU
+    calib = CALIB_STEPS if args.solver == "gmres" else 0
     e = c.Ensemble([N], c.Props(), precision=c.FP64, solver=solver, tol=1e-12, max_iter=args.max_iter)
     e.set_state(Z0, P0)
-    el = timed_steps(e, args.steps, args.warmup)
+    el = timed_steps(e, args.steps, args.warmup, calib)
     value = args.steps / el
     ms_per_step = 1e3 * el / args.steps
     st_main = e.stats()
@@ -345,6 +415,7 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
+        "warmup_steps_run": max(args.warmup, calib),
         "ms_per_step": round(ms_per_step, 4),
         "higher_is_better": True,
         "scaling": "weak",
@@ -360,29 +431,39 @@ def main():
                      "launches_timed": r["launches"]},
         "stage_ms_per_step": stages,
         "solver_stats": {"gmres_iterations_last": st["last_iterations"], "gmres_unconverged": st_main["unconverged"],
+                         "redone_steps": st_main["redone_steps"], "launches_per_solve": st_main["sequence_length"],
                          "gepp_fallbacks": st["fallback_solves"]},
     }
+    if acc:
+        # the headline path over the workload's full 1000 steps against the oracle's golden states
+        g = golden("c2_water_n2048.npz")
+        with c.Ensemble([N], c.Props(), precision=c.FP64, solver=solver, tol=1e-12, max_iter=args.max_iter) as e:
+            e.set_state(Z0, P0)
+            out["accuracy"] = trajectory_errors(e, g, DT)
+    if not args.no_lu and args.solver == "gmres":
+        out["c2_lu"] = c2_lu_leg(c, args.lu_steps)
+        if acc:
+            g = golden("c2_water_n2048.npz")
+            with c.Ensemble([N], c.Props(), solver=c.LU) as e:
+                e.set_state(Z0, P0)
+                e.step(DT, int(g["steps"][0]))
+                Z, P = e.get_state()
+            out["c2_lu"]["accuracy"] = {"steps": int(g["steps"][0]), "max_abs_dz": float(np.abs(Z - g["Z"][0]).max()),
+                                        "max_abs_dphi": float(np.abs(P - g["phi"][0]).max())}
     if not args.no_batched:
-        out["batched"] = batched_leg(c, args.batched_steps, not args.no_cpu)
+        out["batched"] = batched_leg(c, args.batched_steps, acc)
     if not args.no_c1:
-        out["c1"] = c1_leg(c, not args.no_cpu)
+        out["c1"] = c1_leg(c, acc)
     if not args.no_film:
-        out["film"] = film_leg(c, args.film_steps, not args.no_cpu)
+        out["film"] = film_leg(c, args.film_steps, acc)
     if not args.no_adjoint:
         out["adjoint"] = adjoint_leg(c)
     if not args.no_cpu:
         steps, cel, Zc, Pc, threads = cpu_baseline(Z0, P0)
         out["cpu_baseline"] = {"value": round(steps / cel, 4), "unit": "RK4 steps/s", "cores": threads,
-                               "kind": "port",
+                               "kind": "port", "cpu_model": cpu_model(), "host_threads": os.cpu_count(),
                                "sample": f"{steps} RK4 steps of the same N=2048 workload on the CPU FP64 oracle "
-                                         f"(OpenMP, {threads} threads), {cel:.1f} s"}
-        g = c.Ensemble([N], c.Props(), precision=c.FP64, solver=solver, tol=1e-12, max_iter=args.max_iter)
-        g.set_state(Z0, P0)
-        g.step(DT, steps)
-        Zg, Pg = g.get_state()
-        g.close()
-        out["accuracy"] = {"steps_compared": steps, "max_abs_dz_vs_oracle": float(np.abs(Zg - Zc).max()),
-                           "max_abs_dphi_vs_oracle": float(np.abs(Pg - Pc).max())}
+                                         f"(OpenMP, {threads} threads, {cpu_model()}), {cel:.1f} s"}
     print(json.dumps(out), flush=True)
 
 

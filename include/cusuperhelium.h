@@ -133,9 +133,14 @@ typedef struct sfh_options {
     int precision;    /* SFH_FP64 (the drop-in arithmetic) or SFH_MIXED (SFH_SOLVE_BATCHED or _GMRES) */
     int solver;       /* SFH_SOLVE_LU, SFH_SOLVE_GMRES or SFH_SOLVE_BATCHED */
     int use_graph;    /* replay RK4 steps from a captured hipGraph */
-    int max_iter;     /* iterative solver / refinement iteration cap */
-    int restart;      /* GMRES restart length */
-    int reserved;
+    int max_iter;     /* SFH_SOLVE_GMRES: Arnoldi steps per solve, 1..32 (sfh_create rejects more: the
+                         launch sequence has no restart); SFH_SOLVE_BATCHED: GMRES iteration cap,
+                         rounded up to restart cycles of 8 */
+    int restart;      /* unused (0): the GMRES of SFH_SOLVE_GMRES does not restart, the in-workgroup
+                         GMRES of SFH_SOLVE_BATCHED restarts every 8 basis vectors */
+    int force_trim;   /* test hook (0 = off): length of the trimmed GMRES launch sequence instead of
+                         the calibrated one (at least 2); a chunk of steps in which a solve runs out
+                         of launches is still run again with the length doubled */
     double tol;       /* iterative solver / refinement relative tolerance */
 } sfh_options;
 
@@ -146,12 +151,16 @@ typedef struct sfh_stats {
     int spin_timeouts;
     int last_iterations;   /* GMRES: iterations of surface 0 in the last solve */
     int max_iterations;    /* GMRES: most iterations any surface needed in the last solve */
-    int unconverged;       /* GMRES: solves that reached the iteration cap (accumulated) */
+    int unconverged;       /* GMRES: solves of kept steps that reached the iteration cap (accumulated) */
     int refinements;       /* mixed precision, surface 0, last solve: FP64 refinement sweeps (batched);
                               1 if the refinement solve ran Arnoldi steps, else 0 (GMRES) */
+    int redone_steps;      /* SFH_SOLVE_GMRES: steps run again because a solve of their chunk ran out of
+                              launches of the trimmed sequence (accumulated) */
+    int sequence_length;   /* SFH_SOLVE_GMRES: launches per solve of the trimmed step graph, 0 while the
+                              warm-start history fills (then every solve gets max_iter + 1) */
 } sfh_stats;
 
-/* Defaults: FP64, LU, graph replay on, tol 1e-12. */
+/* Defaults: FP64, LU, graph replay on, max_iter 32, tol 1e-12. */
 void sfh_default_options(sfh_options* opt);
 void sfh_default_props(sfh_props* props);
 
@@ -180,6 +189,16 @@ int sfh_rhs_vjp(sfh_handle h, const double* gvx, const double* gvy, const double
                 double* gphi);
 int sfh_step_vjp(sfh_handle h, double dt, const double* gx_in, const double* gy_in, const double* gphi_in,
                  double* gx_out, double* gy_out, double* gphi_out);
+/* Energies (EnergyContainer and VolumeFlux, Energies.cuh:136-204, computed by the reference on
+ * every right-hand side, BaseBoundaryIntegrator.cuh:270-272).  While tracking is enabled every
+ * right-hand side also writes its per-node energy terms (32 B per node; the step graph is
+ * re-captured when tracking changes).  sfh_get_energies fills out[4 * s + c] for surface s with
+ * c = 0 kinetic, 1 potential (gravitational for water, van der Waals for helium, times g),
+ * 2 surface (kappa), 3 volume flux, of the last right-hand side evaluated: after sfh_step that is
+ * the fourth stage of the last step, as the reference's getEnergy() after runStep; after sfh_rhs,
+ * the current state.  Returns -3 when tracking is off. */
+int sfh_track_energies(sfh_handle h, int enable);
+int sfh_get_energies(sfh_handle h, double* out);
 /* Dipole strengths of the last right-hand-side evaluation. */
 int sfh_get_mu(sfh_handle h, double* mu);
 int sfh_sync(sfh_handle h);

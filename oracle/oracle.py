@@ -102,6 +102,7 @@ def lib():
         L.sfo_rk4.restype = ctypes.c_int
         L.sfo_rk4_rotation.argtypes = [ctypes.c_size_t, ctypes.c_double, ctypes.c_int, dp]
         L.sfo_adimensionalize.argtypes = [ctypes.c_double] * 4 + [dp]
+        L.sfo_energies.argtypes = [ctypes.POINTER(_Props), ctypes.c_int, dp, dp, dp, dp, dp]
         L.sfo_num_threads.restype = ctypes.c_int
         L.sfo_set_num_threads.argtypes = [ctypes.c_int]
         _lib = L
@@ -257,6 +258,17 @@ def rk4_rotation(z0, dt: float, steps: int) -> np.ndarray:
     y = _c128(z0).copy()
     lib().sfo_rk4_rotation(y.size, float(dt), int(steps), _p(y.view(np.float64)))
     return y
+
+
+def energies(props: Props, Z, Phi, Zp, q) -> np.ndarray:
+    """[kinetic, potential, surface, volume flux] of one surface (Energies.cuh:56-204)."""
+    Z, Zp, q = _c128(Z), _c128(Zp), _c128(q)
+    Phi = _f64(Phi)
+    out = np.zeros(4)
+    cp = props.c()
+    lib().sfo_energies(ctypes.byref(cp), Z.size, _p(Z.view(np.float64)), _p(Phi), _p(Zp.view(np.float64)),
+                       _p(q.view(np.float64)), _p(out))
+    return out
 
 
 def adimensionalize(L: float, rho: float, kappa: float, depth: float) -> dict:

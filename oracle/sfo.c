@@ -19,6 +19,8 @@
  *   sfo_rhs               BaseBoundaryIntegrator.cuh:137-306 (energies omitted: they do not
  *                         feed the right-hand side)
  *   sfo_rk4               AutonomousRungeKuttaStepper.cuh:31-36,123-361,417-437
+ *   sfo_energies          Energies.cuh:56-204 (kinetic, gravitational / van der Waals, surface,
+ *                         volume flux), called per RHS at BaseBoundaryIntegrator.cuh:270-272
  *   sfo_adimensionalize   Export.cu:1222-1246
  *
  * Layouts follow the reference: a batch of B surfaces of N nodes is the complex128 state
@@ -568,6 +570,38 @@ static int rotation_rhs(void *ctx, const double *y, double *k)
 int sfo_rk4_rotation(size_t m, double dt, int steps, double *y)
 {
     return sfo_rk4_generic(rotation_rhs, &m, m, dt, steps, y, NULL);
+}
+
+/* ------------------------------------------------------------- energies */
+/* EnergyContainer::CalculateEnergy + VolumeFlux (Energies.cuh:56-204) for one surface of n nodes:
+ * out[0] kinetic   (1/4pi) sum_k (phi_k + U(1+rho)x_k/2)(-Y'_k u_k + X'_k v_k)
+ *                          - (U/2)((u_k + rho u_k) X'_k + (v_k + rho v_k) Y'_k + U(1-rho)X'_k/2) y_k
+ *                  (KineticEnergy passes the lower velocity as the upper one too, Energies.cuh:256)
+ * out[1] potential water: (1+rho)/4pi sum_k y_k^2 X'_k (GravitationalEnergy);
+ *                  helium: h^2/6 sum_k ((1 + y_k/h)^-2 - 1) (VanDerWaalsEnergy); both times g
+ *                  (g = 1 is the reference, which has no g)
+ * out[2] surface   kappa/2pi (sum_k |Z'_k| - 2pi)
+ * out[3] flux      1/2pi sum_k (v_k X'_k + u_k Y'_k)
+ * with Z = x + i y, Z' = X' + i Y', q = u + i v (the lower velocity after conjugation). */
+void sfo_energies(const sfo_props *p, int n, const double *Z, const double *Phi, const double *Zp,
+                  const double *q, double *out)
+{
+    const double U = p->U, rho = p->rho, h = p->depth;
+    double kin = 0, pot = 0, surf = 0, flux = 0;
+    for (int k = 0; k < n; ++k) {
+        const double x = Z[2 * k], y = Z[2 * k + 1], X = Zp[2 * k], Y = Zp[2 * k + 1];
+        const double u = q[2 * k], v = q[2 * k + 1];
+        kin += (Phi[k] + 0.5 * U * (1.0 + rho) * x) * (-Y * u + X * v) -
+               0.5 * U * ((u + rho * u) * X + (v + rho * v) * Y + 0.5 * U * (1.0 - rho) * X) * y;
+        if (p->kind == 0) pot += y * y * X;
+        else pot += 1.0 / pow(1.0 + y / h, 2) - 1.0;
+        surf += sqrt(X * X + Y * Y);
+        flux += v * X + u * Y;
+    }
+    out[0] = kin * 0.25 / PI_D;
+    out[1] = p->g * (p->kind == 0 ? pot * 0.25 * (1.0 + rho) / PI_D : pot * h * h / 6.0);
+    out[2] = (surf - 2.0 * PI_D) * p->kappa / (2.0 * PI_D);
+    out[3] = flux * 0.5 / PI_D;
 }
 
 /* -------------------------------------------------- SI to base units */

@@ -1,6 +1,8 @@
 // export.cpp -- the C ABI (include/cusuperhelium.h): drop-in CuSuperHelium entry points and the
 // native sfh_* handle API, both on top of sfh::Solver.
+#include <algorithm>
 #include <cmath>
+#include <cstdint>
 #include <cstdlib>
 #include <cstring>
 #include <map>
@@ -232,15 +234,22 @@ int integrateSimulationRK4(double* initialState, double** statesOut, size_t* sta
         const double t0 = rkOptions->t0 / u.base_time;
         const double t1 = rkOptions->t1 / u.base_time;
         const double dt = rkOptions->timeStep / u.base_time;
+        if (!std::isfinite(t0) || !std::isfinite(t1) || !std::isfinite(dt))
+            throw sfh::ArgError("t0, t1 and the time step must be finite");
         if (!(dt > 0.0)) throw sfh::ArgError("time step must be positive");
-        const size_t steps = (size_t)((t1 - t0) / dt);   // runEvolution, AutonomousRungeKuttaStepper.cuh:421
+        if (t1 < t0) throw sfh::ArgError("t1 must not precede t0");
+        // runEvolution (AutonomousRungeKuttaStepper.cuh:421): floor((t1 - t0) / dt) steps
+        const double nsteps = std::floor((t1 - t0) / dt);
+        if (!(nsteps < 4.0e15)) throw sfh::ArgError("too many time steps");
+        const size_t steps = (size_t)nsteps;
+        const bool traj = rkOptions->returnTrajectory;
+        const size_t nst = traj ? steps : 1;
+        if (nst > SIZE_MAX / (sizeof(double) * 3 * N) - 1) throw sfh::ArgError("trajectory too large to allocate");
         sfh::Options opt;
         Solver s(helium_phys(u, sp.infinite_depth, sp.use_expansions, sp.expansion_order),
                  std::vector<int>(1, (int)N), opt);
         const double* x = initialState;
         s.set_state(x, x + N, x + 2 * N);
-        const bool traj = rkOptions->returnTrajectory;
-        const size_t nst = traj ? steps : 1;
         double* states = (double*)std::malloc(sizeof(double) * 3 * N * (nst ? nst : 1));
         double* times = traj ? (double*)std::malloc(sizeof(double) * (steps ? steps : 1)) : nullptr;
         if (!states || (traj && !times)) {
@@ -258,7 +267,11 @@ int integrateSimulationRK4(double* initialState, double** statesOut, size_t* sta
                     times[k] = t;
                 }
             } else {
-                s.step(dt, (int)steps);
+                for (size_t done = 0; done < steps;) {   // Solver::step takes an int count
+                    const int chunk = (int)std::min<size_t>(steps - done, 1u << 30);
+                    s.step(dt, chunk);
+                    done += (size_t)chunk;
+                }
                 s.get_state(states, states + N, states + 2 * N);
             }
         } catch (...) {
@@ -285,8 +298,9 @@ void sfh_default_options(sfh_options* opt) {
     opt->precision = SFH_FP64;
     opt->solver = SFH_SOLVE_LU;
     opt->use_graph = 1;
-    opt->max_iter = 50;
-    opt->restart = 30;
+    opt->max_iter = 32;
+    opt->restart = 0;
+    opt->force_trim = 0;
     opt->tol = 1e-12;
 }
 
@@ -314,6 +328,7 @@ int sfh_create(const sfh_props* props, const int* n, int n_surfaces, const sfh_o
             o.use_graph = opt->use_graph;
             o.max_iter = opt->max_iter;
             o.restart = opt->restart;
+            o.force_trim = opt->force_trim;
             o.tol = opt->tol;
         }
         auto h = std::make_unique<sfh_solver>();
@@ -396,7 +411,20 @@ int sfh_get_stats(sfh_handle h, sfh_stats* out) {
         out->max_iterations = st.max_iterations;
         out->unconverged = st.unconverged;
         out->refinements = st.refinements;
+        out->redone_steps = st.redone_steps;
+        out->sequence_length = st.sequence_length;
     });
+}
+
+int sfh_track_energies(sfh_handle h, int enable) {
+    SFH_NEED(h);
+    return guarded([&] { h->s->track_energies(enable != 0); });
+}
+
+int sfh_get_energies(sfh_handle h, double* out) {
+    SFH_NEED(h);
+    if (!out) return -3;
+    return guarded([&] { h->s->energies(out); });
 }
 
 int sfh_device_state(sfh_handle h, void** Z, void** phi) {

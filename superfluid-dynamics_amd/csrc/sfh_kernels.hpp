@@ -21,6 +21,8 @@ struct StageCtl {
     double2* kZ[3];
     double* kP[3];
     double2* qup;      // optional upper-fluid velocity output (may be null)
+    const double* Pin; // phi of the stage's input state (read only when energy is set)
+    double* energy;    // optional: 4 per node, the energy and flux terms of node g at 4 g
 };
 
 // Warm start of a solve by extrapolation over time steps: each RK stage keeps its solutions of the

@@ -97,6 +97,28 @@ __device__ inline void stage_store(const StageCtl& ctl, int g, double2 q1, doubl
         const_cast<double*>(ctl.P0)[g] = s.p0 + ctl.alpha * Kp;
     }
 }
+// Per-node terms of EnergyContainer::CalculateEnergy and VolumeFlux (Energies.cuh:136-204),
+// summed per surface and scaled on the host: kinetic (phi + U(1+rho)x/2)(-Y' u + X' v)
+// - (U/2)((u + rho u) X' + (v + rho v) Y' + U(1-rho) X'/2) y (KineticEnergy passes the lower
+// velocity for the upper one), potential y^2 X' (water) or (1 + y/h)^-2 - 1 (helium), |Z'|, and
+// v X' + u Y'.  q = u + i v is the lower velocity, Z' = X' + i Y'.
+__device__ inline void energy_store(const Phys& ph, const StageCtl& ctl, int g, double x, double y, double phi,
+                                    double2 zp, double2 q) {
+    const double U = ph.U, rho = ph.rho;
+    const double kin = (phi + 0.5 * U * (1.0 + rho) * x) * (-zp.y * q.x + zp.x * q.y) -
+                       0.5 * U * ((q.x + rho * q.x) * zp.x + (q.y + rho * q.y) * zp.y + 0.5 * U * (1.0 - rho) * zp.x) * y;
+    double pot;
+    if (ph.kind == 0) {
+        pot = y * y * zp.x;
+    } else {
+        const double r = 1.0 + y / ph.depth;
+        pot = 1.0 / (r * r) - 1.0;
+    }
+    double2* e = reinterpret_cast<double2*>(ctl.energy) + 2 * (long long)g;
+    e[0] = make_double2(kin, pot);
+    e[1] = make_double2(sqrt(zp.x * zp.x + zp.y * zp.y), q.y * zp.x + q.x * zp.y);
+}
+
 __device__ inline void stage_update(const StageCtl& ctl, int g, double2 q1, double dphi) {
     stage_store(ctl, g, q1, dphi, stage_load(ctl, g));
 }

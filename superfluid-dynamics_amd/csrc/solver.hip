@@ -27,6 +27,12 @@
 
 namespace sfh {
 
+// Steps that run every solve with the full launch cap: kHistDepth while the warm-start history
+// fills, then one with the full history that sets the trimmed length.
+constexpr int kCalibSteps = kHistDepth + 1;
+// Longest chunk of trimmed steps between two checks.
+constexpr int kCheckSteps = 32;
+
 bool supported_n(int n) {
     if (n < 1 || n > 8192) return false;
     int m = n;
@@ -106,9 +112,14 @@ Solver::Solver(const Phys& ph, const std::vector<int>& n, const Options& opt) : 
     if (opt_.precision != kFP64 && opt_.precision != kMixed) throw ArgError("unknown precision");
     if (opt_.precision == kMixed && opt_.solver == kSolveLU)
         throw ArgError("the mixed-precision form runs with the GMRES or the batched solver");
-    if (opt_.solver == kSolveGMRES)
+    if (opt_.solver == kSolveGMRES) {
         for (int v : n)
             if (v > kGmMaxN) throw ArgError("the GMRES solver takes surfaces of at most 4096 nodes");
+        if (opt_.max_iter < 1 || opt_.max_iter > kGmMaxIter)
+            throw ArgError("the GMRES solver runs 1 to " + std::to_string(kGmMaxIter) +
+                           " Arnoldi steps per solve (max_iter), without restart");
+    }
+    if (opt_.force_trim < 0) throw ArgError("negative force_trim");
     if (opt_.solver == kSolveBatched)
         for (int v : n)
             if (v > kBatchMaxN) throw ArgError("the batched solver takes surfaces of at most 1024 nodes");
@@ -207,7 +218,7 @@ Solver::Solver(const Phys& ph, const std::vector<int>& n, const Options& opt) : 
     SFH_HIP(hipMemset(mu_, 0, T_ * sizeof(double)));
     SFH_HIP(hipMemset(mup_, 0, T_ * sizeof(double2)));
     if (opt_.solver == kSolveGMRES) {
-        gm_m_ = std::max(1, std::min(opt_.max_iter, 32));   // kMaxKrylov in gmres.hip
+        gm_m_ = opt_.max_iter;   // at most kMaxKrylov basis vectors (gmres.hip)
         gm_nwg_ = (maxN_ + 3) / 4;   // kRowsPerWG in gmres.hip
         gm_part_stride_ = 2LL * gm_nwg_ * (gm_m_ + 1);
         gm_small_stride_ = 4LL * (gm_m_ + 1) + 2LL * gm_m_ + (long long)gm_m_ * (gm_m_ + 1);
@@ -224,6 +235,11 @@ Solver::Solver(const Phys& ph, const std::vector<int>& n, const Options& opt) : 
         gmhist_ = dalloc<double>(4 * kHistDepth * (size_t)T_);
         gmhvalid_ = dalloc<int>(4);
         SFH_HIP(hipMemset(gmhvalid_, 0, 4 * sizeof(int)));
+        ckZ_ = dalloc<double2>(T_);
+        ckP_ = dalloc<double>(T_);
+        ckMu_ = dalloc<double>(T_);
+        ckHist_ = dalloc<double>(4 * kHistDepth * (size_t)T_);
+        ckHv_ = dalloc<int>(4);
     }
     if (opt_.solver == kSolveBatched) {
         std::vector<int> ord(S_);
@@ -241,13 +257,16 @@ Solver::Solver(const Phys& ph, const std::vector<int>& n, const Options& opt) : 
 }
 
 Solver::~Solver() {
+    // replays of the step graph may still be in flight: the executable graph is destroyed only
+    // once the stream has drained
+    if (stream_) (void)hipStreamSynchronize(stream_);
     if (exec_) (void)hipGraphExecDestroy(exec_);
     if (graph_) (void)hipGraphDestroy(graph_);
-    if (stream_) (void)hipStreamSynchronize(stream_);
     void* bufs[] = {d_desc_, d_tw_, Z0_, Za_, Zb_, P0_, Pa_, Pb_, kZ_[0], kZ_[1], kZ_[2], kP_[0],
                     kP_[1], kP_[2], Zp_, Zpp_, mup_, qup_, b_, work_, y_, mu_, M_, uinv_, flags_,
                     status_, piv_, counters_, gmV_, gmw_, gmpart_, gmsmall_, gmctl_, trig_, order_, bstats_,
-                    gmhist_, gmhvalid_, bhist_, bhvalid_, d_circ_, gmx0_, M32_};
+                    gmhist_, gmhvalid_, bhist_, bhvalid_, d_circ_, gmx0_, M32_, ckZ_, ckP_, ckMu_, ckHist_,
+                    ckHv_, energy_};
     for (void* p : bufs)
         if (p) (void)hipFree(p);
     void* vbufs[] = {vY_[0], vY_[1], vY_[2], vK_[0], vK_[1], vK_[2], vYP_[0], vYP_[1], vYP_[2], vKP_[0],
@@ -499,7 +518,10 @@ void Solver::enqueue_batched(const double2* Z, const double* P, const StageCtl& 
     SFH_HIP(hipGetLastError());
 }
 
-void Solver::enqueue_rhs(const double2* Z, const double* P, const StageCtl& ctl) {
+void Solver::enqueue_rhs(const double2* Z, const double* P, const StageCtl& ctl_in) {
+    StageCtl ctl = ctl_in;
+    ctl.Pin = P;
+    ctl.energy = track_energy_ ? energy_ : nullptr;
     if (opt_.solver == kSolveBatched) {
         enqueue_batched(Z, P, ctl);
         return;
@@ -572,6 +594,9 @@ void Solver::enqueue_step(double dt) {
 }
 
 void Solver::capture(double dt, int len) {
+    // steps replayed from the old graph may still be queued (sfh_step returns without waiting,
+    // and a call with another dt re-captures): drain the stream before destroying it
+    SFH_HIP(hipStreamSynchronize(stream_));
     if (exec_) {
         SFH_HIP(hipGraphExecDestroy(exec_));
         exec_ = nullptr;
@@ -596,39 +621,46 @@ void Solver::capture(double dt, int len) {
     graph_len_ = len;
 }
 
-// Steps that run every solve with the full launch cap: kHistDepth while the warm-start history
-// fills, then one with the full history that sets the trimmed length.
-constexpr int kCalibSteps = kHistDepth + 1;
-// Steps between checks of a trimmed sequence within one step() call.
-constexpr int kCheckSteps = 32;
-
-int Solver::step_len() {
-    if (opt_.solver != kSolveGMRES) return 0;
-    if (hist_steps_ < kCalibSteps || trim_ == 0) return gm_m_;
-    return trim_;
+void Solver::calibrate() {
+    if (++hist_steps_ < kCalibSteps) return;
+    // the first step with the full history: its solves (all four stages) bound the Arnoldi steps
+    // of the following ones.  L launches run up to L - 1 steps.  Solves that need 1 or 2 steps
+    // start from guesses whose error is at the level of the solve tolerance (the extrapolation's
+    // truncation error is smaller), and get no spare launch; slower ones, whose step count varies
+    // more from solve to solve, get one spare step.  A chunk in which a solve still runs out of
+    // launches is run again with the length doubled.
+    sync();
+    const int need = fill_iters_ + 1;
+    trim_ = std::max(2, std::min(gm_m_, fill_iters_ >= 3 ? need + 1 : need));
+    if (opt_.force_trim > 0) trim_ = std::max(2, std::min(gm_m_, opt_.force_trim));
+    chunk_ = 2;
 }
 
-void Solver::after_step(int len) {
-    if (opt_.solver != kSolveGMRES) return;
-    if (hist_steps_ < kCalibSteps) {
-        if (++hist_steps_ < kCalibSteps) return;
-        // the first step with the full history: its solves (all four stages) bound the Arnoldi
-        // steps of the following ones.  L launches run up to L - 1 steps.  Solves that need 1 or
-        // 2 steps start from guesses whose error is at the level of the solve tolerance (the
-        // extrapolation's truncation error is smaller), and get no spare launch; slower ones,
-        // whose step count varies more from solve to solve, get one spare step.  (A third of
-        // their steps as the spare cost the film 2% and still let helium solves whose count
-        // drifts up by 2 run out of launches; the periodic checks in step() catch those.)
-        sync();
-        const int need = fill_iters_ + 1;
-        trim_ = std::max(2, std::min(gm_m_, fill_iters_ >= 3 ? need + 1 : need));
-        unc_seen_ = stats_.unconverged;
-        return;
-    }
-    if (len < gm_m_) {   // a trimmed step: did any solve run out of launches?
-        sync();
-        if (stats_.unconverged > unc_seen_) trim_ = std::min(gm_m_, 2 * trim_);
-        unc_seen_ = stats_.unconverged;
+void Solver::save_checkpoint() {
+    SFH_HIP(hipMemcpyAsync(ckZ_, Z0_, T_ * sizeof(double2), hipMemcpyDeviceToDevice, stream_));
+    SFH_HIP(hipMemcpyAsync(ckP_, P0_, T_ * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+    SFH_HIP(hipMemcpyAsync(ckMu_, mu_, T_ * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+    SFH_HIP(hipMemcpyAsync(ckHist_, gmhist_, 4 * kHistDepth * (size_t)T_ * sizeof(double),
+                           hipMemcpyDeviceToDevice, stream_));
+    SFH_HIP(hipMemcpyAsync(ckHv_, gmhvalid_, 4 * sizeof(int), hipMemcpyDeviceToDevice, stream_));
+}
+
+void Solver::restore_checkpoint() {
+    SFH_HIP(hipMemcpyAsync(Z0_, ckZ_, T_ * sizeof(double2), hipMemcpyDeviceToDevice, stream_));
+    SFH_HIP(hipMemcpyAsync(P0_, ckP_, T_ * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+    SFH_HIP(hipMemcpyAsync(mu_, ckMu_, T_ * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+    SFH_HIP(hipMemcpyAsync(gmhist_, ckHist_, 4 * kHistDepth * (size_t)T_ * sizeof(double),
+                           hipMemcpyDeviceToDevice, stream_));
+    SFH_HIP(hipMemcpyAsync(gmhvalid_, ckHv_, 4 * sizeof(int), hipMemcpyDeviceToDevice, stream_));
+}
+
+void Solver::run_steps(double dt, int len, int count) {
+    if (opt_.use_graph && !ktime_) {
+        if (!exec_ || graph_dt_ != dt || graph_len_ != len) capture(dt, len);
+        for (int b = 0; b < count; ++b) SFH_HIP(hipGraphLaunch(exec_, stream_));
+    } else {
+        seq_len_ = len;
+        for (int b = 0; b < count; ++b) enqueue_step(dt);
     }
 }
 
@@ -636,36 +668,32 @@ void Solver::step(double dt, int nsteps) {
     if (nsteps < 0) throw ArgError("negative step count");
     int i = 0;
     while (i < nsteps) {
-        const int len = step_len();
-        // steps that need no policy check between them go out as one batch: every step while the
-        // history fills is checked (it sets the trimmed length); a trimmed batch is checked after
-        // its first step only
-        const bool check = opt_.solver == kSolveGMRES && (hist_steps_ < kCalibSteps || len < gm_m_);
-        const int batch = check ? 1 : nsteps - i;
-        if (opt_.use_graph && !ktime_) {
-            if (!exec_ || graph_dt_ != dt || graph_len_ != len) capture(dt, len);
-            for (int b = 0; b < batch; ++b) SFH_HIP(hipGraphLaunch(exec_, stream_));
+        if (opt_.solver != kSolveGMRES || (hist_steps_ >= kCalibSteps && trim_ >= gm_m_)) {
+            // no trimming (LU, batched, or a GMRES sequence back at the cap): one batch
+            run_steps(dt, gm_m_, nsteps - i);
+            i = nsteps;
+        } else if (hist_steps_ < kCalibSteps) {
+            // the warm-start history fills: full-length sequences, one step at a time
+            run_steps(dt, gm_m_, 1);
+            ++i;
+            calibrate();
         } else {
-            seq_len_ = len;
-            for (int b = 0; b < batch; ++b) enqueue_step(dt);
-        }
-        i += batch;
-        if (check) {
-            after_step(len);
-            // the rest of a trimmed batch, checked every kCheckSteps steps, so that solves running
-            // out of launches lengthen the sequence within a bounded number of steps
-            while (hist_steps_ >= kCalibSteps && step_len() < gm_m_ && i < nsteps) {
-                const int len2 = step_len();
-                const int chunk = std::min(kCheckSteps, nsteps - i);
-                if (opt_.use_graph && !ktime_) {
-                    if (!exec_ || graph_dt_ != dt || graph_len_ != len2) capture(dt, len2);
-                    for (int b = 0; b < chunk; ++b) SFH_HIP(hipGraphLaunch(exec_, stream_));
-                } else {
-                    seq_len_ = len2;
-                    for (int b = 0; b < chunk; ++b) enqueue_step(dt);
-                }
+            // a trimmed chunk, kept only when none of its solves ran out of launches
+            const int chunk = std::min(chunk_, nsteps - i);
+            const int raw0 = unc_raw_;
+            save_checkpoint();
+            run_steps(dt, trim_, chunk);
+            sync();
+            if (unc_raw_ > raw0) {
+                restore_checkpoint();
+                unc_discard_ += unc_raw_ - raw0;
+                stats_.unconverged = unc_raw_ - unc_discard_;
+                stats_.redone_steps += chunk;
+                trim_ = std::min(gm_m_, 2 * trim_);
+                chunk_ = 2;
+            } else {
                 i += chunk;
-                if (i < nsteps) after_step(len2);
+                chunk_ = std::min(kCheckSteps, 2 * chunk_);
             }
         }
     }
@@ -720,6 +748,35 @@ void Solver::rhs(double* vx, double* vy, double* dphi, double* mu, double2* zp, 
     check_status();
 }
 
+void Solver::track_energies(bool on) {
+    SFH_HIP(hipStreamSynchronize(stream_));
+    if (on && !energy_) {
+        energy_ = dalloc<double>(4 * (size_t)T_);
+        SFH_HIP(hipMemset(energy_, 0, 4 * (size_t)T_ * sizeof(double)));
+    }
+    if (on != track_energy_) graph_len_ = -1;   // the step graph holds the energy pointer: re-capture
+    track_energy_ = on;
+}
+
+void Solver::energies(double* out) {
+    if (!track_energy_) throw ArgError("energies are not tracked (sfh_track_energies)");
+    std::vector<double> e(4 * (size_t)T_);
+    SFH_HIP(hipMemcpyAsync(e.data(), energy_, e.size() * sizeof(double), hipMemcpyDeviceToHost, stream_));
+    SFH_HIP(hipStreamSynchronize(stream_));
+    // scales of KineticEnergy, GravitationalEnergy / VanDerWaalsEnergy, SurfaceEnergy and VolumeFlux
+    // (Energies.cuh:61-128); the potential is multiplied by g (1 in the reference)
+    for (int s = 0; s < S_; ++s) {
+        double sum[4] = {0.0, 0.0, 0.0, 0.0};
+        for (int i = 0; i < n_[s]; ++i)
+            for (int c = 0; c < 4; ++c) sum[c] += e[4 * ((size_t)desc_[s].voff + i) + c];
+        out[4 * s + 0] = sum[0] * 0.25 / kPi;
+        out[4 * s + 1] = ph_.g * (ph_.kind == 0 ? sum[1] * 0.25 * (1.0 + ph_.rho) / kPi
+                                                : sum[1] * ph_.depth * ph_.depth / 6.0);
+        out[4 * s + 2] = (sum[2] - 2.0 * kPi) * ph_.kappa / (2.0 * kPi);
+        out[4 * s + 3] = sum[3] * 0.5 / kPi;
+    }
+}
+
 void Solver::get_mu(double* mu) {
     SFH_HIP(hipMemcpyAsync(mu, mu_, T_ * sizeof(double), hipMemcpyDeviceToHost, stream_));
     SFH_HIP(hipStreamSynchronize(stream_));
@@ -751,6 +808,8 @@ void Solver::check_status() {
             fill = std::max(fill, c.fill_iters);
         }
         fill_iters_ = fill;
+        unc_raw_ = unc;
+        unc -= unc_discard_;
         stats_.last_iterations = ctl[2];
         // mixed precision: 1 when surface 0's last refinement solve ran Arnoldi steps
         if (opt_.precision == kMixed) stats_.refinements = ctl[1] > 0 ? 1 : 0;
@@ -775,6 +834,7 @@ void Solver::check_status() {
 
 Stats Solver::stats() {
     sync();
+    stats_.sequence_length = (opt_.solver == kSolveGMRES && hist_steps_ >= kCalibSteps) ? trim_ : 0;
     return stats_;
 }
 

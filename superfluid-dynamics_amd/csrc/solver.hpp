@@ -35,10 +35,15 @@ struct Options {
     int precision = kFP64;
     int solver = kSolveLU;
     int use_graph = 1;
-    int max_iter = 50;       // GMRES iterations / refinement sweeps
-    int restart = 30;        // GMRES restart length
+    int max_iter = 32;       // GMRES Arnoldi steps per solve (at most kGmMaxIter) / batched GMRES cap
+    int restart = 0;         // unused: the multi-workgroup GMRES does not restart, the batched one
+                             // restarts every 8 vectors
+    int force_trim = 0;      // test hook: trimmed GMRES sequence length instead of the calibrated one
     double tol = 1e-12;      // GMRES relative residual / refinement tolerance
 };
+
+// Most Arnoldi steps of one multi-workgroup GMRES solve (its launch sequence has no restart).
+constexpr int kGmMaxIter = 32;
 
 struct Stats {
     long long rhs_evals = 0;
@@ -49,6 +54,8 @@ struct Stats {
     int max_iterations = 0;    // iterative solvers: most iterations any surface used last solve
     int unconverged = 0;       // iterative solves that hit the iteration cap
     int refinements = 0;       // mixed precision: refinement sweeps of the last solve (surface 0)
+    int redone_steps = 0;      // GMRES: steps discarded and run again with a longer launch sequence
+    int sequence_length = 0;   // GMRES: launches per solve of the trimmed sequence (0 while calibrating)
 };
 
 bool supported_n(int n);
@@ -88,6 +95,12 @@ class Solver {
     void step_vjp(double dt, const double* gx_in, const double* gy_in, const double* gp_in, double* gx_out,
                   double* gy_out, double* gp_out);
 
+    // Energies (EnergyContainer + VolumeFlux, Energies.cuh:136-204): while tracking is on, every
+    // right-hand side also writes its per-node energy terms; energies() returns, per surface,
+    // {kinetic, potential, surface, volume flux} of the last right-hand side evaluated.
+    void track_energies(bool on);
+    void energies(double* out);
+
     // Standalone derivative of a complex field (FftDerivative::exec semantics).
     void derivative(const double2* in_host, double2* out_host, bool second, double scale);
 
@@ -107,8 +120,10 @@ class Solver {
     bool enqueue_solve(const double2* Z, int stage, bool g1);
     void enqueue_step(double dt);
     void capture(double dt, int len);
-    int step_len();                 // GMRES launches per solve for the next step (trimming policy)
-    void after_step(int len);       // bookkeeping of the trimming policy after one step
+    void run_steps(double dt, int len, int count);   // count steps with GMRES sequences of len launches
+    void calibrate();               // bookkeeping after a full-length step while the history fills
+    void save_checkpoint();         // state, last solution and warm-start history, device to device
+    void restore_checkpoint();
     void check_status();
     template <class F>
     void launch(const char* tag, double flops, double bytes, F&& f, int iter = -1);
@@ -200,12 +215,23 @@ class Solver {
     // what the solves of that last step needed (one launch per Arnoldi step, one to finish, a spare
     // step when they needed 3 or more), and doubled (up to the cap) whenever a check finds a solve
     // that ran out of launches.  seq_len_ is the length the kernels are being enqueued with.
+    // A trimmed sequence is run in chunks of steps.  Before a chunk the state, the last solution and
+    // the warm-start history are copied aside; when a solve of the chunk ran out of launches the
+    // copy is restored, the length doubled (up to the cap) and the chunk run again, so every step
+    // that is kept had all of its solves converge (or ran with the full cap).
     int seq_len_ = 0;
     int trim_ = 0;          // 0: not determined yet
     int hist_steps_ = 0;    // steps since set_state (saturates at kCalibSteps)
-    int unc_seen_ = 0;      // unconverged count at the last check
+    int chunk_ = 2;         // steps of the next trimmed chunk (doubles up to kCheckSteps)
+    int unc_raw_ = 0;       // unconverged count of the control blocks (discarded chunks included)
+    int unc_discard_ = 0;   // ... of solves in chunks that were run again
     int fill_iters_ = 0;    // most Arnoldi steps of a solve with the full history, any surface
+    double2* ckZ_ = nullptr;
+    double *ckP_ = nullptr, *ckMu_ = nullptr, *ckHist_ = nullptr;
+    int* ckHv_ = nullptr;
     Stats stats_;
+    double* energy_ = nullptr;   // 4 doubles per node while energies are tracked
+    bool track_energy_ = false;
 };
 
 }  // namespace sfh

@@ -307,6 +307,8 @@ __device__ inline void velocity_rows(const SurfDesc* __restrict__ sd, const Node
     if (ctl.qup) ctl.qup[g] = q2;
     if (!ph.need_upper) q2 = make_double2(0.0, 0.0);
     const double dphi = bernoulli(ph, y_t, q1, q2, q1q, zp_t, zpp_t);
+    // energy terms (tracked on request only: their inputs are loaded here, not held across the sweep)
+    if (ctl.energy) energy_store(ph, ctl, g, z[k].x, y_t, ctl.Pin[g], zp_t, q1);
     stage_store(ctl, g, q1, dphi, st);
 }
 

@@ -38,13 +38,14 @@ class SfhProps(Structure):
 
 class SfhOptions(Structure):
     _fields_ = [("precision", c_int), ("solver", c_int), ("use_graph", c_int), ("max_iter", c_int),
-                ("restart", c_int), ("reserved", c_int), ("tol", c_double)]
+                ("restart", c_int), ("force_trim", c_int), ("tol", c_double)]
 
 
 class SfhStats(Structure):
     _fields_ = [("rhs_evals", c_longlong), ("steps", c_longlong), ("fallback_solves", c_int),
                 ("spin_timeouts", c_int), ("last_iterations", c_int), ("max_iterations", c_int),
-                ("unconverged", c_int), ("refinements", c_int)]
+                ("unconverged", c_int), ("refinements", c_int), ("redone_steps", c_int),
+                ("sequence_length", c_int)]
 
 
 DP = POINTER(c_double)
@@ -74,6 +75,8 @@ SIGNATURES = {
     "sfh_rhs_vjp": (c_int, [c_void_p, DP, DP, DP, DP, DP, DP]),
     "sfh_step_vjp": (c_int, [c_void_p, c_double, DP, DP, DP, DP, DP, DP]),
     "sfh_get_mu": (c_int, [c_void_p, DP]),
+    "sfh_track_energies": (c_int, [c_void_p, c_int]),
+    "sfh_get_energies": (c_int, [c_void_p, DP]),
     "sfh_sync": (c_int, [c_void_p]),
     "sfh_get_stats": (c_int, [c_void_p, POINTER(SfhStats)]),
     "sfh_device_state": (c_int, [c_void_p, POINTER(c_void_p), POINTER(c_void_p)]),

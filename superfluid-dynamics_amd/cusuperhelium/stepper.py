@@ -48,15 +48,15 @@ class Props:
 
 class Ensemble:
     def __init__(self, sizes: Sequence[int] | int, props: Props | None = None, precision: int = FP64,
-                 solver: int = LU, use_graph: bool = True, tol: float = 1e-12, max_iter: int = 50,
-                 restart: int = 30):
+                 solver: int = LU, use_graph: bool = True, tol: float = 1e-12, max_iter: int = 32,
+                 force_trim: int = 0):
         self.sizes = [int(sizes)] if np.isscalar(sizes) else [int(n) for n in sizes]
         self.props = props or Props()
         self._lib = lib()
         opt = SfhOptions()
         self._lib.sfh_default_options(byref(opt))
         opt.precision, opt.solver, opt.use_graph = int(precision), int(solver), int(use_graph)
-        opt.tol, opt.max_iter, opt.restart = float(tol), int(max_iter), int(restart)
+        opt.tol, opt.max_iter, opt.force_trim = float(tol), int(max_iter), int(force_trim)
         arr = (c_int * len(self.sizes))(*self.sizes)
         h = c_void_p()
         cp = self.props.c()
@@ -146,6 +146,16 @@ class Ensemble:
         out = np.empty(self.total)
         check(self._lib.sfh_get_mu(self._h, _dp(out)), "sfh_get_mu")
         return out
+
+    def track_energies(self, enable: bool = True) -> None:
+        """Per-RHS energy terms on (EnergyContainer + VolumeFlux, Energies.cuh:136-204)."""
+        check(self._lib.sfh_track_energies(self._h, int(bool(enable))), "sfh_track_energies")
+
+    def energies(self) -> np.ndarray:
+        """[surfaces, 4]: kinetic, potential, surface, volume flux of the last right-hand side."""
+        out = np.empty(4 * len(self.sizes))
+        check(self._lib.sfh_get_energies(self._h, _dp(out)), "sfh_get_energies")
+        return out.reshape(len(self.sizes), 4)
 
     def sync(self) -> None:
         check(self._lib.sfh_sync(self._h), "sfh_sync")

@@ -1,16 +1,19 @@
 """Writes the long-horizon golden fixtures of the GPU parity tests by running the CPU FP64 oracle
 (oracle/libsfo.so, OpenMP) on the seeded workloads of cusuperhelium.workloads.
 
-    python tests/golden/make_golden.py [c2] [c3] [c3long] [c5]
+    python tests/golden/make_golden.py [c1] [c2] [c3] [c3long] [c5] [mu]
 
+  c1      N = 256 water surface y = 0.1 cos x, dt = 0.01: (Z, phi) after 100 RK4 steps
   c2      N = 2048 water surface, dt = 0.002: (Z, phi) after 100, 500 and 1000 RK4 steps
   c3      all 512 surfaces of the uneven ensemble: (Z, phi) after 20 steps
   c3long  two surfaces of each N in {128, 256, 512, 1024}: (Z, phi) after 100 and 500 steps
   c5      N = 4096 helium film (h0 = 1, image term, g_eff = 0.15): (Z, phi) after 2, 10 and 20 steps
 
 Each fixture is an uncompressed .npz of float64 / complex128 arrays (no pickles) beside this
-script.  The oracle runs one surface at a time; every surface evolves independently, so a fixture
-of a sub-ensemble holds exactly the trajectories of the full one.
+script; `mu` adds to every fixture the oracle's dipole strengths mu of the right-hand side at each
+stored state (BaseBoundaryIntegrator.cuh:275-306), where they are missing.  The oracle runs one
+surface at a time; every surface evolves independently, so a fixture of a sub-ensemble holds
+exactly the trajectories of the full one.
 """
 import os
 import sys
@@ -36,6 +39,12 @@ def _checkpoints(props, Z, P, dt, marks):
         done = m
         out[m] = (Z.copy(), P.copy())
     return out
+
+
+def make_c1():
+    Z0, P0, dt, steps = W.c1()
+    Z, P = O.rk4(O.Props(), Z0, P0, dt, steps)
+    np.savez(os.path.join(HERE, "c1_water_n256.npz"), steps=np.array([steps]), Z=Z[None], phi=P[None])
 
 
 def make_c2():
@@ -82,11 +91,41 @@ def make_c5():
              Z=np.array([cp[m][0] for m in marks]), phi=np.array([cp[m][1] for m in marks]))
 
 
-MAKERS = {"c2": make_c2, "c3": make_c3, "c3long": make_c3long, "c5": make_c5}
+FIXTURES = {"c1_water_n256.npz": None, "c2_water_n2048.npz": None, "c3_ensemble_512_20steps.npz": None,
+            "c3_sample_500steps.npz": None, "c5_film_n4096.npz": W.C5_PROPS}
+
+
+def add_mu():
+    """mu at every stored state of every fixture that lacks it (one oracle RHS per surface)."""
+    for name, kw in FIXTURES.items():
+        path = os.path.join(HERE, name)
+        if not os.path.exists(path):
+            continue
+        with np.load(path) as f:
+            d = {k: f[k] for k in f.files}
+        if "mu" in d:
+            continue
+        props = O.Props(**kw) if kw else O.Props()
+        Zs = d["Z"] if d["Z"].ndim == 2 else d["Z"][None]
+        Ps = d["phi"] if d["phi"].ndim == 2 else d["phi"][None]
+        sizes = [int(v) for v in d["sizes"]] if "sizes" in d else [Zs.shape[1]]
+        mus = []
+        for Z, P in zip(Zs, Ps):
+            mu, o = [], 0
+            for n in sizes:
+                _, _, diag = O.rhs(props, Z[o:o + n], P[o:o + n], diagnostics=True)
+                mu.append(diag["mu"])
+                o += n
+            mus.append(np.concatenate(mu))
+        d["mu"] = np.array(mus) if d["Z"].ndim == 2 else mus[0]
+        np.savez(path, **d)
+
+
+MAKERS = {"c1": make_c1, "c2": make_c2, "c3": make_c3, "c3long": make_c3long, "c5": make_c5, "mu": add_mu}
 
 if __name__ == "__main__":
     O.build()
-    for name in sys.argv[1:] or list(MAKERS):
+    for name in sys.argv[1:] or [k for k in MAKERS if k != "mu"] + ["mu"]:
         t0 = time.perf_counter()
         MAKERS[name]()
         print(f"{name}: {time.perf_counter() - t0:.1f} s on {O.num_threads()} threads", flush=True)

@@ -75,3 +75,15 @@ def test_entry_points_reject_bad_arguments_without_gpu():
     assert lib.sfh_create(None, None, 0, None, None) == -3
     assert lib.calculateRHSFromVectors(None, None, None, None, None, None, 1.0, 0.0, 0.0, 1.0, 100) == -2
     assert lib.integrateSimulationRK4_freeMemory(None, None) == 0
+
+
+def test_gmres_rejects_iteration_cap_above_sequence_limit_without_gpu():
+    # the multi-workgroup GMRES launch sequence has no restart: more than 32 Arnoldi steps per
+    # solve is an argument error (checked before any device call), not a silent clamp
+    import cusuperhelium as c
+    with pytest.raises(c.SolverError) as ei:
+        c.Ensemble([256], solver=c.GMRES, max_iter=40)
+    assert ei.value.code == -3 and "max_iter" in str(ei.value)
+    with pytest.raises(c.SolverError) as ei:
+        c.Ensemble([256], solver=c.GMRES, max_iter=8, force_trim=-1)
+    assert ei.value.code == -3

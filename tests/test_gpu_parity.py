@@ -262,7 +262,7 @@ def test_gmres_rhs_matches_oracle(oracle, csh, n, kw):
     Z = _surface(n, 40 + n, amp=0.05)
     Phi = 0.01 * np.sin(2 * np.pi * np.arange(n) / n + 0.3)
     q, dp, d = oracle.rhs(po, Z, Phi, diagnostics=True)
-    with csh.Ensemble([n], pc, solver=csh.GMRES, tol=1e-13, max_iter=40) as e:
+    with csh.Ensemble([n], pc, solver=csh.GMRES, tol=1e-13, max_iter=32) as e:
         e.set_state(Z, Phi)
         qg, dpg, dg = e.rhs(diagnostics=True)
         st = e.stats()
@@ -311,6 +311,38 @@ def test_gmres_trimmed_sequence_matches_lu(csh, kw):
     print(f"trimmed gmres {kw}: last {st['last_iterations']} max {st['max_iterations']} "
           f"unconverged {st['unconverged']} |dZ| {np.abs(Zg - Zl).max():.2e}", flush=True)
     assert st["unconverged"] == 0
+    assert np.abs(Zg - Zl).max() < 1e-11
+    assert np.abs(Pg - Pl).max() < 1e-11
+
+
+@pytest.mark.parametrize("kw,redo", [(dict(kind=1, depth=1.0), True), ({}, False)])
+def test_gmres_trimmed_chunks_are_redone_until_converged(csh, kw, redo):
+    # force_trim = 2 gives the helium solves (4-6 Arnoldi steps from the warm start) one Arnoldi
+    # step per solve: every solve of the first trimmed chunk runs out of launches, so the chunk is
+    # restored from its checkpoint (state, last solution, warm-start history) and run again with the
+    # sequence doubled until its solves converge.  No kept step may have an unconverged solve, and
+    # the trajectory (split over several step() calls) matches the LU stepper to 1e-11 after 40
+    # steps.  The water surface takes 1-2 Arnoldi steps per solve at tol 1e-13, so length 2 runs out
+    # at most in the first chunk (2 steps), after which length 4 holds.
+    n = 1024
+    Z0 = _surface(n, 91, amp=0.04)
+    P0 = 0.01 * np.cos(2 * np.pi * np.arange(n) / n)
+    pc = csh.Props(**kw)
+    with csh.Ensemble([n], pc, solver=csh.LU) as e:
+        e.set_state(Z0, P0)
+        e.step(0.002, 40)
+        Zl, Pl = e.get_state()
+    with csh.Ensemble([n], pc, solver=csh.GMRES, tol=1e-13, max_iter=32, force_trim=2) as e:
+        e.set_state(Z0, P0)
+        for k in (3, 2, 7, 28):
+            e.step(0.002, k)
+        Zg, Pg = e.get_state()
+        st = e.stats()
+    print(f"forced trim 2 {kw}: redone {st['redone_steps']} unconverged {st['unconverged']} "
+          f"max iterations {st['max_iterations']} |dZ| {np.abs(Zg - Zl).max():.2e}", flush=True)
+    assert st["unconverged"] == 0
+    assert st["redone_steps"] > 0 if redo else st["redone_steps"] <= 2
+    assert st["steps"] == 40
     assert np.abs(Zg - Zl).max() < 1e-11
     assert np.abs(Pg - Pl).max() < 1e-11
 

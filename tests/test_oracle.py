@@ -338,6 +338,83 @@ def test_helium_surface_tension_branch(oracle):
     assert np.abs((r1 - 0.2 * curv) - 20.447761896665416 * plain).max() < 4e-15
 
 
+# ------------------------------------------------------------------ energies (f2)
+def _energies(oracle, props, Z, Phi):
+    q, _, d = oracle.rhs(props, Z, Phi, diagnostics=True)
+    return oracle.energies(props, Z, Phi, d["Zp"], q), q, d
+
+
+@pytest.mark.parametrize("n,m", [(64, 1), (128, 3)])
+def test_energies_flat_surface_closed_form(oracle, n, m):
+    # Deep water, y = 0, phi = eps sin(m x): kinetic energy (1/2) int phi phi_n dx = eps^2 m pi / 2;
+    # the reference's kinetic scale 1/4pi (Energies.cuh:64) on sum_k phi_k Im(conj(Z'_k) q_k)
+    # (X'_k = 2 pi / n) gives eps^2 m / 4.  No potential energy, arc length 2 pi (surface term 0).
+    x = 2 * np.pi * np.arange(n) / n
+    eps = 1e-3
+    e, _, _ = _energies(oracle, oracle.Props(kappa=0.3), x + 0j, eps * np.sin(m * x))
+    assert abs(e[0] - eps * eps * m / 4) < 1e-18
+    assert e[1] == 0.0 and abs(e[2]) < 1e-14 and abs(e[3]) < 1e-18
+
+
+def test_energies_conserved_by_water_flow(oracle):
+    # Kinetic + gravitational energy (Energies.cuh:56-80: scales 1/4pi and (1+rho)/4pi on
+    # sum phi Im(conj Z' q) and sum y^2 X') is an invariant of deep-water flow: over 320 RK4 steps
+    # of a standing wave it moves between the two terms and its sum drifts < 1e-8 of the exchange.
+    # A wrong factor or sign in either term breaks the balance.
+    n = 64
+    x = 2 * np.pi * np.arange(n) / n
+    Z = x + 1j * (0.05 * np.cos(x) + 0.02 * np.sin(2 * x))
+    Phi = np.zeros(n)
+    es = [_energies(oracle, oracle.Props(), Z, Phi)[0]]
+    for _ in range(8):
+        Z, Phi = oracle.rk4(oracle.Props(), Z, Phi, 0.02, 40)
+        es.append(_energies(oracle, oracle.Props(), Z, Phi)[0])
+    es = np.array(es)
+    tot = es[:, 0] + es[:, 1]
+    swing = es[:, 0].max() - es[:, 0].min()
+    assert swing > 0.5 * es[0, 1]
+    assert np.abs(tot - tot[0]).max() < 1e-8 * swing
+
+
+def test_surface_energy_is_arc_length(oracle):
+    # SurfaceEnergy (Energies.cuh:98-111): kappa/2pi (sum_k |Z'_k| - 2 pi); for y = a cos x the sum
+    # is the arc length int sqrt(1 + a^2 sin^2 x) dx = 4 sqrt(1 + a^2) E(a^2 / (1 + a^2)).
+    from scipy.special import ellipe
+    n, a, kappa = 128, 0.3, 0.7
+    x = 2 * np.pi * np.arange(n) / n
+    Z = x + 1j * a * np.cos(x)
+    e, _, _ = _energies(oracle, oracle.Props(kappa=kappa), Z, np.zeros(n))
+    L = 4 * np.sqrt(1 + a * a) * ellipe(a * a / (1 + a * a))
+    assert abs(e[2] - kappa / (2 * np.pi) * (L - 2 * np.pi)) < 1e-14
+
+
+def test_van_der_waals_energy_closed_form(oracle):
+    # VanDerWaalsEnergy (Energies.cuh:82-96): h^2/6 sum_k ((1 + y_k/h)^-2 - 1) over the nodes (no
+    # node weights).  For y = a cos x at n equispaced nodes the sum is n ((1 - b^2)^(-3/2) - 1),
+    # b = a/h, up to aliasing far below rounding.
+    n, h, a = 128, 1.0, 0.3
+    x = 2 * np.pi * np.arange(n) / n
+    Z = x + 1j * a * np.cos(x)
+    zp = np.full(n, 2 * np.pi / n + 0j)
+    e = oracle.energies(oracle.Props(kind=oracle.HELIUM, depth=h, g=2.0), Z, np.zeros(n), zp, np.zeros(n, complex))
+    b = a / h
+    assert abs(e[1] - 2.0 * h * h / 6 * n * ((1 - b * b) ** -1.5 - 1)) < 1e-13 * abs(e[1])
+
+
+def test_volume_flux_sign_convention(oracle):
+    # VolumeFlux (Energies.cuh:113-128, 181-191) sums v X' + u Y' = Im(Z' q), not the normal flux
+    # Im(conj(Z') q) = v X' - u Y', whose sum vanishes for incompressible flow.  Hence the
+    # reference's diagnostic equals (1/pi) sum_k Y'_k u_k on a wavy surface.
+    n = 128
+    x = 2 * np.pi * np.arange(n) / n
+    Z = x + 1j * (0.05 * np.cos(x) + 0.02 * np.sin(3 * x + 0.4))
+    Phi = 0.03 * np.sin(2 * x + 0.1)
+    e, q, d = _energies(oracle, oracle.Props(), Z, Phi)
+    ref = np.sum(d["Zp"].imag * q.real) / np.pi
+    assert abs(ref) > 1e-6
+    assert abs(e[3] - ref) < 1e-15
+
+
 # ------------------------------------------------------------------ RK4 (A10)
 def test_rk4_rotation_is_exact_discrete_map(oracle):
     # ODE_Solvers.RK4 (ODESolverTests.cuh:14-23,73-246): dz/dt = i z.  Classical RK4 maps
