@@ -179,6 +179,24 @@ int sfh_rhs(sfh_handle h, double* vx, double* vy, double* dphi, double* mu, doub
 /* Advances every surface by nsteps classical RK4 steps of size dt, in place. */
 int sfh_step(sfh_handle h, double dt, int nsteps);
 
+/* Device-resident state in the reference's layout: complex128 [Z(T), Phi(T) + 0i] in device memory
+ * (BaseBoundaryIntegrator.cuh:141-146; the imaginary part of Phi is ignored on input).
+ * sfh_rhs_device is AutonomousProblem::run (AutonomousProblem.h:25, BaseBoundaryIntegrator.cuh:308-314):
+ * rhs = [dZ/dt (T), dPhi/dt + 0i (T)] of `state`, leaving the handle's own state untouched.  Each
+ * call has completed on return. */
+int sfh_set_state_device(sfh_handle h, const void* state);
+int sfh_get_state_device(sfh_handle h, void* state);
+int sfh_rhs_device(sfh_handle h, const void* state, void* rhs);
+
+/* Advances nsteps steps like sfh_step and records the state after every `every`-th step
+ * (TrajectoryLogger::logTrajectory, TrajectoryLogger.cuh:65-73, whose device vector grows by one
+ * state per logged step): frame f (f < nsteps / every) is states[f * 3T .. (f + 1) * 3T) =
+ * [x(T), y(T), phi(T)] in the node order of sfh_get_state (T = sfh_total_nodes), and times[f] the
+ * handle's time after that step (0 after sfh_set_state, advanced by dt per step).  Frames pass
+ * through a device ring of up to 32 frames and are copied to host on a second stream while the
+ * stepping continues; the call returns when every frame is in the caller's arrays. */
+int sfh_step_record(sfh_handle h, double dt, int nsteps, int every, double* states, double* times);
+
 /* Reverse mode of the water right-hand side (rho = 0, every N <= 1024); no counterpart in the
  * reference, which has no adjoint.  sfh_rhs_vjp: cotangents (gvx, gvy, gdphi) of the right-hand
  * side (vx, vy, dphi) at the current state -> cotangents (gx, gy, gphi) of the state (x, y, phi).

@@ -260,10 +260,15 @@ int integrateSimulationRK4(double* initialState, double** statesOut, size_t* sta
         double t = t0;
         try {
             if (traj) {
+                // every state through the device trajectory ring (copies overlap the stepping); the
+                // times accumulate from t0 as the reference's currentTime += dt
+                for (size_t done = 0; done < steps;) {
+                    const int chunk = (int)std::min<size_t>(steps - done, 1u << 30);
+                    s.step_record(dt, chunk, 1, states + 3 * N * done, times + done);
+                    done += (size_t)chunk;
+                }
                 for (size_t k = 0; k < steps; ++k) {
-                    s.step(dt, 1);
                     t += dt;
-                    s.get_state(states + 3 * N * k, states + 3 * N * k + N, states + 3 * N * k + 2 * N);
                     times[k] = t;
                 }
             } else {
@@ -414,6 +419,26 @@ int sfh_get_stats(sfh_handle h, sfh_stats* out) {
         out->redone_steps = st.redone_steps;
         out->sequence_length = st.sequence_length;
     });
+}
+
+int sfh_set_state_device(sfh_handle h, const void* state) {
+    SFH_NEED(h);
+    return guarded([&] { h->s->set_state_device(static_cast<const double2*>(state)); });
+}
+
+int sfh_get_state_device(sfh_handle h, void* state) {
+    SFH_NEED(h);
+    return guarded([&] { h->s->get_state_device(static_cast<double2*>(state)); });
+}
+
+int sfh_rhs_device(sfh_handle h, const void* state, void* rhs) {
+    SFH_NEED(h);
+    return guarded([&] { h->s->rhs_device(static_cast<const double2*>(state), static_cast<double2*>(rhs)); });
+}
+
+int sfh_step_record(sfh_handle h, double dt, int nsteps, int every, double* states, double* times) {
+    SFH_NEED(h);
+    return guarded([&] { h->s->step_record(dt, nsteps, every, states, times); });
 }
 
 int sfh_track_energies(sfh_handle h, int enable) {

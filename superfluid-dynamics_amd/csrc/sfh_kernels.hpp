@@ -149,6 +149,12 @@ template <typename T>
 __global__ void gm_iter_kernel(GmArgs a, const SurfDesc* sd, int j);
 __global__ void gm_close_mu_kernel(GmArgs a, const SurfDesc* sd, const double* circ, double2* mup, int i);
 
+// ---- solver.hip: one trajectory frame [x(T), y(T), phi(T)] of the state
+__global__ void record_frame_kernel(const double2* Z, const double* P, double* dst, int total);
+// state in the reference layout [Z(T), Phi(T) + 0i] <-> Z, P; and the right-hand side out
+__global__ void state_split_kernel(const double2* state, double2* Z, double* P, int total);
+__global__ void state_join_kernel(const double2* Z, const double* P, double2* state, int total);
+
 // ---- batched.hip (persistent per-surface stage kernel, ensembles with every N <= kBatchMaxN)
 constexpr int kBatchThreads = 1024;
 constexpr int kBatchMaxN = 1024;

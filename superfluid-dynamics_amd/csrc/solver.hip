@@ -275,6 +275,12 @@ Solver::~Solver() {
     for (void* p : vbufs)
         if (p) (void)hipFree(p);
     for (hipEvent_t e : event_pool_) (void)hipEventDestroy(e);
+    if (rec_stream_) (void)hipStreamSynchronize(rec_stream_);
+    for (hipEvent_t e : rec_packed_) (void)hipEventDestroy(e);
+    for (hipEvent_t e : rec_copied_) (void)hipEventDestroy(e);
+    if (rec_ring_) (void)hipFree(rec_ring_);
+    if (rec_host_) (void)hipHostFree(rec_host_);
+    if (rec_stream_) (void)hipStreamDestroy(rec_stream_);
     if (stream_) (void)hipStreamDestroy(stream_);
 }
 
@@ -699,6 +705,110 @@ void Solver::step(double dt, int nsteps) {
     }
     stats_.steps += nsteps;
     stats_.rhs_evals += 4LL * nsteps;
+    for (int k = 0; k < nsteps; ++k) time_ += dt;   // currentTime += dt per step (AutonomousRungeKuttaStepper.cuh:426)
+}
+
+__global__ __launch_bounds__(256) void record_frame_kernel(const double2* __restrict__ Z, const double* __restrict__ P,
+                                                           double* __restrict__ dst, int total) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    const double2 z = Z[i];
+    dst[i] = z.x;
+    dst[total + i] = z.y;
+    dst[2 * (long long)total + i] = P[i];
+}
+
+void Solver::step_record(double dt, int nsteps, int every, double* states, double* times) {
+    if (nsteps < 0 || every < 1) throw ArgError("nsteps must be >= 0 and every >= 1");
+    if (nsteps / every > 0 && (!states || !times)) throw ArgError("null output array");
+    const size_t frame = 3 * (size_t)T_;
+    if (!rec_ring_) {
+        // ring depth: up to 32 frames, at most 64 MiB of frames
+        rec_slots_ = (int)std::max<size_t>(2, std::min<size_t>(32, (64u << 20) / (frame * sizeof(double))));
+        rec_ring_ = dalloc<double>(frame * rec_slots_);
+        SFH_HIP(hipHostMalloc((void**)&rec_host_, frame * rec_slots_ * sizeof(double), hipHostMallocDefault));
+        SFH_HIP(hipStreamCreateWithFlags(&rec_stream_, hipStreamNonBlocking));
+        rec_packed_.resize(rec_slots_);
+        rec_copied_.resize(rec_slots_);
+        for (int k = 0; k < rec_slots_; ++k) {
+            SFH_HIP(hipEventCreateWithFlags(&rec_packed_[k], hipEventDisableTiming));
+            SFH_HIP(hipEventCreateWithFlags(&rec_copied_[k], hipEventDisableTiming));
+        }
+    }
+    const int frames = nsteps / every;
+    auto drain = [&](int f) {   // frame f's copy has been issued: wait for it, hand it over
+        const int slot = f % rec_slots_;
+        SFH_HIP(hipEventSynchronize(rec_copied_[slot]));
+        std::memcpy(states + (size_t)f * frame, rec_host_ + (size_t)slot * frame, frame * sizeof(double));
+    };
+    for (int f = 0; f < frames; ++f) {
+        step(dt, every);   // a GMRES chunk that was redone never reaches a frame: step() keeps only checked steps
+        times[f] = time_;
+        const int slot = f % rec_slots_;
+        if (f >= rec_slots_) drain(f - rec_slots_);   // the slot's previous frame is on the host
+        double* dst = rec_ring_ + (size_t)slot * frame;
+        hipLaunchKernelGGL(record_frame_kernel, dim3((T_ + 255) / 256), dim3(256), 0, stream_, Z0_, P0_, dst, T_);
+        SFH_HIP(hipGetLastError());
+        SFH_HIP(hipEventRecord(rec_packed_[slot], stream_));
+        SFH_HIP(hipStreamWaitEvent(rec_stream_, rec_packed_[slot], 0));
+        SFH_HIP(hipMemcpyAsync(rec_host_ + (size_t)slot * frame, dst, frame * sizeof(double), hipMemcpyDeviceToHost,
+                               rec_stream_));
+        SFH_HIP(hipEventRecord(rec_copied_[slot], rec_stream_));
+    }
+    for (int f = std::max(0, frames - rec_slots_); f < frames; ++f) drain(f);
+    if (nsteps > frames * every) step(dt, nsteps - frames * every);
+    sync();
+}
+
+__global__ __launch_bounds__(256) void state_split_kernel(const double2* __restrict__ state, double2* __restrict__ Z,
+                                                          double* __restrict__ P, int total) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    Z[i] = state[i];
+    P[i] = state[total + i].x;
+}
+
+__global__ __launch_bounds__(256) void state_join_kernel(const double2* __restrict__ Z, const double* __restrict__ P,
+                                                         double2* __restrict__ state, int total) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    state[i] = Z[i];
+    state[total + i] = make_double2(P[i], 0.0);
+}
+
+void Solver::set_state_device(const double2* state) {
+    if (!state) throw ArgError("null state");
+    hipLaunchKernelGGL(state_split_kernel, dim3((T_ + 255) / 256), dim3(256), 0, stream_, state, Z0_, P0_, T_);
+    SFH_HIP(hipGetLastError());
+    if (gmhvalid_) SFH_HIP(hipMemsetAsync(gmhvalid_, 0, 4 * sizeof(int), stream_));   // a new trajectory
+    if (bhvalid_) SFH_HIP(hipMemsetAsync(bhvalid_, 0, 4 * S_ * sizeof(int), stream_));
+    hist_steps_ = 0;
+    time_ = 0.0;
+    SFH_HIP(hipStreamSynchronize(stream_));
+}
+
+void Solver::get_state_device(double2* state) {
+    if (!state) throw ArgError("null state");
+    hipLaunchKernelGGL(state_join_kernel, dim3((T_ + 255) / 256), dim3(256), 0, stream_, Z0_, P0_, state, T_);
+    SFH_HIP(hipGetLastError());
+    sync();
+}
+
+void Solver::rhs_device(const double2* state, double2* rhs) {
+    if (!state || !rhs) throw ArgError("null state or rhs");
+    // the stage buffers Za/Pa are free between steps
+    hipLaunchKernelGGL(state_split_kernel, dim3((T_ + 255) / 256), dim3(256), 0, stream_, state, Za_, Pa_, T_);
+    SFH_HIP(hipGetLastError());
+    seq_len_ = gm_m_;   // a standalone solve: full launch sequence
+    StageCtl c{};
+    c.stage = 0;
+    c.kZ[0] = kZ_[0];
+    c.kP[0] = kP_[0];
+    enqueue_rhs(Za_, Pa_, c);
+    hipLaunchKernelGGL(state_join_kernel, dim3((T_ + 255) / 256), dim3(256), 0, stream_, kZ_[0], kP_[0], rhs, T_);
+    SFH_HIP(hipGetLastError());
+    stats_.rhs_evals += 1;
+    sync();
 }
 
 void Solver::set_state(const double* x, const double* y, const double* phi) {
@@ -708,6 +818,7 @@ void Solver::set_state(const double* x, const double* y, const double* phi) {
     SFH_HIP(hipMemcpyAsync(P0_, phi, T_ * sizeof(double), hipMemcpyHostToDevice, stream_));
     if (gmhvalid_) SFH_HIP(hipMemsetAsync(gmhvalid_, 0, 4 * sizeof(int), stream_));   // a new trajectory
     hist_steps_ = 0;
+    time_ = 0.0;
     if (bhvalid_) SFH_HIP(hipMemsetAsync(bhvalid_, 0, 4 * S_ * sizeof(int), stream_));
     SFH_HIP(hipStreamSynchronize(stream_));
 }

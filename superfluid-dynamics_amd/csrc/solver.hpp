@@ -74,11 +74,24 @@ class Solver {
     hipStream_t stream() const { return stream_; }
 
     void set_state(const double* x, const double* y, const double* phi);
+    // Device-resident state in the reference's layout, complex128 [Z(T), Phi(T) + 0i]
+    // (BaseBoundaryIntegrator.cuh:141-146).  rhs_device is AutonomousProblem::run (AutonomousProblem.h:25):
+    // rhs = [dZ/dt (T), dPhi/dt + 0i (T)] of `state`, the handle's own state left untouched; all
+    // three are enqueued on the handle's stream and complete before the call returns.
+    void set_state_device(const double2* state);
+    void get_state_device(double2* state);
+    void rhs_device(const double2* state, double2* rhs);
     void get_state(double* x, double* y, double* phi);
     // Right-hand side of the current state: vx + i vy = dZ/dt, dphi = dPhi/dt (host arrays).
     void rhs(double* vx, double* vy, double* dphi, double* mu = nullptr, double2* zp = nullptr,
              double2* zpp = nullptr, double2* qup = nullptr);
     void step(double dt, int nsteps);
+    // nsteps steps recording the state after every `every`-th one (TrajectoryLogger::logTrajectory,
+    // TrajectoryLogger.cuh:65-73): frame f = [x(T), y(T), phi(T)] into states[f * 3 T], the handle's
+    // time after that step into times[f].  Frames go through a device ring and are copied to host
+    // memory on a second stream while the stepping continues.
+    void step_record(double dt, int nsteps, int every, double* states, double* times);
+    double time() const { return time_; }
     void get_mu(double* mu);
     void sync();
     Stats stats();
@@ -232,6 +245,14 @@ class Solver {
     Stats stats_;
     double* energy_ = nullptr;   // 4 doubles per node while energies are tracked
     bool track_energy_ = false;
+    double time_ = 0.0;          // advanced by dt per step, 0 after set_state
+    // trajectory recording (allocated on first use): device ring of frames, pinned host staging,
+    // copy stream and one event pair per slot
+    int rec_slots_ = 0;
+    double* rec_ring_ = nullptr;
+    double* rec_host_ = nullptr;
+    hipStream_t rec_stream_ = nullptr;
+    std::vector<hipEvent_t> rec_packed_, rec_copied_;
 };
 
 }  // namespace sfh

@@ -72,6 +72,10 @@ SIGNATURES = {
     "sfh_get_state": (c_int, [c_void_p, DP, DP, DP]),
     "sfh_rhs": (c_int, [c_void_p, DP, DP, DP, DP, DP, DP, DP]),
     "sfh_step": (c_int, [c_void_p, c_double, c_int]),
+    "sfh_step_record": (c_int, [c_void_p, c_double, c_int, c_int, DP, DP]),
+    "sfh_set_state_device": (c_int, [c_void_p, c_void_p]),
+    "sfh_get_state_device": (c_int, [c_void_p, c_void_p]),
+    "sfh_rhs_device": (c_int, [c_void_p, c_void_p, c_void_p]),
     "sfh_rhs_vjp": (c_int, [c_void_p, DP, DP, DP, DP, DP, DP]),
     "sfh_step_vjp": (c_int, [c_void_p, c_double, DP, DP, DP, DP, DP, DP]),
     "sfh_get_mu": (c_int, [c_void_p, DP]),

@@ -119,6 +119,18 @@ class Ensemble:
     def step(self, dt: float, nsteps: int = 1) -> None:
         check(self._lib.sfh_step(self._h, float(dt), int(nsteps)), "sfh_step")
 
+    def step_record(self, dt: float, nsteps: int, every: int = 1):
+        """Advances nsteps steps and returns the states after every `every`-th one:
+        (Z [frames, total] complex, phi [frames, total], times [frames])."""
+        frames = int(nsteps) // int(every)
+        st = np.empty((max(frames, 1), 3 * self.total))
+        tm = np.empty(max(frames, 1))
+        check(self._lib.sfh_step_record(self._h, float(dt), int(nsteps), int(every), _dp(st), _dp(tm)),
+              "sfh_step_record")
+        st, tm = st[:frames], tm[:frames]
+        T = self.total
+        return st[:, :T] + 1j * st[:, T:2 * T], st[:, 2 * T:], tm
+
     # ---- reverse mode (water, rho = 0, every N <= 1024)
     def rhs_vjp(self, gq, gdphi):
         """Cotangent of (Z, Phi) from cotangents of (q = dZ/dt, dPhi/dt) at the current state.

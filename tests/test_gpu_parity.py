@@ -347,6 +347,35 @@ def test_gmres_trimmed_chunks_are_redone_until_converged(csh, kw, redo):
     assert np.abs(Pg - Pl).max() < 1e-11
 
 
+@pytest.mark.parametrize("solver", ["gmres", "lu", "batched"])
+def test_step_record_matches_stepping_and_reading(csh, solver):
+    # 100 steps recorded every 3rd (33 frames through the 32-slot device ring: it wraps) against a
+    # twin handle stepped 3 at a time and read back with sfh_get_state; the leftover step runs too
+    sizes = [256, 128]
+    sv = {"gmres": csh.GMRES, "lu": csh.LU, "batched": csh.BATCHED}[solver]
+    Z0 = np.concatenate([_surface(n, 60 + i) for i, n in enumerate(sizes)])
+    P0 = 0.005 * np.sin(np.arange(sum(sizes)) * 0.1)
+    with csh.Ensemble(sizes, solver=sv, tol=1e-12) as e:
+        e.set_state(Z0, P0)
+        Zf, Pf, tf = e.step_record(0.002, 100, 3)
+        Zend, Pend = e.get_state()
+    assert Zf.shape == (33, sum(sizes)) and tf.shape == (33,)
+    with csh.Ensemble(sizes, solver=sv, tol=1e-12) as e:
+        e.set_state(Z0, P0)
+        for f in range(33):
+            e.step(0.002, 3)
+            Z, P = e.get_state()
+            assert np.array_equal(Z, Zf[f]) and np.array_equal(P, Pf[f]), f
+        e.step(0.002, 1)
+        Z, P = e.get_state()
+    assert np.array_equal(Z, Zend) and np.array_equal(P, Pend)
+    t = 0.0
+    for f in range(33):
+        for _ in range(3):
+            t += 0.002
+        assert tf[f] == t
+
+
 def test_gmres_one_step_sequence_ends_in_velocity_kernel(csh):
     # Once every solve takes one Arnoldi step, the two-launch sequence (the product, then
     # velocity_gm1_kernel, which finishes the step, stores mu and forms mu' from the partial sums
