@@ -53,6 +53,35 @@ typedef struct RK4SolverOptions {
     bool returnTrajectory;
 } RK4SolverOptions;
 
+/* ExportTypes.cuh:21-34 (times and step in the nondimensional units the reference passes through). */
+typedef struct GaussLegendreOptions {
+    double t0;
+    double t1;
+    double stepSize;
+    double newtonTolerance;
+    size_t maxNewtonIterations;
+    bool allowSimplifiedFallback;
+    bool returnTrajectory;
+    double armijo_c;
+    double backtrack;
+    double minAlpha;
+    size_t maxStepsHalves;
+} GaussLegendreOptions;
+
+/* ExportTypes.cuh:41-64, SI units (adimensionalizeOptomechanicalVariables, Export.cu:1248-1270). */
+typedef struct COptomechanicalVariables {
+    double detuning;
+    double gamma;
+    double G;
+    double tau;
+    double max_intensity;
+    double initial_time;
+    double location_x0_mode;
+    double sigma_optical_mode;
+    double beta;
+    double damping_strength;
+} COptomechanicalVariables;
+
 /* ------------------------------------------------------------------------------------------
  * Drop-in entry points.
  * ---------------------------------------------------------------------------------------- */
@@ -97,6 +126,40 @@ int integrateSimulationRK4(double* initialState, double** statesOut, size_t* sta
 /* Export.cuh:70 / Export.cu:772-777 -- releases the arrays returned by integrateSimulationRK4. */
 int integrateSimulationRK4_freeMemory(double* statesOut, double* timesOut);
 
+/* Export.cuh:50 / Export.cu:393-518 -- central-difference Jacobian (JacobianCalculator.cuh:223-284) of
+ * the helium film right-hand side [vx, vy, dphi] with respect to the nondimensional state
+ * state = [x(N), y(N), phi(N)] (SI L, rho, kappa, depth as for calculateRHSFromVectors):
+ * jac[k * 3N + r] = d f_r / d state_k (column-major 3N x 3N), step epsilon.  N in {32, ..., 2048}.
+ * The 6N perturbed surfaces are evaluated as batches of the FP64 LU path. */
+int calculateJacobian(const double* state, double* jac, double L, double rho, double kappa, double depth,
+                      double epsilon, size_t N);
+
+/* Export.cuh:66 / Export.cu:663-740 -- two-stage Gauss-Legendre (GaussLegendre.cuh:190-560) of the
+ * helium film from glCOptions->t0 to t1 with step stepSize (nondimensional, used as given, like the
+ * reference), Newton on the stage slopes with the central-difference Jacobian (epsilon 1e-6), Armijo
+ * backtracking, optional frozen-Jacobian fallback, step halving up to maxStepsHalves times.
+ * initialState = [x(N), y(N), phi(N)].  With returnTrajectory: *statesOut = malloc'd [frames][3N]
+ * starting with the initial state, *timesOut = malloc'd [frames]; without it both are NULL and the
+ * counts 0, as in the reference.  N in {32, ..., 1024}.  -1 when a step fails at every size. */
+int integrateSimulationGL2(double* initialState, double** statesOut, size_t* statesCount, double** timesOut,
+                           size_t* timesCount, SimProperties* simProperties, GaussLegendreOptions* glCOptions,
+                           size_t N);
+/* Export.cuh:67 -- releases the arrays returned by integrateSimulationGL2. */
+int integrateSimulationGL2_freeMemory(double* statesOut, double* timesOut);
+
+/* Export.cuh:75 / Export.cu:981-1093 -- the optically driven helium film with the delayed intensity D
+ * as a third state field (HeliumDrivenAutonomousProblem + AugmentedBoundaryIntegrator), classical RK4
+ * from t0 to t1 (SI seconds), floor((t1 - t0)/dt) steps, every state logged.
+ * initialState = [x(N), y(N), phi(N), D(N)]; *statesOut = malloc'd [steps][4N] laid out as the
+ * reference writes it: x, y, phi and, in the fourth block, phi again (Export.cu:1057 copies the phi
+ * entry there, not D); *timesOut = malloc'd [steps] nondimensional times.  N in {32, 64, ..., 8192}. */
+int integrateAugmentedOptomechanicalSimulationRK4(double* initialState, double** statesOut, size_t* statesCount,
+                                                  double** timesOut, size_t* timesCount, SimProperties* simProperties,
+                                                  RK4SolverOptions* rkOptions,
+                                                  COptomechanicalVariables* optomechanicalVariables, size_t N);
+/* Export.cuh:76 -- releases the arrays returned by integrateAugmentedOptomechanicalSimulationRK4. */
+int integrateAugmentedOptomechanicalSimulationRK4_freeMemory(double* statesOut, double* timesOut);
+
 /* ------------------------------------------------------------------------------------------
  * Native handle API.
  * ---------------------------------------------------------------------------------------- */
@@ -110,7 +173,7 @@ enum { SFH_FP64 = 0, SFH_MIXED = 1 };
  * BATCHED: one persistent launch per RK stage, one workgroup per surface (every N <= 1024), GMRES
  * inside the workgroup; with SFH_MIXED the matrix is FP32 and mu is refined in FP64 against a
  * matrix-free FP64 residual. */
-enum { SFH_SOLVE_LU = 0, SFH_SOLVE_GMRES = 1, SFH_SOLVE_BATCHED = 2 };
+enum { SFH_SOLVE_LU = 0, SFH_SOLVE_GMRES = 1, SFH_SOLVE_BATCHED = 2, SFH_SOLVE_AUTO = 3 };
 
 /* Nondimensional problem (ProblemProperties.hpp:5-32 plus kind and g).  g scales the restoring
  * term of the Bernoulli equation (1 reproduces the reference).  rho_quirk = 1 reproduces the
@@ -142,7 +205,31 @@ typedef struct sfh_options {
                          the calibrated one (at least 2); a chunk of steps in which a solve runs out
                          of launches is still run again with the length doubled */
     double tol;       /* iterative solver / refinement relative tolerance */
+    int pair_rows;    /* rows per workgroup of the pair kernels (assembly, velocity): 1, 2 or 4; 0 = 4 */
+    int reserved;
 } sfh_options;
+
+/* SFH_SOLVE_AUTO picks solver, precision (SFH_MIXED in the options allows the mixed forms) and
+ * pair_rows from the autotuner: a single size (one surface or a uniform batch) takes the cached or
+ * freshly measured sfh_tuned() result of that size; an uneven ensemble of surfaces of at most 1024
+ * nodes takes SFH_SOLVE_BATCHED, a larger one SFH_SOLVE_GMRES with the tuned rows of its largest
+ * size.  No counterpart in the reference, whose launch shapes and solver are fixed
+ * (BoundaryProblem.cuh:27-34, MatrixSolver.cuh:113-172). */
+typedef struct sfh_tuning {
+    int solver;
+    int precision;
+    int pair_rows;
+    int from_cache;       /* 1 when sfh_tuned found the result in the cache */
+    double ms_per_step;   /* measured ms per RK4 step of the chosen form */
+} sfh_tuning;
+/* Times every candidate (LU and GMRES with 1, 2 and 4 rows per workgroup, the batched form for
+ * n <= 1024; the mixed forms too when allow_mixed) for `steps` RK4 steps of a seeded surface of n nodes
+ * after an 8-step warm-up, keeps the fastest whose solves converge, and caches it per (GPU
+ * architecture, kind, image term, n, allow_mixed) in $SFH_TUNE_CACHE, $XDG_CACHE_HOME or
+ * ~/.cache (cusuperhelium/tuning.txt). */
+int sfh_autotune(const sfh_props* props, int n, int allow_mixed, int steps, sfh_tuning* out);
+/* The cached result, or sfh_autotune(..., 10, ...) on a miss. */
+int sfh_tuned(const sfh_props* props, int n, int allow_mixed, sfh_tuning* out);
 
 typedef struct sfh_stats {
     long long rhs_evals;
@@ -158,6 +245,10 @@ typedef struct sfh_stats {
                               launches of the trimmed sequence (accumulated) */
     int sequence_length;   /* SFH_SOLVE_GMRES: launches per solve of the trimmed step graph, 0 while the
                               warm-start history fills (then every solve gets max_iter + 1) */
+    int solver;            /* the form the handle runs (what SFH_SOLVE_AUTO resolved to) */
+    int precision;
+    int pair_rows;
+    int reserved;
 } sfh_stats;
 
 /* Defaults: FP64, LU, graph replay on, max_iter 32, tol 1e-12. */
@@ -178,6 +269,39 @@ int sfh_rhs(sfh_handle h, double* vx, double* vy, double* dphi, double* mu, doub
             double* qup);
 /* Advances every surface by nsteps classical RK4 steps of size dt, in place. */
 int sfh_step(sfh_handle h, double dt, int nsteps);
+
+/* Central-difference Jacobian of the right-hand side of one surface (nondimensional problem, n <= 2048,
+ * n = 2^k or 3 * 2^k): jac[k * 3n + r] = d f_r / d s_k for s = [x, y, phi], f = [vx, vy, dphi]. */
+int sfh_jacobian(const sfh_props* props, int n, const double* state, double eps, double* jac);
+
+/* Optical drive, nondimensional (OptomechanicalVariables after adimensionalizeOptomechanicalVariables,
+ * OptomechanicalVariables.h:4-27; strength = get_current_intensity_drive_strength, LightIntensity.cuh:31-34):
+ *   dPhi/dt += damping v + strength I(y, x)            (HeliumDrivenAutonomousProblem, createM.cuh:138-149)
+ *   I = gamma^2 max_intensity / 4 / ((detuning - G y)^2 + (gamma/2)^2) exp(-(x - x0)^2 / (2 sigma^2))
+ * and with augmented = 1 the delayed intensity D as a third state field (AugmentedBoundaryIntegrator,
+ * DelayedIntensityIntegrator.cuh:20-32):  dPhi/dt += D,  dD/dt = beta I - D / tau.
+ * sfh_set_drive (NULL: drive off) zeroes D; sfh_set_state zeroes it too.  sfh_set_delayed /
+ * sfh_get_delayed move D (total nodes); sfh_get_delayed_rhs returns dD/dt of the last sfh_rhs.
+ * With the augmented drive, sfh_step_record frames are [x, y, phi, D] (4T).  Not with the adjoint
+ * or sfh_rhs_device. */
+typedef struct sfh_drive {
+    int augmented;
+    int reserved;
+    double damping;
+    double strength;
+    double detuning;
+    double gamma;
+    double G;
+    double max_intensity;
+    double x0;
+    double sigma;
+    double beta;
+    double tau;
+} sfh_drive;
+int sfh_set_drive(sfh_handle h, const sfh_drive* drive);
+int sfh_set_delayed(sfh_handle h, const double* D);
+int sfh_get_delayed(sfh_handle h, double* D);
+int sfh_get_delayed_rhs(sfh_handle h, double* dD);
 
 /* Device-resident state in the reference's layout: complex128 [Z(T), Phi(T) + 0i] in device memory
  * (BaseBoundaryIntegrator.cuh:141-146; the imaginary part of Phi is ignored on input).

@@ -37,6 +37,12 @@ class _Props(ctypes.Structure):
     ]
 
 
+class _Drive(ctypes.Structure):
+    _fields_ = [("augmented", ctypes.c_int), ("pad_", ctypes.c_int)] + [
+        (k, ctypes.c_double) for k in ("damping", "strength", "detuning", "gamma", "G", "max_intensity", "x0",
+                                       "sigma", "beta", "tau")]
+
+
 class _Diag(ctypes.Structure):
     _fields_ = [(n, ctypes.c_void_p) for n in ("mu", "Zp", "Zpp", "Phip", "mup", "qup")]
 
@@ -65,6 +71,29 @@ class Props:
         return _Props(int(self.kind), int(self.infinite_depth), int(self.use_expansions),
                       int(self.expansion_order), int(self.rho_quirk), int(self.image_zp), float(self.rho),
                       float(self.kappa), float(self.depth), float(self.U), float(self.g))
+
+
+@dataclass
+class Drive:
+    """Nondimensional optical drive (OptomechanicalVariables, OptomechanicalVariables.h:4-27, after
+    adimensionalizeOptomechanicalVariables); strength = get_current_intensity_drive_strength."""
+
+    augmented: bool = False
+    damping: float = 0.0
+    strength: float = 0.0
+    detuning: float = 0.0
+    gamma: float = 1.0
+    G: float = 1.0
+    max_intensity: float = 0.0
+    x0: float = 0.0
+    sigma: float = 1.0
+    beta: float = 0.0
+    tau: float = 1.0
+
+    def c(self) -> _Drive:
+        return _Drive(int(self.augmented), 0, float(self.damping), float(self.strength), float(self.detuning),
+                      float(self.gamma), float(self.G), float(self.max_intensity), float(self.x0),
+                      float(self.sigma), float(self.beta), float(self.tau))
 
 
 def build(force: bool = False) -> str:
@@ -102,6 +131,21 @@ def lib():
         L.sfo_rk4.restype = ctypes.c_int
         L.sfo_rk4_rotation.argtypes = [ctypes.c_size_t, ctypes.c_double, ctypes.c_int, dp]
         L.sfo_adimensionalize.argtypes = [ctypes.c_double] * 4 + [dp]
+        L.sfo_intensity.argtypes = [ctypes.POINTER(_Drive), ctypes.c_double, ctypes.c_double]
+        L.sfo_intensity.restype = ctypes.c_double
+        L.sfo_rhs_driven.argtypes = [ctypes.POINTER(_Props), ctypes.POINTER(_Drive), ctypes.c_int, ctypes.c_int,
+                                     dp, dp]
+        L.sfo_rhs_driven.restype = ctypes.c_int
+        L.sfo_rk4_driven.argtypes = [ctypes.POINTER(_Props), ctypes.POINTER(_Drive), ctypes.c_int, ctypes.c_int,
+                                     ctypes.c_double, ctypes.c_int, dp, dp]
+        L.sfo_rk4_driven.restype = ctypes.c_int
+        L.sfo_jacobian.argtypes = [ctypes.POINTER(_Props), ctypes.c_int, dp, ctypes.c_double, dp]
+        L.sfo_jacobian.restype = ctypes.c_int
+        L.sfo_gl2.argtypes = [ctypes.POINTER(_Props), ctypes.c_int, dp, ctypes.c_double, ctypes.c_double,
+                              ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_double, dp, dp, ctypes.c_int]
+        L.sfo_gl2.restype = ctypes.c_int
+        L.sfo_gl2_rotation.argtypes = [ctypes.c_int, dp, ctypes.c_double, ctypes.c_double, ctypes.c_double]
+        L.sfo_gl2_rotation.restype = ctypes.c_int
         L.sfo_energies.argtypes = [ctypes.POINTER(_Props), ctypes.c_int, dp, dp, dp, dp, dp]
         L.sfo_num_threads.restype = ctypes.c_int
         L.sfo_set_num_threads.argtypes = [ctypes.c_int]
@@ -257,6 +301,68 @@ def rk4(props: Props, Z, Phi, dt: float, steps: int, batch: int = 1, trajectory:
 def rk4_rotation(z0, dt: float, steps: int) -> np.ndarray:
     y = _c128(z0).copy()
     lib().sfo_rk4_rotation(y.size, float(dt), int(steps), _p(y.view(np.float64)))
+    return y
+
+
+def intensity(drive: Drive, y: float, x: float) -> float:
+    cd = drive.c()
+    return lib().sfo_intensity(ctypes.byref(cd), float(y), float(x))
+
+
+def _driven_state(drive, Z, Phi, D):
+    parts = [_c128(Z).ravel(), _c128(np.asarray(Phi, dtype=np.float64)).ravel()]
+    if drive.augmented:
+        parts.append(_c128(np.asarray(D, dtype=np.float64)).ravel())
+    return np.concatenate(parts)
+
+
+def rhs_driven(props: Props, drive: Drive, Z, Phi, D=None, batch: int = 1):
+    """(q, dPhi/dt, dD/dt or None) of the driven (augmented) helium system."""
+    state = _driven_state(drive, Z, Phi, D)
+    tot = _c128(Z).size
+    out = np.empty_like(state)
+    cp, cd = props.c(), drive.c()
+    lib().sfo_rhs_driven(ctypes.byref(cp), ctypes.byref(cd), tot // batch, batch, _p(state.view(np.float64)),
+                         _p(out.view(np.float64)))
+    return out[:tot].copy(), out[tot:2 * tot].real.copy(), (out[2 * tot:].real.copy() if drive.augmented else None)
+
+
+def rk4_driven(props: Props, drive: Drive, Z, Phi, D, dt: float, steps: int, batch: int = 1):
+    """(Z, Phi, D or None) after `steps` RK4 steps of the driven (augmented) system."""
+    state = _driven_state(drive, Z, Phi, D)
+    tot = _c128(Z).size
+    cp, cd = props.c(), drive.c()
+    lib().sfo_rk4_driven(ctypes.byref(cp), ctypes.byref(cd), tot // batch, batch, float(dt), int(steps),
+                         _p(state.view(np.float64)), None)
+    return state[:tot].copy(), state[tot:2 * tot].real.copy(), (state[2 * tot:].real.copy() if drive.augmented else None)
+
+
+def jacobian(props: Props, state, eps: float = 1e-6) -> np.ndarray:
+    """Central-difference Jacobian of [vx, vy, dphi] w.r.t. the real state [x, y, phi] (3n), as the
+    (3n x 3n) matrix J[r, k] (the reference stores it column-major: column k = perturbed variable)."""
+    s = _f64(state).copy()
+    m = s.size
+    J = np.empty(m * m)
+    lib().sfo_jacobian(ctypes.byref(props.c()), m // 3, _p(s), float(eps), _p(J))
+    return J.reshape(m, m).T.copy()
+
+
+def gl2(props: Props, state, t0: float, t1: float, h: float, tol: float = 1e-10, max_newton: int = 20,
+        eps: float = 1e-6, max_frames: int = 10000):
+    """Gauss-Legendre (2 stages) trajectory of the real state [x, y, phi]: (frames [F, 3n], times [F])."""
+    s = _f64(state).copy()
+    traj = np.empty((max_frames, s.size))
+    times = np.empty(max_frames)
+    F = lib().sfo_gl2(ctypes.byref(props.c()), s.size // 3, _p(s), float(t0), float(t1), float(h), float(tol),
+                      int(max_newton), float(eps), _p(traj), _p(times), int(max_frames))
+    if F < 0:
+        raise RuntimeError("Gauss-Legendre step failed to converge")
+    return traj[:F].copy(), times[:F].copy()
+
+
+def gl2_rotation(y0, t1: float, h: float, tol: float = 1e-12) -> np.ndarray:
+    y = _f64(y0).copy()
+    lib().sfo_gl2_rotation(y.size // 2, _p(y), float(t1), float(h), float(tol))
     return y
 
 

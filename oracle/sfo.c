@@ -19,6 +19,16 @@
  *   sfo_rhs               BaseBoundaryIntegrator.cuh:137-306 (energies omitted: they do not
  *                         feed the right-hand side)
  *   sfo_rk4               AutonomousRungeKuttaStepper.cuh:31-36,123-361,417-437
+ *   sfo_intensity         LightIntensity::compute_intensity (LightIntensity.cuh:16-29)
+ *   sfo_rhs_driven        HeliumDrivenAutonomousProblem::CalculateRhsPhi (HeliumDrivenAutonomousProblem.cuh:19-25,
+ *                         add_optical_field_drive_terms_no_time_depence createM.cuh:138-149) and the
+ *                         augmented system's DelayedIntensityIntegrator::run (DelayedIntensityIntegrator.cuh:20-32,
+ *                         createM.cuh:151-169; AugmentedBoundaryIntegrator.cuh:32-36)
+ *   sfo_rk4_driven        the same RK4 over [Z, Phi(, D)]
+ *   sfo_jacobian          JacobianCalculator::calculateJacobian (JacobianCalculator.cuh:223-284): central
+ *                         differences of the RHS, column-major 3N x 3N
+ *   sfo_gl2               GaussLegendre2::runEvolution / gaussLegendreS2Step (GaussLegendre.cuh:190-560),
+ *                         two-stage Gauss-Legendre with Newton, Armijo backtracking and step halving
  *   sfo_energies          Energies.cuh:56-204 (kinetic, gravitational / van der Waals, surface,
  *                         volume flux), called per RHS at BaseBoundaryIntegrator.cuh:270-272
  *   sfo_adimensionalize   Export.cu:1222-1246
@@ -251,7 +261,7 @@ void sfo_assemble(const sfo_props *p, int n, int batch, const double *Z, const d
         const cplx *zp = (const cplx *)Zp + (size_t)b * n;
         const cplx *zpp = (const cplx *)Zpp + (size_t)b * n;
         double *m = M + (size_t)b * n * n;
-        #pragma omp parallel for schedule(static)
+        #pragma omp parallel for schedule(static) if (n >= 256)
         for (int k = 0; k < n; ++k) {
             for (int j = 0; j < n; ++j) {
                 double v;
@@ -324,7 +334,7 @@ int sfo_lu_factor(int n, double *A, int *piv)
             }
         }
         /* A22 -= L21 U12 */
-        #pragma omp parallel for schedule(static)
+        #pragma omp parallel for schedule(static) if (n - je > 256)
         for (int i = je; i < n; ++i) {
             double *ri = A + (size_t)i * n;
             for (int q = j0; q < je; ++q) {
@@ -373,7 +383,7 @@ void sfo_velocity(const sfo_props *p, int n, int batch, const double *Z, const d
         const double *a = mu + (size_t)b * n;
         const cplx *ap = (const cplx *)mup + (size_t)b * n;
         cplx *out = (cplx *)q + (size_t)b * n;
-        #pragma omp parallel for schedule(static)
+        #pragma omp parallel for schedule(static) if (n >= 256)
         for (int k = 0; k < n; ++k) {
             cplx v2 = I * (1.0 / (2.0 * PI_D * zp[k]));
             cplx w = v2 * ap[k];
@@ -570,6 +580,259 @@ static int rotation_rhs(void *ctx, const double *y, double *k)
 int sfo_rk4_rotation(size_t m, double dt, int steps, double *y)
 {
     return sfo_rk4_generic(rotation_rhs, &m, m, dt, steps, y, NULL);
+}
+
+/* ------------------------------------------------------- optical drive */
+/* Nondimensional optomechanical drive (OptomechanicalVariables after
+ * adimensionalizeOptomechanicalVariables, Export.cu:1248-1270).  strength is
+ * LightIntensity::get_current_intensity_drive_strength (LightIntensity.cuh:31-34). */
+typedef struct sfo_drive {
+    int augmented;    /* 1: state [Z, Phi, D], dD/dt = beta I - D / tau, dPhi/dt += D */
+    int pad_;
+    double damping;   /* DampingStrength */
+    double strength;
+    double detuning, gamma, G, max_intensity, x0, sigma, beta, tau;
+} sfo_drive;
+
+/* I = gamma^2 Imax / 4 / ((detuning - G y)^2 + (gamma/2)^2) * exp(-(x - x0)^2 / (2 sigma^2)) */
+double sfo_intensity(const sfo_drive *d, double y, double x)
+{
+    const double df = d->detuning - d->G * y;
+    const double prof = exp(-pow(x - d->x0, 2.0) / (2.0 * pow(d->sigma, 2.0)));
+    return 0.25 * pow(d->gamma, 2.0) * d->max_intensity / (pow(df, 2.0) + pow(d->gamma / 2.0, 2.0)) * prof;
+}
+
+/* state / rhs: complex [Z(T), Phi(T)(, D(T))] (T = n batch nodes), Phi and D with imaginary part 0.
+ * dPhi/dt = ((Bernoulli + damping v) + strength I) + D, in the reference's order of additions. */
+int sfo_rhs_driven(const sfo_props *p, const sfo_drive *d, int n, int batch, const double *state, double *rhs)
+{
+    const size_t tot = (size_t)n * batch;
+    const int info = sfo_rhs(p, n, batch, state, rhs, NULL);
+    for (size_t i = 0; i < tot; ++i) {
+        const double x = state[2 * i], y = state[2 * i + 1], v = rhs[2 * i + 1];
+        const double inten = sfo_intensity(d, y, x);
+        double r = rhs[2 * tot + 2 * i];
+        r += d->damping * v;
+        r += d->strength * inten;
+        if (d->augmented) {
+            const double D = state[4 * tot + 2 * i];
+            r += D;
+            rhs[4 * tot + 2 * i] = d->beta * inten - 1.0 / d->tau * D;
+            rhs[4 * tot + 2 * i + 1] = 0.0;
+        }
+        rhs[2 * tot + 2 * i] = r;
+    }
+    return info;
+}
+
+typedef struct { const sfo_props *p; const sfo_drive *d; int n, batch; } drive_ctx;
+
+static int driven_adapter(void *ctx, const double *y, double *k)
+{
+    drive_ctx *c = (drive_ctx *)ctx;
+    return sfo_rhs_driven(c->p, c->d, c->n, c->batch, y, k);
+}
+
+int sfo_rk4_driven(const sfo_props *p, const sfo_drive *d, int n, int batch, double dt, int steps, double *state,
+                   double *traj)
+{
+    drive_ctx c = {p, d, n, batch};
+    return sfo_rk4_generic(driven_adapter, &c, (size_t)(d->augmented ? 3 : 2) * n * batch, dt, steps, state, traj);
+}
+
+/* ------------------------------------------- FD Jacobian and Gauss-Legendre */
+/* Right-hand side of a real state [x(n), y(n), phi(n)] -> [vx(n), vy(n), dphi(n)]
+ * (RealBoundaryItegralCalculator, RealBoundaryIntegralCalculator.cuh:4-40). */
+typedef int (*sfo_real_fn)(void *ctx, const double *y, double *f);
+
+typedef struct { const sfo_props *p; int n; } real_ctx;
+
+static int real_rhs(void *ctx, const double *s, double *f)
+{
+    real_ctx *c = (real_ctx *)ctx;
+    const int n = c->n;
+    double *st = (double *)calloc(4 * (size_t)n + 1, sizeof(double));
+    double *rh = (double *)calloc(4 * (size_t)n + 1, sizeof(double));
+    for (int i = 0; i < n; ++i) {
+        st[2 * i] = s[i];
+        st[2 * i + 1] = s[n + i];
+        st[2 * n + 2 * i] = s[2 * n + i];
+        st[2 * n + 2 * i + 1] = 0.0;
+    }
+    int info = sfo_rhs(c->p, n, 1, st, rh, NULL);
+    for (int i = 0; i < n; ++i) {
+        f[i] = rh[2 * i];
+        f[n + i] = rh[2 * i + 1];
+        f[2 * n + i] = rh[2 * n + 2 * i];
+    }
+    free(st);
+    free(rh);
+    return info;
+}
+
+/* J[k m + r] = (f_r(s + eps e_k) - f_r(s - eps e_k)) / (2 eps), m = dim (column-major: column k is the
+ * perturbed variable, createJacobianMatrixFromPerturbedRhs, JacobianCalculator.cuh:68-104). */
+static int jacobian_generic(sfo_real_fn f, void *ctx, int m, const double *s, double eps, double *J)
+{
+    double *sp = (double *)malloc(sizeof(double) * m);
+    double *fp = (double *)malloc(sizeof(double) * m);
+    double *fm = (double *)malloc(sizeof(double) * m);
+    int info = 0, r;
+    for (int k = 0; k < m; ++k) {
+        memcpy(sp, s, sizeof(double) * m);
+        sp[k] = s[k] + eps;
+        if ((r = f(ctx, sp, fp)) && !info) info = r;
+        sp[k] = s[k] - eps;
+        if ((r = f(ctx, sp, fm)) && !info) info = r;
+        for (int q = 0; q < m; ++q) J[(size_t)k * m + q] = (fp[q] - fm[q]) / (2.0 * eps);
+    }
+    free(sp);
+    free(fp);
+    free(fm);
+    return info;
+}
+
+int sfo_jacobian(const sfo_props *p, int n, const double *s, double eps, double *J)
+{
+    real_ctx c = {p, n};
+    return jacobian_generic(real_rhs, &c, 3 * n, s, eps, J);
+}
+
+/* Gauss-Legendre two-stage coefficients (GLCoefficients.hpp): A = [[1/4, 1/4 - sqrt3/6],
+ * [1/4 + sqrt3/6, 1/4]], b = [1/2, 1/2]. */
+typedef struct {
+    double tol, armijo_c, backtrack, min_alpha;
+    int max_newton, max_halves;
+} gl2_opts;
+
+static double gl2_residual(sfo_real_fn f, void *ctx, int m, const double *y, double h, const double *k, double *R,
+                           double *ys, double *fs)
+{
+    const double s3 = 1.7320508075688772935;
+    const double A11 = 0.25, A12 = 0.25 - s3 / 6.0, A21 = 0.25 + s3 / 6.0, A22 = 0.25;
+    const double *k1 = k, *k2 = k + m;
+    for (int i = 0; i < m; ++i) ys[i] = y[i] + h * (A11 * k1[i] + A12 * k2[i]);
+    f(ctx, ys, fs);
+    for (int i = 0; i < m; ++i) R[i] = k1[i] - fs[i];
+    for (int i = 0; i < m; ++i) ys[i] = y[i] + h * (A21 * k1[i] + A22 * k2[i]);
+    f(ctx, ys, fs);
+    for (int i = 0; i < m; ++i) R[m + i] = k2[i] - fs[i];
+    double ss = 0.0;
+    for (int i = 0; i < 2 * m; ++i) ss += R[i] * R[i];
+    return ss;
+}
+
+/* One step of size h from y into yn.  Newton on R(k) = [k1 - f(y + h(A11 k1 + A12 k2)),
+ * k2 - f(y + h(A21 k1 + A22 k2))] with the matrix [[I - h A11 J1, -h A12 J1], [-h A21 J2, I - h A22 J2]]
+ * (J_i the Jacobian at stage state i), Armijo backtracking on |R|^2 / 2.  Returns 1 if converged. */
+static int gl2_step(sfo_real_fn f, void *ctx, int m, const double *y, double h, const gl2_opts *o, double eps,
+                    double *yn)
+{
+    const double s3 = 1.7320508075688772935;
+    const double A[2][2] = {{0.25, 0.25 - s3 / 6.0}, {0.25 + s3 / 6.0, 0.25}};
+    const int m2 = 2 * m;
+    double *k = (double *)malloc(sizeof(double) * m2), *kt = (double *)malloc(sizeof(double) * m2);
+    double *R = (double *)malloc(sizeof(double) * m2), *dk = (double *)malloc(sizeof(double) * m2);
+    double *ys = (double *)malloc(sizeof(double) * m), *fs = (double *)malloc(sizeof(double) * m);
+    double *J1 = (double *)malloc(sizeof(double) * m * m), *J2 = (double *)malloc(sizeof(double) * m * m);
+    double *Mx = (double *)malloc(sizeof(double) * m2 * m2);
+    int *piv = (int *)malloc(sizeof(int) * m2);
+    f(ctx, y, k);
+    memcpy(k + m, k, sizeof(double) * m);
+    int ok = 0;
+    for (int it = 0; it < o->max_newton; ++it) {
+        double ss = gl2_residual(f, ctx, m, y, h, k, R, ys, fs);
+        double kk = 0.0;
+        for (int i = 0; i < m2; ++i) kk += k[i] * k[i];
+        if (sqrt(ss) <= o->tol * (1.0 + kk)) { ok = 1; break; }
+        for (int st = 0; st < 2; ++st) {   /* stage states and their Jacobians */
+            for (int i = 0; i < m; ++i) ys[i] = y[i] + h * (A[st][0] * k[i] + A[st][1] * k[m + i]);
+            jacobian_generic(f, ctx, m, ys, eps, st ? J2 : J1);
+        }
+        for (int r = 0; r < m2; ++r) {   /* row-major Newton matrix */
+            const int bi = r / m, rr = r % m;
+            const double *J = bi ? J2 : J1;
+            for (int c = 0; c < m2; ++c) {
+                const int bj = c / m, cc = c % m;
+                Mx[(size_t)r * m2 + c] = (r == c ? 1.0 : 0.0) - h * A[bi][bj] * J[(size_t)cc * m + rr];
+            }
+        }
+        for (int i = 0; i < m2; ++i) dk[i] = -R[i];
+        sfo_lu_factor(m2, Mx, piv);
+        sfo_lu_solve(m2, Mx, piv, dk);
+        double alpha = 1.0, phi = 0.5 * ss;
+        for (;;) {
+            for (int i = 0; i < m2; ++i) kt[i] = k[i] + alpha * dk[i];
+            double sst = gl2_residual(f, ctx, m, y, h, kt, R, ys, fs);
+            if (0.5 * sst <= phi - o->armijo_c * alpha * ss) { memcpy(k, kt, sizeof(double) * m2); break; }
+            alpha *= o->backtrack;
+            if (alpha < o->min_alpha) goto done;
+        }
+    }
+    if (!ok) {   /* the last Newton update may have converged */
+        double ss = gl2_residual(f, ctx, m, y, h, k, R, ys, fs), kk = 0.0;
+        for (int i = 0; i < m2; ++i) kk += k[i] * k[i];
+        ok = sqrt(ss) <= o->tol * (1.0 + kk);
+    }
+done:
+    for (int i = 0; i < m; ++i) yn[i] = y[i] + h * (0.5 * k[i] + 0.5 * k[m + i]);
+    free(k); free(kt); free(R); free(dk); free(ys); free(fs); free(J1); free(J2); free(Mx); free(piv);
+    return ok;
+}
+
+/* runEvolution (GaussLegendre.cuh:190-290): steps of at most h until t1, halving a step that does
+ * not converge (up to max_halves times) and keeping the reduced size afterwards.  traj receives the
+ * initial state and every step's state (max_frames of them at most); returns the frame count, or
+ * -1 when a step fails at every size. */
+static int gl2_run(sfo_real_fn f, void *ctx, int m, double *y, double t0, double t1, double h, const gl2_opts *o,
+                   double eps, double *traj, double *times, int max_frames)
+{
+    double *yn = (double *)malloc(sizeof(double) * m);
+    int frames = 0;
+    if (traj && frames < max_frames) { memcpy(traj, y, sizeof(double) * m); times[0] = t0; }
+    frames = 1;
+    double t = t0;
+    while (t < t1) {
+        double heff = h < t1 - t ? h : t1 - t, ht = heff;
+        int ok = 0;
+        for (int i = 0; i <= o->max_halves; ++i) {
+            if ((ok = gl2_step(f, ctx, m, y, ht, o, eps, yn))) break;
+            ht *= 0.5;
+        }
+        if (!ok) { free(yn); return -1; }
+        memcpy(y, yn, sizeof(double) * m);
+        t += ht;
+        h = ht;
+        if (traj && frames < max_frames) {
+            memcpy(traj + (size_t)frames * m, y, sizeof(double) * m);
+            times[frames] = t;
+        }
+        ++frames;
+    }
+    free(yn);
+    return frames;
+}
+
+int sfo_gl2(const sfo_props *p, int n, double *s, double t0, double t1, double h, double tol, int max_newton,
+            double eps, double *traj, double *times, int max_frames)
+{
+    real_ctx c = {p, n};
+    gl2_opts o = {tol, 1e-4, 0.5, 1e-6, max_newton, 6};
+    return gl2_run(real_rhs, &c, 3 * n, s, t0, t1, h, &o, eps, traj, times, max_frames);
+}
+
+/* dx/dt = -y, dy/dt = x on m/2 pairs (the rotation of the RK4 test), for pinning the integrator */
+static int rotation_real(void *ctx, const double *y, double *f)
+{
+    const int h = *(int *)ctx;
+    for (int i = 0; i < h; ++i) { f[i] = -y[h + i]; f[h + i] = y[i]; }
+    return 0;
+}
+
+int sfo_gl2_rotation(int pairs, double *y, double t1, double h, double tol)
+{
+    gl2_opts o = {tol, 1e-4, 0.5, 1e-6, 20, 6};
+    return gl2_run(rotation_real, &pairs, 2 * pairs, y, 0.0, t1, h, &o, 1e-6, NULL, NULL, 0);
 }
 
 /* ------------------------------------------------------------- energies */

@@ -22,12 +22,11 @@ namespace sfh {
 // The rows of M (stored as T: FP64, or FP32 rounded from the FP64 entries; not stored when M is
 // null) and, with gm_on, r = b - M x in FP64 for x = the initial guess, or (residual pass) x = the
 // solution, whose own rows the pass then copies into the initial guess of the next solve.
-template <typename T>
+template <typename T, int R>
 __device__ inline void assemble_rows(const SurfDesc* __restrict__ sd, const NodeTrig* __restrict__ trig,
                                      const double2* __restrict__ Z, const double2* __restrict__ Zp,
                                      const double2* __restrict__ Zpp, T* __restrict__ M, const Phys& ph,
                                      const int* __restrict__ only_if, const GmArgs& gm, int gm_on, bool residual) {
-    constexpr int R = kPairRows;
     if (only_if && only_if[2 * blockIdx.y] == 0) return;   // per-surface status word
     const SurfDesc d = sd[blockIdx.y];
     const int n = d.n;
@@ -114,6 +113,7 @@ __device__ inline void assemble_rows(const SurfDesc* __restrict__ sd, const Node
     }
 }
 
+template <int R>
 __global__ __launch_bounds__(256) void assemble64_kernel(const SurfDesc* __restrict__ sd,
                                                          const NodeTrig* __restrict__ trig,
                                                          const double2* __restrict__ Z,
@@ -121,28 +121,42 @@ __global__ __launch_bounds__(256) void assemble64_kernel(const SurfDesc* __restr
                                                          const double2* __restrict__ Zpp,
                                                          double* __restrict__ M, Phys ph,
                                                          const int* __restrict__ only_if, GmArgs gm, int gm_on) {
-    assemble_rows<double>(sd, trig, Z, Zp, Zpp, M, ph, only_if, gm, gm_on, false);
+    assemble_rows<double, R>(sd, trig, Z, Zp, Zpp, M, ph, only_if, gm, gm_on, false);
 }
 
 // Mixed precision: M stored in FP32, r0 = b - M x0 from the FP64 entries.
+template <int R>
 __global__ __launch_bounds__(256) void assemble32_kernel(const SurfDesc* __restrict__ sd,
                                                          const NodeTrig* __restrict__ trig,
                                                          const double2* __restrict__ Z,
                                                          const double2* __restrict__ Zp,
                                                          const double2* __restrict__ Zpp,
                                                          float* __restrict__ M, Phys ph, GmArgs gm) {
-    assemble_rows<float>(sd, trig, Z, Zp, Zpp, M, ph, nullptr, gm, 1, false);
+    assemble_rows<float, R>(sd, trig, Z, Zp, Zpp, M, ph, nullptr, gm, 1, false);
 }
 
 // Mixed precision, refinement: r = b - M mu in FP64, matrix-free (the assembly's entries, not
 // stored), into the first basis vector; mu becomes the next solve's initial guess, and the
 // control block is re-armed (done = 0) for that solve.
+template <int R>
 __global__ __launch_bounds__(256) void residual64_kernel(const SurfDesc* __restrict__ sd,
                                                          const NodeTrig* __restrict__ trig,
                                                          const double2* __restrict__ Z,
                                                          const double2* __restrict__ Zp,
                                                          const double2* __restrict__ Zpp, Phys ph, GmArgs gm) {
-    assemble_rows<double>(sd, trig, Z, Zp, Zpp, static_cast<double*>(nullptr), ph, nullptr, gm, 1, true);
+    assemble_rows<double, R>(sd, trig, Z, Zp, Zpp, static_cast<double*>(nullptr), ph, nullptr, gm, 1, true);
 }
+
+// rows per workgroup the solver may launch with (chosen per surface size by the autotuner)
+#define SFH_ASSEMBLE_INSTANCES(R)                                                                             \
+    template __global__ void assemble64_kernel<R>(const SurfDesc*, const NodeTrig*, const double2*, const double2*, \
+                                                  const double2*, double*, Phys, const int*, GmArgs, int);        \
+    template __global__ void assemble32_kernel<R>(const SurfDesc*, const NodeTrig*, const double2*, const double2*, \
+                                                  const double2*, float*, Phys, GmArgs);                          \
+    template __global__ void residual64_kernel<R>(const SurfDesc*, const NodeTrig*, const double2*, const double2*, \
+                                                  const double2*, Phys, GmArgs);
+SFH_ASSEMBLE_INSTANCES(1)
+SFH_ASSEMBLE_INSTANCES(2)
+SFH_ASSEMBLE_INSTANCES(4)
 
 }  // namespace sfh

@@ -266,8 +266,13 @@ __global__ __launch_bounds__(1024) void batched_stage_kernel(BatchArgs a, Phys p
     const int gi = off + t;
     if (ctl.qup) ctl.qup[gi] = q2;
     if (!ph.need_upper) q2 = make_double2(0.0, 0.0);
-    const double dphi = bernoulli(ph, zk.y, q1, q2, q1q, zpk, zppk);
+    double dphi = bernoulli(ph, zk.y, q1, q2, q1q, zpk, zppk);
     if (ctl.energy) energy_store(ph, ctl, gi, zk.x, zk.y, ctl.Pin[gi], zpk, q1);
+    if (ph.drive) {
+        double dD;
+        dphi = drive_terms(ph, ctl, gi, zk.x, zk.y, q1, dphi, dD);
+        if (ph.augmented) stage_store_delayed(ctl, gi, dD);
+    }
     stage_update(ctl, gi, q1, dphi);
 }
 

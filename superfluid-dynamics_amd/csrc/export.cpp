@@ -45,7 +45,7 @@ int guarded(F&& f) {
 // adimensionalizeProperties (Export.cu:1222-1246): alpha = 3.5e-24 (constants.cuh:165),
 // rho_He = 150.
 struct Units {
-    double rho, kappa, depth, base_length, base_time;
+    double rho, kappa, depth, base_length, base_time, base_energy;
 };
 Units adimensionalize(double L, double rho, double kappa, double depth) {
     const double alpha = 3.5e-24, rho_he = 150.0;
@@ -53,6 +53,7 @@ Units adimensionalize(double L, double rho, double kappa, double depth) {
     u.base_length = L / (2.0 * sfh::kPi);
     const double base_acc = 3 * alpha / std::pow(depth, 4);
     u.base_time = std::sqrt(u.base_length / base_acc);
+    u.base_energy = 3.0 * rho_he * alpha * std::pow(u.base_length, 4) / std::pow(depth, 4);
     const double stf = rho_he * u.base_length * u.base_length * u.base_length / (u.base_time * u.base_time);
     u.kappa = kappa / stf;
     u.depth = depth / u.base_length;
@@ -297,6 +298,171 @@ int integrateSimulationRK4_freeMemory(double* statesOut, double* timesOut) {
     return 0;
 }
 
+int integrateAugmentedOptomechanicalSimulationRK4(double* initialState, double** statesOut, size_t* statesCount,
+                                                  double** timesOut, size_t* timesCount, SimProperties* simProperties,
+                                                  RK4SolverOptions* rkOptions, COptomechanicalVariables* ov, size_t N) {
+    if (!initialState || !statesOut || !statesCount || !timesOut || !timesCount || !simProperties || !rkOptions ||
+        !ov) {
+        g_last_error = "null pointer argument";
+        return -3;
+    }
+    if (N < 32 || N > 8192 || (N & (N - 1)) != 0) {   // the sizes Export.cu:1076-1093 dispatches
+        g_last_error = "unsupported particle number " + std::to_string(N);
+        return -2;
+    }
+    return guarded([&] {
+        const SimProperties sp = *simProperties;
+        const Units u = adimensionalize(sp.L, sp.rho, sp.kappa, sp.depth);
+        const double t0 = rkOptions->t0 / u.base_time;
+        const double t1 = rkOptions->t1 / u.base_time;
+        const double dt = rkOptions->timeStep / u.base_time;
+        if (!std::isfinite(t0) || !std::isfinite(t1) || !std::isfinite(dt))
+            throw sfh::ArgError("t0, t1 and the time step must be finite");
+        if (!(dt > 0.0)) throw sfh::ArgError("time step must be positive");
+        if (t1 < t0) throw sfh::ArgError("t1 must not precede t0");
+        const double nsteps = std::floor((t1 - t0) / dt);
+        if (!(nsteps < 2.0e9)) throw sfh::ArgError("too many time steps");
+        const size_t steps = (size_t)nsteps;
+        if (steps > SIZE_MAX / (sizeof(double) * 4 * N) - 1) throw sfh::ArgError("trajectory too large to allocate");
+        // adimensionalizeOptomechanicalVariables (Export.cu:1248-1270) and
+        // get_current_intensity_drive_strength (LightIntensity.cuh:31-34), with the adimensional rho
+        const double hbar = 1.054571817e-34;   // constants.cuh:10
+        sfh::Phys d{};
+        d.drive = 1;
+        d.augmented = 1;
+        d.gamma = ov->gamma * u.base_time;
+        d.detuning = ov->detuning * u.base_time;
+        d.G = ov->G * u.base_time * u.base_length;
+        d.tau = ov->tau / u.base_time;
+        d.x0 = ov->location_x0_mode / u.base_length;
+        d.sigma = ov->sigma_optical_mode / u.base_length;
+        d.max_intensity = ov->max_intensity;
+        const double hbar_adim = hbar / u.base_energy / u.base_time;
+        d.beta = ov->beta * (hbar_adim * d.G / d.tau / (std::pow(d.sigma, 2.0) * u.rho));
+        d.damping = ov->damping_strength;
+        d.strength = hbar / (u.base_energy * u.base_time * u.rho) * d.G / std::pow(d.sigma, 2.0);
+        sfh::Options opt;
+        Solver s(helium_phys(u, sp.infinite_depth, sp.use_expansions, sp.expansion_order),
+                 std::vector<int>(1, (int)N), opt);
+        s.set_drive(d);
+        const double* x = initialState;
+        s.set_state(x, x + N, x + 2 * N);
+        s.set_delayed(x + 3 * N);
+        double* states = (double*)std::malloc(sizeof(double) * 4 * N * (steps ? steps : 1));
+        double* times = (double*)std::malloc(sizeof(double) * (steps ? steps : 1));
+        if (!states || !times) {
+            std::free(states);
+            std::free(times);
+            throw std::runtime_error("host allocation failed");
+        }
+        try {
+            for (size_t done = 0; done < steps;) {
+                const int chunk = (int)std::min<size_t>(steps - done, 1u << 30);
+                s.step_record(dt, chunk, 1, states + 4 * N * done, times + done);
+                done += (size_t)chunk;
+            }
+        } catch (...) {
+            std::free(states);
+            std::free(times);
+            throw;
+        }
+        double t = t0;   // runEvolution's currentTime, logged after each step
+        for (size_t k = 0; k < steps; ++k) {
+            t += dt;
+            times[k] = t;
+            // Export.cu:1057 writes the phi entry into the fourth block
+            std::memcpy(states + 4 * N * k + 3 * N, states + 4 * N * k + 2 * N, N * sizeof(double));
+        }
+        *statesOut = states;
+        *statesCount = steps;
+        *timesOut = times;
+        *timesCount = steps;
+    });
+}
+
+int calculateJacobian(const double* state, double* jac, double L, double rho, double kappa, double depth,
+                      double epsilon, size_t N) {
+    if (!state || !jac) {
+        g_last_error = "null pointer argument";
+        return -3;
+    }
+    if (N < 32 || N > 2048 || (N & (N - 1)) != 0) {   // the sizes Export.cu:495-518 dispatches
+        g_last_error = "unsupported particle number " + std::to_string(N);
+        return -2;
+    }
+    return guarded([&] {
+        const Units u = adimensionalize(L, rho, kappa, depth);
+        sfh::fd_jacobian(helium_phys(u, false, false, 1), (int)N, state, epsilon, jac);
+    });
+}
+
+int integrateSimulationGL2(double* initialState, double** statesOut, size_t* statesCount, double** timesOut,
+                           size_t* timesCount, SimProperties* simProperties, GaussLegendreOptions* go, size_t N) {
+    if (!initialState || !statesOut || !statesCount || !timesOut || !timesCount || !simProperties || !go) {
+        g_last_error = "null pointer argument";
+        return -3;
+    }
+    if (N < 32 || N > 1024 || (N & (N - 1)) != 0) {   // the sizes Export.cu:705-725 dispatches
+        g_last_error = "unsupported particle number " + std::to_string(N);
+        return -2;
+    }
+    return guarded([&] {
+        const SimProperties sp = *simProperties;
+        const Units u = adimensionalize(sp.L, sp.rho, sp.kappa, sp.depth);
+        sfh::Gl2Options o;
+        o.tol = go->newtonTolerance;
+        o.max_newton = (int)std::min<size_t>(go->maxNewtonIterations, 1000);
+        o.max_halves = (int)std::min<size_t>(go->maxStepsHalves, 60);
+        o.simplified_fallback = go->allowSimplifiedFallback;
+        o.armijo_c = go->armijo_c;
+        o.backtrack = go->backtrack;
+        o.min_alpha = go->minAlpha;
+        if (!(o.backtrack > 0.0 && o.backtrack < 1.0) || !(o.min_alpha > 0.0))
+            throw sfh::ArgError("backtrack must lie in (0, 1) and minAlpha be positive");
+        std::vector<double> s(initialState, initialState + 3 * N), traj, times;
+        const bool want = go->returnTrajectory;
+        sfh::gl2_integrate(helium_phys(u, sp.infinite_depth, sp.use_expansions, sp.expansion_order), (int)N, s.data(),
+                           go->t0, go->t1, go->stepSize, o, want ? &traj : nullptr, want ? &times : nullptr);
+        *statesOut = nullptr;
+        *timesOut = nullptr;
+        *statesCount = *timesCount = 0;
+        if (want) {
+            double* st = (double*)std::malloc(traj.size() * sizeof(double));
+            double* tm = (double*)std::malloc(times.size() * sizeof(double));
+            if (!st || !tm) {
+                std::free(st);
+                std::free(tm);
+                throw std::runtime_error("host allocation failed");
+            }
+            std::memcpy(st, traj.data(), traj.size() * sizeof(double));
+            std::memcpy(tm, times.data(), times.size() * sizeof(double));
+            *statesOut = st;
+            *timesOut = tm;
+            *statesCount = *timesCount = times.size();
+        }
+    });
+}
+
+int integrateSimulationGL2_freeMemory(double* statesOut, double* timesOut) {
+    std::free(statesOut);
+    std::free(timesOut);
+    return 0;
+}
+
+int sfh_jacobian(const sfh_props* props, int n, const double* state, double eps, double* jac) {
+    if (!props || !state || !jac) {
+        g_last_error = "null pointer argument";
+        return -3;
+    }
+    return guarded([&] { sfh::fd_jacobian(phys_from(*props), n, state, eps, jac); });
+}
+
+int integrateAugmentedOptomechanicalSimulationRK4_freeMemory(double* statesOut, double* timesOut) {
+    std::free(statesOut);
+    std::free(timesOut);
+    return 0;
+}
+
 void sfh_default_options(sfh_options* opt) {
     if (!opt) return;
     std::memset(opt, 0, sizeof(*opt));
@@ -335,9 +501,35 @@ int sfh_create(const sfh_props* props, const int* n, int n_surfaces, const sfh_o
             o.restart = opt->restart;
             o.force_trim = opt->force_trim;
             o.tol = opt->tol;
+            o.pair_rows = opt->pair_rows;
+        }
+        const std::vector<int> sizes(n, n + n_surfaces);
+        const sfh::Phys ph = phys_from(*props);
+        if (o.solver == SFH_SOLVE_AUTO) {
+            const bool mixed = o.precision == SFH_MIXED;
+            int mx = 0;
+            bool uniform = true;
+            for (int v : sizes) {
+                mx = std::max(mx, v);
+                uniform = uniform && v == sizes[0];
+            }
+            if (uniform) {
+                const sfh::Tuning t = sfh::tuned(ph, mx, mixed);
+                o.solver = t.solver;
+                o.precision = t.precision;
+                o.pair_rows = t.pair_rows;
+            } else if (mx <= sfh::kBatchMaxN) {
+                o.solver = SFH_SOLVE_BATCHED;
+            } else {
+                const sfh::Tuning t = sfh::tuned(ph, mx, mixed);
+                o.solver = SFH_SOLVE_GMRES;
+                o.pair_rows = t.pair_rows;
+            }
+            if (o.solver == SFH_SOLVE_BATCHED) o.max_iter = std::min(o.max_iter, 16);
+            if (o.solver != SFH_SOLVE_LU) o.max_iter = std::min(o.max_iter, sfh::kGmMaxIter);
         }
         auto h = std::make_unique<sfh_solver>();
-        h->s = std::make_unique<Solver>(phys_from(*props), std::vector<int>(n, n + n_surfaces), o);
+        h->s = std::make_unique<Solver>(ph, sizes, o);
         *out = h.release();
     });
 }
@@ -418,7 +610,68 @@ int sfh_get_stats(sfh_handle h, sfh_stats* out) {
         out->refinements = st.refinements;
         out->redone_steps = st.redone_steps;
         out->sequence_length = st.sequence_length;
+        out->solver = h->s->solver();
+        out->precision = h->s->precision();
+        out->pair_rows = h->s->pair_rows();
     });
+}
+
+static void tuning_out(const sfh::Tuning& t, sfh_tuning* out) {
+    out->solver = t.solver;
+    out->precision = t.precision;
+    out->pair_rows = t.pair_rows;
+    out->from_cache = t.from_cache;
+    out->ms_per_step = t.ms_per_step;
+}
+
+int sfh_autotune(const sfh_props* props, int n, int allow_mixed, int steps, sfh_tuning* out) {
+    if (!props || !out) return -3;
+    return guarded([&] { tuning_out(sfh::autotune(phys_from(*props), n, allow_mixed != 0, steps), out); });
+}
+
+int sfh_tuned(const sfh_props* props, int n, int allow_mixed, sfh_tuning* out) {
+    if (!props || !out) return -3;
+    return guarded([&] { tuning_out(sfh::tuned(phys_from(*props), n, allow_mixed != 0), out); });
+}
+
+int sfh_set_drive(sfh_handle h, const sfh_drive* dr) {
+    SFH_NEED(h);
+    return guarded([&] {
+        sfh::Phys d{};
+        if (dr) {
+            d.drive = 1;
+            d.augmented = dr->augmented ? 1 : 0;
+            d.damping = dr->damping;
+            d.strength = dr->strength;
+            d.detuning = dr->detuning;
+            d.gamma = dr->gamma;
+            d.G = dr->G;
+            d.max_intensity = dr->max_intensity;
+            d.x0 = dr->x0;
+            d.sigma = dr->sigma;
+            d.beta = dr->beta;
+            d.tau = dr->tau;
+        }
+        h->s->set_drive(d);
+    });
+}
+
+int sfh_set_delayed(sfh_handle h, const double* D) {
+    SFH_NEED(h);
+    if (!D) return -3;
+    return guarded([&] { h->s->set_delayed(D); });
+}
+
+int sfh_get_delayed(sfh_handle h, double* D) {
+    SFH_NEED(h);
+    if (!D) return -3;
+    return guarded([&] { h->s->get_delayed(D); });
+}
+
+int sfh_get_delayed_rhs(sfh_handle h, double* dD) {
+    SFH_NEED(h);
+    if (!dD) return -3;
+    return guarded([&] { h->s->get_delayed_rhs(dD); });
 }
 
 int sfh_set_state_device(sfh_handle h, const void* state) {

@@ -37,6 +37,11 @@ struct Phys {
     int rho_quirk;      // water Bernoulli reads V1[1] (createM.cuh:104)
     int need_upper;     // water with rho != 0 needs the upper-fluid velocity
     double rho, kappa, depth, U, g;
+    // optical drive (HeliumDrivenAutonomousProblem, AugmentedBoundaryIntegrator), nondimensional:
+    // dPhi/dt += damping v + strength I(y, x) [+ D], dD/dt = beta I - D / tau when augmented
+    int drive;
+    int augmented;
+    double damping, strength, detuning, gamma, G, max_intensity, x0, sigma, beta, tau;
 };
 
 // ------------------------------------------------------------------ complex helpers

@@ -23,6 +23,12 @@ struct StageCtl {
     double2* qup;      // optional upper-fluid velocity output (may be null)
     const double* Pin; // phi of the stage's input state (read only when energy is set)
     double* energy;    // optional: 4 per node, the energy and flux terms of node g at 4 g
+    // augmented optical drive: the delayed intensity D of the stage input, of y0 (updated in place
+    // by stage 4), the stage output and the stage slopes (null without the augmented drive)
+    const double* Din;
+    double* D0;
+    double* Dout;
+    double* kD[3];
 };
 
 // Warm start of a solve by extrapolation over time steps: each RK stage keeps its solutions of the
@@ -76,7 +82,7 @@ struct GmArgs {
     int stage;
     int check_first;     // refinement solve: every launch checks for a finished solve before its matrix loads
 };
-constexpr int kPairRows = 4;        // rows per workgroup of the N^2 pair kernels (assembly, velocity)
+constexpr int kPairRows = 4;        // default rows per workgroup of the N^2 pair kernels (assembly, velocity)
 // ---- spectral.hip
 // circulant derivative products: rows per workgroup (wave w owns rows w and w + 4), columns
 // staged in LDS per round, and the kernel window one round needs
@@ -110,18 +116,25 @@ __global__ void plain_deriv_kernel(const SurfDesc* sd, const double2* tw_all, co
                                    double2* out, int second, double scale, int lds_tw);
 
 // ---- assemble.hip
+// R = rows per workgroup: 1, 2 or 4 (instantiated in assemble.hip)
+template <int R>
 __global__ void assemble64_kernel(const SurfDesc* sd, const NodeTrig* trig, const double2* Z,
                                   const double2* Zp, const double2* Zpp, double* M, Phys ph,
                                   const int* only_if, GmArgs gm, int gm_on);
+template <int R>
 __global__ void assemble32_kernel(const SurfDesc* sd, const NodeTrig* trig, const double2* Z,
                                   const double2* Zp, const double2* Zpp, float* M, Phys ph, GmArgs gm);
+template <int R>
 __global__ void residual64_kernel(const SurfDesc* sd, const NodeTrig* trig, const double2* Z,
                                   const double2* Zp, const double2* Zpp, Phys ph, GmArgs gm);
 
 // ---- velocity.hip
+// R = rows per workgroup: 1, 2 or 4 (instantiated in velocity.hip)
+template <int R>
 __global__ void velocity_stage_kernel(const SurfDesc* sd, const NodeTrig* trig, const double2* Z,
                                       const double2* Zp, const double2* Zpp, const double* mu,
                                       const double2* mup, Phys ph, StageCtl ctl);
+template <int R>
 __global__ void velocity_gm1_kernel(const SurfDesc* sd, const NodeTrig* trig, const double2* Z,
                                     const double2* Zp, const double2* Zpp, double2* mup, Phys ph,
                                     StageCtl ctl, GmArgs gm, const double* circ);
@@ -150,7 +163,7 @@ __global__ void gm_iter_kernel(GmArgs a, const SurfDesc* sd, int j);
 __global__ void gm_close_mu_kernel(GmArgs a, const SurfDesc* sd, const double* circ, double2* mup, int i);
 
 // ---- solver.hip: one trajectory frame [x(T), y(T), phi(T)] of the state
-__global__ void record_frame_kernel(const double2* Z, const double* P, double* dst, int total);
+__global__ void record_frame_kernel(const double2* Z, const double* P, const double* D, double* dst, int total);
 // state in the reference layout [Z(T), Phi(T) + 0i] <-> Z, P; and the right-hand side out
 __global__ void state_split_kernel(const double2* state, double2* Z, double* P, int total);
 __global__ void state_join_kernel(const double2* Z, const double* P, double2* state, int total);

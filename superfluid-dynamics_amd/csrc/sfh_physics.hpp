@@ -57,6 +57,33 @@ __device__ inline void velocity_from_sums(const Phys& ph, double2 zp, double2 zp
     q2 = cconj(csub(base, cscale(half_inv, mk)));
 }
 
+// LightIntensity::compute_intensity (LightIntensity.cuh:16-29)
+__device__ inline double intensity(const Phys& ph, double y, double x) {
+    const double df = ph.detuning - ph.G * y;
+    const double dx = x - ph.x0;
+    const double hg = ph.gamma / 2.0;
+    return 0.25 * (ph.gamma * ph.gamma) * ph.max_intensity / (df * df + hg * hg) *
+           exp(-(dx * dx) / (2.0 * (ph.sigma * ph.sigma)));
+}
+
+// Optical drive on the Bernoulli right-hand side of node g, in the reference's order:
+// add_optical_field_drive_terms_no_time_depence (createM.cuh:138-149: damping v, then strength I),
+// then add_delayed_intensity_phi_rhs (createM.cuh:151-159: + D) and the delayed intensity's own
+// slope calculate_intensity_delayed_rhs (createM.cuh:161-169: beta I - D / tau).
+__device__ inline double drive_terms(const Phys& ph, const StageCtl& ctl, int g, double x, double y, double2 q1,
+                                     double dphi, double& dD) {
+    const double inten = intensity(ph, y, x);
+    dphi += ph.damping * q1.y;
+    dphi += ph.strength * inten;
+    dD = 0.0;
+    if (ph.augmented) {
+        const double D = ctl.Din[g];
+        dphi += D;
+        dD = ph.beta * inten - 1.0 / ph.tau * D;
+    }
+    return dphi;
+}
+
 // RK4 stage bookkeeping of node g (AutonomousRungeKuttaStepper.cuh:338-375, utilities.cuh:78-83),
 // in two halves so that a kernel can load a node's stage inputs long before it has the node's
 // velocity: stage_load reads what the stage combines, stage_store writes the stage's results.
@@ -79,6 +106,18 @@ __device__ inline StageIn stage_load(const StageCtl& ctl, int g) {
         s.p3 = ctl.kP[2][g];
     }
     return s;
+}
+// The delayed intensity D of the augmented drive takes the same stage combination as phi.
+__device__ inline void stage_store_delayed(const StageCtl& ctl, int g, double dD) {
+    if (ctl.stage == 0) {
+        ctl.kD[0][g] = dD;
+    } else if (ctl.stage <= 3) {
+        ctl.kD[ctl.stage - 1][g] = dD;
+        ctl.Dout[g] = ctl.D0[g] + ctl.alpha * dD;
+    } else {
+        const double K = ctl.kD[0][g] + 2.0 * ctl.kD[1][g] + 2.0 * ctl.kD[2][g] + dD;
+        ctl.D0[g] = ctl.D0[g] + ctl.alpha * K;
+    }
 }
 __device__ inline void stage_store(const StageCtl& ctl, int g, double2 q1, double dphi, const StageIn& s) {
     if (ctl.stage == 0) {

@@ -27,6 +27,15 @@
 
 namespace sfh {
 
+// Launches a kernel templated on the rows per workgroup R of the pair kernels (1, 2 or 4) with the
+// run-time choice rows_: the statement sees R as a constant.
+#define SFH_WITH_ROWS(rows_, ...)              \
+    switch (rows_) {                           \
+    case 1: { constexpr int R = 1; __VA_ARGS__; } break; \
+    case 2: { constexpr int R = 2; __VA_ARGS__; } break; \
+    default: { constexpr int R = 4; __VA_ARGS__; } break; \
+    }
+
 // Steps that run every solve with the full launch cap: kHistDepth while the warm-start history
 // fills, then one with the full history that sets the trimmed length.
 constexpr int kCalibSteps = kHistDepth + 1;
@@ -120,6 +129,9 @@ Solver::Solver(const Phys& ph, const std::vector<int>& n, const Options& opt) : 
                            " Arnoldi steps per solve (max_iter), without restart");
     }
     if (opt_.force_trim < 0) throw ArgError("negative force_trim");
+    if (opt_.pair_rows != 0 && opt_.pair_rows != 1 && opt_.pair_rows != 2 && opt_.pair_rows != 4)
+        throw ArgError("pair_rows must be 1, 2 or 4");
+    rows_ = opt_.pair_rows ? opt_.pair_rows : kPairRows;
     if (opt_.solver == kSolveBatched)
         for (int v : n)
             if (v > kBatchMaxN) throw ArgError("the batched solver takes surfaces of at most 1024 nodes");
@@ -266,7 +278,7 @@ Solver::~Solver() {
                     kP_[1], kP_[2], Zp_, Zpp_, mup_, qup_, b_, work_, y_, mu_, M_, uinv_, flags_,
                     status_, piv_, counters_, gmV_, gmw_, gmpart_, gmsmall_, gmctl_, trig_, order_, bstats_,
                     gmhist_, gmhvalid_, bhist_, bhvalid_, d_circ_, gmx0_, M32_, ckZ_, ckP_, ckMu_, ckHist_,
-                    ckHv_, energy_};
+                    ckHv_, energy_, D0_, Da_, Db_, ckD_, kD_[0], kD_[1], kD_[2]};
     for (void* p : bufs)
         if (p) (void)hipFree(p);
     void* vbufs[] = {vY_[0], vY_[1], vY_[2], vK_[0], vK_[1], vK_[2], vYP_[0], vYP_[1], vYP_[2], vKP_[0],
@@ -439,8 +451,8 @@ bool Solver::enqueue_solve(const double2* Z, int stage, bool g1) {
         double nsq = 0.0;
         for (int v : n_) nsq += (double)v * v;
         launch("residual", (ph_.image ? 2.0 : 1.0) * nsq, 16.0 * T_, [&] {
-            go(residual64_kernel, dim3((maxN_ + kPairRows - 1) / kPairRows, S_), dim3(256), 0,
-                               d_desc_, (const NodeTrig*)trig_, Z, Zp_, Zpp_, ph_, refine);
+            SFH_WITH_ROWS(rows_, go(residual64_kernel<R>, dim3((maxN_ + R - 1) / R, S_), dim3(256), 0, d_desc_,
+                                    (const NodeTrig*)trig_, Z, Zp_, Zpp_, ph_, refine));
         });
         refine.check_first = 1;
         return enqueue_gmres(refine);
@@ -482,8 +494,8 @@ bool Solver::enqueue_solve(const double2* Z, int stage, bool g1) {
         for (int s = 0; s < S_; ++s) run(s, 1, n_[s]);
     }
     // systems whose interchange-free check failed: re-assemble and solve with GEPP
-    go(assemble64_kernel, dim3((maxN_ + kPairRows - 1) / kPairRows, S_), dim3(256), 0, d_desc_,
-                       (const NodeTrig*)trig_, Z, Zp_, Zpp_, M_, ph_, (const int*)status_, GmArgs{}, 0);
+    go(assemble64_kernel<kPairRows>, dim3((maxN_ + kPairRows - 1) / kPairRows, S_), dim3(256), 0, d_desc_,
+       (const NodeTrig*)trig_, Z, Zp_, Zpp_, M_, ph_, (const int*)status_, GmArgs{}, 0);
     go(gepp_fallback_kernel, dim3(S_), dim3(256), 0, d_desc_, M_, b_, mu_, piv_,
                        status_, counters_);
     return false;
@@ -547,11 +559,13 @@ void Solver::enqueue_rhs(const double2* Z, const double* P, const StageCtl& ctl_
     // assembly: flops counted as cot evaluations (the transcendental-bound unit), bytes = M written
     if (gm_on && opt_.precision == kMixed) {
         launch("assemble", img * nsq, 4.0 * nsq, [&] {
-            go(assemble32_kernel, dim3((maxN_ + kPairRows - 1) / kPairRows, S_), dim3(256), 0, d_desc_, (const NodeTrig*)trig_, Z, Zp_, Zpp_, M32_, ph_, ga);
+            SFH_WITH_ROWS(rows_, go(assemble32_kernel<R>, dim3((maxN_ + R - 1) / R, S_), dim3(256), 0, d_desc_,
+                                    (const NodeTrig*)trig_, Z, Zp_, Zpp_, M32_, ph_, ga));
         });
     } else {
         launch("assemble", img * nsq, 8.0 * nsq, [&] {
-            go(assemble64_kernel, dim3((maxN_ + kPairRows - 1) / kPairRows, S_), dim3(256), 0, d_desc_, (const NodeTrig*)trig_, Z, Zp_, Zpp_, M_, ph_, (const int*)nullptr, ga, gm_on);
+            SFH_WITH_ROWS(rows_, go(assemble64_kernel<R>, dim3((maxN_ + R - 1) / R, S_), dim3(256), 0, d_desc_,
+                                    (const NodeTrig*)trig_, Z, Zp_, Zpp_, M_, ph_, (const int*)nullptr, ga, gm_on));
         });
     }
     // an FP64 GMRES sequence of two launches (one Arnoldi step) ends in the velocity kernel; not
@@ -564,16 +578,15 @@ void Solver::enqueue_rhs(const double2* Z, const double* P, const StageCtl& ctl_
                                (const double*)mu_, mup_);
         });
     }
-    const dim3 vgrid((maxN_ + kPairRows - 1) / kPairRows, S_);
     if (g1) {
         launch("velocity", img * nsq, 96.0 * T_, [&] {
-            go(velocity_gm1_kernel, vgrid, dim3(256), 0, d_desc_, (const NodeTrig*)trig_, Z,
-                               Zp_, Zpp_, mup_, ph_, ctl, ga, (const double*)d_circ_);
+            SFH_WITH_ROWS(rows_, go(velocity_gm1_kernel<R>, dim3((maxN_ + R - 1) / R, S_), dim3(256), 0, d_desc_,
+                                    (const NodeTrig*)trig_, Z, Zp_, Zpp_, mup_, ph_, ctl, ga, (const double*)d_circ_));
         });
     } else {
         launch("velocity", img * nsq, 96.0 * T_, [&] {
-            go(velocity_stage_kernel, vgrid, dim3(256), 0, d_desc_, (const NodeTrig*)trig_, Z,
-                               Zp_, Zpp_, mu_, mup_, ph_, ctl);
+            SFH_WITH_ROWS(rows_, go(velocity_stage_kernel<R>, dim3((maxN_ + R - 1) / R, S_), dim3(256), 0, d_desc_,
+                                    (const NodeTrig*)trig_, Z, Zp_, Zpp_, mu_, mup_, ph_, ctl));
         });
     }
     SFH_HIP(hipGetLastError());
@@ -590,13 +603,31 @@ void Solver::enqueue_step(double dt) {
     }
     c.qup = nullptr;
     c.stage = 1; c.alpha = half; c.Zout = Za_; c.Pout = Pa_;
+    set_delayed_ctl(c, D0_, Da_);
     enqueue_rhs(Z0_, P0_, c);
     c.stage = 2; c.alpha = half; c.Zout = Zb_; c.Pout = Pb_;
+    set_delayed_ctl(c, Da_, Db_);
     enqueue_rhs(Za_, Pa_, c);
     c.stage = 3; c.alpha = dt; c.Zout = Za_; c.Pout = Pa_;
+    set_delayed_ctl(c, Db_, Da_);
     enqueue_rhs(Zb_, Pb_, c);
     c.stage = 4; c.alpha = sixth; c.Zout = nullptr; c.Pout = nullptr;
+    set_delayed_ctl(c, Da_, nullptr);
     enqueue_rhs(Za_, Pa_, c);
+}
+
+// The delayed-intensity fields of a stage (null without the augmented drive).
+void Solver::set_delayed_ctl(StageCtl& c, const double* Din, double* Dout) const {
+    if (!ph_.augmented) {
+        c.Din = nullptr;
+        c.D0 = c.Dout = nullptr;
+        c.kD[0] = c.kD[1] = c.kD[2] = nullptr;
+        return;
+    }
+    c.Din = Din;
+    c.D0 = D0_;
+    c.Dout = Dout;
+    for (int i = 0; i < 3; ++i) c.kD[i] = kD_[i];
 }
 
 void Solver::capture(double dt, int len) {
@@ -649,6 +680,7 @@ void Solver::save_checkpoint() {
     SFH_HIP(hipMemcpyAsync(ckHist_, gmhist_, 4 * kHistDepth * (size_t)T_ * sizeof(double),
                            hipMemcpyDeviceToDevice, stream_));
     SFH_HIP(hipMemcpyAsync(ckHv_, gmhvalid_, 4 * sizeof(int), hipMemcpyDeviceToDevice, stream_));
+    if (ph_.augmented) SFH_HIP(hipMemcpyAsync(ckD_, D0_, T_ * sizeof(double), hipMemcpyDeviceToDevice, stream_));
 }
 
 void Solver::restore_checkpoint() {
@@ -658,6 +690,7 @@ void Solver::restore_checkpoint() {
     SFH_HIP(hipMemcpyAsync(gmhist_, ckHist_, 4 * kHistDepth * (size_t)T_ * sizeof(double),
                            hipMemcpyDeviceToDevice, stream_));
     SFH_HIP(hipMemcpyAsync(gmhvalid_, ckHv_, 4 * sizeof(int), hipMemcpyDeviceToDevice, stream_));
+    if (ph_.augmented) SFH_HIP(hipMemcpyAsync(D0_, ckD_, T_ * sizeof(double), hipMemcpyDeviceToDevice, stream_));
 }
 
 void Solver::run_steps(double dt, int len, int count) {
@@ -709,30 +742,41 @@ void Solver::step(double dt, int nsteps) {
 }
 
 __global__ __launch_bounds__(256) void record_frame_kernel(const double2* __restrict__ Z, const double* __restrict__ P,
-                                                           double* __restrict__ dst, int total) {
+                                                           const double* __restrict__ D, double* __restrict__ dst,
+                                                           int total) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= total) return;
     const double2 z = Z[i];
     dst[i] = z.x;
     dst[total + i] = z.y;
     dst[2 * (long long)total + i] = P[i];
+    if (D) dst[3 * (long long)total + i] = D[i];
 }
 
 void Solver::step_record(double dt, int nsteps, int every, double* states, double* times) {
     if (nsteps < 0 || every < 1) throw ArgError("nsteps must be >= 0 and every >= 1");
     if (nsteps / every > 0 && (!states || !times)) throw ArgError("null output array");
-    const size_t frame = 3 * (size_t)T_;
+    const size_t frame = (ph_.augmented ? 4 : 3) * (size_t)T_;   // [x, y, phi(, D)]
+    if (rec_ring_ && rec_frame_ != frame) {   // the frame size changed (augmented drive): a new ring
+        SFH_HIP(hipStreamSynchronize(rec_stream_));
+        SFH_HIP(hipFree(rec_ring_));
+        SFH_HIP(hipHostFree(rec_host_));
+        rec_ring_ = nullptr;
+        rec_host_ = nullptr;
+    }
     if (!rec_ring_) {
+        rec_frame_ = frame;
         // ring depth: up to 32 frames, at most 64 MiB of frames
         rec_slots_ = (int)std::max<size_t>(2, std::min<size_t>(32, (64u << 20) / (frame * sizeof(double))));
         rec_ring_ = dalloc<double>(frame * rec_slots_);
         SFH_HIP(hipHostMalloc((void**)&rec_host_, frame * rec_slots_ * sizeof(double), hipHostMallocDefault));
-        SFH_HIP(hipStreamCreateWithFlags(&rec_stream_, hipStreamNonBlocking));
-        rec_packed_.resize(rec_slots_);
-        rec_copied_.resize(rec_slots_);
-        for (int k = 0; k < rec_slots_; ++k) {
-            SFH_HIP(hipEventCreateWithFlags(&rec_packed_[k], hipEventDisableTiming));
-            SFH_HIP(hipEventCreateWithFlags(&rec_copied_[k], hipEventDisableTiming));
+        if (!rec_stream_) SFH_HIP(hipStreamCreateWithFlags(&rec_stream_, hipStreamNonBlocking));
+        while ((int)rec_packed_.size() < rec_slots_) {
+            hipEvent_t a, b;
+            SFH_HIP(hipEventCreateWithFlags(&a, hipEventDisableTiming));
+            SFH_HIP(hipEventCreateWithFlags(&b, hipEventDisableTiming));
+            rec_packed_.push_back(a);
+            rec_copied_.push_back(b);
         }
     }
     const int frames = nsteps / every;
@@ -747,7 +791,8 @@ void Solver::step_record(double dt, int nsteps, int every, double* states, doubl
         const int slot = f % rec_slots_;
         if (f >= rec_slots_) drain(f - rec_slots_);   // the slot's previous frame is on the host
         double* dst = rec_ring_ + (size_t)slot * frame;
-        hipLaunchKernelGGL(record_frame_kernel, dim3((T_ + 255) / 256), dim3(256), 0, stream_, Z0_, P0_, dst, T_);
+        hipLaunchKernelGGL(record_frame_kernel, dim3((T_ + 255) / 256), dim3(256), 0, stream_, Z0_, P0_,
+                           (const double*)(ph_.augmented ? D0_ : nullptr), dst, T_);
         SFH_HIP(hipGetLastError());
         SFH_HIP(hipEventRecord(rec_packed_[slot], stream_));
         SFH_HIP(hipStreamWaitEvent(rec_stream_, rec_packed_[slot], 0));
@@ -796,6 +841,7 @@ void Solver::get_state_device(double2* state) {
 
 void Solver::rhs_device(const double2* state, double2* rhs) {
     if (!state || !rhs) throw ArgError("null state or rhs");
+    if (ph_.augmented) throw ArgError("rhs_device takes the two-field state; the augmented drive has three");
     // the stage buffers Za/Pa are free between steps
     hipLaunchKernelGGL(state_split_kernel, dim3((T_ + 255) / 256), dim3(256), 0, stream_, state, Za_, Pa_, T_);
     SFH_HIP(hipGetLastError());
@@ -819,6 +865,7 @@ void Solver::set_state(const double* x, const double* y, const double* phi) {
     if (gmhvalid_) SFH_HIP(hipMemsetAsync(gmhvalid_, 0, 4 * sizeof(int), stream_));   // a new trajectory
     hist_steps_ = 0;
     time_ = 0.0;
+    if (D0_) SFH_HIP(hipMemsetAsync(D0_, 0, T_ * sizeof(double), stream_));
     if (bhvalid_) SFH_HIP(hipMemsetAsync(bhvalid_, 0, 4 * S_ * sizeof(int), stream_));
     SFH_HIP(hipStreamSynchronize(stream_));
 }
@@ -842,6 +889,7 @@ void Solver::rhs(double* vx, double* vy, double* dphi, double* mu, double2* zp, 
     c.kZ[0] = kZ_[0];
     c.kP[0] = kP_[0];
     c.qup = qup ? qup_ : nullptr;
+    set_delayed_ctl(c, D0_, nullptr);
     enqueue_rhs(Z0_, P0_, c);
     std::vector<double2> q(T_);
     SFH_HIP(hipMemcpyAsync(q.data(), kZ_[0], T_ * sizeof(double2), hipMemcpyDeviceToHost, stream_));
@@ -857,6 +905,51 @@ void Solver::rhs(double* vx, double* vy, double* dphi, double* mu, double2* zp, 
     }
     stats_.rhs_evals += 1;
     check_status();
+}
+
+void Solver::set_drive(const Phys& d) {
+    if (d.augmented && !d.drive) throw ArgError("the augmented system needs the drive on");
+    if (d.drive && !(d.tau != 0.0)) throw ArgError("tau must be nonzero");
+    SFH_HIP(hipStreamSynchronize(stream_));
+    ph_.drive = d.drive;
+    ph_.augmented = d.augmented;
+    ph_.damping = d.damping;
+    ph_.strength = d.strength;
+    ph_.detuning = d.detuning;
+    ph_.gamma = d.gamma;
+    ph_.G = d.G;
+    ph_.max_intensity = d.max_intensity;
+    ph_.x0 = d.x0;
+    ph_.sigma = d.sigma;
+    ph_.beta = d.beta;
+    ph_.tau = d.tau;
+    if (ph_.augmented && !D0_) {
+        D0_ = dalloc<double>(T_);
+        Da_ = dalloc<double>(T_);
+        Db_ = dalloc<double>(T_);
+        ckD_ = dalloc<double>(T_);
+        for (int i = 0; i < 3; ++i) kD_[i] = dalloc<double>(T_);
+    }
+    if (D0_) SFH_HIP(hipMemset(D0_, 0, T_ * sizeof(double)));
+    graph_len_ = -1;   // the step graph holds the physics: re-capture
+}
+
+void Solver::set_delayed(const double* D) {
+    if (!ph_.augmented) throw ArgError("the handle has no augmented drive (sfh_set_drive)");
+    SFH_HIP(hipMemcpyAsync(D0_, D, T_ * sizeof(double), hipMemcpyHostToDevice, stream_));
+    SFH_HIP(hipStreamSynchronize(stream_));
+}
+
+void Solver::get_delayed(double* D) {
+    if (!ph_.augmented) throw ArgError("the handle has no augmented drive (sfh_set_drive)");
+    SFH_HIP(hipMemcpyAsync(D, D0_, T_ * sizeof(double), hipMemcpyDeviceToHost, stream_));
+    SFH_HIP(hipStreamSynchronize(stream_));
+}
+
+void Solver::get_delayed_rhs(double* dD) {
+    if (!ph_.augmented) throw ArgError("the handle has no augmented drive (sfh_set_drive)");
+    SFH_HIP(hipMemcpyAsync(dD, kD_[0], T_ * sizeof(double), hipMemcpyDeviceToHost, stream_));
+    SFH_HIP(hipStreamSynchronize(stream_));
 }
 
 void Solver::track_energies(bool on) {
@@ -1026,8 +1119,8 @@ void debug_lu_solve(int n, const double* A, const double* b, double* x, int* int
 // ------------------------------------------------------------------ reverse mode
 
 void Solver::ensure_vjp() {
-    if (ph_.kind != 0 || ph_.rho != 0.0 || ph_.image)
-        throw ArgError("the adjoint is implemented for water with rho = 0");
+    if (ph_.kind != 0 || ph_.rho != 0.0 || ph_.image || ph_.drive)
+        throw ArgError("the adjoint is implemented for water with rho = 0 (no drive)");
     if (maxN_ > kBatchMaxN) throw ArgError("the adjoint takes surfaces of at most 1024 nodes");
     if (vjp_ready_) return;
     for (int i = 0; i < 3; ++i) {

@@ -39,6 +39,7 @@ struct Options {
     int restart = 0;         // unused: the multi-workgroup GMRES does not restart, the batched one
                              // restarts every 8 vectors
     int force_trim = 0;      // test hook: trimmed GMRES sequence length instead of the calibrated one
+    int pair_rows = 0;       // rows per workgroup of the pair kernels: 1, 2 or 4 (0: 4)
     double tol = 1e-12;      // GMRES relative residual / refinement tolerance
 };
 
@@ -61,6 +62,34 @@ struct Stats {
 bool supported_n(int n);
 void debug_lu_solve(int n, const double* A, const double* b, double* x, int* interchanged);
 
+// Autotuner (autotune.cpp): the fastest (solver, precision, pair-kernel rows) for one surface of n
+// nodes of this physics on the current GPU, measured; tuned() returns the cached result or tunes.
+struct Tuning {
+    int solver = kSolveLU;
+    int precision = kFP64;
+    int pair_rows = kPairRows;
+    int from_cache = 0;
+    double ms_per_step = 0.0;
+};
+Tuning autotune(const Phys& ph, int n, bool allow_mixed, int steps);
+Tuning tuned(const Phys& ph, int n, bool allow_mixed);
+
+// Central-difference Jacobian of [vx, vy, dphi] with respect to the real state [x, y, phi] of one
+// surface of n <= 2048 nodes: jac[k 3n + r] = d f_r / d s_k (implicit.cpp).
+void fd_jacobian(const Phys& ph, int n, const double* state, double eps, double* jac);
+
+// Two-stage Gauss-Legendre integration of one surface (n <= 1024) from t0 to t1 (implicit.cpp).
+struct Gl2Options {
+    double tol = 1e-10;          // Newton: |R| <= tol (1 + |k|^2)
+    int max_newton = 20;
+    int max_halves = 6;
+    bool simplified_fallback = false;
+    double armijo_c = 1e-4, backtrack = 0.5, min_alpha = 1e-6;
+    double eps = 1e-6;           // Jacobian perturbation
+};
+int gl2_integrate(const Phys& ph, int n, double* state, double t0, double t1, double h, const Gl2Options& o,
+                  std::vector<double>* traj, std::vector<double>* times);
+
 class Solver {
   public:
     Solver(const Phys& ph, const std::vector<int>& n, const Options& opt);
@@ -79,6 +108,14 @@ class Solver {
     // rhs = [dZ/dt (T), dPhi/dt + 0i (T)] of `state`, the handle's own state left untouched; all
     // three are enqueued on the handle's stream and complete before the call returns.
     void set_state_device(const double2* state);
+    // Optical drive (HeliumDrivenAutonomousProblem, HeliumDrivenAutonomousProblem.cuh:19-25) and,
+    // with ph.augmented, the delayed intensity D as a third state field (AugmentedBoundaryIntegrator,
+    // AugmentedBoundaryIntegrator.cuh:32-36).  set_drive takes the drive fields of ph (the rest of
+    // ph is kept), zeroes D and re-captures the step graph.  D is zeroed by set_state too.
+    void set_drive(const Phys& drive);
+    void set_delayed(const double* D);
+    void get_delayed(double* D);
+    void get_delayed_rhs(double* dD);   // dD/dt of the last standalone right-hand side (rhs())
     void get_state_device(double2* state);
     void rhs_device(const double2* state, double2* rhs);
     void get_state(double* x, double* y, double* phi);
@@ -95,6 +132,10 @@ class Solver {
     void get_mu(double* mu);
     void sync();
     Stats stats();
+    // The form the handle runs: solver, precision and pair-kernel rows after defaults are applied.
+    int solver() const { return opt_.solver; }
+    int precision() const { return opt_.precision; }
+    int pair_rows() const { return rows_; }
 
     // Raw device buffers (state y0) for zero-copy interop.
     double2* device_Z() { return Z0_; }
@@ -160,6 +201,7 @@ class Solver {
 
     Phys ph_;
     Options opt_;
+    int rows_ = kPairRows;   // rows per workgroup of the pair kernels
     std::vector<int> n_;
     std::vector<SurfDesc> desc_;
     int S_ = 0, T_ = 0, maxN_ = 0;
@@ -246,9 +288,14 @@ class Solver {
     double* energy_ = nullptr;   // 4 doubles per node while energies are tracked
     bool track_energy_ = false;
     double time_ = 0.0;          // advanced by dt per step, 0 after set_state
+    // augmented drive: delayed intensity of y0, the stage states, the slopes, the checkpoint copy
+    double *D0_ = nullptr, *Da_ = nullptr, *Db_ = nullptr, *ckD_ = nullptr;
+    double* kD_[3] = {nullptr, nullptr, nullptr};
+    void set_delayed_ctl(StageCtl& c, const double* Din, double* Dout) const;
     // trajectory recording (allocated on first use): device ring of frames, pinned host staging,
     // copy stream and one event pair per slot
     int rec_slots_ = 0;
+    size_t rec_frame_ = 0;
     double* rec_ring_ = nullptr;
     double* rec_host_ = nullptr;
     hipStream_t rec_stream_ = nullptr;

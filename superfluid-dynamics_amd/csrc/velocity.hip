@@ -32,14 +32,13 @@
 
 namespace sfh {
 
-template <bool G1>
+template <bool G1, int R>
 __device__ inline void velocity_rows(const SurfDesc* __restrict__ sd, const NodeTrig* __restrict__ trig,
                                      const double2* __restrict__ Z, const double2* __restrict__ Zp,
                                      const double2* __restrict__ Zpp, const double* __restrict__ mu,
                                      const double2* __restrict__ mup, double2* __restrict__ mup_out,
                                      const Phys& ph, const StageCtl& ctl,
                                      const GmArgs& gm, const double* __restrict__ circ) {
-    constexpr int R = kPairRows;
     static_assert(R <= kCircRowsPerWG, "the rows of a workgroup fit the circulant window");
     const int s = blockIdx.y;
     const SurfDesc d = sd[s];
@@ -306,12 +305,18 @@ __device__ inline void velocity_rows(const SurfDesc* __restrict__ sd, const Node
     }
     if (ctl.qup) ctl.qup[g] = q2;
     if (!ph.need_upper) q2 = make_double2(0.0, 0.0);
-    const double dphi = bernoulli(ph, y_t, q1, q2, q1q, zp_t, zpp_t);
-    // energy terms (tracked on request only: their inputs are loaded here, not held across the sweep)
+    double dphi = bernoulli(ph, y_t, q1, q2, q1q, zp_t, zpp_t);
+    // energy and drive terms (on request only: their inputs are loaded here, not held across the sweep)
     if (ctl.energy) energy_store(ph, ctl, g, z[k].x, y_t, ctl.Pin[g], zp_t, q1);
+    if (ph.drive) {
+        double dD;
+        dphi = drive_terms(ph, ctl, g, z[k].x, y_t, q1, dphi, dD);
+        if (ph.augmented) stage_store_delayed(ctl, g, dD);
+    }
     stage_store(ctl, g, q1, dphi, st);
 }
 
+template <int R>
 __global__ __launch_bounds__(256) void velocity_stage_kernel(const SurfDesc* __restrict__ sd,
                                                              const NodeTrig* __restrict__ trig,
                                                              const double2* __restrict__ Z,
@@ -320,7 +325,7 @@ __global__ __launch_bounds__(256) void velocity_stage_kernel(const SurfDesc* __r
                                                              const double* __restrict__ mu,
                                                              const double2* __restrict__ mup,
                                                              Phys ph, StageCtl ctl) {
-    velocity_rows<false>(sd, trig, Z, Zp, Zpp, mu, mup, nullptr, ph, ctl, GmArgs{}, nullptr);
+    velocity_rows<false, R>(sd, trig, Z, Zp, Zpp, mu, mup, nullptr, ph, ctl, GmArgs{}, nullptr);
 }
 
 // Finishes a GMRES solve of one Arnoldi step (launch 0 of gm_iter_kernel ran its product), forms
@@ -328,11 +333,23 @@ __global__ __launch_bounds__(256) void velocity_stage_kernel(const SurfDesc* __r
 // row outside the workgroup's.  mup receives mu' and gm.sol mu.
 // Two waves per SIMD (all 512 workgroups of an N = 2048 surface resident at once): the compiler
 // keeps the kernel within 256 VGPRs, spilling 2.
+template <int R>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void velocity_gm1_kernel(
     const SurfDesc* __restrict__ sd, const NodeTrig* __restrict__ trig, const double2* __restrict__ Z,
     const double2* __restrict__ Zp, const double2* __restrict__ Zpp, double2* __restrict__ mup, Phys ph,
     StageCtl ctl, GmArgs gm, const double* __restrict__ circ) {
-    velocity_rows<true>(sd, trig, Z, Zp, Zpp, nullptr, nullptr, mup, ph, ctl, gm, circ);
+    velocity_rows<true, R>(sd, trig, Z, Zp, Zpp, nullptr, nullptr, mup, ph, ctl, gm, circ);
 }
+
+// rows per workgroup the solver may launch with (chosen per surface size by the autotuner)
+#define SFH_VELOCITY_INSTANCES(R)                                                                                  \
+    template __global__ void velocity_stage_kernel<R>(const SurfDesc*, const NodeTrig*, const double2*,                \
+                                                      const double2*, const double2*, const double*, const double2*, \
+                                                      Phys, StageCtl);                                              \
+    template __global__ void velocity_gm1_kernel<R>(const SurfDesc*, const NodeTrig*, const double2*, const double2*, \
+                                                    const double2*, double2*, Phys, StageCtl, GmArgs, const double*);
+SFH_VELOCITY_INSTANCES(1)
+SFH_VELOCITY_INSTANCES(2)
+SFH_VELOCITY_INSTANCES(4)
 
 }  // namespace sfh

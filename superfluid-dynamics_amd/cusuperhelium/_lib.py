@@ -38,14 +38,43 @@ class SfhProps(Structure):
 
 class SfhOptions(Structure):
     _fields_ = [("precision", c_int), ("solver", c_int), ("use_graph", c_int), ("max_iter", c_int),
-                ("restart", c_int), ("force_trim", c_int), ("tol", c_double)]
+                ("restart", c_int), ("force_trim", c_int), ("tol", c_double), ("pair_rows", c_int),
+                ("reserved", c_int)]
+
+
+class SfhTuning(Structure):
+    _fields_ = [("solver", c_int), ("precision", c_int), ("pair_rows", c_int), ("from_cache", c_int),
+                ("ms_per_step", c_double)]
+
+
+class SfhDrive(Structure):
+    _fields_ = [("augmented", c_int), ("reserved", c_int)] + [
+        (k, c_double) for k in ("damping", "strength", "detuning", "gamma", "G", "max_intensity", "x0", "sigma",
+                                "beta", "tau")]
+
+
+class COptomechanicalVariables(Structure):
+    """COptomechanicalVariables (ExportTypes.cuh:41-64), SI units."""
+
+    _fields_ = [(k, c_double) for k in ("detuning", "gamma", "G", "tau", "max_intensity", "initial_time",
+                                        "location_x0_mode", "sigma_optical_mode", "beta", "damping_strength")]
+
+
+class CGaussLegendreOptions(Structure):
+    """GaussLegendreOptions (ExportTypes.cuh:21-34)."""
+
+    _fields_ = [("t0", c_double), ("t1", c_double), ("stepSize", c_double), ("newtonTolerance", c_double),
+                ("maxNewtonIterations", c_size_t), ("allowSimplifiedFallback", c_bool),
+                ("returnTrajectory", c_bool), ("armijo_c", c_double), ("backtrack", c_double),
+                ("minAlpha", c_double), ("maxStepsHalves", c_size_t)]
 
 
 class SfhStats(Structure):
     _fields_ = [("rhs_evals", c_longlong), ("steps", c_longlong), ("fallback_solves", c_int),
                 ("spin_timeouts", c_int), ("last_iterations", c_int), ("max_iterations", c_int),
                 ("unconverged", c_int), ("refinements", c_int), ("redone_steps", c_int),
-                ("sequence_length", c_int)]
+                ("sequence_length", c_int), ("solver", c_int), ("precision", c_int), ("pair_rows", c_int),
+                ("reserved", c_int)]
 
 
 DP = POINTER(c_double)
@@ -63,6 +92,22 @@ SIGNATURES = {
     "integrateSimulationRK4": (c_int, [DP, POINTER(DP), POINTER(c_size_t), POINTER(DP), POINTER(c_size_t),
                                        POINTER(CSimulationProperties), POINTER(CRK4Options), c_size_t]),
     "integrateSimulationRK4_freeMemory": (c_int, [DP, DP]),
+    "integrateAugmentedOptomechanicalSimulationRK4": (c_int, [DP, POINTER(DP), POINTER(c_size_t), POINTER(DP),
+                                                              POINTER(c_size_t), POINTER(CSimulationProperties),
+                                                              POINTER(CRK4Options), POINTER(COptomechanicalVariables),
+                                                              c_size_t]),
+    "integrateAugmentedOptomechanicalSimulationRK4_freeMemory": (c_int, [DP, DP]),
+    "calculateJacobian": (c_int, [DP, DP, c_double, c_double, c_double, c_double, c_double, c_size_t]),
+    "integrateSimulationGL2": (c_int, [DP, POINTER(DP), POINTER(c_size_t), POINTER(DP), POINTER(c_size_t),
+                                       POINTER(CSimulationProperties), POINTER(CGaussLegendreOptions), c_size_t]),
+    "integrateSimulationGL2_freeMemory": (c_int, [DP, DP]),
+    "sfh_jacobian": (c_int, [POINTER(SfhProps), c_int, DP, c_double, DP]),
+    "sfh_autotune": (c_int, [POINTER(SfhProps), c_int, c_int, c_int, POINTER(SfhTuning)]),
+    "sfh_tuned": (c_int, [POINTER(SfhProps), c_int, c_int, POINTER(SfhTuning)]),
+    "sfh_set_drive": (c_int, [c_void_p, POINTER(SfhDrive)]),
+    "sfh_set_delayed": (c_int, [c_void_p, DP]),
+    "sfh_get_delayed": (c_int, [c_void_p, DP]),
+    "sfh_get_delayed_rhs": (c_int, [c_void_p, DP]),
     "sfh_default_options": (None, [POINTER(SfhOptions)]),
     "sfh_default_props": (None, [POINTER(SfhProps)]),
     "sfh_create": (c_int, [POINTER(SfhProps), POINTER(c_int), c_int, POINTER(SfhOptions), POINTER(c_void_p)]),

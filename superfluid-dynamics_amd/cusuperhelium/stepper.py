@@ -14,11 +14,11 @@ from typing import Sequence
 
 import numpy as np
 
-from ._lib import SfhOptions, SfhProps, SfhStats, check, lib
+from ._lib import SfhDrive, SfhOptions, SfhProps, SfhStats, SfhTuning, check, lib
 
 WATER, HELIUM = 0, 1
 FP64, MIXED = 0, 1
-LU, GMRES, BATCHED = 0, 1, 2
+LU, GMRES, BATCHED, AUTO = 0, 1, 2, 3
 
 
 def _dp(a):
@@ -49,7 +49,7 @@ class Props:
 class Ensemble:
     def __init__(self, sizes: Sequence[int] | int, props: Props | None = None, precision: int = FP64,
                  solver: int = LU, use_graph: bool = True, tol: float = 1e-12, max_iter: int = 32,
-                 force_trim: int = 0):
+                 force_trim: int = 0, pair_rows: int = 0):
         self.sizes = [int(sizes)] if np.isscalar(sizes) else [int(n) for n in sizes]
         self.props = props or Props()
         self._lib = lib()
@@ -57,12 +57,14 @@ class Ensemble:
         self._lib.sfh_default_options(byref(opt))
         opt.precision, opt.solver, opt.use_graph = int(precision), int(solver), int(use_graph)
         opt.tol, opt.max_iter, opt.force_trim = float(tol), int(max_iter), int(force_trim)
+        opt.pair_rows = int(pair_rows)
         arr = (c_int * len(self.sizes))(*self.sizes)
         h = c_void_p()
         cp = self.props.c()
         check(self._lib.sfh_create(byref(cp), arr, len(self.sizes), byref(opt), byref(h)), "sfh_create")
         self._h = h
         self.total = int(sum(self.sizes))
+        self.augmented = False
         self.offsets = np.concatenate([[0], np.cumsum(self.sizes)]).astype(np.int64)
 
     def close(self):
@@ -119,17 +121,49 @@ class Ensemble:
     def step(self, dt: float, nsteps: int = 1) -> None:
         check(self._lib.sfh_step(self._h, float(dt), int(nsteps)), "sfh_step")
 
+    # ---- optical drive (HeliumDrivenAutonomousProblem / AugmentedBoundaryIntegrator)
+    def set_drive(self, augmented: bool = False, **kw) -> None:
+        """Turns the optical drive on (keywords of sfh_drive: damping, strength, detuning, gamma, G,
+        max_intensity, x0, sigma, beta, tau), or off with set_drive(None)."""
+        if augmented is None:
+            check(self._lib.sfh_set_drive(self._h, None), "sfh_set_drive")
+            self.augmented = False
+            return
+        d = SfhDrive()
+        d.augmented = int(bool(augmented))
+        d.gamma, d.G, d.sigma, d.tau = 1.0, 1.0, 1.0, 1.0
+        for k, v in kw.items():
+            setattr(d, k, float(v))
+        check(self._lib.sfh_set_drive(self._h, byref(d)), "sfh_set_drive")
+        self.augmented = bool(augmented)
+
+    def set_delayed(self, D) -> None:
+        D = np.ascontiguousarray(D, dtype=np.float64).ravel()
+        check(self._lib.sfh_set_delayed(self._h, _dp(D)), "sfh_set_delayed")
+
+    def delayed(self) -> np.ndarray:
+        out = np.empty(self.total)
+        check(self._lib.sfh_get_delayed(self._h, _dp(out)), "sfh_get_delayed")
+        return out
+
+    def delayed_rhs(self) -> np.ndarray:
+        out = np.empty(self.total)
+        check(self._lib.sfh_get_delayed_rhs(self._h, _dp(out)), "sfh_get_delayed_rhs")
+        return out
+
     def step_record(self, dt: float, nsteps: int, every: int = 1):
         """Advances nsteps steps and returns the states after every `every`-th one:
         (Z [frames, total] complex, phi [frames, total], times [frames])."""
         frames = int(nsteps) // int(every)
-        st = np.empty((max(frames, 1), 3 * self.total))
+        fields = 4 if getattr(self, "augmented", False) else 3
+        st = np.empty((max(frames, 1), fields * self.total))
         tm = np.empty(max(frames, 1))
         check(self._lib.sfh_step_record(self._h, float(dt), int(nsteps), int(every), _dp(st), _dp(tm)),
               "sfh_step_record")
         st, tm = st[:frames], tm[:frames]
         T = self.total
-        return st[:, :T] + 1j * st[:, T:2 * T], st[:, 2 * T:], tm
+        out = (st[:, :T] + 1j * st[:, T:2 * T], st[:, 2 * T:3 * T], tm)
+        return out + (st[:, 3 * T:],) if fields == 4 else out
 
     # ---- reverse mode (water, rho = 0, every N <= 1024)
     def rhs_vjp(self, gq, gdphi):
@@ -191,3 +225,24 @@ class Ensemble:
         z, p = c_void_p(), c_void_p()
         check(self._lib.sfh_device_state(self._h, byref(z), byref(p)), "sfh_device_state")
         return z.value, p.value
+
+
+def _tuning(t: SfhTuning) -> dict:
+    return {"solver": t.solver, "precision": t.precision, "pair_rows": t.pair_rows, "from_cache": bool(t.from_cache),
+            "ms_per_step": t.ms_per_step}
+
+
+def autotune(n: int, props: Props | None = None, allow_mixed: bool = False, steps: int = 10) -> dict:
+    """Times the candidate forms for one surface of n nodes and caches the fastest (sfh_autotune)."""
+    t = SfhTuning()
+    cp = (props or Props()).c()
+    check(lib().sfh_autotune(byref(cp), int(n), int(bool(allow_mixed)), int(steps), byref(t)), "sfh_autotune")
+    return _tuning(t)
+
+
+def tuned(n: int, props: Props | None = None, allow_mixed: bool = False) -> dict:
+    """The cached autotuner result for one surface of n nodes, tuning on a miss (sfh_tuned)."""
+    t = SfhTuning()
+    cp = (props or Props()).c()
+    check(lib().sfh_tuned(byref(cp), int(n), int(bool(allow_mixed)), byref(t)), "sfh_tuned")
+    return _tuning(t)

@@ -1,13 +1,15 @@
 """Writes the long-horizon golden fixtures of the GPU parity tests by running the CPU FP64 oracle
 (oracle/libsfo.so, OpenMP) on the seeded workloads of cusuperhelium.workloads.
 
-    python tests/golden/make_golden.py [c1] [c2] [c3] [c3long] [c5] [mu]
+    python tests/golden/make_golden.py [c1] [c2] [c3] [c3long] [c5] [gl2] [mu]
 
   c1      N = 256 water surface y = 0.1 cos x, dt = 0.01: (Z, phi) after 100 RK4 steps
   c2      N = 2048 water surface, dt = 0.002: (Z, phi) after 100, 500 and 1000 RK4 steps
   c3      all 512 surfaces of the uneven ensemble: (Z, phi) after 20 steps
   c3long  two surfaces of each N in {128, 256, 512, 1024}: (Z, phi) after 100 and 500 steps
   c5      N = 4096 helium film (h0 = 1, image term, g_eff = 0.15): (Z, phi) after 2, 10 and 20 steps
+  gl2     N = 32 helium film of depth 1 (the SI film L = 2 pi 1e-7 m, depth 1e-7 m): two-stage
+          Gauss-Legendre frames [x, y, phi] of steps h = 0.05 to t = 0.2 (Newton tol 1e-12)
 
 Each fixture is an uncompressed .npz of float64 / complex128 arrays (no pickles) beside this
 script; `mu` adds to every fixture the oracle's dipole strengths mu of the right-hand side at each
@@ -91,6 +93,17 @@ def make_c5():
              Z=np.array([cp[m][0] for m in marks]), phi=np.array([cp[m][1] for m in marks]))
 
 
+def make_gl2():
+    n = 32
+    rng = np.random.default_rng(2)
+    x = 2 * np.pi * np.arange(n) / n
+    y = sum(rng.uniform(0, 0.03 / k) * np.cos(k * x + rng.uniform(0, 2 * np.pi)) for k in range(1, 6))
+    s0 = np.concatenate([x, y, 0.01 * np.sin(x + 0.4)])
+    d = O.adimensionalize(2 * np.pi * 1e-7, 150.0, 0.0, 1e-7)
+    traj, times = O.gl2(O.Props(kind=O.HELIUM, depth=d["depth"], rho=d["rho"]), s0, 0.0, 0.2, 0.05, tol=1e-12)
+    np.savez(os.path.join(HERE, "gl2_helium_n32.npz"), s0=s0, depth=np.array(d["depth"]), frames=traj, times=times)
+
+
 FIXTURES = {"c1_water_n256.npz": None, "c2_water_n2048.npz": None, "c3_ensemble_512_20steps.npz": None,
             "c3_sample_500steps.npz": None, "c5_film_n4096.npz": W.C5_PROPS}
 
@@ -121,7 +134,8 @@ def add_mu():
         np.savez(path, **d)
 
 
-MAKERS = {"c1": make_c1, "c2": make_c2, "c3": make_c3, "c3long": make_c3long, "c5": make_c5, "mu": add_mu}
+MAKERS = {"c1": make_c1, "c2": make_c2, "c3": make_c3, "c3long": make_c3long, "c5": make_c5, "gl2": make_gl2,
+          "mu": add_mu}
 
 if __name__ == "__main__":
     O.build()

@@ -37,7 +37,10 @@ def W():
 
 
 def _golden(name):
-    with np.load(os.path.join(GOLDEN, name)) as f:
+    path = os.path.join(GOLDEN, name)
+    if not os.path.exists(path):
+        pytest.skip(f"fixture {name} not generated (tests/golden/make_golden.py)")
+    with np.load(path) as f:
         return {k: f[k] for k in f.files}
 
 

@@ -338,6 +338,118 @@ def test_helium_surface_tension_branch(oracle):
     assert np.abs((r1 - 0.2 * curv) - 20.447761896665416 * plain).max() < 4e-15
 
 
+# ------------------------------------------------------------------ optical drive (f3)
+def test_intensity_closed_form(oracle):
+    # LightIntensity::compute_intensity (LightIntensity.cuh:16-29): a Lorentzian in the detuning
+    # shifted by G y times a Gaussian in x; at resonance (G y = detuning) and x = x0 it is Imax,
+    # half width gamma / 2, and a Gaussian of width sigma
+    d = oracle.Drive(detuning=0.6, gamma=0.4, G=3.0, max_intensity=2.5, x0=1.0, sigma=0.3)
+    assert abs(oracle.intensity(d, 0.2, 1.0) - 2.5) < 1e-15
+    assert abs(oracle.intensity(d, 0.2 + 0.2 / 3.0, 1.0) - 1.25) < 1e-15
+    assert abs(oracle.intensity(d, 0.2, 1.3) - 2.5 * np.exp(-0.5)) < 1e-15
+
+
+def test_augmented_flat_film_closed_form(oracle):
+    # AugmentedBoundaryIntegrator (AugmentedBoundaryIntegrator.cuh:32-36): on a flat film at rest
+    # under a uniform intensity I0 (sigma huge), dD/dt = beta I0 - D / tau and dphi/dt = D, so
+    # D = beta I0 tau (1 - e^{-t/tau}), phi = beta I0 tau (t - tau (1 - e^{-t/tau})), and a uniform phi
+    # drives no flow: the surface stays flat.
+    n = 32
+    x = 2 * np.pi * np.arange(n) / n
+    d = oracle.Drive(augmented=True, detuning=0.3, gamma=1.0, G=2.0, max_intensity=0.5, sigma=1e8, beta=0.7, tau=2.0)
+    I0 = 0.25 * 0.5 / (0.09 + 0.25)
+    Z, P, D = oracle.rk4_driven(oracle.Props(kind=oracle.HELIUM, depth=1.0), d, x + 0j, np.zeros(n), np.zeros(n),
+                                0.01, 300)
+    t, b = 3.0, 0.7 * I0 * 2.0
+    assert np.abs(D - b * (1 - np.exp(-t / 2))).max() < 1e-11
+    assert np.abs(P - b * (t - 2 * (1 - np.exp(-t / 2)))).max() < 1e-11
+    assert np.abs(Z.imag).max() < 1e-14
+
+
+def _mode1(Z):
+    n = Z.size
+    return 2 * np.real(np.fft.fft(Z.imag)[1]) / n
+
+
+def test_damping_term_linear_growth_rate(oracle):
+    # add_optical_field_drive_terms_no_time_depence (createM.cuh:138-149): dphi/dt += c v.  For a
+    # small deep mode k = 1 (restoring force -y): y'' - c k y' + k y = 0, so from y = a cos x at rest
+    # y_1(t) = a e^{c t/2} (cos w t - (c / 2w) sin w t), w = sqrt(1 - c^2/4).
+    n, a, c, dt, steps = 64, 1e-6, 0.2, np.pi / 200, 400
+    x = 2 * np.pi * np.arange(n) / n
+    d = oracle.Drive(damping=c)
+    Z, _, _ = oracle.rk4_driven(oracle.Props(kind=oracle.HELIUM, depth=1.0, infinite_depth=True), d,
+                                x + 1j * a * np.cos(x), np.zeros(n), None, dt, steps)
+    t = dt * steps
+    w = np.sqrt(1 - c * c / 4)
+    expect = a * np.exp(c * t / 2) * (np.cos(w * t) - c / (2 * w) * np.sin(w * t))
+    assert abs(_mode1(Z) - expect) < 1e-3 * a
+    # the sign matters: without the term the amplitude is a cos t
+    assert abs(expect - a * np.cos(t)) > 0.1 * a
+
+
+def test_intensity_drive_shifts_linear_frequency(oracle):
+    # strength * I(y): linearised, S I(y) = S I0 + S I'(0) y, so the restoring force becomes
+    # -(1 - S I'(0)) y and the k = 1 mode oscillates at w = sqrt(1 - S I'(0)) (a uniform phi adds no flow).
+    n, a, dt, steps = 64, 1e-6, np.pi / 200, 400
+    x = 2 * np.pi * np.arange(n) / n
+    d = oracle.Drive(strength=0.4, detuning=0.5, gamma=1.0, G=2.0, max_intensity=1.0, sigma=1e8)
+    dI = 0.25 * 1.0 * 1.0 * 2 * 2.0 * 0.5 / (0.25 + 0.25) ** 2   # dI/dy at y = 0
+    Z, _, _ = oracle.rk4_driven(oracle.Props(kind=oracle.HELIUM, depth=1.0, infinite_depth=True), d,
+                                x + 1j * a * np.cos(x), np.zeros(n), None, dt, steps)
+    t = dt * steps
+    w = np.sqrt(1 - 0.4 * dI)
+    assert abs(_mode1(Z) - a * np.cos(w * t)) < 1e-3 * a
+    assert abs(a * np.cos(w * t) - a * np.cos(t)) > 0.1 * a
+
+
+# ------------------------------------------------------------------ FD Jacobian and GL2 (f4)
+@pytest.mark.parametrize("m", [1, 3])
+def test_jacobian_flat_rest_state_closed_form(oracle, m):
+    # JacobianCalculator (JacobianCalculator.cuh:223-284) at a flat surface at rest: the flow of a
+    # potential perturbation eps sin(m x) is q = eps m (cos m x + i sin m x) and a height perturbation
+    # only enters Bernoulli (-y), so J [0, 0, sin m x] = [m cos, m sin, 0], J [0, cos m x, 0] = [0, 0, -cos]
+    n = 32
+    x = 2 * np.pi * np.arange(n) / n
+    J = oracle.jacobian(oracle.Props(), np.concatenate([x, np.zeros(n), np.zeros(n)]))
+    Jv = J @ np.concatenate([np.zeros(2 * n), np.sin(m * x)])
+    assert np.abs(Jv[:n] - m * np.cos(m * x)).max() < 1e-12
+    assert np.abs(Jv[n:2 * n] - m * np.sin(m * x)).max() < 1e-12
+    assert np.abs(Jv[2 * n:]).max() < 1e-12
+    Jv = J @ np.concatenate([np.zeros(n), np.cos(m * x), np.zeros(n)])
+    assert np.abs(Jv[2 * n:] + np.cos(m * x)).max() < 1e-12 and np.abs(Jv[:2 * n]).max() < 1e-12
+
+
+def test_gl2_rotation_is_pade_map(oracle):
+    # two-stage Gauss-Legendre (GLCoefficients.hpp) maps dz/dt = i z by the (2,2) Pade approximant
+    # R(w) = (1 + w/2 + w^2/12) / (1 - w/2 + w^2/12) of e^w, w = i h
+    y0 = np.array([1.0, 0.3, 0.0, -0.5])
+    h, steps = 0.1, 20
+    y = oracle.gl2_rotation(y0, h * steps, h)
+    w = 1j * h
+    R = (1 + w / 2 + w * w / 12) / (1 - w / 2 + w * w / 12)
+    z = R ** steps * (y0[:2] + 1j * y0[2:])
+    assert np.abs(y[:2] + 1j * y[2:] - z).max() < 1e-12
+    assert abs(abs(R) - 1) < 1e-15   # A-stable and norm-preserving on the imaginary axis
+
+
+def test_gl2_is_fourth_order_on_the_surface(oracle):
+    # Gauss-Legendre 2 on the boundary-integral right-hand side: the error against a fine RK4 run
+    # falls by 2^4 when the step halves
+    n = 16
+    x = 2 * np.pi * np.arange(n) / n
+    Z0 = x + 1j * 0.05 * np.cos(x)
+    s0 = np.concatenate([x, 0.05 * np.cos(x), np.zeros(n)])
+    Zr, Pr = oracle.rk4(oracle.Props(), Z0, np.zeros(n), 0.4 / 256, 256)
+    ref = np.concatenate([Zr.real, Zr.imag, Pr])
+    err = []
+    for h in (0.1, 0.05):
+        traj, t = oracle.gl2(oracle.Props(), s0, 0.0, 0.4, h, tol=1e-13)
+        assert abs(t[-1] - 0.4) < 1e-14
+        err.append(np.abs(traj[-1] - ref).max())
+    assert 12 < err[0] / err[1] < 20
+
+
 # ------------------------------------------------------------------ energies (f2)
 def _energies(oracle, props, Z, Phi):
     q, _, d = oracle.rhs(props, Z, Phi, diagnostics=True)
