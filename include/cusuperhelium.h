@@ -355,6 +355,21 @@ int sfh_device_state(sfh_handle h, void** Z, void** phi);
 int sfh_kernel_timing(sfh_handle h, int enable);
 int sfh_kernel_report(sfh_handle h, const char* tag, int* launches, double* ms, double* flops, double* bytes);
 
+/* Right-hand side of ONE surface without any device allocation.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  Computes what sfh_rhs returns for a fresh
+ * handle of that surface created with SFH_SOLVE_BATCHED, SFH_FP64, tol and max_iter (zero initial
+ * guess), from DEVICE arrays x, y, phi (n entries) to DEVICE arrays vx, vy, dphi and, if non-null,
+ * mu, enqueued on `stream` (a hipStream_t; NULL for the default stream) without waiting for it.
+ * Limits: n = 2^k or 3 * 2^k with 1 <= n <= 1024 (-2 otherwise).  Every intermediate (the n x n
+ * FP64 matrix, derivatives, solve vectors, twiddles) lives in a static device array of
+ * sfh_rhs_noalloc_scratch_bytes() bytes (about 8.5 MiB), one per GPU, held until the process
+ * ends.  All calls on one GPU share it: calls that overlap on a GPU race.  Returns 0, -2, -3 or -1
+ * (launch error). */
+int sfh_rhs_noalloc(const sfh_props* props, int n, const double* x, const double* y, const double* phi, double* vx,
+                    double* vy, double* dphi, double* mu, double tol, int max_iter, void* stream);
+size_t sfh_rhs_noalloc_scratch_bytes(void);
+
 /* Message of the last failed call on this thread. */
 const char* sfh_last_error(void);
 

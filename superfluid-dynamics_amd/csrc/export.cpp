@@ -616,6 +616,20 @@ int sfh_get_stats(sfh_handle h, sfh_stats* out) {
     });
 }
 
+int sfh_rhs_noalloc(const sfh_props* props, int n, const double* x, const double* y, const double* phi, double* vx,
+                    double* vy, double* dphi, double* mu, double tol, int max_iter, void* stream) {
+    if (!props) {
+        g_last_error = "null props";
+        return -3;
+    }
+    return guarded([&] {
+        sfh::rhs_noalloc(phys_from(*props), n, x, y, phi, vx, vy, dphi, mu, tol, max_iter,
+                         static_cast<hipStream_t>(stream));
+    });
+}
+
+size_t sfh_rhs_noalloc_scratch_bytes(void) { return sfh::noalloc_scratch_bytes(); }
+
 static void tuning_out(const sfh::Tuning& t, sfh_tuning* out) {
     out->solver = t.solver;
     out->precision = t.precision;

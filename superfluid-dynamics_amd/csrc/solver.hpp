@@ -89,6 +89,11 @@ struct Gl2Options {
 };
 int gl2_integrate(const Phys& ph, int n, double* state, double t0, double t1, double h, const Gl2Options& o,
                   std::vector<double>* traj, std::vector<double>* times);
+// Right-hand side of one surface (n <= 1024) from device arrays to device arrays on `stream`, every
+// intermediate in a static device array (noalloc.hip): no device memory is allocated.
+void rhs_noalloc(const Phys& ph, int n, const double* x, const double* y, const double* phi, double* vx,
+                 double* vy, double* dphi, double* mu, double tol, int max_iter, hipStream_t stream);
+size_t noalloc_scratch_bytes();
 
 class Solver {
   public:

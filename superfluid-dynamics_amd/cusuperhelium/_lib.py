@@ -130,6 +130,9 @@ SIGNATURES = {
     "sfh_get_stats": (c_int, [c_void_p, POINTER(SfhStats)]),
     "sfh_device_state": (c_int, [c_void_p, POINTER(c_void_p), POINTER(c_void_p)]),
     "sfh_last_error": (ctypes.c_char_p, []),
+    "sfh_rhs_noalloc": (c_int, [POINTER(SfhProps), c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                c_void_p, c_double, c_int, c_void_p]),
+    "sfh_rhs_noalloc_scratch_bytes": (c_size_t, []),
     "sfh_debug_lu_solve": (c_int, [c_int, DP, DP, DP, POINTER(c_int)]),
     "sfh_kernel_timing": (c_int, [c_void_p, c_int]),
     "sfh_kernel_report": (c_int, [c_void_p, ctypes.c_char_p, POINTER(c_int), DP, DP, DP]),
