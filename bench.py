@@ -57,6 +57,14 @@ DT, SEED = W.DT, W.SEED
 surface = W.surface
 
 
+_T0 = time.perf_counter()
+
+
+def note(msg):
+    """Progress on stderr (stdout carries only the result line)."""
+    print(f"[bench {time.perf_counter() - _T0:7.1f} s] {msg}", file=sys.stderr, flush=True)
+
+
 def golden(name):
     """Golden oracle states (tests/golden/*.npz): steps, Z, phi, mu[, sizes, index]."""
     with np.load(os.path.join(GOLDEN, name)) as f:
@@ -382,6 +390,7 @@ def main():
     value = args.steps / el
     ms_per_step = 1e3 * el / args.steps
     st_main = e.stats()
+    note(f"headline: {value:.1f} steps/s")
 
     # live kernel timing: a few steps with a hipEvent pair on every launch (its start and end)
     e.kernel_timing(True)
@@ -440,6 +449,7 @@ def main():
         with c.Ensemble([N], c.Props(), precision=c.FP64, solver=solver, tol=1e-12, max_iter=args.max_iter) as e:
             e.set_state(Z0, P0)
             out["accuracy"] = trajectory_errors(e, g, DT)
+        note("headline accuracy done")
     if not args.no_lu and args.solver == "gmres":
         out["c2_lu"] = c2_lu_leg(c, args.lu_steps)
         if acc:
@@ -450,14 +460,19 @@ def main():
                 Z, P = e.get_state()
             out["c2_lu"]["accuracy"] = {"steps": int(g["steps"][0]), "max_abs_dz": float(np.abs(Z - g["Z"][0]).max()),
                                         "max_abs_dphi": float(np.abs(P - g["phi"][0]).max())}
+        note("C2 LU leg done")
     if not args.no_batched:
         out["batched"] = batched_leg(c, args.batched_steps, acc)
+        note("batched leg done")
     if not args.no_c1:
         out["c1"] = c1_leg(c, acc)
+        note("C1 leg done")
     if not args.no_film:
         out["film"] = film_leg(c, args.film_steps, acc)
+        note("film leg done")
     if not args.no_adjoint:
         out["adjoint"] = adjoint_leg(c)
+        note("adjoint leg done")
     if not args.no_cpu:
         steps, cel, Zc, Pc, threads = cpu_baseline(Z0, P0)
         out["cpu_baseline"] = {"value": round(steps / cel, 4), "unit": "RK4 steps/s", "cores": threads,

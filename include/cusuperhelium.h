@@ -10,7 +10,10 @@
  *    (adimensionalizeProperties, Export.cu:1222-1246).  Like the reference they build the
  *    helium-film problem (HeliumBoundaryProblem).  Return codes: 0 success, -1 runtime/device
  *    error, -2 unsupported size (the reference falls off the end of the function here),
- *    -3 invalid argument.  No entry point ever calls exit().
+ *    -3 invalid argument (a NaN or infinite input state included), -4 a right-hand side came out
+ *    NaN or infinite (coincident nodes make the dipole matrix singular, a trajectory blew up).
+ *    No entry point ever calls exit().  The reference returns NaN states in these cases
+ *    (ValueLogger.h:55-58 throws only when logging).
  *
  * 2. The native handle API (sfh_*): run-time sizes, many surfaces of different N per handle,
  *    nondimensional parameters, problem kind (water or helium film), state kept in HBM across
@@ -267,7 +270,11 @@ int sfh_get_state(sfh_handle h, double* x, double* y, double* phi);
  * (upper-fluid velocity, interleaved) are optional outputs (may be NULL). */
 int sfh_rhs(sfh_handle h, double* vx, double* vy, double* dphi, double* mu, double* zp, double* zpp,
             double* qup);
-/* Advances every surface by nsteps classical RK4 steps of size dt, in place. */
+/* Advances every surface by nsteps classical RK4 steps of size dt, in place.  The call may return
+ * before the steps have run on the device; a stage slope that came out NaN or infinite is then
+ * reported (-4) by the next call that waits for the device (sfh_get_state, sfh_rhs, sfh_get_stats,
+ * sfh_sync, ...) until sfh_set_state gives the handle a new state.  sfh_set_state rejects a NaN or
+ * infinite state (-3). */
 int sfh_step(sfh_handle h, double dt, int nsteps);
 
 /* Central-difference Jacobian of the right-hand side of one surface (nondimensional problem, n <= 2048,

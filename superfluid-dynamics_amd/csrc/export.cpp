@@ -33,6 +33,9 @@ int guarded(F&& f) {
     } catch (const sfh::ArgError& e) {
         g_last_error = e.what();
         return std::string(e.what()).rfind("unsupported node count", 0) == 0 ? -2 : -3;
+    } catch (const sfh::NonFiniteError& e) {
+        g_last_error = e.what();
+        return -4;
     } catch (const std::exception& e) {
         g_last_error = e.what();
         return -1;

@@ -29,6 +29,7 @@ struct StageCtl {
     double* D0;
     double* Dout;
     double* kD[3];
+    int* nonfinite;    // optional: set to 1 when a stage slope of any node is NaN or infinite
 };
 
 // Warm start of a solve by extrapolation over time steps: each RK stage keeps its solutions of the

@@ -120,6 +120,8 @@ __device__ inline void stage_store_delayed(const StageCtl& ctl, int g, double dD
     }
 }
 __device__ inline void stage_store(const StageCtl& ctl, int g, double2 q1, double dphi, const StageIn& s) {
+    // every writer stores the same value, so the unordered plain store is enough
+    if (ctl.nonfinite && !(isfinite(q1.x) && isfinite(q1.y) && isfinite(dphi))) *ctl.nonfinite = 1;
     if (ctl.stage == 0) {
         ctl.kZ[0][g] = q1;
         ctl.kP[0][g] = dphi;

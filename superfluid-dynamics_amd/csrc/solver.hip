@@ -540,6 +540,7 @@ void Solver::enqueue_rhs(const double2* Z, const double* P, const StageCtl& ctl_
     StageCtl ctl = ctl_in;
     ctl.Pin = P;
     ctl.energy = track_energy_ ? energy_ : nullptr;
+    ctl.nonfinite = counters_ + 1;
     if (opt_.solver == kSolveBatched) {
         enqueue_batched(Z, P, ctl);
         return;
@@ -821,10 +822,27 @@ __global__ __launch_bounds__(256) void state_join_kernel(const double2* __restri
     state[total + i] = make_double2(P[i], 0.0);
 }
 
+// A new trajectory: the non-finite flag and every solver vector a solve may read before writing it
+// (the last mu, the initial guess, the GMRES basis, products, partial sums and small state) start
+// from zero, so that the same state gives the same results whatever the handle ran before, and a
+// trajectory that blew up leaves nothing non-finite behind.  Not part of any captured graph.
+void Solver::clear_solver_state() {
+    SFH_HIP(hipMemsetAsync(counters_ + 1, 0, sizeof(int), stream_));
+    SFH_HIP(hipMemsetAsync(mu_, 0, T_ * sizeof(double), stream_));
+    if (opt_.solver == kSolveGMRES) {
+        SFH_HIP(hipMemsetAsync(gmx0_, 0, T_ * sizeof(double), stream_));
+        SFH_HIP(hipMemsetAsync(gmV_, 0, (size_t)(gm_m_ + 1) * T_ * sizeof(double), stream_));
+        SFH_HIP(hipMemsetAsync(gmw_, 0, 2 * (size_t)T_ * sizeof(double), stream_));
+        SFH_HIP(hipMemsetAsync(gmpart_, 0, gm_part_stride_ * S_ * sizeof(double), stream_));
+        SFH_HIP(hipMemsetAsync(gmsmall_, 0, gm_small_stride_ * S_ * sizeof(double), stream_));
+    }
+}
+
 void Solver::set_state_device(const double2* state) {
     if (!state) throw ArgError("null state");
     hipLaunchKernelGGL(state_split_kernel, dim3((T_ + 255) / 256), dim3(256), 0, stream_, state, Z0_, P0_, T_);
     SFH_HIP(hipGetLastError());
+    clear_solver_state();
     if (gmhvalid_) SFH_HIP(hipMemsetAsync(gmhvalid_, 0, 4 * sizeof(int), stream_));   // a new trajectory
     if (bhvalid_) SFH_HIP(hipMemsetAsync(bhvalid_, 0, 4 * S_ * sizeof(int), stream_));
     hist_steps_ = 0;
@@ -857,9 +875,42 @@ void Solver::rhs_device(const double2* state, double2* rhs) {
     sync();
 }
 
+// Two nodes of one surface at the same point (x mod 2 pi, y) make cot((z_k - z_j)/2) infinite and
+// the dipole matrix singular.  Rounded in FMA the pair terms come out huge but finite rather than
+// NaN, so the case is caught here, by sorting each surface's nodes.
+static void check_distinct_nodes(const double* x, const double* y, const std::vector<int>& sizes) {
+    std::vector<std::pair<double, double>> pts;
+    std::vector<int> idx;
+    int off = 0;
+    for (size_t s = 0; s < sizes.size(); ++s) {
+        const int n = sizes[s];
+        pts.resize(n);
+        idx.resize(n);
+        for (int i = 0; i < n; ++i) {
+            double xr = std::fmod(x[off + i], 2.0 * kPi);
+            if (xr < 0.0) xr += 2.0 * kPi;
+            if (xr >= 2.0 * kPi) xr = 0.0;
+            pts[i] = {xr, y[off + i]};
+            idx[i] = i;
+        }
+        std::sort(idx.begin(), idx.end(), [&](int a, int b) { return pts[a] < pts[b]; });
+        for (int i = 1; i < n; ++i)
+            if (pts[idx[i]] == pts[idx[i - 1]])
+                throw ArgError("coincident nodes " + std::to_string(std::min(idx[i], idx[i - 1])) + " and " +
+                               std::to_string(std::max(idx[i], idx[i - 1])) + " of surface " + std::to_string(s) +
+                               ": the dipole matrix is singular");
+        off += n;
+    }
+}
+
 void Solver::set_state(const double* x, const double* y, const double* phi) {
+    for (int i = 0; i < T_; ++i)
+        if (!std::isfinite(x[i]) || !std::isfinite(y[i]) || !std::isfinite(phi[i]))
+            throw ArgError("non-finite state at node " + std::to_string(i));
+    check_distinct_nodes(x, y, n_);
     std::vector<double2> z(T_);
     for (int i = 0; i < T_; ++i) z[i] = make_double2(x[i], y[i]);
+    clear_solver_state();
     SFH_HIP(hipMemcpyAsync(Z0_, z.data(), T_ * sizeof(double2), hipMemcpyHostToDevice, stream_));
     SFH_HIP(hipMemcpyAsync(P0_, phi, T_ * sizeof(double), hipMemcpyHostToDevice, stream_));
     if (gmhvalid_) SFH_HIP(hipMemsetAsync(gmhvalid_, 0, 4 * sizeof(int), stream_));   // a new trajectory
@@ -1034,6 +1085,8 @@ void Solver::check_status() {
         stats_.refinements = bs[2];
     }
     if (timeouts) throw HipError("back-substitution wait timed out; results are invalid");
+    if (cnt[1])
+        throw NonFiniteError("non-finite right-hand side (NaN or infinite state, or coincident nodes); set a new state");
 }
 
 Stats Solver::stats() {

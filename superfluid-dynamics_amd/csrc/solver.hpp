@@ -27,6 +27,11 @@ struct HipError : std::runtime_error {
 struct ArgError : std::runtime_error {
     using std::runtime_error::runtime_error;
 };
+// A right-hand side came out NaN or infinite (a NaN state given on the device, coincident nodes,
+// a blown-up trajectory); the handle's state is then invalid until the next set_state.
+struct NonFiniteError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
 
 enum Precision { kFP64 = 0, kMixed = 1 };
 enum SolveKind { kSolveLU = 0, kSolveGMRES = 1, kSolveBatched = 2 };
@@ -137,6 +142,7 @@ class Solver {
     void get_mu(double* mu);
     void sync();
     Stats stats();
+    void clear_solver_state();
     // The form the handle runs: solver, precision and pair-kernel rows after defaults are applied.
     int solver() const { return opt_.solver; }
     int precision() const { return opt_.precision; }
