@@ -163,6 +163,41 @@ int integrateAugmentedOptomechanicalSimulationRK4(double* initialState, double**
 /* Export.cuh:76 -- releases the arrays returned by integrateAugmentedOptomechanicalSimulationRK4. */
 int integrateAugmentedOptomechanicalSimulationRK4_freeMemory(double* statesOut, double* timesOut);
 
+/* Export.cuh:72 / Export.cu:780-968 -- the optically driven helium film with the time-dependent
+ * delayed intensity (HeliumWithOptomechanicalDrivingProblem + TimedBoundaryIntegrator + the RK4 of
+ * RK4_Time_Dependent.cuh:170-412): stages at t_n, t_n + h/2, t_n + h/2, t_n + h; at a stage of time ts
+ * node i takes D = I_i when ts equals the saved time, else a Dsaved_i + beta tau (1 - a) I_i with
+ * a = exp(-(ts - saved) / tau); the first stage of each step saves D and its time (the clock starts at
+ * t0); dPhi/dt += damping v + beta D + strength I.  initialState = [x(N), y(N), phi(N)].  With
+ * returnTrajectory, *statesOut = malloc'd [steps][3N] (the state after each step) and *timesOut =
+ * malloc'd [steps] nondimensional times of the start of each step, as the reference logs them
+ * (RK4_Time_Dependent.cuh:402-408); without it both counts are 0, as in the reference.
+ * Steps: (t1 - t0) / dt truncated.  N in {32, 64, ..., 8192}.  Differs from the reference where that
+ * is racy: there every thread of the saving stage stores the saved time while others still read it
+ * (DelayedIntensityTerm.cuh:20-37), so some nodes take D = I; here every node of a stage reads the
+ * saved time of the previous step. */
+int integrateOptomechanicalSimulationRK4(double* initialState, double** statesOut, size_t* statesCount,
+                                         double** timesOut, size_t* timesCount, SimProperties* simProperties,
+                                         RK4SolverOptions* rkOptions, COptomechanicalVariables* optomechanicalVariables,
+                                         size_t N);
+/* Export.cuh:73 -- releases the arrays returned by integrateOptomechanicalSimulationRK4. */
+int integrateOptomechanicalSimulationRK4_freeMemory(double* statesOut, double* timesOut);
+
+/* Export.cuh:79 / Export.cu:1108-1209 -- right-hand side of the augmented system (the drive of
+ * integrateAugmentedOptomechanicalSimulationRK4) at state = [x(N), y(N), phi(N), D(N)]:
+ * rhs = [vx(N), vy(N), dphi/dt(N), dD/dt(N)].  N in {32, 64, ..., 8192}.  Returns error codes where
+ * the reference returns 0 after printing an exception. */
+int calculateRhsAugmentedOptomechanical(double* state, double* rhs, SimProperties* simProperties,
+                                        COptomechanicalVariables* optomechanicalVariables, size_t N);
+
+/* Export.cuh:53 / Export.cu:601-660 -- the +epsilon perturbed states of the N = 256 surface
+ * (createInitialBatchedZ, JacobianCalculator.cuh:11-76): Zperturbed holds 6 * 256^2 complex, batch
+ * b = c 256 + p (c = 0: x, 1: y, 2: phi of node p perturbed) with its positions at [256 b, 256 b + 256)
+ * and its potentials phi + 0i at [3 * 256^2 + 256 b, ...).  L, rho, kappa and depth are unused, as in
+ * the reference. */
+int calculatePerturbedStates256(const double* x, const double* y, const double* phi, c_double* Zperturbed, double L,
+                                double rho, double kappa, double depth, double epsilon);
+
 /* ------------------------------------------------------------------------------------------
  * Native handle API.
  * ---------------------------------------------------------------------------------------- */
@@ -293,7 +328,9 @@ int sfh_jacobian(const sfh_props* props, int n, const double* state, double eps,
  * or sfh_rhs_device. */
 typedef struct sfh_drive {
     int augmented;
-    int reserved;
+    int timed;   /* 1: the time-dependent delayed intensity of integrateOptomechanicalSimulationRK4 (D
+                    saved by the first stage of each step; sfh_set_time starts its clock); not with
+                    augmented */
     double damping;
     double strength;
     double detuning;
@@ -306,6 +343,9 @@ typedef struct sfh_drive {
     double tau;
 } sfh_drive;
 int sfh_set_drive(sfh_handle h, const sfh_drive* drive);
+/* Sets the handle's time (0 after sfh_set_state); with the timed drive also the saved time of the
+ * delayed intensity (TimedBoundaryIntegrator::setStartingTime, TimedBoundaryIntegrator.cuh:48-52). */
+int sfh_set_time(sfh_handle h, double t);
 int sfh_set_delayed(sfh_handle h, const double* D);
 int sfh_get_delayed(sfh_handle h, double* D);
 int sfh_get_delayed_rhs(sfh_handle h, double* dD);

@@ -139,6 +139,9 @@ def lib():
         L.sfo_rk4_driven.argtypes = [ctypes.POINTER(_Props), ctypes.POINTER(_Drive), ctypes.c_int, ctypes.c_int,
                                      ctypes.c_double, ctypes.c_int, dp, dp]
         L.sfo_rk4_driven.restype = ctypes.c_int
+        L.sfo_rk4_timed.argtypes = [ctypes.POINTER(_Props), ctypes.POINTER(_Drive), ctypes.c_int, ctypes.c_int,
+                                    ctypes.c_double, ctypes.c_double, ctypes.c_int, dp, dp, dp]
+        L.sfo_rk4_timed.restype = ctypes.c_int
         L.sfo_jacobian.argtypes = [ctypes.POINTER(_Props), ctypes.c_int, dp, ctypes.c_double, dp]
         L.sfo_jacobian.restype = ctypes.c_int
         L.sfo_gl2.argtypes = [ctypes.POINTER(_Props), ctypes.c_int, dp, ctypes.c_double, ctypes.c_double,
@@ -335,6 +338,19 @@ def rk4_driven(props: Props, drive: Drive, Z, Phi, D, dt: float, steps: int, bat
     lib().sfo_rk4_driven(ctypes.byref(cp), ctypes.byref(cd), tot // batch, batch, float(dt), int(steps),
                          _p(state.view(np.float64)), None)
     return state[:tot].copy(), state[tot:2 * tot].real.copy(), (state[2 * tot:].real.copy() if drive.augmented else None)
+
+
+def rk4_timed(props: Props, drive: Drive, Z, Phi, dt: float, t0: float, steps: int, Dsav=None, batch: int = 1):
+    """(Z, Phi, Dsav) after `steps` RK4 steps of the time-dependent drive from time t0; Dsav is the
+    saved delayed intensity (read only after the first step, whose first stage stores it)."""
+    Zc = _c128(Z)
+    tot = Zc.size
+    state = np.concatenate([Zc, _c128(_f64(Phi))])
+    ds = np.zeros(tot) if Dsav is None else _f64(Dsav).copy()
+    cp, cd = props.c(), drive.c()
+    lib().sfo_rk4_timed(ctypes.byref(cp), ctypes.byref(cd), tot // batch, batch, float(dt), float(t0), int(steps),
+                        _p(state.view(np.float64)), _p(ds), None)
+    return state[:tot].copy(), state[tot:].real.copy(), ds
 
 
 def jacobian(props: Props, state, eps: float = 1e-6) -> np.ndarray:

@@ -25,6 +25,8 @@
  *                         augmented system's DelayedIntensityIntegrator::run (DelayedIntensityIntegrator.cuh:20-32,
  *                         createM.cuh:151-169; AugmentedBoundaryIntegrator.cuh:32-36)
  *   sfo_rk4_driven        the same RK4 over [Z, Phi(, D)]
+ *   sfo_rk4_timed         the time-dependent drive and its RK4 (RK4_Time_Dependent.cuh:170-412,
+ *                         createM.cuh:119-136, DelayedIntensityTerm.cuh:14-38)
  *   sfo_jacobian          JacobianCalculator::calculateJacobian (JacobianCalculator.cuh:223-284): central
  *                         differences of the RHS, column-major 3N x 3N
  *   sfo_gl2               GaussLegendre2::runEvolution / gaussLegendreS2Step (GaussLegendre.cuh:190-560),
@@ -638,6 +640,84 @@ int sfo_rk4_driven(const sfo_props *p, const sfo_drive *d, int n, int batch, dou
 {
     drive_ctx c = {p, d, n, batch};
     return sfo_rk4_generic(driven_adapter, &c, (size_t)(d->augmented ? 3 : 2) * n * batch, dt, steps, state, traj);
+}
+
+/* Time-dependent drive (HeliumWithOptomechanicalDrivingProblem + TimedBoundaryIntegrator +
+ * RungeKuttaStepper of RK4_Time_Dependent.cuh:170-412, add_optical_field_drive_terms createM.cuh:119-136,
+ * DelayedIntensityTerm.cuh:14-38).  The stages of the step from t_n run at t_n, t_n + h/2, t_n + h/2
+ * and t_n + h (h/2 = h * 0.5, t_n accumulated as currentTime += h from t0).  At a stage of time ts the
+ * delayed intensity of node i is
+ *     Ds = I_i                                       if ts == prev
+ *     Ds = a Dsav_i + beta tau (1 - a) I_i,  a = exp(-(ts - prev) / tau)   otherwise,
+ * with I_i the intensity at the stage state; the first stage of every step then stores Dsav = Ds and
+ * prev = t_n (prev starts at t0: setStartingTime).  dPhi/dt = ((Bernoulli + damping v) + beta Ds)
+ * + strength I.  Every node of a stage sees the same prev: the reference stores prev from every
+ * thread of the first stage while other threads still read it, and this restatement takes the
+ * order in which all reads come first. */
+typedef struct {
+    const sfo_props *p;
+    const sfo_drive *d;
+    int n, batch;
+    double ts, prev;
+    int save;
+    double *dsav;
+} timed_ctx;
+
+static int timed_rhs(void *ctx, const double *y, double *k)
+{
+    timed_ctx *c = (timed_ctx *)ctx;
+    const sfo_drive *d = c->d;
+    const size_t tot = (size_t)c->n * c->batch;
+    const int info = sfo_rhs(c->p, c->n, c->batch, y, k, NULL);
+    const int same = c->ts == c->prev;
+    const double a = same ? 0.0 : exp(-(c->ts - c->prev) / d->tau);
+    const double bt = same ? 1.0 : d->beta * d->tau * (1.0 - a);
+    for (size_t i = 0; i < tot; ++i) {
+        const double x = y[2 * i], yy = y[2 * i + 1], v = k[2 * i + 1];
+        const double inten = sfo_intensity(d, yy, x);
+        const double D = same ? inten : a * c->dsav[i] + bt * inten;
+        if (c->save) c->dsav[i] = D;
+        double r = k[2 * tot + 2 * i];
+        r += d->damping * v;
+        r += d->beta * D;
+        r += d->strength * inten;
+        k[2 * tot + 2 * i] = r;
+    }
+    return info;
+}
+
+int sfo_rk4_timed(const sfo_props *p, const sfo_drive *d, int n, int batch, double dt, double t0, int steps,
+                  double *state, double *dsav, double *traj)
+{
+    const size_t len = 4 * (size_t)n * batch;
+    const double half = dt * 0.5, sixth = dt / 6.0;
+    double *k1 = (double *)malloc(sizeof(double) * len), *k2 = (double *)malloc(sizeof(double) * len);
+    double *k3 = (double *)malloc(sizeof(double) * len), *k4 = (double *)malloc(sizeof(double) * len);
+    double *ys = (double *)malloc(sizeof(double) * len);
+    timed_ctx c = {p, d, n, batch, t0, t0, 0, dsav};
+    double t = t0;
+    int info = 0, r;
+    for (int s = 0; s < steps; ++s) {
+        c.ts = t; c.save = 1;
+        if ((r = timed_rhs(&c, state, k1)) && !info) info = r;
+        c.prev = t; c.save = 0;
+        for (size_t i = 0; i < len; ++i) ys[i] = state[i] + half * k1[i];
+        c.ts = t + half;
+        if ((r = timed_rhs(&c, ys, k2)) && !info) info = r;
+        for (size_t i = 0; i < len; ++i) ys[i] = state[i] + half * k2[i];
+        if ((r = timed_rhs(&c, ys, k3)) && !info) info = r;
+        for (size_t i = 0; i < len; ++i) ys[i] = state[i] + dt * k3[i];
+        c.ts = t + dt;
+        if ((r = timed_rhs(&c, ys, k4)) && !info) info = r;
+        for (size_t i = 0; i < len; ++i) {
+            double kk = k1[i] + 2.0 * k2[i] + 2.0 * k3[i] + k4[i];
+            state[i] = state[i] + sixth * kk;
+        }
+        if (traj) memcpy(traj + (size_t)s * len, state, sizeof(double) * len);
+        t += dt;
+    }
+    free(k1); free(k2); free(k3); free(k4); free(ys);
+    return info;
 }
 
 /* ------------------------------------------- FD Jacobian and Gauss-Legendre */

@@ -301,6 +301,34 @@ int integrateSimulationRK4_freeMemory(double* statesOut, double* timesOut) {
     return 0;
 }
 
+namespace {
+
+// adimensionalizeOptomechanicalVariables (Export.cu:1248-1270) and
+// get_current_intensity_drive_strength (LightIntensity.cuh:31-34), with the adimensional rho
+sfh::Phys opto_drive(const Units& u, const COptomechanicalVariables& ov, bool augmented, bool timed) {
+    const double hbar = 1.054571817e-34;   // constants.cuh:10
+    sfh::Phys d{};
+    d.drive = 1;
+    d.augmented = augmented ? 1 : 0;
+    d.timed = timed ? 1 : 0;
+    d.gamma = ov.gamma * u.base_time;
+    d.detuning = ov.detuning * u.base_time;
+    d.G = ov.G * u.base_time * u.base_length;
+    d.tau = ov.tau / u.base_time;
+    d.x0 = ov.location_x0_mode / u.base_length;
+    d.sigma = ov.sigma_optical_mode / u.base_length;
+    d.max_intensity = ov.max_intensity;
+    const double hbar_adim = hbar / u.base_energy / u.base_time;
+    d.beta = ov.beta * (hbar_adim * d.G / d.tau / (std::pow(d.sigma, 2.0) * u.rho));
+    d.damping = ov.damping_strength;
+    d.strength = hbar / (u.base_energy * u.base_time * u.rho) * d.G / std::pow(d.sigma, 2.0);
+    return d;
+}
+
+bool opto_n(size_t N) { return N >= 32 && N <= 8192 && (N & (N - 1)) == 0; }   // Export.cu:950-968,1186-1207
+
+}  // namespace
+
 int integrateAugmentedOptomechanicalSimulationRK4(double* initialState, double** statesOut, size_t* statesCount,
                                                   double** timesOut, size_t* timesCount, SimProperties* simProperties,
                                                   RK4SolverOptions* rkOptions, COptomechanicalVariables* ov, size_t N) {
@@ -309,7 +337,7 @@ int integrateAugmentedOptomechanicalSimulationRK4(double* initialState, double**
         g_last_error = "null pointer argument";
         return -3;
     }
-    if (N < 32 || N > 8192 || (N & (N - 1)) != 0) {   // the sizes Export.cu:1076-1093 dispatches
+    if (!opto_n(N)) {   // the sizes Export.cu:1076-1093 dispatches
         g_last_error = "unsupported particle number " + std::to_string(N);
         return -2;
     }
@@ -327,23 +355,7 @@ int integrateAugmentedOptomechanicalSimulationRK4(double* initialState, double**
         if (!(nsteps < 2.0e9)) throw sfh::ArgError("too many time steps");
         const size_t steps = (size_t)nsteps;
         if (steps > SIZE_MAX / (sizeof(double) * 4 * N) - 1) throw sfh::ArgError("trajectory too large to allocate");
-        // adimensionalizeOptomechanicalVariables (Export.cu:1248-1270) and
-        // get_current_intensity_drive_strength (LightIntensity.cuh:31-34), with the adimensional rho
-        const double hbar = 1.054571817e-34;   // constants.cuh:10
-        sfh::Phys d{};
-        d.drive = 1;
-        d.augmented = 1;
-        d.gamma = ov->gamma * u.base_time;
-        d.detuning = ov->detuning * u.base_time;
-        d.G = ov->G * u.base_time * u.base_length;
-        d.tau = ov->tau / u.base_time;
-        d.x0 = ov->location_x0_mode / u.base_length;
-        d.sigma = ov->sigma_optical_mode / u.base_length;
-        d.max_intensity = ov->max_intensity;
-        const double hbar_adim = hbar / u.base_energy / u.base_time;
-        d.beta = ov->beta * (hbar_adim * d.G / d.tau / (std::pow(d.sigma, 2.0) * u.rho));
-        d.damping = ov->damping_strength;
-        d.strength = hbar / (u.base_energy * u.base_time * u.rho) * d.G / std::pow(d.sigma, 2.0);
+        const sfh::Phys d = opto_drive(u, *ov, true, false);
         sfh::Options opt;
         Solver s(helium_phys(u, sp.infinite_depth, sp.use_expansions, sp.expansion_order),
                  std::vector<int>(1, (int)N), opt);
@@ -464,6 +476,126 @@ int integrateAugmentedOptomechanicalSimulationRK4_freeMemory(double* statesOut, 
     std::free(statesOut);
     std::free(timesOut);
     return 0;
+}
+
+int integrateOptomechanicalSimulationRK4(double* initialState, double** statesOut, size_t* statesCount,
+                                         double** timesOut, size_t* timesCount, SimProperties* simProperties,
+                                         RK4SolverOptions* rkOptions, COptomechanicalVariables* ov, size_t N) {
+    if (!initialState || !statesOut || !statesCount || !timesOut || !timesCount || !simProperties || !rkOptions ||
+        !ov) {
+        g_last_error = "null pointer argument";
+        return -3;
+    }
+    if (!opto_n(N)) {
+        g_last_error = "unsupported particle number " + std::to_string(N);
+        return -2;
+    }
+    return guarded([&] {
+        const SimProperties sp = *simProperties;
+        const Units u = adimensionalize(sp.L, sp.rho, sp.kappa, sp.depth);
+        const double t0 = rkOptions->t0 / u.base_time;
+        const double t1 = rkOptions->t1 / u.base_time;
+        const double dt = rkOptions->timeStep / u.base_time;
+        if (!std::isfinite(t0) || !std::isfinite(t1) || !std::isfinite(dt))
+            throw sfh::ArgError("t0, t1 and the time step must be finite");
+        if (!(dt > 0.0)) throw sfh::ArgError("time step must be positive");
+        if (t1 < t0) throw sfh::ArgError("t1 must not precede t0");
+        // runEvolution (RK4_Time_Dependent.cuh:386-412): (t1 - t0) / dt steps, truncated
+        const double nsteps = std::floor((t1 - t0) / dt);
+        if (!(nsteps < 2.0e9)) throw sfh::ArgError("too many time steps");
+        const size_t steps = (size_t)nsteps;
+        const bool traj = rkOptions->returnTrajectory;
+        // the reference collects states and times only when the trajectory is returned
+        const size_t nst = traj ? steps : 0;
+        if (nst > SIZE_MAX / (sizeof(double) * 3 * N) - 1) throw sfh::ArgError("trajectory too large to allocate");
+        sfh::Options opt;
+        Solver s(helium_phys(u, sp.infinite_depth, sp.use_expansions, sp.expansion_order),
+                 std::vector<int>(1, (int)N), opt);
+        s.set_drive(opto_drive(u, *ov, false, true));
+        const double* x = initialState;
+        s.set_state(x, x + N, x + 2 * N);
+        s.set_time(t0);   // setStartingTime(t0): the delayed intensity's clock starts at t0
+        double* states = (double*)std::malloc(sizeof(double) * 3 * N * (nst ? nst : 1));
+        double* times = (double*)std::malloc(sizeof(double) * (nst ? nst : 1));
+        if (!states || !times) {
+            std::free(states);
+            std::free(times);
+            throw std::runtime_error("host allocation failed");
+        }
+        try {
+            if (traj) {
+                for (size_t done = 0; done < steps;) {
+                    const int chunk = (int)std::min<size_t>(steps - done, 1u << 30);
+                    s.step_record(dt, chunk, 1, states + 3 * N * done, times + done);
+                    done += (size_t)chunk;
+                }
+                // devTimes receives currentTime before it advances: the time at the start of the step
+                double t = t0;
+                for (size_t k = 0; k < steps; ++k) {
+                    times[k] = t;
+                    t += dt;
+                }
+            } else {
+                for (size_t done = 0; done < steps;) {
+                    const int chunk = (int)std::min<size_t>(steps - done, 1u << 30);
+                    s.step(dt, chunk);
+                    done += (size_t)chunk;
+                }
+                s.sync();
+            }
+        } catch (...) {
+            std::free(states);
+            std::free(times);
+            throw;
+        }
+        *statesOut = states;
+        *statesCount = nst;
+        *timesOut = times;
+        *timesCount = nst;
+    });
+}
+
+int integrateOptomechanicalSimulationRK4_freeMemory(double* statesOut, double* timesOut) {
+    std::free(statesOut);
+    std::free(timesOut);
+    return 0;
+}
+
+int calculateRhsAugmentedOptomechanical(double* state, double* rhs, SimProperties* simProperties,
+                                        COptomechanicalVariables* ov, size_t N) {
+    if (!state || !rhs || !simProperties || !ov) {
+        g_last_error = "null pointer argument";
+        return -3;
+    }
+    if (!opto_n(N)) {
+        g_last_error = "unsupported particle number " + std::to_string(N);
+        return -2;
+    }
+    return guarded([&] {
+        const SimProperties sp = *simProperties;
+        const Units u = adimensionalize(sp.L, sp.rho, sp.kappa, sp.depth);
+        sfh::Options opt;
+        Solver s(helium_phys(u, sp.infinite_depth, sp.use_expansions, sp.expansion_order),
+                 std::vector<int>(1, (int)N), opt);
+        s.set_drive(opto_drive(u, *ov, true, false));
+        s.set_state(state, state + N, state + 2 * N);
+        s.set_delayed(state + 3 * N);
+        s.rhs(rhs, rhs + N, rhs + 2 * N);
+        s.get_delayed_rhs(rhs + 3 * N);
+    });
+}
+
+int calculatePerturbedStates256(const double* x, const double* y, const double* phi, c_double* Zperturbed, double L,
+                                double rho, double kappa, double depth, double epsilon) {
+    (void)L;
+    (void)rho;
+    (void)kappa;
+    (void)depth;   // the reference takes them and does not use them (Export.cu:601-660)
+    if (!x || !y || !phi || !Zperturbed) {
+        g_last_error = "null pointer argument";
+        return -3;
+    }
+    return guarded([&] { sfh::perturbed_states(x, y, phi, 256, epsilon, reinterpret_cast<double*>(Zperturbed)); });
 }
 
 void sfh_default_options(sfh_options* opt) {
@@ -658,6 +790,7 @@ int sfh_set_drive(sfh_handle h, const sfh_drive* dr) {
         if (dr) {
             d.drive = 1;
             d.augmented = dr->augmented ? 1 : 0;
+            d.timed = dr->timed ? 1 : 0;
             d.damping = dr->damping;
             d.strength = dr->strength;
             d.detuning = dr->detuning;
@@ -671,6 +804,11 @@ int sfh_set_drive(sfh_handle h, const sfh_drive* dr) {
         }
         h->s->set_drive(d);
     });
+}
+
+int sfh_set_time(sfh_handle h, double t) {
+    SFH_NEED(h);
+    return guarded([&] { h->s->set_time(t); });
 }
 
 int sfh_set_delayed(sfh_handle h, const double* D) {

@@ -38,9 +38,13 @@ struct Phys {
     int need_upper;     // water with rho != 0 needs the upper-fluid velocity
     double rho, kappa, depth, U, g;
     // optical drive (HeliumDrivenAutonomousProblem, AugmentedBoundaryIntegrator), nondimensional:
-    // dPhi/dt += damping v + strength I(y, x) [+ D], dD/dt = beta I - D / tau when augmented
+    // dPhi/dt += damping v + strength I(y, x) [+ D], dD/dt = beta I - D / tau when augmented;
+    // timed: the delayed intensity of HeliumWithOptomechanicalDrivingProblem, an exponential
+    // integrator over the stage times (dPhi/dt += damping v + beta D + strength I)
     int drive;
     int augmented;
+    int timed;
+    int pad_drive;
     double damping, strength, detuning, gamma, G, max_intensity, x0, sigma, beta, tau;
 };
 

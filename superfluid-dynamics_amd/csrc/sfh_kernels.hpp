@@ -30,6 +30,10 @@ struct StageCtl {
     double* Dout;
     double* kD[3];
     int* nonfinite;    // optional: set to 1 when a stage slope of any node is NaN or infinite
+    // timed drive: the stage's delayed intensity is dca * Din + dcb * I; dsave: store it in Dout
+    double dca, dcb;
+    int dsave;
+    int pad_timed;
 };
 
 // Warm start of a solve by extrapolation over time steps: each RK stage keeps its solutions of the

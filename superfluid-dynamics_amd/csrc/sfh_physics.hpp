@@ -73,9 +73,20 @@ __device__ inline double intensity(const Phys& ph, double y, double x) {
 __device__ inline double drive_terms(const Phys& ph, const StageCtl& ctl, int g, double x, double y, double2 q1,
                                      double dphi, double& dD) {
     const double inten = intensity(ph, y, x);
+    dD = 0.0;
+    if (ph.timed) {
+        // add_optical_field_drive_terms (createM.cuh:119-136) with DelayedIntensityTerm's update
+        // (DelayedIntensityTerm.cuh:20-37): D = a Dsaved + beta tau (1 - a) I, the coefficients of
+        // the stage's time formed on the host; the first stage of a step saves D
+        const double D = ctl.dca * ctl.Din[g] + ctl.dcb * inten;
+        if (ctl.dsave) ctl.Dout[g] = D;
+        dphi += ph.damping * q1.y;
+        dphi += ph.beta * D;
+        dphi += ph.strength * inten;
+        return dphi;
+    }
     dphi += ph.damping * q1.y;
     dphi += ph.strength * inten;
-    dD = 0.0;
     if (ph.augmented) {
         const double D = ctl.Din[g];
         dphi += D;

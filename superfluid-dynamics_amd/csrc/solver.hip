@@ -603,22 +603,61 @@ void Solver::enqueue_step(double dt) {
         c.kP[i] = kP_[i];
     }
     c.qup = nullptr;
+    // timed drive: stages at t_n, t_n + h/2, t_n + h/2, t_n + h (RK4_Time_Dependent.cuh:173,237,263,270);
+    // the first saves the delayed intensity and moves the saved time to t_n
     c.stage = 1; c.alpha = half; c.Zout = Za_; c.Pout = Pa_;
     set_delayed_ctl(c, D0_, Da_);
+    timed_coefficients(c, drive_t_, 1);
     enqueue_rhs(Z0_, P0_, c);
+    if (ph_.timed) drive_prev_ = drive_t_;
     c.stage = 2; c.alpha = half; c.Zout = Zb_; c.Pout = Pb_;
     set_delayed_ctl(c, Da_, Db_);
+    timed_coefficients(c, drive_t_ + half, 0);
     enqueue_rhs(Za_, Pa_, c);
     c.stage = 3; c.alpha = dt; c.Zout = Za_; c.Pout = Pa_;
     set_delayed_ctl(c, Db_, Da_);
+    timed_coefficients(c, drive_t_ + half, 0);
     enqueue_rhs(Zb_, Pb_, c);
     c.stage = 4; c.alpha = sixth; c.Zout = nullptr; c.Pout = nullptr;
     set_delayed_ctl(c, Da_, nullptr);
+    timed_coefficients(c, drive_t_ + dt, 0);
     enqueue_rhs(Za_, Pa_, c);
+    if (ph_.timed) drive_t_ += dt;   // currentTime += timeStep (RK4_Time_Dependent.cuh:408)
+}
+
+// calculate_new_delayed_intensity (DelayedIntensityTerm.cuh:20-30) at stage time ts: D = I when ts
+// is the saved time, else a Dsaved + beta tau (1 - a) I with a = exp(-(ts - prev) / tau).  Every node
+// of the stage takes the same saved time: the reference stores it from every thread of the saving
+// stage while other threads may still read it, so some of its nodes see ts == prev; here the time
+// moves only after the stage.
+void Solver::timed_coefficients(StageCtl& c, double ts, int save) const {
+    c.dca = c.dcb = 0.0;
+    c.dsave = 0;
+    if (!ph_.timed) return;
+    if (ts == drive_prev_) {
+        c.dcb = 1.0;
+    } else {
+        const double a = std::exp(-(ts - drive_prev_) / ph_.tau);
+        c.dca = a;
+        c.dcb = ph_.beta * ph_.tau * (1.0 - a);
+    }
+    c.dsave = save;
+}
+
+void Solver::set_time(double t) {
+    if (!std::isfinite(t)) throw ArgError("time must be finite");
+    time_ = drive_t_ = drive_prev_ = t;
 }
 
 // The delayed-intensity fields of a stage (null without the augmented drive).
 void Solver::set_delayed_ctl(StageCtl& c, const double* Din, double* Dout) const {
+    if (ph_.timed) {   // the saved delayed intensity, read by every stage, written by the first
+        c.Din = D0_;
+        c.Dout = D0_;
+        c.D0 = nullptr;
+        c.kD[0] = c.kD[1] = c.kD[2] = nullptr;
+        return;
+    }
     if (!ph_.augmented) {
         c.Din = nullptr;
         c.D0 = c.Dout = nullptr;
@@ -681,7 +720,10 @@ void Solver::save_checkpoint() {
     SFH_HIP(hipMemcpyAsync(ckHist_, gmhist_, 4 * kHistDepth * (size_t)T_ * sizeof(double),
                            hipMemcpyDeviceToDevice, stream_));
     SFH_HIP(hipMemcpyAsync(ckHv_, gmhvalid_, 4 * sizeof(int), hipMemcpyDeviceToDevice, stream_));
-    if (ph_.augmented) SFH_HIP(hipMemcpyAsync(ckD_, D0_, T_ * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+    if (ph_.augmented || ph_.timed)
+        SFH_HIP(hipMemcpyAsync(ckD_, D0_, T_ * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+    ck_drive_t_ = drive_t_;
+    ck_drive_prev_ = drive_prev_;
 }
 
 void Solver::restore_checkpoint() {
@@ -691,11 +733,15 @@ void Solver::restore_checkpoint() {
     SFH_HIP(hipMemcpyAsync(gmhist_, ckHist_, 4 * kHistDepth * (size_t)T_ * sizeof(double),
                            hipMemcpyDeviceToDevice, stream_));
     SFH_HIP(hipMemcpyAsync(gmhvalid_, ckHv_, 4 * sizeof(int), hipMemcpyDeviceToDevice, stream_));
-    if (ph_.augmented) SFH_HIP(hipMemcpyAsync(D0_, ckD_, T_ * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+    if (ph_.augmented || ph_.timed)
+        SFH_HIP(hipMemcpyAsync(D0_, ckD_, T_ * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+    drive_t_ = ck_drive_t_;
+    drive_prev_ = ck_drive_prev_;
 }
 
 void Solver::run_steps(double dt, int len, int count) {
-    if (opt_.use_graph && !ktime_) {
+    // the timed drive's stage coefficients change from step to step: launched directly
+    if (opt_.use_graph && !ktime_ && !ph_.timed) {
         if (!exec_ || graph_dt_ != dt || graph_len_ != len) capture(dt, len);
         for (int b = 0; b < count; ++b) SFH_HIP(hipGraphLaunch(exec_, stream_));
     } else {
@@ -846,7 +892,7 @@ void Solver::set_state_device(const double2* state) {
     if (gmhvalid_) SFH_HIP(hipMemsetAsync(gmhvalid_, 0, 4 * sizeof(int), stream_));   // a new trajectory
     if (bhvalid_) SFH_HIP(hipMemsetAsync(bhvalid_, 0, 4 * S_ * sizeof(int), stream_));
     hist_steps_ = 0;
-    time_ = 0.0;
+    time_ = drive_t_ = drive_prev_ = 0.0;
     SFH_HIP(hipStreamSynchronize(stream_));
 }
 
@@ -868,6 +914,8 @@ void Solver::rhs_device(const double2* state, double2* rhs) {
     c.stage = 0;
     c.kZ[0] = kZ_[0];
     c.kP[0] = kP_[0];
+    set_delayed_ctl(c, D0_, nullptr);
+    timed_coefficients(c, drive_t_, 0);
     enqueue_rhs(Za_, Pa_, c);
     hipLaunchKernelGGL(state_join_kernel, dim3((T_ + 255) / 256), dim3(256), 0, stream_, kZ_[0], kP_[0], rhs, T_);
     SFH_HIP(hipGetLastError());
@@ -915,7 +963,7 @@ void Solver::set_state(const double* x, const double* y, const double* phi) {
     SFH_HIP(hipMemcpyAsync(P0_, phi, T_ * sizeof(double), hipMemcpyHostToDevice, stream_));
     if (gmhvalid_) SFH_HIP(hipMemsetAsync(gmhvalid_, 0, 4 * sizeof(int), stream_));   // a new trajectory
     hist_steps_ = 0;
-    time_ = 0.0;
+    time_ = drive_t_ = drive_prev_ = 0.0;
     if (D0_) SFH_HIP(hipMemsetAsync(D0_, 0, T_ * sizeof(double), stream_));
     if (bhvalid_) SFH_HIP(hipMemsetAsync(bhvalid_, 0, 4 * S_ * sizeof(int), stream_));
     SFH_HIP(hipStreamSynchronize(stream_));
@@ -941,6 +989,7 @@ void Solver::rhs(double* vx, double* vy, double* dphi, double* mu, double2* zp, 
     c.kP[0] = kP_[0];
     c.qup = qup ? qup_ : nullptr;
     set_delayed_ctl(c, D0_, nullptr);
+    timed_coefficients(c, drive_t_, 0);   // timed drive: at the handle's time, nothing saved
     enqueue_rhs(Z0_, P0_, c);
     std::vector<double2> q(T_);
     SFH_HIP(hipMemcpyAsync(q.data(), kZ_[0], T_ * sizeof(double2), hipMemcpyDeviceToHost, stream_));
@@ -960,10 +1009,13 @@ void Solver::rhs(double* vx, double* vy, double* dphi, double* mu, double2* zp, 
 
 void Solver::set_drive(const Phys& d) {
     if (d.augmented && !d.drive) throw ArgError("the augmented system needs the drive on");
+    if (d.timed && (!d.drive || d.augmented)) throw ArgError("the timed drive needs the drive on and no augmented system");
     if (d.drive && !(d.tau != 0.0)) throw ArgError("tau must be nonzero");
     SFH_HIP(hipStreamSynchronize(stream_));
     ph_.drive = d.drive;
     ph_.augmented = d.augmented;
+    ph_.timed = d.timed;
+    drive_prev_ = drive_t_ = time_;
     ph_.damping = d.damping;
     ph_.strength = d.strength;
     ph_.detuning = d.detuning;
@@ -974,7 +1026,7 @@ void Solver::set_drive(const Phys& d) {
     ph_.sigma = d.sigma;
     ph_.beta = d.beta;
     ph_.tau = d.tau;
-    if (ph_.augmented && !D0_) {
+    if ((ph_.augmented || ph_.timed) && !D0_) {
         D0_ = dalloc<double>(T_);
         Da_ = dalloc<double>(T_);
         Db_ = dalloc<double>(T_);
@@ -986,13 +1038,13 @@ void Solver::set_drive(const Phys& d) {
 }
 
 void Solver::set_delayed(const double* D) {
-    if (!ph_.augmented) throw ArgError("the handle has no augmented drive (sfh_set_drive)");
+    if (!ph_.augmented && !ph_.timed) throw ArgError("the handle has no augmented or timed drive (sfh_set_drive)");
     SFH_HIP(hipMemcpyAsync(D0_, D, T_ * sizeof(double), hipMemcpyHostToDevice, stream_));
     SFH_HIP(hipStreamSynchronize(stream_));
 }
 
 void Solver::get_delayed(double* D) {
-    if (!ph_.augmented) throw ArgError("the handle has no augmented drive (sfh_set_drive)");
+    if (!ph_.augmented && !ph_.timed) throw ArgError("the handle has no augmented or timed drive (sfh_set_drive)");
     SFH_HIP(hipMemcpyAsync(D, D0_, T_ * sizeof(double), hipMemcpyDeviceToHost, stream_));
     SFH_HIP(hipStreamSynchronize(stream_));
 }

@@ -79,6 +79,9 @@ struct Tuning {
 Tuning autotune(const Phys& ph, int n, bool allow_mixed, int steps);
 Tuning tuned(const Phys& ph, int n, bool allow_mixed);
 
+// The +eps perturbed states of calculatePerturbedStates256 (deriv.hip): out holds 6 n^2 complex.
+void perturbed_states(const double* x, const double* y, const double* phi, int n, double eps, double* out_host);
+
 // Central-difference Jacobian of [vx, vy, dphi] with respect to the real state [x, y, phi] of one
 // surface of n <= 2048 nodes: jac[k 3n + r] = d f_r / d s_k (implicit.cpp).
 void fd_jacobian(const Phys& ph, int n, const double* state, double eps, double* jac);
@@ -143,6 +146,8 @@ class Solver {
     void sync();
     Stats stats();
     void clear_solver_state();
+    // Sets the handle's time (and, with the timed drive, the start of its delayed-intensity clock).
+    void set_time(double t);
     // The form the handle runs: solver, precision and pair-kernel rows after defaults are applied.
     int solver() const { return opt_.solver; }
     int precision() const { return opt_.precision; }
@@ -299,6 +304,11 @@ class Solver {
     double* energy_ = nullptr;   // 4 doubles per node while energies are tracked
     bool track_energy_ = false;
     double time_ = 0.0;          // advanced by dt per step, 0 after set_state
+    // timed drive: the clock of the stage times (t_n of the next step) and the time of the last
+    // saved delayed intensity (setStartingTime / save_value, DelayedIntensityTerm.cuh:31-37,66-69),
+    // and their values at the chunk checkpoint
+    double drive_t_ = 0.0, drive_prev_ = 0.0, ck_drive_t_ = 0.0, ck_drive_prev_ = 0.0;
+    void timed_coefficients(StageCtl& c, double ts, int save) const;
     // augmented drive: delayed intensity of y0, the stage states, the slopes, the checkpoint copy
     double *D0_ = nullptr, *Da_ = nullptr, *Db_ = nullptr, *ckD_ = nullptr;
     double* kD_[3] = {nullptr, nullptr, nullptr};

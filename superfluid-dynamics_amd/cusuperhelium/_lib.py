@@ -48,7 +48,7 @@ class SfhTuning(Structure):
 
 
 class SfhDrive(Structure):
-    _fields_ = [("augmented", c_int), ("reserved", c_int)] + [
+    _fields_ = [("augmented", c_int), ("timed", c_int)] + [
         (k, c_double) for k in ("damping", "strength", "detuning", "gamma", "G", "max_intensity", "x0", "sigma",
                                 "beta", "tau")]
 
@@ -96,6 +96,15 @@ SIGNATURES = {
                                                               POINTER(c_size_t), POINTER(CSimulationProperties),
                                                               POINTER(CRK4Options), POINTER(COptomechanicalVariables),
                                                               c_size_t]),
+    "integrateOptomechanicalSimulationRK4": (c_int, [DP, POINTER(DP), POINTER(c_size_t), POINTER(DP),
+                                                     POINTER(c_size_t), POINTER(CSimulationProperties),
+                                                     POINTER(CRK4Options), POINTER(COptomechanicalVariables),
+                                                     c_size_t]),
+    "integrateOptomechanicalSimulationRK4_freeMemory": (c_int, [DP, DP]),
+    "calculateRhsAugmentedOptomechanical": (c_int, [DP, DP, POINTER(CSimulationProperties),
+                                                    POINTER(COptomechanicalVariables), c_size_t]),
+    "calculatePerturbedStates256": (c_int, [DP, DP, DP, c_void_p, c_double, c_double, c_double, c_double,
+                                            c_double]),
     "integrateAugmentedOptomechanicalSimulationRK4_freeMemory": (c_int, [DP, DP]),
     "calculateJacobian": (c_int, [DP, DP, c_double, c_double, c_double, c_double, c_double, c_size_t]),
     "integrateSimulationGL2": (c_int, [DP, POINTER(DP), POINTER(c_size_t), POINTER(DP), POINTER(c_size_t),
@@ -105,6 +114,7 @@ SIGNATURES = {
     "sfh_autotune": (c_int, [POINTER(SfhProps), c_int, c_int, c_int, POINTER(SfhTuning)]),
     "sfh_tuned": (c_int, [POINTER(SfhProps), c_int, c_int, POINTER(SfhTuning)]),
     "sfh_set_drive": (c_int, [c_void_p, POINTER(SfhDrive)]),
+    "sfh_set_time": (c_int, [c_void_p, c_double]),
     "sfh_set_delayed": (c_int, [c_void_p, DP]),
     "sfh_get_delayed": (c_int, [c_void_p, DP]),
     "sfh_get_delayed_rhs": (c_int, [c_void_p, DP]),

@@ -122,20 +122,25 @@ class Ensemble:
         check(self._lib.sfh_step(self._h, float(dt), int(nsteps)), "sfh_step")
 
     # ---- optical drive (HeliumDrivenAutonomousProblem / AugmentedBoundaryIntegrator)
-    def set_drive(self, augmented: bool = False, **kw) -> None:
+    def set_drive(self, augmented: bool = False, timed: bool = False, **kw) -> None:
         """Turns the optical drive on (keywords of sfh_drive: damping, strength, detuning, gamma, G,
-        max_intensity, x0, sigma, beta, tau), or off with set_drive(None)."""
+        max_intensity, x0, sigma, beta, tau), or off with set_drive(None).  timed: the time-dependent
+        delayed intensity (its clock starts at set_time's time)."""
         if augmented is None:
             check(self._lib.sfh_set_drive(self._h, None), "sfh_set_drive")
             self.augmented = False
             return
         d = SfhDrive()
         d.augmented = int(bool(augmented))
+        d.timed = int(bool(timed))
         d.gamma, d.G, d.sigma, d.tau = 1.0, 1.0, 1.0, 1.0
         for k, v in kw.items():
             setattr(d, k, float(v))
         check(self._lib.sfh_set_drive(self._h, byref(d)), "sfh_set_drive")
         self.augmented = bool(augmented)
+
+    def set_time(self, t: float) -> None:
+        check(self._lib.sfh_set_time(self._h, float(t)), "sfh_set_time")
 
     def set_delayed(self, D) -> None:
         D = np.ascontiguousarray(D, dtype=np.float64).ravel()

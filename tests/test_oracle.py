@@ -403,6 +403,34 @@ def test_intensity_drive_shifts_linear_frequency(oracle):
     assert abs(a * np.cos(w * t) - a * np.cos(t)) > 0.1 * a
 
 
+def test_timed_drive_uniform_intensity_closed_form(oracle):
+    # The time-dependent drive (RK4_Time_Dependent.cuh:170-412, createM.cuh:119-136,
+    # DelayedIntensityTerm.cuh:20-37) on a flat film at rest under a uniform intensity I0: the first
+    # stage finds the saved time equal to t0 and takes D = I0; after that the exponential update
+    # is exact for a constant I0, so D(t) = b + (I0 - b) e^{-(t - t0)/tau}, b = beta tau I0, at every
+    # stage time, and dphi/dt = beta D + S I0 integrates to
+    #   phi(T) = beta (b T + (I0 - b) tau (1 - e^{-T/tau})) + S I0 T.
+    # RK4 then is Simpson's rule on a smooth integrand: 4th order in h.  A uniform phi drives no flow.
+    n, t0, T, tau, beta, S = 16, 0.3, 1.0, 0.5, 0.8, 0.2
+    x = 2 * np.pi * np.arange(n) / n
+    d = oracle.Drive(damping=0.3, strength=S, detuning=0.5, gamma=0.7, G=0.0, max_intensity=0.9, x0=1.0, sigma=1e8,
+                     beta=beta, tau=tau)
+    I0 = 0.25 * 0.49 * 0.9 / (0.25 + 0.1225)
+    b = beta * tau * I0
+    exact = beta * (b * T + (I0 - b) * tau * (1 - np.exp(-T / tau))) + S * I0 * T
+    errs = []
+    for steps in (25, 50):
+        dt = T / steps
+        Z, P, D = oracle.rk4_timed(oracle.Props(kind=oracle.HELIUM, depth=1.0), d, x + 0j, np.zeros(n), dt, t0, steps)
+        errs.append(np.abs(P - exact).max())
+        assert np.abs(Z.imag).max() < 1e-13 and np.ptp(P) < 1e-13
+        # the saved D is the one of the last step's first stage, at t0 + T - dt
+        assert np.abs(D - (b + (I0 - b) * np.exp(-(T - dt) / tau))).max() < 1e-14
+    assert errs[1] < 1e-8 and 12 < errs[0] / errs[1] < 20, errs
+    # without the first step's D = I0 (D from 0 instead) phi differs far above these errors
+    assert abs(beta * (I0 * tau * (1 - np.exp(-T / tau)))) > 1e3 * errs[0]
+
+
 # ------------------------------------------------------------------ FD Jacobian and GL2 (f4)
 @pytest.mark.parametrize("m", [1, 3])
 def test_jacobian_flat_rest_state_closed_form(oracle, m):
