@@ -22,6 +22,17 @@
 //     gz_k += conj(gamma_k P1_k),  P1_k = sum_j mu_j (1 + c_kj^2)
 //     gz_j -= mu_j conj(P2_j),     P2_j = sum_k gamma_k (1 + c_jk^2)
 //   gPhi = s Re D1^H gb,  gZ += s D1^H gZ' + s^2 D2^H gZ''.
+// Helium film with finite depth h (createFiniteDepthMKernel createM.cuh:65-92, the helium velocities
+// WaterVelocities.cuh:72-107, compute_rhs_helium_phi_expression createM.cuh:109-117): with the image
+// cotangent e_kj = cot((z_k - conj z_j)/2 + i h) over ALL j (j = k gives the diagonal terms cot(i(y_k + h)))
+//   M_kj += -ci Im e_kj,  w_k += (i/4pi) sum_j mu_j e_kj,  dPhi/dt = g h/3 (u^-3 - 1) + |q|^2/2, u = 1 + y/h
+// (ci = 1/4pi; the film's M image term has no Z'_k, as in the reference).  e_kj = -conj(e_jk), so the
+// image part of M is symmetric, and with d e_kj = -(1 + e_kj^2)(dz_k - conj dz_j)/2:
+//   gmu_j += -Im(Q_j)/4pi,  Q_j = sum_k gw_k e_jk
+//   gammaI_k = (i/8pi) gw_k - (i ci/2) gb_k
+//   gz_k += gammaI_k conj(PI1_k),  PI1_k = sum_j mu_j (1 + e_kj^2)
+//   gz_j -= mu_j conj(PI2_j),      PI2_j = sum_k gammaI_k (1 + e_jk^2)
+//   gz += i (d/dy dPhi/dt) gp:  -g u^-4 (van der Waals) or the expansion's derivative.
 // The RK4 step adjoint around it (stage recomputation, cotangent combination) is host code.
 #include "sfh_wg.hpp"
 
@@ -34,15 +45,16 @@ __device__ inline double mv_free_t(int n, int G, const PairConst& pc, BLds& L) {
     if (t < G * n) {
         const int j = t % n, g = t / n;
         const NodeTrig tj = L.tr[j];
-        double uy = 0.0;   // Im sum v_k Z'_k c_jk
+        double uy = 0.0, ui = 0.0;   // Im sum v_k Z'_k c_jk, and the symmetric image part Im sum v_k e_jk
         for (int k = g; k < n; k += G) {
             if (k == j) continue;
             const double vk = L.vs[k];
             const double2 zk = L.zp[k];
             const double2 ct = cot_pair(tj, L.tr[k]);
             uy += vk * (zk.x * ct.y + zk.y * ct.x);
+            if (pc.image) ui += vk * cot_image(tj, L.tr[k], pc.eh, pc.ehi).y;
         }
-        val = -pc.c * uy;
+        val = -pc.c * uy - pc.ci * ui;
     }
     double w = gsum(val, n, G, L);
     if (t < n) w += L.diag[t] * L.vs[t];
@@ -83,13 +95,21 @@ __global__ __launch_bounds__(1024) void rhs_vjp_kernel(VjpArgs a, Phys ph) {
         L.vs[t] = mu;
     }
     __syncthreads();
-    // pass A: S_k = sum mu_j c_kj,  P1_k = sum mu_j (1 + c_kj^2)
-    double sx = 0.0, sy = 0.0, p1x = 0.0, p1y = 0.0;
+    // pass A: S_k = sum mu_j c_kj,  P1_k = sum mu_j (1 + c_kj^2) (j != k); film: SI_k = sum mu_j e_kj,
+    // PI1_k = sum mu_j (1 + e_kj^2) over all j
+    double sx = 0.0, sy = 0.0, p1x = 0.0, p1y = 0.0, six = 0.0, siy = 0.0, q1x = 0.0, q1y = 0.0;
     if (act) {
         const NodeTrig tk = L.tr[kr];
         for (int j = grp; j < n; j += G) {
-            if (j == kr) continue;
             const double mj = L.vs[j];
+            if (pc.image) {
+                const double2 e = cot_image(tk, L.tr[j], pc.eh, pc.ehi);
+                six += mj * e.x;
+                siy += mj * e.y;
+                q1x += mj * (1.0 + e.x * e.x - e.y * e.y);
+                q1y += mj * (2.0 * e.x * e.y);
+            }
+            if (j == kr) continue;
             const double2 c = cot_pair(tk, L.tr[j]);
             sx += mj * c.x;
             sy += mj * c.y;
@@ -101,6 +121,12 @@ __global__ __launch_bounds__(1024) void rhs_vjp_kernel(VjpArgs a, Phys ph) {
     sy = gsum(sy, n, G, L);
     p1x = gsum(p1x, n, G, L);
     p1y = gsum(p1y, n, G, L);
+    if (pc.image) {
+        six = gsum(six, n, G, L);
+        siy = gsum(siy, n, G, L);
+        q1x = gsum(q1x, n, G, L);
+        q1y = gsum(q1y, n, G, L);
+    }
 
     // ------------------------------------------------ node-local adjoint of velocity + Bernoulli
     double2 gw = make_double2(0.0, 0.0);
@@ -110,7 +136,8 @@ __global__ __launch_bounds__(1024) void rhs_vjp_kernel(VjpArgs a, Phys ph) {
         const double2 izp = crecip(zp), izp2 = cmul(izp, izp);
         const double2 V2 = ctimes_i(cscale(izp, 1.0 / (2.0 * kPi)));
         const double2 V1 = cadd(ctimes_i(cscale(cmul(zpp, izp2), -q4)), cscale(izp, 0.5));
-        const double2 w = cadd(cadd(cmul(V2, mup), cscale(V1, mu)), ctimes_i(make_double2(-q4 * sx, -q4 * sy)));
+        double2 w = cadd(cadd(cmul(V2, mup), cscale(V1, mu)), ctimes_i(make_double2(-q4 * sx, -q4 * sy)));
+        if (pc.image) w = cadd(w, ctimes_i(make_double2(q4 * six, q4 * siy)));
         const double2 q = cconj(w);
         const double2 kq = a.gA_Z[off + t];
         double kp = a.gA_P[off + t];
@@ -134,7 +161,8 @@ __global__ __launch_bounds__(1024) void rhs_vjp_kernel(VjpArgs a, Phys ph) {
         AdjNode& A = an[t];
         A.gzp = gzp;
         A.gzpp = gzpp;
-        A.gz = make_double2(0.0, -ph.g * kp);          // -i g gp from the -g y term
+        A.gz = make_double2(0.0, bernoulli_dy(ph, nv.z.y) * kp);   // i (d dPhi/dt / dy) gp
+        A.pi1 = make_double2(q1x, q1y);
         A.gw = gw;
         A.p1 = make_double2(p1x, p1y);
         A.mu = mu;
@@ -145,13 +173,18 @@ __global__ __launch_bounds__(1024) void rhs_vjp_kernel(VjpArgs a, Phys ph) {
     // gmu += s Re(D1^H gmu')
     wg_first_derivative(d, L, true);
     if (own) gmu += sc * L.fa[lds_index(d, t)].x;
-    // pass B: P3_j = sum_k conj(gw_k) c_jk ;  gmu_j += Re((i/4pi) P3_j)
-    double p3x = 0.0, p3y = 0.0;
+    // pass B: P3_j = sum_k conj(gw_k) c_jk ;  gmu_j += Re((i/4pi) P3_j); film: Q_j = sum_k gw_k e_jk
+    // over all k, gmu_j += -Im(Q_j) / 4pi
+    double p3x = 0.0, p3y = 0.0, qy = 0.0;
     if (act) {
         const NodeTrig tj = L.tr[kr];
         for (int k = grp; k < n; k += G) {
+            const double2 wk = L.fb[k];   // conj(gw_k)
+            if (pc.image) {
+                const double2 e = cot_image(tj, L.tr[k], pc.eh, pc.ehi);
+                qy += wk.x * e.y - wk.y * e.x;   // Im(gw_k e_jk), gw_k = conj(wk)
+            }
             if (k == kr) continue;
-            const double2 wk = L.fb[k];
             const double2 c = cot_pair(tj, L.tr[k]);
             p3x += wk.x * c.x - wk.y * c.y;
             p3y += wk.x * c.y + wk.y * c.x;
@@ -159,7 +192,8 @@ __global__ __launch_bounds__(1024) void rhs_vjp_kernel(VjpArgs a, Phys ph) {
     }
     p3x = gsum(p3x, n, G, L);
     p3y = gsum(p3y, n, G, L);
-    if (own) gmu += -q4 * p3y;
+    if (pc.image) qy = gsum(qy, n, G, L);
+    if (own) gmu += -q4 * p3y - q4 * qy;
 
     // ------------------------------------------------ transpose solve M^T gb = gmu
     double gb = 0.0;
@@ -192,13 +226,26 @@ __global__ __launch_bounds__(1024) void rhs_vjp_kernel(VjpArgs a, Phys ph) {
         gz = cadd(gz, cconj(cmul(gam, A.p1)));
         L.fb[t] = gam;
         L.vs[t] = muk;
+        if (pc.image) {   // gammaI_k = (i/8pi) gw_k - (i ci/2) gb_k;  gz_k += gammaI_k conj(PI1_k)
+            const double2 gI = ctimes_i(csub(cscale(A.gw, 0.5 * q4), make_double2(0.5 * pc.ci * gb, 0.0)));
+            gz = cadd(gz, cmul(gI, cconj(A.pi1)));
+            L.fa[t] = gI;
+        }
     }
     __syncthreads();
-    // pass C: P2_j = sum_k gamma_k (1 + c_jk^2);  gz_j -= mu_j conj(P2_j)
+    // pass C: P2_j = sum_k gamma_k (1 + c_jk^2);  gz_j -= mu_j conj(P2_j); film: PI2_j = sum_k
+    // gammaI_k (1 + e_jk^2) over all k, gz_j -= mu_j conj(PI2_j)
     double p2x = 0.0, p2y = 0.0;
     if (act) {
         const NodeTrig tj = L.tr[kr];
         for (int k = grp; k < n; k += G) {
+            if (pc.image) {
+                const double2 gk = L.fa[k];
+                const double2 e = cot_image(tj, L.tr[k], pc.eh, pc.ehi);
+                const double ex = 1.0 + e.x * e.x - e.y * e.y, ey = 2.0 * e.x * e.y;
+                p2x += gk.x * ex - gk.y * ey;
+                p2y += gk.x * ey + gk.y * ex;
+            }
             if (k == kr) continue;
             const double2 gk = L.fb[k];
             const double2 c = cot_pair(tj, L.tr[k]);

@@ -208,6 +208,7 @@ __global__ void batched_stage_kernel(BatchArgs a, Phys ph, StageCtl ctl);
 // Per-node intermediates kept in global scratch across the transpose solve.
 struct AdjNode {
     double2 gzp, gzpp, gz, gw, p1, S;
+    double2 pi1;   // film: sum over all j of mu_j (1 + cot((z_k - conj z_j)/2 + i h)^2)
     double mu, pad;
 };
 struct VjpArgs {

@@ -37,6 +37,26 @@ __device__ inline double bernoulli(const Phys& ph, double y, double2 q1, double2
     return vdw / (u * u * u) - vdw + 0.5 * q1.x * q1.x + 0.5 * q1.y * q1.y;
 }
 
+// d(dPhi/dt)/dy of the Bernoulli right-hand side at height y, for the adjoint: water -(1 + rho) g,
+// the helium film's van der Waals term -g (1 + y/h)^-4 or the derivative of its expansion (the
+// curvature term of kappa != 0 is not covered).
+__device__ inline double bernoulli_dy(const Phys& ph, double y) {
+    if (ph.kind == 0) return -(1 + ph.rho) * ph.g;
+    const double h = ph.depth;
+    if (ph.use_exp) {
+        double v = 0.0;
+        switch (ph.exp_order) {
+        case 3: v += -10.0 * (y * y) / (h * h); [[fallthrough]];
+        case 2: v += 4.0 * y / h; [[fallthrough]];
+        case 1: v += -1.0; [[fallthrough]];
+        default: break;
+        }
+        return ph.g * v;
+    }
+    const double u = 1.0 + y / h;
+    return -ph.g / ((u * u) * (u * u));
+}
+
 // Velocities at node k from the cot sums over j != k
 //   S = sum_j mu_j cot((z_k - z_j)/2) = sx + i sy,  Si = sum_j mu_j cot((z_k - conj z_j)/2 + i h) = ix + i iy
 //   w = V2_k mu'_k + V1_kk mu_k + i (-1/4pi) S [+ i (1/4pi) Si]

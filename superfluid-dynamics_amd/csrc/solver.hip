@@ -1224,8 +1224,11 @@ void debug_lu_solve(int n, const double* A, const double* b, double* x, int* int
 // ------------------------------------------------------------------ reverse mode
 
 void Solver::ensure_vjp() {
-    if (ph_.kind != 0 || ph_.rho != 0.0 || ph_.image || ph_.drive)
-        throw ArgError("the adjoint is implemented for water with rho = 0 (no drive)");
+    const bool water_ok = ph_.kind == 0 && ph_.rho == 0.0;
+    const bool film_ok = ph_.kind == 1 && ph_.kappa == 0.0;
+    if (!(water_ok || film_ok) || ph_.drive)
+        throw ArgError("the adjoint is implemented for water with rho = 0 and the helium film with kappa = 0 "
+                       "(no drive)");
     if (maxN_ > kBatchMaxN) throw ArgError("the adjoint takes surfaces of at most 1024 nodes");
     if (vjp_ready_) return;
     for (int i = 0; i < 3; ++i) {

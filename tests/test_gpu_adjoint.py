@@ -1,4 +1,5 @@
-"""GPU checks of the hand-written adjoint (water, rho = 0) through the C ABI.
+"""GPU checks of the hand-written adjoint (water with rho = 0, the helium film with kappa = 0) through
+the C ABI.
 
 - right-hand side VJP: dot-product test <J^T l, d> = d/dh <l, f(y + h d)> against central
   differences of the CPU FP64 oracle (h = 1e-6, relative agreement 1e-7);
@@ -57,40 +58,47 @@ def _richardson(f, h=1e-5):
     return (4 * d2 - d1) / 3
 
 
-@pytest.mark.parametrize("sizes", [[64], [256, 96, 1024]])
-def test_rhs_vjp_dot_product(oracle, csh, sizes):
-    print(f"rhs vjp sizes {sizes}", flush=True)
+FILMS = [dict(kind=1, depth=1.0), dict(kind=1, depth=0.4), dict(kind=1, depth=0.7, infinite_depth=True),
+         dict(kind=1, depth=1.0, use_expansions=True, expansion_order=3), dict(kind=1, depth=1.0, g=0.15)]
+CASES = [([64], {}), ([256, 96, 1024], {})] + [([128], f) for f in FILMS] + [([256, 64, 512], FILMS[0])]
+
+
+@pytest.mark.parametrize("sizes,kw", CASES, ids=lambda v: "-".join(f"{k}{x}" for k, x in v.items()) if isinstance(v, dict) else str(v))
+def test_rhs_vjp_dot_product(oracle, csh, sizes, kw):
+    print(f"rhs vjp sizes {sizes} {kw}", flush=True)
     rng = np.random.default_rng(11)
     Zs = [_surface(n, 100 + n) for n in sizes]
     Ps = [0.02 * np.sin(2 * np.pi * np.arange(n) / n + 0.3) for n in sizes]
     lq = [rng.standard_normal(n) + 1j * rng.standard_normal(n) for n in sizes]
     lp = [rng.standard_normal(n) for n in sizes]
-    with csh.Ensemble(sizes, csh.Props(), solver=csh.BATCHED, tol=1e-13) as e:
+    with csh.Ensemble(sizes, csh.Props(**kw), solver=csh.BATCHED, tol=1e-13) as e:
         e.set_state(np.concatenate(Zs), np.concatenate(Ps))
         gZ, gP = e.rhs_vjp(np.concatenate(lq), np.concatenate(lp))
     o = 0
     for i, n in enumerate(sizes):
         dZ, dP = _smooth_direction(rng, n)
-        fd = _richardson(lambda h: _inner(lq[i], lp[i], *oracle.rhs(oracle.Props(), Zs[i] + h * dZ, Ps[i] + h * dP)))
+        fd = _richardson(lambda h: _inner(lq[i], lp[i], *oracle.rhs(oracle.Props(**kw), Zs[i] + h * dZ,
+                                                                      Ps[i] + h * dP)))
         ad = _inner(gZ[o:o + n], gP[o:o + n], dZ, dP)
         print(f"  n={n}: adjoint {ad:.12e}  fd {fd:.12e}", flush=True)
         assert abs(ad - fd) <= 1e-7 * max(abs(fd), 1e-3)
         o += n
 
 
-def test_step_vjp_dot_product(oracle, csh):
+@pytest.mark.parametrize("kw", [{}, FILMS[0]], ids=["water", "film"])
+def test_step_vjp_dot_product(oracle, csh, kw):
     n, dt = 128, 0.005
     rng = np.random.default_rng(5)
     Z, P = _surface(n, 7), 0.01 * np.cos(2 * np.pi * np.arange(n) / n)
     aZ = rng.standard_normal(n) + 1j * rng.standard_normal(n)
     aP = rng.standard_normal(n)
-    with csh.Ensemble([n], csh.Props(), solver=csh.BATCHED, tol=1e-13) as e:
+    with csh.Ensemble([n], csh.Props(**kw), solver=csh.BATCHED, tol=1e-13) as e:
         e.set_state(Z, P)
         gZ, gP = e.step_vjp(dt, aZ, aP)
         Zs, _ = e.get_state()
     assert np.array_equal(Zs, Z)   # the handle's state is left unchanged
     dZ, dP = _smooth_direction(rng, n)
-    fd = _richardson(lambda h: _inner(aZ, aP, *oracle.rk4(oracle.Props(), Z + h * dZ, P + h * dP, dt, 1)))
+    fd = _richardson(lambda h: _inner(aZ, aP, *oracle.rk4(oracle.Props(**kw), Z + h * dZ, P + h * dP, dt, 1)))
     ad = _inner(gZ, gP, dZ, dP)
     print(f"step vjp: adjoint {ad:.12e} fd {fd:.12e}", flush=True)
     assert abs(ad - fd) <= 1e-7 * abs(fd)
@@ -104,7 +112,7 @@ def _amplitude_problem(n, modes, seed):
     return x, c, basis, ct @ basis
 
 
-def _adjoint_gradient(csh, n, modes, seed, dt, steps, every):
+def _adjoint_gradient(csh, n, modes, seed, dt, steps, every, kw=None):
     import torch
     from cusuperhelium.adjoint import Trajectory, evolve
     x, c, basis, target = _amplitude_problem(n, modes, seed)
@@ -112,7 +120,7 @@ def _adjoint_gradient(csh, n, modes, seed, dt, steps, every):
     B = torch.tensor(basis)
     Z0 = torch.tensor(x, dtype=torch.complex128) + 1j * (ct @ B).to(torch.complex128)
     P0 = torch.zeros(n, dtype=torch.float64)
-    with Trajectory([n], checkpoint_every=every) as tr:
+    with Trajectory([n], csh.Props(**(kw or {})), checkpoint_every=every) as tr:
         ZT, _ = evolve(Z0, P0, tr, dt, steps)
         loss = 0.5 * torch.sum((ZT.imag - torch.tensor(target)) ** 2)
         loss.backward()
@@ -163,8 +171,30 @@ def test_differentiable_stepper_small_matches_oracle_fd(oracle, csh):
     assert rel <= 1e-6
 
 
-def test_adjoint_rejects_unsupported_physics(csh):
-    with csh.Ensemble([64], csh.Props(kind=1, depth=1.0), solver=csh.BATCHED) as e:
+def test_differentiable_film_small_matches_oracle_fd(oracle, csh):
+    # the same pipeline on the helium film (depth 1, image terms on) against central differences
+    # of the CPU FP64 oracle
+    kw = FILMS[0]
+    n, modes, seed, dt, steps = 64, 4, 78, 0.005, 20
+    x, c, basis, target, loss, grad = _adjoint_gradient(csh, n, modes, seed, dt, steps, 7, kw)
+    h = 1e-6
+    fd = np.empty(modes)
+    for k in range(modes):
+        Ls = []
+        for sgn in (+1, -1):
+            cc = c.copy()
+            cc[k] += sgn * h
+            ZT, _ = oracle.rk4(oracle.Props(**kw), x + 1j * (cc @ basis), np.zeros(n), dt, steps)
+            Ls.append(0.5 * np.sum((ZT.imag - target) ** 2))
+        fd[k] = (Ls[0] - Ls[1]) / (2 * h)
+    rel = np.abs(grad - fd).max() / np.abs(fd).max()
+    print(f"film oracle fd: grad {grad} fd {fd} rel {rel:.2e}", flush=True)
+    assert rel <= 1e-6
+
+
+@pytest.mark.parametrize("kw", [dict(kind=1, depth=1.0, kappa=0.01), dict(rho=0.3)])
+def test_adjoint_rejects_unsupported_physics(csh, kw):
+    with csh.Ensemble([64], csh.Props(**kw), solver=csh.BATCHED) as e:
         e.set_state(_surface(64, 1), np.zeros(64))
-        with pytest.raises(Exception):
+        with pytest.raises(Exception, match="adjoint is implemented"):
             e.rhs_vjp(np.ones(64, complex), np.ones(64))
