@@ -234,4 +234,40 @@ struct VjpArgs {
 };
 __global__ void rhs_vjp_kernel(VjpArgs a, Phys ph);
 
+// ---- adjoint_mwg.hip (the reverse mode over many workgroups per surface)
+struct AdjMArgs {
+    const SurfDesc* sd;
+    const NodeTrig* trig;      // half-angle table of the state
+    const double2* Z;          // the state, its Z', Z'', mu and mu'
+    const double2* Zp;
+    const double2* Zpp;
+    const double2* mup;
+    const double* mu;
+    const double2* gA_Z;       // input cotangent = ca * gA + cb * gB (gB may be null)
+    const double* gA_P;
+    const double2* gB_Z;
+    const double* gB_P;
+    double ca, cb;
+    AdjNode* node;             // per-node intermediates
+    double2* gwc;              // conj(gw)
+    double2* gmup;             // cotangent of mu'
+    double2* d1;               // D1 gmup (unscaled)
+    double* gmu;               // cotangent of mu: right-hand side of M^T gb = gmu
+    double* gb;                // its solution
+    double2 *gam, *gamI, *gzp, *gzpp, *gz;
+    double2* out_Z;
+    double* out_P;
+    double2* acc_Z;            // optional accumulator += out
+    double* acc_P;
+};
+constexpr int kAdjRows = 2;   // rows per workgroup of the reverse pair kernels
+template <int R> __global__ void adj_pass_a_kernel(AdjMArgs a, Phys ph);
+template <int R> __global__ void adj_pass_b_kernel(AdjMArgs a, Phys ph);
+template <int R> __global__ void adj_pass_c_kernel(AdjMArgs a, Phys ph);
+__global__ void adj_circ_d1_kernel(const SurfDesc* sd, const double* circ, const double2* v, double2* out);
+__global__ void adj_gm_start_kernel(const SurfDesc* sd, const double* b, double* V, double* x, int* ctl, int m);
+__global__ void transpose_kernel(const SurfDesc* sd, const double* M, double* Mt);
+__global__ void adj_gamma_kernel(AdjMArgs a, Phys ph);
+__global__ void adj_final_kernel(AdjMArgs a, const double* circ, const double2* sd1gb);
+
 }  // namespace sfh

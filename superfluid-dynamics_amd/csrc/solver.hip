@@ -229,30 +229,7 @@ Solver::Solver(const Phys& ph, const std::vector<int>& n, const Options& opt) : 
     SFH_HIP(hipMemset(P0_, 0, T_ * sizeof(double)));
     SFH_HIP(hipMemset(mu_, 0, T_ * sizeof(double)));
     SFH_HIP(hipMemset(mup_, 0, T_ * sizeof(double2)));
-    if (opt_.solver == kSolveGMRES) {
-        gm_m_ = opt_.max_iter;   // at most kMaxKrylov basis vectors (gmres.hip)
-        gm_nwg_ = (maxN_ + 3) / 4;   // kRowsPerWG in gmres.hip
-        gm_part_stride_ = 2LL * gm_nwg_ * (gm_m_ + 1);
-        gm_small_stride_ = 4LL * (gm_m_ + 1) + 2LL * gm_m_ + (long long)gm_m_ * (gm_m_ + 1);
-        gmV_ = dalloc<double>((size_t)(gm_m_ + 1) * T_);
-        gmx0_ = dalloc<double>(T_);
-        if (opt_.precision == kMixed) M32_ = dalloc<float>(MT_);
-        SFH_HIP(hipMemset(gmx0_, 0, T_ * sizeof(double)));
-        gmw_ = dalloc<double>(2 * (size_t)T_);
-        gmpart_ = dalloc<double>(gm_part_stride_ * S_);
-        gmsmall_ = dalloc<double>(gm_small_stride_ * S_);
-        gmctl_ = dalloc<int>((size_t)kGmCtlInts * S_);
-        SFH_HIP(hipMemset(gmctl_, 0, sizeof(int) * kGmCtlInts * S_));
-        SFH_HIP(hipMemset(gmsmall_, 0, sizeof(double) * gm_small_stride_ * S_));
-        gmhist_ = dalloc<double>(4 * kHistDepth * (size_t)T_);
-        gmhvalid_ = dalloc<int>(4);
-        SFH_HIP(hipMemset(gmhvalid_, 0, 4 * sizeof(int)));
-        ckZ_ = dalloc<double2>(T_);
-        ckP_ = dalloc<double>(T_);
-        ckMu_ = dalloc<double>(T_);
-        ckHist_ = dalloc<double>(4 * kHistDepth * (size_t)T_);
-        ckHv_ = dalloc<int>(4);
-    }
+    if (opt_.solver == kSolveGMRES) alloc_gmres(opt_.max_iter);
     if (opt_.solver == kSolveBatched) {
         std::vector<int> ord(S_);
         for (int s = 0; s < S_; ++s) ord[s] = s;
@@ -266,6 +243,34 @@ Solver::Solver(const Phys& ph, const std::vector<int>& n, const Options& opt) : 
         SFH_HIP(hipMemset(bhvalid_, 0, 4 * S_ * sizeof(int)));
     }
     SFH_HIP(hipDeviceSynchronize());
+}
+
+// GMRES buffers for a Krylov dimension of m (the GMRES solver, and the transposed solves of the
+// multi-workgroup adjoint of the other solvers)
+void Solver::alloc_gmres(int m) {
+    if (gmV_) return;
+    gm_m_ = m;   // at most kMaxKrylov basis vectors (gmres.hip)
+    gm_nwg_ = (maxN_ + 3) / 4;   // kRowsPerWG in gmres.hip
+    gm_part_stride_ = 2LL * gm_nwg_ * (gm_m_ + 1);
+    gm_small_stride_ = 4LL * (gm_m_ + 1) + 2LL * gm_m_ + (long long)gm_m_ * (gm_m_ + 1);
+    gmV_ = dalloc<double>((size_t)(gm_m_ + 1) * T_);
+    gmx0_ = dalloc<double>(T_);
+    if (opt_.precision == kMixed && opt_.solver == kSolveGMRES) M32_ = dalloc<float>(MT_);
+    SFH_HIP(hipMemset(gmx0_, 0, T_ * sizeof(double)));
+    gmw_ = dalloc<double>(2 * (size_t)T_);
+    gmpart_ = dalloc<double>(gm_part_stride_ * S_);
+    gmsmall_ = dalloc<double>(gm_small_stride_ * S_);
+    gmctl_ = dalloc<int>((size_t)kGmCtlInts * S_);
+    SFH_HIP(hipMemset(gmctl_, 0, sizeof(int) * kGmCtlInts * S_));
+    SFH_HIP(hipMemset(gmsmall_, 0, sizeof(double) * gm_small_stride_ * S_));
+    gmhist_ = dalloc<double>(4 * kHistDepth * (size_t)T_);
+    gmhvalid_ = dalloc<int>(4);
+    SFH_HIP(hipMemset(gmhvalid_, 0, 4 * sizeof(int)));
+    ckZ_ = dalloc<double2>(T_);
+    ckP_ = dalloc<double>(T_);
+    ckMu_ = dalloc<double>(T_);
+    ckHist_ = dalloc<double>(4 * kHistDepth * (size_t)T_);
+    ckHv_ = dalloc<int>(4);
 }
 
 Solver::~Solver() {
@@ -283,7 +288,8 @@ Solver::~Solver() {
         if (p) (void)hipFree(p);
     void* vbufs[] = {vY_[0], vY_[1], vY_[2], vK_[0], vK_[1], vK_[2], vYP_[0], vYP_[1], vYP_[2], vKP_[0],
                      vKP_[1], vKP_[2], vMu_[0], vMu_[1], vMu_[2], vMu_[3], gA_, gR_[0], gR_[1], gAcc_,
-                     gAP_, gRP_[0], gRP_[1], gAccP_, adj_, vstats_};
+                     gAP_, gRP_[0], gRP_[1], gAccP_, adj_, vstats_, Mt_, agwc_, agmup_, ad1_, agam_, agamI_,
+                     agzp_, agzpp_, agz_, agmu_, agb_};
     for (void* p : vbufs)
         if (p) (void)hipFree(p);
     for (hipEvent_t e : event_pool_) (void)hipEventDestroy(e);
@@ -375,8 +381,8 @@ GmArgs Solver::gm_args(int stage) const {
 // last one is gm_close_mu_kernel, which also forms mu' (returns true: no mu_circ_kernel).
 // g1 (a sequence of two launches): only the product launch; velocity_gm1_kernel finishes the solve.
 // Mixed precision runs the products over the FP32 matrix (4 bytes per entry).
-bool Solver::enqueue_gmres(const GmArgs& a, bool g1) {
-    const bool f32 = opt_.precision == kMixed;
+bool Solver::enqueue_gmres(const GmArgs& a, bool g1, bool force64) {
+    const bool f32 = opt_.precision == kMixed && !force64;
     double nsq = 0.0;
     for (int v : n_) nsq += (double)v * v;
     const dim3 grid(gm_nwg_, 1, S_);
@@ -1229,8 +1235,17 @@ void Solver::ensure_vjp() {
     if (!(water_ok || film_ok) || ph_.drive)
         throw ArgError("the adjoint is implemented for water with rho = 0 and the helium film with kappa = 0 "
                        "(no drive)");
-    if (maxN_ > kBatchMaxN) throw ArgError("the adjoint takes surfaces of at most 1024 nodes");
+    if (opt_.solver == kSolveBatched && maxN_ > kBatchMaxN)
+        throw ArgError("the batched adjoint takes surfaces of at most 1024 nodes");
     if (vjp_ready_) return;
+    if (opt_.solver != kSolveBatched) {   // the multi-workgroup reverse pass
+        alloc_gmres(kGmMaxIter);
+        Mt_ = dalloc<double>(MT_);
+        for (double2** b : {&agwc_, &agmup_, &ad1_, &agam_, &agamI_, &agzp_, &agzpp_, &agz_}) *b = dalloc<double2>(T_);
+        agmu_ = dalloc<double>(T_);
+        agb_ = dalloc<double>(T_);
+        SFH_HIP(hipMemset(agamI_, 0, T_ * sizeof(double2)));
+    }
     for (int i = 0; i < 3; ++i) {
         vY_[i] = dalloc<double2>(T_);
         vK_[i] = dalloc<double2>(T_);
@@ -1255,6 +1270,10 @@ void Solver::ensure_vjp() {
 void Solver::enqueue_vjp(const double2* Z, const double* P, const double* mu, const double2* gAZ, const double* gAP,
                          double ca, const double2* gBZ, const double* gBP, double cb, double2* outZ, double* outP,
                          bool accumulate) {
+    if (opt_.solver != kSolveBatched) {
+        enqueue_vjp_mwg(Z, P, mu, gAZ, gAP, ca, gBZ, gBP, cb, outZ, outP, accumulate);
+        return;
+    }
     VjpArgs a{};
     a.sd = d_desc_;
     a.tw_all = d_tw_;
@@ -1279,6 +1298,88 @@ void Solver::enqueue_vjp(const double2* Z, const double* P, const double* mu, co
     for (int v : n_) nsq += (double)v * v;
     launch("vjp", 3.0 * nsq, 0.0, [&] {
         go(rhs_vjp_kernel, dim3(S_), dim3(kBatchThreads), 0, a, ph_);
+    });
+    SFH_HIP(hipGetLastError());
+}
+
+// The reverse pass over many workgroups per surface (adjoint_mwg.hip), at the state (Z, P) with its
+// forward mu: the derivatives and mu' of the state, pass A, D1 of the cotangent of mu', pass B, the
+// transposed solve M^T gb = gmu (M re-assembled and transposed, the GMRES launch sequence from a zero
+// guess, FP64), the matrix terms and pair weights, pass C, and the spectral adjoints.
+void Solver::enqueue_vjp_mwg(const double2* Z, const double* P, const double* mu, const double2* gAZ,
+                             const double* gAP, double ca, const double2* gBZ, const double* gBP, double cb,
+                             double2* outZ, double* outP, bool accumulate) {
+    double nsq = 0.0;
+    for (int v : n_) nsq += (double)v * v;
+    const double img = ph_.image ? 2.0 : 1.0;
+    const dim3 cgrid((maxN_ + kCircRowsPerWG - 1) / kCircRowsPerWG, S_);
+    const dim3 pgrid((maxN_ + kAdjRows - 1) / kAdjRows, S_);
+    const dim3 wgrid((maxN_ + 3) / 4, S_);
+    const dim3 ngrid((maxN_ + 255) / 256, S_);
+    launch("vjp_fwd", 14.0 * nsq, 0.0, [&] {
+        go(zphi_circ_kernel, cgrid, dim3(256), 0, d_desc_, (const double*)d_circ_, Z, P, Zp_, Zpp_, b_, work_, ph_,
+           (int*)nullptr, trig_, GmArgs{}, 0);
+        go(mu_circ_kernel, cgrid, dim3(256), 0, d_desc_, (const double*)d_circ_, mu, mup_);
+    });
+    AdjMArgs a{};
+    a.sd = d_desc_;
+    a.trig = trig_;
+    a.Z = Z;
+    a.Zp = Zp_;
+    a.Zpp = Zpp_;
+    a.mup = mup_;
+    a.mu = mu;
+    a.gA_Z = gAZ;
+    a.gA_P = gAP;
+    a.gB_Z = gBZ;
+    a.gB_P = gBP;
+    a.ca = ca;
+    a.cb = cb;
+    a.node = adj_;
+    a.gwc = agwc_;
+    a.gmup = agmup_;
+    a.d1 = ad1_;
+    a.gmu = agmu_;
+    a.gb = agb_;
+    a.gam = agam_;
+    a.gamI = agamI_;
+    a.gzp = agzp_;
+    a.gzpp = agzpp_;
+    a.gz = agz_;
+    a.out_Z = outZ;
+    a.out_P = outP;
+    a.acc_Z = accumulate ? gAcc_ : nullptr;
+    a.acc_P = accumulate ? gAccP_ : nullptr;
+    launch("vjp_pairs", img * 3.0 * nsq, 0.0, [&] {
+        go(adj_pass_a_kernel<kAdjRows>, pgrid, dim3(256), 0, a, ph_);
+        go(adj_circ_d1_kernel, wgrid, dim3(256), 0, d_desc_, (const double*)d_circ_, (const double2*)agmup_, ad1_);
+        go(adj_pass_b_kernel<kAdjRows>, pgrid, dim3(256), 0, a, ph_);
+    });
+    // M^T gb = gmu
+    launch("vjp_solve", 2.0 * gm_m_ * nsq, 8.0 * gm_m_ * nsq, [&] {
+        go(assemble64_kernel<kPairRows>, dim3((maxN_ + kPairRows - 1) / kPairRows, S_), dim3(256), 0, d_desc_,
+           (const NodeTrig*)trig_, Z, Zp_, Zpp_, M_, ph_, (const int*)nullptr, GmArgs{}, 0);
+        go(transpose_kernel, dim3((maxN_ + 31) / 32, (maxN_ + 31) / 32, S_), dim3(256), 0, d_desc_, (const double*)M_,
+           Mt_);
+        go(adj_gm_start_kernel, ngrid, dim3(256), 0, d_desc_, (const double*)agmu_, gmV_, gmx0_, gmctl_, gm_m_);
+        GmArgs g = gm_args(0);
+        g.M = Mt_;
+        g.M32 = nullptr;
+        g.b = agmu_;
+        g.x = gmx0_;
+        g.sol = agb_;
+        g.m = gm_m_;
+        g.tol = std::min(opt_.tol, 1e-13);
+        g.hist = nullptr;
+        g.hvalid = nullptr;
+        g.stage = 0;
+        g.check_first = 0;
+        enqueue_gmres(g, false, true);   // leaves s D1 gb in mup_
+    });
+    launch("vjp_pairs", img * nsq, 0.0, [&] {
+        go(adj_gamma_kernel, ngrid, dim3(256), 0, a, ph_);
+        go(adj_pass_c_kernel<kAdjRows>, pgrid, dim3(256), 0, a, ph_);
+        go(adj_final_kernel, wgrid, dim3(256), 0, a, (const double*)d_circ_, (const double2*)mup_);
     });
     SFH_HIP(hipGetLastError());
 }
@@ -1345,6 +1446,7 @@ void Solver::step_vjp(double dt, const double* gx_in, const double* gy_in, const
     const double2* zin[4] = {Z0_, vY_[0], vY_[1], vY_[2]};
     const double* pin[4] = {P0_, vYP_[0], vYP_[1], vYP_[2]};
     const double alpha[3] = {0.5 * dt, 0.5 * dt, dt};
+    seq_len_ = gm_m_;   // standalone solves: the full GMRES launch sequence
     for (int st = 0; st < 4; ++st) {
         c.stage = st < 3 ? st + 1 : 0;
         c.alpha = st < 3 ? alpha[st] : 0.0;

@@ -249,7 +249,8 @@ class Solver {
     float* M32_ = nullptr;       // FP32 matrices of the mixed-precision GMRES form
     int* gmctl_ = nullptr;
     long long gm_part_stride_ = 0, gm_small_stride_ = 0;
-    bool enqueue_gmres(const GmArgs& a, bool g1 = false);
+    bool enqueue_gmres(const GmArgs& a, bool g1 = false, bool force64 = false);
+    void alloc_gmres(int m);
     GmArgs gm_args(int stage) const;
     double* gmhist_ = nullptr;   // 4 x kHistDepth x total: per RK stage, mu of the last steps
     int* gmhvalid_ = nullptr;
@@ -274,6 +275,14 @@ class Solver {
     double *gAP_ = nullptr, *gRP_[2] = {nullptr, nullptr}, *gAccP_ = nullptr;
     AdjNode* adj_ = nullptr;
     int* vstats_ = nullptr;
+    // multi-workgroup reverse mode (LU and GMRES handles): M^T and the per-node vectors of adjoint_mwg.hip
+    double* Mt_ = nullptr;
+    double2 *agwc_ = nullptr, *agmup_ = nullptr, *ad1_ = nullptr, *agam_ = nullptr, *agamI_ = nullptr;
+    double2 *agzp_ = nullptr, *agzpp_ = nullptr, *agz_ = nullptr;
+    double *agmu_ = nullptr, *agb_ = nullptr;
+    void enqueue_vjp_mwg(const double2* Z, const double* P, const double* mu, const double2* gAZ, const double* gAP,
+                         double ca, const double2* gBZ, const double* gBP, double cb, double2* outZ, double* outP,
+                         bool accumulate);
 
     hipGraph_t graph_ = nullptr;
     hipGraphExec_t exec_ = nullptr;

@@ -1,4 +1,4 @@
-"""Differentiable RK4 trajectories of the gfx950 stepper (water with rho = 0, the helium film with kappa = 0; every N <= 1024).
+"""Differentiable RK4 trajectories of the gfx950 stepper (water with rho = 0, the helium film with kappa = 0).
 
 ``evolve(Z0, Phi0, dt, steps, ...)`` is a ``torch.autograd.Function``: the forward pass advances
 the state on the GPU (the same stepper as ``Ensemble.step``) and keeps a checkpoint every
@@ -28,7 +28,12 @@ class Trajectory:
     """A GPU handle plus the checkpointing policy used by :func:`evolve`."""
 
     def __init__(self, sizes: Sequence[int] | int, props: Props | None = None, precision: int = FP64,
-                 solver: int = BATCHED, tol: float = 1e-13, max_iter: int = 48, checkpoint_every: int = 20):
+                 solver: int = BATCHED, tol: float = 1e-13, max_iter: int | None = None,
+                 checkpoint_every: int = 20):
+        # solver: BATCHED runs the reverse pass in one workgroup per surface (every N <= 1024), LU and
+        # GMRES over many workgroups per surface (single large surfaces)
+        if max_iter is None:
+            max_iter = 48 if solver == BATCHED else 32
         self.ens = Ensemble(sizes, props or Props(), precision=precision, solver=solver, tol=tol,
                             max_iter=max_iter)
         self.checkpoint_every = int(checkpoint_every)

@@ -104,6 +104,55 @@ def test_step_vjp_dot_product(oracle, csh, kw):
     assert abs(ad - fd) <= 1e-7 * abs(fd)
 
 
+MWG = [([1024], {}, "gmres"), ([768], {}, "lu"), ([1024], FILMS[0], "gmres"), ([512], FILMS[1], "lu"),
+       ([256, 256], FILMS[0], "gmres"), ([2048], {}, "gmres")]
+
+
+@pytest.mark.parametrize("sizes,kw,solver", MWG, ids=lambda v: str(v))
+def test_rhs_vjp_multi_workgroup_dot_product(oracle, csh, sizes, kw, solver):
+    # the reverse pass over many workgroups per surface (LU and GMRES handles) against central
+    # differences of the CPU oracle
+    print(f"multi-workgroup rhs vjp sizes {sizes} {kw} {solver}", flush=True)
+    rng = np.random.default_rng(13)
+    Zs = [_surface(n, 200 + n + i) for i, n in enumerate(sizes)]
+    Ps = [0.02 * np.sin(2 * np.pi * np.arange(n) / n + 0.3 + i) for i, n in enumerate(sizes)]
+    lq = [rng.standard_normal(n) + 1j * rng.standard_normal(n) for n in sizes]
+    lp = [rng.standard_normal(n) for n in sizes]
+    sv = csh.GMRES if solver == "gmres" else csh.LU
+    with csh.Ensemble(sizes, csh.Props(**kw), solver=sv, tol=1e-13) as e:
+        e.set_state(np.concatenate(Zs), np.concatenate(Ps))
+        gZ, gP = e.rhs_vjp(np.concatenate(lq), np.concatenate(lp))
+    o = 0
+    for i, n in enumerate(sizes):
+        dZ, dP = _smooth_direction(rng, n)
+        fd = _richardson(lambda h: _inner(lq[i], lp[i], *oracle.rhs(oracle.Props(**kw), Zs[i] + h * dZ,
+                                                                      Ps[i] + h * dP)))
+        ad = _inner(gZ[o:o + n], gP[o:o + n], dZ, dP)
+        print(f"  n={n}: adjoint {ad:.12e}  fd {fd:.12e}", flush=True)
+        assert abs(ad - fd) <= 1e-7 * max(abs(fd), 1e-3)
+        o += n
+
+
+@pytest.mark.parametrize("kw", [{}, FILMS[0]], ids=["water", "film"])
+def test_step_vjp_multi_workgroup_matches_batched(csh, kw):
+    # one RK4 step's VJP through the multi-workgroup pass (GMRES handle) against the one-workgroup
+    # pass (batched handle): the same cotangent to the solves' tolerance
+    n, dt = 512, 0.004
+    rng = np.random.default_rng(17)
+    Z, P = _surface(n, 19), 0.01 * np.cos(2 * np.pi * np.arange(n) / n)
+    aZ = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    aP = rng.standard_normal(n)
+    out = {}
+    for sv in (csh.BATCHED, csh.GMRES):
+        with csh.Ensemble([n], csh.Props(**kw), solver=sv, tol=1e-13) as e:
+            e.set_state(Z, P)
+            out[sv] = e.step_vjp(dt, aZ, aP)
+    for a, b in zip(out[csh.GMRES], out[csh.BATCHED]):
+        err = np.abs(a - b).max() / np.abs(b).max()
+        print(f"step vjp multi-workgroup vs batched: rel {err:.2e}", flush=True)
+        assert err < 1e-9
+
+
 def _amplitude_problem(n, modes, seed):
     x = 2 * np.pi * np.arange(n) / n
     c = np.random.default_rng(seed).uniform(0, 0.01, modes)
