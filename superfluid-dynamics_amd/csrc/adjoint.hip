@@ -32,7 +32,8 @@
 //   gammaI_k = (i/8pi) gw_k - (i ci/2) gb_k
 //   gz_k += gammaI_k conj(PI1_k),  PI1_k = sum_j mu_j (1 + e_kj^2)
 //   gz_j -= mu_j conj(PI2_j),      PI2_j = sum_k gammaI_k (1 + e_jk^2)
-//   gz += i (d/dy dPhi/dt) gp:  -g u^-4 (van der Waals) or the expansion's derivative.
+//   gz += i (d/dy dPhi/dt) gp:  -g u^-4 (van der Waals) or the expansion's derivative; the
+//   surface-tension term kappa (x'y'' - y'x'')/|Z'|^3 adds its Z' and Z'' derivatives to gZ', gZ''.
 // The RK4 step adjoint around it (stage recomputation, cotangent combination) is host code.
 #include "sfh_wg.hpp"
 
@@ -156,7 +157,8 @@ __global__ __launch_bounds__(1024) void rhs_vjp_kernel(VjpArgs a, Phys ph) {
         const double2 dV1pp = ctimes_i(cscale(izp2, -q4));             // dV1/dZ''
         const double2 gVw = cscale(gw, mu);
         double2 gzp = cadd(cmul(cconj(dV2), cmul(cconj(mup), gw)), cmul(cconj(dV1p), gVw));
-        const double2 gzpp = cmul(cconj(dV1pp), gVw);
+        double2 gzpp = cmul(cconj(dV1pp), gVw);
+        curvature_vjp(ph, zp, zpp, kp, gzp, gzpp);
         gmu = cmul(cconj(V1), gw).x;
         AdjNode& A = an[t];
         A.gzp = gzp;

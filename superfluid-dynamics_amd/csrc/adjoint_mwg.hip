@@ -104,6 +104,7 @@ __global__ __launch_bounds__(256) void adj_pass_a_kernel(AdjMArgs a, Phys ph) {
     AdjNode A;
     A.gzp = cadd(cmul(cconj(dV2), cmul(cconj(mup), gw)), cmul(cconj(dV1p), gVw));
     A.gzpp = cmul(cconj(dV1pp), gVw);
+    curvature_vjp(ph, zp, zpp, kp, A.gzp, A.gzpp);
     A.gz = make_double2(0.0, bernoulli_dy(ph, a.Z[g].y) * kp);
     A.gw = gw;
     A.p1 = make_double2(sv[2], sv[3]);

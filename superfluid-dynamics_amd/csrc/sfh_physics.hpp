@@ -38,8 +38,8 @@ __device__ inline double bernoulli(const Phys& ph, double y, double2 q1, double2
 }
 
 // d(dPhi/dt)/dy of the Bernoulli right-hand side at height y, for the adjoint: water -(1 + rho) g,
-// the helium film's van der Waals term -g (1 + y/h)^-4 or the derivative of its expansion (the
-// curvature term of kappa != 0 is not covered).
+// the helium film's van der Waals term -g (1 + y/h)^-4 (-20.447761896665416 g (1 + y/h)^-4 in the
+// branch with surface tension) or the derivative of its expansion.
 __device__ inline double bernoulli_dy(const Phys& ph, double y) {
     if (ph.kind == 0) return -(1 + ph.rho) * ph.g;
     const double h = ph.depth;
@@ -54,7 +54,21 @@ __device__ inline double bernoulli_dy(const Phys& ph, double y) {
         return ph.g * v;
     }
     const double u = 1.0 + y / h;
-    return -ph.g / ((u * u) * (u * u));
+    const double a = ph.kappa != 0.0 ? 20.447761896665416 : 1.0;
+    return -a * ph.g / ((u * u) * (u * u));
+}
+
+// The surface-tension term kappa curv, curv = (x' y'' - y' x'') / |Z'|^3, of the helium Bernoulli
+// right-hand side: its cotangents w.r.t. Z' and Z'' (g = d/dRe + i d/dIm) for a cotangent kp of
+// dPhi/dt, added to gzp and gzpp (nothing without the term).
+__device__ inline void curvature_vjp(const Phys& ph, double2 zp, double2 zpp, double kp, double2& gzp, double2& gzpp) {
+    if (ph.kind == 0 || ph.use_exp || ph.kappa == 0.0) return;
+    const double s = zp.x * zp.x + zp.y * zp.y;
+    const double r3 = 1.0 / (s * sqrt(s)), r5 = r3 / s;
+    const double cr = zp.x * zpp.y - zp.y * zpp.x;
+    const double w = ph.kappa * kp;
+    gzpp = cadd(gzpp, make_double2(-zp.y * r3 * w, zp.x * r3 * w));
+    gzp = cadd(gzp, make_double2((zpp.y * r3 - 3.0 * cr * zp.x * r5) * w, (-zpp.x * r3 - 3.0 * cr * zp.y * r5) * w));
 }
 
 // Velocities at node k from the cot sums over j != k

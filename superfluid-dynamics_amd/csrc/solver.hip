@@ -1231,10 +1231,9 @@ void debug_lu_solve(int n, const double* A, const double* b, double* x, int* int
 
 void Solver::ensure_vjp() {
     const bool water_ok = ph_.kind == 0 && ph_.rho == 0.0;
-    const bool film_ok = ph_.kind == 1 && ph_.kappa == 0.0;
+    const bool film_ok = ph_.kind == 1;
     if (!(water_ok || film_ok) || ph_.drive)
-        throw ArgError("the adjoint is implemented for water with rho = 0 and the helium film with kappa = 0 "
-                       "(no drive)");
+        throw ArgError("the adjoint is implemented for water with rho = 0 and the helium film (no drive)");
     if (opt_.solver == kSolveBatched && maxN_ > kBatchMaxN)
         throw ArgError("the batched adjoint takes surfaces of at most 1024 nodes");
     if (vjp_ready_) return;

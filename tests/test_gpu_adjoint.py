@@ -1,5 +1,4 @@
-"""GPU checks of the hand-written adjoint (water with rho = 0, the helium film with kappa = 0) through
-the C ABI.
+"""GPU checks of the hand-written adjoint (water with rho = 0, the helium film) through the C ABI.
 
 - right-hand side VJP: dot-product test <J^T l, d> = d/dh <l, f(y + h d)> against central
   differences of the CPU FP64 oracle (h = 1e-6, relative agreement 1e-7);
@@ -59,7 +58,8 @@ def _richardson(f, h=1e-5):
 
 
 FILMS = [dict(kind=1, depth=1.0), dict(kind=1, depth=0.4), dict(kind=1, depth=0.7, infinite_depth=True),
-         dict(kind=1, depth=1.0, use_expansions=True, expansion_order=3), dict(kind=1, depth=1.0, g=0.15)]
+         dict(kind=1, depth=1.0, use_expansions=True, expansion_order=3), dict(kind=1, depth=1.0, g=0.15),
+         dict(kind=1, depth=1.0, kappa=0.01)]
 CASES = [([64], {}), ([256, 96, 1024], {})] + [([128], f) for f in FILMS] + [([256, 64, 512], FILMS[0])]
 
 
@@ -105,7 +105,7 @@ def test_step_vjp_dot_product(oracle, csh, kw):
 
 
 MWG = [([1024], {}, "gmres"), ([768], {}, "lu"), ([1024], FILMS[0], "gmres"), ([512], FILMS[1], "lu"),
-       ([256, 256], FILMS[0], "gmres"), ([2048], {}, "gmres")]
+       ([256, 256], FILMS[0], "gmres"), ([2048], {}, "gmres"), ([512], FILMS[5], "gmres")]
 
 
 @pytest.mark.parametrize("sizes,kw,solver", MWG, ids=lambda v: str(v))
@@ -241,9 +241,11 @@ def test_differentiable_film_small_matches_oracle_fd(oracle, csh):
     assert rel <= 1e-6
 
 
-@pytest.mark.parametrize("kw", [dict(kind=1, depth=1.0, kappa=0.01), dict(rho=0.3)])
-def test_adjoint_rejects_unsupported_physics(csh, kw):
+@pytest.mark.parametrize("kw,drive", [(dict(kind=1, depth=1.0), True), (dict(rho=0.3), False)])
+def test_adjoint_rejects_unsupported_physics(csh, kw, drive):
     with csh.Ensemble([64], csh.Props(**kw), solver=csh.BATCHED) as e:
+        if drive:
+            e.set_drive(damping=0.1)
         e.set_state(_surface(64, 1), np.zeros(64))
         with pytest.raises(Exception, match="adjoint is implemented"):
             e.rhs_vjp(np.ones(64, complex), np.ones(64))
