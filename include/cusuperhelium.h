@@ -380,6 +380,21 @@ int sfh_rhs_vjp(sfh_handle h, const double* gvx, const double* gvy, const double
                 double* gphi);
 int sfh_step_vjp(sfh_handle h, double dt, const double* gx_in, const double* gy_in, const double* gphi_in,
                  double* gx_out, double* gy_out, double* gphi_out);
+/* Differentiable trajectories with device-resident checkpoints (no counterpart in the reference; the
+ * forward map is sfh_step).  sfh_traj_forward advances `steps` RK4 steps of size dt from the current
+ * state exactly as sfh_step(h, dt, ...) does, keeps in device memory the state before steps 0, every,
+ * 2 every, ... (ceil(steps / every) states of 24 T bytes, T = sfh_total_nodes) and returns the id of
+ * that record in *record; the handle's state afterwards is the final state.  sfh_traj_backward takes
+ * the cotangent (gx, gy, gphi) of the state after the recorded steps to the cotangent of the state the
+ * record started from: each segment, last first, is re-run from its checkpoint into a device buffer
+ * of `every` states and swept back one step VJP (sfh_step_vjp) at a time, the cotangent staying in
+ * device memory; the handle's state afterwards is the record's initial state.  A record can be swept
+ * back any number of times until sfh_traj_release frees it (sfh_destroy frees every record).  Same
+ * problem scope as sfh_step_vjp (-3 otherwise, before any step); -3 for an unknown record. */
+int sfh_traj_forward(sfh_handle h, double dt, int steps, int checkpoint_every, int* record);
+int sfh_traj_backward(sfh_handle h, int record, const double* gx_in, const double* gy_in, const double* gphi_in,
+                      double* gx_out, double* gy_out, double* gphi_out);
+int sfh_traj_release(sfh_handle h, int record);
 /* Energies (EnergyContainer and VolumeFlux, Energies.cuh:136-204, computed by the reference on
  * every right-hand side, BaseBoundaryIntegrator.cuh:270-272).  While tracking is enabled every
  * right-hand side also writes its per-node energy terms (32 B per node; the step graph is

@@ -719,6 +719,24 @@ int sfh_step_vjp(sfh_handle h, double dt, const double* gx_in, const double* gy_
     return guarded([&] { h->s->step_vjp(dt, gx_in, gy_in, gphi_in, gx_out, gy_out, gphi_out); });
 }
 
+int sfh_traj_forward(sfh_handle h, double dt, int steps, int checkpoint_every, int* record) {
+    SFH_NEED(h);
+    if (!record) return -3;
+    return guarded([&] { *record = h->s->traj_forward(dt, steps, checkpoint_every); });
+}
+
+int sfh_traj_backward(sfh_handle h, int record, const double* gx_in, const double* gy_in, const double* gphi_in,
+                      double* gx_out, double* gy_out, double* gphi_out) {
+    SFH_NEED(h);
+    if (!gx_in || !gy_in || !gphi_in || !gx_out || !gy_out || !gphi_out) return -3;
+    return guarded([&] { h->s->traj_backward(record, gx_in, gy_in, gphi_in, gx_out, gy_out, gphi_out); });
+}
+
+int sfh_traj_release(sfh_handle h, int record) {
+    SFH_NEED(h);
+    return guarded([&] { h->s->traj_release(record); });
+}
+
 int sfh_get_mu(sfh_handle h, double* mu) {
     SFH_NEED(h);
     return guarded([&] { h->s->get_mu(mu); });

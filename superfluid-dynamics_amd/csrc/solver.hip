@@ -292,6 +292,12 @@ Solver::~Solver() {
                      agzp_, agzpp_, agz_, agmu_, agb_};
     for (void* p : vbufs)
         if (p) (void)hipFree(p);
+    for (auto& kv : traj_) {
+        (void)hipFree(kv.second.Z);
+        (void)hipFree(kv.second.P);
+    }
+    if (segZ_) (void)hipFree(segZ_);
+    if (segP_) (void)hipFree(segP_);
     for (hipEvent_t e : event_pool_) (void)hipEventDestroy(e);
     if (rec_stream_) (void)hipStreamSynchronize(rec_stream_);
     for (hipEvent_t e : rec_packed_) (void)hipEventDestroy(e);
@@ -1432,6 +1438,12 @@ void Solver::step_vjp(double dt, const double* gx_in, const double* gy_in, const
     ensure_vjp();
     SFH_HIP(hipStreamSynchronize(stream_));
     upload_cotangent(gx_in, gy_in, gp_in, gA_, gAP_);
+    enqueue_step_vjp(dt);
+    download_cotangent(gAcc_, gAccP_, gx_out, gy_out, gp_out);
+    check_status();
+}
+
+void Solver::enqueue_step_vjp(double dt) {
     SFH_HIP(hipMemcpyAsync(gAcc_, gA_, T_ * sizeof(double2), hipMemcpyDeviceToDevice, stream_));
     SFH_HIP(hipMemcpyAsync(gAccP_, gAP_, T_ * sizeof(double), hipMemcpyDeviceToDevice, stream_));
     // stage states and their mu
@@ -1459,8 +1471,104 @@ void Solver::step_vjp(double dt, const double* gx_in, const double* gy_in, const
     enqueue_vjp(vY_[1], vYP_[1], vMu_[2], gA_, gAP_, dt / 3.0, gR_[0], gRP_[0], dt, gR_[1], gRP_[1], true);
     enqueue_vjp(vY_[0], vYP_[0], vMu_[1], gA_, gAP_, dt / 3.0, gR_[1], gRP_[1], 0.5 * dt, gR_[0], gRP_[0], true);
     enqueue_vjp(Z0_, P0_, vMu_[0], gA_, gAP_, dt / 6.0, gR_[0], gRP_[0], 0.5 * dt, gR_[1], gRP_[1], true);
-    download_cotangent(gAcc_, gAccP_, gx_out, gy_out, gp_out);
+}
+
+// The same resets as set_state (a new trajectory), from states already in device memory.
+void Solver::load_state(const double2* Z, const double* P) {
+    clear_solver_state();
+    SFH_HIP(hipMemcpyAsync(Z0_, Z, T_ * sizeof(double2), hipMemcpyDeviceToDevice, stream_));
+    SFH_HIP(hipMemcpyAsync(P0_, P, T_ * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+    if (gmhvalid_) SFH_HIP(hipMemsetAsync(gmhvalid_, 0, 4 * sizeof(int), stream_));
+    hist_steps_ = 0;
+    time_ = drive_t_ = drive_prev_ = 0.0;
+    if (D0_) SFH_HIP(hipMemsetAsync(D0_, 0, T_ * sizeof(double), stream_));
+    if (bhvalid_) SFH_HIP(hipMemsetAsync(bhvalid_, 0, 4 * S_ * sizeof(int), stream_));
+}
+
+int Solver::traj_forward(double dt, int steps, int every) {
+    if (steps < 1 || every < 1) throw ArgError("a trajectory needs steps >= 1 and checkpoint_every >= 1");
+    ensure_vjp();   // the problem must have a reverse pass: rejected before any step
+    const int nck = (steps + every - 1) / every;
+    TrajRecord r;
+    r.dt = dt;
+    r.steps = steps;
+    r.every = every;
+    r.Z = dalloc<double2>((size_t)nck * T_);
+    try {
+        r.P = dalloc<double>((size_t)nck * T_);
+    } catch (...) {
+        (void)hipFree(r.Z);
+        throw;
+    }
+    const int id = traj_next_++;
+    traj_[id] = r;
+    try {
+        for (int c = 0; c < nck; ++c) {
+            SFH_HIP(hipMemcpyAsync(r.Z + (size_t)c * T_, Z0_, T_ * sizeof(double2), hipMemcpyDeviceToDevice, stream_));
+            SFH_HIP(hipMemcpyAsync(r.P + (size_t)c * T_, P0_, T_ * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+            step(dt, std::min(every, steps - c * every));
+        }
+        check_status();
+    } catch (...) {
+        traj_release(id);
+        throw;
+    }
+    return id;
+}
+
+void Solver::traj_backward(int record, const double* gx_in, const double* gy_in, const double* gp_in,
+                           double* gx_out, double* gy_out, double* gp_out) {
+    auto it = traj_.find(record);
+    if (it == traj_.end()) throw ArgError("unknown trajectory record " + std::to_string(record));
+    const TrajRecord r = it->second;
+    ensure_vjp();
+    const int seg_len = std::min(r.every, r.steps);
+    if (seg_len > seg_cap_) {
+        SFH_HIP(hipStreamSynchronize(stream_));
+        if (segZ_) SFH_HIP(hipFree(segZ_));
+        if (segP_) SFH_HIP(hipFree(segP_));
+        segZ_ = nullptr;
+        segP_ = nullptr;
+        seg_cap_ = 0;
+        segZ_ = dalloc<double2>((size_t)seg_len * T_);
+        segP_ = dalloc<double>((size_t)seg_len * T_);
+        seg_cap_ = seg_len;
+    }
+    SFH_HIP(hipStreamSynchronize(stream_));
+    upload_cotangent(gx_in, gy_in, gp_in, gA_, gAP_);
+    const int nck = (r.steps + r.every - 1) / r.every;
+    const size_t zb = T_ * sizeof(double2), pb = T_ * sizeof(double);
+    for (int seg = nck - 1; seg >= 0; --seg) {
+        const int n = std::min(r.every, r.steps - seg * r.every);
+        // y_s, ..., y_{s+n-1}: the checkpoint, then n - 1 steps from it
+        const double2* ckZ = r.Z + (size_t)seg * T_;
+        const double* ckP = r.P + (size_t)seg * T_;
+        SFH_HIP(hipMemcpyAsync(segZ_, ckZ, zb, hipMemcpyDeviceToDevice, stream_));
+        SFH_HIP(hipMemcpyAsync(segP_, ckP, pb, hipMemcpyDeviceToDevice, stream_));
+        load_state(ckZ, ckP);
+        for (int k = 1; k < n; ++k) {
+            step(r.dt, 1);
+            SFH_HIP(hipMemcpyAsync(segZ_ + (size_t)k * T_, Z0_, zb, hipMemcpyDeviceToDevice, stream_));
+            SFH_HIP(hipMemcpyAsync(segP_ + (size_t)k * T_, P0_, pb, hipMemcpyDeviceToDevice, stream_));
+        }
+        for (int i = n - 1; i >= 0; --i) {
+            load_state(segZ_ + (size_t)i * T_, segP_ + (size_t)i * T_);
+            enqueue_step_vjp(r.dt);
+            SFH_HIP(hipMemcpyAsync(gA_, gAcc_, zb, hipMemcpyDeviceToDevice, stream_));
+            SFH_HIP(hipMemcpyAsync(gAP_, gAccP_, pb, hipMemcpyDeviceToDevice, stream_));
+        }
+    }
+    download_cotangent(gA_, gAP_, gx_out, gy_out, gp_out);
     check_status();
+}
+
+void Solver::traj_release(int record) {
+    auto it = traj_.find(record);
+    if (it == traj_.end()) throw ArgError("unknown trajectory record " + std::to_string(record));
+    SFH_HIP(hipStreamSynchronize(stream_));
+    SFH_HIP(hipFree(it->second.Z));
+    SFH_HIP(hipFree(it->second.P));
+    traj_.erase(it);
 }
 
 }  // namespace sfh

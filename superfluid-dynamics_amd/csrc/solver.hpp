@@ -6,6 +6,7 @@
 // one Solver owns S surfaces of arbitrary sizes, allocates everything once, and advances the
 // state in HBM without host round trips.
 #pragma once
+#include <map>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -165,6 +166,18 @@ class Solver {
     void step_vjp(double dt, const double* gx_in, const double* gy_in, const double* gp_in, double* gx_out,
                   double* gy_out, double* gp_out);
 
+    // Differentiable trajectories with device-resident checkpoints.  traj_forward advances `steps`
+    // RK4 steps of size dt from the current state exactly as step() does, keeping in device memory the
+    // state before steps 0, every, 2 every, ...; it returns the id of that record.  traj_backward takes
+    // the cotangent of the state after the recorded steps to the cotangent of the state it started
+    // from: each segment, last first, is re-run from its checkpoint into a device buffer of `every`
+    // states, then swept back one step_vjp at a time with the cotangent kept on the device.  The
+    // handle's state afterwards is the record's initial state.  Host arrays of total_nodes() entries.
+    int traj_forward(double dt, int steps, int every);
+    void traj_backward(int record, const double* gx_in, const double* gy_in, const double* gp_in, double* gx_out,
+                       double* gy_out, double* gp_out);
+    void traj_release(int record);
+
     // Energies (EnergyContainer + VolumeFlux, Energies.cuh:136-204): while tracking is on, every
     // right-hand side also writes its per-node energy terms; energies() returns, per surface,
     // {kinetic, potential, surface, volume flux} of the last right-hand side evaluated.
@@ -283,6 +296,21 @@ class Solver {
     void enqueue_vjp_mwg(const double2* Z, const double* P, const double* mu, const double2* gAZ, const double* gAP,
                          double ca, const double2* gBZ, const double* gBP, double cb, double2* outZ, double* outP,
                          bool accumulate);
+    // one RK4 step back at the state Z0_/P0_: cotangent gA_/gAP_ of the state after it -> gAcc_/gAccP_
+    void enqueue_step_vjp(double dt);
+    // the state from device buffers, with the resets of set_state (no host copy, no wait)
+    void load_state(const double2* Z, const double* P);
+    struct TrajRecord {
+        double dt = 0.0;
+        int steps = 0, every = 1;
+        double2* Z = nullptr;   // checkpoint c at Z + c T
+        double* P = nullptr;
+    };
+    std::map<int, TrajRecord> traj_;
+    int traj_next_ = 1;
+    double2* segZ_ = nullptr;   // the states of one segment
+    double* segP_ = nullptr;
+    int seg_cap_ = 0;
 
     hipGraph_t graph_ = nullptr;
     hipGraphExec_t exec_ = nullptr;

@@ -1,11 +1,13 @@
 """Differentiable RK4 trajectories of the gfx950 stepper (water with rho = 0, the helium film).
 
 ``evolve(Z0, Phi0, dt, steps, ...)`` is a ``torch.autograd.Function``: the forward pass advances
-the state on the GPU (the same stepper as ``Ensemble.step``) and keeps a checkpoint every
-``checkpoint_every`` steps; the backward pass re-runs each segment from its checkpoint to recover
-the per-step states and propagates the cotangent back one RK4 step at a time with the hand-written
-adjoint kernels (``sfh_step_vjp``: stage recomputation, four right-hand-side VJPs with a
-transpose solve each).  Memory: one state per checkpoint plus one segment of states.
+the state on the GPU (the same stepper as ``Ensemble.step``) and keeps a device-resident checkpoint
+every ``checkpoint_every`` steps (``sfh_traj_forward``); the backward pass (``sfh_traj_backward``)
+re-runs each segment from its checkpoint into a device buffer and propagates the cotangent back one
+RK4 step at a time with the hand-written adjoint kernels (stage recomputation, four right-hand-side
+VJPs with a transpose solve each), the states and the cotangent staying in device memory: the host
+sends the initial state and the final cotangent once and receives the final state and the gradient
+once.  Device memory: one state per checkpoint plus one segment of states.
 
 The reference has no adjoint; the forward map is AutonomousRungeKuttaStepper::runStep over
 BaseBoundaryIntegralCalculator::runTimeStep (CuSuperHelium/AutonomousRungeKuttaStepper.cuh:338-375,
@@ -22,6 +24,24 @@ import torch
 from .stepper import BATCHED, FP64, Ensemble, Props
 
 __all__ = ["evolve", "Trajectory"]
+
+
+class _Record:
+    """A trajectory record held by the library (device checkpoints); freed with this object."""
+
+    def __init__(self, ens: Ensemble, rid: int):
+        self.ens, self.id = ens, rid
+
+    def release(self):
+        if self.id is not None:
+            rid, self.id = self.id, None
+            self.ens.traj_release(rid)
+
+    def __del__(self):
+        try:
+            self.release()
+        except Exception:
+            pass
 
 
 class Trajectory:
@@ -50,47 +70,33 @@ class Trajectory:
         self.close()
 
     def forward(self, Z0: np.ndarray, P0: np.ndarray, dt: float, steps: int):
-        ck = []
+        """(Z(T), Phi(T), record): the record keeps the checkpoints in device memory."""
         self.ens.set_state(Z0, P0)
-        done = 0
-        while done < steps:
-            ck.append(self.ens.get_state())
-            n = min(self.checkpoint_every, steps - done)
-            self.ens.step(dt, n)
-            done += n
+        rec = _Record(self.ens, self.ens.traj_forward(dt, steps, self.checkpoint_every))
         Z, P = self.ens.get_state()
-        return Z, P, ck
+        return Z, P, rec
 
-    def backward(self, ck, dt: float, steps: int, gZ: np.ndarray, gP: np.ndarray):
-        gZ = np.array(gZ, dtype=np.complex128)
-        gP = np.array(gP, dtype=np.float64)
-        starts = list(range(0, steps, self.checkpoint_every))
-        for seg in range(len(starts) - 1, -1, -1):
-            n = min(self.checkpoint_every, steps - starts[seg])
-            # states y_{s}, ..., y_{s+n-1} of the segment
-            self.ens.set_state(*ck[seg])
-            states = [ck[seg]]
-            for _ in range(n - 1):
-                self.ens.step(dt, 1)
-                states.append(self.ens.get_state())
-            for i in range(n - 1, -1, -1):
-                self.ens.set_state(*states[i])
-                gZ, gP = self.ens.step_vjp(dt, gZ, gP)
-        return gZ, gP
+    def backward(self, rec: _Record, gZ: np.ndarray, gP: np.ndarray):
+        """Cotangent of (Z0, Phi0) from the cotangent of (Z(T), Phi(T))."""
+        if rec.id is None:
+            raise RuntimeError("the trajectory record was released")
+        return self.ens.traj_backward(rec.id, gZ, gP)
 
 
 class _Evolve(torch.autograd.Function):
     @staticmethod
     def forward(ctx, Z0, P0, traj, dt, steps):
-        Zf, Pf, ck = traj.forward(Z0.detach().cpu().numpy(), P0.detach().cpu().numpy(), dt, steps)
-        ctx.traj, ctx.dt, ctx.steps, ctx.ck = traj, dt, steps, ck
+        if steps < 1:
+            raise ValueError("evolve needs steps >= 1")
+        Zf, Pf, rec = traj.forward(Z0.detach().cpu().numpy(), P0.detach().cpu().numpy(), dt, steps)
+        ctx.traj, ctx.rec = traj, rec
         return torch.from_numpy(Zf).to(Z0.device), torch.from_numpy(Pf).to(P0.device)
 
     @staticmethod
     def backward(ctx, gZ, gP):
         gZn = np.zeros(ctx.traj.ens.total, np.complex128) if gZ is None else gZ.detach().cpu().numpy()
         gPn = np.zeros(ctx.traj.ens.total) if gP is None else gP.detach().cpu().numpy()
-        gZ0, gP0 = ctx.traj.backward(ctx.ck, ctx.dt, ctx.steps, gZn, gPn)
+        gZ0, gP0 = ctx.traj.backward(ctx.rec, gZn, gPn)
         dev = gZ.device if gZ is not None else gP.device
         return torch.from_numpy(gZ0).to(dev), torch.from_numpy(gP0).to(dev), None, None, None
 

@@ -193,6 +193,29 @@ class Ensemble:
               "sfh_step_vjp")
         return gx + 1j * gy, gp
 
+    def traj_forward(self, dt: float, steps: int, checkpoint_every: int) -> int:
+        """Advance `steps` RK4 steps like step(), keeping a device-resident checkpoint every
+        `checkpoint_every` steps; returns the record id for traj_backward."""
+        rec = c_int(0)
+        check(self._lib.sfh_traj_forward(self._h, float(dt), int(steps), int(checkpoint_every), byref(rec)),
+              "sfh_traj_forward")
+        return rec.value
+
+    def traj_backward(self, record: int, gZ, gPhi):
+        """Cotangent of a record's initial state from the cotangent of its final state (the whole
+        checkpointed reverse sweep runs in the library, the cotangent in device memory)."""
+        gZ = np.asarray(gZ, dtype=np.complex128).ravel()
+        gxi, gyi = np.ascontiguousarray(gZ.real), np.ascontiguousarray(gZ.imag)
+        gpi = np.ascontiguousarray(gPhi, dtype=np.float64).ravel()
+        gx, gy, gp = (np.empty(self.total) for _ in range(3))
+        check(self._lib.sfh_traj_backward(self._h, int(record), _dp(gxi), _dp(gyi), _dp(gpi), _dp(gx), _dp(gy),
+                                          _dp(gp)), "sfh_traj_backward")
+        return gx + 1j * gy, gp
+
+    def traj_release(self, record: int):
+        if self._h is not None:
+            check(self._lib.sfh_traj_release(self._h, int(record)), "sfh_traj_release")
+
     def mu(self) -> np.ndarray:
         out = np.empty(self.total)
         check(self._lib.sfh_get_mu(self._h, _dp(out)), "sfh_get_mu")

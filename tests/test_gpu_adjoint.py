@@ -249,3 +249,75 @@ def test_adjoint_rejects_unsupported_physics(csh, kw, drive):
         e.set_state(_surface(64, 1), np.zeros(64))
         with pytest.raises(Exception, match="adjoint is implemented"):
             e.rhs_vjp(np.ones(64, complex), np.ones(64))
+
+
+def _host_sweep(e, Z0, P0, dt, steps, every, gZ, gP):
+    # the checkpointed reverse sweep driven step by step through the host (set_state, step, get_state,
+    # step_vjp per step): the device-resident sweep must give the same cotangent
+    e.set_state(Z0, P0)
+    ck = []
+    done = 0
+    while done < steps:
+        ck.append(e.get_state())
+        n = min(every, steps - done)
+        e.step(dt, n)
+        done += n
+    ZT, PT = e.get_state()
+    for seg in range(len(ck) - 1, -1, -1):
+        n = min(every, steps - seg * every)
+        e.set_state(*ck[seg])
+        states = [ck[seg]]
+        for _ in range(n - 1):
+            e.step(dt, 1)
+            states.append(e.get_state())
+        for i in range(n - 1, -1, -1):
+            e.set_state(*states[i])
+            gZ, gP = e.step_vjp(dt, gZ, gP)
+    return ZT, PT, gZ, gP
+
+
+@pytest.mark.parametrize("solver,sizes,every,kw", [
+    ("GMRES", [1024], 4, None),
+    ("LU", [256], 3, None),
+    ("BATCHED", [128, 96], 5, None),
+    ("GMRES", [256], 4, "film"),
+])
+def test_device_trajectory_sweep_matches_host_sweep(csh, solver, sizes, every, kw):
+    dt, steps = 0.004, 11
+    props = csh.Props(**(FILMS[0] if kw == "film" else {}))
+    rng = np.random.default_rng(31)
+    Z0 = np.concatenate([_surface(n, 40 + i) for i, n in enumerate(sizes)])
+    P0 = np.concatenate([0.01 * np.cos(2 * np.pi * np.arange(n) / n) for n in sizes])
+    gZ = rng.standard_normal(len(Z0)) + 1j * rng.standard_normal(len(Z0))
+    gP = rng.standard_normal(len(Z0))
+    print(f"device trajectory sweep {solver} sizes {sizes} every {every} {kw or 'water'}", flush=True)
+    with csh.Ensemble(sizes, props, solver=getattr(csh, solver), tol=1e-13,
+                      max_iter=48 if solver == "BATCHED" else 32) as e:
+        ZT_h, PT_h, gZ_h, gP_h = _host_sweep(e, Z0, P0, dt, steps, every, gZ, gP)
+        e.set_state(Z0, P0)
+        rec = e.traj_forward(dt, steps, every)
+        ZT, PT = e.get_state()
+        gZ_d, gP_d = e.traj_backward(rec, gZ, gP)
+        Zs, Ps = e.get_state()   # the record's initial state
+        gZ_d2, gP_d2 = e.traj_backward(rec, gZ, gP)   # a record can be swept back again
+        e.traj_release(rec)
+        with pytest.raises(RuntimeError):
+            e.traj_backward(rec, gZ, gP)
+        with pytest.raises(RuntimeError):
+            e.traj_release(rec)
+    dz = np.abs(ZT - ZT_h).max()
+    dg = max(np.abs(gZ_d - gZ_h).max(), np.abs(gP_d - gP_h).max()) / max(np.abs(gZ_h).max(), np.abs(gP_h).max())
+    print(f"  final state diff {dz:.2e}, cotangent rel diff {dg:.2e}", flush=True)
+    assert dz <= 1e-14 and dg <= 1e-13
+    assert np.array_equal(Zs, Z0) and np.array_equal(Ps, P0)
+    assert np.array_equal(gZ_d, gZ_d2) and np.array_equal(gP_d, gP_d2)
+
+
+def test_device_trajectory_rejects_unsupported_problem_before_stepping(csh):
+    Z0 = _surface(128, 5)
+    with csh.Ensemble([128], csh.Props(rho=0.3), solver=csh.GMRES) as e:
+        e.set_state(Z0, np.zeros(128))
+        with pytest.raises(RuntimeError):
+            e.traj_forward(0.004, 5, 2)
+        Z, _ = e.get_state()
+    assert np.array_equal(Z, Z0)   # nothing stepped
