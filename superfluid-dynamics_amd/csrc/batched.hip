@@ -17,18 +17,20 @@
 //        mixed : FP64 iterative refinement  r = b - M mu  with the residual evaluated
 //                matrix-free in FP64 (same pair formula as the assembly), correction solves by
 //                GMRES on the stored FP32 matrix; a correction whose GMRES residual plus
-//                ||M - M32||_F |d| is within the tolerance ends the refinement without another
-//                FP64 residual pass;
+//                ||M - M32||_F |d| (bounded by 2^-24 ||M||_F) is within the tolerance ends the
+//                refinement without another FP64 residual pass;
 //   D  mu' (one more transform pair);
 //   E  the cot-sum velocity, Bernoulli and the RK4 stage update (WaterVelocities.cuh:38-107,
 //      createM.cuh:96-117, AutonomousRungeKuttaStepper.cuh:338-375).
 //
-// Thread t owns node t.  Pair loops (assembly, matrix-vector products, velocity) split the
-// columns over G = 1024 / N groups so small surfaces still use the whole workgroup: thread t
-// handles row t % N and columns j = t / N (mod G).  The column-major matrix makes every matrix
-// load of a wave one 512-byte (FP64) or 256-byte (FP32) contiguous segment, and column data
-// (node table, the vector) is read from LDS as a broadcast.
-#include "sfh_wg.hpp"
+// Thread t owns node t.  The pair loops (assembly, the matrix-free FP64 product, velocity) visit
+// every unordered pair of nodes once in 64 x 64 tiles and give both rows of the pair their
+// contribution (sfh_sym.hpp); the matrix is stored in the tiles' skewed layout, so every store
+// and every load of the stored product is one contiguous 512-byte (FP64) or 256-byte (FP32) row
+// per wave, and node data is read from LDS without bank conflicts.
+#include <type_traits>
+
+#include "sfh_sym.hpp"
 
 namespace sfh {
 
@@ -38,7 +40,6 @@ __global__ __launch_bounds__(1024) void batched_stage_kernel(BatchArgs a, Phys p
     const SurfDesc d = a.sd[s];
     const int n = d.n, off = d.voff, t = threadIdx.x;
     const bool own = t < n;
-    const int G = kBT / n > 0 ? kBT / n : 1;
     const double twopi_n = 2.0 * kPi / n;
 
     // ---------------------------------------------------------------- A: derivatives
@@ -63,82 +64,55 @@ __global__ __launch_bounds__(1024) void batched_stage_kernel(BatchArgs a, Phys p
     }
 
     // ---------------------------------------------------------------- B: assembly
-    // The pair loop also forms (M x)_k for the warm start x in FP64, the first residual of the
-    // solve: FP64 from the stored entries, in the order of mv_stored; mixed from the FP64 entries
-    // that are rounded to FP32 for storage, in the order of mv_free.
+    // One sweep over the unordered pairs (sfh_sym.hpp) writes both entries of each pair into the
+    // tile-skewed matrix (FP64, or FP64 entries rounded to FP32) and forms (M x)_k for the warm start
+    // x in FP64 from the unrounded entries: the first residual of the solve.
     const bool water = ph.kind == 0;
     const PairConst pc = wg_diagonal(ph, nv, n, L);
-    if (own) L.vs[t] = x;
+    const int nbk = (n + kTile - 1) / kTile;
+    if (t < nbk * kTile) L.vs[t] = own ? x : 0.0;   // the padding entries stay 0 for every product
     __syncthreads();
-    double mval = 0.0;
-    double ds2 = 0.0;   // mixed: squared rounding errors of the FP32 entries (this thread's)
-    if (t < G * n) {
-        const int k = t % n, g = t / n;
-        const NodeTrig tk = L.tr[k];
-        const double2 zq = L.zp[k];
-        const double czx = pc.c * zq.x, czy = pc.c * zq.y;
-        if (a.mixed) {
-            float* col = reinterpret_cast<float*>(a.M) + d.moff + k;
-            double sx = 0.0, sy = 0.0, si = 0.0;
-            for (int j = g; j < n; j += G) {
-                float v;
-                if (j == k) {
-                    v = (float)L.diag[k];
-                    const double rd = L.diag[k] - (double)v;
-                    ds2 += rd * rd;
-                } else {
-                    const NodeTrig tj = L.tr[j];
-                    const double vj = L.vs[j];
-                    const CotParts cp = cot_parts(tk, tj);   // c Im(Z'_k cot) = id (c y' p - c x' q)
-                    const double w = vj * cp.id;
-                    sx += w * cp.p;
-                    sy -= w * cp.q;
-                    double e = cp.id * (czy * cp.p - czx * cp.q);
-                    if (pc.image) {
-                        const CotParts cq = cot_image_parts(tk, tj, pc.eh, pc.ehi);
-                        si -= vj * (cq.q * cq.id);
-                        e += pc.ci * (cq.q * cq.id);
-                    }
-                    v = (float)e;
-                    const double rd = e - (double)v;
-                    ds2 += rd * rd;
-                }
-                col[(size_t)j * n] = v;
+    double dsq = 0.0;   // mixed: this thread's share of ||M||_F^2
+    {
+        const int kr = sym_row(n);
+        const NodeTrig tk = L.tr[kr];
+        const double2 zq = L.zp[kr];
+        const double2 czk = make_double2(pc.c * zq.x, pc.c * zq.y);
+        const double vk = L.vs[kr];
+        double* const acc[1] = {L.part};
+        auto store = [&](auto* Mt) {
+            using T = typename std::remove_pointer<decltype(Mt)>::type;
+            // the diagonal: offset 0 of the diagonal tiles
+            if (t < nbk * kTile) {
+                const double dg = own ? L.diag[t] : 0.0;
+                Mt[(long long)((t >> 6) * nbk + (t >> 6)) * kTileElems + (t & 63)] = (T)dg;
+                dsq += dg * dg;
             }
-            mval = pc.c * (zq.x * sy + zq.y * sx) - pc.ci * si;
-        } else {
-            double* col = reinterpret_cast<double*>(a.M) + d.moff + k;
-            double acc0 = 0.0, acc1 = 0.0;
-            int u = 0;
-            for (int j = g; j < n; j += G, ++u) {
-                double v;
-                if (j == k) {
-                    v = L.diag[k];
-                } else {
-                    const NodeTrig tj = L.tr[j];
-                    const CotParts cp = cot_parts(tk, tj);
-                    v = cp.id * (czy * cp.p - czx * cp.q);
-                    if (pc.image) {
-                        const CotParts cq = cot_image_parts(tk, tj, pc.eh, pc.ehi);
-                        v += pc.ci * (cq.q * cq.id);
+            sym_sweep<1>(n, acc, [&](int k, int j, int J, int r, bool mirror, double (&row)[1], double (&col)[1]) {
+                const int I = k >> 6, l = k & 63;
+                double e = 0.0, em = 0.0;
+                if (k < n && j < n) {
+                    entry_pair(tk, L.tr[j], czk, L.zp[j], pc, e, em);
+                    row[0] += e * L.vs[j];
+                    dsq += e * e;
+                    if (mirror) {
+                        col[0] += em * vk;
+                        dsq += em * em;
                     }
                 }
-                col[(size_t)j * n] = v;
-                if (u & 1)
-                    acc1 += v * L.vs[j];
-                else
-                    acc0 += v * L.vs[j];
-            }
-            mval = acc0 + acc1;
-        }
+                Mt[(long long)(I * nbk + J) * kTileElems + r * kTile + l] = (T)e;
+                if (mirror)
+                    Mt[(long long)(J * nbk + I) * kTileElems + ((kTile - r) & (kTile - 1)) * kTile + ((l + r) & (kTile - 1))] =
+                        (T)em;
+            });
+        };
+        if (a.mixed)
+            store(reinterpret_cast<float*>(a.M) + d.moff);
+        else
+            store(reinterpret_cast<double*>(a.M) + d.moff);
     }
-    L.part[t] = mval;
-    __syncthreads();   // the workgroup reads back only its own columns: a barrier suffices
     double mx0 = 0.0;   // (M x)_t
-    if (own) {
-        for (int q = 0; q < G; ++q) mx0 += L.part[q * n + t];
-        if (a.mixed) mx0 += L.diag[t] * L.vs[t];
-    }
+    if (own) mx0 = sym_total(L.part, n) + L.diag[t] * L.vs[t];
     __syncthreads();
 
     // ---------------------------------------------------------------- C: solve
@@ -147,18 +121,19 @@ __global__ __launch_bounds__(1024) void batched_stage_kernel(BatchArgs a, Phys p
     int unconv = 0, iters = 0, sweeps = 0;
     if (!a.mixed) {
         const double* Md = reinterpret_cast<const double*>(a.M) + d.moff;
-        iters = wg_gmres([&]() { return mv_stored(Md, n, G, L); }, n, bk, x, false, tol_abs, a.cycles, L, &unconv,
+        iters = wg_gmres([&]() { return mv_tiles(Md, n, L); }, n, bk, x, false, tol_abs, a.cycles, L, &unconv,
                          true, own ? bk - mx0 : 0.0);
     } else {
         const float* Mf = reinterpret_cast<const float*>(a.M) + d.moff;
-        const double dM = sqrt(bsum1(ds2, L));   // ||M - M32||_F
+        // ||M - M32||_F <= 2^-24 ||M||_F (round to nearest, normal range) + N 2^-150 (subnormal range)
+        const double dM = 0x1p-24 * sqrt(bsum1(dsq, L)) + n * 0x1p-150;
         bool done = false;
         for (; sweeps < a.max_refine; ++sweeps) {
             double mx = mx0;   // sweep 0: formed by the assembly
             if (sweeps > 0) {
                 if (own) L.vs[t] = x;
                 __syncthreads();
-                mx = mv_free(n, G, pc, L);   // every thread: it synchronises
+                mx = mv_free_sym(n, pc, L);   // every thread: it synchronises
             }
             const double r = own ? bk - mx : 0.0;
             const double rn = sqrt(bsum1(r * r, L));
@@ -167,7 +142,7 @@ __global__ __launch_bounds__(1024) void batched_stage_kernel(BatchArgs a, Phys p
                 break;
             }
             double dx = 0.0;
-            iters += wg_gmres([&]() { return mv_stored(Mf, n, G, L); }, n, r, dx, true,
+            iters += wg_gmres([&]() { return mv_tiles(Mf, n, L); }, n, r, dx, true,
                               fmax(1e-7 * rn, 0.25 * tol_abs), 1, L, nullptr);
             x += dx;
             // the FP64 residual of the corrected x, bounded without forming it:
@@ -208,47 +183,55 @@ __global__ __launch_bounds__(1024) void batched_stage_kernel(BatchArgs a, Phys p
     __syncthreads();
 
     // ---------------------------------------------------------------- E: velocity + stage
+    // One sweep over the unordered pairs: the cotangent sums of both rows of each pair (the image
+    // sums too for the film).
     double* psy = reinterpret_cast<double*>(L.fa);   // 2 kBT doubles: sy | ix
     double* piy = reinterpret_cast<double*>(L.fb);
-    if (t < G * n) {
-        const int k = t % n, g = t / n;
-        const NodeTrig tk = L.tr[k];
-        double sx = 0.0, sy = 0.0, ix = 0.0, iy = 0.0;
-        for (int j = g; j < n; j += G) {
-            if (j == k) continue;
-            const NodeTrig tj = L.tr[j];
-            const double mj = L.vs[j];
-            const CotParts cp = cot_parts(tk, tj);
-            const double w = mj * cp.id;
-            sx += w * cp.p;
-            sy -= w * cp.q;
-            if (pc.image) {
+    {
+        const int kr = sym_row(n);
+        const NodeTrig tk = L.tr[kr];
+        const double mk = L.vs[kr];
+        if (pc.image) {
+            double* const acc[4] = {L.part, psy, psy + kBT, piy};
+            sym_sweep<4>(n, acc, [&](int k, int j, int, int, bool mirror, double (&row)[4], double (&col)[4]) {
+                if (k >= n || j >= n) return;
+                const NodeTrig tj = L.tr[j];
+                const double mj = L.vs[j];
+                const CotParts cp = cot_parts(tk, tj);
                 const CotParts cq = cot_image_parts(tk, tj, pc.eh, pc.ehi);
-                const double wi = mj * cq.id;
-                ix += wi * cq.p;
-                iy -= wi * cq.q;
-            }
+                const double w = mj * cp.id, wi = mj * cq.id;
+                row[0] += w * cp.p;
+                row[1] -= w * cp.q;
+                row[2] += wi * cq.p;
+                row[3] -= wi * cq.q;
+                if (mirror) {   // p, p' change sign with the order of the pair, q changes, q' does not
+                    const double wk = mk * cp.id, wki = mk * cq.id;
+                    col[0] -= wk * cp.p;
+                    col[1] += wk * cp.q;
+                    col[2] -= wki * cq.p;
+                    col[3] -= wki * cq.q;
+                }
+            });
+        } else {
+            double* const acc[2] = {L.part, psy};
+            sym_sweep<2>(n, acc, [&](int k, int j, int, int, bool mirror, double (&row)[2], double (&col)[2]) {
+                if (k >= n || j >= n) return;
+                const CotParts cp = cot_parts(tk, L.tr[j]);
+                const double w = L.vs[j] * cp.id;
+                row[0] += w * cp.p;
+                row[1] -= w * cp.q;
+                if (mirror) {
+                    const double wk = mk * cp.id;
+                    col[0] -= wk * cp.p;
+                    col[1] += wk * cp.q;
+                }
+            });
         }
-        L.part[t] = sx;
-        psy[t] = sy;
-        psy[kBT + t] = ix;
-        piy[t] = iy;
-    } else {
-        L.part[t] = 0.0;
-        psy[t] = 0.0;
-        psy[kBT + t] = 0.0;
-        piy[t] = 0.0;
     }
-    __syncthreads();
     double2 q1 = make_double2(0.0, 0.0), q2 = make_double2(0.0, 0.0);
     if (own) {
-        double sx = 0.0, sy = 0.0, ix = 0.0, iy = 0.0;
-        for (int q = 0; q < G; ++q) {
-            sx += L.part[q * n + t];
-            sy += psy[q * n + t];
-            ix += psy[kBT + q * n + t];
-            iy += piy[q * n + t];
-        }
+        const double sx = sym_total(L.part, n), sy = sym_total(psy, n);
+        const double ix = pc.image ? sym_total(psy + kBT, n) : 0.0, iy = pc.image ? sym_total(piy, n) : 0.0;
         velocity_from_sums(ph, zpk, zppk, mupk, x, zk.y, sx, sy, ix, iy, q1, q2);
     }
     // createM.cuh:104 reads the lower velocity of node 1 in the rho * q1.q2 term

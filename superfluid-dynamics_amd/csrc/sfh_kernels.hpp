@@ -176,6 +176,14 @@ __global__ void state_join_kernel(const double2* Z, const double* P, double2* st
 // ---- batched.hip (persistent per-surface stage kernel, ensembles with every N <= kBatchMaxN)
 constexpr int kBatchThreads = 1024;
 constexpr int kBatchMaxN = 1024;
+// The batched kernel stores a surface's matrix in 64 x 64 tiles (tile-skewed, sfh_sym.hpp): this
+// many elements per surface.
+constexpr int kTile = 64;
+constexpr int kTileElems = kTile * kTile;
+__host__ __device__ inline long long tiled_matrix_elems(int n) {
+    const long long nb = (n + kTile - 1) / kTile;
+    return nb * nb * kTileElems;
+}
 struct BatchArgs {
     const SurfDesc* sd;
     const int* order;          // workgroup -> surface, by decreasing N

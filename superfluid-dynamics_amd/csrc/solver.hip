@@ -174,7 +174,8 @@ Solver::Solver(const Phys& ph, const std::vector<int>& n, const Options& opt) : 
         }
         desc_.push_back(d);
         voff += n[s];
-        moff += (long long)n[s] * n[s];
+        // the batched kernel keeps each surface's matrix in 64 x 64 tiles
+        moff += opt_.solver == kSolveBatched ? tiled_matrix_elems(n[s]) : (long long)n[s] * n[s];
         maxN_ = std::max(maxN_, n[s]);
         if (n[s] != n[0]) uniform_ = false;
     }
