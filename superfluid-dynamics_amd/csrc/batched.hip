@@ -74,7 +74,8 @@ __global__ __launch_bounds__(1024) void batched_stage_kernel(BatchArgs a, Phys p
     __syncthreads();
     double dsq = 0.0;   // mixed: this thread's share of ||M||_F^2
     {
-        const int kr = sym_row(n);
+        const SymGeo sg = sym_geo(n);
+        const int kr = sg.I * kTile + sg.l;
         const NodeTrig tk = L.tr[kr];
         const double2 zq = L.zp[kr];
         const double2 czk = make_double2(pc.c * zq.x, pc.c * zq.y);
@@ -88,22 +89,23 @@ __global__ __launch_bounds__(1024) void batched_stage_kernel(BatchArgs a, Phys p
                 Mt[(long long)((t >> 6) * nbk + (t >> 6)) * kTileElems + (t & 63)] = (T)dg;
                 dsq += dg * dg;
             }
-            sym_sweep<1>(n, acc, [&](int k, int j, int J, int r, bool mirror, double (&row)[1], double (&col)[1]) {
-                const int I = k >> 6, l = k & 63;
+            sym_sweep<1>(n, acc, [&](int k, int j, int J, int r, auto mirror, auto full, double (&row)[1], double (&col)[1]) {
+                // tile bases are wave-uniform; the offsets within a tile are 32-bit
+                T* const tij = Mt + (long long)(sg.I * nbk + J) * kTileElems;
+                T* const tji = Mt + (long long)(J * nbk + sg.I) * kTileElems;
+                const unsigned l = sg.l;
                 double e = 0.0, em = 0.0;
-                if (k < n && j < n) {
+                if (decltype(full)::value || (k < n && j < n)) {
                     entry_pair(tk, L.tr[j], czk, L.zp[j], pc, e, em);
                     row[0] += e * L.vs[j];
                     dsq += e * e;
-                    if (mirror) {
+                    if constexpr (decltype(mirror)::value) {
                         col[0] += em * vk;
                         dsq += em * em;
                     }
                 }
-                Mt[(long long)(I * nbk + J) * kTileElems + r * kTile + l] = (T)e;
-                if (mirror)
-                    Mt[(long long)(J * nbk + I) * kTileElems + ((kTile - r) & (kTile - 1)) * kTile + ((l + r) & (kTile - 1))] =
-                        (T)em;
+                tij[(unsigned)r * kTile + l] = (T)e;
+                if constexpr (decltype(mirror)::value) tji[((unsigned)(kTile - r) & (kTile - 1)) * kTile + ((l + r) & (kTile - 1))] = (T)em;
             });
         };
         if (a.mixed)
@@ -193,8 +195,8 @@ __global__ __launch_bounds__(1024) void batched_stage_kernel(BatchArgs a, Phys p
         const double mk = L.vs[kr];
         if (pc.image) {
             double* const acc[4] = {L.part, psy, psy + kBT, piy};
-            sym_sweep<4>(n, acc, [&](int k, int j, int, int, bool mirror, double (&row)[4], double (&col)[4]) {
-                if (k >= n || j >= n) return;
+            sym_sweep<4>(n, acc, [&](int k, int j, int, int, auto mirror, auto full, double (&row)[4], double (&col)[4]) {
+                if (!decltype(full)::value && (k >= n || j >= n)) return;
                 const NodeTrig tj = L.tr[j];
                 const double mj = L.vs[j];
                 const CotParts cp = cot_parts(tk, tj);
@@ -204,7 +206,7 @@ __global__ __launch_bounds__(1024) void batched_stage_kernel(BatchArgs a, Phys p
                 row[1] -= w * cp.q;
                 row[2] += wi * cq.p;
                 row[3] -= wi * cq.q;
-                if (mirror) {   // p, p' change sign with the order of the pair, q changes, q' does not
+                if constexpr (decltype(mirror)::value) {   // p, p' change sign with the pair's order, q changes, q' not
                     const double wk = mk * cp.id, wki = mk * cq.id;
                     col[0] -= wk * cp.p;
                     col[1] += wk * cp.q;
@@ -214,13 +216,13 @@ __global__ __launch_bounds__(1024) void batched_stage_kernel(BatchArgs a, Phys p
             });
         } else {
             double* const acc[2] = {L.part, psy};
-            sym_sweep<2>(n, acc, [&](int k, int j, int, int, bool mirror, double (&row)[2], double (&col)[2]) {
-                if (k >= n || j >= n) return;
+            sym_sweep<2>(n, acc, [&](int k, int j, int, int, auto mirror, auto full, double (&row)[2], double (&col)[2]) {
+                if (!decltype(full)::value && (k >= n || j >= n)) return;
                 const CotParts cp = cot_parts(tk, L.tr[j]);
                 const double w = L.vs[j] * cp.id;
                 row[0] += w * cp.p;
                 row[1] -= w * cp.q;
-                if (mirror) {
+                if constexpr (decltype(mirror)::value) {
                     const double wk = mk * cp.id;
                     col[0] -= wk * cp.p;
                     col[1] += wk * cp.q;

@@ -28,6 +28,8 @@
 // of its tile.  A wave's 64 stores of one offset, direct or mirrored, fall in one contiguous row of
 // 64 elements, and so do the 64 loads of one offset in the product.  Rows and columns beyond N hold 0.
 #pragma once
+#include <type_traits>
+
 #include "sfh_wg.hpp"
 
 namespace sfh {
@@ -49,7 +51,7 @@ struct SymGeo {
 
 __device__ inline SymGeo sym_geo(int n) {
     SymGeo g;
-    const int w = threadIdx.x / kTile;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / kTile);   // wave-uniform: scalar
     g.nb = (n + kTile - 1) / kTile;
     g.wpb = kBW / g.nb;
     g.I = w / g.wpb;
@@ -59,13 +61,9 @@ __device__ inline SymGeo sym_geo(int n) {
     return g;
 }
 
-// The sweep.  acc[a] (a < NA) are LDS arrays of kBT doubles: slot s of node i at s * nb * 64 + i.
-// pair(k, j, J, r, mirror, row, col) is called by every lane of an active wave for row
-// k = I*64 + l and column j = J*64 + (l + r) % 64 (either may be >= n: the functor checks); it adds
-// the pair's contribution to row k into row[] and, when mirror, to row j into col[].  On return
-// total of node t is sym_total(acc[a], n); the caller synchronises before reusing acc.
-template <int NA, class F>
-__device__ inline void sym_sweep(int n, double* const (&acc)[NA], F&& pair) {
+// sym_sweep for n a multiple of 64 (FULL) or not.
+template <int NA, bool FULL, class F>
+__device__ inline void sym_sweep_impl(int n, double* const (&acc)[NA], F& pair) {
     const SymGeo g = sym_geo(n);
     const int t = threadIdx.x;
     const int stride = g.nb * kTile;
@@ -96,9 +94,17 @@ __device__ inline void sym_sweep(int n, double* const (&acc)[NA], F&& pair) {
             double col[NA];
 #pragma unroll
             for (int a = 0; a < NA; ++a) col[a] = 0.0;
-            for (int r = r0; r < r1; ++r) {
+            // offset 32 of the diagonal tile (no column sums) is the last offset of its range
+            const int rm = (ph == 0 && r1 == 33) ? 32 : r1;
+            for (int r = r0; r < rm; ++r) {
                 const int j = J * kTile + ((g.l + r) & (kTile - 1));
-                pair(k, j, J, r, !(ph == 0 && r == 32), row, col);
+                pair(k, j, J, r, std::true_type{}, std::integral_constant<bool, FULL>{}, row, col);
+#pragma unroll
+                for (int a = 0; a < NA; ++a) col[a] = dpp_rol1(col[a]);
+            }
+            if (rm < r1) {
+                const int j = J * kTile + ((g.l + 32) & (kTile - 1));
+                pair(k, j, J, 32, std::false_type{}, std::integral_constant<bool, FULL>{}, row, col);
 #pragma unroll
                 for (int a = 0; a < NA; ++a) col[a] = dpp_rol1(col[a]);
             }
@@ -115,6 +121,21 @@ __device__ inline void sym_sweep(int n, double* const (&acc)[NA], F&& pair) {
         for (int a = 0; a < NA; ++a) acc[a][g.sub * stride + k] += row[a];
     }
     __syncthreads();
+}
+
+// The sweep.  acc[a] (a < NA) are LDS arrays of kBT doubles: slot s of node i at s * nb * 64 + i.
+// pair(k, j, J, r, mirror, full, row, col) is called by every lane of an active wave for row
+// k = I*64 + l and column j = J*64 + (l + r) % 64; it adds the pair's contribution to row k into
+// row[] and, when mirror, to row j into col[].  mirror and full are std::integral_constant<bool>:
+// mirror is false only at offset 32 of the diagonal tile, and full (n a multiple of 64) says that
+// k and j are always < n, so the functor checks them only when it is false.  On return the total
+// of node t is sym_total(acc[a], n); the caller synchronises before reusing acc.
+template <int NA, class F>
+__device__ inline void sym_sweep(int n, double* const (&acc)[NA], F&& pair) {
+    if ((n & (kTile - 1)) == 0)
+        sym_sweep_impl<NA, true>(n, acc, pair);
+    else
+        sym_sweep_impl<NA, false>(n, acc, pair);
 }
 
 // The total of node t (< n) over the slots of a sweep's accumulator.
@@ -158,12 +179,12 @@ __device__ inline double mv_free_sym(int n, const PairConst& pc, BLds& L) {
     const double2 czk = make_double2(pc.c * zq.x, pc.c * zq.y);
     const double vk = L.vs[kr];
     double* const acc[1] = {L.part};
-    sym_sweep<1>(n, acc, [&](int k, int j, int, int, bool mirror, double (&row)[1], double (&col)[1]) {
-        if (k >= n || j >= n) return;
+    sym_sweep<1>(n, acc, [&](int k, int j, int, int, auto mirror, auto full, double (&row)[1], double (&col)[1]) {
+        if (!decltype(full)::value && (k >= n || j >= n)) return;
         double e, em;
         entry_pair(tk, L.tr[j], czk, L.zp[j], pc, e, em);
         row[0] += e * L.vs[j];
-        if (mirror) col[0] += em * vk;
+        if constexpr (decltype(mirror)::value) col[0] += em * vk;
     });
     double w = 0.0;
     if (t < n) w = sym_total(L.part, n) + L.diag[t] * L.vs[t];
@@ -183,7 +204,7 @@ __device__ inline double mv_tiles(const T* __restrict__ M, int n, BLds& L) {
         const int span = g.nb * kTile;   // offset rows of the block row
         const int i0 = (span * g.sub) / g.wpb, i1 = (span * (g.sub + 1)) / g.wpb;
         const T* base = M + (long long)g.I * g.nb * kTileElems + g.l;
-        constexpr int U = sizeof(T) == 8 ? 8 : 16;   // loads in flight per lane
+        constexpr int U = sizeof(T) == 8 ? 16 : 32;   // loads in flight per lane
         int i = i0;
         for (; i + U <= i1; i += U) {
             T m[U];
