@@ -24,7 +24,9 @@ namespace sfh {
 constexpr int kNoAllocMaxN = kBatchMaxN;   // 1024 nodes
 constexpr size_t kScratchAlign = 256;
 constexpr size_t align_up(size_t v) { return (v + kScratchAlign - 1) / kScratchAlign * kScratchAlign; }
-constexpr size_t kScrM = align_up(sizeof(double) * kNoAllocMaxN * kNoAllocMaxN);
+// the batched kernel's tile-skewed matrix: at most 1024^2 elements for every N <= 1024
+constexpr size_t kScrM = align_up(sizeof(double) * tiled_matrix_elems(kNoAllocMaxN));
+static_assert(tiled_matrix_elems(kNoAllocMaxN) == (long long)kNoAllocMaxN * kNoAllocMaxN, "full tiles at N = 1024");
 constexpr size_t kScrVec2 = align_up(sizeof(double2) * kNoAllocMaxN);
 constexpr size_t kScrVec = align_up(sizeof(double) * kNoAllocMaxN);
 constexpr size_t kScrSmall = 256;   // descriptor, order, stats

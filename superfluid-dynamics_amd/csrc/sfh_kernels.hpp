@@ -180,7 +180,7 @@ constexpr int kBatchMaxN = 1024;
 // many elements per surface.
 constexpr int kTile = 64;
 constexpr int kTileElems = kTile * kTile;
-__host__ __device__ inline long long tiled_matrix_elems(int n) {
+__host__ __device__ constexpr inline long long tiled_matrix_elems(int n) {
     const long long nb = (n + kTile - 1) / kTile;
     return nb * nb * kTileElems;
 }
