@@ -386,8 +386,9 @@ int sfh_step_vjp(sfh_handle h, double dt, const double* gx_in, const double* gy_
  * 2 every, ... (ceil(steps / every) states of 24 T bytes, T = sfh_total_nodes) and returns the id of
  * that record in *record; the handle's state afterwards is the final state.  sfh_traj_backward takes
  * the cotangent (gx, gy, gphi) of the state after the recorded steps to the cotangent of the state the
- * record started from: each segment, last first, is re-run from its checkpoint into a device buffer
- * of `every` states and swept back one step VJP (sfh_step_vjp) at a time, the cotangent staying in
+ * record started from: each segment, last first, is re-run from its checkpoint, keeping every step's
+ * state, stage states and stage mu in device buffers (104 B per node per step), and swept back one
+ * step VJP (sfh_step_vjp's reverse pass, on the kept stages) at a time, the cotangent staying in
  * device memory; the handle's state afterwards is the record's initial state.  A record can be swept
  * back any number of times until sfh_traj_release frees it (sfh_destroy frees every record).  Same
  * problem scope as sfh_step_vjp (-3 otherwise, before any step); -3 for an unknown record. */

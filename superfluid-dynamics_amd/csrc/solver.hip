@@ -297,8 +297,8 @@ Solver::~Solver() {
         (void)hipFree(kv.second.Z);
         (void)hipFree(kv.second.P);
     }
-    if (segZ_) (void)hipFree(segZ_);
-    if (segP_) (void)hipFree(segP_);
+    for (void* p : {(void*)segZ_, (void*)segP_, (void*)segY_, (void*)segYP_, (void*)segMu_})
+        if (p) (void)hipFree(p);
     for (hipEvent_t e : event_pool_) (void)hipEventDestroy(e);
     if (rec_stream_) (void)hipStreamSynchronize(rec_stream_);
     for (hipEvent_t e : rec_packed_) (void)hipEventDestroy(e);
@@ -606,7 +606,7 @@ void Solver::enqueue_rhs(const double2* Z, const double* P, const StageCtl& ctl_
     SFH_HIP(hipGetLastError());
 }
 
-void Solver::enqueue_step(double dt) {
+void Solver::enqueue_step(double dt, const StageCapture* cap) {
     const double half = dt * 0.5, sixth = dt / 6.0;   // setTimeStep, AutonomousRungeKuttaStepper.cuh:31-36
     StageCtl c{};
     c.Z0 = Z0_;
@@ -622,19 +622,32 @@ void Solver::enqueue_step(double dt) {
     set_delayed_ctl(c, D0_, Da_);
     timed_coefficients(c, drive_t_, 1);
     enqueue_rhs(Z0_, P0_, c);
+    // the stage's mu and the state it produced, for the adjoint's re-run (never inside a captured graph)
+    auto keep = [&](int st, const double2* Zs, const double* Ps) {
+        if (!cap) return;
+        SFH_HIP(hipMemcpyAsync(cap->mu[st], mu_, T_ * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+        if (st < 3) {
+            SFH_HIP(hipMemcpyAsync(cap->Y[st], Zs, T_ * sizeof(double2), hipMemcpyDeviceToDevice, stream_));
+            SFH_HIP(hipMemcpyAsync(cap->YP[st], Ps, T_ * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+        }
+    };
+    keep(0, Za_, Pa_);
     if (ph_.timed) drive_prev_ = drive_t_;
     c.stage = 2; c.alpha = half; c.Zout = Zb_; c.Pout = Pb_;
     set_delayed_ctl(c, Da_, Db_);
     timed_coefficients(c, drive_t_ + half, 0);
     enqueue_rhs(Za_, Pa_, c);
+    keep(1, Zb_, Pb_);
     c.stage = 3; c.alpha = dt; c.Zout = Za_; c.Pout = Pa_;
     set_delayed_ctl(c, Db_, Da_);
     timed_coefficients(c, drive_t_ + half, 0);
     enqueue_rhs(Zb_, Pb_, c);
+    keep(2, Za_, Pa_);
     c.stage = 4; c.alpha = sixth; c.Zout = nullptr; c.Pout = nullptr;
     set_delayed_ctl(c, Da_, nullptr);
     timed_coefficients(c, drive_t_ + dt, 0);
     enqueue_rhs(Za_, Pa_, c);
+    keep(3, nullptr, nullptr);
     if (ph_.timed) drive_t_ += dt;   // currentTime += timeStep (RK4_Time_Dependent.cuh:408)
 }
 
@@ -1439,39 +1452,59 @@ void Solver::step_vjp(double dt, const double* gx_in, const double* gy_in, const
     ensure_vjp();
     SFH_HIP(hipStreamSynchronize(stream_));
     upload_cotangent(gx_in, gy_in, gp_in, gA_, gAP_);
-    enqueue_step_vjp(dt);
+    enqueue_step_vjp(dt, Z0_, P0_, nullptr);
     download_cotangent(gAcc_, gAccP_, gx_out, gy_out, gp_out);
     check_status();
 }
 
-void Solver::enqueue_step_vjp(double dt) {
+void Solver::enqueue_step_vjp(double dt, const double2* Z, const double* P, const StageCapture* stored) {
     SFH_HIP(hipMemcpyAsync(gAcc_, gA_, T_ * sizeof(double2), hipMemcpyDeviceToDevice, stream_));
     SFH_HIP(hipMemcpyAsync(gAccP_, gAP_, T_ * sizeof(double), hipMemcpyDeviceToDevice, stream_));
-    // stage states and their mu
-    StageCtl c{};
-    c.Z0 = Z0_;
-    c.P0 = P0_;
-    for (int i = 0; i < 3; ++i) {
-        c.kZ[i] = vK_[i];
-        c.kP[i] = vKP_[i];
-    }
-    const double2* zin[4] = {Z0_, vY_[0], vY_[1], vY_[2]};
-    const double* pin[4] = {P0_, vYP_[0], vYP_[1], vYP_[2]};
-    const double alpha[3] = {0.5 * dt, 0.5 * dt, dt};
-    seq_len_ = gm_m_;   // standalone solves: the full GMRES launch sequence
-    for (int st = 0; st < 4; ++st) {
-        c.stage = st < 3 ? st + 1 : 0;
-        c.alpha = st < 3 ? alpha[st] : 0.0;
-        c.Zout = st < 3 ? vY_[st] : nullptr;
-        c.Pout = st < 3 ? vYP_[st] : nullptr;
-        enqueue_rhs(zin[st], pin[st], c);
-        SFH_HIP(hipMemcpyAsync(vMu_[st], mu_, T_ * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+    StageCapture sc;
+    if (stored) {
+        sc = *stored;
+    } else {   // stage states and their mu recomputed
+        StageCtl c{};
+        c.Z0 = Z;
+        c.P0 = P;
+        for (int i = 0; i < 3; ++i) {
+            c.kZ[i] = vK_[i];
+            c.kP[i] = vKP_[i];
+        }
+        const double2* zin[4] = {Z, vY_[0], vY_[1], vY_[2]};
+        const double* pin[4] = {P, vYP_[0], vYP_[1], vYP_[2]};
+        const double alpha[3] = {0.5 * dt, 0.5 * dt, dt};
+        seq_len_ = gm_m_;   // standalone solves: the full GMRES launch sequence
+        for (int st = 0; st < 4; ++st) {
+            c.stage = st < 3 ? st + 1 : 0;
+            c.alpha = st < 3 ? alpha[st] : 0.0;
+            c.Zout = st < 3 ? vY_[st] : nullptr;
+            c.Pout = st < 3 ? vYP_[st] : nullptr;
+            enqueue_rhs(zin[st], pin[st], c);
+            SFH_HIP(hipMemcpyAsync(vMu_[st], mu_, T_ * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+        }
+        for (int i = 0; i < 3; ++i) {
+            sc.Y[i] = vY_[i];
+            sc.YP[i] = vYP_[i];
+        }
+        for (int i = 0; i < 4; ++i) sc.mu[i] = vMu_[i];
     }
     // reverse sweep
-    enqueue_vjp(vY_[2], vYP_[2], vMu_[3], gA_, gAP_, dt / 6.0, nullptr, nullptr, 0.0, gR_[0], gRP_[0], true);
-    enqueue_vjp(vY_[1], vYP_[1], vMu_[2], gA_, gAP_, dt / 3.0, gR_[0], gRP_[0], dt, gR_[1], gRP_[1], true);
-    enqueue_vjp(vY_[0], vYP_[0], vMu_[1], gA_, gAP_, dt / 3.0, gR_[1], gRP_[1], 0.5 * dt, gR_[0], gRP_[0], true);
-    enqueue_vjp(Z0_, P0_, vMu_[0], gA_, gAP_, dt / 6.0, gR_[0], gRP_[0], 0.5 * dt, gR_[1], gRP_[1], true);
+    enqueue_vjp(sc.Y[2], sc.YP[2], sc.mu[3], gA_, gAP_, dt / 6.0, nullptr, nullptr, 0.0, gR_[0], gRP_[0], true);
+    enqueue_vjp(sc.Y[1], sc.YP[1], sc.mu[2], gA_, gAP_, dt / 3.0, gR_[0], gRP_[0], dt, gR_[1], gRP_[1], true);
+    enqueue_vjp(sc.Y[0], sc.YP[0], sc.mu[1], gA_, gAP_, dt / 3.0, gR_[1], gRP_[1], 0.5 * dt, gR_[0], gRP_[0], true);
+    enqueue_vjp(Z, P, sc.mu[0], gA_, gAP_, dt / 6.0, gR_[0], gRP_[0], 0.5 * dt, gR_[1], gRP_[1], true);
+}
+
+Solver::StageCapture Solver::seg_capture(int k) const {
+    StageCapture c;
+    const size_t T = T_, cap = seg_cap_;
+    for (int i = 0; i < 3; ++i) {
+        c.Y[i] = segY_ + (i * cap + k) * T;
+        c.YP[i] = segYP_ + (i * cap + k) * T;
+    }
+    for (int i = 0; i < 4; ++i) c.mu[i] = segMu_ + (i * cap + k) * T;
+    return c;
 }
 
 // The same resets as set_state (a new trajectory), from states already in device memory.
@@ -1526,13 +1559,19 @@ void Solver::traj_backward(int record, const double* gx_in, const double* gy_in,
     const int seg_len = std::min(r.every, r.steps);
     if (seg_len > seg_cap_) {
         SFH_HIP(hipStreamSynchronize(stream_));
-        if (segZ_) SFH_HIP(hipFree(segZ_));
-        if (segP_) SFH_HIP(hipFree(segP_));
+        for (void* p : {(void*)segZ_, (void*)segP_, (void*)segY_, (void*)segYP_, (void*)segMu_})
+            if (p) SFH_HIP(hipFree(p));
         segZ_ = nullptr;
         segP_ = nullptr;
+        segY_ = nullptr;
+        segYP_ = nullptr;
+        segMu_ = nullptr;
         seg_cap_ = 0;
         segZ_ = dalloc<double2>((size_t)seg_len * T_);
         segP_ = dalloc<double>((size_t)seg_len * T_);
+        segY_ = dalloc<double2>((size_t)3 * seg_len * T_);
+        segYP_ = dalloc<double>((size_t)3 * seg_len * T_);
+        segMu_ = dalloc<double>((size_t)4 * seg_len * T_);
         seg_cap_ = seg_len;
     }
     SFH_HIP(hipStreamSynchronize(stream_));
@@ -1541,24 +1580,31 @@ void Solver::traj_backward(int record, const double* gx_in, const double* gy_in,
     const size_t zb = T_ * sizeof(double2), pb = T_ * sizeof(double);
     for (int seg = nck - 1; seg >= 0; --seg) {
         const int n = std::min(r.every, r.steps - seg * r.every);
-        // y_s, ..., y_{s+n-1}: the checkpoint, then n - 1 steps from it
+        // y_s, ..., y_{s+n-1}: the checkpoint, then n - 1 steps from it, each step keeping its stage
+        // states and stage mu (the last step runs only for them).  Solves get the full launch
+        // sequence, warm-started from the stage's history as in the forward pass.
         const double2* ckZ = r.Z + (size_t)seg * T_;
         const double* ckP = r.P + (size_t)seg * T_;
         SFH_HIP(hipMemcpyAsync(segZ_, ckZ, zb, hipMemcpyDeviceToDevice, stream_));
         SFH_HIP(hipMemcpyAsync(segP_, ckP, pb, hipMemcpyDeviceToDevice, stream_));
         load_state(ckZ, ckP);
-        for (int k = 1; k < n; ++k) {
-            step(r.dt, 1);
-            SFH_HIP(hipMemcpyAsync(segZ_ + (size_t)k * T_, Z0_, zb, hipMemcpyDeviceToDevice, stream_));
-            SFH_HIP(hipMemcpyAsync(segP_ + (size_t)k * T_, P0_, pb, hipMemcpyDeviceToDevice, stream_));
+        seq_len_ = gm_m_;
+        for (int k = 0; k < n; ++k) {
+            const StageCapture cap = seg_capture(k);
+            enqueue_step(r.dt, &cap);
+            if (k + 1 < n) {
+                SFH_HIP(hipMemcpyAsync(segZ_ + (size_t)(k + 1) * T_, Z0_, zb, hipMemcpyDeviceToDevice, stream_));
+                SFH_HIP(hipMemcpyAsync(segP_ + (size_t)(k + 1) * T_, P0_, pb, hipMemcpyDeviceToDevice, stream_));
+            }
         }
         for (int i = n - 1; i >= 0; --i) {
-            load_state(segZ_ + (size_t)i * T_, segP_ + (size_t)i * T_);
-            enqueue_step_vjp(r.dt);
+            const StageCapture cap = seg_capture(i);
+            enqueue_step_vjp(r.dt, segZ_ + (size_t)i * T_, segP_ + (size_t)i * T_, &cap);
             SFH_HIP(hipMemcpyAsync(gA_, gAcc_, zb, hipMemcpyDeviceToDevice, stream_));
             SFH_HIP(hipMemcpyAsync(gAP_, gAccP_, pb, hipMemcpyDeviceToDevice, stream_));
         }
     }
+    load_state(r.Z, r.P);   // the record's initial state
     download_cotangent(gA_, gAP_, gx_out, gy_out, gp_out);
     check_status();
 }

@@ -170,9 +170,10 @@ class Solver {
     // RK4 steps of size dt from the current state exactly as step() does, keeping in device memory the
     // state before steps 0, every, 2 every, ...; it returns the id of that record.  traj_backward takes
     // the cotangent of the state after the recorded steps to the cotangent of the state it started
-    // from: each segment, last first, is re-run from its checkpoint into a device buffer of `every`
-    // states, then swept back one step_vjp at a time with the cotangent kept on the device.  The
-    // handle's state afterwards is the record's initial state.  Host arrays of total_nodes() entries.
+    // from: each segment, last first, is re-run from its checkpoint, keeping every step's state, stage
+    // states and stage mu in device buffers, then swept back one RK4 step at a time from those (no
+    // solve besides the transposed ones), the cotangent kept on the device.  The handle's state
+    // afterwards is the record's initial state.  Host arrays of total_nodes() entries.
     int traj_forward(double dt, int steps, int every);
     void traj_backward(int record, const double* gx_in, const double* gy_in, const double* gp_in, double* gx_out,
                        double* gy_out, double* gp_out);
@@ -201,7 +202,13 @@ class Solver {
   private:
     void enqueue_rhs(const double2* Z, const double* P, const StageCtl& ctl);
     bool enqueue_solve(const double2* Z, int stage, bool g1);
-    void enqueue_step(double dt);
+    // the stage states Y2..Y4 and the four stage mu of one RK4 step, copied out as the step runs
+    struct StageCapture {
+        double2* Y[3];
+        double* YP[3];
+        double* mu[4];
+    };
+    void enqueue_step(double dt, const StageCapture* cap = nullptr);
     void capture(double dt, int len);
     void run_steps(double dt, int len, int count);   // count steps with GMRES sequences of len launches
     void calibrate();               // bookkeeping after a full-length step while the history fills
@@ -296,8 +303,10 @@ class Solver {
     void enqueue_vjp_mwg(const double2* Z, const double* P, const double* mu, const double2* gAZ, const double* gAP,
                          double ca, const double2* gBZ, const double* gBP, double cb, double2* outZ, double* outP,
                          bool accumulate);
-    // one RK4 step back at the state Z0_/P0_: cotangent gA_/gAP_ of the state after it -> gAcc_/gAccP_
-    void enqueue_step_vjp(double dt);
+    // one RK4 step back from the state (Z, P): cotangent gA_/gAP_ of the state after it -> gAcc_/gAccP_.
+    // The stage states and mu come from `stored` when given, else they are recomputed (solves from
+    // zero guesses with the full launch sequence).
+    void enqueue_step_vjp(double dt, const double2* Z, const double* P, const StageCapture* stored);
     // the state from device buffers, with the resets of set_state (no host copy, no wait)
     void load_state(const double2* Z, const double* P);
     struct TrajRecord {
@@ -310,7 +319,11 @@ class Solver {
     int traj_next_ = 1;
     double2* segZ_ = nullptr;   // the states of one segment
     double* segP_ = nullptr;
+    double2* segY_ = nullptr;   // their stage states (3 per step) and stage mu (4 per step)
+    double* segYP_ = nullptr;
+    double* segMu_ = nullptr;
     int seg_cap_ = 0;
+    StageCapture seg_capture(int k) const;
 
     hipGraph_t graph_ = nullptr;
     hipGraphExec_t exec_ = nullptr;

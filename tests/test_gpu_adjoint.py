@@ -253,7 +253,8 @@ def test_adjoint_rejects_unsupported_physics(csh, kw, drive):
 
 def _host_sweep(e, Z0, P0, dt, steps, every, gZ, gP):
     # the checkpointed reverse sweep driven step by step through the host (set_state, step, get_state,
-    # step_vjp per step): the device-resident sweep must give the same cotangent
+    # step_vjp per step, each recomputing its stages from zero-guess solves): the device-resident sweep,
+    # which keeps the stages of its warm-started re-run, must agree to the solver tolerance
     e.set_state(Z0, P0)
     ck = []
     done = 0
@@ -308,7 +309,7 @@ def test_device_trajectory_sweep_matches_host_sweep(csh, solver, sizes, every, k
     dz = np.abs(ZT - ZT_h).max()
     dg = max(np.abs(gZ_d - gZ_h).max(), np.abs(gP_d - gP_h).max()) / max(np.abs(gZ_h).max(), np.abs(gP_h).max())
     print(f"  final state diff {dz:.2e}, cotangent rel diff {dg:.2e}", flush=True)
-    assert dz <= 1e-14 and dg <= 1e-13
+    assert dz <= 1e-14 and dg <= 1e-10
     assert np.array_equal(Zs, Z0) and np.array_equal(Ps, P0)
     assert np.array_equal(gZ_d, gZ_d2) and np.array_equal(gP_d, gP_d2)
 
