@@ -426,6 +426,28 @@ def main():
     achieved = work_per_launch / (avg_ms * 1e-3) / scale if avg_ms > 0 else 0.0
     stages = {t: round(v["ms"] / kt_steps, 4) for t, v in reps.items() if v["launches"]}
     e.close()
+    # FP64 rooflines of the other kernels of a stage, from their algorithmic operations per launch
+    # (FMA = 2, multiply, add and reciprocal = 1) over the live launch durations:
+    #   pair cotangent parts (sin/cos/exp differences, p, q, 1/(sa^2 + sh^2) with two Newton steps): 24
+    #   assembly per ordered pair: 24 + the entry (4) + the M x0 accumulation (2) = 30, and 8 B stored
+    #   velocity per ordered pair: 24 + the weight and two sums (5) + the mu' circulant pair (4) = 33
+    #   derivatives per (row, column): 7 multiply-adds of the three circulant products = 14
+    per_pair = {"deriv": 14.0, "assemble": 30.0, "velocity": 33.0}
+    stage_roofs = {}
+    if args.solver == "gmres":
+        for t, fl in per_pair.items():
+            rr = reps[t]
+            if not rr["launches"]:
+                continue
+            ms = rr["ms"] / rr["launches"]
+            tf = fl * N * N / (ms * 1e-3) / 1e12
+            stage_roofs[t] = {"bound": "fp64_valu", "achieved": round(tf, 3), "peak": PEAK_FP64_TFLOPS,
+                              "unit": "TFLOP/s", "frac": round(tf / PEAK_FP64_TFLOPS, 4),
+                              "avg_launch_ms": round(ms, 5), "flops_per_launch": fl * N * N}
+        if "assemble" in stage_roofs:
+            ms = stage_roofs["assemble"]["avg_launch_ms"]
+            stage_roofs["assemble"]["stores_gbs"] = round(8.0 * N * N / (ms * 1e-3) / 1e9, 1)
+            stage_roofs["assemble"]["stores_frac_of_hbm"] = round(8.0 * N * N / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4)
 
     out = {
         "metric": METRIC,
@@ -449,6 +471,7 @@ def main():
                      "avg_launch_ms": round(avg_ms, 5), f"{key}_per_launch": work_per_launch,
                      "launches_timed": r["launches"]},
         "stage_ms_per_step": stages,
+        "stage_rooflines": stage_roofs,
         "solver_stats": {"gmres_iterations_last": st["last_iterations"], "gmres_unconverged": st_main["unconverged"],
                          "redone_steps": st_main["redone_steps"], "launches_per_solve": st_main["sequence_length"],
                          "gepp_fallbacks": st["fallback_solves"]},
