@@ -1581,21 +1581,40 @@ void Solver::traj_backward(int record, const double* gx_in, const double* gy_in,
     for (int seg = nck - 1; seg >= 0; --seg) {
         const int n = std::min(r.every, r.steps - seg * r.every);
         // y_s, ..., y_{s+n-1}: the checkpoint, then n - 1 steps from it, each step keeping its stage
-        // states and stage mu (the last step runs only for them).  Solves get the full launch
-        // sequence, warm-started from the stage's history as in the forward pass.
+        // states and stage mu (the last step runs only for them).  The solves are warm-started from
+        // the stage histories as in the forward pass; GMRES solves get the full launch sequence while
+        // the histories fill and the forward pass's trimmed length after that, and a segment in which
+        // a trimmed solve ran out of launches is run again with full sequences.
         const double2* ckZ = r.Z + (size_t)seg * T_;
         const double* ckP = r.P + (size_t)seg * T_;
-        SFH_HIP(hipMemcpyAsync(segZ_, ckZ, zb, hipMemcpyDeviceToDevice, stream_));
-        SFH_HIP(hipMemcpyAsync(segP_, ckP, pb, hipMemcpyDeviceToDevice, stream_));
-        load_state(ckZ, ckP);
-        seq_len_ = gm_m_;
-        for (int k = 0; k < n; ++k) {
-            const StageCapture cap = seg_capture(k);
-            enqueue_step(r.dt, &cap);
-            if (k + 1 < n) {
-                SFH_HIP(hipMemcpyAsync(segZ_ + (size_t)(k + 1) * T_, Z0_, zb, hipMemcpyDeviceToDevice, stream_));
-                SFH_HIP(hipMemcpyAsync(segP_ + (size_t)(k + 1) * T_, P0_, pb, hipMemcpyDeviceToDevice, stream_));
+        auto rerun = [&](bool trimmed) {
+            SFH_HIP(hipMemcpyAsync(segZ_, ckZ, zb, hipMemcpyDeviceToDevice, stream_));
+            SFH_HIP(hipMemcpyAsync(segP_, ckP, pb, hipMemcpyDeviceToDevice, stream_));
+            load_state(ckZ, ckP);
+            for (int k = 0; k < n; ++k) {
+                seq_len_ = (trimmed && k >= kCalibSteps) ? trim_ : gm_m_;
+                const StageCapture cap = seg_capture(k);
+                enqueue_step(r.dt, &cap);
+                if (k + 1 < n) {
+                    SFH_HIP(hipMemcpyAsync(segZ_ + (size_t)(k + 1) * T_, Z0_, zb, hipMemcpyDeviceToDevice, stream_));
+                    SFH_HIP(hipMemcpyAsync(segP_ + (size_t)(k + 1) * T_, P0_, pb, hipMemcpyDeviceToDevice, stream_));
+                }
             }
+        };
+        const bool trim = opt_.solver == kSolveGMRES && trim_ >= 2 && trim_ < gm_m_ && n > kCalibSteps;   // calibrated
+        if (trim) {
+            sync();
+            const int unc0 = unc_raw_;
+            rerun(true);
+            sync();
+            if (unc_raw_ > unc0) {
+                unc_discard_ += unc_raw_ - unc0;
+                stats_.unconverged = unc_raw_ - unc_discard_;
+                stats_.redone_steps += n;
+                rerun(false);
+            }
+        } else {
+            rerun(false);
         }
         for (int i = n - 1; i >= 0; --i) {
             const StageCapture cap = seg_capture(i);

@@ -322,3 +322,35 @@ def test_device_trajectory_rejects_unsupported_problem_before_stepping(csh):
             e.traj_forward(0.004, 5, 2)
         Z, _ = e.get_state()
     assert np.array_equal(Z, Z0)   # nothing stepped
+
+
+def test_device_trajectory_redoes_segments_whose_trimmed_solves_ran_out(csh):
+    # force_trim = 2 gives every trimmed GMRES solve one Arnoldi step; the helium film's solves need
+    # more from their warm starts, so each segment's re-run runs out of launches after its history
+    # fills and must be run again with full sequences.  The forward pass's own redo doubles the
+    # length, so the handle is brought back to the forced length (a new state and the 6 steps of its
+    # calibration) before the backward pass, which needs only the record
+    n, dt, steps, every = 256, 0.004, 14, 7
+    props = csh.Props(**FILMS[0])
+    rng = np.random.default_rng(32)
+    Z0 = _surface(n, 60)
+    P0 = 0.01 * np.cos(2 * np.pi * np.arange(n) / n)
+    gZ = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    gP = rng.standard_normal(n)
+    print("device trajectory sweep, film, forced trim 2", flush=True)
+    with csh.Ensemble([n], props, solver=csh.GMRES, tol=1e-13, max_iter=32) as e:
+        _, _, gZ_h, gP_h = _host_sweep(e, Z0, P0, dt, steps, every, gZ, gP)
+    with csh.Ensemble([n], props, solver=csh.GMRES, tol=1e-13, max_iter=32, force_trim=2) as e:
+        e.set_state(Z0, P0)
+        rec = e.traj_forward(dt, steps, every)
+        e.set_state(Z0, P0)
+        e.step(dt, 6)
+        redone_fwd = e.stats()["redone_steps"]
+        gZ_d, gP_d = e.traj_backward(rec, gZ, gP)
+        st = e.stats()
+        e.traj_release(rec)
+    dg = max(np.abs(gZ_d - gZ_h).max(), np.abs(gP_d - gP_h).max()) / max(np.abs(gZ_h).max(), np.abs(gP_h).max())
+    print(f"  redone steps before the sweep {redone_fwd}, after it {st['redone_steps']}; "
+          f"cotangent rel diff {dg:.2e}", flush=True)
+    assert st["redone_steps"] >= redone_fwd + 2 * every   # both segments were run again
+    assert dg <= 1e-10
