@@ -155,9 +155,11 @@ __device__ inline void velocity_rows(const SurfDesc* __restrict__ sd, const Node
             if (u < kCircWin) c1w[u] = make_double2(ar[e], ai[e]);
         }
         __syncthreads();
-        // a solve finished by launch 0 (|r0| within the tolerance) keeps x0
+        // a solve finished by launch 0 (|r0| within the tolerance, done = 1) keeps x0.  done = 2 is
+        // this launch's own finish, stamped by workgroup 0, which a workgroup that starts after it
+        // (more workgroups than fit the GPU at once) reads here: it does not count
         const double vw = sums[0], ww = sums[1];
-        alpha = (fin == 0 && ww > 0.0) ? vw / ww : 0.0;
+        alpha = (fin != 1 && ww > 0.0) ? vw / ww : 0.0;
     }
     for (int j0 = 0; j0 < n; j0 += 8 * 256) {
         if (G1 && j0 > 0) {   // the kernel window of this pass (n > 2048), one entry at a time

@@ -445,6 +445,38 @@ def test_gmres_one_step_sequence_uneven_ensemble(csh):
     assert np.abs(Pg - Pl).max() < 1e-11
 
 
+def test_gmres_one_step_sequence_n4096_matches_lu(csh):
+    # at N = 4096 the one-step sequence's velocity kernel has 1024 workgroups, more than the GPU holds
+    # at once, so some start after workgroup 0 has stamped the solve finished: they must still apply
+    # the step's correction (only a finish stamped by the product launch keeps the initial guess).
+    # Water, 14 steps (the last 8 on the one-step sequence) against the LU stepper.
+    n, dt = 4096, 0.002
+    Z0 = _surface(n, 23)
+    P0 = 0.004 * np.sin(2 * np.pi * np.arange(n) / n)
+    with csh.Ensemble([n], solver=csh.LU) as e:
+        e.set_state(Z0, P0)
+        e.step(dt, 14)
+        Zl, Pl = e.get_state()
+    with csh.Ensemble([n], solver=csh.GMRES, tol=1e-12, max_iter=32) as e:
+        e.set_state(Z0, P0)
+        e.step(dt, 8)
+        e.kernel_timing(True)
+        e.step(dt, 2)
+        e.sync()
+        vel = e.kernel_report("velocity")
+        close = e.kernel_report("gm_close_mu")
+        e.kernel_timing(False)
+        e.step(dt, 4)
+        Zg, Pg = e.get_state()
+        st = e.stats()
+    print(f"one-step sequence n=4096: velocity {vel['launches']} close {close['launches']} unconverged "
+          f"{st['unconverged']} |dZ| {np.abs(Zg - Zl).max():.2e}", flush=True)
+    assert vel["launches"] == 8 and close["launches"] == 0
+    assert st["unconverged"] == 0
+    assert np.abs(Zg - Zl).max() < 1e-11
+    assert np.abs(Pg - Pl).max() < 1e-11
+
+
 def test_gmres_uneven_ensemble_matches_oracle(oracle, csh):
     # several surfaces of different N in one GMRES solve sequence (one grid slice per surface,
     # each with its own control block, partial dots and history), 12 steps: the trimmed sequence
