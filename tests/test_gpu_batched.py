@@ -34,7 +34,7 @@ def _close(a, b, tol, what=""):
     assert err <= tol * scale, f"{what} max err {err:.3e} > {tol:.1e} * {scale:.3e}"
 
 
-SIZES = [1, 2, 3, 4, 6, 16, 96, 128, 256, 384, 768, 1024, 512, 64]
+SIZES = [1, 2, 3, 4, 6, 16, 96, 128, 192, 256, 384, 768, 1024, 512, 64]
 
 
 def _ensemble_state(sizes, seed, amp=0.05):
