@@ -240,8 +240,9 @@ typedef struct sfh_options {
     int restart;      /* unused (0): the GMRES of SFH_SOLVE_GMRES does not restart, the in-workgroup
                          GMRES of SFH_SOLVE_BATCHED restarts every 8 basis vectors */
     int force_trim;   /* test hook (0 = off): length of the trimmed GMRES launch sequence instead of
-                         the calibrated one (at least 2); a chunk of steps in which a solve runs out
-                         of launches is still run again with the length doubled */
+                         the calibrated one (at least 2), also for the warm-started transposed solves
+                         of sfh_traj_backward; a chunk of steps (a segment of the sweep) in which a
+                         solve runs out of launches is still run again with the length doubled */
     double tol;       /* iterative solver / refinement relative tolerance */
     int pair_rows;    /* rows per workgroup of the pair kernels (assembly, velocity): 1, 2 or 4; 0 = 4 */
     int reserved;

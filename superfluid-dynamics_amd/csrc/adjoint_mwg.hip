@@ -202,6 +202,51 @@ __global__ __launch_bounds__(256) void adj_gm_start_kernel(const SurfDesc* __res
     }
 }
 
+// The warm start of a transposed solve: x0 = the extrapolation (quartic once five are kept) of the
+// solutions of the same stage over the previous reverse steps, kept in a ring of kHistDepth vectors
+// (slot (head - l) mod kHistDepth holds the solution l steps back).
+__global__ __launch_bounds__(256) void adj_extrapolate_kernel(const double* __restrict__ ring, int head, int hv,
+                                                              double* __restrict__ x0, int total) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= total) return;
+    double h[kHistDepth];
+#pragma unroll
+    for (int l = 0; l < kHistDepth; ++l) {
+        const int slot = (head - l + kHistDepth) % kHistDepth;
+        h[l] = l < hv ? ring[(long long)slot * total + i] : 0.0;
+    }
+    x0[i] = extrapolate_values(h, hv);
+}
+
+// The GMRES launch sequence on M^T from the warm start x0: V[0] = b - M^T x0 (one wave per row of the
+// explicit M^T, 4 rows per workgroup), x = x0, and the control block re-armed.
+__global__ __launch_bounds__(256) void adj_gm_start_warm_kernel(const SurfDesc* __restrict__ sd,
+                                                                const double* __restrict__ Mt,
+                                                                const double* __restrict__ b,
+                                                                const double* __restrict__ x0,
+                                                                double* __restrict__ V, double* __restrict__ x,
+                                                                int* __restrict__ ctl, int m) {
+    const SurfDesc d = sd[blockIdx.y];
+    const int n = d.n, lane = threadIdx.x & 63;
+    const int k = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        GmCtl* c = reinterpret_cast<GmCtl*>(ctl) + blockIdx.y;
+        c->done = 0;
+        c->refine_iters = 0;
+        c->iters = 0;
+    }
+    if (k >= n) return;   // the whole wave
+    const double* row = Mt + d.moff + (long long)k * n;
+    const double* xs = x0 + d.voff;
+    double acc = 0.0;
+    for (int j = lane; j < n; j += 64) acc += row[j] * xs[j];
+    acc = wave_sum(acc);
+    if (lane == 0) {
+        V[(long long)(m + 1) * d.voff + k] = b[d.voff + k] - acc;
+        x[d.voff + k] = xs[k];
+    }
+}
+
 // Mt = M^T per surface, through 32 x 32 LDS tiles.
 __global__ __launch_bounds__(256) void transpose_kernel(const SurfDesc* __restrict__ sd, const double* __restrict__ M,
                                                         double* __restrict__ Mt) {

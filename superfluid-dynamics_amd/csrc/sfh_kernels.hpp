@@ -274,6 +274,9 @@ template <int R> __global__ void adj_pass_b_kernel(AdjMArgs a, Phys ph);
 template <int R> __global__ void adj_pass_c_kernel(AdjMArgs a, Phys ph);
 __global__ void adj_circ_d1_kernel(const SurfDesc* sd, const double* circ, const double2* v, double2* out);
 __global__ void adj_gm_start_kernel(const SurfDesc* sd, const double* b, double* V, double* x, int* ctl, int m);
+__global__ void adj_extrapolate_kernel(const double* ring, int head, int hv, double* x0, int total);
+__global__ void adj_gm_start_warm_kernel(const SurfDesc* sd, const double* Mt, const double* b, const double* x0,
+                                         double* V, double* x, int* ctl, int m);
 __global__ void transpose_kernel(const SurfDesc* sd, const double* M, double* Mt);
 __global__ void adj_gamma_kernel(AdjMArgs a, Phys ph);
 __global__ void adj_final_kernel(AdjMArgs a, const double* circ, const double2* sd1gb);

@@ -290,7 +290,7 @@ Solver::~Solver() {
     void* vbufs[] = {vY_[0], vY_[1], vY_[2], vK_[0], vK_[1], vK_[2], vYP_[0], vYP_[1], vYP_[2], vKP_[0],
                      vKP_[1], vKP_[2], vMu_[0], vMu_[1], vMu_[2], vMu_[3], gA_, gR_[0], gR_[1], gAcc_,
                      gAP_, gRP_[0], gRP_[1], gAccP_, adj_, vstats_, Mt_, agwc_, agmup_, ad1_, agam_, agamI_,
-                     agzp_, agzpp_, agz_, agmu_, agb_};
+                     agzp_, agzpp_, agz_, agmu_, agb_, adjRing_, adjSave_, adjX_, adjSaveZ_, adjSaveP_};
     for (void* p : vbufs)
         if (p) (void)hipFree(p);
     for (auto& kv : traj_) {
@@ -1264,6 +1264,11 @@ void Solver::ensure_vjp() {
         agmu_ = dalloc<double>(T_);
         agb_ = dalloc<double>(T_);
         SFH_HIP(hipMemset(agamI_, 0, T_ * sizeof(double2)));
+        adjRing_ = dalloc<double>((size_t)4 * kHistDepth * T_);
+        adjSave_ = dalloc<double>((size_t)4 * kHistDepth * T_);
+        adjX_ = dalloc<double>(T_);
+        adjSaveZ_ = dalloc<double2>(T_);
+        adjSaveP_ = dalloc<double>(T_);
     }
     for (int i = 0; i < 3; ++i) {
         vY_[i] = dalloc<double2>(T_);
@@ -1288,9 +1293,9 @@ void Solver::ensure_vjp() {
 
 void Solver::enqueue_vjp(const double2* Z, const double* P, const double* mu, const double2* gAZ, const double* gAP,
                          double ca, const double2* gBZ, const double* gBP, double cb, double2* outZ, double* outP,
-                         bool accumulate) {
+                         bool accumulate, int stage) {
     if (opt_.solver != kSolveBatched) {
-        enqueue_vjp_mwg(Z, P, mu, gAZ, gAP, ca, gBZ, gBP, cb, outZ, outP, accumulate);
+        enqueue_vjp_mwg(Z, P, mu, gAZ, gAP, ca, gBZ, gBP, cb, outZ, outP, accumulate, stage);
         return;
     }
     VjpArgs a{};
@@ -1327,7 +1332,7 @@ void Solver::enqueue_vjp(const double2* Z, const double* P, const double* mu, co
 // guess, FP64), the matrix terms and pair weights, pass C, and the spectral adjoints.
 void Solver::enqueue_vjp_mwg(const double2* Z, const double* P, const double* mu, const double2* gAZ,
                              const double* gAP, double ca, const double2* gBZ, const double* gBP, double cb,
-                             double2* outZ, double* outP, bool accumulate) {
+                             double2* outZ, double* outP, bool accumulate, int stage) {
     double nsq = 0.0;
     for (int v : n_) nsq += (double)v * v;
     const double img = ph_.image ? 2.0 : 1.0;
@@ -1380,14 +1385,26 @@ void Solver::enqueue_vjp_mwg(const double2* Z, const double* P, const double* mu
            (const NodeTrig*)trig_, Z, Zp_, Zpp_, M_, ph_, (const int*)nullptr, GmArgs{}, 0);
         go(transpose_kernel, dim3((maxN_ + 31) / 32, (maxN_ + 31) / 32, S_), dim3(256), 0, d_desc_, (const double*)M_,
            Mt_);
-        go(adj_gm_start_kernel, ngrid, dim3(256), 0, d_desc_, (const double*)agmu_, gmV_, gmx0_, gmctl_, gm_m_);
+        // inside a trajectory sweep: the guess extrapolated from this stage's solutions of the previous
+        // reverse steps, and once calibrated a trimmed sequence; else a zero guess, the full sequence
+        const bool warm = adj_warm_ && stage >= 0 && adj_hv_[stage] > 0;
+        const int m = (warm && adj_len_ > 0 && adj_hv_[stage] >= kHistDepth) ? adj_len_ : gm_m_;
+        if (warm) {
+            go(adj_extrapolate_kernel, dim3((T_ + 255) / 256), dim3(256), 0,
+               (const double*)(adjRing_ + (size_t)stage * kHistDepth * T_), adj_head_[stage], adj_hv_[stage], adjX_,
+               T_);
+            go(adj_gm_start_warm_kernel, dim3((maxN_ + 3) / 4, S_), dim3(256), 0, d_desc_, (const double*)Mt_,
+               (const double*)agmu_, (const double*)adjX_, gmV_, gmx0_, gmctl_, m);
+        } else {
+            go(adj_gm_start_kernel, ngrid, dim3(256), 0, d_desc_, (const double*)agmu_, gmV_, gmx0_, gmctl_, m);
+        }
         GmArgs g = gm_args(0);
         g.M = Mt_;
         g.M32 = nullptr;
         g.b = agmu_;
         g.x = gmx0_;
         g.sol = agb_;
-        g.m = gm_m_;
+        g.m = m;
         g.tol = std::min(opt_.tol, 1e-13);
         g.hist = nullptr;
         g.hvalid = nullptr;
@@ -1395,6 +1412,19 @@ void Solver::enqueue_vjp_mwg(const double2* Z, const double* P, const double* mu
         g.check_first = 0;
         enqueue_gmres(g, false, true);   // leaves s D1 gb in mup_
     });
+    if (adj_warm_ && stage >= 0) {   // the solution joins the stage's ring
+        const int slot = (adj_head_[stage] + 1) % kHistDepth;
+        SFH_HIP(hipMemcpyAsync(adjRing_ + ((size_t)stage * kHistDepth + slot) * T_, agb_, T_ * sizeof(double),
+                               hipMemcpyDeviceToDevice, stream_));
+        adj_head_[stage] = slot;
+        adj_hv_[stage] = std::min(adj_hv_[stage] + 1, kHistDepth);
+    }
+    if (adj_calib_) {   // the Arnoldi steps this solve took
+        std::vector<int> ctl((size_t)kGmCtlInts * S_);
+        SFH_HIP(hipStreamSynchronize(stream_));
+        SFH_HIP(hipMemcpy(ctl.data(), gmctl_, ctl.size() * sizeof(int), hipMemcpyDeviceToHost));
+        for (int s = 0; s < S_; ++s) adj_calib_max_ = std::max(adj_calib_max_, ctl[(size_t)kGmCtlInts * s + 2]);
+    }
     launch("vjp_pairs", img * nsq, 0.0, [&] {
         go(adj_gamma_kernel, ngrid, dim3(256), 0, a, ph_);
         go(adj_pass_c_kernel<kAdjRows>, pgrid, dim3(256), 0, a, ph_);
@@ -1490,10 +1520,10 @@ void Solver::enqueue_step_vjp(double dt, const double2* Z, const double* P, cons
         for (int i = 0; i < 4; ++i) sc.mu[i] = vMu_[i];
     }
     // reverse sweep
-    enqueue_vjp(sc.Y[2], sc.YP[2], sc.mu[3], gA_, gAP_, dt / 6.0, nullptr, nullptr, 0.0, gR_[0], gRP_[0], true);
-    enqueue_vjp(sc.Y[1], sc.YP[1], sc.mu[2], gA_, gAP_, dt / 3.0, gR_[0], gRP_[0], dt, gR_[1], gRP_[1], true);
-    enqueue_vjp(sc.Y[0], sc.YP[0], sc.mu[1], gA_, gAP_, dt / 3.0, gR_[1], gRP_[1], 0.5 * dt, gR_[0], gRP_[0], true);
-    enqueue_vjp(Z, P, sc.mu[0], gA_, gAP_, dt / 6.0, gR_[0], gRP_[0], 0.5 * dt, gR_[1], gRP_[1], true);
+    enqueue_vjp(sc.Y[2], sc.YP[2], sc.mu[3], gA_, gAP_, dt / 6.0, nullptr, nullptr, 0.0, gR_[0], gRP_[0], true, 3);
+    enqueue_vjp(sc.Y[1], sc.YP[1], sc.mu[2], gA_, gAP_, dt / 3.0, gR_[0], gRP_[0], dt, gR_[1], gRP_[1], true, 2);
+    enqueue_vjp(sc.Y[0], sc.YP[0], sc.mu[1], gA_, gAP_, dt / 3.0, gR_[1], gRP_[1], 0.5 * dt, gR_[0], gRP_[0], true, 1);
+    enqueue_vjp(Z, P, sc.mu[0], gA_, gAP_, dt / 6.0, gR_[0], gRP_[0], 0.5 * dt, gR_[1], gRP_[1], true, 0);
 }
 
 Solver::StageCapture Solver::seg_capture(int k) const {
@@ -1578,6 +1608,25 @@ void Solver::traj_backward(int record, const double* gx_in, const double* gy_in,
     upload_cotangent(gx_in, gy_in, gp_in, gA_, gAP_);
     const int nck = (r.steps + r.every - 1) / r.every;
     const size_t zb = T_ * sizeof(double2), pb = T_ * sizeof(double);
+    // transposed solves (LU and GMRES handles): warm-started from each stage's solutions of the
+    // previous reverse steps; the first reverse step with five of them for every stage measures the
+    // Arnoldi steps its solves take, and from the next segment on the solves get that many launches
+    // plus a spare.  A segment in which one ran out is swept again with full sequences from its saved
+    // start, and the length doubles.
+    const bool mwg = opt_.solver != kSolveBatched;
+    struct SweepState {   // the warm-start state lives only inside this sweep, whatever ends it
+        Solver* s;
+        ~SweepState() {
+            s->adj_warm_ = false;
+            s->adj_calib_ = false;
+            s->adj_len_ = 0;
+        }
+    } sweep_state{this};
+    adj_warm_ = mwg;
+    adj_len_ = 0;
+    for (int st = 0; st < 4; ++st) adj_head_[st] = adj_hv_[st] = 0;
+    bool calibrated = false;
+    int pending_len = 0;
     for (int seg = nck - 1; seg >= 0; --seg) {
         const int n = std::min(r.every, r.steps - seg * r.every);
         // y_s, ..., y_{s+n-1}: the checkpoint, then n - 1 steps from it, each step keeping its stage
@@ -1616,11 +1665,56 @@ void Solver::traj_backward(int record, const double* gx_in, const double* gy_in,
         } else {
             rerun(false);
         }
-        for (int i = n - 1; i >= 0; --i) {
-            const StageCapture cap = seg_capture(i);
-            enqueue_step_vjp(r.dt, segZ_ + (size_t)i * T_, segP_ + (size_t)i * T_, &cap);
-            SFH_HIP(hipMemcpyAsync(gA_, gAcc_, zb, hipMemcpyDeviceToDevice, stream_));
-            SFH_HIP(hipMemcpyAsync(gAP_, gAccP_, pb, hipMemcpyDeviceToDevice, stream_));
+        auto sweep = [&]() {
+            for (int i = n - 1; i >= 0; --i) {
+                bool calib = mwg && !calibrated;
+                for (int st = 0; st < 4; ++st) calib = calib && adj_hv_[st] >= kHistDepth;
+                if (calib) {
+                    adj_calib_ = true;
+                    adj_calib_max_ = 0;
+                }
+                const StageCapture cap = seg_capture(i);
+                enqueue_step_vjp(r.dt, segZ_ + (size_t)i * T_, segP_ + (size_t)i * T_, &cap);
+                if (calib) {
+                    adj_calib_ = false;
+                    calibrated = true;
+                    pending_len = std::min(gm_m_, adj_calib_max_ + 2);
+                    if (opt_.force_trim > 0) pending_len = std::max(2, std::min(gm_m_, opt_.force_trim));
+                }
+                SFH_HIP(hipMemcpyAsync(gA_, gAcc_, zb, hipMemcpyDeviceToDevice, stream_));
+                SFH_HIP(hipMemcpyAsync(gAP_, gAccP_, pb, hipMemcpyDeviceToDevice, stream_));
+            }
+        };
+        const bool trimmed = mwg && adj_len_ > 0;
+        if (trimmed) {
+            const size_t ring = (size_t)4 * kHistDepth * T_ * sizeof(double);
+            SFH_HIP(hipMemcpyAsync(adjSaveZ_, gA_, zb, hipMemcpyDeviceToDevice, stream_));
+            SFH_HIP(hipMemcpyAsync(adjSaveP_, gAP_, pb, hipMemcpyDeviceToDevice, stream_));
+            SFH_HIP(hipMemcpyAsync(adjSave_, adjRing_, ring, hipMemcpyDeviceToDevice, stream_));
+            int head[4], hv[4];
+            std::copy(adj_head_, adj_head_ + 4, head);
+            std::copy(adj_hv_, adj_hv_ + 4, hv);
+            sync();
+            const int unc0 = unc_raw_;
+            sweep();
+            sync();
+            if (unc_raw_ > unc0) {
+                unc_discard_ += unc_raw_ - unc0;
+                stats_.unconverged = unc_raw_ - unc_discard_;
+                stats_.redone_steps += n;
+                SFH_HIP(hipMemcpyAsync(gA_, adjSaveZ_, zb, hipMemcpyDeviceToDevice, stream_));
+                SFH_HIP(hipMemcpyAsync(gAP_, adjSaveP_, pb, hipMemcpyDeviceToDevice, stream_));
+                SFH_HIP(hipMemcpyAsync(adjRing_, adjSave_, ring, hipMemcpyDeviceToDevice, stream_));
+                std::copy(head, head + 4, adj_head_);
+                std::copy(hv, hv + 4, adj_hv_);
+                const int len = adj_len_;
+                adj_len_ = 0;
+                sweep();
+                adj_len_ = std::min(gm_m_, 2 * len);
+            }
+        } else {
+            sweep();
+            if (calibrated && pending_len > 0) adj_len_ = pending_len;   // trimmed from the next segment on
         }
     }
     load_state(r.Z, r.P);   // the record's initial state

@@ -282,9 +282,11 @@ class Solver {
     void enqueue_batched(const double2* Z, const double* P, const StageCtl& ctl);
     // adjoint workspace (allocated on first use)
     void ensure_vjp();
+    // stage: the RK stage (0..3) of the reverse step, which selects the warm-start ring of the
+    // transposed solve inside a trajectory sweep; -1 outside one
     void enqueue_vjp(const double2* Z, const double* P, const double* mu, const double2* gAZ, const double* gAP,
                      double ca, const double2* gBZ, const double* gBP, double cb, double2* outZ, double* outP,
-                     bool accumulate);
+                     bool accumulate, int stage = -1);
     void upload_cotangent(const double* gx, const double* gy, const double* gp, double2* Z, double* P);
     void download_cotangent(const double2* Z, const double* P, double* gx, double* gy, double* gp);
     bool vjp_ready_ = false;
@@ -302,7 +304,19 @@ class Solver {
     double *agmu_ = nullptr, *agb_ = nullptr;
     void enqueue_vjp_mwg(const double2* Z, const double* P, const double* mu, const double2* gAZ, const double* gAP,
                          double ca, const double2* gBZ, const double* gBP, double cb, double2* outZ, double* outP,
-                         bool accumulate);
+                         bool accumulate, int stage);
+    // warm starts of the transposed solves over a trajectory's reverse steps (LU and GMRES handles):
+    // per RK stage a ring of the last kHistDepth solutions, extrapolated like the forward warm start
+    double* adjRing_ = nullptr;      // 4 stages x kHistDepth x T
+    double* adjX_ = nullptr;         // the extrapolated guess
+    double* adjSave_ = nullptr;      // the rings at the start of a segment (for a redo)
+    double2* adjSaveZ_ = nullptr;    // the cotangent at the start of a segment
+    double* adjSaveP_ = nullptr;
+    int adj_head_[4] = {0, 0, 0, 0}, adj_hv_[4] = {0, 0, 0, 0};
+    bool adj_warm_ = false;          // inside a trajectory sweep
+    int adj_len_ = 0;                // launches per warm transposed solve (0: the full sequence)
+    bool adj_calib_ = false;         // this reverse step measures the Arnoldi steps of its solves
+    int adj_calib_max_ = 0;
     // one RK4 step back from the state (Z, P): cotangent gA_/gAP_ of the state after it -> gAcc_/gAccP_.
     // The stage states and mu come from `stored` when given, else they are recomputed (solves from
     // zero guesses with the full launch sequence).

@@ -327,9 +327,11 @@ def test_device_trajectory_rejects_unsupported_problem_before_stepping(csh):
 def test_device_trajectory_redoes_segments_whose_trimmed_solves_ran_out(csh):
     # force_trim = 2 gives every trimmed GMRES solve one Arnoldi step; the helium film's solves need
     # more from their warm starts, so each segment's re-run runs out of launches after its history
-    # fills and must be run again with full sequences.  The forward pass's own redo doubles the
-    # length, so the handle is brought back to the forced length (a new state and the 6 steps of its
-    # calibration) before the backward pass, which needs only the record
+    # fills and must be run again with full sequences, and so does the reverse sweep of the second
+    # segment, whose warm-started transposed solves get the forced length after the calibration step
+    # of the first.  The forward pass's own redo doubles the length, so the handle is brought back to
+    # the forced length (a new state and the 6 steps of its calibration) before the backward pass,
+    # which needs only the record
     n, dt, steps, every = 256, 0.004, 14, 7
     props = csh.Props(**FILMS[0])
     rng = np.random.default_rng(32)
@@ -352,5 +354,5 @@ def test_device_trajectory_redoes_segments_whose_trimmed_solves_ran_out(csh):
     dg = max(np.abs(gZ_d - gZ_h).max(), np.abs(gP_d - gP_h).max()) / max(np.abs(gZ_h).max(), np.abs(gP_h).max())
     print(f"  redone steps before the sweep {redone_fwd}, after it {st['redone_steps']}; "
           f"cotangent rel diff {dg:.2e}", flush=True)
-    assert st["redone_steps"] >= redone_fwd + 2 * every   # both segments were run again
+    assert st["redone_steps"] >= redone_fwd + 3 * every   # both re-runs and one reverse sweep again
     assert dg <= 1e-10
