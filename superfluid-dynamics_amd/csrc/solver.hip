@@ -293,10 +293,7 @@ Solver::~Solver() {
                      agzp_, agzpp_, agz_, agmu_, agb_, adjRing_, adjSave_, adjX_, adjSaveZ_, adjSaveP_};
     for (void* p : vbufs)
         if (p) (void)hipFree(p);
-    for (auto& kv : traj_) {
-        (void)hipFree(kv.second.Z);
-        (void)hipFree(kv.second.P);
-    }
+    for (auto& kv : traj_) free_record(kv.second);
     for (void* p : {(void*)segZ_, (void*)segP_, (void*)segY_, (void*)segYP_, (void*)segMu_})
         if (p) (void)hipFree(p);
     for (hipEvent_t e : event_pool_) (void)hipEventDestroy(e);
@@ -1567,9 +1564,28 @@ int Solver::traj_forward(double dt, int steps, int every) {
     const int id = traj_next_++;
     traj_[id] = r;
     try {
+        TrajRecord& rr = traj_[id];
+        double* hist = opt_.solver == kSolveGMRES ? gmhist_ : opt_.solver == kSolveBatched ? bhist_ : nullptr;
+        int* hvalid = opt_.solver == kSolveGMRES ? gmhvalid_ : opt_.solver == kSolveBatched ? bhvalid_ : nullptr;
+        rr.hist_len = hist ? (size_t)4 * kHistDepth * T_ : 0;
+        rr.hv_len = !hvalid ? 0 : opt_.solver == kSolveGMRES ? 4 : (size_t)4 * S_;
+        rr.Mu = dalloc<double>((size_t)nck * T_);
+        if (hist) {
+            rr.H = dalloc<double>((size_t)nck * rr.hist_len);
+            rr.HV = dalloc<int>((size_t)nck * rr.hv_len);
+        }
+        rr.hist_steps.assign(nck, 0);
         for (int c = 0; c < nck; ++c) {
-            SFH_HIP(hipMemcpyAsync(r.Z + (size_t)c * T_, Z0_, T_ * sizeof(double2), hipMemcpyDeviceToDevice, stream_));
-            SFH_HIP(hipMemcpyAsync(r.P + (size_t)c * T_, P0_, T_ * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+            SFH_HIP(hipMemcpyAsync(rr.Z + (size_t)c * T_, Z0_, T_ * sizeof(double2), hipMemcpyDeviceToDevice, stream_));
+            SFH_HIP(hipMemcpyAsync(rr.P + (size_t)c * T_, P0_, T_ * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+            SFH_HIP(hipMemcpyAsync(rr.Mu + (size_t)c * T_, mu_, T_ * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+            if (hist) {
+                SFH_HIP(hipMemcpyAsync(rr.H + c * rr.hist_len, hist, rr.hist_len * sizeof(double),
+                                       hipMemcpyDeviceToDevice, stream_));
+                SFH_HIP(hipMemcpyAsync(rr.HV + c * rr.hv_len, hvalid, rr.hv_len * sizeof(int), hipMemcpyDeviceToDevice,
+                                       stream_));
+            }
+            rr.hist_steps[c] = hist_steps_;
             step(dt, std::min(every, steps - c * every));
         }
         check_status();
@@ -1636,12 +1652,26 @@ void Solver::traj_backward(int record, const double* gx_in, const double* gy_in,
         // a trimmed solve ran out of launches is run again with full sequences.
         const double2* ckZ = r.Z + (size_t)seg * T_;
         const double* ckP = r.P + (size_t)seg * T_;
+        // the steps from the checkpoint before the stage histories hold kHistDepth solutions get the full
+        // sequence (only in a trajectory's first segment: later checkpoints keep full histories)
+        const int cold = std::max(0, kCalibSteps - r.hist_steps[seg]);
         auto rerun = [&](bool trimmed) {
             SFH_HIP(hipMemcpyAsync(segZ_, ckZ, zb, hipMemcpyDeviceToDevice, stream_));
             SFH_HIP(hipMemcpyAsync(segP_, ckP, pb, hipMemcpyDeviceToDevice, stream_));
+            // the checkpoint's state and the solver's warm-start state there, as the forward pass had them
             load_state(ckZ, ckP);
+            SFH_HIP(hipMemcpyAsync(mu_, r.Mu + (size_t)seg * T_, pb, hipMemcpyDeviceToDevice, stream_));
+            if (r.hist_len) {
+                double* hist = opt_.solver == kSolveGMRES ? gmhist_ : bhist_;
+                int* hvalid = opt_.solver == kSolveGMRES ? gmhvalid_ : bhvalid_;
+                SFH_HIP(hipMemcpyAsync(hist, r.H + seg * r.hist_len, r.hist_len * sizeof(double), hipMemcpyDeviceToDevice,
+                                       stream_));
+                SFH_HIP(hipMemcpyAsync(hvalid, r.HV + seg * r.hv_len, r.hv_len * sizeof(int), hipMemcpyDeviceToDevice,
+                                       stream_));
+            }
+            hist_steps_ = r.hist_steps[seg];
             for (int k = 0; k < n; ++k) {
-                seq_len_ = (trimmed && k >= kCalibSteps) ? trim_ : gm_m_;
+                seq_len_ = (trimmed && k >= cold) ? trim_ : gm_m_;
                 const StageCapture cap = seg_capture(k);
                 enqueue_step(r.dt, &cap);
                 if (k + 1 < n) {
@@ -1650,7 +1680,7 @@ void Solver::traj_backward(int record, const double* gx_in, const double* gy_in,
                 }
             }
         };
-        const bool trim = opt_.solver == kSolveGMRES && trim_ >= 2 && trim_ < gm_m_ && n > kCalibSteps;   // calibrated
+        const bool trim = opt_.solver == kSolveGMRES && trim_ >= 2 && trim_ < gm_m_ && n > cold;   // calibrated
         if (trim) {
             sync();
             const int unc0 = unc_raw_;
@@ -1722,12 +1752,21 @@ void Solver::traj_backward(int record, const double* gx_in, const double* gy_in,
     check_status();
 }
 
+void Solver::free_record(TrajRecord& r) {
+    for (void* p : {(void*)r.Z, (void*)r.P, (void*)r.Mu, (void*)r.H, (void*)r.HV})
+        if (p) (void)hipFree(p);
+    r.Z = nullptr;
+    r.P = nullptr;
+    r.Mu = nullptr;
+    r.H = nullptr;
+    r.HV = nullptr;
+}
+
 void Solver::traj_release(int record) {
     auto it = traj_.find(record);
     if (it == traj_.end()) throw ArgError("unknown trajectory record " + std::to_string(record));
     SFH_HIP(hipStreamSynchronize(stream_));
-    SFH_HIP(hipFree(it->second.Z));
-    SFH_HIP(hipFree(it->second.P));
+    free_record(it->second);
     traj_.erase(it);
 }
 

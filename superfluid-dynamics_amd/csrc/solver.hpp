@@ -328,7 +328,15 @@ class Solver {
         int steps = 0, every = 1;
         double2* Z = nullptr;   // checkpoint c at Z + c T
         double* P = nullptr;
+        // the solver's warm-start state at each checkpoint: the last mu, and (GMRES, batched) the
+        // stage histories and their valid counts, so a segment's re-run starts where the forward pass was
+        double* Mu = nullptr;   // c T
+        double* H = nullptr;    // c * hist_len
+        int* HV = nullptr;      // c * hv_len
+        size_t hist_len = 0, hv_len = 0;
+        std::vector<int> hist_steps;   // hist_steps_ at each checkpoint
     };
+    void free_record(TrajRecord& r);
     std::map<int, TrajRecord> traj_;
     int traj_next_ = 1;
     double2* segZ_ = nullptr;   // the states of one segment
