@@ -384,7 +384,8 @@ int sfh_step_vjp(sfh_handle h, double dt, const double* gx_in, const double* gy_
 /* Differentiable trajectories with device-resident checkpoints (no counterpart in the reference; the
  * forward map is sfh_step).  sfh_traj_forward advances `steps` RK4 steps of size dt from the current
  * state exactly as sfh_step(h, dt, ...) does, keeps in device memory the state before steps 0, every,
- * 2 every, ... (ceil(steps / every) states of 24 T bytes, T = sfh_total_nodes) and returns the id of
+ * 2 every, ... with the solver's warm-start state there (ceil(steps / every) checkpoints of 32 T bytes,
+ * 192 T bytes on GMRES and batched handles, T = sfh_total_nodes) and returns the id of
  * that record in *record; the handle's state afterwards is the final state.  sfh_traj_backward takes
  * the cotangent (gx, gy, gphi) of the state after the recorded steps to the cotangent of the state the
  * record started from: each segment, last first, is re-run from its checkpoint, keeping every step's
