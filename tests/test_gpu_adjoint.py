@@ -277,14 +277,17 @@ def _host_sweep(e, Z0, P0, dt, steps, every, gZ, gP):
     return ZT, PT, gZ, gP
 
 
-@pytest.mark.parametrize("solver,sizes,every,kw", [
-    ("GMRES", [1024], 4, None),
-    ("LU", [256], 3, None),
-    ("BATCHED", [128, 96], 5, None),
-    ("GMRES", [256], 4, "film"),
+@pytest.mark.parametrize("solver,sizes,every,steps,kw", [
+    ("GMRES", [1024], 4, 11, None),
+    ("LU", [256], 3, 11, None),
+    ("BATCHED", [128, 96], 5, 11, None),
+    ("GMRES", [256], 4, 11, "film"),
+    # two surfaces of one GMRES handle: per-surface Krylov blocks of the trimmed, warm-started transposed
+    # solves (the calibration falls in the first swept segment, the second is trimmed)
+    ("GMRES", [512, 256], 8, 16, None),
 ])
-def test_device_trajectory_sweep_matches_host_sweep(csh, solver, sizes, every, kw):
-    dt, steps = 0.004, 11
+def test_device_trajectory_sweep_matches_host_sweep(csh, solver, sizes, every, steps, kw):
+    dt = 0.004
     props = csh.Props(**(FILMS[0] if kw == "film" else {}))
     rng = np.random.default_rng(31)
     Z0 = np.concatenate([_surface(n, 40 + i) for i, n in enumerate(sizes)])
