@@ -7,9 +7,10 @@ depth, dt = 0.002.  A step is one classical RK4 step (four right-hand sides: spe
 dipole-matrix assembly, solve, velocity + Bernoulli + stage update), state resident in HBM, steps
 replayed from a captured hipGraph.  Default solver: FP64 GMRES (relative residual 1e-12, warm
 started); --solver lu selects the blocked FP64 LU.  The warm-up runs the requested W steps and,
-for GMRES, continues (untimed) through the solver's calibration: 5 steps fill the warm-start
-history, 1 sets the trimmed launch sequence and 2 verify it, so the timed K steps run the
-steady-state graph.
+for GMRES, continues (untimed) until the solver is in its steady state: 5 steps fill the warm-start
+history, 1 sets the trimmed launch sequence, and the chunks of steps checked for unconverged
+solves grow 2, 4, 8, 16 steps to the steady check interval of 32 steps (36 steps in all), so the
+timed K steps run the steady-state graph with one host check per 32 steps.
 
 Batched leg (`batched`): 512 independent surfaces with N drawn uniformly from {128, 256, 512, 1024}
 (default_rng(20261202)), surface i drawn with seed 20261202 + i as above, phi = 0, dt = 0.002; one
@@ -49,7 +50,9 @@ PEAK_HBM_GBS = 8000.0
 N = 2048
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic.json")
 GOLDEN = os.path.join(ROOT, "tests", "golden")
-CALIB_STEPS = 8   # GMRES: history fill (5) + calibration (1) + first verified chunk (2)
+# GMRES: history fill (5) + calibration (1) + the checked chunks growing 2, 4, 8, 16 steps to the steady
+# check interval of 32 steps
+CALIB_STEPS = 36
 
 from cusuperhelium import workloads as W  # noqa: E402  (pure numpy: no device, no library)
 
