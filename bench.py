@@ -193,6 +193,23 @@ def batched_leg(c, steps, with_accuracy):
                                         "worst_instance": int(np.argmax(dz))},
                          "max_abs_dmu": {"max": float(dm.max()), "median": float(np.median(dm))},
                          "instances_within_1e-9": int((dz <= 1e-9).sum())}
+        # every instance over the config's 500 steps: the mixed form against the FP64 form (both
+        # pinned to the oracle above), per instance
+        fin = {}
+        for prec, name in ((c.MIXED, "mixed"), (c.FP64, "fp64")):
+            with c.Ensemble(sizes, c.Props(), precision=prec, solver=c.BATCHED, tol=1e-12, max_iter=16) as e:
+                e.set_state(Z0, P0)
+                e.step(DT, 500)
+                fin[name] = e.get_state()
+                _, _, d = e.rhs(diagnostics=True)
+                fin[name] += (d["mu"],)
+        dz = np.array([np.abs(fin["mixed"][0][a:b] - fin["fp64"][0][a:b]).max() for a, b in zip(off[:-1], off[1:])])
+        dm = np.array([np.abs(fin["mixed"][2][a:b] - fin["fp64"][2][a:b]).max() for a, b in zip(off[:-1], off[1:])])
+        out["mixed_vs_fp64_500_steps"] = {"instances": len(sizes), "steps": 500,
+                                          "max_abs_dz": {"max": float(dz.max()), "median": float(np.median(dz)),
+                                                         "worst_instance": int(np.argmax(dz))},
+                                          "max_abs_dmu": {"max": float(dm.max()), "median": float(np.median(dm))},
+                                          "instances_within_1e-9": int((dz <= 1e-9).sum())}
         g = golden("c3_sample_500steps.npz")
         idx = [int(i) for i in g["index"]]
         sub = [sizes[i] for i in idx]

@@ -142,6 +142,26 @@ def test_batched_ensemble_20_steps_within_1e9(oracle, csh):
     assert worst <= 1e-9
 
 
+def test_batched_c3_every_instance_500_steps_mixed_within_1e9_of_fp64(csh):
+    # the whole 512-surface workload over its 500 steps: every instance of the mixed form against the
+    # FP64 form of the same kernel (itself checked against the oracle above and in the golden
+    # 500-step sample)
+    from cusuperhelium import workloads as W
+    sizes, Zs = W.c3()
+    Z0 = np.concatenate(Zs)
+    fin = {}
+    print(f"c3 ensemble: {len(sizes)} surfaces, 500 steps, mixed and fp64", flush=True)
+    for prec in (csh.MIXED, csh.FP64):
+        with csh.Ensemble(sizes, csh.Props(), precision=prec, solver=csh.BATCHED, tol=1e-12, max_iter=16) as e:
+            e.set_state(Z0, np.zeros(len(Z0)))
+            e.step(0.002, 500)
+            fin[prec] = e.get_state()[0]
+            assert e.stats()["unconverged"] == 0
+    dz = np.array([np.abs(a - b).max() for a, b in zip(_split(fin[csh.MIXED], sizes), _split(fin[csh.FP64], sizes))])
+    print(f"  max |z_mixed - z_fp64| per instance: max {dz.max():.2e}, median {np.median(dz):.2e}")
+    assert np.all(np.isfinite(dz)) and dz.max() <= 1e-9
+
+
 def test_batched_rejects_large_surfaces(csh):
     with pytest.raises(Exception):
         csh.Ensemble([2048], csh.Props(), solver=csh.BATCHED)
