@@ -90,19 +90,22 @@ __device__ inline void store_block(double* dst, const double* src, long long ld,
 // The elimination runs on 4 x 4 tiles: thread t owns tile (t / 16, t % 16) of the 64 x 64 block
 // (rows from kb on are padded with identity rows) in registers.  Block step kk (16 of them, two
 // barriers each):
-//   the diagonal tile's thread factors it (getf2 order: multipliers by the reciprocal pivot) and
-//   publishes L_kk \ U_kk and the reciprocals;
-//   the tiles below it form L_ik = A_ik U_kk^{-1}, the tiles right of it U_kj = L_kk^{-1} A_kj,
-//   and publish them;
-//   every trailing tile applies A_ij -= L_ik U_kj.
+//   the tiles below the diagonal tile form L_ik = A_ik U_kk^{-1}, the tiles right of it
+//   U_kj = L_kk^{-1} A_kj, and publish them;
+//   every trailing tile applies A_ij -= L_ik U_kj; the thread of tile (kk + 1, kk + 1) then
+//   factors it (getf2 order: multipliers by the reciprocal pivot) and publishes L \ U and the
+//   reciprocals for the next step (step 0's diagonal tile is factored before the loop).
 // Every entry sees the same multiply-adds in the same order as in the unblocked right-looking
 // elimination (step p's update of entry (r, c), for p = 0 .. min(r, c) - 1), so the factors are
-// those of getf2 without interchanges.  Per step a thread reads 32 published values, against 80
-// for the previous row-slot layout, whose LDS traffic bounded it.
+// those of getf2 without interchanges.  The published tiles are stored element-major (Lp[e][i],
+// Up[e][j]): the 16 threads of a row of tiles read 16 consecutive doubles of Up and 4 of Lp, free
+// of bank conflicts.  A step takes about 1.2 us, most of it the dependent chain of the diagonal
+// tile (four reciprocals with their Newton steps) and the two barrier-separated phases; forming
+// L_ik and U_kj in every trailing tile instead (one barrier per step) made the steps slower.
 // The inverses then follow in blocks (invert_factors).
 __device__ inline void eliminate_tiles(double* D, int kb, int& my_bad) {
     __shared__ double Dk[16], Dinv[4];             // the diagonal tile of the step, its reciprocals
-    __shared__ double Lp[16][16], Up[16][16];      // the step's L_ik (by i) and U_kj (by j) tiles
+    __shared__ double Lp[16][16], Up[16][16];      // the step's L_ik (Lp[e][i]) and U_kj (Up[e][j]) tiles
     const int t = threadIdx.x, ti = t >> 4, tj = t & 15;
     double A[4][4];
 #pragma unroll
@@ -112,30 +115,32 @@ __device__ inline void eliminate_tiles(double* D, int kb, int& my_bad) {
             const int gr = 4 * ti + r, gc = 4 * tj + c;
             A[r][c] = gr < kb ? D[gr * LD + gc] : (gc == gr ? 1.0 : 0.0);
         }
+    // factors this thread's tile as the diagonal tile of step k and publishes it
+    auto factor_diag = [&](int k) {
+        double inv[4];
+#pragma unroll
+        for (int p = 0; p < 4; ++p) {
+            inv[p] = fast_rcp(A[p][p]);
+#pragma unroll
+            for (int r = p + 1; r < 4; ++r) {
+                const double l = A[r][p] * inv[p];
+                if (4 * k + r < kb && !(fabs(l) <= 1.0)) my_bad = 1;
+                A[r][p] = l;
+#pragma unroll
+                for (int c = p + 1; c < 4; ++c) A[r][c] = fma(-l, A[p][c], A[r][c]);
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+#pragma unroll
+            for (int c = 0; c < 4; ++c) Dk[4 * r + c] = A[r][c];
+#pragma unroll
+        for (int p = 0; p < 4; ++p) Dinv[p] = inv[p];
+    };
+    if (t == 0) factor_diag(0);
+    __syncthreads();
 #pragma unroll 1
     for (int kk = 0; kk < 16; ++kk) {
-        if (ti == kk && tj == kk) {
-            double inv[4];
-#pragma unroll
-            for (int p = 0; p < 4; ++p) {
-                inv[p] = fast_rcp(A[p][p]);
-#pragma unroll
-                for (int r = p + 1; r < 4; ++r) {
-                    const double l = A[r][p] * inv[p];
-                    if (4 * kk + r < kb && !(fabs(l) <= 1.0)) my_bad = 1;
-                    A[r][p] = l;
-#pragma unroll
-                    for (int c = p + 1; c < 4; ++c) A[r][c] = fma(-l, A[p][c], A[r][c]);
-                }
-            }
-#pragma unroll
-            for (int r = 0; r < 4; ++r)
-#pragma unroll
-                for (int c = 0; c < 4; ++c) Dk[4 * r + c] = A[r][c];
-#pragma unroll
-            for (int p = 0; p < 4; ++p) Dinv[p] = inv[p];
-        }
-        __syncthreads();
         if (tj == kk && ti > kk) {   // L_ik = A_ik U_kk^{-1}, column by column
             double U[4][4], inv[4];
 #pragma unroll
@@ -158,7 +163,7 @@ __device__ inline void eliminate_tiles(double* D, int kb, int& my_bad) {
 #pragma unroll
             for (int r = 0; r < 4; ++r)
 #pragma unroll
-                for (int c = 0; c < 4; ++c) Lp[ti][4 * r + c] = A[r][c];
+                for (int c = 0; c < 4; ++c) Lp[4 * r + c][ti] = A[r][c];
         } else if (ti == kk && tj > kk) {   // U_kj = L_kk^{-1} A_kj, step by step
             double L[4][4];
 #pragma unroll
@@ -174,7 +179,7 @@ __device__ inline void eliminate_tiles(double* D, int kb, int& my_bad) {
 #pragma unroll
             for (int r = 0; r < 4; ++r)
 #pragma unroll
-                for (int c = 0; c < 4; ++c) Up[tj][4 * r + c] = A[r][c];
+                for (int c = 0; c < 4; ++c) Up[4 * r + c][tj] = A[r][c];
         }
         __syncthreads();
         if (ti > kk && tj > kk) {   // A_ij -= L_ik U_kj, the four steps in order
@@ -183,8 +188,8 @@ __device__ inline void eliminate_tiles(double* D, int kb, int& my_bad) {
             for (int r = 0; r < 4; ++r)
 #pragma unroll
                 for (int c = 0; c < 4; ++c) {
-                    L[r][c] = Lp[ti][4 * r + c];
-                    U[r][c] = Up[tj][4 * r + c];
+                    L[r][c] = Lp[4 * r + c][ti];
+                    U[r][c] = Up[4 * r + c][tj];
                 }
 #pragma unroll
             for (int p = 0; p < 4; ++p)
@@ -192,7 +197,9 @@ __device__ inline void eliminate_tiles(double* D, int kb, int& my_bad) {
                 for (int r = 0; r < 4; ++r)
 #pragma unroll
                     for (int c = 0; c < 4; ++c) A[r][c] = fma(-L[r][p], U[p][c], A[r][c]);
+            if (ti == kk + 1 && tj == kk + 1) factor_diag(kk + 1);
         }
+        __syncthreads();
     }
 #pragma unroll
     for (int r = 0; r < 4; ++r)
