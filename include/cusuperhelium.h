@@ -416,8 +416,8 @@ int sfh_get_stats(sfh_handle h, sfh_stats* out);
 int sfh_device_state(sfh_handle h, void** Z, void** phi);
 /* Kernel timing.  While enabled, sfh_step / sfh_rhs launch without graph replay and bracket
  * every kernel launch with hipEvents on the solver's stream.  sfh_kernel_report sums, for one
- * kernel tag ("deriv", "assemble", "lu_panel", "lu_update", "lu_backsolve", "mu_deriv",
- * "velocity"), the launches, milliseconds, algorithmic flops and algorithmic bytes recorded since
+ * kernel tag ("deriv", "assemble", "lu_step", "lu_backsolve", "mu_deriv", "velocity", and the GMRES
+ * tags "gm_iter", "gm_close", "gm_close_mu"), the launches, milliseconds, algorithmic flops and algorithmic bytes recorded since
  * timing was enabled.  No counterpart in the reference (it has no timers). */
 int sfh_kernel_timing(sfh_handle h, int enable);
 int sfh_kernel_report(sfh_handle h, const char* tag, int* launches, double* ms, double* flops, double* bytes);

@@ -4,12 +4,13 @@
 // Replaces MatrixSolver::solve (CuSuperHelium/MatrixSolver.cuh:113-172): cusolverDnDgetrf +
 // cusolverDnDgetrs (LU with partial pivoting), or getrfBatched/getrsBatched with a host sync.
 //
-// Algorithm.  Right-looking blocked LU with 64-wide panels.  For panel p (columns k0..k0+kb):
-//   lu_panel_kernel : every workgroup factors the kb x kb diagonal block A11 = L11 U11 in LDS
-//                     and inverts both factors; then workgroup 0 stores U11^{-1} and
-//                     y_p = L11^{-1} b_p, each "row" workgroup forms L21_i = A21_i U11^{-1} and
-//                     b_i -= L21_i y_p, each "column" workgroup forms U12_j = L11^{-1} A12_j.
-//   lu_update_kernel: A22 -= L21 U12 with v_mfma_f64_16x16x4 on 64 x 64 tiles.
+// Algorithm.  Right-looking blocked LU with 64-wide panels, one launch (lu_step_kernel) per panel
+// p (columns k0..k0+kb): every panel workgroup applies panel p-1's update to the diagonal block
+// and to its own block, factors the kb x kb diagonal block A11 = L11 U11 in LDS and inverts both
+// factors; then workgroup 0 stores U11^{-1} and y_p = L11^{-1} b_p, each "row" workgroup forms
+// L21_i = A21_i U11^{-1} and b_i -= L21_i y_p, each "column" workgroup forms
+// U12_j = L11^{-1} A12_j; the other workgroups of the launch apply panel p-1's update
+// A22 -= L21 U12 (v_mfma_f64_16x16x4 on 64 x 64 tiles) to the rest of the trailing matrix.
 // lu_backsolve_kernel then runs x_p = U_pp^{-1} (y_p - sum_{q>p} U_pq x_q), one workgroup per
 // block row, chained by per-block ready flags.
 //
@@ -400,8 +401,61 @@ __device__ inline LuView view(const LuArgs& a) {
     return v;
 }
 
-__global__ __launch_bounds__(256) void lu_panel_kernel(LuArgs a, int k0) {
+// C -= A_s B_s for the 64 x 64 tile C at (R0, C0) of the trailing matrix: A_s = the previous
+// panel's L21 block (rows R0.., columns kp..kp+63), B_s = its U12 block (rows kp.., columns C0..),
+// staged in LDS (As, Bs).
+__device__ inline void update_tile(const LuView& v, int n, int kp, int R0, int C0, double* As, double* Bs) {
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6, wr = w >> 1, wc = w & 1;
+    const int l16 = lane & 15, l4 = lane >> 4;
+    const int rb = min(kNB, n - R0), cb = min(kNB, n - C0);
+    load_block(As, v.A + (long long)R0 * n + kp, n, rb, kNB);
+    load_block(Bs, v.A + (long long)kp * n + C0, n, kNB, cb);
+    double* C = v.A + (long long)R0 * n + C0;
+    v4d acc[2][2];
+    for (int i = 0; i < 2; ++i)
+        for (int j = 0; j < 2; ++j)
+            for (int r = 0; r < 4; ++r) {
+                const int row = wr * 32 + i * 16 + 4 * r + l4, col = wc * 32 + j * 16 + l16;
+                acc[i][j][r] = (row < rb && col < cb) ? C[(long long)row * n + col] : 0.0;
+            }
+    __syncthreads();
+    mfma_block(As, Bs, acc, wr, wc, lane, -1.0);
+    for (int i = 0; i < 2; ++i)
+        for (int j = 0; j < 2; ++j)
+            for (int r = 0; r < 4; ++r) {
+                const int row = wr * 32 + i * 16 + 4 * r + l4, col = wc * 32 + j * 16 + l16;
+                if (row < rb && col < cb) C[(long long)row * n + col] = acc[i][j][r];
+            }
+}
+
+// Loads the rows x cols block of global memory at src (leading dimension ld) into the wave's
+// accumulator layout of mfma_block (zero outside the block).
+__device__ inline void load_acc(v4d acc[2][2], const double* src, long long ld, int rows, int cols) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, wr = w >> 1, wc = w & 1;
+    const int l16 = lane & 15, l4 = lane >> 4;
+    for (int i = 0; i < 2; ++i)
+        for (int j = 0; j < 2; ++j)
+            for (int r = 0; r < 4; ++r) {
+                const int row = wr * 32 + i * 16 + 4 * r + l4, col = wc * 32 + j * 16 + l16;
+                acc[i][j][r] = (row < rows && col < cols) ? src[(long long)row * ld + col] : 0.0;
+            }
+}
+
+// Panel p (columns k0 .. k0 + kb - 1) together with the previous panel's trailing update
+// (kp = k0 - 64; A22 -= L21 U12 with panel p-1's blocks), one launch per panel:
+//   roles 0 .. 2 nrb (panel p): each applies panel p-1's update to the diagonal block A_pp (kept in
+//     LDS; the diagonal blocks are never stored) and, for roles >= 1, to its own block A_ip (row
+//     roles) or A_pj (column roles) in registers, then factors A_pp = L11 U11 and inverts both
+//     factors; role 0 stores U11^{-1} and y_p = L11^{-1} b_p, row roles form L21_i = A_ip U11^{-1}
+//     and b_i -= L21_i y_p, column roles form U12_j = L11^{-1} A_pj;
+//   roles above 2 nrb: panel p-1's update of the trailing tiles (i, j), i, j > p, that no panel
+//     role owns.
+// Panel p's dependency chain (the diagonal block) thereby runs beside the bulk of the previous
+// panel's update rather than after it.  Every block sees the same products as with separate
+// update launches: the factors are unchanged bit for bit.
+__global__ __launch_bounds__(256) void lu_step_kernel(LuArgs a, int k0) {
     __shared__ double D[kNB * LD], Li[kNB * LD], Ui[kNB * LD], X[kNB * LD];
+    __shared__ double T[2 * kNB * kNB / 4];   // scratch of the blocked inversion
     __shared__ double yv[kNB], wv[kNB];
     __shared__ int bad;
     const LuView v = view(a);
@@ -411,15 +465,51 @@ __global__ __launch_bounds__(256) void lu_panel_kernel(LuArgs a, int k0) {
     const int nrb = (rem + kNB - 1) / kNB;
     const int role = blockIdx.x;
     const int lane = t & 63, w = t >> 6, wr = w >> 1, wc = w & 1;
+    const bool prev = k0 > 0;
+    const int kp = k0 - kNB;
+    if (role > 2 * nrb) {   // a trailing tile of panel p-1's update (exists only when prev)
+        const int r = role - 1 - 2 * nrb;
+        update_tile(v, n, kp, k0 + kb + (r / nrb) * kNB, k0 + kb + (r % nrb) * kNB, D, Li);
+        return;
+    }
+    const bool row_role = role >= 1 && role <= nrb, col_role = role > nrb;
+    const int R0 = k0 + kb + (role - 1) * kNB, rb = row_role ? min(kNB, n - R0) : 0;
+    const int C0 = k0 + kb + (role - 1 - nrb) * kNB, cb = col_role ? min(kNB, n - C0) : 0;
     if (t == 0) bad = 0;
     if (k0 == 0 && role == 0) {
         const int P = (n + kNB - 1) / kNB;
         for (int i = t; i < P; i += blockDim.x) v.flags[i] = 0;
     }
+    // the own block (rows x cols of A), with panel p-1's update applied below
+    v4d own[2][2];
+    if (row_role)
+        load_acc(own, v.A + (long long)R0 * n + k0, n, rb, kb);
+    else if (col_role)
+        load_acc(own, v.A + (long long)k0 * n + C0, n, kb, cb);
     load_block(D, v.A + (long long)k0 * n + k0, n, kb, kb);
+    if (prev) {
+        load_block(Li, v.A + (long long)k0 * n + kp, n, kb, kNB);    // L21 of panel p-1, block row p
+        load_block(Ui, v.A + (long long)kp * n + k0, n, kNB, kb);    // U12 of panel p-1, block column p
+        if (row_role) load_block(X, v.A + (long long)R0 * n + kp, n, rb, kNB);   // L21 of panel p-1, row i
+        if (col_role) load_block(X, v.A + (long long)kp * n + C0, n, kNB, cb);   // U12 of panel p-1, column j
+    }
     if (t < kNB) wv[t] = (t < kb) ? v.work[k0 + t] : 0.0;
     __syncthreads();
-    factor_invert_diag(D, Li, Ui, X, kb, &bad);   // X is free until the panel products
+    if (prev) {
+        v4d dg[2][2];
+        const int l16 = lane & 15, l4 = lane >> 4;
+        for (int i = 0; i < 2; ++i)
+            for (int j = 0; j < 2; ++j)
+                for (int r = 0; r < 4; ++r)
+                    dg[i][j][r] = D[(wr * 32 + i * 16 + 4 * r + l4) * LD + wc * 32 + j * 16 + l16];
+        mfma_block(Li, Ui, dg, wr, wc, lane, -1.0);                  // A_pp -= L21_p U12_p
+        if (row_role) mfma_block(X, Ui, own, wr, wc, lane, -1.0);   // A_ip -= L21_i U12_p
+        if (col_role) mfma_block(Li, X, own, wr, wc, lane, -1.0);   // A_pj -= L21_p U12_j
+        __syncthreads();
+        acc_to_lds(D, dg, wr, wc, lane);
+        __syncthreads();
+    }
+    factor_invert_diag(D, Li, Ui, T, kb, &bad);
     // y_p = L11^{-1} b_p (4 threads per row, operands staged in registers)
     {
         const int r = t >> 2, part = t & 3;
@@ -449,14 +539,12 @@ __global__ __launch_bounds__(256) void lu_panel_kernel(LuArgs a, int k0) {
         if (t == 0 && bad) atomicOr(&v.status[0], 1);
         return;
     }
+    acc_to_lds(X, own, wr, wc, lane);   // the updated own block
     v4d acc[2][2];
     for (int i = 0; i < 2; ++i)
         for (int j = 0; j < 2; ++j) acc[i][j] = v4d{0.0, 0.0, 0.0, 0.0};
-    if (role <= nrb) {
-        const int R0 = k0 + kb + (role - 1) * kNB;
-        const int rb = min(kNB, n - R0);
-        load_block(X, v.A + (long long)R0 * n + k0, n, rb, kb);
-        __syncthreads();
+    __syncthreads();
+    if (row_role) {
         mfma_block(X, Ui, acc, wr, wc, lane, 1.0);        // L21 = A21 U11^{-1}
         __syncthreads();
         acc_to_lds(X, acc, wr, wc, lane);
@@ -485,10 +573,6 @@ __global__ __launch_bounds__(256) void lu_panel_kernel(LuArgs a, int k0) {
             if (part == 0 && r < rb) v.work[R0 + r] -= s;
         }
     } else {
-        const int C0 = k0 + kb + (role - 1 - nrb) * kNB;
-        const int cb = min(kNB, n - C0);
-        load_block(X, v.A + (long long)k0 * n + C0, n, kb, cb);
-        __syncthreads();
         mfma_block(Li, X, acc, wr, wc, lane, 1.0);        // U12 = L11^{-1} A12
         __syncthreads();
         acc_to_lds(X, acc, wr, wc, lane);
@@ -496,36 +580,6 @@ __global__ __launch_bounds__(256) void lu_panel_kernel(LuArgs a, int k0) {
         store_block(v.A + (long long)k0 * n + C0, X, n, kb, cb);
     }
     if (t == 0 && bad && role == 1) atomicOr(&v.status[0], 1);
-}
-
-// A22 -= L21 U12 on the 64 x 64 tile (blockIdx.y, blockIdx.x) of the trailing matrix.
-__global__ __launch_bounds__(256) void lu_update_kernel(LuArgs a, int k0) {
-    __shared__ double As[kNB * LD], Bs[kNB * LD];
-    const LuView v = view(a);
-    const int n = a.n, t = threadIdx.x;
-    const int kb = min(kNB, n - k0);
-    const int R0 = k0 + kb + blockIdx.y * kNB, C0 = k0 + kb + blockIdx.x * kNB;
-    const int rb = min(kNB, n - R0), cb = min(kNB, n - C0);
-    const int lane = t & 63, w = t >> 6, wr = w >> 1, wc = w & 1;
-    const int l16 = lane & 15, l4 = lane >> 4;
-    load_block(As, v.A + (long long)R0 * n + k0, n, rb, kb);
-    load_block(Bs, v.A + (long long)k0 * n + C0, n, kb, cb);
-    double* C = v.A + (long long)R0 * n + C0;
-    v4d acc[2][2];
-    for (int i = 0; i < 2; ++i)
-        for (int j = 0; j < 2; ++j)
-            for (int r = 0; r < 4; ++r) {
-                const int row = wr * 32 + i * 16 + 4 * r + l4, col = wc * 32 + j * 16 + l16;
-                acc[i][j][r] = (row < rb && col < cb) ? C[(long long)row * n + col] : 0.0;
-            }
-    __syncthreads();
-    mfma_block(As, Bs, acc, wr, wc, lane, -1.0);
-    for (int i = 0; i < 2; ++i)
-        for (int j = 0; j < 2; ++j)
-            for (int r = 0; r < 4; ++r) {
-                const int row = wr * 32 + i * 16 + 4 * r + l4, col = wc * 32 + j * 16 + l16;
-                if (row < rb && col < cb) C[(long long)row * n + col] = acc[i][j][r];
-            }
 }
 
 // x_p = U_pp^{-1} (y_p - sum_{q>p} U_pq x_q).  Workgroup b handles block row p = P-1-b, so a

@@ -155,8 +155,7 @@ struct LuArgs {
     int* status;      // [0] pivot check failed (needs interchanges), [1] spin timeout
     int n;
 };
-__global__ void lu_panel_kernel(LuArgs a, int k0);
-__global__ void lu_update_kernel(LuArgs a, int k0);
+__global__ void lu_step_kernel(LuArgs a, int k0);
 __global__ void lu_backsolve_kernel(LuArgs a);
 __global__ void gepp_fallback_kernel(const SurfDesc* sd, double* M, const double* b, double* x,
                                      int* piv, int* status, int* counters);

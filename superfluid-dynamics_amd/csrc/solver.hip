@@ -481,17 +481,15 @@ bool Solver::enqueue_solve(const double2* Z, int stage, bool g1) {
             const int kb = std::min(kNB, n - k0);
             const int rem = n - k0 - kb;
             const int nrb = (rem + kNB - 1) / kNB;
-            // panel: 2 kb^3 (two 64^3-class products per row/column block) + the diagonal block
-            const double pf = (double)count * (4.0 * kb * kb * rem + 2.0 * kb * kb * kb);
-            launch("lu_panel", pf, 8.0 * count * (2.0 * kb * rem + kb * kb), [&] {
-                go(lu_panel_kernel, dim3(1 + 2 * nrb, 1, count), dim3(256), 0, a, k0);
+            // panel p: L21 and U12 (2 kb^2 rem) and the diagonal block (2 kb^3 class); panel p-1's
+            // update of the whole trailing matrix from column k0 on: 2 (n - k0)^2 64
+            const double pf = 4.0 * kb * kb * rem + 2.0 * kb * kb * kb;
+            const double uf = k0 > 0 ? 2.0 * (n - k0) * (double)(n - k0) * kNB : 0.0;
+            const double ub = k0 > 0 ? 8.0 * (2.0 * (n - k0) * (double)(n - k0) + 2.0 * (n - k0) * kNB) : 0.0;
+            const int grid = 1 + 2 * nrb + (k0 > 0 ? nrb * nrb : 0);
+            launch("lu_step", (double)count * (pf + uf), 8.0 * count * (2.0 * kb * rem + kb * kb) + count * ub, [&] {
+                go(lu_step_kernel, dim3(grid, 1, count), dim3(256), 0, a, k0);
             });
-            if (rem > 0)
-                launch("lu_update", (double)count * 2.0 * rem * rem * kb,
-                       8.0 * count * (2.0 * rem * rem + 2.0 * rem * kb), [&] {
-                           go(lu_update_kernel, dim3(nrb, nrb, count), dim3(256), 0, a,
-                                              k0);
-                       });
         }
         const int P = (n + kNB - 1) / kNB;
         launch("lu_backsolve", (double)count * n * n, 8.0 * count * 0.5 * n * n, [&] {
@@ -1222,8 +1220,8 @@ void debug_lu_solve(int n, const double* A, const double* b, double* x, int* int
         const int kb = std::min(kNB, n - k0);
         const int rem = n - k0 - kb;
         const int nrb = (rem + kNB - 1) / kNB;
-        hipLaunchKernelGGL(lu_panel_kernel, dim3(1 + 2 * nrb, 1, 1), dim3(256), 0, st, a, k0);
-        if (rem > 0) hipLaunchKernelGGL(lu_update_kernel, dim3(nrb, nrb, 1), dim3(256), 0, st, a, k0);
+        hipLaunchKernelGGL(lu_step_kernel, dim3(1 + 2 * nrb + (k0 > 0 ? nrb * nrb : 0), 1, 1), dim3(256), 0, st,
+                           a, k0);
     }
     hipLaunchKernelGGL(lu_backsolve_kernel, dim3(P, 1, 1), dim3(256), 0, st, a);
     SFH_HIP(hipGetLastError());
