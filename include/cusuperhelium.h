@@ -369,7 +369,7 @@ int sfh_rhs_device(sfh_handle h, const void* state, void* rhs);
  * stepping continues; the call returns when every frame is in the caller's arrays. */
 int sfh_step_record(sfh_handle h, double dt, int nsteps, int every, double* states, double* times);
 
-/* Reverse mode of the right-hand side of water with rho = 0 and of the helium film (image terms, van
+/* Reverse mode of the right-hand side of water (any rho, the V1[1] quirk included) and of the helium film (image terms, van
  * der Waals term or its expansion, surface tension), without the drive (-3 otherwise); handles of
  * SFH_SOLVE_BATCHED take every N <= 1024 (one workgroup per surface), LU and GMRES handles any N (many
  * workgroups per surface).  No counterpart in the reference, which has no adjoint.  sfh_rhs_vjp: cotangents (gvx, gvy, gdphi) of the right-hand

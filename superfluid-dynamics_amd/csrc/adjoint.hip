@@ -1,19 +1,23 @@
 // adjoint.hip -- hand-written reverse mode (vector-Jacobian product) of the Roberts right-hand
-// side for water (rho = 0), one workgroup per surface (every N <= 1024).
+// side for water and the helium film, one workgroup per surface (every N <= 1024).
 //
 // Forward map (BaseBoundaryIntegralCalculator::runTimeStep, BaseBoundaryIntegrator.cuh:137-314):
 //   Z' = s D1 (Z - L) + s,  Z'' = s^2 D2 (Z - L),  b = s Re D1 Phi      (s = 2 pi / N)
-//   M_kj = c Im(Z'_k c_kj) (j != k),  M_kk = 1/2 + c Im(Z''_k / Z'_k),  c_kj = cot((z_k - z_j)/2)
+//   M_kj = c Im(Z'_k c_kj) (j != k),  M_kk = (1 + rho)/2 + c Im(Z''_k / Z'_k),  c_kj = cot((z_k - z_j)/2),
+//   c = (1 - rho)/4pi
 //   mu = M^{-1} b,  mu' = s D1 mu
 //   w_k = V2_k mu'_k + V1_k mu_k - (i/4pi) S_k,  S_k = sum_{j!=k} mu_j c_kj,  q = conj(w)
-//   V2 = i / (2 pi Z'),  V1 = -(i/4pi) Z'' / Z'^2 + 1 / (2 Z')
-//   dPhi/dt = -g y + |q|^2 / 2
+//   V2 = i / (2 pi Z'),  V1 = -(i/4pi) Z'' / Z'^2 + 1 / (2 Z')  (q2, the upper velocity: - 1 / (2 Z'))
+//   dPhi/dt = -(1 + rho) g y + |q|^2 / 2 + rho |q2|^2 / 2 - rho (q1q.x q2.x + q.y q2.y), q1q = q, or
+//   node 1's q under the V1[1] quirk
 // (createM.cuh:43-63,96-107; WaterVelocities.cuh:38-70,205-242).  D1, D2 are the spectral
 // derivative operators of the reference's multiplier table; every D1 multiplier is imaginary and
 // every D2 multiplier real, so D1^H = -D1 and D2^H = D2.
 //
 // Cotangents are complex numbers g with dL = Re sum conj(g) dz.  Given (gq, gp) for (q, dPhi/dt):
-//   gw = conj(gq + gp q);  the node-local parts of w give gZ', gZ'', gmu, and gz = -i g gp;
+//   the Bernoulli term gives the cotangents gq1, gq2 of q and q2 (bernoulli_vjp_q; under the quirk
+//   node 1's x part collects -rho gp q2.x of every node), gw = conj(gq1 + gq2);  the node-local
+//   parts of w and of the jump +-mu/(2Z') give gZ', gZ'', gmu (velocity_vjp), and gz = i (d dPhi/dt/dy) gp;
 //   gmu += s Re D1^H (conj V2 gw)  + Re((i/4pi) P3),  P3_j = sum_k conj(gw_k) c_jk;
 //   M^T gb = gmu (GMRES, matrix-free transposed product);
 //   gZ' += -i c gb conj(S) (off-diagonal M), the diagonal of M adds to gZ', gZ'';
@@ -130,8 +134,9 @@ __global__ __launch_bounds__(1024) void rhs_vjp_kernel(VjpArgs a, Phys ph) {
     }
 
     // ------------------------------------------------ node-local adjoint of velocity + Bernoulli
-    double2 gw = make_double2(0.0, 0.0);
-    double gmu = 0.0;
+    const bool quirk = ph.kind == 0 && ph.rho != 0.0 && ph.rho_quirk && n > 1;
+    double2 q1 = make_double2(0.0, 0.0), q2 = q1, kqz = q1;
+    double kp = 0.0;
     if (own) {
         const double2 zp = nv.zp, zpp = nv.zpp;
         const double2 izp = crecip(zp), izp2 = cmul(izp, izp);
@@ -139,27 +144,36 @@ __global__ __launch_bounds__(1024) void rhs_vjp_kernel(VjpArgs a, Phys ph) {
         const double2 V1 = cadd(ctimes_i(cscale(cmul(zpp, izp2), -q4)), cscale(izp, 0.5));
         double2 w = cadd(cadd(cmul(V2, mup), cscale(V1, mu)), ctimes_i(make_double2(-q4 * sx, -q4 * sy)));
         if (pc.image) w = cadd(w, ctimes_i(make_double2(q4 * six, q4 * siy)));
-        const double2 q = cconj(w);
-        const double2 kq = a.gA_Z[off + t];
-        double kp = a.gA_P[off + t];
-        double2 kqz = cscale(kq, a.ca);
-        kp *= a.ca;
+        q1 = cconj(w);                         // lower velocity
+        q2 = cconj(csub(w, cscale(izp, mu)));  // upper velocity (the jump mu/(2Z') subtracted)
+        kqz = cscale(a.gA_Z[off + t], a.ca);
+        kp = a.ca * a.gA_P[off + t];
         if (a.gB_Z) {
             kqz = cadd(kqz, cscale(a.gB_Z[off + t], a.cb));
             kp += a.cb * a.gB_P[off + t];
         }
-        const double2 gq = cadd(kqz, cscale(q, kp));
-        gw = cconj(gq);
-        const double2 gmup = cmul(cconj(V2), gw);                       // cotangent of mu'
-        const double2 dV2 = ctimes_i(cscale(izp2, -1.0 / (2.0 * kPi))); // dV2/dZ'
-        const double2 dV1p = csub(ctimes_i(cscale(cmul(zpp, cmul(izp2, izp)), 1.0 / (2.0 * kPi))),
-                                  cscale(izp2, 0.5));                   // dV1/dZ'
-        const double2 dV1pp = ctimes_i(cscale(izp2, -q4));             // dV1/dZ''
-        const double2 gVw = cscale(gw, mu);
-        double2 gzp = cadd(cmul(cconj(dV2), cmul(cconj(mup), gw)), cmul(cconj(dV1p), gVw));
-        double2 gzpp = cmul(cconj(dV1pp), gVw);
-        curvature_vjp(ph, zp, zpp, kp, gzp, gzpp);
-        gmu = cmul(cconj(V1), gw).x;
+    }
+    double q1qx = 0.0;   // node 1's lower x velocity (the V1[1] quirk)
+    if (quirk) {
+        __syncthreads();
+        if (t == 1) L.scal[2] = q1.x;
+        __syncthreads();
+        q1qx = L.scal[2];
+    }
+    double2 gq1, gq2;
+    bernoulli_vjp_q(ph, q1, q2, q1qx, quirk, kqz, kp, gq1, gq2);
+    if (quirk) {   // node 1's x velocity enters every node's Bernoulli term
+        const double d = bsum1(own ? -ph.rho * kp * q2.x : 0.0, L);
+        if (t == 1) gq1.x += d;
+    }
+    double2 gw = make_double2(0.0, 0.0);
+    double gmu = 0.0;
+    if (own) {
+        const VelVjp vj = velocity_vjp(nv.zp, nv.zpp, mup, mu, gq1, gq2);
+        gw = vj.gw;
+        double2 gzp = vj.gzp, gzpp = vj.gzpp;
+        curvature_vjp(ph, nv.zp, nv.zpp, kp, gzp, gzpp);
+        gmu = vj.gmu;
         AdjNode& A = an[t];
         A.gzp = gzp;
         A.gzpp = gzpp;
@@ -169,7 +183,7 @@ __global__ __launch_bounds__(1024) void rhs_vjp_kernel(VjpArgs a, Phys ph) {
         A.p1 = make_double2(p1x, p1y);
         A.mu = mu;
         A.S = make_double2(sx, sy);
-        L.fa[lds_index(d, t)] = gmup;
+        L.fa[lds_index(d, t)] = vj.gmup;
         L.fb[t] = cconj(gw);
     }
     // gmu += s Re(D1^H gmu')

@@ -89,33 +89,71 @@ __global__ __launch_bounds__(256) void adj_pass_a_kernel(AdjMArgs a, Phys ph) {
     const double2 V1 = cadd(ctimes_i(cscale(cmul(zpp, izp2), -q4)), cscale(izp, 0.5));
     double2 w = cadd(cadd(cmul(V2, mup), cscale(V1, mu)), ctimes_i(make_double2(-q4 * sv[0], -q4 * sv[1])));
     if (pk.image) w = cadd(w, ctimes_i(make_double2(q4 * sv[4], q4 * sv[5])));
-    const double2 q = cconj(w);
+    const double2 q1 = cconj(w), q2 = cconj(csub(w, cscale(izp, mu)));   // lower and upper velocities
     double2 kqz = cscale(a.gA_Z[g], a.ca);
     double kp = a.ca * a.gA_P[g];
     if (a.gB_Z) {
         kqz = cadd(kqz, cscale(a.gB_Z[g], a.cb));
         kp += a.cb * a.gB_P[g];
     }
-    const double2 gw = cconj(cadd(kqz, cscale(q, kp)));
-    const double2 dV2 = ctimes_i(cscale(izp2, -1.0 / (2.0 * kPi)));
-    const double2 dV1p = csub(ctimes_i(cscale(cmul(zpp, cmul(izp2, izp)), 1.0 / (2.0 * kPi))), cscale(izp2, 0.5));
-    const double2 dV1pp = ctimes_i(cscale(izp2, -q4));
-    const double2 gVw = cscale(gw, mu);
+    // under the V1[1] quirk node 1's x velocity is not known here: its terms are added by
+    // adj_rho_quirk_kernel from the values kept in a.quirk
+    const bool quirk = ph.kind == 0 && ph.rho != 0.0 && ph.rho_quirk && n > 1;
+    double2 gq1, gq2;
+    bernoulli_vjp_q(ph, q1, q2, 0.0, quirk, kqz, kp, gq1, gq2);
+    if (quirk) {
+        a.quirk[g] = ph.rho * kp;
+        a.quirk[a.total + g] = q2.x;
+        a.quirk[2 * a.total + g] = q1.x;
+    }
+    const VelVjp vj = velocity_vjp(zp, zpp, mup, mu, gq1, gq2);
     AdjNode A;
-    A.gzp = cadd(cmul(cconj(dV2), cmul(cconj(mup), gw)), cmul(cconj(dV1p), gVw));
-    A.gzpp = cmul(cconj(dV1pp), gVw);
+    A.gzp = vj.gzp;
+    A.gzpp = vj.gzpp;
     curvature_vjp(ph, zp, zpp, kp, A.gzp, A.gzpp);
     A.gz = make_double2(0.0, bernoulli_dy(ph, a.Z[g].y) * kp);
-    A.gw = gw;
+    A.gw = vj.gw;
     A.p1 = make_double2(sv[2], sv[3]);
     A.S = make_double2(sv[0], sv[1]);
     A.pi1 = make_double2(sv[6], sv[7]);
     A.mu = mu;
     A.pad = 0.0;
     a.node[g] = A;
-    a.gwc[g] = cconj(gw);
-    a.gmup[g] = cmul(cconj(V2), gw);
-    a.gmu[g] = cmul(cconj(V1), gw).x;
+    a.gwc[g] = cconj(vj.gw);
+    a.gmup[g] = vj.gmup;
+    a.gmu[g] = vj.gmu;
+}
+
+// The V1[1] quirk's terms after pass A (water, rho != 0): every node's Bernoulli term reads node 1's
+// lower x velocity q1x_1, so node k's gq2.x gains -rho gp_k q1x_1 and node 1's gq1.x gains
+// -rho sum_k gp_k q2.x_k.  Both enter the node-local cotangents linearly (velocity_vjp) and are added
+// to them before any later pass reads them.  Grid (ceil(N / 256), surfaces); workgroup 0 of a surface
+// also forms the sum and applies node 1's term after its own.
+__global__ __launch_bounds__(256) void adj_rho_quirk_kernel(AdjMArgs a) {
+    __shared__ double red[4];
+    const int s = blockIdx.y;
+    const SurfDesc d = a.sd[s];
+    const int n = d.n, t = threadIdx.x, k = blockIdx.x * 256 + t;
+    if (n <= 1 || blockIdx.x * 256 >= n) return;
+    const double q1x1 = a.quirk[2 * a.total + d.voff + 1];
+    auto add = [&](int g, double2 gq1, double2 gq2) {
+        const VelVjp vj = velocity_vjp(a.Zp[g], a.Zpp[g], a.mup[g], a.mu[g], gq1, gq2);
+        AdjNode& A = a.node[g];
+        A.gw = cadd(A.gw, vj.gw);
+        A.gzp = cadd(A.gzp, vj.gzp);
+        A.gzpp = cadd(A.gzpp, vj.gzpp);
+        a.gwc[g] = cadd(a.gwc[g], cconj(vj.gw));
+        a.gmup[g] = cadd(a.gmup[g], vj.gmup);
+        a.gmu[g] += vj.gmu;
+    };
+    if (k < n) add(d.voff + k, make_double2(0.0, 0.0), make_double2(-a.quirk[d.voff + k] * q1x1, 0.0));
+    if (blockIdx.x != 0) return;
+    double v = 0.0;
+    for (int j = t; j < n; j += 256) v -= a.quirk[d.voff + j] * a.quirk[a.total + d.voff + j];
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if ((t & 63) == 0) red[t >> 6] = v;
+    __syncthreads();
+    if (t == 1) add(d.voff + 1, make_double2((red[0] + red[1]) + (red[2] + red[3]), 0.0), make_double2(0.0, 0.0));
 }
 
 // out_k = sum_j c1[(k - j) mod n] v_j (the first-derivative circulant, unscaled), one row per wave.

@@ -266,8 +266,11 @@ struct AdjMArgs {
     double* out_P;
     double2* acc_Z;            // optional accumulator += out
     double* acc_P;
+    double* quirk;             // water rho != 0 with the V1[1] quirk: rho gp, q2.x, q1.x per node (3 x total)
+    int total;                 // nodes of all surfaces
 };
 constexpr int kAdjRows = 2;   // rows per workgroup of the reverse pair kernels
+__global__ void adj_rho_quirk_kernel(AdjMArgs a);
 template <int R> __global__ void adj_pass_a_kernel(AdjMArgs a, Phys ph);
 template <int R> __global__ void adj_pass_b_kernel(AdjMArgs a, Phys ph);
 template <int R> __global__ void adj_pass_c_kernel(AdjMArgs a, Phys ph);

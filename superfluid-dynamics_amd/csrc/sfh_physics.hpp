@@ -91,6 +91,55 @@ __device__ inline void velocity_from_sums(const Phys& ph, double2 zp, double2 zp
     q2 = cconj(csub(base, cscale(half_inv, mk)));
 }
 
+// Reverse mode of the node-local part of velocity_from_sums (the cot sums' terms go to the pair
+// passes).  With w = V2 mu' + V1 mu + (sums), V1 = -(i/4pi) Z''/Z'^2, and the jump mu/(2 Z') added for
+// the lower velocity q1 = conj(w + mu/(2 Z')) and subtracted for the upper one q2 = conj(w - mu/(2 Z')),
+// the cotangents (gq1, gq2) of (q1, q2) give
+//   gw = conj(gq1 + gq2) (the cotangent of w and of every sum), gh = conj(gq1 - gq2) (of the jump),
+//   gmu' = conj(V2) gw,  gmu = Re(conj(V1) gw) + Re(conj(1/(2Z')) gh),
+//   gZ' = conj(dV2/dZ') conj(mu') gw + conj(dV1/dZ') mu gw + conj(-1/(2Z'^2)) mu gh,
+//   gZ'' = conj(dV1/dZ'') mu gw.
+// Linear in (gq1, gq2), with no constant term.
+struct VelVjp {
+    double2 gw, gmup, gzp, gzpp;
+    double gmu;
+};
+__device__ inline VelVjp velocity_vjp(double2 zp, double2 zpp, double2 mup, double mu, double2 gq1, double2 gq2) {
+    const double q4 = 0.25 / kPi;
+    const double2 izp = crecip(zp), izp2 = cmul(izp, izp);
+    const double2 V2 = ctimes_i(cscale(izp, 1.0 / (2.0 * kPi)));
+    const double2 V1 = ctimes_i(cscale(cmul(zpp, izp2), -q4));
+    const double2 dV2 = ctimes_i(cscale(izp2, -1.0 / (2.0 * kPi)));
+    const double2 dV1p = ctimes_i(cscale(cmul(zpp, cmul(izp2, izp)), 1.0 / (2.0 * kPi)));
+    const double2 dV1pp = ctimes_i(cscale(izp2, -q4));
+    VelVjp r;
+    r.gw = cconj(cadd(gq1, gq2));
+    const double2 gh = cconj(csub(gq1, gq2));
+    r.gmup = cmul(cconj(V2), r.gw);
+    r.gmu = cmul(cconj(V1), r.gw).x + cmul(cconj(cscale(izp, 0.5)), gh).x;
+    r.gzp = cadd(cadd(cmul(cconj(dV2), cmul(cconj(mup), r.gw)), cmul(cconj(dV1p), cscale(r.gw, mu))),
+                 cmul(cconj(cscale(izp2, -0.5)), cscale(gh, mu)));
+    r.gzpp = cmul(cconj(dV1pp), cscale(r.gw, mu));
+    return r;
+}
+
+// Cotangents (gq1, gq2) of the lower and upper velocities from those of the right-hand side
+// (kqz for dz/dt = q1, kp for dPhi/dt).  Water (createM.cuh:96-107):
+//   dPhi/dt = -(1 + rho) g y + |q1|^2/2 + rho |q2|^2/2 - rho (q1q.x q2.x + q1.y q2.y),
+// q1q = q1 but, under the V1[1] quirk, node 1's lower velocity (createM.cuh:104).  The quirk's
+// q1q.x term belongs to node 1: the caller passes q1q.x of node 1 (or 0 and adds -rho kp q1q.x to
+// gq2.x later) and adds -rho kp q2.x of every node to node 1's gq1.x.  The film has no upper term.
+__device__ inline void bernoulli_vjp_q(const Phys& ph, double2 q1, double2 q2, double q1qx, bool quirk,
+                                       double2 kqz, double kp, double2& gq1, double2& gq2) {
+    gq1 = cadd(kqz, cscale(q1, kp));
+    gq2 = make_double2(0.0, 0.0);
+    if (ph.kind == 0 && ph.rho != 0.0) {
+        const double r = ph.rho * kp;
+        gq1 = csub(gq1, make_double2(quirk ? 0.0 : r * q2.x, r * q2.y));
+        gq2 = make_double2(r * (q2.x - (quirk ? q1qx : q1.x)), r * (q2.y - q1.y));
+    }
+}
+
 // LightIntensity::compute_intensity (LightIntensity.cuh:16-29)
 __device__ inline double intensity(const Phys& ph, double y, double x) {
     const double df = ph.detuning - ph.G * y;

@@ -290,7 +290,7 @@ Solver::~Solver() {
     void* vbufs[] = {vY_[0], vY_[1], vY_[2], vK_[0], vK_[1], vK_[2], vYP_[0], vYP_[1], vYP_[2], vKP_[0],
                      vKP_[1], vKP_[2], vMu_[0], vMu_[1], vMu_[2], vMu_[3], gA_, gR_[0], gR_[1], gAcc_,
                      gAP_, gRP_[0], gRP_[1], gAccP_, adj_, vstats_, Mt_, agwc_, agmup_, ad1_, agam_, agamI_,
-                     agzp_, agzpp_, agz_, agmu_, agb_, adjRing_, adjSave_, adjX_, adjSaveZ_, adjSaveP_};
+                     agzp_, agzpp_, agz_, agmu_, agb_, aquirk_, adjRing_, adjSave_, adjX_, adjSaveZ_, adjSaveP_};
     for (void* p : vbufs)
         if (p) (void)hipFree(p);
     for (auto& kv : traj_) free_record(kv.second);
@@ -1245,10 +1245,7 @@ void debug_lu_solve(int n, const double* A, const double* b, double* x, int* int
 // ------------------------------------------------------------------ reverse mode
 
 void Solver::ensure_vjp() {
-    const bool water_ok = ph_.kind == 0 && ph_.rho == 0.0;
-    const bool film_ok = ph_.kind == 1;
-    if (!(water_ok || film_ok) || ph_.drive)
-        throw ArgError("the adjoint is implemented for water with rho = 0 and the helium film (no drive)");
+    if (ph_.drive) throw ArgError("the adjoint is implemented for water and the helium film without a drive");
     if (opt_.solver == kSolveBatched && maxN_ > kBatchMaxN)
         throw ArgError("the batched adjoint takes surfaces of at most 1024 nodes");
     if (vjp_ready_) return;
@@ -1258,6 +1255,7 @@ void Solver::ensure_vjp() {
         for (double2** b : {&agwc_, &agmup_, &ad1_, &agam_, &agamI_, &agzp_, &agzpp_, &agz_}) *b = dalloc<double2>(T_);
         agmu_ = dalloc<double>(T_);
         agb_ = dalloc<double>(T_);
+        aquirk_ = dalloc<double>((size_t)3 * T_);
         SFH_HIP(hipMemset(agamI_, 0, T_ * sizeof(double2)));
         adjRing_ = dalloc<double>((size_t)4 * kHistDepth * T_);
         adjSave_ = dalloc<double>((size_t)4 * kHistDepth * T_);
@@ -1369,8 +1367,12 @@ void Solver::enqueue_vjp_mwg(const double2* Z, const double* P, const double* mu
     a.out_P = outP;
     a.acc_Z = accumulate ? gAcc_ : nullptr;
     a.acc_P = accumulate ? gAccP_ : nullptr;
+    a.quirk = aquirk_;
+    a.total = T_;
+    const bool quirk = ph_.kind == 0 && ph_.rho != 0.0 && ph_.rho_quirk;
     launch("vjp_pairs", img * 3.0 * nsq, 0.0, [&] {
         go(adj_pass_a_kernel<kAdjRows>, pgrid, dim3(256), 0, a, ph_);
+        if (quirk) go(adj_rho_quirk_kernel, dim3((maxN_ + 255) / 256, S_), dim3(256), 0, a);
         go(adj_circ_d1_kernel, wgrid, dim3(256), 0, d_desc_, (const double*)d_circ_, (const double2*)agmup_, ad1_);
         go(adj_pass_b_kernel<kAdjRows>, pgrid, dim3(256), 0, a, ph_);
     });

@@ -158,7 +158,7 @@ class Solver {
     double2* device_Z() { return Z0_; }
     double* device_phi() { return P0_; }
 
-    // Reverse mode (water with rho = 0, the helium film; no drive).  rhs_vjp: cotangents of (vx, vy, dphi) at the
+    // Reverse mode (water, the helium film; no drive).  rhs_vjp: cotangents of (vx, vy, dphi) at the
     // current state -> cotangents of (x, y, phi).  step_vjp: cotangent of the state after one RK4
     // step of size dt taken from the current state -> cotangent of the current state (the state is
     // left unchanged).  Host arrays of total_nodes() entries.
@@ -302,6 +302,7 @@ class Solver {
     double2 *agwc_ = nullptr, *agmup_ = nullptr, *ad1_ = nullptr, *agam_ = nullptr, *agamI_ = nullptr;
     double2 *agzp_ = nullptr, *agzpp_ = nullptr, *agz_ = nullptr;
     double *agmu_ = nullptr, *agb_ = nullptr;
+    double* aquirk_ = nullptr;   // the V1[1] quirk terms of the reverse pass (3 x T_)
     void enqueue_vjp_mwg(const double2* Z, const double* P, const double* mu, const double2* gAZ, const double* gAP,
                          double ca, const double2* gBZ, const double* gBP, double cb, double2* outZ, double* outP,
                          bool accumulate, int stage);

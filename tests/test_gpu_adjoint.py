@@ -1,4 +1,5 @@
-"""GPU checks of the hand-written adjoint (water with rho = 0, the helium film) through the C ABI.
+"""GPU checks of the hand-written adjoint (water, with and without an upper fluid, and the helium film)
+through the C ABI.
 
 - right-hand side VJP: dot-product test <J^T l, d> = d/dh <l, f(y + h d)> against central
   differences of the CPU FP64 oracle (h = 1e-6, relative agreement 1e-7);
@@ -60,7 +61,9 @@ def _richardson(f, h=1e-5):
 FILMS = [dict(kind=1, depth=1.0), dict(kind=1, depth=0.4), dict(kind=1, depth=0.7, infinite_depth=True),
          dict(kind=1, depth=1.0, use_expansions=True, expansion_order=3), dict(kind=1, depth=1.0, g=0.15),
          dict(kind=1, depth=1.0, kappa=0.01)]
-CASES = [([64], {}), ([256, 96, 1024], {})] + [([128], f) for f in FILMS] + [([256, 64, 512], FILMS[0])]
+RHO = [dict(rho=0.3), dict(rho=0.3, rho_quirk=False)]   # water under a lighter fluid; the V1[1] quirk on, off
+CASES = [([64], {}), ([256, 96, 1024], {})] + [([128], f) for f in FILMS] + [([256, 64, 512], FILMS[0])] + \
+    [([128, 64], RHO[0]), ([96], RHO[1])]
 
 
 @pytest.mark.parametrize("sizes,kw", CASES, ids=lambda v: "-".join(f"{k}{x}" for k, x in v.items()) if isinstance(v, dict) else str(v))
@@ -85,7 +88,7 @@ def test_rhs_vjp_dot_product(oracle, csh, sizes, kw):
         o += n
 
 
-@pytest.mark.parametrize("kw", [{}, FILMS[0]], ids=["water", "film"])
+@pytest.mark.parametrize("kw", [{}, FILMS[0], RHO[0]], ids=["water", "film", "rho"])
 def test_step_vjp_dot_product(oracle, csh, kw):
     n, dt = 128, 0.005
     rng = np.random.default_rng(5)
@@ -105,7 +108,8 @@ def test_step_vjp_dot_product(oracle, csh, kw):
 
 
 MWG = [([1024], {}, "gmres"), ([768], {}, "lu"), ([1024], FILMS[0], "gmres"), ([512], FILMS[1], "lu"),
-       ([256, 256], FILMS[0], "gmres"), ([2048], {}, "gmres"), ([512], FILMS[5], "gmres")]
+       ([256, 256], FILMS[0], "gmres"), ([2048], {}, "gmres"), ([512], FILMS[5], "gmres"),
+       ([512], RHO[0], "gmres"), ([384, 192], RHO[0], "lu"), ([384], RHO[1], "gmres")]
 
 
 @pytest.mark.parametrize("sizes,kw,solver", MWG, ids=lambda v: str(v))
@@ -241,7 +245,7 @@ def test_differentiable_film_small_matches_oracle_fd(oracle, csh):
     assert rel <= 1e-6
 
 
-@pytest.mark.parametrize("kw,drive", [(dict(kind=1, depth=1.0), True), (dict(rho=0.3), False)])
+@pytest.mark.parametrize("kw,drive", [(dict(kind=1, depth=1.0), True)])
 def test_adjoint_rejects_unsupported_physics(csh, kw, drive):
     with csh.Ensemble([64], csh.Props(**kw), solver=csh.BATCHED) as e:
         if drive:
@@ -285,10 +289,11 @@ def _host_sweep(e, Z0, P0, dt, steps, every, gZ, gP):
     # two surfaces of one GMRES handle: per-surface Krylov blocks of the trimmed, warm-started transposed
     # solves (the calibration falls in the first swept segment, the second is trimmed)
     ("GMRES", [512, 256], 8, 16, None),
+    ("GMRES", [384], 4, 9, "rho"),
 ])
 def test_device_trajectory_sweep_matches_host_sweep(csh, solver, sizes, every, steps, kw):
     dt = 0.004
-    props = csh.Props(**(FILMS[0] if kw == "film" else {}))
+    props = csh.Props(**({"film": FILMS[0], "rho": RHO[0]}.get(kw, {})))
     rng = np.random.default_rng(31)
     Z0 = np.concatenate([_surface(n, 40 + i) for i, n in enumerate(sizes)])
     P0 = np.concatenate([0.01 * np.cos(2 * np.pi * np.arange(n) / n) for n in sizes])
@@ -319,7 +324,8 @@ def test_device_trajectory_sweep_matches_host_sweep(csh, solver, sizes, every, s
 
 def test_device_trajectory_rejects_unsupported_problem_before_stepping(csh):
     Z0 = _surface(128, 5)
-    with csh.Ensemble([128], csh.Props(rho=0.3), solver=csh.GMRES) as e:
+    with csh.Ensemble([128], csh.Props(**FILMS[0]), solver=csh.GMRES) as e:
+        e.set_drive(damping=0.1)   # the drive has no adjoint
         e.set_state(Z0, np.zeros(128))
         with pytest.raises(RuntimeError):
             e.traj_forward(0.004, 5, 2)
