@@ -369,11 +369,13 @@ int sfh_rhs_device(sfh_handle h, const void* state, void* rhs);
  * stepping continues; the call returns when every frame is in the caller's arrays. */
 int sfh_step_record(sfh_handle h, double dt, int nsteps, int every, double* states, double* times);
 
-/* Reverse mode of the right-hand side of water (any rho, the V1[1] quirk included) and of the helium film (image terms, van
- * der Waals term or its expansion, surface tension), without the drive (-3 otherwise); handles of
- * SFH_SOLVE_BATCHED take every N <= 1024 (one workgroup per surface), LU and GMRES handles any N (many
- * workgroups per surface).  No counterpart in the reference, which has no adjoint.  sfh_rhs_vjp: cotangents (gvx, gvy, gdphi) of the right-hand
- * side (vx, vy, dphi) at the current state -> cotangents (gx, gy, gphi) of the state (x, y, phi).
+/* Reverse mode of the right-hand side of water (any rho, the V1[1] quirk included) and of the
+ * helium film (image terms, van der Waals term or its expansion, surface tension), with or without
+ * the autonomous optical drive (the augmented and the time-dependent drive return -3); handles of
+ * SFH_SOLVE_BATCHED take every N <= 1024 (one workgroup per surface), LU and GMRES handles any N
+ * (many workgroups per surface).  No counterpart in the reference, which has no adjoint.
+ * sfh_rhs_vjp: cotangents (gvx, gvy, gdphi) of the right-hand side (vx, vy, dphi) at the current
+ * state -> cotangents (gx, gy, gphi) of the state (x, y, phi).
  * sfh_step_vjp: cotangent of the state one RK4 step of size dt after the current state ->
  * cotangent of the current state; the handle's state is not changed.  Together with saved
  * checkpoints this gives the gradient of any function of a trajectory. */

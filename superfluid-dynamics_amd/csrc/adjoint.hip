@@ -177,7 +177,7 @@ __global__ __launch_bounds__(1024) void rhs_vjp_kernel(VjpArgs a, Phys ph) {
         AdjNode& A = an[t];
         A.gzp = gzp;
         A.gzpp = gzpp;
-        A.gz = make_double2(0.0, bernoulli_dy(ph, nv.z.y) * kp);   // i (d dPhi/dt / dy) gp
+        A.gz = bernoulli_gz(ph, nv.z.x, nv.z.y, kp);   // dPhi/dt's own dependence on z
         A.pi1 = make_double2(q1x, q1y);
         A.gw = gw;
         A.p1 = make_double2(p1x, p1y);

@@ -111,7 +111,7 @@ __global__ __launch_bounds__(256) void adj_pass_a_kernel(AdjMArgs a, Phys ph) {
     A.gzp = vj.gzp;
     A.gzpp = vj.gzpp;
     curvature_vjp(ph, zp, zpp, kp, A.gzp, A.gzpp);
-    A.gz = make_double2(0.0, bernoulli_dy(ph, a.Z[g].y) * kp);
+    A.gz = bernoulli_gz(ph, a.Z[g].x, a.Z[g].y, kp);
     A.gw = vj.gw;
     A.p1 = make_double2(sv[2], sv[3]);
     A.S = make_double2(sv[0], sv[1]);

@@ -124,7 +124,8 @@ __device__ inline VelVjp velocity_vjp(double2 zp, double2 zpp, double2 mup, doub
 }
 
 // Cotangents (gq1, gq2) of the lower and upper velocities from those of the right-hand side
-// (kqz for dz/dt = q1, kp for dPhi/dt).  Water (createM.cuh:96-107):
+// (kqz for dz/dt = q1, kp for dPhi/dt), with the autonomous drive's damping term damping q1.y
+// (createM.cuh:138-149).  Water (createM.cuh:96-107):
 //   dPhi/dt = -(1 + rho) g y + |q1|^2/2 + rho |q2|^2/2 - rho (q1q.x q2.x + q1.y q2.y),
 // q1q = q1 but, under the V1[1] quirk, node 1's lower velocity (createM.cuh:104).  The quirk's
 // q1q.x term belongs to node 1: the caller passes q1q.x of node 1 (or 0 and adds -rho kp q1q.x to
@@ -132,6 +133,7 @@ __device__ inline VelVjp velocity_vjp(double2 zp, double2 zpp, double2 mup, doub
 __device__ inline void bernoulli_vjp_q(const Phys& ph, double2 q1, double2 q2, double q1qx, bool quirk,
                                        double2 kqz, double kp, double2& gq1, double2& gq2) {
     gq1 = cadd(kqz, cscale(q1, kp));
+    if (ph.drive) gq1.y += kp * ph.damping;   // the drive's damping term, damping q1.y
     gq2 = make_double2(0.0, 0.0);
     if (ph.kind == 0 && ph.rho != 0.0) {
         const double r = ph.rho * kp;
@@ -176,6 +178,22 @@ __device__ inline double drive_terms(const Phys& ph, const StageCtl& ctl, int g,
         dD = ph.beta * inten - 1.0 / ph.tau * D;
     }
     return dphi;
+}
+
+// The cotangent of the node position z = x + i y from dPhi/dt's explicit dependence on it (kp its
+// cotangent): i (d dPhi/dt / dy) kp from the restoring term, plus for the autonomous drive
+// strength kp (dI/dx + i dI/dy), I = (gamma^2/4) I_max / ((delta - G y)^2 + gamma^2/4) exp(-(x - x0)^2 / (2 sigma^2)):
+// dI/dy = 2 G (delta - G y) I / ((delta - G y)^2 + gamma^2/4), dI/dx = -(x - x0) I / sigma^2.
+__device__ inline double2 bernoulli_gz(const Phys& ph, double x, double y, double kp) {
+    double2 g = make_double2(0.0, bernoulli_dy(ph, y) * kp);
+    if (ph.drive) {
+        const double inten = intensity(ph, y, x);
+        const double df = ph.detuning - ph.G * y, hg = ph.gamma / 2.0;
+        const double s = ph.strength * kp * inten;
+        g.x += s * (-(x - ph.x0) / (ph.sigma * ph.sigma));
+        g.y += s * (2.0 * ph.G * df / (df * df + hg * hg));
+    }
+    return g;
 }
 
 // RK4 stage bookkeeping of node g (AutonomousRungeKuttaStepper.cuh:338-375, utilities.cuh:78-83),

@@ -1245,7 +1245,9 @@ void debug_lu_solve(int n, const double* A, const double* b, double* x, int* int
 // ------------------------------------------------------------------ reverse mode
 
 void Solver::ensure_vjp() {
-    if (ph_.drive) throw ArgError("the adjoint is implemented for water and the helium film without a drive");
+    if (ph_.drive && (ph_.augmented || ph_.timed))
+        throw ArgError("the adjoint is implemented for water, the helium film and the autonomous drive (not the "
+                       "augmented or time-dependent delayed intensity)");
     if (opt_.solver == kSolveBatched && maxN_ > kBatchMaxN)
         throw ArgError("the batched adjoint takes surfaces of at most 1024 nodes");
     if (vjp_ready_) return;

@@ -158,7 +158,7 @@ class Solver {
     double2* device_Z() { return Z0_; }
     double* device_phi() { return P0_; }
 
-    // Reverse mode (water, the helium film; no drive).  rhs_vjp: cotangents of (vx, vy, dphi) at the
+    // Reverse mode (water, the helium film, the autonomous drive).  rhs_vjp: cotangents of (vx, vy, dphi) at the
     // current state -> cotangents of (x, y, phi).  step_vjp: cotangent of the state after one RK4
     // step of size dt taken from the current state -> cotangent of the current state (the state is
     // left unchanged).  Host arrays of total_nodes() entries.

@@ -1,4 +1,4 @@
-"""Differentiable RK4 trajectories of the gfx950 stepper (water, the helium film; no drive).
+"""Differentiable RK4 trajectories of the gfx950 stepper (water, the helium film, the autonomous drive).
 
 ``evolve(Z0, Phi0, dt, steps, ...)`` is a ``torch.autograd.Function``: the forward pass advances
 the state on the GPU (the same stepper as ``Ensemble.step``) and keeps a device-resident checkpoint

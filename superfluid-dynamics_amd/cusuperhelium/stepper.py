@@ -170,7 +170,7 @@ class Ensemble:
         out = (st[:, :T] + 1j * st[:, T:2 * T], st[:, 2 * T:3 * T], tm)
         return out + (st[:, 3 * T:],) if fields == 4 else out
 
-    # ---- reverse mode (water, the helium film; no drive)
+    # ---- reverse mode (water, the helium film, the autonomous drive)
     def rhs_vjp(self, gq, gdphi):
         """Cotangent of (Z, Phi) from cotangents of (q = dZ/dt, dPhi/dt) at the current state.
 

@@ -245,11 +245,45 @@ def test_differentiable_film_small_matches_oracle_fd(oracle, csh):
     assert rel <= 1e-6
 
 
-@pytest.mark.parametrize("kw,drive", [(dict(kind=1, depth=1.0), True)])
+DRIVE = dict(damping=0.05, strength=0.3, detuning=0.4, gamma=0.8, G=2.0, max_intensity=0.6, x0=np.pi,
+             sigma=0.7, beta=0.9, tau=1.5)
+
+
+@pytest.mark.parametrize("solver,sizes", [("BATCHED", [128, 64]), ("GMRES", [512]), ("LU", [256])])
+def test_rhs_vjp_autonomous_drive_dot_product(oracle, csh, solver, sizes):
+    # the optically driven film (damping q1.y and the intensity term in dPhi/dt) against central
+    # differences of the oracle's driven right-hand side
+    kw = FILMS[0]
+    print(f"driven rhs vjp {solver} {sizes}", flush=True)
+    rng = np.random.default_rng(17)
+    Zs = [_surface(n, 300 + n) for n in sizes]
+    Ps = [0.02 * np.sin(2 * np.pi * np.arange(n) / n + 0.5) for n in sizes]
+    lq = [rng.standard_normal(n) + 1j * rng.standard_normal(n) for n in sizes]
+    lp = [rng.standard_normal(n) for n in sizes]
+    with csh.Ensemble(sizes, csh.Props(**kw), solver=getattr(csh, solver), tol=1e-13) as e:
+        e.set_drive(**DRIVE)
+        e.set_state(np.concatenate(Zs), np.concatenate(Ps))
+        gZ, gP = e.rhs_vjp(np.concatenate(lq), np.concatenate(lp))
+    drive = oracle.Drive(augmented=False, **DRIVE)
+    o = 0
+    for i, n in enumerate(sizes):
+        dZ, dP = _smooth_direction(rng, n)
+
+        def f(h):
+            q, dp, _ = oracle.rhs_driven(oracle.Props(**kw), drive, Zs[i] + h * dZ, Ps[i] + h * dP, np.zeros(n))
+            return _inner(lq[i], lp[i], q, dp)
+        fd = _richardson(f)
+        ad = _inner(gZ[o:o + n], gP[o:o + n], dZ, dP)
+        print(f"  n={n}: adjoint {ad:.12e}  fd {fd:.12e}", flush=True)
+        assert abs(ad - fd) <= 1e-7 * max(abs(fd), 1e-3)
+        o += n
+
+
+@pytest.mark.parametrize("kw,drive", [(dict(kind=1, depth=1.0), "augmented"), (dict(kind=1, depth=1.0), "timed")])
 def test_adjoint_rejects_unsupported_physics(csh, kw, drive):
     with csh.Ensemble([64], csh.Props(**kw), solver=csh.BATCHED) as e:
         if drive:
-            e.set_drive(damping=0.1)
+            e.set_drive(**{drive: True}, damping=0.1)
         e.set_state(_surface(64, 1), np.zeros(64))
         with pytest.raises(Exception, match="adjoint is implemented"):
             e.rhs_vjp(np.ones(64, complex), np.ones(64))
@@ -325,7 +359,7 @@ def test_device_trajectory_sweep_matches_host_sweep(csh, solver, sizes, every, s
 def test_device_trajectory_rejects_unsupported_problem_before_stepping(csh):
     Z0 = _surface(128, 5)
     with csh.Ensemble([128], csh.Props(**FILMS[0]), solver=csh.GMRES) as e:
-        e.set_drive(damping=0.1)   # the drive has no adjoint
+        e.set_drive(augmented=True, damping=0.1)   # the augmented drive has no adjoint
         e.set_state(Z0, np.zeros(128))
         with pytest.raises(RuntimeError):
             e.traj_forward(0.004, 5, 2)
