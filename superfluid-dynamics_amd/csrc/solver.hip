@@ -308,13 +308,16 @@ Solver::~Solver() {
 
 template <typename K, typename... Args>
 void Solver::go(K kernel, dim3 grid, dim3 block, unsigned shm, Args... args) {
-    if (ev_a_)
+    if (ev_b_) {
         hipExtLaunchKernelGGL(kernel, grid, block, shm, stream_, ev_a_, ev_b_, 0, args...);
-    else
+        ev_a_ = nullptr;   // a later launch of the same timed group records only the stop event
+    } else {
         hipLaunchKernelGGL(kernel, grid, block, shm, stream_, args...);
+    }
 }
 
-// f makes exactly one kernel launch through go()
+// f makes one or more kernel launches through go(): the event pair brackets them all, from the first
+// kernel's start to the last one's end
 template <class F>
 void Solver::launch(const char* tag, double flops, double bytes, F&& f, int iter) {
     if (!ktime_) {

@@ -217,8 +217,9 @@ class Solver {
     void check_status();
     template <class F>
     void launch(const char* tag, double flops, double bytes, F&& f, int iter = -1);
-    // A kernel launch on stream_.  Inside a timed launch() it records the kernel's own start and
-    // end into that launch's event pair (hipExtLaunchKernelGGL), as a profiler times a kernel.
+    // A kernel launch on stream_.  Inside a timed launch() it records the kernel's own start (the
+    // group's first kernel) and end into that launch's event pair (hipExtLaunchKernelGGL), as a
+    // profiler times a kernel.
     template <typename K, typename... Args>
     void go(K kernel, dim3 grid, dim3 block, unsigned shm, Args... args);
     hipEvent_t ev_a_ = nullptr, ev_b_ = nullptr;
