@@ -315,6 +315,18 @@ def film_leg(c, steps, with_accuracy):
             _, _, d = e.rhs(diagnostics=True)
             out["accuracy_lu"] = dict(steps=int(g["steps"][0]),
                                       **errors(Z, P, d["mu"], g["Z"][0], g["phi"][0], g["mu"][0]))
+        # the config's 500 steps: the mixed form and FP64 GMRES against the plain FP64 stepper (the LU,
+        # pinned to the oracle's golden states above)
+        fin = {}
+        for solver, prec, name in ((c.LU, c.FP64, "lu"), (c.GMRES, c.FP64, "gmres"), (c.GMRES, c.MIXED, "gmres_mixed")):
+            with c.Ensemble([n], c.Props(**props), precision=prec, solver=solver, tol=1e-12, max_iter=24) as e:
+                e.set_state(Z0, P0)
+                e.step(DT, 500)
+                Z, P = e.get_state()
+                _, _, d = e.rhs(diagnostics=True)
+                fin[name] = (Z, P, d["mu"])
+        out["vs_plain_lu_500_steps"] = {k: errors(*fin[k], *fin["lu"]) for k in ("gmres", "gmres_mixed")}
+        out["vs_plain_lu_500_steps"]["max_abs_y_lu"] = float(np.abs(fin["lu"][0].imag).max())
     return out
 
 

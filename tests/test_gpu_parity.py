@@ -540,3 +540,26 @@ def test_mixed_gmres_film_matches_oracle(oracle, csh):
     assert st["unconverged"] == 0
     assert np.abs(Zg - Zo).max() < 1e-9
     assert np.abs(Pg - Po).max() < 1e-9
+
+
+def test_c5_film_500_steps_mixed_and_gmres_match_the_plain_lu(csh):
+    # the N = 4096 film over the config's 500 steps: mixed GMRES (FP32 matrix, FP64 refinement) and FP64
+    # GMRES against the plain FP64 stepper (blocked LU, itself checked against the oracle's golden states
+    # over its first 20 steps)
+    from cusuperhelium import workloads as W
+    Z0, P0 = W.c5()
+    n = len(Z0)
+    out = {}
+    for solver, prec in ((csh.LU, csh.FP64), (csh.GMRES, csh.FP64), (csh.GMRES, csh.MIXED)):
+        print(f"c5 film 500 steps solver {solver} precision {prec}", flush=True)
+        with csh.Ensemble([n], csh.Props(**W.C5_PROPS), precision=prec, solver=solver, tol=1e-12, max_iter=24) as e:
+            e.set_state(Z0, P0)
+            e.step(0.002, 500)
+            out[solver, prec] = e.get_state()
+    ZL, PL = out[csh.LU, csh.FP64]
+    assert np.all(np.isfinite(ZL))
+    for key in ((csh.GMRES, csh.FP64), (csh.GMRES, csh.MIXED)):
+        Z, P = out[key]
+        dz, dp = np.abs(Z - ZL).max(), np.abs(P - PL).max()
+        print(f"  {key}: max |dz| {dz:.2e} |dphi| {dp:.2e}", flush=True)
+        assert dz <= 1e-9 and dp <= 1e-9
