@@ -371,7 +371,8 @@ int sfh_step_record(sfh_handle h, double dt, int nsteps, int every, double* stat
 
 /* Reverse mode of the right-hand side of water (any rho, the V1[1] quirk included) and of the
  * helium film (image terms, van der Waals term or its expansion, surface tension), with or without
- * the autonomous optical drive (the augmented and the time-dependent drive return -3); handles of
+ * the autonomous or augmented optical drive (the augmented one through the _augmented entry points
+ * below; the time-dependent drive returns -3); handles of
  * SFH_SOLVE_BATCHED take every N <= 1024 (one workgroup per surface), LU and GMRES handles any N
  * (many workgroups per surface).  No counterpart in the reference, which has no adjoint.
  * sfh_rhs_vjp: cotangents (gvx, gvy, gdphi) of the right-hand side (vx, vy, dphi) at the current
@@ -383,6 +384,15 @@ int sfh_rhs_vjp(sfh_handle h, const double* gvx, const double* gvy, const double
                 double* gphi);
 int sfh_step_vjp(sfh_handle h, double dt, const double* gx_in, const double* gy_in, const double* gphi_in,
                  double* gx_out, double* gy_out, double* gphi_out);
+/* The same for a handle with the augmented drive, whose state has the delayed intensity D as a third
+ * field: gdD is the cotangent of dD/dt (sfh_rhs_vjp_augmented) or of D one step later
+ * (sfh_step_vjp_augmented) and gD receives the cotangent of the current D.  sfh_rhs_vjp and
+ * sfh_step_vjp return -3 on such a handle, these two on any other.  No counterpart in the
+ * reference (AugmentedBoundaryIntegrator.cuh:32-36 is the forward system). */
+int sfh_rhs_vjp_augmented(sfh_handle h, const double* gvx, const double* gvy, const double* gdphi,
+                          const double* gdD, double* gx, double* gy, double* gphi, double* gD);
+int sfh_step_vjp_augmented(sfh_handle h, double dt, const double* gx_in, const double* gy_in, const double* gphi_in,
+                           const double* gD_in, double* gx_out, double* gy_out, double* gphi_out, double* gD_out);
 /* Differentiable trajectories with device-resident checkpoints (no counterpart in the reference; the
  * forward map is sfh_step).  sfh_traj_forward advances `steps` RK4 steps of size dt from the current
  * state exactly as sfh_step(h, dt, ...) does, keeps in device memory the state before steps 0, every,

@@ -156,6 +156,29 @@ __global__ __launch_bounds__(256) void adj_rho_quirk_kernel(AdjMArgs a) {
     if (t == 1) add(d.voff + 1, make_double2((red[0] + red[1]) + (red[2] + red[3]), 0.0), make_double2(0.0, 0.0));
 }
 
+// The augmented drive's delayed intensity D (AugmentedBoundaryIntegrator.cuh:32-36, createM.cuh:151-169):
+// dPhi/dt gains + D and dD/dt = beta I(x, y) - D / tau, both linear in D and local to the node.  With
+// the stage cotangents kp (of dPhi/dt) and kD (of dD/dt), each ca * gA + cb * gB:
+//   gD = kp - kD / tau,  gz += kD beta (dI/dx + i dI/dy),
+// added to the reverse pass of the two-field right-hand side (outZ, and accZ when accumulating).
+__global__ __launch_bounds__(256) void aug_vjp_kernel(const double2* __restrict__ Z, const double* gA_P,
+                                                      const double* gB_P, const double* gA_D, const double* gB_D,
+                                                      double ca, double cb, double2* outZ, double2* accZ,
+                                                      double* outD, double* accD, Phys ph, int total) {
+    const int g = blockIdx.x * 256 + threadIdx.x;
+    if (g >= total) return;
+    double kp = ca * gA_P[g], kD = ca * gA_D[g];
+    if (gB_P) kp += cb * gB_P[g];
+    if (gB_D) kD += cb * gB_D[g];
+    const double oD = kp - kD / ph.tau;
+    outD[g] = oD;
+    if (accD) accD[g] += oD;
+    const double2 gi = intensity_grad(ph, Z[g].x, Z[g].y);
+    const double2 gz = make_double2(kD * ph.beta * gi.x, kD * ph.beta * gi.y);
+    outZ[g] = cadd(outZ[g], gz);
+    if (accZ) accZ[g] = cadd(accZ[g], gz);
+}
+
 // out_k = sum_j c1[(k - j) mod n] v_j (the first-derivative circulant, unscaled), one row per wave.
 __global__ __launch_bounds__(256) void adj_circ_d1_kernel(const SurfDesc* __restrict__ sd,
                                                           const double* __restrict__ circ,

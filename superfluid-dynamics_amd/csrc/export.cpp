@@ -719,6 +719,20 @@ int sfh_step_vjp(sfh_handle h, double dt, const double* gx_in, const double* gy_
     return guarded([&] { h->s->step_vjp(dt, gx_in, gy_in, gphi_in, gx_out, gy_out, gphi_out); });
 }
 
+int sfh_rhs_vjp_augmented(sfh_handle h, const double* gvx, const double* gvy, const double* gdphi,
+                          const double* gdD, double* gx, double* gy, double* gphi, double* gD) {
+    SFH_NEED(h);
+    if (!gvx || !gvy || !gdphi || !gdD || !gx || !gy || !gphi || !gD) return -3;
+    return guarded([&] { h->s->rhs_vjp(gvx, gvy, gdphi, gx, gy, gphi, gdD, gD); });
+}
+
+int sfh_step_vjp_augmented(sfh_handle h, double dt, const double* gx_in, const double* gy_in, const double* gphi_in,
+                           const double* gD_in, double* gx_out, double* gy_out, double* gphi_out, double* gD_out) {
+    SFH_NEED(h);
+    if (!gx_in || !gy_in || !gphi_in || !gD_in || !gx_out || !gy_out || !gphi_out || !gD_out) return -3;
+    return guarded([&] { h->s->step_vjp(dt, gx_in, gy_in, gphi_in, gx_out, gy_out, gphi_out, gD_in, gD_out); });
+}
+
 int sfh_traj_forward(sfh_handle h, double dt, int steps, int checkpoint_every, int* record) {
     SFH_NEED(h);
     if (!record) return -3;

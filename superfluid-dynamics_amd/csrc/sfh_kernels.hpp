@@ -271,6 +271,9 @@ struct AdjMArgs {
 };
 constexpr int kAdjRows = 2;   // rows per workgroup of the reverse pair kernels
 __global__ void adj_rho_quirk_kernel(AdjMArgs a);
+__global__ void aug_vjp_kernel(const double2* Z, const double* gA_P, const double* gB_P, const double* gA_D,
+                               const double* gB_D, double ca, double cb, double2* outZ, double2* accZ, double* outD,
+                               double* accD, Phys ph, int total);
 template <int R> __global__ void adj_pass_a_kernel(AdjMArgs a, Phys ph);
 template <int R> __global__ void adj_pass_b_kernel(AdjMArgs a, Phys ph);
 template <int R> __global__ void adj_pass_c_kernel(AdjMArgs a, Phys ph);

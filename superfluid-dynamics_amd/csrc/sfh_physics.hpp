@@ -184,14 +184,17 @@ __device__ inline double drive_terms(const Phys& ph, const StageCtl& ctl, int g,
 // cotangent): i (d dPhi/dt / dy) kp from the restoring term, plus for the autonomous drive
 // strength kp (dI/dx + i dI/dy), I = (gamma^2/4) I_max / ((delta - G y)^2 + gamma^2/4) exp(-(x - x0)^2 / (2 sigma^2)):
 // dI/dy = 2 G (delta - G y) I / ((delta - G y)^2 + gamma^2/4), dI/dx = -(x - x0) I / sigma^2.
+__device__ inline double2 intensity_grad(const Phys& ph, double x, double y) {
+    const double inten = intensity(ph, y, x);
+    const double df = ph.detuning - ph.G * y, hg = ph.gamma / 2.0;
+    return make_double2(inten * (-(x - ph.x0) / (ph.sigma * ph.sigma)), inten * (2.0 * ph.G * df / (df * df + hg * hg)));
+}
 __device__ inline double2 bernoulli_gz(const Phys& ph, double x, double y, double kp) {
     double2 g = make_double2(0.0, bernoulli_dy(ph, y) * kp);
     if (ph.drive) {
-        const double inten = intensity(ph, y, x);
-        const double df = ph.detuning - ph.G * y, hg = ph.gamma / 2.0;
-        const double s = ph.strength * kp * inten;
-        g.x += s * (-(x - ph.x0) / (ph.sigma * ph.sigma));
-        g.y += s * (2.0 * ph.G * df / (df * df + hg * hg));
+        const double2 gi = intensity_grad(ph, x, y);
+        g.x += ph.strength * kp * gi.x;
+        g.y += ph.strength * kp * gi.y;
     }
     return g;
 }

@@ -290,7 +290,8 @@ Solver::~Solver() {
     void* vbufs[] = {vY_[0], vY_[1], vY_[2], vK_[0], vK_[1], vK_[2], vYP_[0], vYP_[1], vYP_[2], vKP_[0],
                      vKP_[1], vKP_[2], vMu_[0], vMu_[1], vMu_[2], vMu_[3], gA_, gR_[0], gR_[1], gAcc_,
                      gAP_, gRP_[0], gRP_[1], gAccP_, adj_, vstats_, Mt_, agwc_, agmup_, ad1_, agam_, agamI_,
-                     agzp_, agzpp_, agz_, agmu_, agb_, aquirk_, adjRing_, adjSave_, adjX_, adjSaveZ_, adjSaveP_};
+                     agzp_, agzpp_, agz_, agmu_, agb_, aquirk_, gAD_, gAccD_, gRD_[0], gRD_[1], vYD_[0], vYD_[1],
+                     vYD_[2], adjRing_, adjSave_, adjX_, adjSaveZ_, adjSaveP_};
     for (void* p : vbufs)
         if (p) (void)hipFree(p);
     for (auto& kv : traj_) free_record(kv.second);
@@ -1248,9 +1249,9 @@ void debug_lu_solve(int n, const double* A, const double* b, double* x, int* int
 // ------------------------------------------------------------------ reverse mode
 
 void Solver::ensure_vjp() {
-    if (ph_.drive && (ph_.augmented || ph_.timed))
-        throw ArgError("the adjoint is implemented for water, the helium film and the autonomous drive (not the "
-                       "augmented or time-dependent delayed intensity)");
+    if (ph_.drive && ph_.timed)
+        throw ArgError("the adjoint is implemented for water, the helium film and the autonomous and augmented "
+                       "drives (not the time-dependent delayed intensity)");
     if (opt_.solver == kSolveBatched && maxN_ > kBatchMaxN)
         throw ArgError("the batched adjoint takes surfaces of at most 1024 nodes");
     if (vjp_ready_) return;
@@ -1291,9 +1292,15 @@ void Solver::ensure_vjp() {
 
 void Solver::enqueue_vjp(const double2* Z, const double* P, const double* mu, const double2* gAZ, const double* gAP,
                          double ca, const double2* gBZ, const double* gBP, double cb, double2* outZ, double* outP,
-                         bool accumulate, int stage) {
+                         bool accumulate, int stage, const double* gAD, const double* gBD, double* outD) {
+    auto aug = [&] {   // the delayed intensity's terms (augmented drive)
+        if (!ph_.augmented || !outD) return;
+        go(aug_vjp_kernel, dim3((T_ + 255) / 256), dim3(256), 0, Z, gAP, gBP, gAD, gBD, ca, cb, outZ,
+           accumulate ? gAcc_ : nullptr, outD, accumulate ? gAccD_ : nullptr, ph_, T_);
+    };
     if (opt_.solver != kSolveBatched) {
         enqueue_vjp_mwg(Z, P, mu, gAZ, gAP, ca, gBZ, gBP, cb, outZ, outP, accumulate, stage);
+        aug();
         return;
     }
     VjpArgs a{};
@@ -1321,6 +1328,7 @@ void Solver::enqueue_vjp(const double2* Z, const double* P, const double* mu, co
     launch("vjp", 3.0 * nsq, 0.0, [&] {
         go(rhs_vjp_kernel, dim3(S_), dim3(kBatchThreads), 0, a, ph_);
     });
+    aug();
     SFH_HIP(hipGetLastError());
 }
 
@@ -1454,10 +1462,15 @@ void Solver::download_cotangent(const double2* Z, const double* P, double* gx, d
 }
 
 void Solver::rhs_vjp(const double* gvx, const double* gvy, const double* gdphi, double* gx, double* gy,
-                     double* gphi) {
+                     double* gphi, const double* gdD, double* gD) {
     ensure_vjp();
+    if (ph_.augmented != (gdD != nullptr && gD != nullptr))
+        throw ArgError(ph_.augmented ? "the augmented drive's reverse pass takes the cotangents of D (sfh_rhs_vjp_augmented)"
+                                     : "the handle has no augmented drive");
+    ensure_vjp_d();
     SFH_HIP(hipStreamSynchronize(stream_));
     upload_cotangent(gvx, gvy, gdphi, gA_, gAP_);
+    if (gdD) SFH_HIP(hipMemcpy(gAD_, gdD, T_ * sizeof(double), hipMemcpyHostToDevice));
     StageCtl c{};
     c.stage = 0;
     c.Z0 = Z0_;
@@ -1466,11 +1479,21 @@ void Solver::rhs_vjp(const double* gvx, const double* gvy, const double* gdphi, 
         c.kZ[i] = vK_[i];
         c.kP[i] = vKP_[i];
     }
+    set_delayed_ctl(c, D0_, nullptr);
     enqueue_rhs(Z0_, P0_, c);   // the forward mu at the current state
     SFH_HIP(hipMemcpyAsync(vMu_[0], mu_, T_ * sizeof(double), hipMemcpyDeviceToDevice, stream_));
-    enqueue_vjp(Z0_, P0_, vMu_[0], gA_, gAP_, 1.0, nullptr, nullptr, 0.0, gR_[0], gRP_[0], false);
+    enqueue_vjp(Z0_, P0_, vMu_[0], gA_, gAP_, 1.0, nullptr, nullptr, 0.0, gR_[0], gRP_[0], false, -1, gAD_, nullptr,
+                gRD_[0]);
     download_cotangent(gR_[0], gRP_[0], gx, gy, gphi);
+    if (gD) SFH_HIP(hipMemcpy(gD, gRD_[0], T_ * sizeof(double), hipMemcpyDeviceToHost));
     check_status();
+}
+
+void Solver::ensure_vjp_d() {
+    if (!ph_.augmented || gAD_) return;
+    gAD_ = dalloc<double>(T_);
+    gAccD_ = dalloc<double>(T_);
+    for (double** b : {&gRD_[0], &gRD_[1], &vYD_[0], &vYD_[1], &vYD_[2]}) *b = dalloc<double>(T_);
 }
 
 // Adjoint of one classical RK4 step from y_n (AutonomousRungeKuttaStepper.cuh:338-375):
@@ -1480,18 +1503,26 @@ void Solver::rhs_vjp(const double* gvx, const double* gvy, const double* gdphi, 
 //   r4 from kbar4 = dt/6 a;  r3 from kbar3 = dt/3 a + dt r4;  r2 from kbar2 = dt/3 a + dt/2 r3;
 //   r1 from kbar1 = dt/6 a + dt/2 r2;  cotangent of y_n = a + r1 + r2 + r3 + r4.
 void Solver::step_vjp(double dt, const double* gx_in, const double* gy_in, const double* gp_in, double* gx_out,
-                      double* gy_out, double* gp_out) {
+                      double* gy_out, double* gp_out, const double* gD_in, double* gD_out) {
     ensure_vjp();
+    if (ph_.augmented != (gD_in != nullptr && gD_out != nullptr))
+        throw ArgError(ph_.augmented ? "the augmented drive's reverse step takes the cotangents of D (sfh_step_vjp_augmented)"
+                                     : "the handle has no augmented drive");
+    ensure_vjp_d();
     SFH_HIP(hipStreamSynchronize(stream_));
     upload_cotangent(gx_in, gy_in, gp_in, gA_, gAP_);
+    if (gD_in) SFH_HIP(hipMemcpy(gAD_, gD_in, T_ * sizeof(double), hipMemcpyHostToDevice));
     enqueue_step_vjp(dt, Z0_, P0_, nullptr);
     download_cotangent(gAcc_, gAccP_, gx_out, gy_out, gp_out);
+    if (gD_out) SFH_HIP(hipMemcpy(gD_out, gAccD_, T_ * sizeof(double), hipMemcpyDeviceToHost));
     check_status();
 }
 
 void Solver::enqueue_step_vjp(double dt, const double2* Z, const double* P, const StageCapture* stored) {
     SFH_HIP(hipMemcpyAsync(gAcc_, gA_, T_ * sizeof(double2), hipMemcpyDeviceToDevice, stream_));
     SFH_HIP(hipMemcpyAsync(gAccP_, gAP_, T_ * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+    const bool aug = ph_.augmented;
+    if (aug) SFH_HIP(hipMemcpyAsync(gAccD_, gAD_, T_ * sizeof(double), hipMemcpyDeviceToDevice, stream_));
     StageCapture sc;
     if (stored) {
         sc = *stored;
@@ -1512,6 +1543,8 @@ void Solver::enqueue_step_vjp(double dt, const double2* Z, const double* P, cons
             c.alpha = st < 3 ? alpha[st] : 0.0;
             c.Zout = st < 3 ? vY_[st] : nullptr;
             c.Pout = st < 3 ? vYP_[st] : nullptr;
+            // the augmented drive's D: stage input D0 or the previous stage's value, output the next
+            set_delayed_ctl(c, st == 0 ? D0_ : vYD_[st - 1], st < 3 ? vYD_[st] : nullptr);
             enqueue_rhs(zin[st], pin[st], c);
             SFH_HIP(hipMemcpyAsync(vMu_[st], mu_, T_ * sizeof(double), hipMemcpyDeviceToDevice, stream_));
         }
@@ -1522,10 +1555,15 @@ void Solver::enqueue_step_vjp(double dt, const double2* Z, const double* P, cons
         for (int i = 0; i < 4; ++i) sc.mu[i] = vMu_[i];
     }
     // reverse sweep
-    enqueue_vjp(sc.Y[2], sc.YP[2], sc.mu[3], gA_, gAP_, dt / 6.0, nullptr, nullptr, 0.0, gR_[0], gRP_[0], true, 3);
-    enqueue_vjp(sc.Y[1], sc.YP[1], sc.mu[2], gA_, gAP_, dt / 3.0, gR_[0], gRP_[0], dt, gR_[1], gRP_[1], true, 2);
-    enqueue_vjp(sc.Y[0], sc.YP[0], sc.mu[1], gA_, gAP_, dt / 3.0, gR_[1], gRP_[1], 0.5 * dt, gR_[0], gRP_[0], true, 1);
-    enqueue_vjp(Z, P, sc.mu[0], gA_, gAP_, dt / 6.0, gR_[0], gRP_[0], 0.5 * dt, gR_[1], gRP_[1], true, 0);
+    const double* gAD = aug ? gAD_ : nullptr;
+    enqueue_vjp(sc.Y[2], sc.YP[2], sc.mu[3], gA_, gAP_, dt / 6.0, nullptr, nullptr, 0.0, gR_[0], gRP_[0], true, 3, gAD,
+                nullptr, aug ? gRD_[0] : nullptr);
+    enqueue_vjp(sc.Y[1], sc.YP[1], sc.mu[2], gA_, gAP_, dt / 3.0, gR_[0], gRP_[0], dt, gR_[1], gRP_[1], true, 2, gAD,
+                aug ? gRD_[0] : nullptr, aug ? gRD_[1] : nullptr);
+    enqueue_vjp(sc.Y[0], sc.YP[0], sc.mu[1], gA_, gAP_, dt / 3.0, gR_[1], gRP_[1], 0.5 * dt, gR_[0], gRP_[0], true, 1,
+                gAD, aug ? gRD_[1] : nullptr, aug ? gRD_[0] : nullptr);
+    enqueue_vjp(Z, P, sc.mu[0], gA_, gAP_, dt / 6.0, gR_[0], gRP_[0], 0.5 * dt, gR_[1], gRP_[1], true, 0, gAD,
+                aug ? gRD_[0] : nullptr, aug ? gRD_[1] : nullptr);
 }
 
 Solver::StageCapture Solver::seg_capture(int k) const {
@@ -1554,6 +1592,7 @@ void Solver::load_state(const double2* Z, const double* P) {
 int Solver::traj_forward(double dt, int steps, int every) {
     if (steps < 1 || every < 1) throw ArgError("a trajectory needs steps >= 1 and checkpoint_every >= 1");
     ensure_vjp();   // the problem must have a reverse pass: rejected before any step
+    if (ph_.augmented) throw ArgError("device trajectories take the two-field state; the augmented drive has three");
     const int nck = (steps + every - 1) / every;
     TrajRecord r;
     r.dt = dt;

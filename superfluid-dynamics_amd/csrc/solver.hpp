@@ -162,9 +162,12 @@ class Solver {
     // current state -> cotangents of (x, y, phi).  step_vjp: cotangent of the state after one RK4
     // step of size dt taken from the current state -> cotangent of the current state (the state is
     // left unchanged).  Host arrays of total_nodes() entries.
-    void rhs_vjp(const double* gvx, const double* gvy, const double* gdphi, double* gx, double* gy, double* gphi);
+    // With the augmented drive the state has a third field D: gdD / gD_in is its cotangent in and
+    // gD / gD_out out (both required then, and refused otherwise).
+    void rhs_vjp(const double* gvx, const double* gvy, const double* gdphi, double* gx, double* gy, double* gphi,
+                 const double* gdD = nullptr, double* gD = nullptr);
     void step_vjp(double dt, const double* gx_in, const double* gy_in, const double* gp_in, double* gx_out,
-                  double* gy_out, double* gp_out);
+                  double* gy_out, double* gp_out, const double* gD_in = nullptr, double* gD_out = nullptr);
 
     // Differentiable trajectories with device-resident checkpoints.  traj_forward advances `steps`
     // RK4 steps of size dt from the current state exactly as step() does, keeping in device memory the
@@ -287,7 +290,8 @@ class Solver {
     // transposed solve inside a trajectory sweep; -1 outside one
     void enqueue_vjp(const double2* Z, const double* P, const double* mu, const double2* gAZ, const double* gAP,
                      double ca, const double2* gBZ, const double* gBP, double cb, double2* outZ, double* outP,
-                     bool accumulate, int stage = -1);
+                     bool accumulate, int stage = -1, const double* gAD = nullptr, const double* gBD = nullptr,
+                     double* outD = nullptr);
     void upload_cotangent(const double* gx, const double* gy, const double* gp, double2* Z, double* P);
     void download_cotangent(const double2* Z, const double* P, double* gx, double* gy, double* gp);
     bool vjp_ready_ = false;
@@ -304,6 +308,10 @@ class Solver {
     double2 *agzp_ = nullptr, *agzpp_ = nullptr, *agz_ = nullptr;
     double *agmu_ = nullptr, *agb_ = nullptr;
     double* aquirk_ = nullptr;   // the V1[1] quirk terms of the reverse pass (3 x T_)
+    // the augmented drive's D in the reverse pass: input cotangent, accumulator, stage cotangents,
+    // recomputed stage values (allocated by ensure_vjp_d)
+    double *gAD_ = nullptr, *gAccD_ = nullptr, *gRD_[2] = {nullptr, nullptr}, *vYD_[3] = {nullptr, nullptr, nullptr};
+    void ensure_vjp_d();
     void enqueue_vjp_mwg(const double2* Z, const double* P, const double* mu, const double2* gAZ, const double* gAP,
                          double ca, const double2* gBZ, const double* gBP, double cb, double2* outZ, double* outP,
                          bool accumulate, int stage);

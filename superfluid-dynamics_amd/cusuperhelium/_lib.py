@@ -133,6 +133,8 @@ SIGNATURES = {
     "sfh_rhs_device": (c_int, [c_void_p, c_void_p, c_void_p]),
     "sfh_rhs_vjp": (c_int, [c_void_p, DP, DP, DP, DP, DP, DP]),
     "sfh_step_vjp": (c_int, [c_void_p, c_double, DP, DP, DP, DP, DP, DP]),
+    "sfh_rhs_vjp_augmented": (c_int, [c_void_p, DP, DP, DP, DP, DP, DP, DP, DP]),
+    "sfh_step_vjp_augmented": (c_int, [c_void_p, c_double, DP, DP, DP, DP, DP, DP, DP, DP]),
     "sfh_traj_forward": (c_int, [c_void_p, c_double, c_int, c_int, POINTER(c_int)]),
     "sfh_traj_backward": (c_int, [c_void_p, c_int, DP, DP, DP, DP, DP, DP]),
     "sfh_traj_release": (c_int, [c_void_p, c_int]),

@@ -171,27 +171,42 @@ class Ensemble:
         return out + (st[:, 3 * T:],) if fields == 4 else out
 
     # ---- reverse mode (water, the helium film, the autonomous drive)
-    def rhs_vjp(self, gq, gdphi):
-        """Cotangent of (Z, Phi) from cotangents of (q = dZ/dt, dPhi/dt) at the current state.
+    def rhs_vjp(self, gq, gdphi, gdD=None):
+        """Cotangent of (Z, Phi) from cotangents of (q = dZ/dt, dPhi/dt) at the current state; with
+        the augmented drive gdD, the cotangent of dD/dt, is required and the cotangent of D is
+        returned third.
 
         Complex cotangents follow dL = Re sum conj(g) dz, i.e. g = dL/dRe + i dL/dIm."""
         gq = np.asarray(gq, dtype=np.complex128).ravel()
         gvx, gvy = np.ascontiguousarray(gq.real), np.ascontiguousarray(gq.imag)
         gp = np.ascontiguousarray(gdphi, dtype=np.float64).ravel()
         gx, gy, gphi = (np.empty(self.total) for _ in range(3))
-        check(self._lib.sfh_rhs_vjp(self._h, _dp(gvx), _dp(gvy), _dp(gp), _dp(gx), _dp(gy), _dp(gphi)),
-              "sfh_rhs_vjp")
-        return gx + 1j * gy, gphi
+        if gdD is None:
+            check(self._lib.sfh_rhs_vjp(self._h, _dp(gvx), _dp(gvy), _dp(gp), _dp(gx), _dp(gy), _dp(gphi)),
+                  "sfh_rhs_vjp")
+            return gx + 1j * gy, gphi
+        gd = np.ascontiguousarray(gdD, dtype=np.float64).ravel()
+        gD = np.empty(self.total)
+        check(self._lib.sfh_rhs_vjp_augmented(self._h, _dp(gvx), _dp(gvy), _dp(gp), _dp(gd), _dp(gx), _dp(gy),
+                                              _dp(gphi), _dp(gD)), "sfh_rhs_vjp_augmented")
+        return gx + 1j * gy, gphi, gD
 
-    def step_vjp(self, dt: float, gZ, gPhi):
-        """Cotangent of the current state from the cotangent of the state one RK4 step later."""
+    def step_vjp(self, dt: float, gZ, gPhi, gD=None):
+        """Cotangent of the current state from the cotangent of the state one RK4 step later (with the
+        augmented drive also of D: gD in, the cotangent of the current D returned third)."""
         gZ = np.asarray(gZ, dtype=np.complex128).ravel()
         gxi, gyi = np.ascontiguousarray(gZ.real), np.ascontiguousarray(gZ.imag)
         gpi = np.ascontiguousarray(gPhi, dtype=np.float64).ravel()
         gx, gy, gp = (np.empty(self.total) for _ in range(3))
-        check(self._lib.sfh_step_vjp(self._h, float(dt), _dp(gxi), _dp(gyi), _dp(gpi), _dp(gx), _dp(gy), _dp(gp)),
-              "sfh_step_vjp")
-        return gx + 1j * gy, gp
+        if gD is None:
+            check(self._lib.sfh_step_vjp(self._h, float(dt), _dp(gxi), _dp(gyi), _dp(gpi), _dp(gx), _dp(gy),
+                                         _dp(gp)), "sfh_step_vjp")
+            return gx + 1j * gy, gp
+        gdi = np.ascontiguousarray(gD, dtype=np.float64).ravel()
+        gdo = np.empty(self.total)
+        check(self._lib.sfh_step_vjp_augmented(self._h, float(dt), _dp(gxi), _dp(gyi), _dp(gpi), _dp(gdi), _dp(gx),
+                                               _dp(gy), _dp(gp), _dp(gdo)), "sfh_step_vjp_augmented")
+        return gx + 1j * gy, gp, gdo
 
     def traj_forward(self, dt: float, steps: int, checkpoint_every: int) -> int:
         """Advance `steps` RK4 steps like step(), keeping a device-resident checkpoint every

@@ -279,7 +279,7 @@ def test_rhs_vjp_autonomous_drive_dot_product(oracle, csh, solver, sizes):
         o += n
 
 
-@pytest.mark.parametrize("kw,drive", [(dict(kind=1, depth=1.0), "augmented"), (dict(kind=1, depth=1.0), "timed")])
+@pytest.mark.parametrize("kw,drive", [(dict(kind=1, depth=1.0), "timed")])
 def test_adjoint_rejects_unsupported_physics(csh, kw, drive):
     with csh.Ensemble([64], csh.Props(**kw), solver=csh.BATCHED) as e:
         if drive:
@@ -399,3 +399,68 @@ def test_device_trajectory_redoes_segments_whose_trimmed_solves_ran_out(csh):
           f"cotangent rel diff {dg:.2e}", flush=True)
     assert st["redone_steps"] >= redone_fwd + 3 * every   # both re-runs and one reverse sweep again
     assert dg <= 1e-10
+
+
+@pytest.mark.parametrize("solver,sizes", [("BATCHED", [128, 64]), ("GMRES", [512]), ("LU", [256])])
+def test_rhs_vjp_augmented_drive_dot_product(oracle, csh, solver, sizes):
+    # the augmented system (the delayed intensity D a third field: dPhi/dt + D, dD/dt = beta I - D/tau)
+    # against central differences of the oracle's augmented right-hand side
+    kw = FILMS[0]
+    print(f"augmented rhs vjp {solver} {sizes}", flush=True)
+    rng = np.random.default_rng(19)
+    Zs = [_surface(n, 400 + n) for n in sizes]
+    Ps = [0.02 * np.sin(2 * np.pi * np.arange(n) / n + 0.7) for n in sizes]
+    Ds = [0.05 * np.cos(2 * np.pi * np.arange(n) / n) for n in sizes]
+    lq = [rng.standard_normal(n) + 1j * rng.standard_normal(n) for n in sizes]
+    lp = [rng.standard_normal(n) for n in sizes]
+    ld = [rng.standard_normal(n) for n in sizes]
+    with csh.Ensemble(sizes, csh.Props(**kw), solver=getattr(csh, solver), tol=1e-13) as e:
+        e.set_drive(augmented=True, **DRIVE)
+        e.set_state(np.concatenate(Zs), np.concatenate(Ps))
+        e.set_delayed(np.concatenate(Ds))
+        gZ, gP, gD = e.rhs_vjp(np.concatenate(lq), np.concatenate(lp), np.concatenate(ld))
+        with pytest.raises(Exception):
+            e.rhs_vjp(np.concatenate(lq), np.concatenate(lp))   # D's cotangent is required
+    drive = oracle.Drive(augmented=True, **DRIVE)
+    o = 0
+    for i, n in enumerate(sizes):
+        dZ, dP = _smooth_direction(rng, n)
+        dD = 0.3 * np.sin(2 * np.pi * np.arange(n) / n + 0.2)
+
+        def f(h):
+            q, dp, dd = oracle.rhs_driven(oracle.Props(**kw), drive, Zs[i] + h * dZ, Ps[i] + h * dP, Ds[i] + h * dD)
+            return _inner(lq[i], lp[i], q, dp) + float(np.sum(ld[i] * dd))
+        fd = _richardson(f)
+        ad = _inner(gZ[o:o + n], gP[o:o + n], dZ, dP) + float(np.sum(gD[o:o + n] * dD))
+        print(f"  n={n}: adjoint {ad:.12e}  fd {fd:.12e}", flush=True)
+        assert abs(ad - fd) <= 1e-7 * max(abs(fd), 1e-3)
+        o += n
+
+
+@pytest.mark.parametrize("solver", ["BATCHED", "GMRES"])
+def test_step_vjp_augmented_drive_dot_product(oracle, csh, solver):
+    kw = FILMS[0]
+    n, dt = 128, 0.005
+    rng = np.random.default_rng(23)
+    Z, P = _surface(n, 9), 0.01 * np.cos(2 * np.pi * np.arange(n) / n)
+    D = 0.05 * np.sin(2 * np.pi * np.arange(n) / n)
+    aZ = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    aP, aD = rng.standard_normal(n), rng.standard_normal(n)
+    with csh.Ensemble([n], csh.Props(**kw), solver=getattr(csh, solver), tol=1e-13) as e:
+        e.set_drive(augmented=True, **DRIVE)
+        e.set_state(Z, P)
+        e.set_delayed(D)
+        gZ, gP, gD = e.step_vjp(dt, aZ, aP, aD)
+        Zs, _ = e.get_state()
+        assert np.array_equal(Zs, Z) and np.array_equal(e.delayed(), D)   # the state is left unchanged
+    drive = oracle.Drive(augmented=True, **DRIVE)
+    dZ, dP = _smooth_direction(rng, n)
+    dD = 0.3 * np.cos(2 * np.pi * np.arange(n) / n + 0.4)
+
+    def f(h):
+        Z1, P1, D1 = oracle.rk4_driven(oracle.Props(**kw), drive, Z + h * dZ, P + h * dP, D + h * dD, dt, 1)
+        return _inner(aZ, aP, Z1, P1) + float(np.sum(aD * D1))
+    fd = _richardson(f)
+    ad = _inner(gZ, gP, dZ, dP) + float(np.sum(gD * dD))
+    print(f"augmented step vjp {solver}: adjoint {ad:.12e} fd {fd:.12e}", flush=True)
+    assert abs(ad - fd) <= 1e-7 * abs(fd)
