@@ -324,10 +324,11 @@ def _host_sweep(e, Z0, P0, dt, steps, every, gZ, gP):
     # solves (the calibration falls in the first swept segment, the second is trimmed)
     ("GMRES", [512, 256], 8, 16, None),
     ("GMRES", [384], 4, 9, "rho"),
+    ("BATCHED", [128], 4, 9, "drive"),   # the film under the autonomous optical drive
 ])
 def test_device_trajectory_sweep_matches_host_sweep(csh, solver, sizes, every, steps, kw):
     dt = 0.004
-    props = csh.Props(**({"film": FILMS[0], "rho": RHO[0]}.get(kw, {})))
+    props = csh.Props(**({"film": FILMS[0], "rho": RHO[0], "drive": FILMS[0]}.get(kw, {})))
     rng = np.random.default_rng(31)
     Z0 = np.concatenate([_surface(n, 40 + i) for i, n in enumerate(sizes)])
     P0 = np.concatenate([0.01 * np.cos(2 * np.pi * np.arange(n) / n) for n in sizes])
@@ -336,6 +337,8 @@ def test_device_trajectory_sweep_matches_host_sweep(csh, solver, sizes, every, s
     print(f"device trajectory sweep {solver} sizes {sizes} every {every} {kw or 'water'}", flush=True)
     with csh.Ensemble(sizes, props, solver=getattr(csh, solver), tol=1e-13,
                       max_iter=48 if solver == "BATCHED" else 32) as e:
+        if kw == "drive":
+            e.set_drive(**DRIVE)
         ZT_h, PT_h, gZ_h, gP_h = _host_sweep(e, Z0, P0, dt, steps, every, gZ, gP)
         e.set_state(Z0, P0)
         rec = e.traj_forward(dt, steps, every)
