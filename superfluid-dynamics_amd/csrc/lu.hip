@@ -368,18 +368,19 @@ __device__ void factor_invert_diag(double* D, double* Li, double* Ui, double* T,
     eliminate_tiles(D, kb, my_bad);
     if (my_bad) *bad = 1;
     __syncthreads();
-    invert_factors(D, Li, Ui, T);
-    // zero outside the kb x kb corner (padded rows were identity rows)
+    invert_factors(D, Li, Ui, T);   // ends with a barrier
+    if (kb < kNB) {   // zero outside the kb x kb corner (padded rows were identity rows)
 #pragma unroll
-    for (int m = 0; m < 16; ++m) {
-        const int c = part + 4 * m;
-        if (!(r < kb && c < kb)) {
-            D[r * LD + c] = 0.0;
-            Li[r * LD + c] = 0.0;
-            Ui[r * LD + c] = 0.0;
+        for (int m = 0; m < 16; ++m) {
+            const int c = part + 4 * m;
+            if (!(r < kb && c < kb)) {
+                D[r * LD + c] = 0.0;
+                Li[r * LD + c] = 0.0;
+                Ui[r * LD + c] = 0.0;
+            }
         }
+        __syncthreads();
     }
-    __syncthreads();
 }
 
 // Per-surface views for gridDim.z batches of equal size: surface s uses A + s n^2, vectors + s n,
@@ -546,14 +547,19 @@ __global__ __launch_bounds__(256) void lu_step_kernel(LuArgs a, int k0) {
     __syncthreads();
     if (row_role) {
         mfma_block(X, Ui, acc, wr, wc, lane, 1.0);        // L21 = A21 U11^{-1}
+        int my_bad = 0;   // the multipliers against GEPP's pivot rule, from the accumulators
+        {
+            const int l16 = lane & 15, l4 = lane >> 4;
+            for (int i = 0; i < 2; ++i)
+                for (int j = 0; j < 2; ++j)
+                    for (int r = 0; r < 4; ++r) {
+                        const int row = wr * 32 + i * 16 + 4 * r + l4, col = wc * 32 + j * 16 + l16;
+                        if (row < rb && col < kb && !(fabs(acc[i][j][r]) <= 1.0)) my_bad = 1;
+                    }
+        }
         __syncthreads();
         acc_to_lds(X, acc, wr, wc, lane);
         __syncthreads();
-        int my_bad = 0;
-        for (int e = t; e < kNB * kNB; e += blockDim.x) {
-            const int r = e >> 6, c = e & 63;
-            if (r < rb && c < kb && !(fabs(X[r * LD + c]) <= 1.0)) my_bad = 1;
-        }
         if (my_bad) atomicOr(&v.status[0], 1);
         store_block(v.A + (long long)R0 * n + k0, X, n, rb, kb);
         {                                                 // b_i -= L21_i y_p
