@@ -344,18 +344,26 @@ def adjoint_leg(c):
     target = torch.tensor(tgt @ basis)
     B = torch.tensor(basis)
 
+    split = [0.0, 0.0]   # forward (trajectory with checkpoints) and backward (reverse sweep) seconds
+
     def gradient(tr):
+        t0 = time.perf_counter()
         ct = torch.tensor(amp, dtype=torch.float64, requires_grad=True)
         Z0 = torch.tensor(x, dtype=torch.complex128) + 1j * (ct @ B).to(torch.complex128)
-        ZT, _ = evolve(Z0, torch.zeros(n, dtype=torch.float64), tr, dt, steps)
+        ZT, _ = evolve(Z0, torch.zeros(n, dtype=torch.float64), tr, dt, steps)   # returns host tensors
         loss = 0.5 * torch.sum((ZT.imag - target) ** 2)
+        t1 = time.perf_counter()
         loss.backward()
+        t2 = time.perf_counter()
+        split[0] += t1 - t0
+        split[1] += t2 - t1
         return float(loss.detach()), ct.grad.numpy()
 
     def timed(solver, reps):
         # one handle for the warm-up (first launches, adjoint workspace) and the timed gradients
         with Trajectory([n], checkpoint_every=every, solver=solver) as tr:
             gradient(tr)
+            split[0] = split[1] = 0.0
             t0 = time.perf_counter()
             for _ in range(reps):
                 out = gradient(tr)
@@ -363,6 +371,7 @@ def adjoint_leg(c):
 
     # the reverse pass over many workgroups (GMRES handle), and over one workgroup per surface (batched)
     el, (loss, grad) = timed(c.GMRES, 3)
+    fwd_s, bwd_s = split[0] / 3, split[1] / 3
     el_b, (_, grad_b) = timed(c.BATCHED, 1)
     Z0s = []
     for k in range(modes):
@@ -381,6 +390,8 @@ def adjoint_leg(c):
                         "many workgroups; GMRES forward; device-resident checkpoints and cotangent; one handle, "
                         "mean of 3 gradients after a warm-up)",
             "gradient_s": round(el, 4), "ms_per_step_forward_and_backward": round(1e3 * el / steps, 3),
+            "forward_s": round(fwd_s, 4), "backward_s": round(bwd_s, 4),
+            "backward_over_forward": round(bwd_s / fwd_s, 2) if fwd_s > 0 else None,
             "loss": loss, "max_rel_err_vs_fd": float(np.abs(grad - fd).max() / np.abs(fd).max()),
             "one_workgroup_per_surface": {"gradient_s": round(el_b, 4),
                                           "ms_per_step_forward_and_backward": round(1e3 * el_b / steps, 3),
